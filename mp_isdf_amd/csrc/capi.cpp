@@ -1,0 +1,202 @@
+// C ABI of the mp_isdf_amd library (declarations in include/mp_isdf_amd.h).
+#include "../../include/mp_isdf_amd.h"
+
+#include "axis.hpp"
+#include "context.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+
+struct mpisdf_context {
+    std::unique_ptr<mpisdf::Context> impl;
+};
+
+namespace {
+
+template <class F>
+int guard(F&& f)
+{
+    try {
+        f();
+        return mpisdf::kOk;
+    } catch (const mpisdf::NoDevice& e) {
+        mpisdf::set_last_error(e.what());
+        return mpisdf::kNoDevice;
+    } catch (const std::invalid_argument& e) {
+        mpisdf::set_last_error(e.what());
+        return mpisdf::kInvalid;
+    } catch (const std::overflow_error& e) {
+        mpisdf::set_last_error(e.what());
+        return mpisdf::kOverflow;
+    } catch (const std::bad_alloc&) {
+        mpisdf::set_last_error("host allocation failed");
+        return mpisdf::kRuntime;
+    } catch (const std::logic_error& e) {
+        mpisdf::set_last_error(e.what());
+        return mpisdf::kLogic;
+    } catch (const std::exception& e) {
+        mpisdf::set_last_error(e.what());
+        return mpisdf::kRuntime;
+    }
+}
+
+mpisdf::Context& unwrap(const mpisdf_context* ctx)
+{
+    if (!ctx || !ctx->impl) throw std::invalid_argument("null mp_isdf context");
+    return *ctx->impl;
+}
+
+// Points per device batch for host-buffer evaluation (bounds device memory).
+constexpr int64_t kHostBatch = 1 << 18;
+
+}  // namespace
+
+extern "C" {
+
+const char* mpisdf_last_error(void) { return mpisdf::last_error(); }
+
+int mpisdf_device_count(int* count)
+{
+    return guard([&] {
+        int c = 0;
+        if (hipGetDeviceCount(&c) != hipSuccess) { (void)hipGetLastError(); c = 0; }
+        *count = c;
+    });
+}
+
+int mpisdf_context_create(const int32_t* atm, int64_t natm, const int32_t* bas, int64_t nbas,
+                          const double* env, int64_t nenv, int cart, mpisdf_context** out)
+{
+    return guard([&] {
+        if (!out) throw std::invalid_argument("null output handle");
+        auto handle = std::make_unique<mpisdf_context>();
+        handle->impl = mpisdf::make_context(atm, natm, bas, nbas, env, nenv, cart != 0);
+        *out = handle.release();
+    });
+}
+
+int mpisdf_context_destroy(mpisdf_context* ctx)
+{
+    return guard([&] { delete ctx; });
+}
+
+int mpisdf_context_nao(const mpisdf_context* ctx, int64_t* nao)
+{
+    return guard([&] { *nao = unwrap(ctx).nao; });
+}
+
+int mpisdf_eval_ao(const mpisdf_context* ctx, const double* coords, int64_t n, int gradient,
+                   double* out)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0) throw std::invalid_argument("expected coords[ngrid,3] in Bohr");
+        const int64_t nao = c.nao;
+        if (n == 0 || nao == 0) return;
+        for (int64_t i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(coords[i])) throw std::invalid_argument("grid coordinates must be finite");
+        const int comps = gradient ? 3 : 1;
+        const int64_t batch = std::min(n, kHostBatch);
+        mpisdf::DeviceBuffer<double> d_xyz(3 * batch), d_out(comps * batch * nao);
+        hipStream_t s = nullptr;
+        for (int64_t start = 0; start < n; start += batch) {
+            const int64_t count = std::min(batch, n - start);
+            MPISDF_HIP(hipMemcpyAsync(d_xyz.ptr, coords + 3 * start, 3 * count * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+            mpisdf::launch_ao_eval(c, d_xyz.ptr, count, gradient ? mpisdf::kGradient : mpisdf::kValues,
+                                   mpisdf::kF64, d_out.ptr, nao, count * nao, s);
+            for (int comp = 0; comp < comps; ++comp)
+                MPISDF_HIP(hipMemcpyAsync(out + comp * n * nao + start * nao, d_out.ptr + comp * count * nao,
+                                          count * nao * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        MPISDF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int mpisdf_ao_eval_device(const mpisdf_context* ctx, const double* xyz, int64_t n, int mode,
+                          int dtype, void* out, int64_t ld, int64_t cstride, void* stream)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0) throw std::invalid_argument("negative point count");
+        if (mode < 0 || mode > 2) throw std::invalid_argument("mode must be 0, 1 or 2");
+        mpisdf::launch_ao_eval(c, xyz, n, static_cast<mpisdf::AoMode>(mode),
+                               static_cast<mpisdf::DType>(dtype), out, ld, cstride,
+                               static_cast<hipStream_t>(stream));
+    });
+}
+
+int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_t n,
+                         const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc,
+                         double* out, int64_t* width)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        const auto spec = mpisdf::axis_spec(c, axes, naxes, mo_coeff, nocc);
+        if (width) *width = spec.width;
+        if (!out || n <= 0) return;
+        for (int64_t i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(points[i])) throw std::invalid_argument("grid coordinates must be finite");
+        const int64_t batch = std::min(n, kHostBatch);
+        mpisdf::DeviceBuffer<double> d_xyz(3 * batch), d_feat(batch * spec.width);
+        mpisdf::AxisWorkspace ws;
+        hipStream_t s = nullptr;
+        for (int64_t start = 0; start < n; start += batch) {
+            const int64_t count = std::min(batch, n - start);
+            MPISDF_HIP(hipMemcpyAsync(d_xyz.ptr, points + 3 * start, 3 * count * sizeof(double),
+                                      hipMemcpyHostToDevice, s));
+            mpisdf::evaluate_axis_device(c, spec, d_xyz.ptr, count, nullptr, mo_coeff, nocc, d_feat.ptr,
+                                         ws, s);
+            MPISDF_HIP(hipMemcpyAsync(out + start * spec.width, d_feat.ptr,
+                                      count * spec.width * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        MPISDF_HIP(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"
+
+#include "select.hpp"
+
+extern "C" int mpisdf_select(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
+                             const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc,
+                             const double* axis_weights, int64_t naxis_weights, double tolerance,
+                             int64_t max_rank, int32_t algorithm, int64_t batch_size, int32_t debug,
+                             int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
+                             int64_t* out, int64_t capacity, int64_t* count, double* diagnostics)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0 || !count) throw std::invalid_argument("expected points[ngrid,3] and weights[ngrid]");
+        if (naxes <= 0 || !axes) throw std::invalid_argument("axis list must not be empty");
+        if (algorithm < 0 || algorithm > 5) throw std::invalid_argument("unknown anchor selection algorithm");
+        if (algorithm > 2)
+            throw std::invalid_argument("CVT and density Khatri-Rao selection are not part of this build");
+        if (precision < 0 || precision > 3) throw std::invalid_argument("precision must be 0..3");
+        if (batch_size < 0 || sketch_size < 0 || oversampling < 0 || max_rank < 0)
+            throw std::invalid_argument("sizes must be non-negative");
+        mpisdf::SelectArgs a;
+        a.points = points;
+        a.weights = weights;
+        a.n = n;
+        a.axes.assign(axes, axes + naxes);
+        a.mo_coeff = mo_coeff;
+        a.nocc = nocc;
+        if (axis_weights && naxis_weights > 0) a.axis_weights.assign(axis_weights, axis_weights + naxis_weights);
+        a.tolerance = tolerance;
+        a.max_rank = max_rank;
+        a.algorithm = algorithm;
+        a.batch_size = batch_size;
+        a.debug = debug != 0;
+        a.sketch_size = sketch_size;
+        a.oversampling = oversampling;
+        a.seed = seed;
+        a.precision = precision;
+        const auto sel = mpisdf::select_anchors(c, a, diagnostics);
+        if ((int64_t)sel.size() > capacity) throw std::invalid_argument("output capacity is too small");
+        std::copy(sel.begin(), sel.end(), out);
+        *count = (int64_t)sel.size();
+    });
+}

@@ -1,0 +1,43 @@
+// Kernel launchers of the mp_isdf_amd HIP library (host declarations).
+#pragma once
+
+#include "common.hpp"
+
+namespace mpisdf {
+
+struct Context;
+
+// ---- AO evaluation (ao_eval.hip) -----------------------------------------
+// out[comp * cstride + g * ld + ao]; comp = 0 (values) / 0..2 (gradient) /
+// 0..3 (values, then gradient) depending on ``mode``.
+void launch_ao_eval(Context& ctx, const double* xyz, int64_t n, AoMode mode, DType dtype,
+                    void* out, int64_t ld, int64_t cstride, hipStream_t stream);
+
+// One axis block of the concatenated feature matrix.
+struct AxisSegment {
+    int kind;        // 0 AO, 1 dX, 2 dY, 3 dZ, 4 occupied
+    int64_t offset;  // first feature column
+    int64_t width;   // nao or nocc
+    double scale;
+};
+void launch_assemble_axis(const double* ao, const double* occ, int64_t n, int64_t nao, int64_t nocc,
+                          const AxisSegment& seg, double* features, int64_t width, hipStream_t stream);
+
+// ---- Dense FP64 GEMM (gemm.hip) -------------------------------------------
+// Strided view: element (i, j) lives at p[i * rs + j * cs].
+struct MatView {
+    const double* p;
+    int64_t rs, cs;
+};
+struct MutView {
+    double* p;
+    int64_t rs, cs;
+};
+// C = alpha * A[M,K] * B[K,N] + beta * C with FP64 accumulation.
+void gemm_f64(int64_t M, int64_t N, int64_t K, double alpha, MatView A, MatView B, double beta,
+              MutView C, hipStream_t stream);
+// Same with an FP32 B operand (element (i, j) at B[i * brs + j * bcs]).
+void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, const float* B,
+                   int64_t brs, int64_t bcs, double beta, MutView C, hipStream_t stream);
+
+}  // namespace mpisdf

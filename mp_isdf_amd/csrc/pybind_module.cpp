@@ -1,0 +1,293 @@
+// pybind11 module ``mp_isdf``: the drop-in Python surface of the reference
+// extension (cpp/src/ao_eval_pybind.cpp:70-120, cpp/src/anchor_select_pybind.cpp:93-233).
+//
+// Host C++ only: every computation goes through the C ABI of
+// libmp_isdf_amd.so (include/mp_isdf_amd.h), which runs on the GPU.  This
+// build has no host execution space: ``space="host"`` raises ValueError, the
+// same way the reference raises for ``space="device"`` in a build without
+// its accelerator backend (cpp/src/ao_eval.cpp:293-295).
+#include "../../include/mp_isdf_amd.h"
+
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace py = pybind11;
+
+namespace {
+
+using IntArray = py::array_t<int32_t, py::array::c_style | py::array::forcecast>;
+using FloatArray = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+enum class AxisType { AtomicOrbital, AtomicOrbitalGradX, AtomicOrbitalGradY, AtomicOrbitalGradZ, OccupiedOrbital };
+enum class LocalAlgorithm {
+    QRColumnPivoting,
+    OnlineQRColumnPivoting,
+    RandomizedQuadratureKhatriRao,
+    CentroidalVoronoi,
+    RandomizedAODensityKhatriRao,
+    RandomizedAODensityNormalizedKhatriRao
+};
+enum class DistributedAlgorithm { SVD, RandomizedQRColumnPivoting };
+
+void check(int status)
+{
+    if (status == 0) return;
+    const std::string msg = mpisdf_last_error();
+    if (status == 1) throw py::value_error(msg);
+    if (status == 2) { PyErr_SetString(PyExc_OverflowError, msg.c_str()); throw py::error_already_set(); }
+    throw std::runtime_error(msg);
+}
+
+// Execution spaces of the reference (cpp/src/evaluation_space_pybind.hpp:8-14).
+void parse_space(const std::string& space)
+{
+    if (space == "auto" || space == "default" || space == "device") return;
+    if (space == "host")
+        throw py::value_error("this mp_isdf build runs on the GPU only; space='host' is not available "
+                              "(use 'auto' or 'device')");
+    throw py::value_error("space must be 'auto', 'host', or 'device'");
+}
+
+bool device_available()
+{
+    int count = 0;
+    return mpisdf_device_count(&count) == 0 && count > 0;
+}
+
+class Context {
+public:
+    Context(const IntArray& atm, const IntArray& bas, const FloatArray& env, bool cart)
+    {
+        if (atm.ndim() != 2 || atm.shape(1) != 6 || bas.ndim() != 2 || bas.shape(1) != 8 || env.ndim() != 1)
+            throw py::value_error("expected atm[natm,6], bas[nbas,8], env[nenv]");
+        mpisdf_context* raw = nullptr;
+        check(mpisdf_context_create(atm.data(), atm.shape(0), bas.data(), bas.shape(0), env.data(),
+                                    env.shape(0), cart ? 1 : 0, &raw));
+        handle_.reset(raw, [](mpisdf_context* c) { mpisdf_context_destroy(c); });
+        int64_t n = 0;
+        check(mpisdf_context_nao(raw, &n));
+        nao_ = n;
+        cart_ = cart;
+    }
+    int64_t nao() const { return nao_; }
+    bool cart() const { return cart_; }
+    const mpisdf_context* get() const { return handle_.get(); }
+
+    py::array_t<double> evaluate(const FloatArray& coords, bool gradient, const std::string& space) const
+    {
+        parse_space(space);
+        if (coords.ndim() != 2 || coords.shape(1) != 3)
+            throw py::value_error("expected coords[ngrid,3] in Bohr");
+        const py::ssize_t n = coords.shape(0);
+        std::vector<py::ssize_t> shape = gradient ? std::vector<py::ssize_t>{3, n, (py::ssize_t)nao_}
+                                                  : std::vector<py::ssize_t>{n, (py::ssize_t)nao_};
+        py::array_t<double> out(shape);
+        if (n == 0 || nao_ == 0) return out;
+        const double* src = coords.data();
+        double* dst = out.mutable_data();
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_eval_ao(get(), src, n, gradient ? 1 : 0, dst);
+        }
+        check(status);
+        return out;
+    }
+
+private:
+    std::shared_ptr<mpisdf_context> handle_;
+    int64_t nao_ = 0;
+    bool cart_ = false;
+};
+
+std::vector<int32_t> read_axes(py::object axes)
+{
+    std::vector<AxisType> list;
+    if (py::isinstance<py::list>(axes) || py::isinstance<py::tuple>(axes)) list = axes.cast<std::vector<AxisType>>();
+    else list = {axes.cast<AxisType>()};
+    std::vector<int32_t> codes;
+    for (auto a : list) codes.push_back(static_cast<int32_t>(a));
+    return codes;
+}
+
+std::vector<double> read_coefficients(const Context& ctx, py::object mo_coeff)
+{
+    if (mo_coeff.is_none()) return {};
+    const auto c = mo_coeff.cast<FloatArray>();
+    const auto nao = static_cast<py::ssize_t>(ctx.nao());
+    if (c.ndim() != 2 || c.shape(0) != nao || c.shape(1) != nao)
+        throw py::value_error("mo_coeff must have shape (nao,nao)");
+    return {c.data(), c.data() + c.size()};
+}
+
+const double* read_points(const FloatArray& points)
+{
+    if (points.ndim() != 2 || points.shape(1) != 3) throw py::value_error("expected points[ngrid,3]");
+    return points.data();
+}
+
+class Selector {
+public:
+    explicit Selector(const Context& ctx) : ctx_(ctx) {}
+
+    py::array_t<double> evaluate_axis(const FloatArray& points, py::object axes, py::object mo_coeff,
+                                      int64_t nocc, const std::string& space) const
+    {
+        parse_space(space);
+        const double* xyz = read_points(points);
+        const auto codes = read_axes(axes);
+        const auto coeff = read_coefficients(ctx_, mo_coeff);
+        int64_t width = 0;
+        check(mpisdf_evaluate_axis(ctx_.get(), xyz, 0, codes.data(), (int32_t)codes.size(),
+                                   coeff.empty() ? nullptr : coeff.data(), nocc, nullptr, &width));
+        py::array_t<double> out({points.shape(0), (py::ssize_t)width});
+        if (points.shape(0) == 0) return out;
+        double* dst = out.mutable_data();
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_evaluate_axis(ctx_.get(), xyz, points.shape(0), codes.data(), (int32_t)codes.size(),
+                                          coeff.empty() ? nullptr : coeff.data(), nocc, dst, &width);
+        }
+        check(status);
+        return out;
+    }
+
+    std::vector<int64_t> select(const FloatArray& points, const FloatArray& weights, py::object axes,
+                                double tolerance, int64_t max_rank, LocalAlgorithm algorithm,
+                                py::object mo_coeff, int64_t nocc, const std::string& space,
+                                int64_t batch_size, bool debug, int64_t sketch_size, int64_t oversampling,
+                                uint64_t seed, bool audit, py::object cvt_weights, int64_t cvt_max_iter,
+                                double cvt_tol, const std::vector<double>& axis_weights,
+                                py::object point_importance, double ao_screening_threshold,
+                                const std::string& precision) const
+    {
+        (void)cvt_max_iter; (void)cvt_tol; (void)ao_screening_threshold;
+        parse_space(space);
+        const double* xyz = read_points(points);
+        if (weights.ndim() != 1 || weights.shape(0) != points.shape(0))
+            throw py::value_error("expected weights[ngrid]");
+        const auto codes = read_axes(axes);
+        const auto coeff = read_coefficients(ctx_, mo_coeff);
+        if (!cvt_weights.is_none() || !point_importance.is_none() ||
+            algorithm == LocalAlgorithm::CentroidalVoronoi ||
+            algorithm == LocalAlgorithm::RandomizedAODensityKhatriRao ||
+            algorithm == LocalAlgorithm::RandomizedAODensityNormalizedKhatriRao)
+            throw py::value_error("CVT and density Khatri-Rao selection are not part of this build; use "
+                                  "QRColumnPivoting, OnlineQRColumnPivoting or RandomizedQuadratureKhatriRao");
+        if (audit) throw py::value_error("audit=True is not part of this build");
+        if (batch_size < 0 || sketch_size < 0 || oversampling < 0 || max_rank < 0 || nocc < 0)
+            throw py::value_error("sizes must be non-negative");
+        int prec = 0;
+        if (precision == "fp64") prec = 0;
+        else if (precision == "fp32") prec = 1;
+        else if (precision == "bf16") prec = 2;
+        else if (precision == "fp8") prec = 3;
+        else throw py::value_error("precision must be 'fp64', 'fp32', 'bf16' or 'fp8'");
+        std::vector<int64_t> out(std::max<py::ssize_t>(points.shape(0), 1));
+        int64_t count = 0;
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_select(ctx_.get(), xyz, weights.data(), points.shape(0), codes.data(),
+                                   (int32_t)codes.size(), coeff.empty() ? nullptr : coeff.data(), nocc,
+                                   axis_weights.empty() ? nullptr : axis_weights.data(),
+                                   (int64_t)axis_weights.size(), tolerance, max_rank,
+                                   static_cast<int32_t>(algorithm), batch_size, debug ? 1 : 0, sketch_size,
+                                   oversampling, seed, prec, out.data(), (int64_t)out.size(), &count, nullptr);
+        }
+        check(status);
+        out.resize(count);
+        return out;
+    }
+
+private:
+    const Context& ctx_;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(mp_isdf, m)
+{
+    m.doc() = "MI355X (gfx950) AO evaluation and ISDF anchor selection (drop-in for the reference mp_isdf)";
+    const bool dev = device_available();
+    m.attr("device_enabled") = dev;
+    // Reference name of the same flag (cpp/src/ao_eval_pybind.cpp:73-77).
+    m.attr("cuda_enabled") = dev;
+
+    py::class_<Context>(m, "LibcintContext",
+                        "Own a private copy of libcint tables and cache AO data on the GPU.")
+        .def(py::init<const IntArray&, const IntArray&, const FloatArray&, bool>(), py::arg("atm"),
+             py::arg("bas"), py::arg("env"), py::arg("cart") = false)
+        .def_property_readonly("nao", &Context::nao)
+        .def_property_readonly("cart", &Context::cart)
+        .def("eval_ao", [](const Context& c, const FloatArray& coords, const std::string& space) {
+            return c.evaluate(coords, false, space);
+        }, py::arg("coords"), py::arg("space") = "auto", "Return AO values with shape (ngrid, nao).")
+        .def("eval_ao_gradient", [](const Context& c, const FloatArray& coords, const std::string& space) {
+            return c.evaluate(coords, true, space);
+        }, py::arg("coords"), py::arg("space") = "auto",
+           "Return x,y,z AO gradients with shape (3, ngrid, nao).");
+
+    m.def("eval_ao", [](const IntArray& atm, const IntArray& bas, const FloatArray& env,
+                        const FloatArray& coords, const std::string& space) {
+        parse_space(space);
+        return Context(atm, bas, env, false).evaluate(coords, false, space);
+    }, py::arg("atm"), py::arg("bas"), py::arg("env"), py::arg("coords"), py::arg("space") = "auto");
+    m.def("eval_ao_gradient", [](const IntArray& atm, const IntArray& bas, const FloatArray& env,
+                                 const FloatArray& coords, const std::string& space) {
+        parse_space(space);
+        return Context(atm, bas, env, false).evaluate(coords, true, space);
+    }, py::arg("atm"), py::arg("bas"), py::arg("env"), py::arg("coords"), py::arg("space") = "auto");
+
+    py::enum_<AxisType>(m, "AxisType")
+        .value("AtomicOrbital", AxisType::AtomicOrbital)
+        .value("AtomicOrbitalGradX", AxisType::AtomicOrbitalGradX)
+        .value("AtomicOrbitalGradY", AxisType::AtomicOrbitalGradY)
+        .value("AtomicOrbitalGradZ", AxisType::AtomicOrbitalGradZ)
+        .value("OccupiedOrbital", AxisType::OccupiedOrbital);
+    py::enum_<LocalAlgorithm>(m, "LocalAnchorSelectionAlgorithm")
+        .value("QRColumnPivoting", LocalAlgorithm::QRColumnPivoting)
+        .value("OnlineQRColumnPivoting", LocalAlgorithm::OnlineQRColumnPivoting)
+        .value("RandomizedQuadratureKhatriRao", LocalAlgorithm::RandomizedQuadratureKhatriRao)
+        .value("CentroidalVoronoi", LocalAlgorithm::CentroidalVoronoi)
+        .value("RandomizedAODensityKhatriRao", LocalAlgorithm::RandomizedAODensityKhatriRao)
+        .value("RandomizedAODensityNormalizedKhatriRao", LocalAlgorithm::RandomizedAODensityNormalizedKhatriRao);
+    py::enum_<DistributedAlgorithm>(m, "DistributedAnchorSelectionAlgorithm")
+        .value("SVD", DistributedAlgorithm::SVD)
+        .value("RandomizedQRColumnPivoting", DistributedAlgorithm::RandomizedQRColumnPivoting);
+
+    py::class_<Selector>(m, "AnchorSelection")
+        .def(py::init<const Context&>(), py::arg("context"), py::keep_alive<1, 2>())
+        .def("select", &Selector::select, py::arg("points"), py::arg("weights"), py::arg("axes"),
+             py::arg("tolerance") = 1e-10, py::arg("max_rank") = 0,
+             py::arg("algorithm") = LocalAlgorithm::QRColumnPivoting, py::arg("mo_coeff") = py::none(),
+             py::arg("nocc") = 0, py::arg("space") = "auto", py::arg("batch_size") = 128,
+             py::arg("debug") = false, py::arg("sketch_size") = 128, py::arg("oversampling") = 16,
+             py::arg("seed") = 7, py::arg("audit") = false, py::arg("cvt_weights") = py::none(),
+             py::arg("cvt_max_iter") = 100, py::arg("cvt_tol") = 1e-5,
+             py::arg("axis_weights") = std::vector<double>{}, py::arg("point_importance") = py::none(),
+             py::arg("ao_screening_threshold") = 1e-24, py::arg("precision") = "fp64",
+             "Select original grid indices in pivot order (GPU).  precision selects the sketch "
+             "arithmetic of RandomizedQuadratureKhatriRao: fp64 (reference semantics), fp32, bf16 or fp8.")
+        .def("evaluate_axis", &Selector::evaluate_axis, py::arg("points"), py::arg("axes"),
+             py::arg("mo_coeff") = py::none(), py::arg("nocc") = 0, py::arg("space") = "auto",
+             "Evaluate concatenated axis functions on the GPU.");
+
+    struct Distributed {
+        explicit Distributed(const Context&) {}
+    };
+    py::class_<Distributed>(m, "DistributedAnchorSelection")
+        .def(py::init<const Context&>(), py::arg("context"), py::keep_alive<1, 2>())
+        .def("select", [](const Distributed&, py::object, py::object, py::object, DistributedAlgorithm) {
+            throw std::runtime_error("distributed SLATE SVD anchor selection is not implemented");
+        }, py::arg("points"), py::arg("weights"), py::arg("axes"),
+           py::arg("algorithm") = DistributedAlgorithm::SVD);
+}

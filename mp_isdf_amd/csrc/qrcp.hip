@@ -1,0 +1,452 @@
+// Column-pivoted QR with greedy max-norm pivoting, restated for the GPU.
+//
+// Replaces the full LAPACK/MAGMA dgeqp3 factorisation the reference runs
+// (cpp/src/anchor_select_qrcp.cpp:36-90) with an algorithm that yields the
+// same pivot order but whose bulk work is GEMM-shaped and never writes Y:
+//
+//   residual norm of column j after k pivots:
+//        nu_j^2 = ||y_j||^2 - sum_{t<k} (q_t^T y_j)^2,   Q = [q_0 .. q_{k-1}]
+//
+// * Blocks.  After each block of new pivots the projections Q_new^T Y are
+//   folded into the per-column sums by one GEMM over all columns (plus row
+//   energies sum_j (q_t^T y_j)^2, the squared row norms of R).
+// * Candidates.  The C columns with the largest upper bounds
+//   nu_j^2 + margin * ||y_j||^2 get explicit FP64 residual vectors
+//   r_c = y_c - Q Q^T y_c (classical Gram-Schmidt, two passes).  Inside the
+//   block every pivot is the exact argmax of the candidate residual norms
+//   (lowest column index on ties, as dgeqp3's idamax), accepted only while it
+//   beats the bound of every non-candidate column; otherwise the block ends
+//   and the new q's are folded.  Each accepted pivot updates all candidate
+//   residuals with one rank-1 step and recomputes their norms exactly.
+//
+// Non-increasing residual norms make the stale bounds valid, so the pivot
+// sequence equals the greedy QRCP sequence up to FP64 rounding of the norms.
+#include "qrcp.hpp"
+
+#include "kernels.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+namespace mpisdf {
+namespace {
+
+struct ArgMax {
+    double value;
+    int64_t slot;
+    int64_t column;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <class T>
+__global__ void colnorm2_kernel(const T* __restrict__ y, int64_t m, int64_t n, int64_t ldy,
+                                double* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t col = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (col >= n) return;
+    const T* c = y + col * ldy;
+    double s = 0;
+    for (int64_t i = lane; i < m; i += 64) { const double v = (double)c[i]; s += v * v; }
+    s = wave_sum(s);
+    if (lane == 0) out[col] = s;
+}
+
+__global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __restrict__ proj2,
+                            const uint8_t* __restrict__ selected, double margin, int64_t n,
+                            double* __restrict__ keys, int64_t* __restrict__ idx)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        keys[j] = selected[j] ? -1.0 : fmax(ynorm2[j] - proj2[j], 0.0) + margin * ynorm2[j];
+        idx[j] = j;
+    }
+}
+
+template <class T>
+__global__ void gather_kernel(const T* __restrict__ y, int64_t ldy, int64_t m,
+                              const int64_t* __restrict__ cand, int64_t C, double* __restrict__ rc)
+{
+    const int64_t c = blockIdx.y;
+    if (c >= C) return;
+    const T* src = y + cand[c] * ldy;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        rc[i + c * m] = (double)src[i];
+}
+
+__global__ void cand_norm_kernel(const double* __restrict__ rc, int64_t m, int64_t C, double* __restrict__ rn2)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t c = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (c >= C) return;
+    double s = 0;
+    for (int64_t i = lane; i < m; i += 64) { const double v = rc[i + c * m]; s += v * v; }
+    s = wave_sum(s);
+    if (lane == 0) rn2[c] = s;
+}
+
+__global__ __launch_bounds__(1024) void argmax_kernel(const double* __restrict__ rn2,
+                                                      const int64_t* __restrict__ cand, int64_t C,
+                                                      ArgMax* __restrict__ out)
+{
+    __shared__ double sv[1024];
+    __shared__ int64_t ss[1024], sc[1024];
+    double best = -2.0;
+    int64_t slot = -1, col = std::numeric_limits<int64_t>::max();
+    for (int64_t c = threadIdx.x; c < C; c += blockDim.x) {
+        const double v = rn2[c];
+        if (v > best || (v == best && cand[c] < col)) { best = v; slot = c; col = cand[c]; }
+    }
+    sv[threadIdx.x] = best; ss[threadIdx.x] = slot; sc[threadIdx.x] = col;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double v = sv[threadIdx.x + s];
+            const int64_t cc = sc[threadIdx.x + s];
+            if (v > sv[threadIdx.x] || (v == sv[threadIdx.x] && cc < sc[threadIdx.x])) {
+                sv[threadIdx.x] = v; ss[threadIdx.x] = ss[threadIdx.x + s]; sc[threadIdx.x] = cc;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out->value = sv[0]; out->slot = ss[0]; out->column = sc[0]; }
+}
+
+// q <- r_c* / ||r_c*||  (the norm comes from the exact residual norm)
+__global__ void make_q_kernel(const double* __restrict__ rc, int64_t m, const ArgMax* __restrict__ am,
+                              double* __restrict__ q, uint8_t* __restrict__ selected)
+{
+    const double inv = 1.0 / sqrt(am->value);
+    const double* src = rc + am->slot * m;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        q[i] = src[i] * inv;
+    if (blockIdx.x == 0 && threadIdx.x == 0) selected[am->column] = 1;
+}
+
+// d[t] = Q[:, t0 + t]^T q, one workgroup per t.
+__global__ void dots_kernel(const double* __restrict__ Q, int64_t m, int64_t t0, const double* __restrict__ q,
+                            double* __restrict__ d)
+{
+    __shared__ double part[4];
+    const double* col = Q + (t0 + blockIdx.x) * m;
+    double s = 0;
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += col[i] * q[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) d[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
+// q -= sum_t d[t] Q[:, t0 + t]; then written (unnormalised) to out
+__global__ void reorth_kernel(double* __restrict__ q, const double* __restrict__ Q, int64_t m, int64_t t0,
+                              int64_t nt, const double* __restrict__ d)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = q[i];
+        for (int64_t t = 0; t < nt; ++t) v -= d[t] * Q[(t0 + t) * m + i];
+        q[i] = v;
+    }
+}
+
+// Normalise q in one workgroup and store it as column k of Q.
+__global__ __launch_bounds__(1024) void store_q_kernel(const double* __restrict__ q, int64_t m,
+                                                       double* __restrict__ qk)
+{
+    __shared__ double part[16];
+    double s = 0;
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += q[i] * q[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        double v = threadIdx.x < blockDim.x / 64 ? part[threadIdx.x] : 0.0;
+        v = wave_sum(v);
+        if (threadIdx.x == 0) part[0] = v;
+    }
+    __syncthreads();
+    const double inv = 1.0 / sqrt(part[0]);
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) qk[i] = q[i] * inv;
+}
+
+// For every candidate c: h = q^T r_c; r_c -= h q; rn2[c] = ||r_c||^2.  The
+// pivot's own slot is retired (rn2 = -1).
+__global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ rc, int64_t m, int64_t C,
+                                                          const double* __restrict__ q, double* __restrict__ rn2,
+                                                          const ArgMax* __restrict__ am)
+{
+    __shared__ double part[4];
+    __shared__ double hsh;
+    const int64_t c = blockIdx.x;
+    if (c >= C) return;
+    if (c == am->slot || rn2[c] < 0) {
+        if (threadIdx.x == 0) rn2[c] = -1.0;
+        return;
+    }
+    double* col = rc + c * m;
+    double s = 0;
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += q[i] * col[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) hsh = part[0] + part[1] + part[2] + part[3];
+    __syncthreads();
+    const double h = hsh;
+    double n2 = 0;
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const double v = col[i] - h * q[i];
+        col[i] = v;
+        n2 += v * v;
+    }
+    n2 = wave_sum(n2);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = n2;
+    __syncthreads();
+    if (threadIdx.x == 0) rn2[c] = part[0] + part[1] + part[2] + part[3];
+}
+
+// proj2[j] += sum_t W[t, j]^2 ; energy[t] += sum_j W[t, j]^2   (W is b x nc, ld b)
+__global__ __launch_bounds__(256) void fold_kernel(const double* __restrict__ W, int64_t b, int64_t nc,
+                                                   double* __restrict__ proj2, double* __restrict__ energy)
+{
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nc) {
+        double s = 0;
+        for (int64_t t = 0; t < b; ++t) { const double w = W[t + j * b]; s += w * w; }
+        proj2[j] += s;
+    }
+    if (energy) {
+        // Row energies: one pass per row over this block's columns.
+        __shared__ double part[4];
+        for (int64_t t = 0; t < b; ++t) {
+            const double w = j < nc ? W[t + j * b] : 0.0;
+            double s = wave_sum(w * w);
+            if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
+            __syncthreads();
+            if (threadIdx.x == 0) atomicAdd(&energy[t], part[0] + part[1] + part[2] + part[3]);
+            __syncthreads();
+        }
+    }
+}
+
+// tail2 += sum (Y - Q W)^2 over a column chunk; R holds Y chunk - Q W (m x nc).
+__global__ void sumsq_kernel(const double* __restrict__ r, int64_t count, double* __restrict__ out)
+{
+    __shared__ double part[4];
+    double s = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        s += r[i] * r[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
+template <class T>
+__global__ void widen_kernel(const T* __restrict__ y, int64_t ldy, int64_t m, int64_t nc, double* __restrict__ out)
+{
+    const int64_t total = m * nc;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = i / m, r = i - j * m;
+        out[i] = (double)y[r + j * ldy];
+    }
+}
+
+int grid_for(int64_t work, int threads = 256, int64_t cap = 65536)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>((work + threads - 1) / threads, cap));
+}
+
+void project(const void* y, DType ytype, int64_t m, int64_t ldy, int64_t j0, int64_t nc, const double* Qb,
+             int64_t b, double* W, hipStream_t s)
+{
+    // W (b x nc) = Qb^T (b x m) * Y[:, j0:j0+nc]
+    if (ytype == kF64)
+        gemm_f64(b, nc, m, 1.0, MatView{Qb, m, 1}, MatView{static_cast<const double*>(y) + j0 * ldy, 1, ldy}, 0.0,
+                 MutView{W, 1, b}, s);
+    else
+        gemm_f64_f32b(b, nc, m, 1.0, MatView{Qb, m, 1}, static_cast<const float*>(y) + j0 * ldy, 1, ldy, 0.0,
+                      MutView{W, 1, b}, s);
+}
+
+}  // namespace
+
+QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t ldy, const QrcpOptions& opt,
+                       hipStream_t s)
+{
+    if (ytype != kF64 && ytype != kF32) throw std::invalid_argument("QRCP input must be f64 or f32");
+    if (m <= 0 || n <= 0) return {};
+    if (ldy < m) throw std::invalid_argument("QRCP leading dimension smaller than rows");
+    const int64_t kmax = std::min({opt.kmax > 0 ? opt.kmax : std::min(m, n), m, n});
+    const int64_t C = std::max<int64_t>(1, std::min(opt.candidates, n));
+    QrcpResult res;
+
+    DeviceBuffer<double> ynorm2(n), proj2(n), keys(n), keys_sorted(n);
+    DeviceBuffer<int64_t> idx(n), idx_sorted(n);
+    DeviceBuffer<uint8_t> selected(n);
+    DeviceBuffer<double> Q((size_t)m * std::max<int64_t>(kmax, 1)), rc((size_t)m * C), rn2(C), H, qv(m), dv;
+    DeviceBuffer<double> energy(std::max<int64_t>(kmax, 1)), tail(1);
+    DeviceBuffer<ArgMax> am(1);
+    MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
+    MPISDF_HIP(hipMemsetAsync(selected.ptr, 0, n, s));
+    MPISDF_HIP(hipMemsetAsync(energy.ptr, 0, std::max<int64_t>(kmax, 1) * sizeof(double), s));
+
+    if (ytype == kF64)
+        hipLaunchKernelGGL(colnorm2_kernel<double>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s,
+                           static_cast<const double*>(y), m, n, ldy, ynorm2.ptr);
+    else
+        hipLaunchKernelGGL(colnorm2_kernel<float>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s,
+                           static_cast<const float*>(y), m, n, ldy, ynorm2.ptr);
+    MPISDF_HIP(hipGetLastError());
+    {
+        const auto h = ynorm2.download(s);
+        double f = 0;
+        for (double v : h) f += v;
+        res.frob2 = f;
+    }
+
+    // hipCUB sort workspace
+    size_t sort_bytes = 0;
+    MPISDF_HIP(hipcub::DeviceRadixSort::SortPairsDescending(nullptr, sort_bytes, keys.ptr, keys_sorted.ptr,
+                                                            idx.ptr, idx_sorted.ptr, (int)n, 0, 64, s));
+    if (n > std::numeric_limits<int>::max()) throw std::overflow_error("QRCP column count exceeds int range");
+    DeviceBuffer<uint8_t> sort_ws(std::max<size_t>(sort_bytes, 1));
+
+    // Projection scratch: W is b x chunk.
+    const int64_t wcap = 1ll << 27;  // doubles (1 GiB)
+    DeviceBuffer<double> W;
+
+    int64_t k = 0, folded = 0;
+    double r00 = 0;
+    bool done = false;
+    auto fold = [&](double* energy_out) {
+        const int64_t b = k - folded;
+        if (b <= 0) return;
+        const int64_t nc = std::max<int64_t>(1, std::min<int64_t>(n, wcap / b));
+        if (W.count < (size_t)(b * nc)) W.resize((size_t)(b * nc));
+        for (int64_t j0 = 0; j0 < n; j0 += nc) {
+            const int64_t cnt = std::min(nc, n - j0);
+            project(y, ytype, m, ldy, j0, cnt, Q.ptr + folded * m, b, W.ptr, s);
+            hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, W.ptr, b, cnt,
+                               proj2.ptr + j0, energy_out ? energy_out + folded : nullptr);
+            MPISDF_HIP(hipGetLastError());
+        }
+        folded = k;
+        ++res.folds;
+    };
+
+    while (!done && k < kmax) {
+        fold(opt.want_energy ? energy.ptr : nullptr);
+        hipLaunchKernelGGL(keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, ynorm2.ptr, proj2.ptr, selected.ptr,
+                           opt.margin, n, keys.ptr, idx.ptr);
+        MPISDF_HIP(hipGetLastError());
+        MPISDF_HIP(hipcub::DeviceRadixSort::SortPairsDescending(sort_ws.ptr, sort_bytes, keys.ptr, keys_sorted.ptr,
+                                                                idx.ptr, idx_sorted.ptr, (int)n, 0, 64, s));
+        const int64_t remaining = n - k;
+        const int64_t Ceff = std::min(C, remaining);
+        double bound2 = -1.0;
+        if (Ceff < remaining)
+            MPISDF_HIP(hipMemcpyAsync(&bound2, keys_sorted.ptr + Ceff, sizeof(double), hipMemcpyDeviceToHost, s));
+        if (ytype == kF64)
+            hipLaunchKernelGGL(gather_kernel<double>, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
+                               static_cast<const double*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr);
+        else
+            hipLaunchKernelGGL(gather_kernel<float>, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
+                               static_cast<const float*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr);
+        MPISDF_HIP(hipGetLastError());
+        DeviceBuffer<int64_t> cand;
+        cand.resize(Ceff);
+        MPISDF_HIP(hipMemcpyAsync(cand.ptr, idx_sorted.ptr, Ceff * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        if (k > 0) {
+            if (H.count < (size_t)(k * Ceff)) H.resize((size_t)(k * Ceff));
+            for (int pass = 0; pass < 2; ++pass) {
+                gemm_f64(k, Ceff, m, 1.0, MatView{Q.ptr, m, 1}, MatView{rc.ptr, 1, m}, 0.0, MutView{H.ptr, 1, k}, s);
+                gemm_f64(m, Ceff, k, -1.0, MatView{Q.ptr, 1, m}, MatView{H.ptr, 1, k}, 1.0, MutView{rc.ptr, 1, m}, s);
+            }
+        }
+        hipLaunchKernelGGL(cand_norm_kernel, dim3((unsigned)((Ceff + 3) / 4)), dim3(256), 0, s, rc.ptr, m, Ceff,
+                           rn2.ptr);
+        MPISDF_HIP(hipGetLastError());
+        ++res.blocks;
+        const int64_t block_start = k;
+        if (dv.count < (size_t)std::max<int64_t>(kmax, 1)) dv.resize(std::max<int64_t>(kmax, 1));
+        while (k < kmax) {
+            hipLaunchKernelGGL(argmax_kernel, dim3(1), dim3(1024), 0, s, rn2.ptr, cand.ptr, Ceff, am.ptr);
+            MPISDF_HIP(hipGetLastError());
+            ArgMax best{};
+            MPISDF_HIP(hipMemcpyAsync(&best, am.ptr, sizeof(ArgMax), hipMemcpyDeviceToHost, s));
+            MPISDF_HIP(hipStreamSynchronize(s));
+            if (best.slot < 0 || !(best.value > 0)) {
+                if (bound2 <= 0) done = true;  // every remaining column is exactly zero
+                break;
+            }
+            const double norm = std::sqrt(best.value);
+            if (k == 0) r00 = norm;
+            if (norm <= opt.stop_abs || norm <= opt.stop_rel * r00) { done = true; break; }
+            // Certification against the stale bound of the non-candidates.  The
+            // first pivot after a fold may tie the bound: candidates are the
+            // lowest-index columns among equal keys (stable sort).
+            const bool first_after_fold = (k == block_start);
+            if (!(best.value > bound2 || (first_after_fold && best.value >= bound2 && opt.margin == 0.0))) break;
+            res.pivots.push_back(best.column);
+            res.rdiag.push_back(norm);
+            hipLaunchKernelGGL(make_q_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, rc.ptr, m, am.ptr, qv.ptr,
+                               selected.ptr);
+            const int64_t nt = k - block_start;
+            if (nt > 0) {
+                hipLaunchKernelGGL(dots_kernel, dim3((unsigned)nt), dim3(256), 0, s, Q.ptr, m, block_start, qv.ptr,
+                                   dv.ptr);
+                hipLaunchKernelGGL(reorth_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, qv.ptr, Q.ptr, m,
+                                   block_start, nt, dv.ptr);
+            }
+            hipLaunchKernelGGL(store_q_kernel, dim3(1), dim3(1024), 0, s, qv.ptr, m, Q.ptr + k * m);
+            hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, Ceff,
+                               Q.ptr + k * m, rn2.ptr, am.ptr);
+            MPISDF_HIP(hipGetLastError());
+            ++k;
+        }
+        if (k == kmax) break;
+    }
+    if (opt.want_energy) {
+        fold(energy.ptr);
+        res.energy.resize(k);
+        if (k) MPISDF_HIP(hipMemcpyAsync(res.energy.data(), energy.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    if (opt.want_tail) {
+        // Exact residual after the last pivot: Y - Q Q^T Y, column chunks.
+        MPISDF_HIP(hipMemsetAsync(tail.ptr, 0, sizeof(double), s));
+        const int64_t nc = std::max<int64_t>(1, std::min<int64_t>(n, (1ll << 26) / std::max<int64_t>(m, 1)));
+        DeviceBuffer<double> R((size_t)(m * nc));
+        DeviceBuffer<double> Wt(k > 0 ? (size_t)(k * nc) : 1);
+        for (int64_t j0 = 0; j0 < n; j0 += nc) {
+            const int64_t cnt = std::min(nc, n - j0);
+            if (ytype == kF64)
+                hipLaunchKernelGGL(widen_kernel<double>, dim3(grid_for(m * cnt)), dim3(256), 0, s,
+                                   static_cast<const double*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);
+            else
+                hipLaunchKernelGGL(widen_kernel<float>, dim3(grid_for(m * cnt)), dim3(256), 0, s,
+                                   static_cast<const float*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);
+            if (k > 0) {
+                for (int pass = 0; pass < 2; ++pass) {
+                    gemm_f64(k, cnt, m, 1.0, MatView{Q.ptr, m, 1}, MatView{R.ptr, 1, m}, 0.0, MutView{Wt.ptr, 1, k}, s);
+                    gemm_f64(m, cnt, k, -1.0, MatView{Q.ptr, 1, m}, MatView{Wt.ptr, 1, k}, 1.0, MutView{R.ptr, 1, m}, s);
+                }
+            }
+            hipLaunchKernelGGL(sumsq_kernel, dim3(grid_for(m * cnt, 256, 4096)), dim3(256), 0, s, R.ptr, m * cnt,
+                               tail.ptr);
+            MPISDF_HIP(hipGetLastError());
+        }
+        MPISDF_HIP(hipMemcpyAsync(&res.tail2, tail.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    MPISDF_HIP(hipStreamSynchronize(s));
+    return res;
+}
+
+}  // namespace mpisdf

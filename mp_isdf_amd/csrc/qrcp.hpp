@@ -1,0 +1,35 @@
+// Column-pivoted QR (greedy Businger-Golub pivot order) on the GPU.
+#pragma once
+
+#include "common.hpp"
+
+namespace mpisdf {
+
+struct QrcpOptions {
+    int64_t kmax = 0;            // maximum pivots (<= min(m, n))
+    double stop_abs = 0.0;       // stop before a pivot whose norm is <= stop_abs
+    double stop_rel = 0.0;       // ... or <= stop_rel * |R_00|
+    int64_t candidates = 1024;   // explicit FP64 residual columns per block
+    double margin = 0.0;         // relative error bound of the folded norms
+    bool want_energy = false;    // row energies sum_j (q_i^T y_j)^2 (randomized tail rule)
+    bool want_tail = false;      // exact residual Frobenius^2 after the last pivot
+    bool debug = false;
+};
+
+struct QrcpResult {
+    std::vector<int64_t> pivots;  // 0-based columns in pivot order
+    std::vector<double> rdiag;    // |R_ii|
+    std::vector<double> energy;   // per pivot row (if requested)
+    double frob2 = 0.0;           // ||Y||_F^2
+    double tail2 = 0.0;           // residual Frobenius^2 after the last pivot (if requested)
+    int64_t blocks = 0, folds = 0;
+};
+
+// Y is m x n column-major with leading dimension ldy, element type f64 or f32.
+// Y is read only.  Pivot decisions use exact FP64 residual norms of the
+// candidate columns (LAPACK dgeqp3 pivot semantics: max norm, lowest index on
+// ties).
+QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t ldy,
+                       const QrcpOptions& opt, hipStream_t stream);
+
+}  // namespace mpisdf

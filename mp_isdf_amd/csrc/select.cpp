@@ -1,0 +1,330 @@
+// Anchor selection on the GPU.
+//
+// Mirrors the reference's AnchorSelection::select (cpp/src/anchor_select.cpp:79-173):
+// validation, global multi-axis Frobenius scales, then one of
+//   * QRColumnPivoting          cpp/src/anchor_select_qrcp.cpp:95-225
+//   * OnlineQRColumnPivoting    cpp/src/anchor_select_online_qr.cpp:127-375
+//   * RandomizedQuadratureKhatriRao  cpp/src/anchor_select_randomized_id.cpp:92-374
+// with the same validation messages, rank/tolerance rules and returned
+// index order (original grid indices, pivot order).  All grid work runs on the
+// device; only pivots, norms and small vectors return to the host.
+#include "select.hpp"
+
+#include "kernels.hpp"
+#include "online.hpp"
+#include "qrcp.hpp"
+#include "sketch.hpp"
+#include "util.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <iostream>
+#include <limits>
+#include <random>
+
+namespace mpisdf {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+double seconds_since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+constexpr double kEps = std::numeric_limits<double>::epsilon();
+
+struct Kept {
+    std::vector<double> xyz, w;
+    std::vector<int64_t> original;
+};
+
+Kept filter_points(const SelectArgs& a)
+{
+    Kept k;
+    k.xyz.reserve(3 * a.n);
+    for (int64_t i = 0; i < a.n; ++i) {
+        if (!std::isfinite(a.weights[i])) throw std::invalid_argument("grid weights must be finite");
+        if (a.weights[i] == 0) continue;
+        for (int c = 0; c < 3; ++c)
+            if (!std::isfinite(a.points[3 * i + c])) throw std::invalid_argument("grid coordinates must be finite");
+        k.xyz.insert(k.xyz.end(), a.points + 3 * i, a.points + 3 * i + 3);
+        k.w.push_back(a.weights[i]);
+        k.original.push_back(i);
+    }
+    return k;
+}
+
+// Device batch for feature evaluation (independent of the reference's
+// host-memory batch_size, which does not change results).
+int64_t device_batch(int64_t width, int64_t nao, int64_t extra_per_point)
+{
+    const int64_t per_point = 8 * (width + 4 * nao + extra_per_point);
+    return std::max<int64_t>(1024, std::min<int64_t>(1 << 16, (int64_t(2) << 30) / std::max<int64_t>(per_point, 1)));
+}
+
+// Global per-axis scales (cpp/src/anchor_select.cpp:103-147).
+std::vector<double> axis_scales(Context& ctx, const SelectArgs& a, const AxisSpec& spec, hipStream_t s)
+{
+    std::vector<double> scales;
+    if (a.axes.size() <= 1) return scales;
+    if (a.n > 0)
+        for (int64_t g = 0; g < a.n; ++g)
+            if (!std::isfinite(a.weights[g])) throw std::invalid_argument("grid weights must be finite");
+    Kept kept = filter_points(a);
+    const int64_t n = (int64_t)kept.original.size();
+    if (n == 0) return scales;
+    const int64_t batch = std::min(n, device_batch(spec.width, ctx.nao, 0));
+    DeviceBuffer<double> d_xyz(3 * batch), feat(batch * spec.width), sums(spec.width);
+    MPISDF_HIP(hipMemsetAsync(sums.ptr, 0, spec.width * sizeof(double), s));
+    AxisWorkspace ws;
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t count = std::min(batch, n - start);
+        MPISDF_HIP(hipMemcpyAsync(d_xyz.ptr, kept.xyz.data() + 3 * start, 3 * count * sizeof(double),
+                                  hipMemcpyHostToDevice, s));
+        evaluate_axis_device(ctx, spec, d_xyz.ptr, count, nullptr, a.mo_coeff, a.nocc, feat.ptr, ws, s);
+        launch_colsumsq(feat.ptr, count, spec.width, sums.ptr, s);
+    }
+    const auto h = sums.download(s);
+    int64_t off = 0;
+    for (size_t ax = 0; ax < spec.kinds.size(); ++ax) {
+        double total = 0;
+        for (int64_t j = 0; j < spec.widths[ax]; ++j) total += h[off + j];
+        off += spec.widths[ax];
+        const double norm = std::sqrt(total);
+        if (!(norm > 0) || !std::isfinite(norm))
+            throw std::invalid_argument("axis Frobenius norm must be finite and positive");
+        const double scale = (a.axis_weights.empty() ? 1.0 : a.axis_weights[ax]) / norm;
+        if (!(scale > 0) || !std::isfinite(scale))
+            throw std::invalid_argument("normalized axis scale must be finite and positive");
+        scales.push_back(scale);
+    }
+    return scales;
+}
+
+// ---------------------------------------------------------------------------
+// qrcp_dense (cpp/src/anchor_select_qrcp.cpp:95-225)
+// ---------------------------------------------------------------------------
+std::vector<int64_t> select_dense(Context& ctx, const SelectArgs& a, const AxisSpec& spec,
+                                  const std::vector<double>& scales, double* diag, hipStream_t s)
+{
+    if (!std::isfinite(a.tolerance) || a.tolerance < 0)
+        throw std::invalid_argument("invalid weights, axis list, or tolerance");
+    Kept kept = filter_points(a);
+    const int64_t ng = (int64_t)kept.original.size();
+    if (ng == 0) return {};
+    const int64_t w = spec.width;
+    if (w > (1 << 15) || ng > std::numeric_limits<int64_t>::max() / (w * w))
+        throw std::overflow_error("pair-product matrix is too large");
+    const int64_t m = w * w;
+    DeviceBuffer<double> d_xyz, feat(ng * w), sw(ng), A((size_t)(m * ng));
+    d_xyz.upload(kept.xyz.data(), kept.xyz.size(), s);
+    std::vector<double> hsw(ng);
+    for (int64_t g = 0; g < ng; ++g) hsw[g] = std::sqrt(std::abs(kept.w[g]));
+    sw.upload(hsw.data(), ng, s);
+    AxisWorkspace ws;
+    const int64_t batch = std::min(ng, device_batch(w, ctx.nao, 0));
+    for (int64_t start = 0; start < ng; start += batch) {
+        const int64_t count = std::min(batch, ng - start);
+        evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
+                             a.mo_coeff, a.nocc, feat.ptr + start * w, ws, s);
+    }
+    DeviceBuffer<unsigned long long> bad(1);
+    MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
+    launch_pair_product(feat.ptr, ng, w, sw.ptr, A.ptr, bad.ptr, s);
+    if (bad.download(s)[0]) throw std::overflow_error("non-finite weighted pair product");
+    const int64_t cap = std::min({m, ng, a.max_rank ? a.max_rank : std::min(m, ng)});
+    QrcpOptions opt;
+    opt.kmax = cap;
+    opt.stop_rel = a.tolerance;
+    opt.candidates = std::min<int64_t>(ng, 2048);
+    opt.debug = a.debug;
+    const auto t0 = Clock::now();
+    const QrcpResult r = qrcp_device(A.ptr, kF64, m, ng, m, opt, s);
+    std::vector<int64_t> out;
+    for (int64_t p : r.pivots) {
+        if (p < 0 || p >= ng) throw std::runtime_error("QRCP returned an invalid grid pivot");
+        out.push_back(kept.original[p]);
+    }
+    if (diag) {
+        diag[kDiagRows] = (double)m;
+        diag[kDiagColumns] = (double)ng;
+        diag[kDiagCap] = (double)cap;
+        diag[kDiagRank] = (double)out.size();
+        diag[kDiagBlocks] = (double)r.blocks;
+        diag[kDiagFolds] = (double)r.folds;
+        diag[kDiagSecondsQrcp] = seconds_since(t0);
+    }
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// rand_sketch, quadrature policy (cpp/src/anchor_select_randomized_id.cpp:92-374)
+// ---------------------------------------------------------------------------
+std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const AxisSpec& spec,
+                                       const std::vector<double>& scales, double* diag, hipStream_t s)
+{
+    const auto started = Clock::now();
+    if (!std::isfinite(a.tolerance) || a.tolerance < 0 || a.batch_size == 0 || a.oversampling == 0 ||
+        (!a.max_rank && a.sketch_size <= a.oversampling))
+        throw std::invalid_argument(
+            "invalid randomized ID grid, tolerance, batch, sketch or rank/oversampling");
+    if (a.max_rank > std::numeric_limits<int64_t>::max() - a.oversampling)
+        throw std::overflow_error("randomized ID rank + oversampling overflows size_t");
+    const int64_t requested = a.max_rank ? a.max_rank + a.oversampling : a.sketch_size;
+    const int64_t d = spec.width;
+    Kept kept = filter_points(a);
+    const int64_t n = (int64_t)kept.original.size();
+    if (n == 0) return {};
+    const int64_t physical_rank = d * (d + 1) / 2;
+    const int64_t physical_limit = physical_rank + a.oversampling;
+    const int64_t backend_limit = std::numeric_limits<int64_t>::max() / n;
+    const int64_t ell = requested <= backend_limit ? requested
+                                                   : std::min({requested, physical_limit, backend_limit});
+    if (ell <= a.oversampling || (a.max_rank && std::min(a.max_rank, physical_rank) > ell - a.oversampling))
+        throw std::invalid_argument(
+            "randomized ID sketch/rank exceeds the ILP64 QRCP backend limit for this grid; "
+            "reduce the sketch/rank or use a tiled QRCP backend");
+    if (a.max_rank > std::min(n, physical_rank))
+        throw std::invalid_argument("requested anchor_rank exceeds eligible candidates or physical pair rank");
+    if (a.precision != kPrecF64)
+        throw std::invalid_argument("reduced-precision sketches are provided by the sketch engine (pending)");
+
+    std::vector<double> hu, hv;
+    make_omega(d, ell, a.seed, hu, hv);
+    DeviceBuffer<double> u, v;
+    u.upload(hu.data(), hu.size(), s);
+    v.upload(hv.data(), hv.size(), s);
+
+    DeviceBuffer<double> y((size_t)ell * n);
+    DeviceBuffer<double> d_xyz, sw(n);
+    d_xyz.upload(kept.xyz.data(), kept.xyz.size(), s);
+    {
+        std::vector<double> hsw(n);
+        for (int64_t g = 0; g < n; ++g) hsw[g] = std::sqrt(std::abs(kept.w[g]));
+        sw.upload(hsw.data(), n, s);
+    }
+    DeviceBuffer<unsigned long long> bad(1);
+    MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
+    const int64_t batch = std::min(n, device_batch(d, ctx.nao, 2 * ell));
+    DeviceBuffer<double> feat((size_t)batch * d);
+    AxisWorkspace ws;
+    SketchScratch scratch;
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t count = std::min(batch, n - start);
+        evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
+                             a.mo_coeff, a.nocc, feat.ptr, ws, s);
+        sketch_f64(feat.ptr, count, d, u.ptr, v.ptr, ell, sw.ptr + start, y.ptr + start * ell, ell, scratch, bad.ptr, s);
+    }
+    if (bad.download(s)[0]) throw std::overflow_error("non-finite randomized ID sketch");
+    const double t_sketch = seconds_since(started);
+
+    const int64_t steps = std::min(ell, n);
+    const int64_t structural = std::min({ell - a.oversampling, steps, physical_rank});
+    QrcpOptions opt;
+    opt.kmax = a.max_rank ? std::min(a.max_rank, structural) : structural;
+    // Numerical-rank floor eps * max(ell, n) * max|R_ii| (|R_00| is the max).
+    opt.stop_rel = kEps * (double)std::max(ell, n);
+    opt.candidates = std::min<int64_t>(n, 2048);
+    opt.want_energy = a.max_rank == 0;
+    opt.want_tail = a.max_rank == 0;
+    opt.debug = a.debug;
+    const auto tq = Clock::now();
+    const QrcpResult r = qrcp_device(y.ptr, kF64, ell, n, ell, opt, s);
+    const double t_qrcp = seconds_since(tq);
+    const int64_t K = (int64_t)r.pivots.size();
+    // numerical_rank: first i with |R_ii| <= floor; the engine stops there.
+    const int64_t numerical_rank = K < opt.kmax ? K : steps;
+    const int64_t cap = std::min({a.max_rank ? a.max_rank : ell - a.oversampling, ell - a.oversampling, steps,
+                                  physical_rank, numerical_rank});
+    if (a.max_rank && numerical_rank < a.max_rank)
+        throw std::invalid_argument("requested anchor_rank exceeds the numerical rank of the randomized sketch");
+    int64_t rank = a.max_rank ? a.max_rank : cap;
+    double norm = std::sqrt(r.frob2), rel_tail = 0;
+    if (!a.max_rank) {
+        // tails[i]^2 = tail2 + sum_{i' >= i} energy[i'] (row norms of R).
+        std::vector<double> tails2(K + 1, 0.0);
+        tails2[K] = r.tail2;
+        for (int64_t i = K; i-- > 0;) tails2[i] = tails2[i + 1] + r.energy[i];
+        norm = std::sqrt(tails2[0]);
+        if (!std::isfinite(norm)) throw std::overflow_error("non-finite randomized ID QR residual");
+        for (int64_t i = 0; i <= std::min(cap, K); ++i)
+            if (std::sqrt(tails2[i]) <= a.tolerance * norm) { rank = i; break; }
+        rank = std::min(rank, K);
+        rel_tail = norm > 0 ? std::sqrt(tails2[rank]) / norm : 0.0;
+    }
+    std::vector<int64_t> out;
+    for (int64_t i = 0; i < rank; ++i) {
+        if (i >= K || r.pivots[i] < 0 || r.pivots[i] >= n)
+            throw std::runtime_error("randomized ID QRCP returned invalid pivot");
+        out.push_back(kept.original[r.pivots[i]]);
+    }
+    if (a.debug)
+        std::cerr << "[rand_sketch:quadrature] rank=" << rank << " sketch_size=" << ell
+                  << " oversampling=" << a.oversampling << " columns=" << n
+                  << " sketch_relative_frobenius=" << rel_tail << " numerical_rank=" << numerical_rank
+                  << " rank_cap=" << cap << " seed=" << a.seed << " space=device blocks=" << r.blocks
+                  << " folds=" << r.folds << " elapsed=" << seconds_since(started) << " s\n";
+    if (diag) {
+        diag[kDiagRows] = (double)ell;
+        diag[kDiagColumns] = (double)n;
+        diag[kDiagNumericalRank] = (double)numerical_rank;
+        diag[kDiagCap] = (double)cap;
+        diag[kDiagRank] = (double)rank;
+        diag[kDiagNorm] = norm;
+        diag[kDiagRelTail] = rel_tail;
+        diag[kDiagBlocks] = (double)r.blocks;
+        diag[kDiagFolds] = (double)r.folds;
+        diag[kDiagSecondsSketch] = t_sketch;
+        diag[kDiagSecondsQrcp] = t_qrcp;
+    }
+    return out;
+}
+
+}  // namespace
+
+void make_omega(int64_t d, int64_t ell, uint64_t seed, std::vector<double>& u, std::vector<double>& v)
+{
+    u.resize((size_t)(d * ell));
+    v.resize((size_t)(d * ell));
+    std::mt19937_64 gen_u(seed), gen_v(seed ^ 0x9e3779b97f4a7c15ULL);
+    std::normal_distribution<double> normal_u(0, 1), normal_v(0, 1);
+    for (int64_t i = 0; i < d * ell; ++i) {
+        u[i] = normal_u(gen_u);
+        v[i] = normal_v(gen_v);
+    }
+}
+
+std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& a, double* diag)
+{
+    const auto started = Clock::now();
+    if (diag) std::fill(diag, diag + kDiagCount, 0.0);
+    if (a.n < 0) throw std::invalid_argument("points and weights must have equal lengths");
+    if (!a.axis_weights.empty() && a.axis_weights.size() != a.axes.size())
+        throw std::invalid_argument("axis weights must match axes");
+    for (double w : a.axis_weights)
+        if (!std::isfinite(w) || w <= 0) throw std::invalid_argument("axis weights must be finite and positive");
+    const AxisSpec spec = axis_spec(ctx, a.axes.data(), (int)a.axes.size(), a.mo_coeff, a.nocc);
+    hipStream_t s = nullptr;
+    std::vector<int64_t> out;
+    switch (a.algorithm) {
+    case kDense: {
+        const auto scales = axis_scales(ctx, a, spec, s);
+        out = select_dense(ctx, a, spec, scales, diag, s);
+        break;
+    }
+    case kOnline: {
+        const auto scales = axis_scales(ctx, a, spec, s);
+        out = select_online(ctx, a, spec, scales, diag, s);
+        break;
+    }
+    case kRandomized: {
+        const auto scales = axis_scales(ctx, a, spec, s);
+        out = select_randomized(ctx, a, spec, scales, diag, s);
+        break;
+    }
+    default:
+        throw std::invalid_argument("unknown anchor selection algorithm");
+    }
+    if (diag) diag[kDiagSecondsTotal] = seconds_since(started);
+    return out;
+}
+
+}  // namespace mpisdf

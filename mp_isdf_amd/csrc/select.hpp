@@ -1,0 +1,54 @@
+// Anchor selection drivers (device counterparts of the reference's
+// AnchorSelection::select and its three local algorithms).
+#pragma once
+
+#include "axis.hpp"
+#include "context.hpp"
+
+namespace mpisdf {
+
+enum Algorithm : int { kDense = 0, kOnline = 1, kRandomized = 2 };
+enum Precision : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecFP8 = 3 };
+
+struct SelectArgs {
+    const double* points = nullptr;   // host [n][3]
+    const double* weights = nullptr;  // host [n]
+    int64_t n = 0;
+    std::vector<int32_t> axes;
+    const double* mo_coeff = nullptr;
+    int64_t nocc = 0;
+    std::vector<double> axis_weights;
+    double tolerance = 1e-10;
+    int64_t max_rank = 0;
+    int algorithm = kDense;
+    int64_t batch_size = 128;
+    bool debug = false;
+    int64_t sketch_size = 128;
+    int64_t oversampling = 16;
+    uint64_t seed = 7;
+    int precision = kPrecF64;
+};
+
+// Diagnostics slots (double[16]) written when requested.
+enum Diag : int {
+    kDiagRows = 0,          // sketch rows ell (randomized) or pair rows d^2
+    kDiagColumns = 1,       // kept grid columns
+    kDiagNumericalRank = 2,
+    kDiagCap = 3,
+    kDiagRank = 4,
+    kDiagNorm = 5,          // sketch Frobenius norm (randomized)
+    kDiagRelTail = 6,       // tails[rank] / norm (tolerance mode)
+    kDiagBlocks = 7,
+    kDiagFolds = 8,
+    kDiagSecondsSketch = 9,
+    kDiagSecondsQrcp = 10,
+    kDiagSecondsTotal = 11,
+    kDiagCount = 16
+};
+
+std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& args, double* diagnostics);
+
+// Gaussian factors (cpp/src/anchor_select_randomized_id.cpp:269-279), host.
+void make_omega(int64_t d, int64_t ell, uint64_t seed, std::vector<double>& u, std::vector<double>& v);
+
+}  // namespace mpisdf

@@ -1,0 +1,14 @@
+#pragma once
+
+#include "common.hpp"
+
+namespace mpisdf {
+
+// Dense weighted pair matrix, m = w^2 rows per grid column (column-major).
+void launch_pair_product(const double* f, int64_t n, int64_t w, const double* sw, double* a,
+                         unsigned long long* bad, hipStream_t s);
+
+// out[j] += sum_g f[g * w + j]^2 (out must be initialised).
+void launch_colsumsq(const double* f, int64_t n, int64_t w, double* out, hipStream_t s);
+
+}  // namespace mpisdf

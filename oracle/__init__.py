@@ -1,0 +1,260 @@
+"""CPU ORACLE -- test infrastructure, not product code.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg
+may import this package, and only as the checker (or the timed CPU baseline).
+The product package ``mp_isdf_amd`` never imports it.
+
+``liboracle.so`` (built from ``ao_oracle.cpp`` and ``select_oracle.cpp`` by
+``oracle/Makefile``) restates the reference's C++ CPU path in FP64; this module
+wraps it with ctypes and adds NumPy restatements of the reference's Python
+driver maths:
+
+* ``physical_pair_refit``  python/isdf.py:238-263 (SVD solve, 1e-12 cutoff)
+* ``pair_space_error_norms``  python/isdf.py:154-173
+* ``stream_pair_space_error_norms``  python/isdf.py:176-191
+
+Reference build: not possible here (CMake FetchContent of Kokkos, LAPACK++,
+libcint, pybind11 plus an ILP64 LAPACK), so there is no ``oracle/_ref``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+AXIS_CODES = {"ao": 0, "dx": 1, "dy": 2, "dz": 3, "occ": 4}
+ALGORITHMS = {"qrcp_dense": 0, "qrcp_online": 1, "rand_sketch": 2}
+
+
+def build(force: bool = False) -> Path:
+    """Compile liboracle.so with the committed Makefile (g++ only)."""
+    if force or not LIB_PATH.exists():
+        subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        _lib.oracle_last_error.restype = ctypes.c_char_p
+        _lib.oracle_select_last_error.restype = ctypes.c_char_p
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc, selection=False):
+    if rc == 0:
+        return
+    msg = (lib().oracle_select_last_error() if selection else lib().oracle_last_error()).decode()
+    if rc == 1:
+        raise ValueError(msg)
+    if rc == 2:
+        raise OverflowError(msg)
+    raise RuntimeError(msg)
+
+
+def _tables(atm, bas, env):
+    atm = np.ascontiguousarray(atm, dtype=np.int32)
+    bas = np.ascontiguousarray(bas, dtype=np.int32)
+    env = np.ascontiguousarray(env, dtype=np.float64)
+    return atm, bas, env
+
+
+def c2s(l: int, cart: bool = False) -> np.ndarray:
+    ncart = (l + 1) * (l + 2) // 2
+    ncomp = ncart if cart else 2 * l + 1
+    out = np.empty((ncomp, ncart))
+    _check(lib().oracle_c2s(ctypes.c_int(l), ctypes.c_int(int(cart)), _p(out)))
+    return out
+
+
+def nao(atm, bas, env, cart=False) -> int:
+    atm, bas, env = _tables(atm, bas, env)
+    n = ctypes.c_int64()
+    _check(lib().oracle_nao(_p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)),
+                            _p(env), ctypes.c_int64(len(env)), ctypes.c_int(int(cart)), ctypes.byref(n)))
+    return n.value
+
+
+def ao_eval(atm, bas, env, coords, deriv=0, cart=False) -> np.ndarray:
+    """deriv 0: (n, nao) values; 1: (3, n, nao) gradient; 2: (4, n, nao) both."""
+    atm, bas, env = _tables(atm, bas, env)
+    xyz = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+    m = nao(atm, bas, env, cart)
+    comps = (1, 3, 4)[deriv]
+    out = np.empty((comps, len(xyz), m))
+    _check(lib().oracle_ao_eval(_p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)),
+                                _p(env), ctypes.c_int64(len(env)), ctypes.c_int(int(cart)),
+                                _p(xyz), ctypes.c_int64(len(xyz)), ctypes.c_int(deriv), _p(out)))
+    return out[0] if deriv == 0 else out
+
+
+def _axes(axes):
+    if isinstance(axes, (str, int)):
+        axes = [axes]
+    return np.asarray([AXIS_CODES[a] if isinstance(a, str) else int(a) for a in axes], dtype=np.int32)
+
+
+def axis_eval(atm, bas, env, coords, axes, mo_coeff=None, nocc=0, scales=None, cart=False):
+    atm, bas, env = _tables(atm, bas, env)
+    xyz = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1, 3)
+    codes = _axes(axes)
+    mo = None if mo_coeff is None else np.ascontiguousarray(mo_coeff, dtype=np.float64)
+    sc = None if scales is None else np.ascontiguousarray(scales, dtype=np.float64)
+    width = ctypes.c_int64()
+    args = (_p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)), _p(env),
+            ctypes.c_int64(len(env)), ctypes.c_int(int(cart)))
+    _check(lib().oracle_axis_eval(*args, _p(xyz), ctypes.c_int64(0), _p(codes), ctypes.c_int(len(codes)),
+                                  _p(mo), ctypes.c_int64(nocc), None, None, ctypes.byref(width)))
+    out = np.empty((len(xyz), width.value))
+    _check(lib().oracle_axis_eval(*args, _p(xyz), ctypes.c_int64(len(xyz)), _p(codes),
+                                  ctypes.c_int(len(codes)), _p(mo), ctypes.c_int64(nocc), _p(sc),
+                                  _p(out), ctypes.byref(width)))
+    return out
+
+
+def geqp3(a: np.ndarray):
+    """dgeqp3 semantics on a copy of ``a``: returns (packed QR, 0-based pivots, tau)."""
+    m, n = a.shape
+    f = np.asfortranarray(a, dtype=np.float64).copy(order="F")
+    jpvt = np.empty(n, dtype=np.int64)
+    tau = np.empty(min(m, n))
+    _check(lib().oracle_geqp3(ctypes.c_int64(m), ctypes.c_int64(n), _p(f), ctypes.c_int64(m),
+                              _p(jpvt), _p(tau)), selection=True)
+    return f, jpvt - 1, tau
+
+
+def omega(d: int, ell: int, seed: int = 7):
+    """u, v as (d, ell) row-major arrays (u[a*ell + s])."""
+    u = np.empty((d, ell))
+    v = np.empty((d, ell))
+    _check(lib().oracle_omega(ctypes.c_int64(d), ctypes.c_int64(ell), ctypes.c_uint64(seed),
+                              _p(u), _p(v)), selection=True)
+    return u, v
+
+
+def sketch(phi, u, v, sqrt_weights):
+    """Y as (count, ell): Y[g, s] = ell^-1/2 sw_g (phi_g.u_s)(phi_g.v_s)."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    count, d = phi.shape
+    ell = u.shape[1]
+    sw = np.ascontiguousarray(sqrt_weights, dtype=np.float64)
+    y = np.empty((count, ell))
+    _check(lib().oracle_sketch(_p(phi), ctypes.c_int64(count), ctypes.c_int64(d),
+                               _p(np.ascontiguousarray(u)), _p(np.ascontiguousarray(v)),
+                               ctypes.c_int64(ell), _p(sw), _p(y)), selection=True)
+    return y
+
+
+def select(atm, bas, env, points, weights, axes="ao", tolerance=1e-10, max_rank=0,
+           algorithm="qrcp_dense", mo_coeff=None, nocc=0, axis_weights=None,
+           batch_size=128, sketch_size=128, oversampling=16, seed=7, cart=False,
+           return_stats=False):
+    atm, bas, env = _tables(atm, bas, env)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    codes = _axes(axes)
+    mo = None if mo_coeff is None else np.ascontiguousarray(mo_coeff, dtype=np.float64)
+    aw = None if axis_weights is None or len(axis_weights) == 0 else \
+        np.ascontiguousarray(axis_weights, dtype=np.float64)
+    if aw is not None and len(aw) != len(codes):
+        raise ValueError("axis weights must match axes")
+    cap = max(1, len(pts))
+    out = np.empty(cap, dtype=np.int64)
+    count = ctypes.c_int64()
+    stats = np.zeros(4, dtype=np.int64)
+    alg = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    _check(lib().oracle_select(
+        _p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)), _p(env),
+        ctypes.c_int64(len(env)), ctypes.c_int(int(cart)), _p(pts), _p(w), ctypes.c_int64(len(pts)),
+        _p(codes), ctypes.c_int(len(codes)), _p(mo), ctypes.c_int64(nocc), _p(aw),
+        ctypes.c_double(tolerance), ctypes.c_int64(max_rank), ctypes.c_int(alg),
+        ctypes.c_int64(batch_size), ctypes.c_int64(sketch_size), ctypes.c_int64(oversampling),
+        ctypes.c_uint64(seed), _p(out), ctypes.c_int64(cap), ctypes.byref(count), _p(stats)),
+        selection=True)
+    result = out[:count.value].tolist()
+    return (result, stats) if return_stats else result
+
+
+# --------------------------------------------------------------------------
+# NumPy restatements of the driver maths (python/isdf.py)
+# --------------------------------------------------------------------------
+
+def physical_pair_refit(anchor_pairs, anchor_weights, ao, batch_size):
+    """python/isdf.py:238-263: SVD solve C X ~= A, theta = diag(w_J)^-1 X."""
+    u, s, vh = np.linalg.svd(anchor_pairs.T, full_matrices=False)
+    retained = s > (1e-12 * s[0] if len(s) else 0)
+    physical_transform = (vh[retained].T / s[retained]) @ u[:, retained].T
+    transform = physical_transform / anchor_weights[:, None]
+    theta = np.empty((len(anchor_weights), len(ao)), dtype=np.float64)
+    batch_size = min(batch_size, max(1, 8_000_000 // anchor_pairs.shape[1]))
+    coefficient_max = 0.0
+    for first in range(0, len(ao), batch_size):
+        block = ao[first:first + batch_size]
+        pairs = (block[:, :, None] * block[:, None, :]).reshape(len(block), -1)
+        physical_coeff = physical_transform @ pairs.T
+        coefficient_max = max(coefficient_max, float(np.max(np.abs(physical_coeff))))
+        theta[:, first:first + len(block)] = physical_coeff / anchor_weights[:, None]
+    diagnostics = {
+        "retained_rank": int(np.count_nonzero(retained)),
+        "condition": float(s[0] / s[-1]) if len(s) and s[-1] > 0 else float("inf"),
+        "singular_values": s,
+        "max_abs_physical_coefficient": coefficient_max,
+    }
+    return theta, transform, diagnostics
+
+
+def pair_space_error_norms(pair_grid, pair_anchor, anchor_weights, theta, grid_weights,
+                           block_size=256):
+    """python/isdf.py:154-173."""
+    weighted_anchor = pair_anchor * anchor_weights[:, None]
+    residual_sq = exact_sq = 0.0
+    for first in range(0, len(grid_weights), block_size):
+        last = min(first + block_size, len(grid_weights))
+        exact = pair_grid[first:last]
+        reconstructed = theta[:, first:last].T @ weighted_anchor
+        weight = np.abs(grid_weights[first:last])
+        residual_sq += np.einsum("g,gi,gi->", weight, exact - reconstructed,
+                                 exact - reconstructed, optimize=True)
+        exact_sq += np.einsum("g,gi,gi->", weight, exact, exact, optimize=True)
+    absolute = np.sqrt(residual_sq)
+    relative = absolute / np.sqrt(exact_sq) if exact_sq else (0.0 if absolute == 0 else np.inf)
+    return {"l2": absolute, "relative": relative}
+
+
+def stream_pair_space_error_norms(left, right, pair_anchor, anchor_weights, theta, grid_weights):
+    """python/isdf.py:176-191 (one AO-pair matrix of scratch per point)."""
+    anchor = (pair_anchor * anchor_weights[:, None]).reshape(
+        len(anchor_weights), left.shape[1], right.shape[1])
+    residual_sq = exact_sq = 0.0
+    for g in range(len(grid_weights)):
+        exact = np.outer(left[g], right[g])
+        reconstructed = np.einsum("k,kij->ij", theta[:, g], anchor, optimize=True)
+        weight = abs(grid_weights[g])
+        residual_sq += weight * np.sum((exact - reconstructed) ** 2)
+        exact_sq += weight * np.sum(exact ** 2)
+    absolute = np.sqrt(residual_sq)
+    relative = absolute / np.sqrt(exact_sq) if exact_sq else (0.0 if absolute == 0 else np.inf)
+    return {"l2": absolute, "relative": relative}
+
+
+def threads() -> int:
+    """Cores the OpenMP oracle uses (OMP_NUM_THREADS or the affinity mask)."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        return int(env)
+    return len(os.sched_getaffinity(0))
