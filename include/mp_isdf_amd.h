@@ -80,6 +80,50 @@ int mpisdf_select(const mpisdf_context* ctx, const double* points, const double*
                   int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
                   int64_t* out, int64_t capacity, int64_t* count, double* diagnostics);
 
+/* ---- Khatri-Rao sketch stages (device pointers) ---------------------------- */
+/* Gaussian factors u, v [d][ell] row-major from std::mt19937_64(seed) and
+ * (seed ^ 0x9e3779b97f4a7c15) with std::normal_distribution<double>(0, 1),
+ * filled in index order (cpp/src/anchor_select_randomized_id.cpp:269-279).
+ * Host buffers. */
+int mpisdf_omega(int64_t d, int64_t ell, uint64_t seed, double* u, double* v);
+
+/* FP64 sketch (the reference's id_factored_sketch, randomized_id.cpp:299-309):
+ * y[g*ldy + s] = ell^-1/2 sw[g] (sum_a phi[g*d+a] u[a*ell+s]) (sum_a phi[g*d+a] v[a*ell+s]);
+ * *bad (device, may be NULL) counts non-finite outputs. */
+int mpisdf_kr_sketch_f64_device(const double* phi, int64_t count, int64_t d, const double* u, const double* v,
+                                int64_t ell, const double* sw, double* y, int64_t ldy,
+                                unsigned long long* bad, void* stream);
+
+/* Reduced-precision operands: ut/vt[s*ldo + a] from u/v (dtype 2 bf16, 3 fp8
+ * scaled by 32), zero-padded to ldo (a multiple of 16). */
+int mpisdf_sketch_prepare_omega(const double* u, const double* v, int64_t d, int64_t ell, int64_t ldo, int dtype,
+                                void* ut, void* vt, void* stream);
+/* BF16 point scales scale[g] = sw[g] / sqrt(ell). */
+int mpisdf_sketch_scales_bf16(const double* sw, int64_t count, int64_t ell, float* scale, void* stream);
+/* FP8 rows with per-point power-of-two scales; scale[g] absorbs them. */
+int mpisdf_sketch_quantize_fp8(const float* phi, int64_t ld, int64_t count, int64_t d, const double* sw,
+                               int64_t ell, uint8_t* out, int64_t ldo, float* scale, void* stream);
+/* MFMA sketch (dtype 2 bf16, 3 fp8): FP32 y[g*ldy + s], ynorm2[g] += ||y_g||^2. */
+int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi,
+                            int64_t ldphi, int64_t count, const float* scale, float* y, int64_t ldy,
+                            double* ynorm2, unsigned long long* bad, void* stream);
+
+/* ---- QRCP fold (device pointers) --------------------------------------------- */
+/* Block update of the column-pivoted QR's residual norms (replaces the
+ * norm downdating inside LAPACK/MAGMA dgeqp3, cpp/src/anchor_select_qrcp.cpp:36-90):
+ * W = Qb^T Y with Qb[m x b] column-major FP64 and Y[m x n] (ldy) FP64 or FP32
+ * (y_f32 != 0); proj2[j] += sum_t W[t,j]^2 and, if energy != NULL,
+ * energy[t] += sum_j W[t,j]^2.  W is never stored. */
+int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, int y_f32,
+                            int64_t ldy, double* proj2, double* energy, void* stream);
+
+/* ---- Grid front end (no reference counterpart: the reference takes its grid
+ * from PySCF's dft.gen_grid.Grids, python/isdf.py:488-496) ------------------ */
+/* Becke-partitioned atom-centred grid, Treutler radial nodes, product angular
+ * rules.  Call with xyz == NULL to get *count; then xyz[3*count], w[count]. */
+int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level,
+                      double* xyz, double* weights, int64_t capacity, int64_t* count);
+
 #ifdef __cplusplus
 }
 #endif

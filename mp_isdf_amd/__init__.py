@@ -18,6 +18,15 @@ LIBRARY = PACKAGE_DIR / "libmp_isdf_amd.so"
 if str(PACKAGE_DIR) not in sys.path:
     sys.path.insert(0, str(PACKAGE_DIR))
 
+# torch bundles its own HIP runtime under the same SONAME (libamdhip64.so.7)
+# as the system ROCm the extension links.  Whichever loads first serves the
+# whole process, and torch cannot see the GPU when it arrives second, so
+# torch (plumbing for device buffers and autograd) is imported first.
+try:
+    import torch  # noqa: F401
+except ImportError:  # the extension itself does not need torch
+    pass
+
 
 def __getattr__(name):
     if name == "mp_isdf":

@@ -33,8 +33,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", "-I", str(ROOT / "include"),
 ]
 SOURCES = [
-    "common.cpp", "context.cpp", "axis.cpp", "select.cpp", "online.cpp", "capi.cpp",
-    "ao_eval.hip", "gemm.hip", "qrcp.hip", "sketch.hip", "util.hip",
+    "common.cpp", "context.cpp", "axis.cpp", "select.cpp", "online.cpp", "capi.cpp", "grid.cpp",
+    "ao_eval.hip", "gemm.hip", "qrcp.hip", "sketch.hip", "util.hip", "online_kernels.hip", "sketch_mfma.hip",
 ]
 
 
@@ -60,7 +60,7 @@ def build(verbose: bool = False) -> None:
     with cf.ThreadPoolExecutor(jobs) as pool:
         objects = list(pool.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects), "-lpthread"])
     os.replace(tmp, LIB)
     import pybind11
     ext_tmp = EXT.with_name(EXT.name + ".tmp")

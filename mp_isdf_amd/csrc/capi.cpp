@@ -158,6 +158,9 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
 
 }  // extern "C"
 
+#include "grid.hpp"
+#include "sketch.hpp"
+#include "sketch_mfma.hpp"
 #include "select.hpp"
 
 extern "C" int mpisdf_select(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
@@ -198,5 +201,94 @@ extern "C" int mpisdf_select(const mpisdf_context* ctx, const double* points, co
         if ((int64_t)sel.size() > capacity) throw std::invalid_argument("output capacity is too small");
         std::copy(sel.begin(), sel.end(), out);
         *count = (int64_t)sel.size();
+    });
+}
+
+extern "C" int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level,
+                                 double* xyz, double* weights, int64_t capacity, int64_t* count)
+{
+    return guard([&] {
+        if (!charges || !coords || !count) throw std::invalid_argument("null grid argument");
+        std::vector<int> z(charges, charges + natm);
+        auto grid = mpisdf::becke_grid(z.data(), coords, natm, level);
+        const int64_t n = (int64_t)grid.second.size();
+        *count = n;
+        if (!xyz) return;
+        if (capacity < n) throw std::invalid_argument("grid output capacity is too small");
+        std::copy(grid.first.begin(), grid.first.end(), xyz);
+        std::copy(grid.second.begin(), grid.second.end(), weights);
+    });
+}
+
+extern "C" int mpisdf_omega(int64_t d, int64_t ell, uint64_t seed, double* u, double* v)
+{
+    return guard([&] {
+        if (d < 0 || ell < 0 || !u || !v) throw std::invalid_argument("invalid omega arguments");
+        std::vector<double> hu, hv;
+        mpisdf::make_omega(d, ell, seed, hu, hv);
+        std::copy(hu.begin(), hu.end(), u);
+        std::copy(hv.begin(), hv.end(), v);
+    });
+}
+
+extern "C" int mpisdf_kr_sketch_f64_device(const double* phi, int64_t count, int64_t d, const double* u,
+                                           const double* v, int64_t ell, const double* sw, double* y, int64_t ldy,
+                                           unsigned long long* bad, void* stream)
+{
+    return guard([&] {
+        if (count < 0 || d <= 0 || ell <= 0 || ldy < ell) throw std::invalid_argument("invalid sketch shape");
+        mpisdf::SketchScratch scratch;
+        mpisdf::DeviceBuffer<unsigned long long> own;
+        if (!bad) {
+            own.resize(1);
+            MPISDF_HIP(hipMemsetAsync(own.ptr, 0, sizeof(unsigned long long), static_cast<hipStream_t>(stream)));
+            bad = own.ptr;
+        }
+        mpisdf::sketch_f64(phi, count, d, u, v, ell, sw, y, ldy, scratch, bad, static_cast<hipStream_t>(stream));
+        MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
+extern "C" int mpisdf_sketch_prepare_omega(const double* u, const double* v, int64_t d, int64_t ell, int64_t ldo,
+                                           int dtype, void* ut, void* vt, void* stream)
+{
+    return guard([&] {
+        if (ldo < d || ldo % 16) throw std::invalid_argument("ldo must be >= d and a multiple of 16");
+        mpisdf::prepare_omega_lowp(u, v, d, ell, ldo, static_cast<mpisdf::DType>(dtype), ut, vt,
+                                   static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int mpisdf_sketch_scales_bf16(const double* sw, int64_t count, int64_t ell, float* scale, void* stream)
+{
+    return guard([&] { mpisdf::bf16_point_scales(sw, count, ell, scale, static_cast<hipStream_t>(stream)); });
+}
+
+extern "C" int mpisdf_sketch_quantize_fp8(const float* phi, int64_t ld, int64_t count, int64_t d, const double* sw,
+                                          int64_t ell, uint8_t* out, int64_t ldo, float* scale, void* stream)
+{
+    return guard([&] {
+        if (ldo < d || ld < d) throw std::invalid_argument("invalid fp8 row strides");
+        mpisdf::quantize_fp8_rows(phi, ld, count, d, sw, ell, out, ldo, scale, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t ell, int64_t d,
+                                       const void* phi, int64_t ldphi, int64_t count, const float* scale, float* y,
+                                       int64_t ldy, double* ynorm2, unsigned long long* bad, void* stream)
+{
+    return guard([&] {
+        if (ldy < ell || ldphi < d || !ynorm2 || !bad) throw std::invalid_argument("invalid sketch arguments");
+        mpisdf::kr_sketch_lowp(static_cast<mpisdf::DType>(dtype), ut, vt, ell, d, phi, ldphi, count, scale, y, ldy,
+                               ynorm2, bad, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, int y_f32,
+                                       int64_t ldy, double* proj2, double* energy, void* stream)
+{
+    return guard([&] {
+        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2) throw std::invalid_argument("invalid fold arguments");
+        mpisdf::fold_projections(b, n, m, Qb, Y, y_f32 != 0, ldy, proj2, energy, static_cast<hipStream_t>(stream));
     });
 }

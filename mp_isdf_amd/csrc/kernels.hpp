@@ -40,4 +40,10 @@ void gemm_f64(int64_t M, int64_t N, int64_t K, double alpha, MatView A, MatView 
 void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, const float* B,
                    int64_t brs, int64_t bcs, double beta, MutView C, hipStream_t stream);
 
+// QRCP fold: W = Qb^T Y (Qb: m x b column-major, Y: m x n with leading
+// dimension ldy, FP32 or FP64); proj2[j] += sum_t W[t,j]^2 and, when energy
+// is given, energy[t] += sum_j W[t,j]^2.  FP64 MFMA, W never stored.
+void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, bool y_f32, int64_t ldy,
+                      double* proj2, double* energy, hipStream_t stream);
+
 }  // namespace mpisdf

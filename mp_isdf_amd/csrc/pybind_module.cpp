@@ -167,7 +167,7 @@ public:
                                 uint64_t seed, bool audit, py::object cvt_weights, int64_t cvt_max_iter,
                                 double cvt_tol, const std::vector<double>& axis_weights,
                                 py::object point_importance, double ao_screening_threshold,
-                                const std::string& precision) const
+                                const std::string& precision, double* diagnostics = nullptr) const
     {
         (void)cvt_max_iter; (void)cvt_tol; (void)ao_screening_threshold;
         parse_space(space);
@@ -201,11 +201,29 @@ public:
                                    axis_weights.empty() ? nullptr : axis_weights.data(),
                                    (int64_t)axis_weights.size(), tolerance, max_rank,
                                    static_cast<int32_t>(algorithm), batch_size, debug ? 1 : 0, sketch_size,
-                                   oversampling, seed, prec, out.data(), (int64_t)out.size(), &count, nullptr);
+                                   oversampling, seed, prec, out.data(), (int64_t)out.size(), &count, diagnostics);
         }
         check(status);
         out.resize(count);
         return out;
+    }
+
+    py::tuple select_with_diagnostics(const FloatArray& points, const FloatArray& weights, py::object axes,
+                                      double tolerance, int64_t max_rank, LocalAlgorithm algorithm,
+                                      py::object mo_coeff, int64_t nocc, const std::string& space, int64_t batch_size,
+                                      bool debug, int64_t sketch_size, int64_t oversampling, uint64_t seed,
+                                      const std::vector<double>& axis_weights, const std::string& precision) const
+    {
+        std::vector<double> diag(16, 0.0);
+        auto idx = select(points, weights, axes, tolerance, max_rank, algorithm, mo_coeff, nocc, space, batch_size,
+                          debug, sketch_size, oversampling, seed, false, py::none(), 100, 1e-5, axis_weights,
+                          py::none(), 1e-24, precision, diag.data());
+        static const char* names[] = {"rows", "columns", "numerical_rank", "rank_cap", "rank", "sketch_norm",
+                                      "relative_tail", "blocks", "folds", "seconds_sketch", "seconds_qrcp",
+                                      "seconds_total"};
+        py::dict d;
+        for (int i = 0; i < 12; ++i) d[names[i]] = diag[i];
+        return py::make_tuple(idx, d);
     }
 
 private:
@@ -264,9 +282,37 @@ PYBIND11_MODULE(mp_isdf, m)
         .value("SVD", DistributedAlgorithm::SVD)
         .value("RandomizedQRColumnPivoting", DistributedAlgorithm::RandomizedQRColumnPivoting);
 
+    m.def("becke_grid", [](const IntArray& charges, const FloatArray& coords, int level) {
+        if (charges.ndim() != 1 || coords.ndim() != 2 || coords.shape(1) != 3 || coords.shape(0) != charges.shape(0))
+            throw py::value_error("expected charges[natm] and coords[natm,3] (Bohr)");
+        int64_t count = 0;
+        check(mpisdf_becke_grid(charges.data(), coords.data(), charges.shape(0), level, nullptr, nullptr, 0, &count));
+        py::array_t<double> xyz({(py::ssize_t)count, py::ssize_t(3)}), w((py::ssize_t)count);
+        int status;
+        double* px = xyz.mutable_data();
+        double* pw = w.mutable_data();
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_becke_grid(charges.data(), coords.data(), charges.shape(0), level, px, pw, count, &count);
+        }
+        check(status);
+        return py::make_tuple(xyz, w);
+    }, py::arg("charges"), py::arg("coords"), py::arg("level") = 3,
+       "Becke-partitioned molecular grid (Bohr): returns (coords[n,3], weights[n]).  Not part of the reference "
+       "extension, which receives PySCF's grid.");
+
     py::class_<Selector>(m, "AnchorSelection")
         .def(py::init<const Context&>(), py::arg("context"), py::keep_alive<1, 2>())
-        .def("select", &Selector::select, py::arg("points"), py::arg("weights"), py::arg("axes"),
+        .def("select", [](const Selector& sel, const FloatArray& points, const FloatArray& weights, py::object axes,
+                          double tolerance, int64_t max_rank, LocalAlgorithm algorithm, py::object mo_coeff,
+                          int64_t nocc, const std::string& space, int64_t batch_size, bool debug, int64_t sketch_size,
+                          int64_t oversampling, uint64_t seed, bool audit, py::object cvt_weights, int64_t cvt_max_iter,
+                          double cvt_tol, const std::vector<double>& axis_weights, py::object point_importance,
+                          double ao_screening_threshold, const std::string& precision) {
+            return sel.select(points, weights, axes, tolerance, max_rank, algorithm, mo_coeff, nocc, space, batch_size,
+                              debug, sketch_size, oversampling, seed, audit, cvt_weights, cvt_max_iter, cvt_tol,
+                              axis_weights, point_importance, ao_screening_threshold, precision);
+        }, py::arg("points"), py::arg("weights"), py::arg("axes"),
              py::arg("tolerance") = 1e-10, py::arg("max_rank") = 0,
              py::arg("algorithm") = LocalAlgorithm::QRColumnPivoting, py::arg("mo_coeff") = py::none(),
              py::arg("nocc") = 0, py::arg("space") = "auto", py::arg("batch_size") = 128,
@@ -277,6 +323,13 @@ PYBIND11_MODULE(mp_isdf, m)
              py::arg("ao_screening_threshold") = 1e-24, py::arg("precision") = "fp64",
              "Select original grid indices in pivot order (GPU).  precision selects the sketch "
              "arithmetic of RandomizedQuadratureKhatriRao: fp64 (reference semantics), fp32, bf16 or fp8.")
+        .def("select_with_diagnostics", &Selector::select_with_diagnostics, py::arg("points"), py::arg("weights"),
+             py::arg("axes"), py::arg("tolerance") = 1e-10, py::arg("max_rank") = 0,
+             py::arg("algorithm") = LocalAlgorithm::QRColumnPivoting, py::arg("mo_coeff") = py::none(),
+             py::arg("nocc") = 0, py::arg("space") = "auto", py::arg("batch_size") = 128, py::arg("debug") = false,
+             py::arg("sketch_size") = 128, py::arg("oversampling") = 16, py::arg("seed") = 7,
+             py::arg("axis_weights") = std::vector<double>{}, py::arg("precision") = "fp64",
+             "select() plus a dict of stage timings and QRCP block statistics (not in the reference).")
         .def("evaluate_axis", &Selector::evaluate_axis, py::arg("points"), py::arg("axes"),
              py::arg("mo_coeff") = py::none(), py::arg("nocc") = 0, py::arg("space") = "auto",
              "Evaluate concatenated axis functions on the GPU.");

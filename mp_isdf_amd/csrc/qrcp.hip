@@ -212,30 +212,6 @@ __global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ r
     if (threadIdx.x == 0) rn2[c] = part[0] + part[1] + part[2] + part[3];
 }
 
-// proj2[j] += sum_t W[t, j]^2 ; energy[t] += sum_j W[t, j]^2   (W is b x nc, ld b)
-__global__ __launch_bounds__(256) void fold_kernel(const double* __restrict__ W, int64_t b, int64_t nc,
-                                                   double* __restrict__ proj2, double* __restrict__ energy)
-{
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nc) {
-        double s = 0;
-        for (int64_t t = 0; t < b; ++t) { const double w = W[t + j * b]; s += w * w; }
-        proj2[j] += s;
-    }
-    if (energy) {
-        // Row energies: one pass per row over this block's columns.
-        __shared__ double part[4];
-        for (int64_t t = 0; t < b; ++t) {
-            const double w = j < nc ? W[t + j * b] : 0.0;
-            double s = wave_sum(w * w);
-            if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
-            __syncthreads();
-            if (threadIdx.x == 0) atomicAdd(&energy[t], part[0] + part[1] + part[2] + part[3]);
-            __syncthreads();
-        }
-    }
-}
-
 // tail2 += sum (Y - Q W)^2 over a column chunk; R holds Y chunk - Q W (m x nc).
 __global__ void sumsq_kernel(const double* __restrict__ r, int64_t count, double* __restrict__ out)
 {
@@ -264,18 +240,6 @@ int grid_for(int64_t work, int threads = 256, int64_t cap = 65536)
     return (int)std::max<int64_t>(1, std::min<int64_t>((work + threads - 1) / threads, cap));
 }
 
-void project(const void* y, DType ytype, int64_t m, int64_t ldy, int64_t j0, int64_t nc, const double* Qb,
-             int64_t b, double* W, hipStream_t s)
-{
-    // W (b x nc) = Qb^T (b x m) * Y[:, j0:j0+nc]
-    if (ytype == kF64)
-        gemm_f64(b, nc, m, 1.0, MatView{Qb, m, 1}, MatView{static_cast<const double*>(y) + j0 * ldy, 1, ldy}, 0.0,
-                 MutView{W, 1, b}, s);
-    else
-        gemm_f64_f32b(b, nc, m, 1.0, MatView{Qb, m, 1}, static_cast<const float*>(y) + j0 * ldy, 1, ldy, 0.0,
-                      MutView{W, 1, b}, s);
-}
-
 }  // namespace
 
 QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t ldy, const QrcpOptions& opt,
@@ -298,7 +262,9 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     MPISDF_HIP(hipMemsetAsync(selected.ptr, 0, n, s));
     MPISDF_HIP(hipMemsetAsync(energy.ptr, 0, std::max<int64_t>(kmax, 1) * sizeof(double), s));
 
-    if (ytype == kF64)
+    if (opt.ynorm2)
+        MPISDF_HIP(hipMemcpyAsync(ynorm2.ptr, opt.ynorm2, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    else if (ytype == kF64)
         hipLaunchKernelGGL(colnorm2_kernel<double>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s,
                            static_cast<const double*>(y), m, n, ldy, ynorm2.ptr);
     else
@@ -319,9 +285,6 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     if (n > std::numeric_limits<int>::max()) throw std::overflow_error("QRCP column count exceeds int range");
     DeviceBuffer<uint8_t> sort_ws(std::max<size_t>(sort_bytes, 1));
 
-    // Projection scratch: W is b x chunk.
-    const int64_t wcap = 1ll << 27;  // doubles (1 GiB)
-    DeviceBuffer<double> W;
 
     int64_t k = 0, folded = 0;
     double r00 = 0;
@@ -329,15 +292,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     auto fold = [&](double* energy_out) {
         const int64_t b = k - folded;
         if (b <= 0) return;
-        const int64_t nc = std::max<int64_t>(1, std::min<int64_t>(n, wcap / b));
-        if (W.count < (size_t)(b * nc)) W.resize((size_t)(b * nc));
-        for (int64_t j0 = 0; j0 < n; j0 += nc) {
-            const int64_t cnt = std::min(nc, n - j0);
-            project(y, ytype, m, ldy, j0, cnt, Q.ptr + folded * m, b, W.ptr, s);
-            hipLaunchKernelGGL(fold_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, s, W.ptr, b, cnt,
-                               proj2.ptr + j0, energy_out ? energy_out + folded : nullptr);
-            MPISDF_HIP(hipGetLastError());
-        }
+        fold_projections(b, n, m, Q.ptr + folded * m, y, ytype == kF32, ldy, proj2.ptr,
+                         energy_out ? energy_out + folded : nullptr, s);
         folded = k;
         ++res.folds;
     };

@@ -14,6 +14,7 @@ struct QrcpOptions {
     bool want_energy = false;    // row energies sum_j (q_i^T y_j)^2 (randomized tail rule)
     bool want_tail = false;      // exact residual Frobenius^2 after the last pivot
     bool debug = false;
+    const double* ynorm2 = nullptr;  // optional device ||y_j||^2 (skips the norm pass)
 };
 
 struct QrcpResult {

@@ -14,6 +14,7 @@
 #include "online.hpp"
 #include "qrcp.hpp"
 #include "sketch.hpp"
+#include "sketch_mfma.hpp"
 #include "util.hpp"
 
 #include <algorithm>
@@ -184,8 +185,10 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             "reduce the sketch/rank or use a tiled QRCP backend");
     if (a.max_rank > std::min(n, physical_rank))
         throw std::invalid_argument("requested anchor_rank exceeds eligible candidates or physical pair rank");
-    if (a.precision != kPrecF64)
-        throw std::invalid_argument("reduced-precision sketches are provided by the sketch engine (pending)");
+    if (a.precision == kPrecF32)
+        throw std::invalid_argument("precision 'fp32' is not provided; use fp64, bf16 or fp8");
+    const bool lowp = a.precision != kPrecF64;
+    const DType in_type = a.precision == kPrecFP8 ? kFP8 : kBF16;
 
     std::vector<double> hu, hv;
     make_omega(d, ell, a.seed, hu, hv);
@@ -193,7 +196,6 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     u.upload(hu.data(), hu.size(), s);
     v.upload(hv.data(), hv.size(), s);
 
-    DeviceBuffer<double> y((size_t)ell * n);
     DeviceBuffer<double> d_xyz, sw(n);
     d_xyz.upload(kept.xyz.data(), kept.xyz.size(), s);
     {
@@ -203,15 +205,62 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     }
     DeviceBuffer<unsigned long long> bad(1);
     MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
-    const int64_t batch = std::min(n, device_batch(d, ctx.nao, 2 * ell));
-    DeviceBuffer<double> feat((size_t)batch * d);
+    DeviceBuffer<double> y64;
+    DeviceBuffer<float> y32;
+    DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;
-    SketchScratch scratch;
-    for (int64_t start = 0; start < n; start += batch) {
-        const int64_t count = std::min(batch, n - start);
-        evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
-                             a.mo_coeff, a.nocc, feat.ptr, ws, s);
-        sketch_f64(feat.ptr, count, d, u.ptr, v.ptr, ell, sw.ptr + start, y.ptr + start * ell, ell, scratch, bad.ptr, s);
+    if (!lowp) {
+        y64.resize((size_t)ell * n);
+        const int64_t batch = std::min(n, device_batch(d, ctx.nao, 2 * ell));
+        DeviceBuffer<double> feat((size_t)batch * d);
+        SketchScratch scratch;
+        for (int64_t start = 0; start < n; start += batch) {
+            const int64_t count = std::min(batch, n - start);
+            evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
+                                 a.mo_coeff, a.nocc, feat.ptr, ws, s);
+            sketch_f64(feat.ptr, count, d, u.ptr, v.ptr, ell, sw.ptr + start, y64.ptr + start * ell, ell, scratch,
+                       bad.ptr, s);
+        }
+    } else {
+        // Reduced precision: operands padded to a multiple of 16 features.
+        const int64_t dp = (d + 15) / 16 * 16;
+        const size_t esz = in_type == kFP8 ? 1 : 2;
+        DeviceBuffer<uint8_t> ut((size_t)ell * dp * esz), vt((size_t)ell * dp * esz);
+        prepare_omega_lowp(u.ptr, v.ptr, d, ell, dp, in_type, ut.ptr, vt.ptr, s);
+        y32.resize((size_t)ell * n);
+        ynorm2.resize(n);
+        MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
+        const bool ao_only = spec.kinds.size() == 1 && spec.kinds[0] == 0;
+        const int64_t batch = std::min<int64_t>(n, 1 << 16);
+        DeviceBuffer<uint8_t> phi((size_t)batch * dp * esz);
+        MPISDF_HIP(hipMemsetAsync(phi.ptr, 0, phi.count, s));  // padding columns stay zero
+        DeviceBuffer<float> phi32, scale(batch);
+        DeviceBuffer<double> feat;
+        if (!ao_only) feat.resize((size_t)batch * d);
+        if (in_type == kFP8) phi32.resize((size_t)batch * d);
+        for (int64_t start = 0; start < n; start += batch) {
+            const int64_t count = std::min(batch, n - start);
+            const double* xyz = d_xyz.ptr + 3 * start;
+            if (in_type == kBF16) {
+                if (ao_only) launch_ao_eval(ctx, xyz, count, kValues, kBF16, phi.ptr, dp, count * dp, s);
+                else {
+                    evaluate_axis_device(ctx, spec, xyz, count, scales.empty() ? nullptr : scales.data(), a.mo_coeff,
+                                         a.nocc, feat.ptr, ws, s);
+                    cast_rows(feat.ptr, count, d, dp, kBF16, phi.ptr, s);
+                }
+                bf16_point_scales(sw.ptr + start, count, ell, scale.ptr, s);
+            } else {
+                if (ao_only) launch_ao_eval(ctx, xyz, count, kValues, kF32, phi32.ptr, d, count * d, s);
+                else {
+                    evaluate_axis_device(ctx, spec, xyz, count, scales.empty() ? nullptr : scales.data(), a.mo_coeff,
+                                         a.nocc, feat.ptr, ws, s);
+                    cast_rows(feat.ptr, count, d, d, kF32, phi32.ptr, s);
+                }
+                quantize_fp8_rows(phi32.ptr, d, count, d, sw.ptr + start, ell, phi.ptr, dp, scale.ptr, s);
+            }
+            kr_sketch_lowp(in_type, ut.ptr, vt.ptr, ell, dp, phi.ptr, dp, count, scale.ptr, y32.ptr + start * ell, ell,
+                           ynorm2.ptr + start, bad.ptr, s);
+        }
     }
     if (bad.download(s)[0]) throw std::overflow_error("non-finite randomized ID sketch");
     const double t_sketch = seconds_since(started);
@@ -227,7 +276,9 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     opt.want_tail = a.max_rank == 0;
     opt.debug = a.debug;
     const auto tq = Clock::now();
-    const QrcpResult r = qrcp_device(y.ptr, kF64, ell, n, ell, opt, s);
+    opt.ynorm2 = lowp ? ynorm2.ptr : nullptr;
+    const QrcpResult r = lowp ? qrcp_device(y32.ptr, kF32, ell, n, ell, opt, s)
+                              : qrcp_device(y64.ptr, kF64, ell, n, ell, opt, s);
     const double t_qrcp = seconds_since(tq);
     const int64_t K = (int64_t)r.pivots.size();
     // numerical_rank: first i with |R_ii| <= floor; the engine stops there.

@@ -112,3 +112,53 @@ def test_randomized_water_matches_oracle(mp_isdf):
     want = oracle.select(mol.atm, mol.bas, mol.env, pts, w, "ao", algorithm="rand_sketch", max_rank=120,
                          oversampling=12, seed=7)
     assert got == want
+
+
+class ThreeS17:
+    atm, bas, env = basis.three_s_shell_tables()
+    points = np.random.default_rng(23).normal(size=(17, 3))
+    weights = np.linspace(0.5, 2, 17)
+    weights[2] = 0
+    weights[4] *= -1
+
+
+def test_online_matches_oracle(mp_isdf):
+    """python/test/test_online_anchor_select.py:44-101 cases, GPU vs oracle."""
+    c = ThreeS17
+    ctx = mp_isdf.LibcintContext(c.atm, c.bas, c.env)
+    sel = mp_isdf.AnchorSelection(ctx)
+    online = mp_isdf.LocalAnchorSelectionAlgorithm.OnlineQRColumnPivoting
+    ao = mp_isdf.AxisType.AtomicOrbital
+    for batch in (1, 3, 17):
+        got = sel.select(c.points, c.weights, ao, algorithm=online, batch_size=batch)
+        assert got == oracle.select(c.atm, c.bas, c.env, c.points, c.weights, "ao", algorithm="qrcp_online",
+                                    batch_size=batch)
+        assert len(got) == 6
+        assert sel.select(c.points, c.weights, ao, algorithm=online, batch_size=batch, max_rank=3) == got[:3]
+    axes = [ao, mp_isdf.AxisType.AtomicOrbitalGradX, mp_isdf.AxisType.OccupiedOrbital]
+    opts = dict(mo_coeff=np.eye(3), nocc=1, max_rank=3)
+    assert (sel.select(c.points, c.weights, axes, algorithm=online, batch_size=2, **opts)
+            == sel.select(c.points, c.weights, axes, **opts))
+    assert sel.select(c.points, c.weights, ao, algorithm=online, tolerance=1) == []
+    for cutoff in (0.1, 0.5):
+        assert (sel.select(c.points, c.weights, ao, algorithm=online, tolerance=cutoff, batch_size=2)
+                == sel.select(c.points, c.weights, ao, tolerance=cutoff))
+    assert sel.select(c.points, np.zeros_like(c.weights), ao, algorithm=online) == []
+    with pytest.raises(ValueError):
+        sel.select(c.points, c.weights, ao, algorithm=online, batch_size=0)
+
+
+def test_online_water_matches_oracle(mp_isdf):
+    mol = molecule.single_water("cc-pvdz")
+    rng = np.random.default_rng(8)
+    pts = rng.normal(scale=1.5, size=(1500, 3))
+    w = rng.uniform(0.05, 1.0, size=1500)
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    sel = mp_isdf.AnchorSelection(ctx)
+    online = mp_isdf.LocalAnchorSelectionAlgorithm.OnlineQRColumnPivoting
+    got = sel.select(pts, w, mp_isdf.AxisType.AtomicOrbital, algorithm=online, max_rank=40, batch_size=256)
+    want = oracle.select(mol.atm, mol.bas, mol.env, pts, w, "ao", algorithm="qrcp_online", max_rank=40,
+                         batch_size=256)
+    assert got == want
+    dense = sel.select(pts, w, mp_isdf.AxisType.AtomicOrbital, max_rank=40)
+    assert dense == oracle.select(mol.atm, mol.bas, mol.env, pts, w, "ao", max_rank=40)

@@ -1,0 +1,82 @@
+"""Khatri-Rao sketch kernels against the FP64 oracle sketch on the same
+phi, Omega and weights (FP64 form: 1e-12; BF16 / FP8 forms: the rounding of
+their inputs, stated per test)."""
+import numpy as np
+import pytest
+
+import oracle
+from mp_isdf_amd import capi, molecule
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case(mp_isdf):
+    import torch
+    mol = molecule.water_cluster(8)
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-9, 9, size=(700, 3))
+    w = rng.uniform(0.01, 2.0, size=700)
+    phi = oracle.ao_eval(mol.atm, mol.bas, mol.env, pts)
+    d, ell = phi.shape[1], 300
+    u, v = oracle.omega(d, ell, 11)
+    y = oracle.sketch(phi, u, v, np.sqrt(w))  # [g, s]
+    return dict(torch=torch, mol=mol, pts=pts, w=w, phi=phi, d=d, ell=ell, u=u, v=v, y=y)
+
+
+def test_fp64_sketch_matches_oracle(case):
+    torch = case["torch"]
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), device="cuda")
+    y = torch.empty((len(case["w"]), case["ell"]), dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_kr_sketch_f64_device", dev(case["phi"]), len(case["w"]), case["d"], dev(case["u"]),
+              dev(case["v"]), case["ell"], dev(np.sqrt(case["w"])), y, case["ell"], None, None)
+    np.testing.assert_allclose(y.cpu().numpy(), case["y"], rtol=1e-12, atol=1e-14 * np.abs(case["y"]).max())
+
+
+@pytest.mark.parametrize("dtype,tol", [(2, 2e-2), (3, 2e-1)])
+def test_lowp_sketch_matches_oracle(case, dtype, tol):
+    torch = case["torch"]
+    dev = lambda a: torch.tensor(np.ascontiguousarray(a), device="cuda")
+    n, d, ell = len(case["w"]), case["d"], case["ell"]
+    dp = (d + 15) // 16 * 16
+    esz = torch.bfloat16 if dtype == 2 else torch.uint8
+    ut = torch.empty((ell, dp), dtype=esz, device="cuda")
+    vt = torch.empty((ell, dp), dtype=esz, device="cuda")
+    capi.call("mpisdf_sketch_prepare_omega", dev(case["u"]), dev(case["v"]), d, ell, dp, dtype, ut, vt, None)
+    sw = dev(np.sqrt(case["w"]))
+    scale = torch.empty(n, dtype=torch.float32, device="cuda")
+    if dtype == 2:
+        phi = torch.zeros((n, dp), dtype=torch.bfloat16, device="cuda")
+        phi[:, :d] = dev(case["phi"]).to(torch.bfloat16)
+        capi.call("mpisdf_sketch_scales_bf16", sw, n, ell, scale, None)
+    else:
+        phi = torch.zeros((n, dp), dtype=torch.uint8, device="cuda")
+        phi32 = dev(case["phi"]).float().contiguous()
+        capi.call("mpisdf_sketch_quantize_fp8", phi32, d, n, d, sw, ell, phi, dp, scale, None)
+    y = torch.empty((n, ell), dtype=torch.float32, device="cuda")
+    ynorm2 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    bad = torch.zeros(1, dtype=torch.int64, device="cuda")
+    capi.call("mpisdf_kr_sketch_device", dtype, ut, vt, ell, dp, phi, dp, n, scale, y, ell, ynorm2, bad, None)
+    torch.cuda.synchronize()
+    got = y.double().cpu().numpy()
+    ref = case["y"]
+    rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert rel < tol, rel
+    np.testing.assert_allclose(ynorm2.cpu().numpy(), np.sum(got ** 2, axis=1), rtol=1e-6)
+    assert int(bad.item()) == 0
+
+
+def test_lowp_selection_is_valid(mp_isdf):
+    mol = molecule.single_water("cc-pvdz")
+    rng = np.random.default_rng(5)
+    pts = rng.normal(scale=1.5, size=(3000, 3))
+    w = rng.uniform(0.05, 1.0, size=3000)
+    w[::17] = 0
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    sel = mp_isdf.AnchorSelection(ctx)
+    alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
+    for precision in ("bf16", "fp8"):
+        got = sel.select(pts, w, mp_isdf.AxisType.AtomicOrbital, algorithm=alg, max_rank=120, oversampling=12,
+                         seed=7, precision=precision)
+        assert len(got) == 120 and len(set(got)) == 120
+        assert not set(got) & set(np.flatnonzero(w == 0).tolist())
