@@ -111,11 +111,33 @@ int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t e
 /* ---- QRCP fold (device pointers) --------------------------------------------- */
 /* Block update of the column-pivoted QR's residual norms (replaces the
  * norm downdating inside LAPACK/MAGMA dgeqp3, cpp/src/anchor_select_qrcp.cpp:36-90):
- * W = Qb^T Y with Qb[m x b] column-major FP64 and Y[m x n] (ldy) FP64 or FP32
- * (y_f32 != 0); proj2[j] += sum_t W[t,j]^2 and, if energy != NULL,
- * energy[t] += sum_j W[t,j]^2.  W is never stored. */
+ * W = Qb^T Y with Qb[m x b] column-major FP64 and Y[m x n] (ldy);
+ * y_f32 = 0: FP64 Y, FP64 MFMA; 1: FP32 Y, FP64 MFMA (exact products);
+ * 2: FP32 Y, split-BF16 MFMA with FP64 chunk accumulation.
+ * proj2[j] += sum_t W[t,j]^2 and, if energy != NULL, energy[t] += sum_j W[t,j]^2.
+ * W is never stored. */
 int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, int y_f32,
                             int64_t ldy, double* proj2, double* energy, void* stream);
+
+/* ---- ISDF fit on selected anchors ------------------------------------------- */
+/* Replaces the driver refit and pair-space error of the reference
+ * (python/isdf.py:238-263 _physical_pair_refit, :176-191 pair error) with the
+ * normal-equations form on the GPU: G2 = (Phi_a Phi_a^T)^2 = L L^T (FP64),
+ * theta_g = L^-T L^-1 (Phi_a phi_g)^2 / w_anchor, and
+ * error_g^2 = ||phi_g||^4 - ||L^-1 (Phi_a phi_g)^2||^2.
+ * Host arrays: anchor_xyz[k*3], anchor_w[k], xyz[n*3], w[n] (grid points with
+ * nonzero weight).  theta (optional) receives row-major [k][n]; tsum
+ * (optional) sum_g theta[:, g] w_g.  results[8]: 0 l2 error, 1 relative error,
+ * 2 condition estimate of G2, 3 max diagonal of G2, 4 seconds (factor),
+ * 5 seconds (grid pass). */
+int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w, int64_t k,
+                    const double* xyz, const double* w, int64_t n, int64_t batch, double* theta, double* tsum,
+                    double* results);
+/* Same with device-resident grid (xyz_dev[n*3], w_dev[n]) and anchors
+ * (anchor_xyz_dev[k*3]); anchor_w stays on the host; no theta output. */
+int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,
+                           const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* results,
+                           void* stream);
 
 /* ---- Grid front end (no reference counterpart: the reference takes its grid
  * from PySCF's dft.gen_grid.Grids, python/isdf.py:488-496) ------------------ */

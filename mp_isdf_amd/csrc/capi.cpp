@@ -6,6 +6,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <new>
 
@@ -158,6 +159,7 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
 
 }  // extern "C"
 
+#include "fit.hpp"
 #include "grid.hpp"
 #include "sketch.hpp"
 #include "sketch_mfma.hpp"
@@ -288,7 +290,75 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
                                        int64_t ldy, double* proj2, double* energy, void* stream)
 {
     return guard([&] {
-        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2) throw std::invalid_argument("invalid fold arguments");
-        mpisdf::fold_projections(b, n, m, Qb, Y, y_f32 != 0, ldy, proj2, energy, static_cast<hipStream_t>(stream));
+        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 2)
+            throw std::invalid_argument("invalid fold arguments");
+        if (y_f32 == 2) {
+            mpisdf::FoldScratch scratch;
+            mpisdf::fold_projections_bf16x3(b, n, m, Qb, static_cast<const float*>(Y), ldy, proj2, energy, scratch,
+                                            static_cast<hipStream_t>(stream));
+            MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+        } else {
+            mpisdf::fold_projections(b, n, m, Qb, Y, y_f32 != 0, ldy, proj2, energy, static_cast<hipStream_t>(stream));
+        }
+    });
+}
+
+namespace {
+
+void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,
+             const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* theta, double* tsum,
+             double* results, hipStream_t s)
+{
+    if (k <= 0) throw std::invalid_argument("the fit needs at least one anchor");
+    for (int64_t i = 0; i < k; ++i)
+        if (!std::isfinite(anchor_w[i]) || anchor_w[i] == 0)
+            throw std::invalid_argument("anchor weights must be finite and nonzero");
+    if (batch <= 0) throw std::invalid_argument("fit_batch_size must be positive");
+    const auto t0 = std::chrono::steady_clock::now();
+    mpisdf::FitFactor f;
+    mpisdf::fit_factor(c, anchor_xyz_dev, k, f, s);
+    const auto t1 = std::chrono::steady_clock::now();
+    double err[2] = {0, 0};
+    if (n > 0) mpisdf::fit_stream(c, f, anchor_w, xyz_dev, w_dev, n, batch, theta, err, tsum, s);
+    MPISDF_HIP(hipStreamSynchronize(s));
+    const auto t2 = std::chrono::steady_clock::now();
+    if (results) {
+        results[0] = std::sqrt(err[0]);
+        results[1] = err[1] > 0 ? std::sqrt(err[0] / err[1]) : (err[0] == 0 ? 0.0 : INFINITY);
+        results[2] = f.condition_estimate;
+        results[3] = f.gram_diag_max;
+        results[4] = std::chrono::duration<double>(t1 - t0).count();
+        results[5] = std::chrono::duration<double>(t2 - t1).count();
+        results[6] = results[7] = 0;
+    }
+}
+
+}  // namespace
+
+extern "C" int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w, int64_t k,
+                               const double* xyz, const double* w, int64_t n, int64_t batch, double* theta,
+                               double* tsum, double* results)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0 || k < 0) throw std::invalid_argument("negative sizes");
+        for (int64_t i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(xyz[i])) throw std::invalid_argument("grid coordinates must be finite");
+        mpisdf::DeviceBuffer<double> dxyz, dw, dax;
+        dxyz.upload(xyz, 3 * n, nullptr);
+        dw.upload(w, n, nullptr);
+        dax.upload(anchor_xyz, 3 * k, nullptr);
+        run_fit(c, dax.ptr, anchor_w, k, dxyz.ptr, dw.ptr, n, batch, theta, tsum, results, nullptr);
+    });
+}
+
+extern "C" int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_dev, const double* anchor_w,
+                                      int64_t k, const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch,
+                                      double* results, void* stream)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        run_fit(c, anchor_xyz_dev, anchor_w, k, xyz_dev, w_dev, n, batch, nullptr, nullptr, results,
+                static_cast<hipStream_t>(stream));
     });
 }

@@ -1,0 +1,256 @@
+// ISDF fit on the selected anchors (FP64, GPU).
+//
+// The reference refits with an SVD of the k x nao^2 anchor-pair matrix P
+// (python/isdf.py:238-263: theta_g = V S^-1 U^T (phi_g (x) phi_g) / w_anchor)
+// and measures the grid-weighted pair-space error with one nao^2 outer
+// product per grid point (isdf.py:176-191).  Both touch nao^2-length
+// vectors.  Here the same least-squares solution is reached through the
+// k x k Gram of the pair columns,
+//     P P^T = (Phi_a Phi_a^T) o (Phi_a Phi_a^T) = G2,   P a_g = (Phi_a phi_g)^2 = s_g,
+// factorised G2 = L L^T in FP64:
+//     theta_g = L^-T L^-1 s_g / w_a,
+//     ||a_g - P^T theta_phys||^2 = ||phi_g||^4 - ||L^-1 s_g||^2.
+// Gram, RHS and triangular products are FP64 MFMA GEMMs; the diagonal
+// blocks of the Cholesky factor are factorised and inverted in LDS.
+#include "fit.hpp"
+
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mpisdf {
+namespace {
+
+constexpr int NB = 64;  // diagonal block of the blocked Cholesky / inverse
+constexpr double kDependent = 1e-14;
+
+int grid_for(int64_t work, int64_t cap = 65536)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, cap));
+}
+
+__global__ void square_kernel(double* __restrict__ a, int64_t count)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = a[i] * a[i];
+}
+
+// Factor the jb x jb diagonal block in place (lower Cholesky) and write its
+// inverse (lower, row-major jb x jb) to ``inv``.  A non-positive pivot sets
+// *info to the 1-based global row.
+__global__ __launch_bounds__(256) void potrf_block_kernel(double* __restrict__ a, int64_t lda, int jb, int64_t row0,
+                                                          const double* __restrict__ gdiag, double* __restrict__ inv,
+                                                          int* __restrict__ info)
+{
+    __shared__ double s[NB][NB + 1];
+    __shared__ double x[NB][NB + 1];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int i = tid; i < jb * jb; i += 256) s[i / jb][i % jb] = a[(i / jb) * lda + (i % jb)];
+    __syncthreads();
+    for (int c = 0; c < jb; ++c) {
+        if (tid == 0) {
+            // A Schur pivot below 1e-14 of the column's own G2 diagonal means the
+            // pair column is dependent to ~1e-7 relative: beyond what the FP64
+            // normal equations resolve.
+            const double d = s[c][c];
+            if (!(d > kDependent * gdiag[row0 + c]) && !bad) { bad = 1; *info = (int)(row0 + c + 1); }
+            s[c][c] = d > 0 ? sqrt(d) : 1.0;
+        }
+        __syncthreads();
+        for (int r = c + 1 + tid; r < jb; r += 256) s[r][c] /= s[c][c];
+        __syncthreads();
+        const int rem = jb - c - 1;
+        for (int i = tid; i < rem * rem; i += 256) {
+            const int r = c + 1 + i / rem, q = c + 1 + i % rem;
+            if (q <= r) s[r][q] -= s[r][c] * s[q][c];
+        }
+        __syncthreads();
+    }
+    // inverse of the lower factor, one column per thread
+    if (tid < jb) {
+        const int c = tid;
+        for (int r = 0; r < jb; ++r) {
+            if (r < c) { x[r][c] = 0.0; continue; }
+            double v = r == c ? 1.0 : 0.0;
+            for (int p = c; p < r; ++p) v -= s[r][p] * x[p][c];
+            x[r][c] = v / s[r][r];
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < jb * jb; i += 256) {
+        const int r = i / jb, q = i % jb;
+        a[r * lda + q] = q <= r ? s[r][q] : 0.0;
+        inv[r * jb + q] = x[r][q];
+    }
+}
+
+// dst[r][c] += 1 for the identity block inside R (jb x width, row-major).
+__global__ void add_identity_kernel(double* __restrict__ r, int64_t ld, int jb, int64_t col0)
+{
+    const int i = threadIdx.x;
+    if (i < jb) r[i * ld + col0 + i] += 1.0;
+}
+
+// err2 accumulation: per grid column g of Z (k x nb, column-major ld k):
+//   qn = (sum_a phi_ga^2)^2 ; e = max(qn - ||z_g||^2, 0) ; acc += |w_g| e, tot += |w_g| qn
+__global__ void pair_error_kernel(const double* __restrict__ z, int64_t k, const double* __restrict__ phi, int64_t d,
+                                  const double* __restrict__ w, int64_t nb, double* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    double e = 0, t = 0;
+    if (g < nb) {
+        double zz = 0, pp = 0;
+        for (int64_t i = lane; i < k; i += 64) { const double v = z[i + g * k]; zz += v * v; }
+        for (int64_t a = lane; a < d; a += 64) { const double v = phi[g * d + a]; pp += v * v; }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) { zz += __shfl_xor(zz, off); pp += __shfl_xor(pp, off); }
+        const double qn = pp * pp;
+        const double aw = fabs(w[g]);
+        e = aw * fmax(qn - zz, 0.0);
+        t = aw * qn;
+    }
+    if (lane == 0 && g < nb) { atomicAdd(&out[0], e); atomicAdd(&out[1], t); }
+}
+
+// theta[t, g] /= w_anchor[t]   (theta k x nb row-major)
+__global__ void scale_rows_kernel(double* __restrict__ theta, int64_t k, int64_t nb, const double* __restrict__ wa)
+{
+    const int64_t total = k * nb;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        theta[i] /= wa[i / nb];
+}
+
+}  // namespace
+
+void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f, hipStream_t s)
+{
+    const int64_t d = ctx.nao;
+    f.k = k;
+    f.d = d;
+    f.phi_a.resize((size_t)(k * d));
+    launch_ao_eval(ctx, anchor_xyz, k, kValues, kF64, f.phi_a.ptr, d, k * d, s);
+    // G2 = (Phi_a Phi_a^T) o (Phi_a Phi_a^T), row-major k x k
+    f.linv.resize((size_t)(k * k));
+    DeviceBuffer<double> gdiag;
+    DeviceBuffer<double> a((size_t)(k * k));
+    gemm_f64(k, k, d, 1.0, MatView{f.phi_a.ptr, d, 1}, MatView{f.phi_a.ptr, 1, d}, 0.0, MutView{a.ptr, k, 1}, s);
+    hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * k)), dim3(256), 0, s, a.ptr, k * k);
+    MPISDF_HIP(hipGetLastError());
+    // Diagonal of G2 gives the pair column norms; keep the max for diagnostics.
+    {
+        std::vector<double> diag(k);
+        MPISDF_HIP(hipMemcpy2DAsync(diag.data(), sizeof(double), a.ptr, (k + 1) * sizeof(double), sizeof(double), k,
+                                    hipMemcpyDeviceToHost, s));
+        MPISDF_HIP(hipStreamSynchronize(s));
+        f.gram_diag_max = *std::max_element(diag.begin(), diag.end());
+        gdiag.upload(diag.data(), k, s);
+    }
+    // Blocked right-looking Cholesky, lower, in place.
+    DeviceBuffer<int> info(1);
+    MPISDF_HIP(hipMemsetAsync(info.ptr, 0, sizeof(int), s));
+    const int64_t nblocks = (k + NB - 1) / NB;
+    DeviceBuffer<double> invd((size_t)(nblocks * NB * NB)), panel;
+    for (int64_t j0 = 0; j0 < k; j0 += NB) {
+        const int jb = (int)std::min<int64_t>(NB, k - j0);
+        double* inv_j = invd.ptr + (j0 / NB) * NB * NB;
+        hipLaunchKernelGGL(potrf_block_kernel, dim3(1), dim3(256), 0, s, a.ptr + j0 * k + j0, k, jb, j0, gdiag.ptr,
+                           inv_j, info.ptr);
+        MPISDF_HIP(hipGetLastError());
+        const int64_t rem = k - j0 - jb;
+        if (rem <= 0) continue;
+        // panel <- A[j0+jb:, j0:j0+jb] inv(L_jj)^T
+        if (panel.count < (size_t)(rem * jb)) panel.resize((size_t)(rem * jb));
+        gemm_f64(rem, jb, jb, 1.0, MatView{a.ptr + (j0 + jb) * k + j0, k, 1}, MatView{inv_j, 1, jb}, 0.0,
+                 MutView{panel.ptr, jb, 1}, s);
+        MPISDF_HIP(hipMemcpy2DAsync(a.ptr + (j0 + jb) * k + j0, k * sizeof(double), panel.ptr, jb * sizeof(double),
+                                    jb * sizeof(double), rem, hipMemcpyDeviceToDevice, s));
+        // trailing A[j0+jb:, j0+jb:] -= panel panel^T
+        gemm_f64(rem, rem, jb, -1.0, MatView{panel.ptr, jb, 1}, MatView{panel.ptr, 1, jb}, 1.0,
+                 MutView{a.ptr + (j0 + jb) * k + (j0 + jb), k, 1}, s);
+    }
+    int hinfo = 0;
+    MPISDF_HIP(hipMemcpyAsync(&hinfo, info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+    MPISDF_HIP(hipStreamSynchronize(s));
+    if (hinfo)
+        throw std::runtime_error("ISDF fit Gram (Phi_a Phi_a^T)^2 is not positive definite at row " +
+                                 std::to_string(hinfo) + " (pivot below 1e-14 of its diagonal): the anchor pair "
+                                 "columns are numerically dependent");
+    // L^-1 by blocks: Linv[j, :] = inv(L_jj) (I_j - L[j, :j] Linv[:j, :])
+    MPISDF_HIP(hipMemsetAsync(f.linv.ptr, 0, (size_t)(k * k) * sizeof(double), s));
+    DeviceBuffer<double> r((size_t)(NB * k));
+    std::vector<double> ldiag(k);
+    for (int64_t j0 = 0; j0 < k; j0 += NB) {
+        const int jb = (int)std::min<int64_t>(NB, k - j0);
+        const int64_t width = j0 + jb;
+        MPISDF_HIP(hipMemsetAsync(r.ptr, 0, (size_t)(jb * width) * sizeof(double), s));
+        if (j0 > 0)
+            gemm_f64(jb, j0, j0, -1.0, MatView{a.ptr + j0 * k, k, 1}, MatView{f.linv.ptr, k, 1}, 0.0,
+                     MutView{r.ptr, width, 1}, s);
+        hipLaunchKernelGGL(add_identity_kernel, dim3(1), dim3(64), 0, s, r.ptr, width, jb, j0);
+        gemm_f64(jb, width, jb, 1.0, MatView{invd.ptr + (j0 / NB) * NB * NB, jb, 1}, MatView{r.ptr, width, 1}, 0.0,
+                 MutView{f.linv.ptr + j0 * k, k, 1}, s);
+    }
+    MPISDF_HIP(hipMemcpy2DAsync(ldiag.data(), sizeof(double), a.ptr, (k + 1) * sizeof(double), sizeof(double), k,
+                                hipMemcpyDeviceToHost, s));
+    MPISDF_HIP(hipStreamSynchronize(s));
+    double lo = ldiag[0], hi = ldiag[0];
+    for (double v : ldiag) { lo = std::min(lo, v); hi = std::max(hi, v); }
+    // cond(G2) estimate from the factor's diagonal (a lower bound of the true condition)
+    f.condition_estimate = (hi / lo) * (hi / lo);
+}
+
+void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const double* xyz, const double* w, int64_t n,
+                int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s)
+{
+    const int64_t k = f.k, d = f.d;
+    batch = std::max<int64_t>(1, std::min(batch, n));
+    DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
+    DeviceBuffer<double> th, tsum;
+    wa.upload(anchor_w, k, s);
+    if (tsum_out) {
+        tsum.resize(k);
+        MPISDF_HIP(hipMemsetAsync(tsum.ptr, 0, k * sizeof(double), s));
+        th.resize((size_t)(k * batch));
+    }
+    if (theta_host && !th.count) th.resize((size_t)(k * batch));
+    MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t nb = std::min(batch, n - start);
+        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
+        // S = (Phi_a Phi_b^T)^2, k x nb column-major
+        gemm_f64(k, nb, d, 1.0, MatView{f.phi_a.ptr, d, 1}, MatView{phi.ptr, 1, d}, 0.0, MutView{sm.ptr, 1, k}, s);
+        hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * nb)), dim3(256), 0, s, sm.ptr, k * nb);
+        // Z = L^-1 S
+        gemm_f64(k, nb, k, 1.0, MatView{f.linv.ptr, k, 1}, MatView{sm.ptr, 1, k}, 0.0, MutView{z.ptr, 1, k}, s);
+        if (err_out)
+            hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
+                               w + start, nb, acc.ptr);
+        if (theta_host || tsum_out) {
+            // theta_phys = L^-T Z, stored row-major k x nb; theta = theta_phys / w_anchor
+            gemm_f64(k, nb, k, 1.0, MatView{f.linv.ptr, 1, k}, MatView{z.ptr, 1, k}, 0.0, MutView{th.ptr, nb, 1}, s);
+            hipLaunchKernelGGL(scale_rows_kernel, dim3(grid_for(k * nb)), dim3(256), 0, s, th.ptr, k, nb, wa.ptr);
+            if (tsum_out)  // t += theta w_grid
+                gemm_f64(k, 1, nb, 1.0, MatView{th.ptr, nb, 1}, MatView{w + start, 1, 1}, 1.0, MutView{tsum.ptr, 1, 1},
+                         s);
+            if (theta_host)  // rows of theta[k][n] on the host
+                MPISDF_HIP(hipMemcpy2DAsync(theta_host + start, n * sizeof(double), th.ptr, nb * sizeof(double),
+                                            nb * sizeof(double), k, hipMemcpyDeviceToHost, s));
+        }
+        MPISDF_HIP(hipGetLastError());
+    }
+    if (err_out) {
+        const auto h = acc.download(s);
+        err_out[0] = h[0];
+        err_out[1] = h[1];
+    }
+    if (tsum_out) {
+        MPISDF_HIP(hipMemcpyAsync(tsum_out, tsum.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
+        MPISDF_HIP(hipStreamSynchronize(s));
+    }
+}
+
+}  // namespace mpisdf

@@ -1,0 +1,28 @@
+// ISDF fit on selected anchors (see fit.hip).
+#pragma once
+
+#include "context.hpp"
+
+namespace mpisdf {
+
+struct FitFactor {
+    int64_t k = 0, d = 0;
+    DeviceBuffer<double> phi_a;  // [k][d] anchor AO values
+    DeviceBuffer<double> linv;   // [k][k] row-major inverse of the Cholesky factor of G2
+    double condition_estimate = 0, gram_diag_max = 0;
+};
+
+// Anchor AOs, G2 = (Phi_a Phi_a^T)^2, Cholesky and triangular inverse.
+// anchor_xyz is a device pointer [k][3].  Throws std::runtime_error when G2
+// is not positive definite.
+void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f, hipStream_t s);
+
+// Streams grid batches (device xyz[n][3], device w[n]) through
+//   S = (Phi_a Phi_b^T)^2, Z = L^-1 S, theta = L^-T Z / w_anchor (host anchor_w[k]).
+// err_out (host, 2 doubles): sum |w| (||phi||^4 - ||z||^2), sum |w| ||phi||^4.
+// theta_host (optional): row-major [k][n].  tsum_out (optional host [k]):
+// sum_g theta[:, g] w_g.
+void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const double* xyz, const double* w, int64_t n,
+                int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s);
+
+}  // namespace mpisdf

@@ -46,4 +46,18 @@ void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, con
 void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, bool y_f32, int64_t ldy,
                       double* proj2, double* energy, hipStream_t stream);
 
+// Same fold for FP32 Y with split-BF16 MFMA (hi*hi + hi*lo + lo*hi) and FP64
+// accumulation across 64-wide K chunks; error ~1e-6 ||q|| ||y_j|| per entry.
+struct FoldScratch {
+    DeviceBuffer<uint16_t> hi, lo;
+};
+// Per-entry error scale of the split fold relative to ||q|| ||y_j|| (FP32
+// chunk accumulation over 64 K values, dropped lo*lo term, split residuals).
+constexpr double kFoldEpsW = 2e-6;
+// When ynorm2/err2 are given, err2[j] accumulates a bound on the error of the
+// folded sum: 2 eps ||y_j|| sqrt(s_j) + b eps^2 ||y_j||^2 per fold.
+void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, const float* Y, int64_t ldy,
+                             double* proj2, double* energy, FoldScratch& scratch, hipStream_t stream,
+                             const double* ynorm2 = nullptr, double* err2 = nullptr);
+
 }  // namespace mpisdf

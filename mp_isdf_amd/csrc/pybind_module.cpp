@@ -220,9 +220,9 @@ public:
                           py::none(), 1e-24, precision, diag.data());
         static const char* names[] = {"rows", "columns", "numerical_rank", "rank_cap", "rank", "sketch_norm",
                                       "relative_tail", "blocks", "folds", "seconds_sketch", "seconds_qrcp",
-                                      "seconds_total"};
+                                      "seconds_total", "stalls", "refreshes"};
         py::dict d;
-        for (int i = 0; i < 12; ++i) d[names[i]] = diag[i];
+        for (int i = 0; i < 14; ++i) d[names[i]] = diag[i];
         return py::make_tuple(idx, d);
     }
 
@@ -300,6 +300,48 @@ PYBIND11_MODULE(mp_isdf, m)
     }, py::arg("charges"), py::arg("coords"), py::arg("level") = 3,
        "Becke-partitioned molecular grid (Bohr): returns (coords[n,3], weights[n]).  Not part of the reference "
        "extension, which receives PySCF's grid.");
+
+    m.def("isdf_fit", [](const Context& ctx, const FloatArray& anchor_coords, const FloatArray& anchor_weights,
+                         const FloatArray& coords, const FloatArray& weights, int64_t batch_size, bool interp_vecs,
+                         bool weight_sums) {
+        if (anchor_coords.ndim() != 2 || anchor_coords.shape(1) != 3 || anchor_weights.ndim() != 1 ||
+            anchor_weights.shape(0) != anchor_coords.shape(0))
+            throw py::value_error("expected anchor_coords[k,3] and anchor_weights[k]");
+        if (coords.ndim() != 2 || coords.shape(1) != 3 || weights.ndim() != 1 || weights.shape(0) != coords.shape(0))
+            throw py::value_error("expected coords[n,3] and weights[n]");
+        const py::ssize_t k = anchor_coords.shape(0), n = coords.shape(0);
+        py::array_t<double> theta, tsum;
+        double* pt = nullptr;
+        double* ps = nullptr;
+        if (interp_vecs) { theta = py::array_t<double>({k, n}); pt = theta.mutable_data(); }
+        if (weight_sums) { tsum = py::array_t<double>(k); ps = tsum.mutable_data(); }
+        std::vector<double> res(8, 0.0);
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_isdf_fit(ctx.get(), anchor_coords.data(), anchor_weights.data(), k, coords.data(),
+                                     weights.data(), n, batch_size, pt, ps, res.data());
+        }
+        check(status);
+        py::dict out;
+        py::dict err;
+        err["l2"] = res[0];
+        err["relative"] = res[1];
+        out["pair_space_error"] = err;
+        py::dict diag;
+        diag["condition_estimate"] = res[2];
+        diag["gram_diag_max"] = res[3];
+        diag["seconds_factor"] = res[4];
+        diag["seconds_grid_pass"] = res[5];
+        out["fit_diagnostics"] = diag;
+        out["interp_vecs"] = interp_vecs ? py::object(theta) : py::object(py::none());
+        out["theta_weight_sums"] = weight_sums ? py::object(tsum) : py::object(py::none());
+        return out;
+    }, py::arg("context"), py::arg("anchor_coords"), py::arg("anchor_weights"), py::arg("coords"),
+       py::arg("weights"), py::arg("batch_size") = 4096, py::arg("interp_vecs") = false,
+       py::arg("weight_sums") = false,
+       "GPU ISDF fit on anchors (normal equations of the anchor-pair least squares, FP64 Cholesky): "
+       "pair-space error, optional interpolation vectors theta[k,n] and sum_g theta[:,g] w_g.");
 
     py::class_<Selector>(m, "AnchorSelection")
         .def(py::init<const Context&>(), py::arg("context"), py::keep_alive<1, 2>())

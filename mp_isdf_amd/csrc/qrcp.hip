@@ -29,6 +29,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <limits>
 
 namespace mpisdf {
@@ -61,12 +62,14 @@ __global__ void colnorm2_kernel(const T* __restrict__ y, int64_t m, int64_t n, i
     if (lane == 0) out[col] = s;
 }
 
+// Upper bound of the residual norm^2 of every unselected column: folded value
+// plus the fold's accumulated error bound plus a relative margin.
 __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __restrict__ proj2,
-                            const uint8_t* __restrict__ selected, double margin, int64_t n,
-                            double* __restrict__ keys, int64_t* __restrict__ idx)
+                            const double* __restrict__ err2, const uint8_t* __restrict__ selected, double margin,
+                            int64_t n, double* __restrict__ keys, int64_t* __restrict__ idx)
 {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-        keys[j] = selected[j] ? -1.0 : fmax(ynorm2[j] - proj2[j], 0.0) + margin * ynorm2[j];
+        keys[j] = selected[j] ? -1.0 : fmax(ynorm2[j] - proj2[j], 0.0) + err2[j] + margin * ynorm2[j];
         idx[j] = j;
     }
 }
@@ -249,16 +252,17 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     if (m <= 0 || n <= 0) return {};
     if (ldy < m) throw std::invalid_argument("QRCP leading dimension smaller than rows");
     const int64_t kmax = std::min({opt.kmax > 0 ? opt.kmax : std::min(m, n), m, n});
-    const int64_t C = std::max<int64_t>(1, std::min(opt.candidates, n));
+    const int64_t C0 = std::max<int64_t>(1, std::min(opt.candidates, n));
     QrcpResult res;
 
-    DeviceBuffer<double> ynorm2(n), proj2(n), keys(n), keys_sorted(n);
+    DeviceBuffer<double> ynorm2(n), proj2(n), err2(n), keys(n), keys_sorted(n);
     DeviceBuffer<int64_t> idx(n), idx_sorted(n);
     DeviceBuffer<uint8_t> selected(n);
-    DeviceBuffer<double> Q((size_t)m * std::max<int64_t>(kmax, 1)), rc((size_t)m * C), rn2(C), H, qv(m), dv;
+    DeviceBuffer<double> Q((size_t)m * std::max<int64_t>(kmax, 1)), rc, rn2, H, qv(m), dv;
     DeviceBuffer<double> energy(std::max<int64_t>(kmax, 1)), tail(1);
     DeviceBuffer<ArgMax> am(1);
     MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
+    MPISDF_HIP(hipMemsetAsync(err2.ptr, 0, n * sizeof(double), s));
     MPISDF_HIP(hipMemsetAsync(selected.ptr, 0, n, s));
     MPISDF_HIP(hipMemsetAsync(energy.ptr, 0, std::max<int64_t>(kmax, 1) * sizeof(double), s));
 
@@ -289,24 +293,42 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     int64_t k = 0, folded = 0;
     double r00 = 0;
     bool done = false;
+    FoldScratch fold_scratch;
+    const bool split = opt.split_bf16_fold && ytype == kF32;
     auto fold = [&](double* energy_out) {
         const int64_t b = k - folded;
         if (b <= 0) return;
-        fold_projections(b, n, m, Q.ptr + folded * m, y, ytype == kF32, ldy, proj2.ptr,
-                         energy_out ? energy_out + folded : nullptr, s);
+        if (split)
+            fold_projections_bf16x3(b, n, m, Q.ptr + folded * m, static_cast<const float*>(y), ldy, proj2.ptr,
+                                    energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
+        else
+            fold_projections(b, n, m, Q.ptr + folded * m, y, ytype == kF32, ldy, proj2.ptr,
+                             energy_out ? energy_out + folded : nullptr, s);
         folded = k;
         ++res.folds;
     };
 
+    // Exact FP64 recomputation of every column's folded norm (clears the
+    // split fold's error bounds); used when no pivot can be certified.
+    auto refresh = [&]() {
+        MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
+        MPISDF_HIP(hipMemsetAsync(err2.ptr, 0, n * sizeof(double), s));
+        if (k > 0) fold_projections(k, n, m, Q.ptr, y, ytype == kF32, ldy, proj2.ptr, nullptr, s);
+        ++res.refreshes;
+    };
+    int64_t C = C0;
+    bool refreshed = false, force = false;
+    const int64_t Cmax = std::max<int64_t>(C0, std::min<int64_t>(n, (int64_t(1) << 30) / std::max<int64_t>(m, 1)));
     while (!done && k < kmax) {
         fold(opt.want_energy ? energy.ptr : nullptr);
-        hipLaunchKernelGGL(keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, ynorm2.ptr, proj2.ptr, selected.ptr,
-                           opt.margin, n, keys.ptr, idx.ptr);
+        hipLaunchKernelGGL(keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, ynorm2.ptr, proj2.ptr, err2.ptr,
+                           selected.ptr, opt.margin, n, keys.ptr, idx.ptr);
         MPISDF_HIP(hipGetLastError());
         MPISDF_HIP(hipcub::DeviceRadixSort::SortPairsDescending(sort_ws.ptr, sort_bytes, keys.ptr, keys_sorted.ptr,
                                                                 idx.ptr, idx_sorted.ptr, (int)n, 0, 64, s));
         const int64_t remaining = n - k;
         const int64_t Ceff = std::min(C, remaining);
+        if (rc.count < (size_t)(m * Ceff)) { rc.resize((size_t)(m * Ceff)); rn2.resize(Ceff); }
         double bound2 = -1.0;
         if (Ceff < remaining)
             MPISDF_HIP(hipMemcpyAsync(&bound2, keys_sorted.ptr + Ceff, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -350,7 +372,9 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             // first pivot after a fold may tie the bound: candidates are the
             // lowest-index columns among equal keys (stable sort).
             const bool first_after_fold = (k == block_start);
-            if (!(best.value > bound2 || (first_after_fold && best.value >= bound2 && opt.margin == 0.0))) break;
+            if (!(best.value > bound2 || (first_after_fold && best.value >= bound2 && opt.margin == 0.0) ||
+                  (first_after_fold && force)))
+                break;
             res.pivots.push_back(best.column);
             res.rdiag.push_back(norm);
             hipLaunchKernelGGL(make_q_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, rc.ptr, m, am.ptr, qv.ptr,
@@ -368,7 +392,24 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             MPISDF_HIP(hipGetLastError());
             ++k;
         }
-        if (k == kmax) break;
+        if (opt.debug)
+            fprintf(stderr, "[qrcp] block %lld: pivots %lld..%lld candidates %lld bound2 %.3e last_r %.3e folds %lld\n",
+                    (long long)res.blocks, (long long)block_start, (long long)k, (long long)Ceff, bound2,
+                    res.rdiag.empty() ? 0.0 : res.rdiag.back(), (long long)res.folds);
+        if (k == kmax || done) break;
+        if (k == block_start) {
+            // No pivot could be certified: a non-candidate's bound ties or beats
+            // every exact candidate.  Widen the candidate set; once it cannot
+            // grow, make the bounds exact; if exact bounds still tie (FP64
+            // rounding of ||y||^2 - ||Q^T y||^2), take the best candidate.
+            ++res.stalls;
+            if (Ceff < remaining && C < Cmax) C = std::min(2 * C, Cmax);
+            else if (split && !refreshed) { refresh(); refreshed = true; }
+            else force = true;
+        } else {
+            refreshed = false;
+            force = false;
+        }
     }
     if (opt.want_energy) {
         fold(energy.ptr);

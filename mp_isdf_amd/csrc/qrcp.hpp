@@ -15,6 +15,7 @@ struct QrcpOptions {
     bool want_tail = false;      // exact residual Frobenius^2 after the last pivot
     bool debug = false;
     const double* ynorm2 = nullptr;  // optional device ||y_j||^2 (skips the norm pass)
+    bool split_bf16_fold = false;    // FP32 Y: fold with split-BF16 MFMA (needs margin > 0)
 };
 
 struct QrcpResult {
@@ -23,7 +24,7 @@ struct QrcpResult {
     std::vector<double> energy;   // per pivot row (if requested)
     double frob2 = 0.0;           // ||Y||_F^2
     double tail2 = 0.0;           // residual Frobenius^2 after the last pivot (if requested)
-    int64_t blocks = 0, folds = 0;
+    int64_t blocks = 0, folds = 0, stalls = 0, refreshes = 0;
 };
 
 // Y is m x n column-major with leading dimension ldy, element type f64 or f32.

@@ -207,6 +207,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
     DeviceBuffer<double> y64;
     DeviceBuffer<float> y32;
+    const int64_t ldy32 = (ell + 3) / 4 * 4;  // 16-byte aligned FP32 columns
     DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;
     if (!lowp) {
@@ -227,7 +228,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         const size_t esz = in_type == kFP8 ? 1 : 2;
         DeviceBuffer<uint8_t> ut((size_t)ell * dp * esz), vt((size_t)ell * dp * esz);
         prepare_omega_lowp(u.ptr, v.ptr, d, ell, dp, in_type, ut.ptr, vt.ptr, s);
-        y32.resize((size_t)ell * n);
+        y32.resize((size_t)ldy32 * n);
         ynorm2.resize(n);
         MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
         const bool ao_only = spec.kinds.size() == 1 && spec.kinds[0] == 0;
@@ -258,8 +259,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
                 }
                 quantize_fp8_rows(phi32.ptr, d, count, d, sw.ptr + start, ell, phi.ptr, dp, scale.ptr, s);
             }
-            kr_sketch_lowp(in_type, ut.ptr, vt.ptr, ell, dp, phi.ptr, dp, count, scale.ptr, y32.ptr + start * ell, ell,
-                           ynorm2.ptr + start, bad.ptr, s);
+            kr_sketch_lowp(in_type, ut.ptr, vt.ptr, ell, dp, phi.ptr, dp, count, scale.ptr, y32.ptr + start * ldy32,
+                           ldy32, ynorm2.ptr + start, bad.ptr, s);
         }
     }
     if (bad.download(s)[0]) throw std::overflow_error("non-finite randomized ID sketch");
@@ -277,7 +278,11 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     opt.debug = a.debug;
     const auto tq = Clock::now();
     opt.ynorm2 = lowp ? ynorm2.ptr : nullptr;
-    const QrcpResult r = lowp ? qrcp_device(y32.ptr, kF32, ell, n, ell, opt, s)
+    // The reduced-precision forms fold with exact FP64 products of the FP32
+    // sketch: certifying greedy pivots late in the factorisation needs folded
+    // norms accurate to ~1e-9 of ||y_j||^2, beyond what split-BF16 products
+    // deliver (the split fold stays available through QrcpOptions).
+    const QrcpResult r = lowp ? qrcp_device(y32.ptr, kF32, ell, n, ldy32, opt, s)
                               : qrcp_device(y64.ptr, kF64, ell, n, ell, opt, s);
     const double t_qrcp = seconds_since(tq);
     const int64_t K = (int64_t)r.pivots.size();
@@ -325,6 +330,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         diag[kDiagFolds] = (double)r.folds;
         diag[kDiagSecondsSketch] = t_sketch;
         diag[kDiagSecondsQrcp] = t_qrcp;
+        diag[kDiagStalls] = (double)r.stalls;
+        diag[kDiagRefreshes] = (double)r.refreshes;
     }
     return out;
 }

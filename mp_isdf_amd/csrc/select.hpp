@@ -43,6 +43,8 @@ enum Diag : int {
     kDiagSecondsSketch = 9,
     kDiagSecondsQrcp = 10,
     kDiagSecondsTotal = 11,
+    kDiagStalls = 12,       // QRCP blocks without a certified pivot
+    kDiagRefreshes = 13,    // exact FP64 refreshes of the folded norms
     kDiagCount = 16
 };
 

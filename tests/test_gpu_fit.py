@@ -1,0 +1,75 @@
+"""GPU ISDF fit against the oracle's restatement of the reference refit
+(python/isdf.py:238-263, SVD with 1e-12 cutoff) and pair-space error
+(isdf.py:176-191).  Normal equations square the conditioning, so agreement
+is checked at 1e-7 relative on well-conditioned anchor sets."""
+import numpy as np
+import pytest
+
+import oracle
+from mp_isdf_amd import molecule
+
+pytestmark = pytest.mark.gpu
+
+
+def water_case(nanchor, seed=3):
+    mol = molecule.single_water("sto-3g")
+    rng = np.random.default_rng(seed)
+    pts = rng.normal(scale=1.4, size=(900, 3))
+    w = rng.uniform(0.05, 1.0, size=900)
+    w[5] = -0.3
+    anchors = oracle.select(mol.atm, mol.bas, mol.env, pts, w, "ao", max_rank=nanchor)
+    return mol, pts, w, np.asarray(anchors)
+
+
+@pytest.mark.parametrize("nanchor", [6, 20])
+def test_fit_matches_oracle_refit(mp_isdf, nanchor):
+    mol, pts, w, anchors = water_case(nanchor)
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    fit = mp_isdf.isdf_fit(ctx, pts[anchors], w[anchors], pts, w, batch_size=128, interp_vecs=True,
+                           weight_sums=True)
+    ao = oracle.ao_eval(mol.atm, mol.bas, mol.env, pts)
+    aa = ao[anchors]
+    pairs = (aa[:, :, None] * aa[:, None, :]).reshape(len(anchors), -1)
+    theta, _, diag = oracle.physical_pair_refit(pairs, w[anchors], ao, 256)
+    assert diag["retained_rank"] == len(anchors)
+    np.testing.assert_allclose(fit["interp_vecs"], theta, rtol=1e-7, atol=1e-9 * np.abs(theta).max())
+    err = oracle.stream_pair_space_error_norms(ao, ao, pairs, w[anchors], theta, w)
+    np.testing.assert_allclose(fit["pair_space_error"]["l2"], err["l2"], rtol=1e-6)
+    np.testing.assert_allclose(fit["pair_space_error"]["relative"], err["relative"], rtol=1e-6)
+    np.testing.assert_allclose(fit["theta_weight_sums"], theta @ w, rtol=1e-7, atol=1e-9 * np.abs(theta @ w).max())
+
+
+def test_fit_batch_invariance(mp_isdf):
+    mol, pts, w, anchors = water_case(12)
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    ref = mp_isdf.isdf_fit(ctx, pts[anchors], w[anchors], pts, w, batch_size=100000, interp_vecs=True)
+    for batch in (1, 7):
+        got = mp_isdf.isdf_fit(ctx, pts[anchors], w[anchors], pts, w, batch_size=batch, interp_vecs=True)
+        np.testing.assert_allclose(got["interp_vecs"], ref["interp_vecs"], rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(got["pair_space_error"]["l2"], ref["pair_space_error"]["l2"], rtol=1e-10)
+
+
+def test_dependent_anchors_raise(mp_isdf):
+    mol, pts, w, anchors = water_case(6)
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    dup = np.concatenate([anchors, anchors[:1]])
+    with pytest.raises(RuntimeError, match="positive definite"):
+        mp_isdf.isdf_fit(ctx, pts[dup], w[dup], pts, w)
+
+
+def test_build_isdf_driver(mp_isdf):
+    from mp_isdf_amd.isdf import build_isdf
+    mol = molecule.hydrogen_molecule("sto-3g")
+    res = build_isdf(mol, level=1, anchor_rank=3, pair_space_error=True)
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    from mp_isdf_amd import grid
+    xyz, w = grid.make_grid(mol, 1)
+    expected = mp_isdf.AnchorSelection(ctx).select(xyz, w, mp_isdf.AxisType.AtomicOrbital, max_rank=3)
+    np.testing.assert_array_equal(res["anchor_indices"], expected)
+    assert list(res["timings"]) == ["grid", "context", "anchor_selection", "ao_evaluation", "isdf_fit",
+                                    "pair_space_check", "overlap_check"]
+    # 3 anchors span the 3 symmetric STO-3G H2 pair functions exactly
+    assert res["pair_space_errors"]["scalar"]["relative"] < 1e-6
+    assert res["overlap_relative_error"] < 1e-6
+    with pytest.raises(ValueError, match="mutually exclusive"):
+        build_isdf(mol, anchor_algorithm="rand_sketch", anchor_rank=2, anchor_tol=1e-6, selection_only=True)

@@ -80,3 +80,31 @@ def test_lowp_selection_is_valid(mp_isdf):
                          seed=7, precision=precision)
         assert len(got) == 120 and len(set(got)) == 120
         assert not set(got) & set(np.flatnonzero(w == 0).tolist())
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_qrcp_fold_modes(mp_isdf, mode):
+    """Fold = column / row sums of (Q^T Y)^2 against NumPy; the split-BF16
+    mode is checked against its stated bound (1e-5 of ||y_j||^2)."""
+    import torch
+    rng = np.random.default_rng(3)
+    m, n, b = 1003, 3001, 150
+    y = rng.normal(size=(n, m)) * rng.uniform(0.1, 3.0, size=(n, 1))
+    y = y.astype(np.float32).astype(np.float64)
+    q = np.linalg.qr(rng.normal(size=(m, b)))[0]
+    w = q.T @ y.T
+    want_cols = np.sum(w ** 2, axis=0)
+    want_rows = np.sum(w ** 2, axis=1)
+    ld = m + 1 if mode == 0 else (m + 3) // 4 * 4
+    ybuf = np.zeros((n, ld), dtype=np.float64 if mode == 0 else np.float32)
+    ybuf[:, :m] = y
+    Y = torch.tensor(ybuf, device="cuda")
+    Qb = torch.tensor(np.ascontiguousarray(q.T), device="cuda")  # column-major m x b
+    proj = torch.zeros(n, dtype=torch.float64, device="cuda")
+    energy = torch.zeros(b, dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_qrcp_fold_device", b, n, m, Qb, Y, mode, ld, proj, energy, None)
+    torch.cuda.synchronize()
+    ynorm2 = np.sum(y ** 2, axis=1)
+    tol = 1e-12 if mode < 2 else 1e-5
+    assert np.max(np.abs(proj.cpu().numpy() - want_cols) / ynorm2) < tol
+    np.testing.assert_allclose(energy.cpu().numpy(), want_rows, rtol=max(tol, 1e-12))
