@@ -92,14 +92,35 @@ def main():
     alg = mp.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
     ao = mp.AxisType.AtomicOrbital
 
+    cctx = capi.Context(mol.atm, mol.bas, mol.env)
+    xyz_dev = torch.tensor(xyz[keep], device="cuda")
+    w_dev = torch.tensor(w[keep], device="cuda")
+
+    def fit(anchors, grid_pass):
+        """Fit factorisation (Gram, Cholesky, L^-1); grid_pass adds the
+        pair-space error pass over every kept grid point."""
+        ax = torch.tensor(xyz[anchors], device="cuda")
+        aw = np.ascontiguousarray(w[anchors])
+        res = np.zeros(8)
+        capi.call("mpisdf_isdf_fit_device", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev,
+                  n if grid_pass else 0, 8192, res, None)
+        return res
+
     def step(precision):
-        return sel.select_with_diagnostics(xyz, w, ao, algorithm=alg, max_rank=k, oversampling=oversampling,
-                                           seed=7, precision=precision)
+        t = time.perf_counter()
+        anchors, d = sel.select_with_diagnostics(xyz, w, ao, algorithm=alg, max_rank=k, oversampling=oversampling,
+                                                 seed=7, precision=precision)
+        t_sel = time.perf_counter() - t
+        res = fit(anchors, grid_pass=False)
+        d = dict(d)
+        d["seconds_select"] = t_sel
+        d["seconds_fit"] = res[4]
+        return anchors, d
 
     for i in range(args.warmup):
         _, d = step(args.precision)
-        log(f"warmup {i}: {d['seconds_total']:.2f} s (sketch {d['seconds_sketch']:.2f}, qrcp {d['seconds_qrcp']:.2f}, "
-            f"blocks {int(d['blocks'])})")
+        log(f"warmup {i}: select {d['seconds_select']:.2f} s (sketch {d['seconds_sketch']:.2f}, qrcp "
+            f"{d['seconds_qrcp']:.2f}, blocks {int(d['blocks'])}), fit {d['seconds_fit']:.2f} s")
     torch.cuda.synchronize()
     times, diags, anchors = [], [], None
     for i in range(args.steps):
@@ -110,9 +131,17 @@ def main():
         times.append(time.perf_counter() - t)
         diags.append(d)
         log(f"step {i}: {times[-1]:.2f} s (sketch {d['seconds_sketch']:.2f}, qrcp {d['seconds_qrcp']:.2f}, "
-            f"blocks {int(d['blocks'])})")
+            f"blocks {int(d['blocks'])}, fit {d['seconds_fit']:.2f})")
     t_step = max(times) if times else float("nan")
     d = diags[-1] if diags else {}
+
+    # Fit quality of the value form: grid pass with the pair-space error.
+    t = time.perf_counter()
+    err = fit(anchors, grid_pass=True)
+    t_pass = time.perf_counter() - t
+    quality = {"pair_space_relative_error": float(err[1]), "grid_pass_seconds": round(err[5], 3),
+               "gram_condition_estimate": float(err[2])}
+    log(f"pair-space relative error {err[1]:.4e} (grid pass {err[5]:.2f} s)")
 
     fp64_form = None
     if not args.no_fp64_form and args.precision != "fp64":
@@ -122,10 +151,13 @@ def main():
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t
         overlap = len(set(ref) & set(anchors)) / max(1, len(ref))
+        err64 = fit(ref, grid_pass=True)
         fp64_form = {"seconds": round(t64, 4), "seconds_sketch": round(d64["seconds_sketch"], 4),
-                     "seconds_qrcp": round(d64["seconds_qrcp"], 4), "blocks": int(d64["blocks"]),
-                     "speedup_of_value_form": round(t64 / t_step, 3), "anchor_overlap": round(overlap, 4)}
-        log(f"fp64 form: {t64:.2f} s; overlap {overlap:.3f}")
+                     "seconds_qrcp": round(d64["seconds_qrcp"], 4), "seconds_fit": round(d64["seconds_fit"], 4),
+                     "blocks": int(d64["blocks"]), "speedup_of_value_form": round(t64 / t_step, 3),
+                     "anchor_overlap": round(overlap, 4), "pair_space_relative_error": float(err64[1]),
+                     "relative_error_difference": float(abs(err[1] - err64[1]) / err64[1]) if err64[1] else None}
+        log(f"fp64 form: {t64:.2f} s; overlap {overlap:.3f}; pair error {err64[1]:.4e}")
 
     # ---- dominant kernels, timed with HIP events on the default stream -------
     roofline = roofline_sketch = None
@@ -136,7 +168,6 @@ def main():
     u_h = np.empty((mol.nao, ell))
     v_h = np.empty((mol.nao, ell))
     capi.call("mpisdf_omega", mol.nao, ell, 7, u_h, v_h)
-    cctx = capi.Context(mol.atm, mol.bas, mol.env)
     if args.precision in ("bf16", "fp8"):
         code = 2 if args.precision == "bf16" else 3
         et = torch.bfloat16 if code == 2 else torch.uint8
@@ -203,15 +234,17 @@ def main():
 
     value = n / t_step
     out = {
-        "metric": "anchor-select grid-pts/s (rand_sketch, BF16/FP8 vs FP64 GPU)",
+        "metric": "anchor-select+fit grid-pts/s (rand_sketch, BF16/FP8 vs FP64 GPU)",
         "value": round(value, 1), "unit": "grid-pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(t_step * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.precision, "data": "synthetic: seeded water cluster, library Becke grid, seed-7 Gaussian sketch",
         "config": {"workload": f"(H2O){nwater}/{basis} level-{level} grid rand_sketch k={k}",
-                   "grid_points": n, "nao": mol.nao, "anchors": k, "sketch_rows": ell, "step": "selection",
-                   "precision": args.precision},
+                   "grid_points": n, "nao": mol.nao, "anchors": k, "sketch_rows": ell,
+                   "step": "selection + fit factorisation", "precision": args.precision},
         "stages": {"seconds_sketch": round(d.get("seconds_sketch", 0), 4),
-                   "seconds_qrcp": round(d.get("seconds_qrcp", 0), 4), "qrcp_blocks": int(d.get("blocks", 0))},
+                   "seconds_qrcp": round(d.get("seconds_qrcp", 0), 4), "qrcp_blocks": int(d.get("blocks", 0)),
+                   "seconds_fit": round(d.get("seconds_fit", 0), 4)},
+        "fit_quality": quality,
         "fp64_form": fp64_form,
         "roofline": roofline,
         "roofline_sketch": roofline_sketch,
