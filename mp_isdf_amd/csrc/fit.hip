@@ -222,10 +222,10 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         const int64_t nb = std::min(batch, n - start);
         launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
         // S = (Phi_a Phi_b^T)^2, k x nb column-major
-        gemm_f64(k, nb, d, 1.0, MatView{f.phi_a.ptr, d, 1}, MatView{phi.ptr, 1, d}, 0.0, MutView{sm.ptr, 1, k}, s);
+        gemm_tn_f64(k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, 0.0, sm.ptr, k, s);
         hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * nb)), dim3(256), 0, s, sm.ptr, k * nb);
         // Z = L^-1 S
-        gemm_f64(k, nb, k, 1.0, MatView{f.linv.ptr, k, 1}, MatView{sm.ptr, 1, k}, 0.0, MutView{z.ptr, 1, k}, s);
+        gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s);
         if (err_out)
             hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
                                w + start, nb, acc.ptr);

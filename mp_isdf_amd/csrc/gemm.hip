@@ -4,11 +4,8 @@
 // (element (i, j) of a view at p[i * rs + j * cs]).  B may be FP32 (sketch
 // matrices) and is widened exactly on load.  128 x 128 output tiles per
 // 256-thread workgroup, four waves of 64 x 64 (4 x 4 MFMA tiles), K staged
-// through LDS in slices of 16.
-//
-// The same tile loop also serves the QRCP fold: instead of storing W = Q^T Y
-// the epilogue adds sum_rows W^2 to per-column accumulators (and optionally
-// sum_cols W^2 to per-row energies) with FP64 atomics.
+// through LDS in slices of 16.  Products whose operands are both contiguous
+// along K go through the TN kernel (gemm_tn.hip) instead.
 //
 // Operand lane layout of the 16x16x4 f64 MFMA: A[l % 16][l / 16],
 // B[l / 16][l % 16]; result element i of lane l is D[l / 16 + 4 i][l % 16].
@@ -23,7 +20,7 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int BM = 128, BN = 128, BK = 16;
 
-enum Epilogue { kStore = 0, kFoldColumns = 1, kFoldBoth = 2 };
+enum Epilogue { kStore = 0 };
 
 template <class TB, int EPI>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(int64_t M, int64_t N, int64_t K, double alpha,
@@ -104,48 +101,6 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int64_t M, int64_t N, in
                         *c = alpha * acc[mi][ni][i] + (beta == 0.0 ? 0.0 : beta * *c);
                     }
                 }
-    } else {
-        // Column sums of squares: rows of this tile -> one atomic per column.
-        __shared__ double colpart[2][BN];
-#pragma unroll
-        for (int ni = 0; ni < 4; ++ni) {
-            double s = 0;
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int64_t gm = m0 + wm * 64 + mi * 16 + (lane >> 4) + 4 * i;
-                    const double w = gm < M ? acc[mi][ni][i] : 0.0;
-                    s += w * w;
-                }
-            s += __shfl_xor(s, 16);
-            s += __shfl_xor(s, 32);
-            if (lane < 16) colpart[wm][wn * 64 + ni * 16 + lane] = s;
-        }
-        __syncthreads();
-        if (tid < BN) {
-            const int64_t gn = n0 + tid;
-            if (gn < N) atomicAdd(&colacc[gn], colpart[0][tid] + colpart[1][tid]);
-        }
-        if (EPI == kFoldBoth) {
-            // Row energies: lanes sharing a row are l % 16 = 0..15 within a quarter.
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    double s = 0;
-#pragma unroll
-                    for (int ni = 0; ni < 4; ++ni) {
-                        const int64_t gn = n0 + wn * 64 + ni * 16 + (lane & 15);
-                        const double w = gn < N ? acc[mi][ni][i] : 0.0;
-                        s += w * w;
-                    }
-#pragma unroll
-                    for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
-                    const int64_t gm = m0 + wm * 64 + mi * 16 + (lane >> 4) + 4 * i;
-                    if ((lane & 15) == 0 && gm < M) atomicAdd(&rowacc[gm], s);
-                }
-        }
     }
 }
 
@@ -174,21 +129,6 @@ void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, con
                    int64_t bcs, double beta, MutView C, hipStream_t stream)
 {
     launch<float, kStore>(M, N, K, alpha, A, B, brs, bcs, beta, C, nullptr, nullptr, stream);
-}
-
-void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, bool y_f32, int64_t ldy,
-                      double* proj2, double* energy, hipStream_t stream)
-{
-    // W (b x n) = Qb^T (b x m) Y (m x n); proj2[j] += sum_t W[t,j]^2; energy[t] += sum_j W[t,j]^2
-    const MatView A{Qb, m, 1};
-    const MutView none{nullptr, 0, 0};
-    if (y_f32) {
-        if (energy) launch<float, kFoldBoth>(b, n, m, 1.0, A, static_cast<const float*>(Y), 1, ldy, 0.0, none, proj2, energy, stream);
-        else launch<float, kFoldColumns>(b, n, m, 1.0, A, static_cast<const float*>(Y), 1, ldy, 0.0, none, proj2, nullptr, stream);
-    } else {
-        if (energy) launch<double, kFoldBoth>(b, n, m, 1.0, A, static_cast<const double*>(Y), 1, ldy, 0.0, none, proj2, energy, stream);
-        else launch<double, kFoldColumns>(b, n, m, 1.0, A, static_cast<const double*>(Y), 1, ldy, 0.0, none, proj2, nullptr, stream);
-    }
 }
 
 }  // namespace mpisdf

@@ -1,0 +1,262 @@
+// TN GEMM on FP64 MFMA with both operands K-contiguous:
+//     W[t, j] = sum_k A[k + t * lda] * B[k + j * ldb]      (t < M, j < N, k < K)
+// A is FP64 (the QRCP's orthonormal columns), B is FP32 or FP64 (sketch
+// columns, candidate residuals).  Used for the QRCP fold (W consumed by a
+// sum-of-squares epilogue, never stored) and for the projections H = Q^T R
+// of the candidate block.
+//
+// v_mfma_f64_16x16x4_f64 takes A[l % 16][k(l)] and B[k(l)][l % 16] from lane
+// l, where the four K values of one step are spread over the lane quarters.
+// The reduction over K does not care which K index a quarter supplies, so
+// quarter g feeds K = 4 g + s at step s of a 16-wide K chunk: every lane then
+// consumes four consecutive K values of one row / column, which it loads as a
+// single 16- or 32-byte vector.  Result element i of lane l is
+// D[l / 16 + 4 i][l % 16].
+//
+// Workgroup: 4 waves, 128 columns (32 per wave = 2 column tiles) x up to 128
+// rows (8 row tiles, grid.y walks further row groups).  The A slice of 128 rows
+// x 32 K is staged in double-buffered LDS and shared by the four waves; B goes
+// straight from HBM to registers, one slice ahead.
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace mpisdf {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int TBM = 128;       // rows per workgroup (row group)
+constexpr int TBN = 128;       // columns per workgroup
+constexpr int TBK = 32;        // K per slice
+constexpr int LDA_S = TBK + 2; // LDS row pitch (doubles): 68 dwords, conflict-free b128 reads
+
+enum TnEpilogue { kTnStore = 0, kTnFold = 1, kTnFoldEnergy = 2 };
+
+template <class TB>
+struct BVec;
+template <>
+struct BVec<float> {
+    typedef float4 type;
+};
+template <>
+struct BVec<double> {
+    typedef double4 type;
+};
+
+template <class TB, bool VEC, class TO>
+__device__ __forceinline__ void load4(const TB* __restrict__ p, int64_t k, int64_t K, bool col_ok, TO out[4])
+{
+    if (VEC) {
+        if (col_ok && k + 4 <= K) {
+            const typename BVec<TB>::type v = *reinterpret_cast<const typename BVec<TB>::type*>(p + k);
+            out[0] = v.x;
+            out[1] = v.y;
+            out[2] = v.z;
+            out[3] = v.w;
+            return;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) out[s] = (col_ok && k + s < K) ? p[k + s] : TB(0);
+}
+
+template <class TB, int EPI, bool VEC>
+__global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, int64_t K,
+                                                          const double* __restrict__ A, int64_t lda,
+                                                          const TB* __restrict__ B, int64_t ldb, double alpha,
+                                                          double beta, double* __restrict__ C, int64_t ldc,
+                                                          double* __restrict__ colacc, double* __restrict__ rowacc)
+{
+    __shared__ __align__(16) double sA[2][TBM * LDA_S];
+    __shared__ double red[4][TBM];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int quarter = lane >> 4, l16 = lane & 15;
+    const int64_t n0 = (int64_t)blockIdx.x * TBN + wave * 32;
+    const int64_t m0 = (int64_t)blockIdx.y * TBM;
+    const int R = (int)std::min<int64_t>(8, (M - m0 + 15) / 16);  // row tiles in this group (uniform)
+
+    d4 acc[8][2];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[t][c] = d4{0.0, 0.0, 0.0, 0.0};
+
+    // A staging: 128 rows x 32 K = 1024 four-wide chunks, four per thread.
+    double qa[4][4];
+    auto load_a = [&](int64_t k0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = tid + 256 * r;
+            const int row = c >> 3, kc = (c & 7) * 4;
+            const int64_t gm = m0 + row;
+            load4<double, VEC, double>(A + gm * lda, k0 + kc, K, gm < M, qa[r]);
+        }
+    };
+    auto stash_a = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int c = tid + 256 * r;
+            const int row = c >> 3, kc = (c & 7) * 4;
+            double* dst = &sA[buf][row * LDA_S + kc];
+            *reinterpret_cast<double2*>(dst) = make_double2(qa[r][0], qa[r][1]);
+            *reinterpret_cast<double2*>(dst + 2) = make_double2(qa[r][2], qa[r][3]);
+        }
+    };
+    // B: per column tile and 16-K half, four consecutive K of column l16.
+    TB yb[2][2][4], yn[2][2][4];  // raw B values, widened at use
+    const bool ok0 = n0 + l16 < N, ok1 = n0 + 16 + l16 < N;
+    const TB* b0 = B + (ok0 ? (n0 + l16) : 0) * ldb;
+    const TB* b1 = B + (ok1 ? (n0 + 16 + l16) : 0) * ldb;
+    auto load_b = [&](int64_t k0, TB dst[2][2][4]) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int64_t k = k0 + 16 * h + 4 * quarter;
+            load4<TB, VEC, TB>(b0, k, K, ok0, dst[0][h]);
+            load4<TB, VEC, TB>(b1, k, K, ok1, dst[1][h]);
+        }
+    };
+
+    const int64_t nslices = (K + TBK - 1) / TBK;
+    if (nslices > 0) {
+        load_a(0);
+        load_b(0, yb);
+        stash_a(0);
+    }
+    __syncthreads();
+    for (int64_t sl = 0; sl < nslices; ++sl) {
+        const int buf = (int)(sl & 1);
+        const bool more = sl + 1 < nslices;
+        if (more) {
+            load_a((sl + 1) * TBK);
+            load_b((sl + 1) * TBK, yn);
+        }
+        const double* As = sA[buf];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                if (t < R) {
+                    const double* ap = As + (t * 16 + l16) * LDA_S + 16 * h + 4 * quarter;
+                    const double2 a01 = *reinterpret_cast<const double2*>(ap);
+                    const double2 a23 = *reinterpret_cast<const double2*>(ap + 2);
+                    const double av[4] = {a01.x, a01.y, a23.x, a23.y};
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        acc[t][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], (double)yb[0][h][s], acc[t][0], 0, 0, 0);
+                        acc[t][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[s], (double)yb[1][h][s], acc[t][1], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        if (more) {
+            stash_a(buf ^ 1);
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) yb[c][h][s] = yn[c][h][s];
+        }
+        __syncthreads();
+    }
+
+    if (EPI == kTnStore) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            if (t >= R) continue;
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int64_t gn = n0 + 16 * c + l16;
+                if (gn >= N) continue;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t gm = m0 + t * 16 + quarter + 4 * i;
+                    if (gm < M) {
+                        double* p = C + gm + gn * ldc;
+                        *p = alpha * acc[t][c][i] + (beta == 0.0 ? 0.0 : beta * *p);
+                    }
+                }
+            }
+        }
+        return;
+    }
+    // Column sums of W^2 over this row group: each column lives in one wave.
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        double s = 0;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) s += acc[t][c][i] * acc[t][c][i];  // rows >= M hold zeros
+        s += __shfl_xor(s, 16);
+        s += __shfl_xor(s, 32);
+        const int64_t gn = n0 + 16 * c + l16;
+        if (quarter == 0 && gn < N) {
+            if (gridDim.y == 1) colacc[gn] += s;
+            else atomicAdd(&colacc[gn], s);
+        }
+    }
+    if (EPI == kTnFoldEnergy) {
+        // Row sums of W^2 over the workgroup's columns: lanes of one quarter
+        // share rows; reduce over l16 and the two column tiles, then the waves.
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                double s = acc[t][0][i] * acc[t][0][i] + acc[t][1][i] * acc[t][1][i];
+#pragma unroll
+                for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
+                if (l16 == 0) red[wave][t * 16 + quarter + 4 * i] = s;
+            }
+        __syncthreads();
+        if (tid < TBM) {
+            const int64_t gm = m0 + tid;
+            const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+            if (gm < M && tid < R * 16) atomicAdd(&rowacc[gm], s);
+        }
+    }
+}
+
+template <class TB, int EPI>
+void launch_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const TB* B, int64_t ldb, double alpha,
+               double beta, double* C, int64_t ldc, double* colacc, double* rowacc, hipStream_t stream)
+{
+    if (M <= 0 || N <= 0) return;
+    const int64_t gx = (N + TBN - 1) / TBN, gy = (M + TBM - 1) / TBM;
+    if (gx > 0x7fffffff || gy > 65535) throw std::overflow_error("TN GEMM grid too large");
+    const bool vec = lda % 4 == 0 && ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
+                     (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB))) == 0;
+    if (vec)
+        hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
+                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc);
+    else
+        hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, false>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
+                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc);
+    MPISDF_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
+                 int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream)
+{
+    launch_tn<double, kTnStore>(M, N, K, A, lda, B, ldb, alpha, beta, C, ldc, nullptr, nullptr, stream);
+}
+
+void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, bool y_f32, int64_t ldy,
+                      double* proj2, double* energy, hipStream_t stream)
+{
+    // W (b x n) = Qb^T Y; proj2[j] += sum_t W[t,j]^2; energy[t] += sum_j W[t,j]^2
+    if (y_f32) {
+        const float* y = static_cast<const float*>(Y);
+        if (energy) launch_tn<float, kTnFoldEnergy>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
+        else launch_tn<float, kTnFold>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
+    } else {
+        const double* y = static_cast<const double*>(Y);
+        if (energy) launch_tn<double, kTnFoldEnergy>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
+        else launch_tn<double, kTnFold>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
+    }
+}
+
+}  // namespace mpisdf

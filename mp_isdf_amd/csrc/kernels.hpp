@@ -40,6 +40,11 @@ void gemm_f64(int64_t M, int64_t N, int64_t K, double alpha, MatView A, MatView 
 void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, const float* B,
                    int64_t brs, int64_t bcs, double beta, MutView C, hipStream_t stream);
 
+// TN product with both operands contiguous along K (gemm_tn.hip):
+// C[t + j * ldc] = alpha * sum_k A[k + t * lda] B[k + j * ldb] + beta * C[...].
+void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
+                 int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream);
+
 // QRCP fold: W = Qb^T Y (Qb: m x b column-major, Y: m x n with leading
 // dimension ldy, FP32 or FP64); proj2[j] += sum_t W[t,j]^2 and, when energy
 // is given, energy[t] += sum_j W[t,j]^2.  FP64 MFMA, W never stored.

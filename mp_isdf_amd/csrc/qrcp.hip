@@ -345,7 +345,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         if (k > 0) {
             if (H.count < (size_t)(k * Ceff)) H.resize((size_t)(k * Ceff));
             for (int pass = 0; pass < 2; ++pass) {
-                gemm_f64(k, Ceff, m, 1.0, MatView{Q.ptr, m, 1}, MatView{rc.ptr, 1, m}, 0.0, MutView{H.ptr, 1, k}, s);
+                gemm_tn_f64(k, Ceff, m, 1.0, Q.ptr, m, rc.ptr, m, 0.0, H.ptr, k, s);
                 gemm_f64(m, Ceff, k, -1.0, MatView{Q.ptr, 1, m}, MatView{H.ptr, 1, k}, 1.0, MutView{rc.ptr, 1, m}, s);
             }
         }
@@ -432,7 +432,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                                    static_cast<const float*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);
             if (k > 0) {
                 for (int pass = 0; pass < 2; ++pass) {
-                    gemm_f64(k, cnt, m, 1.0, MatView{Q.ptr, m, 1}, MatView{R.ptr, 1, m}, 0.0, MutView{Wt.ptr, 1, k}, s);
+                    gemm_tn_f64(k, cnt, m, 1.0, Q.ptr, m, R.ptr, m, 0.0, Wt.ptr, k, s);
                     gemm_f64(m, cnt, k, -1.0, MatView{Q.ptr, 1, m}, MatView{Wt.ptr, 1, k}, 1.0, MutView{R.ptr, 1, m}, s);
                 }
             }
