@@ -4,17 +4,15 @@
 //
 // (the reference's id_factored_sketch, cpp/src/anchor_select_randomized_id.cpp:299-309,
 // with scale_g = sqrt(|w_g| / ell) times the inverse input scales).  Both
-// projections share the phi tile, so one workgroup computes a 128 (s) x 128
-// (g) block of U.phi and V.phi with two accumulator sets and multiplies them
-// in the epilogue; Y is written once in FP32 and the FP64 column norms
-// ||y_g||^2 needed by the QRCP are accumulated on the way out.
+// projections share the phi tile, so one workgroup computes U.phi and V.phi
+// with two accumulator sets and multiplies them in the epilogue; Y is written
+// once in FP32 and the FP64 column norms ||y_g||^2 needed by the QRCP are
+// accumulated on the way out.
 //
-// Operands are K-contiguous ("NT"): Ut[s * d + a], Vt[s * d + a] and
-// phi[g * ldphi + a], staged through LDS in K slices.
-//   * BF16: v_mfma_f32_32x32x16_bf16, 4 waves as 2 x 2, each 64 x 64.
-//   * FP8 e4m3 (OCP): v_mfma_scale_f32_16x16x128_f8f6f4 with unit block
-//     scales; phi rows carry a per-point power-of-two scale chosen on
-//     quantisation, Ut/Vt a global 2^5.
+// Operands are K-contiguous: Ut[s * d + a], Vt[s * d + a] and
+// phi[g * ldphi + a].  BF16 and FP8 e4m3 (OCP) inputs, FP32 accumulation;
+// FP8 phi rows carry a per-point power-of-two scale chosen on quantisation,
+// Ut/Vt a global 2^5.
 #include "sketch_mfma.hpp"
 
 #include <algorithm>
@@ -28,7 +26,6 @@ typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef int i8v __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 128, BN = 128;
 
 // ---------------------------------------------------------------------------
 // Epilogue helpers
@@ -45,257 +42,204 @@ __device__ __forceinline__ void store_column_group(float* __restrict__ y, int64_
     }
 }
 
+// Sketch kernel (BF16 and FP8)
+//
+// Workgroup: 256 Omega rows (s) x 128 grid points (g); wave w owns rows
+// 64 w .. 64 w + 63 for both U and V and all 128 points, i.e. 2 x 4 MFMA
+// tiles of 32 x 32 per operand (256 FP32 accumulators per lane).  K advances
+// in slices of 128 bytes (64 BF16 / 128 FP8 values).
+//
+// Both MFMA operands are read K-contiguously.  Lane l feeds row / column
+// l % 32; the K order inside a slice is permuted so that each lane consumes
+// the 64 consecutive bytes at offset 64 (l / 32) of the slice (the reduction
+// is indifferent to the order, as long as A and B agree).  The U / V
+// fragments therefore go straight from memory into registers, one slice
+// ahead; only the phi slice, shared by the four waves, is staged in
+// double-buffered LDS with a 144-byte row pitch (conflict-free 16-byte reads).
+//   BF16: v_mfma_f32_32x32x16_bf16, 4 steps of 16 bytes per lane per slice.
+//   FP8:  v_mfma_scale_f32_32x32x64_f8f6f4 (e4m3, unit block scales),
+//         2 steps of 32 bytes per lane per slice.
+// Result element i of lane l: D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32].
 // ---------------------------------------------------------------------------
-// BF16 kernel
-// ---------------------------------------------------------------------------
-constexpr int BK16 = 32;
-constexpr int LDK16 = BK16 + 8;  // padded LDS row (bf16 elements)
+constexpr int SK_S = 256;        // Omega rows per workgroup
+constexpr int SK_G = 128;        // grid points per workgroup
+constexpr int SK_SLICE = 128;    // K bytes per slice
+constexpr int SK_PITCH = 144;    // LDS row pitch of the phi slice (bytes)
+// Word 3 of a raw buffer descriptor on gfx9-class GPUs (32-bit data format).
+constexpr int kBufferWord3 = 0x00020000;
 
-__global__ __launch_bounds__(256) void kr_sketch_bf16_kernel(
-    const __bf16* __restrict__ ut, const __bf16* __restrict__ vt, int64_t ell, int64_t d,
-    const __bf16* __restrict__ phi, int64_t ldphi, int64_t count, const float* __restrict__ scale,
+template <int FMT>  // 0: BF16, 1: FP8 e4m3
+__global__ __launch_bounds__(256) void kr_sketch_kernel(
+    const uint8_t* __restrict__ ut, const uint8_t* __restrict__ vt, int64_t ell, int64_t dbytes,
+    const uint8_t* __restrict__ phi, int64_t ldphi_bytes, int64_t count, const float* __restrict__ scale,
     float* __restrict__ y, int64_t ldy, double* __restrict__ ynorm2, unsigned long long* __restrict__ bad)
 {
-    __shared__ __align__(16) __bf16 sU[BM * LDK16];
-    __shared__ __align__(16) __bf16 sV[BM * LDK16];
-    __shared__ __align__(16) __bf16 sP[BN * LDK16];
+    __shared__ __align__(16) uint8_t sP[2][SK_G * SK_PITCH];
+    __shared__ double colred[4][SK_G];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int64_t s0 = (int64_t)blockIdx.y * BM, g0 = (int64_t)blockIdx.x * BN;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int64_t s0 = (int64_t)blockIdx.y * SK_S + wave * 64, g0 = (int64_t)blockIdx.x * SK_G;
 
-    f16v accU[2][2], accV[2][2];
+    f16v accU[2][4], accV[2][4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int gt = 0; gt < 4; ++gt)
 #pragma unroll
-            for (int e = 0; e < 16; ++e) { accU[i][j][e] = 0.f; accV[i][j][e] = 0.f; }
+            for (int e = 0; e < 16; ++e) { accU[t][gt][e] = 0.f; accV[t][gt][e] = 0.f; }
 
-    // Each thread moves 2 chunks of 8 bf16 per operand per K slice.
-    uint4 ru[2], rv[2], rp[2];
-    auto load = [&](int64_t k0) {
+    // Raw buffer loads: an offset past the descriptor's range returns zeros,
+    // which pads rows past ell / count and K past d without branches.  The
+    // host guarantees every offset below fits in 32 bits.
+    const uint32_t db = (uint32_t)dbytes;
+    const __amdgpu_buffer_rsrc_t ursrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(ut), (short)0, (int)(ell * dbytes), kBufferWord3);
+    const __amdgpu_buffer_rsrc_t vrsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(vt), (short)0, (int)(ell * dbytes), kBufferWord3);
+    const int64_t prows = std::min<int64_t>(SK_G, count - g0);
+    const __amdgpu_buffer_rsrc_t prsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint8_t*>(phi + g0 * ldphi_bytes), (short)0, (int)(prows * ldphi_bytes), kBufferWord3);
+    uint32_t uoff[2];
 #pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int c = tid + 256 * r;
-            const int row = c >> 2, kc = (c & 3) * 8;
-            const int64_t k = k0 + kc;
-            const bool kin = k + 8 <= d;
-            const int64_t s = s0 + row, g = g0 + row;
-            uint4 zero = make_uint4(0, 0, 0, 0);
-            if (kin) {
-                ru[r] = s < ell ? *reinterpret_cast<const uint4*>(ut + s * d + k) : zero;
-                rv[r] = s < ell ? *reinterpret_cast<const uint4*>(vt + s * d + k) : zero;
-                rp[r] = g < count ? *reinterpret_cast<const uint4*>(phi + g * ldphi + k) : zero;
-            } else {
-                // K tail: element-wise with zero fill.
-                __bf16 tu[8], tv[8], tp[8];
-                for (int e = 0; e < 8; ++e) {
-                    const bool ok = k + e < d;
-                    tu[e] = (ok && s < ell) ? ut[s * d + k + e] : (__bf16)0.f;
-                    tv[e] = (ok && s < ell) ? vt[s * d + k + e] : (__bf16)0.f;
-                    tp[e] = (ok && g < count) ? phi[g * ldphi + k + e] : (__bf16)0.f;
-                }
-                ru[r] = *reinterpret_cast<uint4*>(tu);
-                rv[r] = *reinterpret_cast<uint4*>(tv);
-                rp[r] = *reinterpret_cast<uint4*>(tp);
-            }
-        }
-    };
-    auto stash = [&]() {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-            const int c = tid + 256 * r;
-            const int row = c >> 2, kc = (c & 3) * 8;
-            *reinterpret_cast<uint4*>(sU + row * LDK16 + kc) = ru[r];
-            *reinterpret_cast<uint4*>(sV + row * LDK16 + kc) = rv[r];
-            *reinterpret_cast<uint4*>(sP + row * LDK16 + kc) = rp[r];
-        }
-    };
+    for (int t = 0; t < 2; ++t) uoff[t] = (uint32_t)(s0 + 32 * t + l32) * db;  // >= range for rows past ell
+    // phi chunks staged by this thread: chunk c = tid + 256 r -> row c / 8, byte 16 (c % 8).
+    const int prow = tid >> 3, pkc = (tid & 7) * 16;
 
-    load(0);
-    for (int64_t k0 = 0; k0 < d; k0 += BK16) {
-        __syncthreads();
-        stash();
-        __syncthreads();
-        if (k0 + BK16 < d) load(k0 + BK16);  // prefetch next slice into registers
+    // ua/va[t][c]: 16-byte chunk c of this lane's 64 bytes of the current
+    // slice.  A chunk is reloaded for the next slice right after the step that
+    // consumed it, so one register set covers both slices.
+    uint4 ua[2][4], va[2][4];
+    uint4 rp[4];
+#define MPISDF_LOAD_UV(k0, t, c)                                                              \
+    do {                                                                                     \
+        const uint32_t k_ = (uint32_t)(k0) + 64u * half + 16u * (c);                         \
+        const uint32_t off_ = k_ < db ? uoff[t] + k_ : 0x80000000u;                          \
+        ua[t][c] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(ursrc, off_, 0, 0)); \
+        va[t][c] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(vrsrc, off_, 0, 0)); \
+    } while (0)
+#define MPISDF_LOAD_P(k0)                                                                     \
+    do {                                                                                     \
+        _Pragma("unroll") for (int r_ = 0; r_ < 4; ++r_) {                                   \
+            const uint32_t k_ = (uint32_t)(k0) + (uint32_t)pkc;                              \
+            const uint32_t off_ = k_ < db ? (uint32_t)(prow + 32 * r_) * (uint32_t)ldphi_bytes + k_ : 0x80000000u; \
+            rp[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(prsrc, off_, 0, 0)); \
+        }                                                                                    \
+    } while (0)
+#define MPISDF_STASH_P(buf)                                                                   \
+    do {                                                                                     \
+        _Pragma("unroll") for (int r_ = 0; r_ < 4; ++r_)                                     \
+            *reinterpret_cast<uint4*>(&sP[buf][(prow + 32 * r_) * SK_PITCH + pkc]) = rp[r_];  \
+    } while (0)
+
+    const int64_t nsl = (dbytes + SK_SLICE - 1) / SK_SLICE;
+    if (nsl > 0) {
 #pragma unroll
-        for (int kk = 0; kk < BK16; kk += 16) {
-            bf8v a_u[2], a_v[2], b_p[2];
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int mi = 0; mi < 2; ++mi) {
-                const int row = wm * 64 + mi * 32 + (lane & 31);
-                a_u[mi] = *reinterpret_cast<const bf8v*>(sU + row * LDK16 + kk + 8 * (lane >> 5));
-                a_v[mi] = *reinterpret_cast<const bf8v*>(sV + row * LDK16 + kk + 8 * (lane >> 5));
-            }
-#pragma unroll
-            for (int ni = 0; ni < 2; ++ni) {
-                const int row = wn * 64 + ni * 32 + (lane & 31);
-                b_p[ni] = *reinterpret_cast<const bf8v*>(sP + row * LDK16 + kk + 8 * (lane >> 5));
-            }
-#pragma unroll
-            for (int mi = 0; mi < 2; ++mi)
-#pragma unroll
-                for (int ni = 0; ni < 2; ++ni) {
-                    accU[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_u[mi], b_p[ni], accU[mi][ni], 0, 0, 0);
-                    accV[mi][ni] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_v[mi], b_p[ni], accV[mi][ni], 0, 0, 0);
-                }
-        }
+            for (int c = 0; c < 4; ++c) MPISDF_LOAD_UV(0, t, c);
+        MPISDF_LOAD_P(0);
+        MPISDF_STASH_P(0);
     }
+    __syncthreads();
+    for (int64_t sl = 0; sl < nsl; ++sl) {
+        const int buf = (int)(sl & 1);
+        // Prefetches run unconditionally: past the last slice their offsets
+        // fall outside the descriptors and return zeros (no branch, so the
+        // wait counts stay exact).
+        const int64_t knext = (sl + 1) * SK_SLICE;
+        MPISDF_LOAD_P(knext);
+        __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
+        const uint8_t* P = sP[buf];
+        if (FMT == 0) {
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                bf8v b[4];
+#pragma unroll
+                for (int gt = 0; gt < 4; ++gt)
+                    b[gt] = *reinterpret_cast<const bf8v*>(P + (gt * 32 + l32) * SK_PITCH + 64 * half + 16 * st);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const bf8v au = __builtin_bit_cast(bf8v, ua[t][st]);
+                    const bf8v av = __builtin_bit_cast(bf8v, va[t][st]);
+#pragma unroll
+                    for (int gt = 0; gt < 4; ++gt) {
+                        accU[t][gt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(au, b[gt], accU[t][gt], 0, 0, 0);
+                        accV[t][gt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[gt], accV[t][gt], 0, 0, 0);
+                    }
+                    MPISDF_LOAD_UV(knext, t, st);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int st = 0; st < 2; ++st) {
+                i8v b[4];
+#pragma unroll
+                for (int gt = 0; gt < 4; ++gt) {
+                    const uint8_t* p = P + (gt * 32 + l32) * SK_PITCH + 64 * half + 32 * st;
+                    const uint4 lo = *reinterpret_cast<const uint4*>(p);
+                    const uint4 hi = *reinterpret_cast<const uint4*>(p + 16);
+                    b[gt] = i8v{(int)lo.x, (int)lo.y, (int)lo.z, (int)lo.w, (int)hi.x, (int)hi.y, (int)hi.z, (int)hi.w};
+                }
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const uint4 ul = ua[t][2 * st], uh = ua[t][2 * st + 1];
+                    const uint4 vl = va[t][2 * st], vh = va[t][2 * st + 1];
+                    const i8v au = i8v{(int)ul.x, (int)ul.y, (int)ul.z, (int)ul.w, (int)uh.x, (int)uh.y, (int)uh.z, (int)uh.w};
+                    const i8v av = i8v{(int)vl.x, (int)vl.y, (int)vl.z, (int)vl.w, (int)vh.x, (int)vh.y, (int)vh.z, (int)vh.w};
+#pragma unroll
+                    for (int gt = 0; gt < 4; ++gt) {
+                        accU[t][gt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(au, b[gt], accU[t][gt], 0, 0, 0,
+                                                                                      127, 0, 127);
+                        accV[t][gt] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, b[gt], accV[t][gt], 0, 0, 0,
+                                                                                      127, 0, 127);
+                    }
+                    MPISDF_LOAD_UV(knext, t, 2 * st);
+                    MPISDF_LOAD_UV(knext, t, 2 * st + 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        MPISDF_STASH_P(buf ^ 1);
+        __syncthreads();
+    }
+#undef MPISDF_LOAD_UV
+#undef MPISDF_LOAD_P
+#undef MPISDF_STASH_P
 
-    // Epilogue: D[8(i/4) + 4(lane/32) + i%4][lane%32] per 32x32 tile.
+    // Epilogue: y = scale_g * (U phi) * (V phi), FP32 stores of four
+    // consecutive s per lane; column sums of y^2 reduced over the workgroup.
     unsigned long long nonfinite = 0;
 #pragma unroll
-    for (int ni = 0; ni < 2; ++ni) {
-        const int64_t g = g0 + wn * 64 + ni * 32 + (lane & 31);
+    for (int gt = 0; gt < 4; ++gt) {
+        const int64_t g = g0 + gt * 32 + l32;
         const bool gin = g < count;
         const float sc = gin ? scale[g] : 0.f;
         double colsq = 0;
 #pragma unroll
-        for (int mi = 0; mi < 2; ++mi) {
+        for (int t = 0; t < 2; ++t) {
 #pragma unroll
             for (int grp = 0; grp < 4; ++grp) {
-                const int64_t s = s0 + wm * 64 + mi * 32 + 8 * grp + 4 * (lane >> 5);
+                const int64_t srow = s0 + 32 * t + 8 * grp + 4 * half;
                 float v[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    v[e] = sc * accU[mi][ni][4 * grp + e] * accV[mi][ni][4 * grp + e];
-                    if (s + e < ell && gin) {
+                    v[e] = sc * accU[t][gt][4 * grp + e] * accV[t][gt][4 * grp + e];
+                    if (srow + e < ell && gin) {
                         colsq += (double)v[e] * (double)v[e];
                         nonfinite += !isfinite(v[e]);
                     }
                 }
-                if (gin) store_column_group(y, ldy, ell, s, g, v);
+                if (gin && srow < ell) store_column_group(y, ldy, ell, srow, g, v);
             }
         }
         colsq += __shfl_xor(colsq, 32);
-        if (gin && lane < 32) atomicAdd(&ynorm2[g], colsq);
+        if (half == 0) colred[wave][gt * 32 + l32] = colsq;
     }
-    if (nonfinite) atomicAdd(bad, nonfinite);
-}
-
-// ---------------------------------------------------------------------------
-// FP8 (e4m3) kernel: 16x16x128 MFMA, per wave 64 x 64 = 4 x 4 tiles.
-// ---------------------------------------------------------------------------
-constexpr int BK8 = 128;
-constexpr int LDK8 = BK8 + 16;  // padded LDS row (bytes)
-
-__global__ __launch_bounds__(256) void kr_sketch_fp8_kernel(
-    const uint8_t* __restrict__ ut, const uint8_t* __restrict__ vt, int64_t ell, int64_t d,
-    const uint8_t* __restrict__ phi, int64_t ldphi, int64_t count, const float* __restrict__ scale,
-    float* __restrict__ y, int64_t ldy, double* __restrict__ ynorm2, unsigned long long* __restrict__ bad)
-{
-    __shared__ __align__(16) uint8_t sU[BM * LDK8];
-    __shared__ __align__(16) uint8_t sV[BM * LDK8];
-    __shared__ __align__(16) uint8_t sP[BN * LDK8];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int64_t s0 = (int64_t)blockIdx.y * BM, g0 = (int64_t)blockIdx.x * BN;
-
-    f4 accU[4][4], accV[4][4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) { accU[i][j][e] = 0.f; accV[i][j][e] = 0.f; }
-
-    // Tile = 128 rows x 128 bytes per operand = 1024 chunks of 16 B; 4 per thread.
-    uint4 ru[4], rv[4], rp[4];
-    auto load = [&](int64_t k0) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int c = tid + 256 * r;
-            const int row = c >> 3, kc = (c & 7) * 16;
-            const int64_t k = k0 + kc;
-            const int64_t s = s0 + row, g = g0 + row;
-            const uint4 zero = make_uint4(0, 0, 0, 0);
-            if (k + 16 <= d) {
-                ru[r] = s < ell ? *reinterpret_cast<const uint4*>(ut + s * d + k) : zero;
-                rv[r] = s < ell ? *reinterpret_cast<const uint4*>(vt + s * d + k) : zero;
-                rp[r] = g < count ? *reinterpret_cast<const uint4*>(phi + g * ldphi + k) : zero;
-            } else {
-                uint8_t tu[16], tv[16], tp[16];
-                for (int e = 0; e < 16; ++e) {
-                    const bool ok = k + e < d;
-                    tu[e] = (ok && s < ell) ? ut[s * d + k + e] : 0;
-                    tv[e] = (ok && s < ell) ? vt[s * d + k + e] : 0;
-                    tp[e] = (ok && g < count) ? phi[g * ldphi + k + e] : 0;
-                }
-                ru[r] = *reinterpret_cast<uint4*>(tu);
-                rv[r] = *reinterpret_cast<uint4*>(tv);
-                rp[r] = *reinterpret_cast<uint4*>(tp);
-            }
-        }
-    };
-    auto stash = [&]() {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int c = tid + 256 * r;
-            const int row = c >> 3, kc = (c & 7) * 16;
-            *reinterpret_cast<uint4*>(sU + row * LDK8 + kc) = ru[r];
-            *reinterpret_cast<uint4*>(sV + row * LDK8 + kc) = rv[r];
-            *reinterpret_cast<uint4*>(sP + row * LDK8 + kc) = rp[r];
-        }
-    };
-    // Lane's 32 K bytes: 32 * (lane / 16) .. +31, read as two 16-byte halves.
-    auto frag = [&](const uint8_t* base, int row) {
-        const uint8_t* p = base + row * LDK8 + 32 * (lane >> 4);
-        const uint4 lo = *reinterpret_cast<const uint4*>(p);
-        const uint4 hi = *reinterpret_cast<const uint4*>(p + 16);
-        i8v v;
-        v[0] = (int)lo.x; v[1] = (int)lo.y; v[2] = (int)lo.z; v[3] = (int)lo.w;
-        v[4] = (int)hi.x; v[5] = (int)hi.y; v[6] = (int)hi.z; v[7] = (int)hi.w;
-        return v;
-    };
-
-    load(0);
-    for (int64_t k0 = 0; k0 < d; k0 += BK8) {
-        __syncthreads();
-        stash();
-        __syncthreads();
-        if (k0 + BK8 < d) load(k0 + BK8);
-        i8v b_p[4];
-#pragma unroll
-        for (int ni = 0; ni < 4; ++ni) b_p[ni] = frag(sP, wn * 64 + ni * 16 + (lane & 15));
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi) {
-            const i8v a_u = frag(sU, wm * 64 + mi * 16 + (lane & 15));
-            const i8v a_v = frag(sV, wm * 64 + mi * 16 + (lane & 15));
-#pragma unroll
-            for (int ni = 0; ni < 4; ++ni) {
-                accU[mi][ni] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a_u, b_p[ni], accU[mi][ni], 0, 0, 0, 127,
-                                                                                 0, 127);
-                accV[mi][ni] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a_v, b_p[ni], accV[mi][ni], 0, 0, 0, 127,
-                                                                                 0, 127);
-            }
-        }
-    }
-
-    // Epilogue: D[4(lane/16) + e][lane%16] per 16x16 tile.
-    unsigned long long nonfinite = 0;
-#pragma unroll
-    for (int ni = 0; ni < 4; ++ni) {
-        const int64_t g = g0 + wn * 64 + ni * 16 + (lane & 15);
-        const bool gin = g < count;
-        const float sc = gin ? scale[g] : 0.f;
-        double colsq = 0;
-#pragma unroll
-        for (int mi = 0; mi < 4; ++mi) {
-            const int64_t s = s0 + wm * 64 + mi * 16 + 4 * (lane >> 4);
-            float v[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                v[e] = sc * accU[mi][ni][e] * accV[mi][ni][e];
-                if (s + e < ell && gin) {
-                    colsq += (double)v[e] * (double)v[e];
-                    nonfinite += !isfinite(v[e]);
-                }
-            }
-            if (gin) store_column_group(y, ldy, ell, s, g, v);
-        }
-        colsq += __shfl_xor(colsq, 16);
-        colsq += __shfl_xor(colsq, 32);
-        if (gin && lane < 16) atomicAdd(&ynorm2[g], colsq);
+    __syncthreads();
+    if (tid < SK_G) {
+        const int64_t g = g0 + tid;
+        if (g < count) atomicAdd(&ynorm2[g], colred[0][tid] + colred[1][tid] + colred[2][tid] + colred[3][tid]);
     }
     if (nonfinite) atomicAdd(bad, nonfinite);
 }
@@ -441,18 +385,24 @@ void kr_sketch_lowp(DType dtype, const void* ut, const void* vt, int64_t ell, in
                     hipStream_t s)
 {
     if (count <= 0) return;
-    const int64_t gx = (count + BN - 1) / BN, gy = (ell + BM - 1) / BM;
+    const int64_t gx = (count + SK_G - 1) / SK_G, gy = (ell + SK_S - 1) / SK_S;
     if (gx > 0x7fffffff || gy > 65535) throw std::overflow_error("sketch grid too large");
+    // Buffer offsets inside the kernel are 32-bit: Omega rows (padded to the
+    // 256-row tile) and one 128-point phi tile must stay below 2^31 bytes.
+    const int64_t dbytes = dtype == kBF16 ? 2 * d : d, pbytes = dtype == kBF16 ? 2 * ldphi : ldphi;
+    if ((ell + SK_S) * dbytes >= (int64_t(1) << 31) || SK_G * pbytes >= (int64_t(1) << 31))
+        throw std::overflow_error("sketch operands exceed the 32-bit buffer range");
+    const auto* u8 = static_cast<const uint8_t*>(ut);
+    const auto* v8 = static_cast<const uint8_t*>(vt);
+    const auto* p8 = static_cast<const uint8_t*>(phi);
     if (dtype == kBF16) {
         if (ldphi % 8 || d % 8) throw std::invalid_argument("bf16 sketch needs d and ldphi multiples of 8");
-        hipLaunchKernelGGL(kr_sketch_bf16_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s,
-                           static_cast<const __bf16*>(ut), static_cast<const __bf16*>(vt), ell, d,
-                           static_cast<const __bf16*>(phi), ldphi, count, scale, y, ldy, ynorm2, bad);
+        hipLaunchKernelGGL(kr_sketch_kernel<0>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, u8, v8, ell, 2 * d,
+                           p8, 2 * ldphi, count, scale, y, ldy, ynorm2, bad);
     } else if (dtype == kFP8) {
         if (ldphi % 16 || d % 16) throw std::invalid_argument("fp8 sketch needs d and ldphi multiples of 16");
-        hipLaunchKernelGGL(kr_sketch_fp8_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s,
-                           static_cast<const uint8_t*>(ut), static_cast<const uint8_t*>(vt), ell, d,
-                           static_cast<const uint8_t*>(phi), ldphi, count, scale, y, ldy, ynorm2, bad);
+        hipLaunchKernelGGL(kr_sketch_kernel<1>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, u8, v8, ell, d, p8,
+                           ldphi, count, scale, y, ldy, ynorm2, bad);
     } else {
         throw std::invalid_argument("reduced-precision sketch must be bf16 or fp8");
     }
