@@ -31,7 +31,7 @@ constexpr int TBN = 128;       // columns per workgroup
 constexpr int TBK = 32;        // K per slice
 constexpr int LDA_S = TBK + 2; // LDS row pitch (doubles): 68 dwords, conflict-free b128 reads
 
-enum TnEpilogue { kTnStore = 0, kTnFold = 1, kTnFoldEnergy = 2 };
+enum TnEpilogue { kTnStore = 0, kTnFold = 1, kTnFoldEnergy = 2, kTnKhatriRao = 3 };
 
 template <class TB>
 struct BVec;
@@ -66,15 +66,22 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
                                                           const double* __restrict__ A, int64_t lda,
                                                           const TB* __restrict__ B, int64_t ldb, double alpha,
                                                           double beta, double* __restrict__ C, int64_t ldc,
-                                                          double* __restrict__ colacc, double* __restrict__ rowacc)
+                                                          double* __restrict__ colacc, double* __restrict__ rowacc,
+                                                          const double* __restrict__ A2,
+                                                          const double* __restrict__ cscale,
+                                                          unsigned long long* __restrict__ bad)
 {
     __shared__ __align__(16) double sA[2][TBM * LDA_S];
     __shared__ double red[4][TBM];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int quarter = lane >> 4, l16 = lane & 15;
     const int64_t n0 = (int64_t)blockIdx.x * TBN + wave * 32;
-    const int64_t m0 = (int64_t)blockIdx.y * TBM;
-    const int R = (int)std::min<int64_t>(8, (M - m0 + 15) / 16);  // row tiles in this group (uniform)
+    // Khatri-Rao mode: the 128 A rows are 64 rows of A (U^T) stacked on the
+    // same 64 rows of A2 (V^T); row tiles t and t + 4 then hold U phi and V phi
+    // for identical (s, g) positions in every lane.
+    constexpr bool KR = EPI == kTnKhatriRao;
+    const int64_t m0 = (int64_t)blockIdx.y * (KR ? TBM / 2 : TBM);
+    const int R = KR ? 8 : (int)std::min<int64_t>(8, (M - m0 + 15) / 16);  // row tiles in this group (uniform)
 
     d4 acc[8][2];
 #pragma unroll
@@ -89,8 +96,9 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         for (int r = 0; r < 4; ++r) {
             const int c = tid + 256 * r;
             const int row = c >> 3, kc = (c & 7) * 4;
-            const int64_t gm = m0 + row;
-            load4<double, VEC, double>(A + gm * lda, k0 + kc, K, gm < M, qa[r]);
+            const int64_t gm = m0 + (KR ? (row & 63) : row);
+            const double* base = (KR && row >= 64) ? A2 : A;
+            load4<double, VEC, double>(base + gm * lda, k0 + kc, K, gm < M, qa[r]);
         }
     };
     auto stash_a = [&](int buf) {
@@ -161,6 +169,34 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         __syncthreads();
     }
 
+    if (KR) {
+        // y[g * ldc + s] = cscale[g] (U phi)[s, g] (V phi)[s, g]; colacc[g] += sum_s y^2.
+        unsigned long long nonfinite = 0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int64_t gn = n0 + 16 * c + l16;
+            const bool gin = gn < N;
+            const double sc = gin ? cscale[gn] : 0.0;
+            double s2 = 0;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int64_t gm = m0 + t * 16 + quarter + 4 * i;
+                    const double v = sc * acc[t][c][i] * acc[t + 4][c][i];
+                    if (gin && gm < M) {
+                        C[gm + gn * ldc] = v;
+                        s2 += v * v;
+                        nonfinite += !isfinite(v);
+                    }
+                }
+            s2 += __shfl_xor(s2, 16);
+            s2 += __shfl_xor(s2, 32);
+            if (quarter == 0 && gin && colacc) atomicAdd(&colacc[gn], s2);
+        }
+        if (nonfinite) atomicAdd(bad, nonfinite);
+        return;
+    }
     if (EPI == kTnStore) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -220,19 +256,22 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
 
 template <class TB, int EPI>
 void launch_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const TB* B, int64_t ldb, double alpha,
-               double beta, double* C, int64_t ldc, double* colacc, double* rowacc, hipStream_t stream)
+               double beta, double* C, int64_t ldc, double* colacc, double* rowacc, hipStream_t stream,
+               const double* A2 = nullptr, const double* cscale = nullptr, unsigned long long* bad = nullptr)
 {
     if (M <= 0 || N <= 0) return;
-    const int64_t gx = (N + TBN - 1) / TBN, gy = (M + TBM - 1) / TBM;
+    const int64_t rows = EPI == kTnKhatriRao ? TBM / 2 : TBM;
+    const int64_t gx = (N + TBN - 1) / TBN, gy = (M + rows - 1) / rows;
     if (gx > 0x7fffffff || gy > 65535) throw std::overflow_error("TN GEMM grid too large");
     const bool vec = lda % 4 == 0 && ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
+                     (reinterpret_cast<uintptr_t>(A2) % 32) == 0 &&
                      (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB))) == 0;
     if (vec)
         hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
-                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc);
+                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad);
     else
         hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, false>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
-                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc);
+                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad);
     MPISDF_HIP(hipGetLastError());
 }
 
@@ -257,6 +296,14 @@ void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, const v
         if (energy) launch_tn<double, kTnFoldEnergy>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
         else launch_tn<double, kTnFold>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
     }
+}
+
+void kr_sketch_f64_tn(const double* ut, const double* vt, int64_t ell, int64_t d, const double* phi, int64_t ldphi,
+                      int64_t count, const double* scale, double* y, int64_t ldy, double* ynorm2,
+                      unsigned long long* bad, hipStream_t stream)
+{
+    launch_tn<double, kTnKhatriRao>(ell, count, d, ut, d, phi, ldphi, 1.0, 0.0, y, ldy, ynorm2, nullptr, stream, vt,
+                                    scale, bad);
 }
 
 }  // namespace mpisdf

@@ -45,6 +45,14 @@ void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, con
 void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
                  int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream);
 
+// FP64 Khatri-Rao sketch on the TN kernel: ut/vt are ell x d row-major
+// (ut[s * d + a]), phi[g * ldphi + a];  y[g * ldy + s] = scale[g] (ut_s . phi_g)
+// (vt_s . phi_g) for g < count, ynorm2[g] += ||y_g||^2 (when given), non-finite
+// outputs counted into *bad.
+void kr_sketch_f64_tn(const double* ut, const double* vt, int64_t ell, int64_t d, const double* phi, int64_t ldphi,
+                      int64_t count, const double* scale, double* y, int64_t ldy, double* ynorm2,
+                      unsigned long long* bad, hipStream_t stream);
+
 // QRCP fold: W = Qb^T Y (Qb: m x b column-major, Y: m x n with leading
 // dimension ldy, FP32 or FP64); proj2[j] += sum_t W[t,j]^2 and, when energy
 // is given, energy[t] += sum_j W[t,j]^2.  FP64 MFMA, W never stored.

@@ -210,9 +210,11 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     const int64_t ldy32 = (ell + 3) / 4 * 4;  // 16-byte aligned FP32 columns
     DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;
+    ynorm2.resize(n);
+    MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
     if (!lowp) {
         y64.resize((size_t)ell * n);
-        const int64_t batch = std::min(n, device_batch(d, ctx.nao, 2 * ell));
+        const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
         DeviceBuffer<double> feat((size_t)batch * d);
         SketchScratch scratch;
         for (int64_t start = 0; start < n; start += batch) {
@@ -220,7 +222,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
                                  a.mo_coeff, a.nocc, feat.ptr, ws, s);
             sketch_f64(feat.ptr, count, d, u.ptr, v.ptr, ell, sw.ptr + start, y64.ptr + start * ell, ell, scratch,
-                       bad.ptr, s);
+                       bad.ptr, s, ynorm2.ptr + start);
         }
     } else {
         // Reduced precision: operands padded to a multiple of 16 features.
@@ -229,8 +231,6 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         DeviceBuffer<uint8_t> ut((size_t)ell * dp * esz), vt((size_t)ell * dp * esz);
         prepare_omega_lowp(u.ptr, v.ptr, d, ell, dp, in_type, ut.ptr, vt.ptr, s);
         y32.resize((size_t)ldy32 * n);
-        ynorm2.resize(n);
-        MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
         const bool ao_only = spec.kinds.size() == 1 && spec.kinds[0] == 0;
         const int64_t batch = std::min<int64_t>(n, 1 << 16);
         DeviceBuffer<uint8_t> phi((size_t)batch * dp * esz);
@@ -277,7 +277,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     opt.want_tail = a.max_rank == 0;
     opt.debug = a.debug;
     const auto tq = Clock::now();
-    opt.ynorm2 = lowp ? ynorm2.ptr : nullptr;
+    opt.ynorm2 = ynorm2.ptr;
     // The reduced-precision forms fold with exact FP64 products of the FP32
     // sketch: certifying greedy pivots late in the factorisation needs folded
     // norms accurate to ~1e-9 of ||y_j||^2, beyond what split-BF16 products
