@@ -294,11 +294,11 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
             throw std::invalid_argument("invalid fold arguments");
         if (y_f32 == 2) {
             mpisdf::FoldScratch scratch;
-            mpisdf::fold_projections_bf16x3(b, n, m, Qb, static_cast<const float*>(Y), ldy, proj2, energy, scratch,
+            mpisdf::fold_projections_bf16x3(b, n, m, Qb, m, static_cast<const float*>(Y), ldy, proj2, energy, scratch,
                                             static_cast<hipStream_t>(stream));
             MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
         } else {
-            mpisdf::fold_projections(b, n, m, Qb, Y, y_f32 != 0, ldy, proj2, energy, static_cast<hipStream_t>(stream));
+            mpisdf::fold_projections(b, n, m, Qb, m, Y, y_f32 != 0, ldy, proj2, energy, static_cast<hipStream_t>(stream));
         }
     });
 }

@@ -30,18 +30,19 @@ __device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo)
     lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
-// Qh/Ql[t * ldq + k] from Qb[k + t * m] (FP64, column t), zero padded.
-__global__ void split_q_kernel(const double* __restrict__ qb, int64_t m, int64_t b, int64_t ldq, int64_t bpad,
+// Qh/Ql[t * ldq + k] from Qb[k + t * ldqb] (FP64, column t), zero padded.
+__global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int64_t m, int64_t b, int64_t ldq,
+                               int64_t bpad,
                                __bf16* __restrict__ qh, __bf16* __restrict__ ql)
 {
     const int64_t total = bpad * ldq;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t t = i / ldq, k = i - t * ldq;
-        const float x = (t < b && k < m) ? (float)qb[k + t * m] : 0.f;
+        const float x = (t < b && k < m) ? (float)qb[k + t * ldqb] : 0.f;
         __bf16 h, l;
         split_bf16(x, h, l);
         // hi/lo of the FP64 value: the lo part absorbs the FP32 rounding too.
-        const double full = (t < b && k < m) ? qb[k + t * m] : 0.0;
+        const double full = (t < b && k < m) ? qb[k + t * ldqb] : 0.0;
         l = static_cast<__bf16>((float)(full - (double)static_cast<float>(h)));
         qh[i] = h;
         ql[i] = l;
@@ -235,7 +236,8 @@ __global__ __launch_bounds__(256) void fold_bf16x3_kernel(const __bf16* __restri
 
 }  // namespace
 
-void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, const float* Y, int64_t ldy,
+void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const float* Y,
+                             int64_t ldy,
                              double* proj2, double* energy, FoldScratch& scratch, hipStream_t stream,
                              const double* ynorm2, double* err2)
 {
@@ -246,7 +248,7 @@ void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, 
     if (scratch.hi.count < need) { scratch.hi.resize(need); scratch.lo.resize(need); }
     const int64_t total = bpad * ldq;
     hipLaunchKernelGGL(split_q_kernel, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)), dim3(256), 0,
-                       stream, Qb, m, b, ldq, bpad, reinterpret_cast<__bf16*>(scratch.hi.ptr),
+                       stream, Qb, ldqb, m, b, ldq, bpad, reinterpret_cast<__bf16*>(scratch.hi.ptr),
                        reinterpret_cast<__bf16*>(scratch.lo.ptr));
     MPISDF_HIP(hipGetLastError());
     const int64_t ntiles = (n + BN - 1) / BN, mtiles = bpad / BM;

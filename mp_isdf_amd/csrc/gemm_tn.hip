@@ -283,18 +283,18 @@ void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A,
     launch_tn<double, kTnStore>(M, N, K, A, lda, B, ldb, alpha, beta, C, ldc, nullptr, nullptr, stream);
 }
 
-void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, bool y_f32, int64_t ldy,
-                      double* proj2, double* energy, hipStream_t stream)
+void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldq, const void* Y, bool y_f32,
+                      int64_t ldy, double* proj2, double* energy, hipStream_t stream)
 {
     // W (b x n) = Qb^T Y; proj2[j] += sum_t W[t,j]^2; energy[t] += sum_j W[t,j]^2
     if (y_f32) {
         const float* y = static_cast<const float*>(Y);
-        if (energy) launch_tn<float, kTnFoldEnergy>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
-        else launch_tn<float, kTnFold>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
+        if (energy) launch_tn<float, kTnFoldEnergy>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
+        else launch_tn<float, kTnFold>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
     } else {
         const double* y = static_cast<const double*>(Y);
-        if (energy) launch_tn<double, kTnFoldEnergy>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
-        else launch_tn<double, kTnFold>(b, n, m, Qb, m, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
+        if (energy) launch_tn<double, kTnFoldEnergy>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
+        else launch_tn<double, kTnFold>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
     }
 }
 

@@ -53,11 +53,12 @@ void kr_sketch_f64_tn(const double* ut, const double* vt, int64_t ell, int64_t d
                       int64_t count, const double* scale, double* y, int64_t ldy, double* ynorm2,
                       unsigned long long* bad, hipStream_t stream);
 
-// QRCP fold: W = Qb^T Y (Qb: m x b column-major, Y: m x n with leading
-// dimension ldy, FP32 or FP64); proj2[j] += sum_t W[t,j]^2 and, when energy
-// is given, energy[t] += sum_j W[t,j]^2.  FP64 MFMA, W never stored.
-void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, bool y_f32, int64_t ldy,
-                      double* proj2, double* energy, hipStream_t stream);
+// QRCP fold: W = Qb^T Y (Qb: m x b column-major with leading dimension ldq,
+// Y: m x n with leading dimension ldy, FP32 or FP64); proj2[j] += sum_t
+// W[t,j]^2 and, when energy is given, energy[t] += sum_j W[t,j]^2.  FP64
+// MFMA, W never stored.
+void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldq, const void* Y, bool y_f32,
+                      int64_t ldy, double* proj2, double* energy, hipStream_t stream);
 
 // Same fold for FP32 Y with split-BF16 MFMA (hi*hi + hi*lo + lo*hi) and FP64
 // accumulation across 64-wide K chunks; error ~1e-6 ||q|| ||y_j|| per entry.
@@ -69,7 +70,8 @@ struct FoldScratch {
 constexpr double kFoldEpsW = 2e-6;
 // When ynorm2/err2 are given, err2[j] accumulates a bound on the error of the
 // folded sum: 2 eps ||y_j|| sqrt(s_j) + b eps^2 ||y_j||^2 per fold.
-void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, const float* Y, int64_t ldy,
+void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const float* Y,
+                             int64_t ldy,
                              double* proj2, double* energy, FoldScratch& scratch, hipStream_t stream,
                              const double* ynorm2 = nullptr, double* err2 = nullptr);
 

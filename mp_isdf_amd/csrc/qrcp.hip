@@ -76,22 +76,23 @@ __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __r
 
 template <class T>
 __global__ void gather_kernel(const T* __restrict__ y, int64_t ldy, int64_t m,
-                              const int64_t* __restrict__ cand, int64_t C, double* __restrict__ rc)
+                              const int64_t* __restrict__ cand, int64_t C, double* __restrict__ rc, int64_t ldr)
 {
     const int64_t c = blockIdx.y;
     if (c >= C) return;
     const T* src = y + cand[c] * ldy;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-        rc[i + c * m] = (double)src[i];
+        rc[i + c * ldr] = (double)src[i];
 }
 
-__global__ void cand_norm_kernel(const double* __restrict__ rc, int64_t m, int64_t C, double* __restrict__ rn2)
+__global__ void cand_norm_kernel(const double* __restrict__ rc, int64_t m, int64_t ldr, int64_t C,
+                                 double* __restrict__ rn2)
 {
     const int lane = threadIdx.x & 63;
     const int64_t c = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     if (c >= C) return;
     double s = 0;
-    for (int64_t i = lane; i < m; i += 64) { const double v = rc[i + c * m]; s += v * v; }
+    for (int64_t i = lane; i < m; i += 64) { const double v = rc[i + c * ldr]; s += v * v; }
     s = wave_sum(s);
     if (lane == 0) rn2[c] = s;
 }
@@ -124,22 +125,22 @@ __global__ __launch_bounds__(1024) void argmax_kernel(const double* __restrict__
 }
 
 // q <- r_c* / ||r_c*||  (the norm comes from the exact residual norm)
-__global__ void make_q_kernel(const double* __restrict__ rc, int64_t m, const ArgMax* __restrict__ am,
+__global__ void make_q_kernel(const double* __restrict__ rc, int64_t m, int64_t ldr, const ArgMax* __restrict__ am,
                               double* __restrict__ q, uint8_t* __restrict__ selected)
 {
     const double inv = 1.0 / sqrt(am->value);
-    const double* src = rc + am->slot * m;
+    const double* src = rc + am->slot * ldr;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
         q[i] = src[i] * inv;
     if (blockIdx.x == 0 && threadIdx.x == 0) selected[am->column] = 1;
 }
 
 // d[t] = Q[:, t0 + t]^T q, one workgroup per t.
-__global__ void dots_kernel(const double* __restrict__ Q, int64_t m, int64_t t0, const double* __restrict__ q,
-                            double* __restrict__ d)
+__global__ void dots_kernel(const double* __restrict__ Q, int64_t m, int64_t ldq, int64_t t0,
+                            const double* __restrict__ q, double* __restrict__ d)
 {
     __shared__ double part[4];
-    const double* col = Q + (t0 + blockIdx.x) * m;
+    const double* col = Q + (t0 + blockIdx.x) * ldq;
     double s = 0;
     for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += col[i] * q[i];
     s = wave_sum(s);
@@ -149,19 +150,21 @@ __global__ void dots_kernel(const double* __restrict__ Q, int64_t m, int64_t t0,
 }
 
 // q -= sum_t d[t] Q[:, t0 + t]; then written (unnormalised) to out
-__global__ void reorth_kernel(double* __restrict__ q, const double* __restrict__ Q, int64_t m, int64_t t0,
-                              int64_t nt, const double* __restrict__ d)
+__global__ void reorth_kernel(double* __restrict__ q, const double* __restrict__ Q, int64_t m, int64_t ldq,
+                              int64_t t0, int64_t nt, const double* __restrict__ d)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         double v = q[i];
-        for (int64_t t = 0; t < nt; ++t) v -= d[t] * Q[(t0 + t) * m + i];
+        for (int64_t t = 0; t < nt; ++t) v -= d[t] * Q[(t0 + t) * ldq + i];
         q[i] = v;
     }
 }
 
-// Normalise q in one workgroup and store it as column k of Q.
+// Normalise q in one workgroup and store it as column k of Q and of the
+// row-major copy Qt (qt_k[i * ldqt] = q_i).
 __global__ __launch_bounds__(1024) void store_q_kernel(const double* __restrict__ q, int64_t m,
-                                                       double* __restrict__ qk)
+                                                       double* __restrict__ qk, double* __restrict__ qt_k,
+                                                       int64_t ldqt)
 {
     __shared__ double part[16];
     double s = 0;
@@ -176,14 +179,18 @@ __global__ __launch_bounds__(1024) void store_q_kernel(const double* __restrict_
     }
     __syncthreads();
     const double inv = 1.0 / sqrt(part[0]);
-    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) qk[i] = q[i] * inv;
+    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+        const double v = q[i] * inv;
+        qk[i] = v;
+        qt_k[i * ldqt] = v;
+    }
 }
 
 // For every candidate c: h = q^T r_c; r_c -= h q; rn2[c] = ||r_c||^2.  The
 // pivot's own slot is retired (rn2 = -1).
-__global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ rc, int64_t m, int64_t C,
-                                                          const double* __restrict__ q, double* __restrict__ rn2,
-                                                          const ArgMax* __restrict__ am)
+__global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ rc, int64_t m, int64_t ldr,
+                                                          int64_t C, const double* __restrict__ q,
+                                                          double* __restrict__ rn2, const ArgMax* __restrict__ am)
 {
     __shared__ double part[4];
     __shared__ double hsh;
@@ -193,7 +200,7 @@ __global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ r
         if (threadIdx.x == 0) rn2[c] = -1.0;
         return;
     }
-    double* col = rc + c * m;
+    double* col = rc + c * ldr;
     double s = 0;
     for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += q[i] * col[i];
     s = wave_sum(s);
@@ -258,7 +265,11 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     DeviceBuffer<double> ynorm2(n), proj2(n), err2(n), keys(n), keys_sorted(n);
     DeviceBuffer<int64_t> idx(n), idx_sorted(n);
     DeviceBuffer<uint8_t> selected(n);
-    DeviceBuffer<double> Q((size_t)m * std::max<int64_t>(kmax, 1)), rc, rn2, H, qv(m), dv;
+    // Q (column-major) and its row-major copy Qt; leading dimensions padded to
+    // four doubles so the TN products load 32-byte vectors.
+    const int64_t ldq = (m + 3) / 4 * 4, ldqt = (std::max<int64_t>(kmax, 1) + 3) / 4 * 4, ldr = ldq;
+    DeviceBuffer<double> Q((size_t)ldq * std::max<int64_t>(kmax, 1)), Qt((size_t)m * ldqt), rc, rn2, H, qv(m), dv;
+    MPISDF_HIP(hipMemsetAsync(Qt.ptr, 0, Qt.count * sizeof(double), s));
     DeviceBuffer<double> energy(std::max<int64_t>(kmax, 1)), tail(1);
     DeviceBuffer<ArgMax> am(1);
     MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
@@ -299,10 +310,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         const int64_t b = k - folded;
         if (b <= 0) return;
         if (split)
-            fold_projections_bf16x3(b, n, m, Q.ptr + folded * m, static_cast<const float*>(y), ldy, proj2.ptr,
+            fold_projections_bf16x3(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const float*>(y), ldy, proj2.ptr,
                                     energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
         else
-            fold_projections(b, n, m, Q.ptr + folded * m, y, ytype == kF32, ldy, proj2.ptr,
+            fold_projections(b, n, m, Q.ptr + folded * ldq, ldq, y, ytype == kF32, ldy, proj2.ptr,
                              energy_out ? energy_out + folded : nullptr, s);
         folded = k;
         ++res.folds;
@@ -313,7 +324,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     auto refresh = [&]() {
         MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
         MPISDF_HIP(hipMemsetAsync(err2.ptr, 0, n * sizeof(double), s));
-        if (k > 0) fold_projections(k, n, m, Q.ptr, y, ytype == kF32, ldy, proj2.ptr, nullptr, s);
+        if (k > 0) fold_projections(k, n, m, Q.ptr, ldq, y, ytype == kF32, ldy, proj2.ptr, nullptr, s);
         ++res.refreshes;
     };
     int64_t C = C0;
@@ -328,28 +339,30 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                                                                 idx.ptr, idx_sorted.ptr, (int)n, 0, 64, s));
         const int64_t remaining = n - k;
         const int64_t Ceff = std::min(C, remaining);
-        if (rc.count < (size_t)(m * Ceff)) { rc.resize((size_t)(m * Ceff)); rn2.resize(Ceff); }
+        if (rc.count < (size_t)(ldr * Ceff)) { rc.resize((size_t)(ldr * Ceff)); rn2.resize(Ceff); }
         double bound2 = -1.0;
         if (Ceff < remaining)
             MPISDF_HIP(hipMemcpyAsync(&bound2, keys_sorted.ptr + Ceff, sizeof(double), hipMemcpyDeviceToHost, s));
         if (ytype == kF64)
             hipLaunchKernelGGL(gather_kernel<double>, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
-                               static_cast<const double*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr);
+                               static_cast<const double*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr, ldr);
         else
             hipLaunchKernelGGL(gather_kernel<float>, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
-                               static_cast<const float*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr);
+                               static_cast<const float*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr, ldr);
         MPISDF_HIP(hipGetLastError());
         DeviceBuffer<int64_t> cand;
         cand.resize(Ceff);
         MPISDF_HIP(hipMemcpyAsync(cand.ptr, idx_sorted.ptr, Ceff * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
         if (k > 0) {
-            if (H.count < (size_t)(k * Ceff)) H.resize((size_t)(k * Ceff));
+            // Two passes of classical Gram-Schmidt: H = Q^T R, R -= Q H (via Qt).
+            const int64_t ldh = (k + 3) / 4 * 4;
+            if (H.count < (size_t)(ldh * Ceff)) H.resize((size_t)(ldh * Ceff));
             for (int pass = 0; pass < 2; ++pass) {
-                gemm_tn_f64(k, Ceff, m, 1.0, Q.ptr, m, rc.ptr, m, 0.0, H.ptr, k, s);
-                gemm_f64(m, Ceff, k, -1.0, MatView{Q.ptr, 1, m}, MatView{H.ptr, 1, k}, 1.0, MutView{rc.ptr, 1, m}, s);
+                gemm_tn_f64(k, Ceff, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
+                gemm_tn_f64(m, Ceff, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
             }
         }
-        hipLaunchKernelGGL(cand_norm_kernel, dim3((unsigned)((Ceff + 3) / 4)), dim3(256), 0, s, rc.ptr, m, Ceff,
+        hipLaunchKernelGGL(cand_norm_kernel, dim3((unsigned)((Ceff + 3) / 4)), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
                            rn2.ptr);
         MPISDF_HIP(hipGetLastError());
         ++res.blocks;
@@ -377,18 +390,19 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                 break;
             res.pivots.push_back(best.column);
             res.rdiag.push_back(norm);
-            hipLaunchKernelGGL(make_q_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, rc.ptr, m, am.ptr, qv.ptr,
+            hipLaunchKernelGGL(make_q_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, rc.ptr, m, ldr, am.ptr, qv.ptr,
                                selected.ptr);
             const int64_t nt = k - block_start;
             if (nt > 0) {
-                hipLaunchKernelGGL(dots_kernel, dim3((unsigned)nt), dim3(256), 0, s, Q.ptr, m, block_start, qv.ptr,
+                hipLaunchKernelGGL(dots_kernel, dim3((unsigned)nt), dim3(256), 0, s, Q.ptr, m, ldq, block_start, qv.ptr,
                                    dv.ptr);
-                hipLaunchKernelGGL(reorth_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, qv.ptr, Q.ptr, m,
+                hipLaunchKernelGGL(reorth_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, qv.ptr, Q.ptr, m, ldq,
                                    block_start, nt, dv.ptr);
             }
-            hipLaunchKernelGGL(store_q_kernel, dim3(1), dim3(1024), 0, s, qv.ptr, m, Q.ptr + k * m);
-            hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, Ceff,
-                               Q.ptr + k * m, rn2.ptr, am.ptr);
+            hipLaunchKernelGGL(store_q_kernel, dim3(1), dim3(1024), 0, s, qv.ptr, m, Q.ptr + k * ldq, Qt.ptr + k,
+                               ldqt);
+            hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
+                               Q.ptr + k * ldq, rn2.ptr, am.ptr);
             MPISDF_HIP(hipGetLastError());
             ++k;
         }
@@ -432,8 +446,9 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                                    static_cast<const float*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);
             if (k > 0) {
                 for (int pass = 0; pass < 2; ++pass) {
-                    gemm_tn_f64(k, cnt, m, 1.0, Q.ptr, m, R.ptr, m, 0.0, Wt.ptr, k, s);
-                    gemm_f64(m, cnt, k, -1.0, MatView{Q.ptr, 1, m}, MatView{Wt.ptr, 1, k}, 1.0, MutView{R.ptr, 1, m}, s);
+                    gemm_tn_f64(k, cnt, m, 1.0, Q.ptr, ldq, R.ptr, m, 0.0, Wt.ptr, k, s);
+                    gemm_f64(m, cnt, k, -1.0, MatView{Q.ptr, 1, ldq}, MatView{Wt.ptr, 1, k}, 1.0, MutView{R.ptr, 1, m},
+                             s);
                 }
             }
             hipLaunchKernelGGL(sumsq_kernel, dim3(grid_for(m * cnt, 256, 4096)), dim3(256), 0, s, R.ptr, m * cnt,

@@ -208,12 +208,13 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     DeviceBuffer<double> y64;
     DeviceBuffer<float> y32;
     const int64_t ldy32 = (ell + 3) / 4 * 4;  // 16-byte aligned FP32 columns
+    const int64_t ldy64 = (ell + 3) / 4 * 4;  // 32-byte aligned FP64 columns
     DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;
     ynorm2.resize(n);
     MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
     if (!lowp) {
-        y64.resize((size_t)ell * n);
+        y64.resize((size_t)ldy64 * n);
         const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
         DeviceBuffer<double> feat((size_t)batch * d);
         SketchScratch scratch;
@@ -221,7 +222,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             const int64_t count = std::min(batch, n - start);
             evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
                                  a.mo_coeff, a.nocc, feat.ptr, ws, s);
-            sketch_f64(feat.ptr, count, d, u.ptr, v.ptr, ell, sw.ptr + start, y64.ptr + start * ell, ell, scratch,
+            sketch_f64(feat.ptr, count, d, u.ptr, v.ptr, ell, sw.ptr + start, y64.ptr + start * ldy64, ldy64, scratch,
                        bad.ptr, s, ynorm2.ptr + start);
         }
     } else {
@@ -283,7 +284,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     // norms accurate to ~1e-9 of ||y_j||^2, beyond what split-BF16 products
     // deliver (the split fold stays available through QrcpOptions).
     const QrcpResult r = lowp ? qrcp_device(y32.ptr, kF32, ell, n, ldy32, opt, s)
-                              : qrcp_device(y64.ptr, kF64, ell, n, ell, opt, s);
+                              : qrcp_device(y64.ptr, kF64, ell, n, ldy64, opt, s);
     const double t_qrcp = seconds_since(tq);
     const int64_t K = (int64_t)r.pivots.size();
     // numerical_rank: first i with |R_ii| <= floor; the engine stops there.
