@@ -5,8 +5,10 @@ Workload (default ``--config h2o64``): seeded (H2O)_64 cluster in an 18 A box,
 cc-pVDZ (1536 spherical AOs), level-3 Becke grid (~2.2e6 points), randomized
 quadrature Khatri-Rao selection (rand_sketch) of k = 12,000 anchors with
 ell = k + 10% sketch rows, seed 7.  A step is one full selection -- AO
-evaluation on grid batches, sketch GEMMs, GPU QRCP -- with the grid
-coordinates and weights already resident in HBM.
+evaluation on grid batches, sketch GEMMs, GPU QRCP -- followed by the FP64
+fit on the chosen anchors (anchor-pair Gram, Cholesky, theta for every kept
+grid point, pair-space error), with the grid coordinates and weights already
+resident in HBM.
 
 Printed: one JSON line (metric, grid-pts/s value, per-stage seconds, the
 all-FP64 form for comparison, roofline of the dominant kernel measured with
@@ -50,19 +52,6 @@ def parse():
     return p.parse_args()
 
 
-def time_events(torch, fn, repeat=3):
-    """Average duration (s) of fn() on the current stream, HIP events."""
-    fn()
-    torch.cuda.synchronize()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    for _ in range(repeat):
-        fn()
-    stop.record()
-    torch.cuda.synchronize()
-    return start.elapsed_time(stop) / 1e3 / repeat
-
-
 def main():
     args = parse()
     if args.gpus != 1:
@@ -96,14 +85,16 @@ def main():
     xyz_dev = torch.tensor(xyz[keep], device="cuda")
     w_dev = torch.tensor(w[keep], device="cuda")
 
-    def fit(anchors, grid_pass):
-        """Fit factorisation (Gram, Cholesky, L^-1); grid_pass adds the
-        pair-space error pass over every kept grid point."""
+    def fit(anchors):
+        """ISDF fit on the kept grid: Gram, Cholesky and L^-1 of the anchor pairs,
+        then theta for every kept grid point (formed batch by batch on the GPU
+        and consumed there as theta . w) and the pair-space error."""
         ax = torch.tensor(xyz[anchors], device="cuda")
         aw = np.ascontiguousarray(w[anchors])
         res = np.zeros(8)
-        capi.call("mpisdf_isdf_fit_device", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev,
-                  n if grid_pass else 0, 8192, res, None)
+        tsum = np.zeros(len(anchors))
+        capi.call("mpisdf_isdf_fit_device", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev, n, 8192, tsum, res,
+                  None)
         return res
 
     def step(precision):
@@ -111,16 +102,24 @@ def main():
         anchors, d = sel.select_with_diagnostics(xyz, w, ao, algorithm=alg, max_rank=k, oversampling=oversampling,
                                                  seed=7, precision=precision)
         t_sel = time.perf_counter() - t
-        res = fit(anchors, grid_pass=False)
+        res = fit(anchors)
         d = dict(d)
         d["seconds_select"] = t_sel
-        d["seconds_fit"] = res[4]
+        d["seconds_fit_factor"] = res[4]
+        d["seconds_fit_grid"] = res[5]
+        d["pair_space_relative_error"] = float(res[1])
+        d["gram_condition_estimate"] = float(res[2])
         return anchors, d
 
+    def describe(i, label, t_total, d):
+        log(f"{label} {i}: {t_total:.2f} s (sketch {d['seconds_sketch']:.2f}, qrcp {d['seconds_qrcp']:.2f}, "
+            f"blocks {int(d['blocks'])}, fit factor {d['seconds_fit_factor']:.2f}, fit grid "
+            f"{d['seconds_fit_grid']:.2f}; pair-space error {d['pair_space_relative_error']:.4e})")
+
     for i in range(args.warmup):
+        t = time.perf_counter()
         _, d = step(args.precision)
-        log(f"warmup {i}: select {d['seconds_select']:.2f} s (sketch {d['seconds_sketch']:.2f}, qrcp "
-            f"{d['seconds_qrcp']:.2f}, blocks {int(d['blocks'])}), fit {d['seconds_fit']:.2f} s")
+        describe(i, "warmup", time.perf_counter() - t, d)
     torch.cuda.synchronize()
     times, diags, anchors = [], [], None
     for i in range(args.steps):
@@ -130,18 +129,9 @@ def main():
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t)
         diags.append(d)
-        log(f"step {i}: {times[-1]:.2f} s (sketch {d['seconds_sketch']:.2f}, qrcp {d['seconds_qrcp']:.2f}, "
-            f"blocks {int(d['blocks'])}, fit {d['seconds_fit']:.2f})")
+        describe(i, "step", times[-1], d)
     t_step = max(times) if times else float("nan")
     d = diags[-1] if diags else {}
-
-    # Fit quality of the value form: grid pass with the pair-space error.
-    t = time.perf_counter()
-    err = fit(anchors, grid_pass=True)
-    t_pass = time.perf_counter() - t
-    quality = {"pair_space_relative_error": float(err[1]), "grid_pass_seconds": round(err[5], 3),
-               "gram_condition_estimate": float(err[2])}
-    log(f"pair-space relative error {err[1]:.4e} (grid pass {err[5]:.2f} s)")
 
     fp64_form = None
     if not args.no_fp64_form and args.precision != "fp64":
@@ -150,70 +140,39 @@ def main():
         ref, d64 = step("fp64")
         torch.cuda.synchronize()
         t64 = time.perf_counter() - t
+        describe(0, "fp64 form", t64, d64)
         overlap = len(set(ref) & set(anchors)) / max(1, len(ref))
-        err64 = fit(ref, grid_pass=True)
+        e, e64 = d["pair_space_relative_error"], d64["pair_space_relative_error"]
         fp64_form = {"seconds": round(t64, 4), "seconds_sketch": round(d64["seconds_sketch"], 4),
-                     "seconds_qrcp": round(d64["seconds_qrcp"], 4), "seconds_fit": round(d64["seconds_fit"], 4),
+                     "seconds_qrcp": round(d64["seconds_qrcp"], 4),
+                     "seconds_fit": round(d64["seconds_fit_factor"] + d64["seconds_fit_grid"], 4),
                      "blocks": int(d64["blocks"]), "speedup_of_value_form": round(t64 / t_step, 3),
-                     "anchor_overlap": round(overlap, 4), "pair_space_relative_error": float(err64[1]),
-                     "relative_error_difference": float(abs(err[1] - err64[1]) / err64[1]) if err64[1] else None}
-        log(f"fp64 form: {t64:.2f} s; overlap {overlap:.3f}; pair error {err64[1]:.4e}")
+                     "anchor_overlap": round(overlap, 4), "pair_space_relative_error": e64,
+                     "pair_space_error_difference": abs(e - e64),
+                     "relative_error_difference": abs(e - e64) / e64 if e64 else None}
+        log(f"fp64 form: overlap {overlap:.3f}; pair error {e64:.4e} vs {e:.4e}")
 
-    # ---- dominant kernels, timed with HIP events on the default stream -------
-    roofline = roofline_sketch = None
-    dp = (mol.nao + 15) // 16 * 16
-    count = min(65536, n)
-    pts = torch.tensor(xyz[keep][:count], device="cuda")
-    sw = torch.tensor(np.sqrt(np.abs(w[keep][:count])), device="cuda")
+    # ---- dominant kernels of the timed step, from its own HIP-event timings ----
+    y_type = "FP64" if args.precision == "fp64" else "FP32"
+    fold_s, fold_f, fold_n = d.get("fold_seconds", 0.0), d.get("fold_flops", 0.0), max(1, int(d.get("fold_launches", 0)))
+    roofline = {"kernel": f"tn_gemm_kernel fold (QRCP fold on {y_type} Y)", "bound": "mfma",
+                "achieved": round(fold_f / fold_s / 1e12, 2) if fold_s else None,
+                "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                "frac": round(fold_f / fold_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if fold_s else None,
+                "traffic": None, "launches": fold_n, "launch_ms": round(fold_s / fold_n * 1e3, 3),
+                "flops_per_launch": fold_f / fold_n,
+                "work": f"2*b*ell*n per fold, b = pivots folded (mean {k / fold_n:.1f}), ell={ell}, n={n}"}
+    sk_s, sk_f, sk_n = d.get("sketch_kernel_seconds", 0.0), d.get("sketch_flops", 0.0), max(1, int(d.get("sketch_launches", 0)))
+    peak_sk = PEAK["fp64_mfma_tflops" if args.precision == "fp64" else
+                   ("bf16_mfma_tflops" if args.precision == "bf16" else "fp8_mfma_tflops")]
+    roofline_sketch = {"kernel": f"kr_sketch ({args.precision})", "bound": "mfma",
+                       "achieved": round(sk_f / sk_s / 1e12, 2) if sk_s else None, "peak": peak_sk,
+                       "unit": "TFLOP/s", "frac": round(sk_f / sk_s / 1e12 / peak_sk, 4) if sk_s else None,
+                       "launches": sk_n, "launch_ms": round(sk_s / sk_n * 1e3, 3),
+                       "work": f"4*ell*d*n = 4*{ell}*{mol.nao}*{n} over {sk_n} grid batches"}
     u_h = np.empty((mol.nao, ell))
     v_h = np.empty((mol.nao, ell))
     capi.call("mpisdf_omega", mol.nao, ell, 7, u_h, v_h)
-    if args.precision in ("bf16", "fp8"):
-        code = 2 if args.precision == "bf16" else 3
-        et = torch.bfloat16 if code == 2 else torch.uint8
-        ut = torch.empty((ell, dp), dtype=et, device="cuda")
-        vt = torch.empty((ell, dp), dtype=et, device="cuda")
-        capi.call("mpisdf_sketch_prepare_omega", torch.tensor(u_h, device="cuda"), torch.tensor(v_h, device="cuda"),
-                  mol.nao, ell, dp, code, ut, vt, None)
-        scale = torch.empty(count, dtype=torch.float32, device="cuda")
-        if code == 2:
-            phi = torch.zeros((count, dp), dtype=torch.bfloat16, device="cuda")
-            cctx.ao_eval(pts, phi, mode=0, dtype=2, ld=dp, cstride=count * dp)
-            capi.call("mpisdf_sketch_scales_bf16", sw, count, ell, scale, None)
-        else:
-            phi32 = torch.empty((count, mol.nao), dtype=torch.float32, device="cuda")
-            cctx.ao_eval(pts, phi32, mode=0, dtype=1)
-            phi = torch.zeros((count, dp), dtype=torch.uint8, device="cuda")
-            capi.call("mpisdf_sketch_quantize_fp8", phi32, mol.nao, count, mol.nao, sw, ell, phi, dp, scale, None)
-        ybuf = torch.empty((count, ell), dtype=torch.float32, device="cuda")
-        yn = torch.zeros(count, dtype=torch.float64, device="cuda")
-        bad = torch.zeros(1, dtype=torch.int64, device="cuda")
-        t_sk = time_events(torch, lambda: capi.call("mpisdf_kr_sketch_device", code, ut, vt, ell, dp, phi, dp, count,
-                                                    scale, ybuf, ell, yn, bad, None))
-        flops = 4.0 * ell * mol.nao * count
-        peak = PEAK["bf16_mfma_tflops" if code == 2 else "fp8_mfma_tflops"]
-        roofline_sketch = {"kernel": f"kr_sketch_{args.precision}_kernel", "bound": "mfma",
-                           "achieved": round(flops / t_sk / 1e12, 2), "peak": peak, "unit": "TFLOP/s",
-                           "frac": round(flops / t_sk / 1e12 / peak, 4), "launch_ms": round(t_sk * 1e3, 3),
-                           "work": f"4*ell*d*points = 4*{ell}*{mol.nao}*{count}"}
-        del ybuf, ut, vt, phi
-    # QRCP fold on the value-form's Y type, b = mean pivots per block.
-    b = max(1, int(round(k / max(1.0, d.get("blocks", 1.0)))))
-    y_f32 = args.precision != "fp64"
-    ycols = min(n, 262144)
-    Y = torch.randn((ycols, ell), dtype=torch.float32 if y_f32 else torch.float64, device="cuda")
-    Q = torch.linalg.qr(torch.randn((ell, b), dtype=torch.float64, device="cuda"))[0].t().contiguous().t()
-    Qb = Q.t().contiguous()  # column-major ell x b == row-major b x ell
-    proj = torch.zeros(ycols, dtype=torch.float64, device="cuda")
-    t_fold = time_events(torch, lambda: capi.call("mpisdf_qrcp_fold_device", b, ycols, ell, Qb, Y, int(y_f32), ell,
-                                                  proj, None, None))
-    fold_flops = 2.0 * b * ell * ycols
-    roofline = {"kernel": "gemm_mfma_kernel<fold> (QRCP fold)", "bound": "mfma",
-                "achieved": round(fold_flops / t_fold / 1e12, 2), "peak": PEAK["fp64_mfma_tflops"],
-                "unit": "TFLOP/s", "frac": round(fold_flops / t_fold / 1e12 / PEAK["fp64_mfma_tflops"], 4),
-                "traffic": None, "launch_ms": round(t_fold * 1e3, 3),
-                "work": f"2*b*ell*columns = 2*{b}*{ell}*{ycols} (FP64 MFMA on {'FP32' if y_f32 else 'FP64'} Y)"}
-    del Y, Q, Qb, proj
 
     # ---- CPU baseline: oracle AO + FP64 sketch on a bounded sample ----------
     cpu = None
@@ -234,17 +193,20 @@ def main():
 
     value = n / t_step
     out = {
-        "metric": "anchor-select+fit grid-pts/s (rand_sketch, BF16/FP8 vs FP64 GPU)",
+        "metric": "anchor-select+fit s and grid-pts/s, BF16/FP8 vs FP64 GPU at equal fit error",
         "value": round(value, 1), "unit": "grid-pts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(t_step * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": args.precision, "data": "synthetic: seeded water cluster, library Becke grid, seed-7 Gaussian sketch",
         "config": {"workload": f"(H2O){nwater}/{basis} level-{level} grid rand_sketch k={k}",
                    "grid_points": n, "nao": mol.nao, "anchors": k, "sketch_rows": ell,
-                   "step": "selection + fit factorisation", "precision": args.precision},
+                   "step": "selection + FP64 fit (theta on every kept grid point, pair-space error)",
+                   "precision": args.precision},
         "stages": {"seconds_sketch": round(d.get("seconds_sketch", 0), 4),
                    "seconds_qrcp": round(d.get("seconds_qrcp", 0), 4), "qrcp_blocks": int(d.get("blocks", 0)),
-                   "seconds_fit": round(d.get("seconds_fit", 0), 4)},
-        "fit_quality": quality,
+                   "seconds_fit_factor": round(d.get("seconds_fit_factor", 0), 4),
+                   "seconds_fit_grid": round(d.get("seconds_fit_grid", 0), 4)},
+        "fit_quality": {"pair_space_relative_error": d.get("pair_space_relative_error"),
+                        "gram_condition_estimate": d.get("gram_condition_estimate")},
         "fp64_form": fp64_form,
         "roofline": roofline,
         "roofline_sketch": roofline_sketch,

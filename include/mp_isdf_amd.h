@@ -72,7 +72,7 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
  * precision (rand_sketch only): 0 fp64, 1 fp32, 2 bf16, 3 fp8 e4m3.
  * axis_weights: NULL or naxis_weights == naxes finite positive factors.
  * Writes original grid indices in pivot order to out[0..*count).
- * diagnostics: NULL or double[16] (see mp_isdf_amd/csrc/select.hpp). */
+ * diagnostics: NULL or double[24] (see mp_isdf_amd/csrc/select.hpp). */
 int mpisdf_select(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
                   const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc,
                   const double* axis_weights, int64_t naxis_weights, double tolerance,
@@ -134,10 +134,12 @@ int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_xyz, const d
                     const double* xyz, const double* w, int64_t n, int64_t batch, double* theta, double* tsum,
                     double* results);
 /* Same with device-resident grid (xyz_dev[n*3], w_dev[n]) and anchors
- * (anchor_xyz_dev[k*3]); anchor_w stays on the host; no theta output. */
+ * (anchor_xyz_dev[k*3]); anchor_w stays on the host.  theta is formed batch
+ * by batch on the device when tsum (host, k entries) is given and consumed
+ * there: tsum = sum_g theta[:, g] w_g. */
 int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,
-                           const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* results,
-                           void* stream);
+                           const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* tsum,
+                           double* results, void* stream);
 
 /* ---- Grid front end (no reference counterpart: the reference takes its grid
  * from PySCF's dft.gen_grid.Grids, python/isdf.py:488-496) ------------------ */

@@ -354,11 +354,11 @@ extern "C" int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_x
 
 extern "C" int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_dev, const double* anchor_w,
                                       int64_t k, const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch,
-                                      double* results, void* stream)
+                                      double* tsum, double* results, void* stream)
 {
     return guard([&] {
         auto& c = unwrap(ctx);
-        run_fit(c, anchor_xyz_dev, anchor_w, k, xyz_dev, w_dev, n, batch, nullptr, nullptr, results,
+        run_fit(c, anchor_xyz_dev, anchor_w, k, xyz_dev, w_dev, n, batch, nullptr, tsum, results,
                 static_cast<hipStream_t>(stream));
     });
 }

@@ -20,4 +20,37 @@ void ensure_device()
     }
 }
 
+void LaunchTimer::begin(hipStream_t s)
+{
+    hipEvent_t e0, e1;
+    MPISDF_HIP(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        throw std::runtime_error("hipEventCreate failed");
+    }
+    events.emplace_back(e0, e1);
+    MPISDF_HIP(hipEventRecord(e0, s));
+}
+
+void LaunchTimer::end(hipStream_t s) { MPISDF_HIP(hipEventRecord(events.back().second, s)); }
+
+double LaunchTimer::seconds() const
+{
+    double total = 0;
+    for (const auto& ev : events) {
+        float ms = 0.f;
+        MPISDF_HIP(hipEventElapsedTime(&ms, ev.first, ev.second));
+        total += 1e-3 * ms;
+    }
+    return total;
+}
+
+LaunchTimer::~LaunchTimer()
+{
+    for (auto& ev : events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+}
+
 }  // namespace mpisdf

@@ -103,4 +103,18 @@ struct DeviceBuffer {
     }
 };
 
+// Durations of a sequence of launches on one stream, bracketed by HIP events
+// (begin/end pairs); read after the stream has been synchronised.
+struct LaunchTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    void begin(hipStream_t s);
+    void end(hipStream_t s);
+    double seconds() const;  // sum of the recorded durations
+    size_t launches() const { return events.size(); }
+    LaunchTimer() = default;
+    LaunchTimer(const LaunchTimer&) = delete;
+    LaunchTimer& operator=(const LaunchTimer&) = delete;
+    ~LaunchTimer();
+};
+
 }  // namespace mpisdf

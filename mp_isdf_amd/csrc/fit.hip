@@ -116,12 +116,29 @@ __global__ void pair_error_kernel(const double* __restrict__ z, int64_t k, const
     if (lane == 0 && g < nb) { atomicAdd(&out[0], e); atomicAdd(&out[1], t); }
 }
 
-// theta[t, g] /= w_anchor[t]   (theta k x nb row-major)
-__global__ void scale_rows_kernel(double* __restrict__ theta, int64_t k, int64_t nb, const double* __restrict__ wa)
+// out[c * rows + r] = in[r * cols + c] / (div ? div[c] : 1)
+__global__ void transpose_kernel(const double* __restrict__ in, int64_t rows, int64_t cols,
+                                 const double* __restrict__ div, double* __restrict__ out)
 {
-    const int64_t total = k * nb;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
-        theta[i] /= wa[i / nb];
+    __shared__ double tile[32][33];
+    const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t rr = r0 + r, cc = c0 + tx;
+        tile[r][tx] = (rr < rows && cc < cols) ? in[rr * cols + cc] : 0.0;
+    }
+    __syncthreads();
+    for (int c = ty; c < 32; c += 8) {
+        const int64_t cc = c0 + c, rr = r0 + tx;
+        if (rr < rows && cc < cols) out[cc * rows + rr] = div ? tile[tx][c] / div[cc] : tile[tx][c];
+    }
+}
+
+void transpose(const double* in, int64_t rows, int64_t cols, const double* div, double* out, hipStream_t s)
+{
+    const dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+    hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, s, in, rows, cols, div, out);
+    MPISDF_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -137,7 +154,7 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
     f.linv.resize((size_t)(k * k));
     DeviceBuffer<double> gdiag;
     DeviceBuffer<double> a((size_t)(k * k));
-    gemm_f64(k, k, d, 1.0, MatView{f.phi_a.ptr, d, 1}, MatView{f.phi_a.ptr, 1, d}, 0.0, MutView{a.ptr, k, 1}, s);
+    gemm_tn_f64(k, k, d, 1.0, f.phi_a.ptr, d, f.phi_a.ptr, d, 0.0, a.ptr, k, s);  // symmetric: layout-free
     hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * k)), dim3(256), 0, s, a.ptr, k * k);
     MPISDF_HIP(hipGetLastError());
     // Diagonal of G2 gives the pair column norms; keep the max for diagnostics.
@@ -168,9 +185,9 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
                  MutView{panel.ptr, jb, 1}, s);
         MPISDF_HIP(hipMemcpy2DAsync(a.ptr + (j0 + jb) * k + j0, k * sizeof(double), panel.ptr, jb * sizeof(double),
                                     jb * sizeof(double), rem, hipMemcpyDeviceToDevice, s));
-        // trailing A[j0+jb:, j0+jb:] -= panel panel^T
-        gemm_f64(rem, rem, jb, -1.0, MatView{panel.ptr, jb, 1}, MatView{panel.ptr, 1, jb}, 1.0,
-                 MutView{a.ptr + (j0 + jb) * k + (j0 + jb), k, 1}, s);
+        // trailing A[j0+jb:, j0+jb:] -= panel panel^T (symmetric, so the TN
+        // product's column-major store is the row-major block)
+        gemm_tn_f64(rem, rem, jb, -1.0, panel.ptr, jb, panel.ptr, jb, 1.0, a.ptr + (j0 + jb) * k + (j0 + jb), k, s);
     }
     int hinfo = 0;
     MPISDF_HIP(hipMemcpyAsync(&hinfo, info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -194,6 +211,8 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
         gemm_f64(jb, width, jb, 1.0, MatView{invd.ptr + (j0 / NB) * NB * NB, jb, 1}, MatView{r.ptr, width, 1}, 0.0,
                  MutView{f.linv.ptr + j0 * k, k, 1}, s);
     }
+    f.linvt.resize((size_t)(k * k));
+    transpose(f.linv.ptr, k, k, nullptr, f.linvt.ptr, s);
     MPISDF_HIP(hipMemcpy2DAsync(ldiag.data(), sizeof(double), a.ptr, (k + 1) * sizeof(double), sizeof(double), k,
                                 hipMemcpyDeviceToHost, s));
     MPISDF_HIP(hipStreamSynchronize(s));
@@ -209,14 +228,14 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     const int64_t k = f.k, d = f.d;
     batch = std::max<int64_t>(1, std::min(batch, n));
     DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
-    DeviceBuffer<double> th, tsum;
+    DeviceBuffer<double> th, thr, tsum;
     wa.upload(anchor_w, k, s);
+    if (tsum_out || theta_host) th.resize((size_t)(k * batch));
+    if (theta_host) thr.resize((size_t)(k * batch));
     if (tsum_out) {
         tsum.resize(k);
         MPISDF_HIP(hipMemsetAsync(tsum.ptr, 0, k * sizeof(double), s));
-        th.resize((size_t)(k * batch));
     }
-    if (theta_host && !th.count) th.resize((size_t)(k * batch));
     MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
     for (int64_t start = 0; start < n; start += batch) {
         const int64_t nb = std::min(batch, n - start);
@@ -224,21 +243,23 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         // S = (Phi_a Phi_b^T)^2, k x nb column-major
         gemm_tn_f64(k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, 0.0, sm.ptr, k, s);
         hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * nb)), dim3(256), 0, s, sm.ptr, k * nb);
-        // Z = L^-1 S
-        gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s);
+        // Z = L^-1 S (lower triangular A)
+        gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s, 1);
         if (err_out)
             hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
                                w + start, nb, acc.ptr);
         if (theta_host || tsum_out) {
-            // theta_phys = L^-T Z, stored row-major k x nb; theta = theta_phys / w_anchor
-            gemm_f64(k, nb, k, 1.0, MatView{f.linv.ptr, 1, k}, MatView{z.ptr, 1, k}, 0.0, MutView{th.ptr, nb, 1}, s);
-            hipLaunchKernelGGL(scale_rows_kernel, dim3(grid_for(k * nb)), dim3(256), 0, s, th.ptr, k, nb, wa.ptr);
-            if (tsum_out)  // t += theta w_grid
-                gemm_f64(k, 1, nb, 1.0, MatView{th.ptr, nb, 1}, MatView{w + start, 1, 1}, 1.0, MutView{tsum.ptr, 1, 1},
+            // theta_phys = L^-T Z (upper triangular A), column-major k x nb;
+            // theta = theta_phys / w_anchor
+            gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, th.ptr, k, s, 2);
+            if (tsum_out)  // t_phys += theta_phys w_grid (divided by w_anchor at the end)
+                gemm_f64(k, 1, nb, 1.0, MatView{th.ptr, 1, k}, MatView{w + start, 1, 1}, 1.0, MutView{tsum.ptr, 1, 1},
                          s);
-            if (theta_host)  // rows of theta[k][n] on the host
-                MPISDF_HIP(hipMemcpy2DAsync(theta_host + start, n * sizeof(double), th.ptr, nb * sizeof(double),
+            if (theta_host) {  // rows of theta[k][n] on the host
+                transpose(th.ptr, nb, k, wa.ptr, thr.ptr, s);
+                MPISDF_HIP(hipMemcpy2DAsync(theta_host + start, n * sizeof(double), thr.ptr, nb * sizeof(double),
                                             nb * sizeof(double), k, hipMemcpyDeviceToHost, s));
+            }
         }
         MPISDF_HIP(hipGetLastError());
     }
@@ -250,6 +271,7 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     if (tsum_out) {
         MPISDF_HIP(hipMemcpyAsync(tsum_out, tsum.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
         MPISDF_HIP(hipStreamSynchronize(s));
+        for (int64_t t = 0; t < k; ++t) tsum_out[t] /= anchor_w[t];
     }
 }
 

@@ -9,6 +9,7 @@ struct FitFactor {
     int64_t k = 0, d = 0;
     DeviceBuffer<double> phi_a;  // [k][d] anchor AO values
     DeviceBuffer<double> linv;   // [k][k] row-major inverse of the Cholesky factor of G2
+    DeviceBuffer<double> linvt;  // [k][k] row-major transpose of linv (upper triangular)
     double condition_estimate = 0, gram_diag_max = 0;
 };
 

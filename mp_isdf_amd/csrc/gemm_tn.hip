@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
                                                           double* __restrict__ colacc, double* __restrict__ rowacc,
                                                           const double* __restrict__ A2,
                                                           const double* __restrict__ cscale,
-                                                          unsigned long long* __restrict__ bad)
+                                                          unsigned long long* __restrict__ bad, int tri)
 {
     __shared__ __align__(16) double sA[2][TBM * LDA_S];
     __shared__ double red[4][TBM];
@@ -125,10 +125,14 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         }
     };
 
-    const int64_t nslices = (K + TBK - 1) / TBK;
+    // Triangular A (tri = 1 lower, 2 upper): this row group's nonzero columns
+    // are k < m0 + 128 (lower) or k >= m0 (upper); the rest of K is skipped.
+    const int64_t k_lo = tri == 2 ? m0 : 0;
+    const int64_t k_hi = tri == 1 ? std::min<int64_t>(K, m0 + TBM) : K;
+    const int64_t nslices = k_hi > k_lo ? (k_hi - k_lo + TBK - 1) / TBK : 0;
     if (nslices > 0) {
-        load_a(0);
-        load_b(0, yb);
+        load_a(k_lo);
+        load_b(k_lo, yb);
         stash_a(0);
     }
     __syncthreads();
@@ -136,8 +140,8 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         const int buf = (int)(sl & 1);
         const bool more = sl + 1 < nslices;
         if (more) {
-            load_a((sl + 1) * TBK);
-            load_b((sl + 1) * TBK, yn);
+            load_a(k_lo + (sl + 1) * TBK);
+            load_b(k_lo + (sl + 1) * TBK, yn);
         }
         const double* As = sA[buf];
 #pragma unroll
@@ -257,7 +261,8 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
 template <class TB, int EPI>
 void launch_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const TB* B, int64_t ldb, double alpha,
                double beta, double* C, int64_t ldc, double* colacc, double* rowacc, hipStream_t stream,
-               const double* A2 = nullptr, const double* cscale = nullptr, unsigned long long* bad = nullptr)
+               const double* A2 = nullptr, const double* cscale = nullptr, unsigned long long* bad = nullptr,
+               int tri = 0)
 {
     if (M <= 0 || N <= 0) return;
     const int64_t rows = EPI == kTnKhatriRao ? TBM / 2 : TBM;
@@ -268,19 +273,20 @@ void launch_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, co
                      (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB))) == 0;
     if (vec)
         hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
-                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad);
+                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad, tri);
     else
         hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, false>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
-                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad);
+                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad, tri);
     MPISDF_HIP(hipGetLastError());
 }
 
 }  // namespace
 
 void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
-                 int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream)
+                 int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream, int triangular)
 {
-    launch_tn<double, kTnStore>(M, N, K, A, lda, B, ldb, alpha, beta, C, ldc, nullptr, nullptr, stream);
+    launch_tn<double, kTnStore>(M, N, K, A, lda, B, ldb, alpha, beta, C, ldc, nullptr, nullptr, stream, nullptr,
+                                nullptr, nullptr, triangular);
 }
 
 void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldq, const void* Y, bool y_f32,

@@ -42,8 +42,10 @@ void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, con
 
 // TN product with both operands contiguous along K (gemm_tn.hip):
 // C[t + j * ldc] = alpha * sum_k A[k + t * lda] B[k + j * ldb] + beta * C[...].
+// triangular = 1 (A lower: A[t][k] = 0 for k > t) or 2 (A upper: zero for
+// k < t) skips the zero part of K.
 void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
-                 int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream);
+                 int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream, int triangular = 0);
 
 // FP64 Khatri-Rao sketch on the TN kernel: ut/vt are ell x d row-major
 // (ut[s * d + a]), phi[g * ldphi + a];  y[g * ldy + s] = scale[g] (ut_s . phi_g)

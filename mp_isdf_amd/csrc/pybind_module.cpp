@@ -214,15 +214,16 @@ public:
                                       bool debug, int64_t sketch_size, int64_t oversampling, uint64_t seed,
                                       const std::vector<double>& axis_weights, const std::string& precision) const
     {
-        std::vector<double> diag(16, 0.0);
+        std::vector<double> diag(24, 0.0);
         auto idx = select(points, weights, axes, tolerance, max_rank, algorithm, mo_coeff, nocc, space, batch_size,
                           debug, sketch_size, oversampling, seed, false, py::none(), 100, 1e-5, axis_weights,
                           py::none(), 1e-24, precision, diag.data());
         static const char* names[] = {"rows", "columns", "numerical_rank", "rank_cap", "rank", "sketch_norm",
                                       "relative_tail", "blocks", "folds", "seconds_sketch", "seconds_qrcp",
-                                      "seconds_total", "stalls", "refreshes"};
+                                      "seconds_total", "stalls", "refreshes", "fold_seconds", "fold_flops",
+                                      "fold_launches", "sketch_kernel_seconds", "sketch_flops", "sketch_launches"};
         py::dict d;
-        for (int i = 0; i < 14; ++i) d[names[i]] = diag[i];
+        for (int i = 0; i < 20; ++i) d[names[i]] = diag[i];
         return py::make_tuple(idx, d);
     }
 

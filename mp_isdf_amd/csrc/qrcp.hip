@@ -306,15 +306,20 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     bool done = false;
     FoldScratch fold_scratch;
     const bool split = opt.split_bf16_fold && ytype == kF32;
+    // Fold launches are bracketed by HIP events; durations are read at the end.
+    LaunchTimer fold_timer;
     auto fold = [&](double* energy_out) {
         const int64_t b = k - folded;
         if (b <= 0) return;
+        fold_timer.begin(s);
+        res.fold_flops += 2.0 * (double)b * (double)m * (double)n;
         if (split)
             fold_projections_bf16x3(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const float*>(y), ldy, proj2.ptr,
                                     energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
         else
             fold_projections(b, n, m, Q.ptr + folded * ldq, ldq, y, ytype == kF32, ldy, proj2.ptr,
                              energy_out ? energy_out + folded : nullptr, s);
+        fold_timer.end(s);
         folded = k;
         ++res.folds;
     };
@@ -458,6 +463,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         MPISDF_HIP(hipMemcpyAsync(&res.tail2, tail.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
     }
     MPISDF_HIP(hipStreamSynchronize(s));
+    res.fold_seconds = fold_timer.seconds();
     return res;
 }
 

@@ -24,6 +24,8 @@ struct QrcpResult {
     std::vector<double> energy;   // per pivot row (if requested)
     double frob2 = 0.0;           // ||Y||_F^2
     double tail2 = 0.0;           // residual Frobenius^2 after the last pivot (if requested)
+    double fold_seconds = 0.0;    // summed HIP-event duration of the fold launches
+    double fold_flops = 0.0;      // their algorithmic flops, sum 2 b m n
     int64_t blocks = 0, folds = 0, stalls = 0, refreshes = 0;
 };
 

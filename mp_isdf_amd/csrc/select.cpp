@@ -23,6 +23,7 @@
 #include <iostream>
 #include <limits>
 #include <random>
+#include <thread>
 
 namespace mpisdf {
 namespace {
@@ -151,6 +152,9 @@ std::vector<int64_t> select_dense(Context& ctx, const SelectArgs& a, const AxisS
         diag[kDiagRank] = (double)out.size();
         diag[kDiagBlocks] = (double)r.blocks;
         diag[kDiagFolds] = (double)r.folds;
+        diag[kDiagFoldSeconds] = r.fold_seconds;
+        diag[kDiagFoldFlops] = r.fold_flops;
+        diag[kDiagFoldLaunches] = (double)r.folds;
         diag[kDiagSecondsQrcp] = seconds_since(t0);
     }
     return out;
@@ -211,6 +215,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     const int64_t ldy64 = (ell + 3) / 4 * 4;  // 32-byte aligned FP64 columns
     DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;
+    double t_sketch_kernel = 0, sketch_launches = 0;
     ynorm2.resize(n);
     MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
     if (!lowp) {
@@ -225,6 +230,9 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             sketch_f64(feat.ptr, count, d, u.ptr, v.ptr, ell, sw.ptr + start, y64.ptr + start * ldy64, ldy64, scratch,
                        bad.ptr, s, ynorm2.ptr + start);
         }
+        MPISDF_HIP(hipStreamSynchronize(s));
+        t_sketch_kernel = scratch.timer.seconds();
+        sketch_launches = (double)scratch.timer.launches();
     } else {
         // Reduced precision: operands padded to a multiple of 16 features.
         const int64_t dp = (d + 15) / 16 * 16;
@@ -233,6 +241,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         prepare_omega_lowp(u.ptr, v.ptr, d, ell, dp, in_type, ut.ptr, vt.ptr, s);
         y32.resize((size_t)ldy32 * n);
         const bool ao_only = spec.kinds.size() == 1 && spec.kinds[0] == 0;
+        LaunchTimer timer;
         const int64_t batch = std::min<int64_t>(n, 1 << 16);
         DeviceBuffer<uint8_t> phi((size_t)batch * dp * esz);
         MPISDF_HIP(hipMemsetAsync(phi.ptr, 0, phi.count, s));  // padding columns stay zero
@@ -260,9 +269,14 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
                 }
                 quantize_fp8_rows(phi32.ptr, d, count, d, sw.ptr + start, ell, phi.ptr, dp, scale.ptr, s);
             }
+            timer.begin(s);
             kr_sketch_lowp(in_type, ut.ptr, vt.ptr, ell, dp, phi.ptr, dp, count, scale.ptr, y32.ptr + start * ldy32,
                            ldy32, ynorm2.ptr + start, bad.ptr, s);
+            timer.end(s);
         }
+        MPISDF_HIP(hipStreamSynchronize(s));
+        t_sketch_kernel = timer.seconds();
+        sketch_launches = (double)timer.launches();
     }
     if (bad.download(s)[0]) throw std::overflow_error("non-finite randomized ID sketch");
     const double t_sketch = seconds_since(started);
@@ -333,6 +347,12 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         diag[kDiagSecondsQrcp] = t_qrcp;
         diag[kDiagStalls] = (double)r.stalls;
         diag[kDiagRefreshes] = (double)r.refreshes;
+        diag[kDiagFoldSeconds] = r.fold_seconds;
+        diag[kDiagFoldFlops] = r.fold_flops;
+        diag[kDiagFoldLaunches] = (double)r.folds;
+        diag[kDiagSketchSeconds] = t_sketch_kernel;
+        diag[kDiagSketchFlops] = 4.0 * (double)ell * (double)d * (double)n;
+        diag[kDiagSketchLaunches] = sketch_launches;
     }
     return out;
 }
@@ -341,14 +361,18 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
 
 void make_omega(int64_t d, int64_t ell, uint64_t seed, std::vector<double>& u, std::vector<double>& v)
 {
+    // Two independent serial streams (each must replay its generator in
+    // order), drawn on two threads.
     u.resize((size_t)(d * ell));
     v.resize((size_t)(d * ell));
-    std::mt19937_64 gen_u(seed), gen_v(seed ^ 0x9e3779b97f4a7c15ULL);
-    std::normal_distribution<double> normal_u(0, 1), normal_v(0, 1);
-    for (int64_t i = 0; i < d * ell; ++i) {
-        u[i] = normal_u(gen_u);
-        v[i] = normal_v(gen_v);
-    }
+    auto fill = [d, ell](std::vector<double>& out, uint64_t s) {
+        std::mt19937_64 gen(s);
+        std::normal_distribution<double> normal(0, 1);
+        for (int64_t i = 0; i < d * ell; ++i) out[i] = normal(gen);
+    };
+    std::thread tv(fill, std::ref(v), seed ^ 0x9e3779b97f4a7c15ULL);
+    fill(u, seed);
+    tv.join();
 }
 
 std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& a, double* diag)

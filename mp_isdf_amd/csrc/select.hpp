@@ -45,7 +45,13 @@ enum Diag : int {
     kDiagSecondsTotal = 11,
     kDiagStalls = 12,       // QRCP blocks without a certified pivot
     kDiagRefreshes = 13,    // exact FP64 refreshes of the folded norms
-    kDiagCount = 16
+    kDiagFoldSeconds = 14,  // summed duration of the QRCP fold launches (HIP events)
+    kDiagFoldFlops = 15,    // their algorithmic FP64 flops: sum 2 b ell n
+    kDiagFoldLaunches = 16,
+    kDiagSketchSeconds = 17,  // summed duration of the sketch kernel launches (HIP events)
+    kDiagSketchFlops = 18,    // their algorithmic flops: sum 4 ell d count
+    kDiagSketchLaunches = 19,
+    kDiagCount = 24
 };
 
 std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& args, double* diagnostics);

@@ -62,8 +62,10 @@ void sketch_f64(const double* phi, int64_t count, int64_t d, const double* u, co
     hipLaunchKernelGGL(point_scale_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, sw, count,
                        1.0 / std::sqrt((double)ell), scratch.scale.ptr);
     MPISDF_HIP(hipGetLastError());
+    scratch.timer.begin(stream);
     kr_sketch_f64_tn(scratch.ut.ptr, scratch.vt.ptr, ell, d, phi, d, count, scratch.scale.ptr, y, ldy, ynorm2, bad,
                      stream);
+    scratch.timer.end(stream);
 }
 
 }  // namespace mpisdf

@@ -11,6 +11,7 @@ struct SketchScratch {
     const double* u_src = nullptr;
     const double* v_src = nullptr;
     int64_t ell = 0, d = 0;
+    LaunchTimer timer;  // sketch kernel launches
 };
 
 // Y[g * ldy + s] for g < count, s < ell; phi is [count][d] FP64, u and v are
