@@ -49,6 +49,9 @@ def parse():
     p.add_argument("--no-fp64-form", action="store_true", help="skip the all-FP64 comparison run")
     p.add_argument("--cpu-points", type=int, default=768, help="grid points in the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="finish with three device-to-device copies of this many MiB (a known-size kernel for "
+                        "calibrating HBM byte counters under rocprofv3 --pmc)")
     return p.parse_args()
 
 
@@ -213,6 +216,13 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe_mib > 0:
+        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 x {args.copy_probe_mib} MiB device-to-device")
 
 
 if __name__ == "__main__":
