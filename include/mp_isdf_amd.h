@@ -141,6 +141,23 @@ int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_d
                            const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* tsum,
                            double* results, void* stream);
 
+/* ---- Adjoint (no reference counterpart: the reference has no nuclear
+ * derivatives; it provides the dx/dy/dz AO derivatives the adjoint is checked
+ * against, cpp/src/ao_eval.cpp:215-244) ----------------------------------- */
+/* gout[c*nao + mu] += sum_p abar[p*ldab + mu] * d phi_mu(r_p)/d r_c for the
+ * device points xyz[n*3]; the AO gradients are recomputed, never stored. */
+int mpisdf_ao_adjoint_device(const mpisdf_context* ctx, const double* xyz, int64_t n, const double* abar,
+                             int64_t ldab, double* gout, void* stream);
+/* Host-array form: xyz[n*3], abar[n*nao] row-major, gout[3*nao] overwritten. */
+int mpisdf_ao_adjoint_host(const mpisdf_context* ctx, const double* xyz, int64_t n, const double* abar, double* gout);
+/* Pair-reconstruction loss L = sum_g |w_g| ||phi_g phi_g^T - sum_k c_gk phi_k phi_k^T||^2
+ * of the FP64 fit on fixed anchor and grid points (host arrays) and its
+ * gradient with respect to the nuclear coordinates of the context:
+ * *loss, grad_ao[3*nao] (optional), grad_atoms[natm*3] (optional). */
+int mpisdf_isdf_fit_gradient(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w, int64_t k,
+                             const double* xyz, const double* w, int64_t n, int64_t batch, double* loss,
+                             double* grad_ao, double* grad_atoms);
+
 /* ---- Grid front end (no reference counterpart: the reference takes its grid
  * from PySCF's dft.gen_grid.Grids, python/isdf.py:488-496) ------------------ */
 /* Becke-partitioned atom-centred grid, Treutler radial nodes, product angular

@@ -110,12 +110,17 @@ __device__ __forceinline__ void eval_shell(const DevShell& sh, const double* __r
     }
 }
 
-template <class C, class OutT, int NC>
+// ADJOINT = true turns the kernel into the adjoint of gradient evaluation:
+// nothing is stored; instead gout[c * nao + mu] += sum_p abar[p * ldab + mu]
+// d_c phi_mu(r_p) (NC = 3), i.e. the AO gradients are recomputed tile by tile
+// and contracted with the incoming adjoint without ever reaching HBM.
+template <class C, class OutT, int NC, bool ADJOINT = false>
 __global__ __launch_bounds__(kThreads) void ao_eval_kernel(
     const DevShell* __restrict__ shells, const double* __restrict__ env,
     const double* __restrict__ tabs, int ntab, const AoChunk* __restrict__ chunks, int nchunks,
     const double* __restrict__ xyz, int64_t n, OutT* __restrict__ out, int64_t ld, int64_t cstride,
-    int width)
+    int width, const double* __restrict__ abar = nullptr, int64_t ldab = 0, double* __restrict__ gout = nullptr,
+    int64_t nao = 0)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     C* tile = reinterpret_cast<C*>(smem);
@@ -186,6 +191,20 @@ __global__ __launch_bounds__(kThreads) void ao_eval_kernel(
             }
         }
         __syncthreads();
+        if (ADJOINT) {
+            // Contract the staged gradient block with the adjoint rows and
+            // reduce over the tile's points (adjoint rows read coalesced).
+            for (int i = tid; i < NC * cw; i += kThreads) {
+                const int comp = i / cw, j = i - comp * cw;
+                const C* plane = tile + comp * kTile * width;
+                const double* arow = abar + g0 * ldab + ck.a0 + j;
+                double s = 0;
+                for (int p = 0; p < np; ++p) s += (double)plane[p * width + j] * arow[(int64_t)p * ldab];
+                if (s != 0.0) atomicAdd(&gout[comp * nao + ck.a0 + j], s);
+            }
+            __syncthreads();
+            continue;
+        }
         // Coalesced store of the staged block: each point's AO range is one
         // contiguous run in the grid-major output.
         for (int comp = 0; comp < NC; ++comp) {
@@ -220,7 +239,8 @@ void launch_typed(Context& ctx, const double* xyz, int64_t n, void* out, int64_t
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kThreads), lds, stream,
                        ctx.d_shells.ptr, ctx.d_env.ptr, ctx.d_tabs.ptr, ntab, table.second->ptr,
-                       (int)table.first.size(), xyz, n, static_cast<OutT*>(out), ld, cstride, width);
+                       (int)table.first.size(), xyz, n, static_cast<OutT*>(out), ld, cstride, width,
+                       static_cast<const double*>(nullptr), (int64_t)0, static_cast<double*>(nullptr), (int64_t)0);
     MPISDF_HIP(hipGetLastError());
 }
 
@@ -237,6 +257,33 @@ void launch_mode(Context& ctx, const double* xyz, int64_t n, AoMode mode, void* 
 }
 
 }  // namespace
+
+void launch_ao_adjoint(Context& ctx, const double* xyz, int64_t n, const double* abar, int64_t ldab, double* gout,
+                       hipStream_t stream)
+{
+    if (n <= 0 || ctx.nao == 0) return;
+    if (ldab < ctx.nao) throw std::invalid_argument("AO adjoint leading dimension is smaller than nao");
+    constexpr int NC = 3;
+    int width = kTileBudget / (NC * kTile * (int)sizeof(double));
+    width = std::max(width, ctx.max_shell_width);
+    width = std::min<int64_t>(width, std::max<int64_t>(ctx.nao, 1));
+    const auto& table = ctx.chunk_table(width);
+    const int ntab = ctx.tab_offset[ctx.lmax + 1];
+    const size_t lds = (size_t)NC * kTile * width * sizeof(double) + (size_t)ntab * sizeof(double);
+    for (const auto& ck : table.first)
+        if (ck.s1 - ck.s0 > kMaxChunkShells) throw std::logic_error("AO chunk holds too many shells");
+    const int64_t blocks = (n + kTile - 1) / kTile;
+    if (blocks > 0x7fffffff) throw std::overflow_error("AO grid is too large for one launch");
+    auto kernel = ao_eval_kernel<double, double, NC, true>;
+    if (lds > 64 * 1024)
+        MPISDF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kThreads), lds, stream, ctx.d_shells.ptr, ctx.d_env.ptr,
+                       ctx.d_tabs.ptr, ntab, table.second->ptr, (int)table.first.size(), xyz, n,
+                       static_cast<double*>(nullptr), (int64_t)0, (int64_t)0, width, abar, ldab, gout,
+                       (int64_t)ctx.nao);
+    MPISDF_HIP(hipGetLastError());
+}
 
 void launch_ao_eval(Context& ctx, const double* xyz, int64_t n, AoMode mode, DType dtype,
                     void* out, int64_t ld, int64_t cstride, hipStream_t stream)

@@ -362,3 +362,66 @@ extern "C" int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* a
                 static_cast<hipStream_t>(stream));
     });
 }
+
+extern "C" int mpisdf_ao_adjoint_device(const mpisdf_context* ctx, const double* xyz, int64_t n, const double* abar,
+                                        int64_t ldab, double* gout, void* stream)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0 || !gout || (n > 0 && (!xyz || !abar))) throw std::invalid_argument("invalid AO adjoint arguments");
+        mpisdf::launch_ao_adjoint(c, xyz, n, abar, ldab, gout, static_cast<hipStream_t>(stream));
+        MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
+extern "C" int mpisdf_isdf_fit_gradient(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w,
+                                        int64_t k, const double* xyz, const double* w, int64_t n, int64_t batch,
+                                        double* loss, double* grad_ao, double* grad_atoms)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0 || k <= 0 || batch <= 0) throw std::invalid_argument("invalid fit gradient sizes");
+        for (int64_t i = 0; i < k; ++i)
+            if (!std::isfinite(anchor_w[i]) || anchor_w[i] == 0)
+                throw std::invalid_argument("anchor weights must be finite and nonzero");
+        mpisdf::DeviceBuffer<double> dxyz, dw, dax;
+        dxyz.upload(xyz, 3 * n, nullptr);
+        dw.upload(w, n, nullptr);
+        dax.upload(anchor_xyz, 3 * k, nullptr);
+        mpisdf::FitFactor f;
+        mpisdf::fit_factor(c, dax.ptr, k, f, nullptr);
+        std::vector<double> gao(3 * c.nao, 0.0);
+        double l = 0;
+        mpisdf::fit_loss_gradient(c, f, dax.ptr, dxyz.ptr, dw.ptr, n, batch, &l, gao.data(), nullptr);
+        if (loss) *loss = l;
+        if (grad_ao) std::copy(gao.begin(), gao.end(), grad_ao);
+        if (grad_atoms) {
+            // dL/dR_A = -sum_{mu on A} gao[:, mu]: moving a nucleus moves its AOs.
+            const int64_t natm = (int64_t)c.atm.size() / 6;
+            std::fill(grad_atoms, grad_atoms + 3 * natm, 0.0);
+            for (size_t sidx = 0; sidx < c.shells.size(); ++sidx) {
+                const int atom = c.bas[sidx * 8 + 0];
+                const auto& sh = c.shells[sidx];
+                for (int mu = sh.ao_off; mu < sh.ao_off + sh.nctr * sh.ncomp; ++mu)
+                    for (int comp = 0; comp < 3; ++comp) grad_atoms[3 * atom + comp] -= gao[comp * c.nao + mu];
+            }
+        }
+    });
+}
+
+extern "C" int mpisdf_ao_adjoint_host(const mpisdf_context* ctx, const double* xyz, int64_t n, const double* abar,
+                                      double* gout)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0 || !gout) throw std::invalid_argument("invalid AO adjoint arguments");
+        mpisdf::DeviceBuffer<double> dxyz, dab, dg(3 * c.nao);
+        MPISDF_HIP(hipMemsetAsync(dg.ptr, 0, 3 * c.nao * sizeof(double), nullptr));
+        if (n > 0) {
+            dxyz.upload(xyz, 3 * n, nullptr);
+            dab.upload(abar, n * c.nao, nullptr);
+            mpisdf::launch_ao_adjoint(c, dxyz.ptr, n, dab.ptr, c.nao, dg.ptr, nullptr);
+        }
+        MPISDF_HIP(hipMemcpy(gout, dg.ptr, 3 * c.nao * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}

@@ -134,6 +134,47 @@ __global__ void transpose_kernel(const double* __restrict__ in, int64_t rows, in
     }
 }
 
+// out[i] = a[i] * (b ? b[i] : 1) * (w ? |w[i / rows]| : 1)   (k x nb column-major)
+__global__ void hadamard_weight_kernel(const double* a, const double* __restrict__ b,
+                                       const double* __restrict__ w, int64_t rows, int64_t total, double* out)
+{
+    // a and out may alias (in-place weighting)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = a[i] * (b ? b[i] : 1.0) * (w ? fabs(w[i / rows]) : 1.0);
+}
+
+// Adjoint rows of the grid AOs: abar[g][mu] = 4 |w_g| (||phi_g||^2 phi_g[mu] - v[g][mu]),
+// one wave per point.
+__global__ void grid_adjoint_kernel(const double* __restrict__ phi, const double* __restrict__ v, int64_t d,
+                                    const double* __restrict__ w, int64_t nb, double* __restrict__ abar)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (g >= nb) return;
+    const double* row = phi + g * d;
+    double pp = 0;
+    for (int64_t a = lane; a < d; a += 64) pp += row[a] * row[a];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) pp += __shfl_xor(pp, off);
+    const double f = 4.0 * fabs(w[g]);
+    for (int64_t a = lane; a < d; a += 64) abar[g * d + a] = f * (pp * row[a] - v[g * d + a]);
+}
+
+// m2 = h o g elementwise (k x k), then used as m2 Phi_a
+__global__ void mul_kernel(double* __restrict__ h, const double* __restrict__ g, int64_t total)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        h[i] *= g[i];
+}
+
+// abar_a = -4 (m1 - m2)
+__global__ void anchor_adjoint_kernel(const double* __restrict__ m1, const double* __restrict__ m2, int64_t total,
+                                      double* __restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = -4.0 * (m1[i] - m2[i]);
+}
+
 void transpose(const double* in, int64_t rows, int64_t cols, const double* div, double* out, hipStream_t s)
 {
     const dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
@@ -273,6 +314,75 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         MPISDF_HIP(hipStreamSynchronize(s));
         for (int64_t t = 0; t < k; ++t) tsum_out[t] /= anchor_w[t];
     }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the fit with respect to the AO values (no reference counterpart).
+//
+// L = sum_g |w_g| || phi_g phi_g^T - sum_k c_gk phi_k phi_k^T ||_F^2 at the
+// least-squares coefficients c_g = theta_phys[:, g].  Because c is optimal,
+// dL/dc = 0 and the total derivative of L is its partial derivative at fixed c
+// (envelope theorem).  With t_g = Phi_a phi_g, U = C o T (k x n), G = Phi_a Phi_a^T:
+//   dL/dphi_g = 4 |w_g| ( ||phi_g||^2 phi_g - Phi_a^T (c_g o t_g) )
+//   dL/dPhi_a = -4 ( (U W) Phi_grid - ((C W C^T) o G) Phi_a )
+// Both are k x batch or k x k GEMMs; no nao^2 pair object is formed.  The
+// AO adjoint kernel then contracts them with AO gradients recomputed on the
+// fly: gao[c][mu] = sum_points dL/dphi_mu(r) d phi_mu(r)/d r_c.
+// ---------------------------------------------------------------------------
+void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xyz, const double* xyz,
+                       const double* w, int64_t n, int64_t batch, double* loss_out, double* gao_out, hipStream_t s)
+{
+    const int64_t k = f.k, d = f.d, nao = ctx.nao;
+    batch = std::max<int64_t>(1, std::min(batch, std::max<int64_t>(n, 1)));
+    auto blocks = [](int64_t work) { return dim3((unsigned)grid_for(work)); };
+    DeviceBuffer<double> phi((size_t)(batch * d)), tm((size_t)(k * batch)), sm((size_t)(k * batch)),
+        z((size_t)(k * batch)), cm((size_t)(k * batch)), um((size_t)(k * batch)), vt((size_t)(batch * d)),
+        abar((size_t)(std::max(batch, k) * d)), acc(2), gao((size_t)(3 * nao)), m1((size_t)(k * d)),
+        cwc((size_t)(k * k)), gm((size_t)(k * k)), m2((size_t)(k * d)), phiat((size_t)(d * k));
+    MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
+    MPISDF_HIP(hipMemsetAsync(gao.ptr, 0, 3 * nao * sizeof(double), s));
+    MPISDF_HIP(hipMemsetAsync(m1.ptr, 0, (size_t)(k * d) * sizeof(double), s));
+    MPISDF_HIP(hipMemsetAsync(cwc.ptr, 0, (size_t)(k * k) * sizeof(double), s));
+    transpose(f.phi_a.ptr, k, d, nullptr, phiat.ptr, s);  // Phi_a^T, row-major d x k
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t nb = std::min(batch, n - start);
+        const double* wb = w + start;
+        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
+        // T = Phi_a Phi_b^T, S = T o T, Z = L^-1 S, C = L^-T Z  (k x nb column-major)
+        gemm_tn_f64(k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, 0.0, tm.ptr, k, s);
+        hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, tm.ptr, tm.ptr,
+                           (const double*)nullptr, k, k * nb, sm.ptr);
+        gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s, 1);
+        hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
+                           wb, nb, acc.ptr);
+        gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, cm.ptr, k, s, 2);
+        // Grid adjoint: V = Phi_a^T (C o T) stored [g][mu]; abar_g = 4|w|(||phi||^2 phi - V)
+        hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, cm.ptr, tm.ptr,
+                           (const double*)nullptr, k, k * nb, um.ptr);
+        gemm_tn_f64(d, nb, k, 1.0, phiat.ptr, k, um.ptr, k, 0.0, vt.ptr, d, s);
+        hipLaunchKernelGGL(grid_adjoint_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, phi.ptr, vt.ptr, d,
+                           wb, nb, abar.ptr);
+        launch_ao_adjoint(ctx, xyz + 3 * start, nb, abar.ptr, d, gao.ptr, s);
+        // Anchor terms: M1 += (U W) Phi_b, CWC^T += (C W) C^T
+        hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, um.ptr,
+                           (const double*)nullptr, wb, k, k * nb, um.ptr);
+        gemm_f64(k, d, nb, 1.0, MatView{um.ptr, 1, k}, MatView{phi.ptr, d, 1}, 1.0, MutView{m1.ptr, d, 1}, s);
+        hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, cm.ptr,
+                           (const double*)nullptr, wb, k, k * nb, tm.ptr);
+        gemm_f64(k, k, nb, 1.0, MatView{tm.ptr, 1, k}, MatView{cm.ptr, k, 1}, 1.0, MutView{cwc.ptr, k, 1}, s);
+        MPISDF_HIP(hipGetLastError());
+    }
+    // dL/dPhi_a = -4 (M1 - ((C W C^T) o G) Phi_a)
+    gemm_tn_f64(k, k, d, 1.0, f.phi_a.ptr, d, f.phi_a.ptr, d, 0.0, gm.ptr, k, s);  // G, symmetric
+    hipLaunchKernelGGL(mul_kernel, blocks(k * k), dim3(256), 0, s, cwc.ptr, gm.ptr, k * k);
+    gemm_f64(k, d, k, 1.0, MatView{cwc.ptr, k, 1}, MatView{f.phi_a.ptr, d, 1}, 0.0, MutView{m2.ptr, d, 1}, s);
+    hipLaunchKernelGGL(anchor_adjoint_kernel, blocks(k * d), dim3(256), 0, s, m1.ptr, m2.ptr, k * d, abar.ptr);
+    MPISDF_HIP(hipGetLastError());
+    launch_ao_adjoint(ctx, anchor_xyz, k, abar.ptr, d, gao.ptr, s);
+    const auto h = acc.download(s);
+    if (loss_out) *loss_out = h[0];
+    if (gao_out) MPISDF_HIP(hipMemcpyAsync(gao_out, gao.ptr, 3 * nao * sizeof(double), hipMemcpyDeviceToHost, s));
+    MPISDF_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace mpisdf

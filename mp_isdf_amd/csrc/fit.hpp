@@ -26,4 +26,13 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
 void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const double* xyz, const double* w, int64_t n,
                 int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s);
 
+// Pair-reconstruction loss L = sum_g |w_g| ||phi_g phi_g^T - sum_k c_gk phi_k phi_k^T||_F^2
+// at the fitted coefficients (= the squared l2 pair-space error) and its
+// adjoint contracted with AO gradients: gao_out (host, [3][nao]) =
+// sum over grid and anchor points of dL/dphi_mu(r) d phi_mu(r)/d r_c.  The
+// nuclear gradient is dL/dR_A = -sum_{mu on A} gao[:, mu] (grid and anchor
+// points held fixed).
+void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xyz, const double* xyz,
+                       const double* w, int64_t n, int64_t batch, double* loss_out, double* gao_out, hipStream_t s);
+
 }  // namespace mpisdf

@@ -13,6 +13,11 @@ struct Context;
 void launch_ao_eval(Context& ctx, const double* xyz, int64_t n, AoMode mode, DType dtype,
                     void* out, int64_t ld, int64_t cstride, hipStream_t stream);
 
+// Adjoint of gradient evaluation (FP64): gout[c * nao + mu] += sum_p
+// abar[p * ldab + mu] * d phi_mu(r_p) / d r_c, gradients recomputed on the fly.
+void launch_ao_adjoint(Context& ctx, const double* xyz, int64_t n, const double* abar, int64_t ldab, double* gout,
+                       hipStream_t stream);
+
 // One axis block of the concatenated feature matrix.
 struct AxisSegment {
     int kind;        // 0 AO, 1 dX, 2 dY, 3 dZ, 4 occupied

@@ -74,9 +74,11 @@ public:
         int64_t n = 0;
         check(mpisdf_context_nao(raw, &n));
         nao_ = n;
+        natm_ = atm.shape(0);
         cart_ = cart;
     }
     int64_t nao() const { return nao_; }
+    int64_t natm() const { return natm_; }
     bool cart() const { return cart_; }
     const mpisdf_context* get() const { return handle_.get(); }
 
@@ -103,7 +105,7 @@ public:
 
 private:
     std::shared_ptr<mpisdf_context> handle_;
-    int64_t nao_ = 0;
+    int64_t nao_ = 0, natm_ = 0;
     bool cart_ = false;
 };
 
@@ -343,6 +345,56 @@ PYBIND11_MODULE(mp_isdf, m)
        py::arg("weight_sums") = false,
        "GPU ISDF fit on anchors (normal equations of the anchor-pair least squares, FP64 Cholesky): "
        "pair-space error, optional interpolation vectors theta[k,n] and sum_g theta[:,g] w_g.");
+
+    m.def("isdf_fit_gradient", [](const Context& ctx, const FloatArray& anchor_coords,
+                                  const FloatArray& anchor_weights, const FloatArray& coords,
+                                  const FloatArray& weights, int64_t batch_size) {
+        if (anchor_coords.ndim() != 2 || anchor_coords.shape(1) != 3 || anchor_weights.ndim() != 1 ||
+            anchor_weights.shape(0) != anchor_coords.shape(0))
+            throw py::value_error("expected anchor_coords[k,3] and anchor_weights[k]");
+        if (coords.ndim() != 2 || coords.shape(1) != 3 || weights.ndim() != 1 || weights.shape(0) != coords.shape(0))
+            throw py::value_error("expected coords[n,3] and weights[n]");
+        const py::ssize_t k = anchor_coords.shape(0), n = coords.shape(0);
+        int64_t nao = 0;
+        check(mpisdf_context_nao(ctx.get(), &nao));
+        const py::ssize_t natm = (py::ssize_t)ctx.natm();
+        py::array_t<double> gao({(py::ssize_t)3, (py::ssize_t)nao}), gat({natm, (py::ssize_t)3});
+        double loss = 0;
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_isdf_fit_gradient(ctx.get(), anchor_coords.data(), anchor_weights.data(), k,
+                                              coords.data(), weights.data(), n, batch_size, &loss,
+                                              gao.mutable_data(), gat.mutable_data());
+        }
+        check(status);
+        py::dict out;
+        out["loss"] = loss;
+        out["grad_ao"] = gao;
+        out["grad_atoms"] = gat;
+        return out;
+    }, py::arg("context"), py::arg("anchor_coords"), py::arg("anchor_weights"), py::arg("coords"),
+       py::arg("weights"), py::arg("batch_size") = 4096,
+       "Pair-reconstruction loss of the FP64 fit (sum_g |w_g| ||pair_g - fit_g||^2) and its gradient with "
+       "respect to the nuclear coordinates (grid and anchor points fixed), by the hand-written adjoint.");
+
+    m.def("ao_adjoint", [](const Context& ctx, const FloatArray& coords, const FloatArray& abar) {
+        if (coords.ndim() != 2 || coords.shape(1) != 3) throw py::value_error("expected coords[n,3]");
+        int64_t nao = 0;
+        check(mpisdf_context_nao(ctx.get(), &nao));
+        const py::ssize_t n = coords.shape(0);
+        if (abar.ndim() != 2 || abar.shape(0) != n || abar.shape(1) != nao)
+            throw py::value_error("expected abar[n,nao]");
+        py::array_t<double> gout({(py::ssize_t)3, (py::ssize_t)nao});
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_ao_adjoint_host(ctx.get(), coords.data(), n, abar.data(), gout.mutable_data());
+        }
+        check(status);
+        return gout;
+    }, py::arg("context"), py::arg("coords"), py::arg("abar"),
+       "sum_p abar[p, mu] d phi_mu(r_p) / d r (shape [3, nao]), AO gradients recomputed on the GPU.");
 
     py::class_<Selector>(m, "AnchorSelection")
         .def(py::init<const Context&>(), py::arg("context"), py::keep_alive<1, 2>())
