@@ -165,6 +165,16 @@ def main():
                 "traffic": None, "launches": fold_n, "launch_ms": round(fold_s / fold_n * 1e3, 3),
                 "flops_per_launch": fold_f / fold_n,
                 "work": f"2*b*ell*n per fold, b = pivots folded (mean {k / fold_n:.1f}), ell={ell}, n={n}"}
+    # HBM bytes per fold launch from the committed counter passes (FETCH_SIZE and
+    # WRITE_SIZE, calibrated on a known-size copy) when they cover this workload.
+    traffic_file = ROOT / "profiles" / "hbm_traffic_h2o64_rand_sketch.json"
+    if traffic_file.exists() and args.config == "h2o64" and args.precision == "bf16":
+        measured = json.loads(traffic_file.read_text())["qrcp_fold"]
+        roofline["traffic"] = round(measured["traffic_bytes_per_launch"] / 1e9, 2)
+        roofline["traffic_unit"] = "GB per launch"
+        roofline["traffic_source"] = ("profiles/hbm_traffic_h2o64_rand_sketch.json (rocprofv3 --pmc FETCH_SIZE, "
+                                      "WRITE_SIZE; algorithmic "
+                                      f"{measured['algorithmic_bytes_per_launch'] / 1e9:.1f} GB per launch)")
     sk_s, sk_f, sk_n = d.get("sketch_kernel_seconds", 0.0), d.get("sketch_flops", 0.0), max(1, int(d.get("sketch_launches", 0)))
     peak_sk = PEAK["fp64_mfma_tflops" if args.precision == "fp64" else
                    ("bf16_mfma_tflops" if args.precision == "bf16" else "fp8_mfma_tflops")]
