@@ -113,7 +113,8 @@ int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t e
  * norm downdating inside LAPACK/MAGMA dgeqp3, cpp/src/anchor_select_qrcp.cpp:36-90):
  * W = Qb^T Y with Qb[m x b] column-major FP64 and Y[m x n] (ldy);
  * y_f32 = 0: FP64 Y, FP64 MFMA; 1: FP32 Y, FP64 MFMA (exact products);
- * 2: FP32 Y, split-BF16 MFMA with FP64 chunk accumulation.
+ * 2: FP32 Y, split-BF16 MFMA with FP64 chunk accumulation;
+ * 3: FP32 Y, INT8 MFMA on base-128 digit splits (~1e-10 relative; m < 65536).
  * proj2[j] += sum_t W[t,j]^2 and, if energy != NULL, energy[t] += sum_j W[t,j]^2.
  * W is never stored. */
 int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, int y_f32,

@@ -4,6 +4,7 @@
 #include "axis.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
+#include "util.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -290,9 +291,23 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
                                        int64_t ldy, double* proj2, double* energy, void* stream)
 {
     return guard([&] {
-        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 2)
+        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 3)
             throw std::invalid_argument("invalid fold arguments");
-        if (y_f32 == 2) {
+        if (y_f32 == 3) {
+            // Digit planes of a copy of Y (pitch P = m rounded up to 16), then the INT8 fold.
+            mpisdf::FoldI8Scratch scratch;
+            auto st = static_cast<hipStream_t>(stream);
+            const int64_t P = (m + 15) / 16 * 16;
+            mpisdf::DeviceBuffer<double> yn(std::max<int64_t>(n, 1)), sc(std::max<int64_t>(n, 1));
+            mpisdf::DeviceBuffer<float> yd((size_t)std::max<int64_t>(n * P, 1));
+            mpisdf::launch_column_norms2_f32(static_cast<const float*>(Y), m, n, ldy, yn.ptr, st);
+            MPISDF_HIP(hipMemcpy2DAsync(yd.ptr, P * sizeof(float), Y, ldy * sizeof(float), m * sizeof(float), n,
+                                        hipMemcpyDeviceToDevice, st));
+            mpisdf::digitize_columns(yd.ptr, m, n, P, yn.ptr, sc.ptr, st);
+            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, sc.ptr, proj2,
+                                        energy, scratch, st);
+            MPISDF_HIP(hipStreamSynchronize(st));
+        } else if (y_f32 == 2) {
             mpisdf::FoldScratch scratch;
             mpisdf::fold_projections_bf16x3(b, n, m, Qb, m, static_cast<const float*>(Y), ldy, proj2, energy, scratch,
                                             static_cast<hipStream_t>(stream));

@@ -82,4 +82,23 @@ void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, 
                              double* proj2, double* energy, FoldScratch& scratch, hipStream_t stream,
                              const double* ynorm2 = nullptr, double* err2 = nullptr);
 
+// Sketch in digit-plane form (fold_i8.hip): in place, FP32 column j
+// (pitch 4 P bytes, P a multiple of 16 >= m) becomes four signed base-128
+// digit planes y_hat = scale[j] sum_r d_r 128^-(r+1); ynorm2[j] (in: of the
+// FP32 column, out: of y_hat).  m <= 36864.
+void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale, hipStream_t stream);
+
+// QRCP fold on the digit planes with INT8 MFMA (digit-pair products in exact
+// INT32, combined in FP64); m < 65536.
+struct FoldI8Scratch {
+    DeviceBuffer<int8_t> digits;
+    DeviceBuffer<double> scales;
+};
+// Relative bound (of ||y_j||^2) on the error a folded sum accumulates over a
+// whole factorisation with the INT8 fold; enters the keys as a margin.
+constexpr double kFoldI8Margin = 1e-9;
+void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const int8_t* Yd,
+                         int64_t P, const double* yscale, double* proj2, double* energy, FoldI8Scratch& scratch,
+                         hipStream_t stream);
+
 }  // namespace mpisdf

@@ -54,4 +54,29 @@ void launch_colsumsq(const double* f, int64_t n, int64_t w, double* out, hipStre
     MPISDF_HIP(hipGetLastError());
 }
 
+namespace {
+__global__ void column_norms2_f32_kernel(const float* __restrict__ y, int64_t m, int64_t n, int64_t ldy,
+                                         double* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (j >= n) return;
+    double s = 0;
+    for (int64_t i = lane; i < m; i += 64) {
+        const double v = (double)y[j * ldy + i];
+        s += v * v;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) out[j] = s;
+}
+}  // namespace
+
+void launch_column_norms2_f32(const float* y, int64_t m, int64_t n, int64_t ldy, double* out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(column_norms2_f32_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, y, m, n, ldy, out);
+    MPISDF_HIP(hipGetLastError());
+}
+
 }  // namespace mpisdf

@@ -234,6 +234,8 @@ std::vector<int64_t> select_online(const Tables& t, const double* points, const 
         int64_t w = 0;
         const auto f = features(t, xyz.data(), (int64_t)ids.size(), scales, w);
         std::vector<double> a((size_t)(ids.size() * m));
+        // Columns are independent: threads change nothing in any column's arithmetic.
+#pragma omp parallel for schedule(static)
         for (size_t g = 0; g < ids.size(); ++g) {
             const double sw = std::sqrt(std::abs(weights[ids[g]]));
             for (int64_t p = 0; p < m; ++p)
@@ -255,6 +257,8 @@ std::vector<int64_t> select_online(const Tables& t, const double* points, const 
     };
     auto norms = [&](const std::vector<double>& a, int64_t count, int64_t first) {
         std::vector<double> r(count);
+        bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
         for (int64_t g = 0; g < count; ++g) {
             double scale = 0, ssq = 1;
             for (int64_t i = first; i < m; ++i) {
@@ -265,8 +269,9 @@ std::vector<int64_t> select_online(const Tables& t, const double* points, const 
                 }
             }
             r[g] = scale * std::sqrt(ssq);
-            if (!std::isfinite(r[g])) throw std::overflow_error("non-finite online AO pair column");
+            bad = bad || !std::isfinite(r[g]);
         }
+        if (bad) throw std::overflow_error("non-finite online AO pair column");
         return r;
     };
     std::vector<double> estimated(n), reference(n);

@@ -162,3 +162,24 @@ def test_online_water_matches_oracle(mp_isdf):
     assert got == want
     dense = sel.select(pts, w, mp_isdf.AxisType.AtomicOrbital, max_rank=40)
     assert dense == oracle.select(mol.atm, mol.bas, mol.env, pts, w, "ao", max_rank=40)
+
+
+def test_online_water8_ccpvdz_bit_identical(mp_isdf):
+    """The (H2O)8 / cc-pVDZ (192 AOs) level-3 grid workload with qrcp_online in
+    16k-point batches and rank 8: anchors bit-identical to the oracle's
+    restatement of anchor_select_online_qr.cpp (precomputed by
+    tests/golden/make_online_water8.py; the grid checksum must match)."""
+    import json
+    from pathlib import Path
+    from mp_isdf_amd import grid as gridmod
+    fixture = json.loads((Path(__file__).resolve().parent / "golden" / "online_water8.json").read_text())
+    mol = molecule.water_cluster(8, "cc-pvdz")
+    assert mol.nao == 192
+    xyz, w = gridmod.make_grid(mol, 3)
+    assert len(w) == fixture["grid_points"]
+    np.testing.assert_allclose([np.sum(xyz), np.sum(w)], [fixture["grid_sum_xyz"], fixture["grid_sum_w"]],
+                               rtol=1e-12)
+    sel = mp_isdf.AnchorSelection(mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env))
+    got = sel.select(xyz, w, mp_isdf.AxisType.AtomicOrbital, max_rank=8, batch_size=16384,
+                     algorithm=mp_isdf.LocalAnchorSelectionAlgorithm.OnlineQRColumnPivoting)
+    assert got == fixture["anchors"]

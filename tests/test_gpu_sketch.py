@@ -82,10 +82,11 @@ def test_lowp_selection_is_valid(mp_isdf):
         assert not set(got) & set(np.flatnonzero(w == 0).tolist())
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_qrcp_fold_modes(mp_isdf, mode):
     """Fold = column / row sums of (Q^T Y)^2 against NumPy; the split-BF16
-    mode is checked against its stated bound (1e-5 of ||y_j||^2)."""
+    mode is checked against its stated bound (1e-5 of ||y_j||^2), the INT8
+    mode (3) against the fold of the four-digit columns it factorises."""
     import torch
     rng = np.random.default_rng(3)
     m, n, b = 1003, 3001, 150
@@ -105,6 +106,23 @@ def test_qrcp_fold_modes(mp_isdf, mode):
     capi.call("mpisdf_qrcp_fold_device", b, n, m, Qb, Y, mode, ld, proj, energy, None)
     torch.cuda.synchronize()
     ynorm2 = np.sum(y ** 2, axis=1)
-    tol = 1e-12 if mode < 2 else 1e-5
+    if mode == 3:
+        # The INT8 fold works on the four-digit form of each FP32 column:
+        # s = 2^(e+1) with ||y|| < 2^e, y_hat = s sum_r d_r 128^-(r+1).
+        scale = np.exp2(np.frexp(np.sqrt(ynorm2))[1] + 1.0)
+        x = (y / scale[:, None]).astype(np.float32)
+        y_hat = np.zeros_like(y)
+        weight = 1.0
+        for _ in range(4):
+            x = (x * np.float32(128)).astype(np.float32)
+            dgt = np.rint(x)
+            x = (x - dgt).astype(np.float32)
+            weight /= 128.0
+            y_hat += dgt.astype(np.float64) * weight
+        y_hat *= scale[:, None]
+        assert np.max(np.abs(y_hat - y)) <= 2.0 ** -29 * scale.max()
+        w_hat = q.T @ y_hat.T
+        want_cols, want_rows = np.sum(w_hat ** 2, axis=0), np.sum(w_hat ** 2, axis=1)
+    tol = {0: 1e-12, 1: 1e-12, 2: 1e-5, 3: 1e-10}[mode]
     assert np.max(np.abs(proj.cpu().numpy() - want_cols) / ynorm2) < tol
     np.testing.assert_allclose(energy.cpu().numpy(), want_rows, rtol=max(tol, 1e-12))
