@@ -36,6 +36,24 @@ __global__ void square_kernel(double* __restrict__ a, int64_t count)
         a[i] = a[i] * a[i];
 }
 
+__global__ void diagonal_kernel(const double* __restrict__ a, int64_t lda, int64_t k, double* __restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = a[i * lda + i];
+}
+
+// The diagonal of a row-major k x k matrix, gathered on the device and read
+// back with one contiguous copy.  (A strided device-to-host 2-D copy into
+// pageable memory has returned before its data landed when the runtime is
+// traced, so no strided copy ends in host memory here.)
+std::vector<double> diagonal(const double* a, int64_t k, hipStream_t s)
+{
+    DeviceBuffer<double> dg((size_t)k);
+    hipLaunchKernelGGL(diagonal_kernel, dim3(grid_for(k)), dim3(256), 0, s, a, k, k, dg.ptr);
+    MPISDF_HIP(hipGetLastError());
+    return dg.download(s);
+}
+
 // Factor the jb x jb diagonal block in place (lower Cholesky) and write its
 // inverse (lower, row-major jb x jb) to ``inv``.  A non-positive pivot sets
 // *info to the 1-based global row.
@@ -200,12 +218,10 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
     MPISDF_HIP(hipGetLastError());
     // Diagonal of G2 gives the pair column norms; keep the max for diagnostics.
     {
-        std::vector<double> diag(k);
-        MPISDF_HIP(hipMemcpy2DAsync(diag.data(), sizeof(double), a.ptr, (k + 1) * sizeof(double), sizeof(double), k,
-                                    hipMemcpyDeviceToHost, s));
-        MPISDF_HIP(hipStreamSynchronize(s));
+        const std::vector<double> diag = diagonal(a.ptr, k, s);
         f.gram_diag_max = *std::max_element(diag.begin(), diag.end());
         gdiag.upload(diag.data(), k, s);
+        MPISDF_HIP(hipStreamSynchronize(s));
     }
     // Blocked right-looking Cholesky, lower, in place.
     DeviceBuffer<int> info(1);
@@ -254,9 +270,7 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
     }
     f.linvt.resize((size_t)(k * k));
     transpose(f.linv.ptr, k, k, nullptr, f.linvt.ptr, s);
-    MPISDF_HIP(hipMemcpy2DAsync(ldiag.data(), sizeof(double), a.ptr, (k + 1) * sizeof(double), sizeof(double), k,
-                                hipMemcpyDeviceToHost, s));
-    MPISDF_HIP(hipStreamSynchronize(s));
+    ldiag = diagonal(a.ptr, k, s);
     double lo = ldiag[0], hi = ldiag[0];
     for (double v : ldiag) { lo = std::min(lo, v); hi = std::max(hi, v); }
     // cond(G2) estimate from the factor's diagonal (a lower bound of the true condition)
@@ -270,6 +284,7 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     batch = std::max<int64_t>(1, std::min(batch, n));
     DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
     DeviceBuffer<double> th, thr, tsum;
+    std::vector<double> stage;
     wa.upload(anchor_w, k, s);
     if (tsum_out || theta_host) th.resize((size_t)(k * batch));
     if (theta_host) thr.resize((size_t)(k * batch));
@@ -296,10 +311,19 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
             if (tsum_out)  // t_phys += theta_phys w_grid (divided by w_anchor at the end)
                 gemm_f64(k, 1, nb, 1.0, MatView{th.ptr, 1, k}, MatView{w + start, 1, 1}, 1.0, MutView{tsum.ptr, 1, 1},
                          s);
-            if (theta_host) {  // rows of theta[k][n] on the host
+            if (theta_host) {  // rows of theta[k][n] on the host, through a contiguous staging copy
                 transpose(th.ptr, nb, k, wa.ptr, thr.ptr, s);
-                MPISDF_HIP(hipMemcpy2DAsync(theta_host + start, n * sizeof(double), thr.ptr, nb * sizeof(double),
-                                            nb * sizeof(double), k, hipMemcpyDeviceToHost, s));
+                if (nb == n) {
+                    MPISDF_HIP(hipMemcpyAsync(theta_host, thr.ptr, (size_t)(k * n) * sizeof(double),
+                                              hipMemcpyDeviceToHost, s));
+                } else {
+                    stage.resize((size_t)(k * nb));
+                    MPISDF_HIP(hipMemcpyAsync(stage.data(), thr.ptr, (size_t)(k * nb) * sizeof(double),
+                                              hipMemcpyDeviceToHost, s));
+                    MPISDF_HIP(hipStreamSynchronize(s));
+                    for (int64_t t = 0; t < k; ++t)
+                        std::copy(stage.data() + t * nb, stage.data() + (t + 1) * nb, theta_host + t * n + start);
+                }
             }
         }
         MPISDF_HIP(hipGetLastError());
