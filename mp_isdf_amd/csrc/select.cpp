@@ -18,10 +18,12 @@
 #include "util.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <iostream>
 #include <limits>
+#include <memory>
 #include <random>
 #include <thread>
 
@@ -359,19 +361,99 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
 
 }  // namespace
 
+namespace {
+
+// Raw mt19937_64 outputs, replayed in order while a producer thread is still
+// recording them (a read waits until its draw is published).  Past the
+// recorded range it returns 3/4 of the range, a draw the polar method accepts,
+// so a call that runs off the end terminates (its value is discarded).
+struct ReplayUrng {
+    using result_type = std::mt19937_64::result_type;
+    const result_type* raw;
+    size_t size, pos;
+    const std::atomic<size_t>* ready;
+    static constexpr result_type min() { return std::mt19937_64::min(); }
+    static constexpr result_type max() { return std::mt19937_64::max(); }
+    result_type operator()()
+    {
+        const size_t i = pos++;
+        if (i >= size) return result_type(3) << 62;
+        while (ready->load(std::memory_order_acquire) <= i) std::this_thread::yield();
+        return raw[i];
+    }
+};
+
+// out[0 .. count) = the first count values of std::normal_distribution<double>(0, 1)
+// over std::mt19937_64(seed), exactly as one sequential loop would draw them.
+//
+// libstdc++ draws normals by the polar method: every attempt consumes two
+// uniforms (one raw 64-bit draw each for mt19937_64 and double), and each
+// accepted attempt yields two values, the second held for the next call.
+// Attempts are independent, so the calling thread records the raw stream
+// (the only sequential part) while workers run the expensive part (log,
+// sqrt) on chunks of attempts: a fresh distribution replaying from an attempt
+// boundary yields exactly the values of the accepted attempts that start
+// inside its chunk.
+void fill_normal(uint64_t seed, int64_t count, double* out, int workers)
+{
+    std::mt19937_64 gen(seed);
+    int64_t produced = 0;
+    while (produced < count) {
+        const int64_t want = (count - produced + 1) / 2;  // accepted attempts still needed
+        const int64_t attempts = want + want / 3 + 4096;  // acceptance is pi/4 ~ 0.785
+        const size_t nraw = (size_t)(2 * attempts);
+        std::unique_ptr<std::mt19937_64::result_type[]> raw(new std::mt19937_64::result_type[nraw]);
+        std::atomic<size_t> ready{0};
+        constexpr int64_t chunk = 1 << 16;
+        const int64_t nchunks = (attempts + chunk - 1) / chunk;
+        std::vector<std::vector<double>> parts((size_t)nchunks);
+        std::atomic<int64_t> next{0};
+        auto work = [&]() {
+            for (int64_t c; (c = next.fetch_add(1)) < nchunks;) {
+                const int64_t a0 = c * chunk, a1 = std::min(attempts, a0 + chunk);
+                ReplayUrng g{raw.get(), nraw, (size_t)(2 * a0), &ready};
+                std::normal_distribution<double> normal(0.0, 1.0);
+                auto& o = parts[(size_t)c];
+                o.reserve((size_t)((a1 - a0) * 8 / 5));
+                for (;;) {
+                    const double first = normal(g);
+                    const int64_t accepted = (int64_t)(g.pos / 2) - 1;  // the attempt that produced it
+                    if (accepted >= a1) break;
+                    o.push_back(first);
+                    o.push_back(normal(g));  // held value: consumes no draw
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int t = 0; t < std::max(1, workers); ++t) pool.emplace_back(work);
+        constexpr size_t publish = 1 << 14;
+        for (size_t i = 0; i < nraw; i += publish) {
+            const size_t e = std::min(nraw, i + publish);
+            for (size_t j = i; j < e; ++j) raw[j] = gen();
+            ready.store(e, std::memory_order_release);
+        }
+        for (auto& t : pool) t.join();
+        for (const auto& o : parts) {
+            const int64_t take = std::min<int64_t>((int64_t)o.size(), count - produced);
+            std::copy(o.begin(), o.begin() + take, out + produced);
+            produced += take;
+            if (produced == count) break;
+        }
+    }
+}
+
+}  // namespace
+
 void make_omega(int64_t d, int64_t ell, uint64_t seed, std::vector<double>& u, std::vector<double>& v)
 {
-    // Two independent serial streams (each must replay its generator in
-    // order), drawn on two threads.
+    // Two independent streams, each recorded by one thread and expanded by
+    // half of the remaining host threads (at most 16 in all).
     u.resize((size_t)(d * ell));
     v.resize((size_t)(d * ell));
-    auto fill = [d, ell](std::vector<double>& out, uint64_t s) {
-        std::mt19937_64 gen(s);
-        std::normal_distribution<double> normal(0, 1);
-        for (int64_t i = 0; i < d * ell; ++i) out[i] = normal(gen);
-    };
-    std::thread tv(fill, std::ref(v), seed ^ 0x9e3779b97f4a7c15ULL);
-    fill(u, seed);
+    const int threads = (int)std::max(4u, std::min(16u, std::thread::hardware_concurrency()));
+    const int workers = (threads - 2) / 2;
+    std::thread tv(fill_normal, seed ^ 0x9e3779b97f4a7c15ULL, d * ell, v.data(), workers);
+    fill_normal(seed, d * ell, u.data(), workers);
     tv.join();
 }
 
