@@ -32,7 +32,12 @@ CONFIGS = {
     "h2o32": (32, "cc-pvdz", 3, 6144),
     "h2o64": (64, "cc-pvdz", 3, 12000),
 }
-PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "hbm_gbs": 8000.0}
+PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "int8_mfma_tops": 5000.0,
+        "hbm_gbs": 8000.0}
+# The INT8 fold splits q into 5 and y into 4 signed base-128 digits and runs
+# the 17 digit pairs with p + r <= 5 as INT8 products.
+I8_PAIRS = 17
+FIT_BATCH = 8192  # grid points per fit batch
 
 
 def log(msg):
@@ -96,7 +101,7 @@ def main():
         aw = np.ascontiguousarray(w[anchors])
         res = np.zeros(8)
         tsum = np.zeros(len(anchors))
-        capi.call("mpisdf_isdf_fit_device", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev, n, 8192, tsum, res,
+        capi.call("mpisdf_isdf_fit_device", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev, n, FIT_BATCH, tsum, res,
                   None)
         return res
 
@@ -110,6 +115,8 @@ def main():
         d["seconds_select"] = t_sel
         d["seconds_fit_factor"] = res[4]
         d["seconds_fit_grid"] = res[5]
+        d["fit_gemm_seconds"] = res[6]
+        d["fit_gemm_flops"] = res[7]
         d["pair_space_relative_error"] = float(res[1])
         d["gram_condition_estimate"] = float(res[2])
         return anchors, d
@@ -156,25 +163,35 @@ def main():
         log(f"fp64 form: overlap {overlap:.3f}; pair error {e64:.4e} vs {e:.4e}")
 
     # ---- dominant kernels of the timed step, from its own HIP-event timings ----
-    y_type = "FP64" if args.precision == "fp64" else "FP32"
     fold_s, fold_f, fold_n = d.get("fold_seconds", 0.0), d.get("fold_flops", 0.0), max(1, int(d.get("fold_launches", 0)))
-    roofline = {"kernel": f"tn_gemm_kernel fold (QRCP fold on {y_type} Y)", "bound": "mfma",
-                "achieved": round(fold_f / fold_s / 1e12, 2) if fold_s else None,
-                "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
-                "frac": round(fold_f / fold_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if fold_s else None,
-                "traffic": None, "launches": fold_n, "launch_ms": round(fold_s / fold_n * 1e3, 3),
-                "flops_per_launch": fold_f / fold_n,
-                "work": f"2*b*ell*n per fold, b = pivots folded (mean {k / fold_n:.1f}), ell={ell}, n={n}"}
-    # HBM bytes per fold launch from the committed counter passes (FETCH_SIZE and
-    # WRITE_SIZE, calibrated on a known-size copy) when they cover this workload.
-    traffic_file = ROOT / "profiles" / "hbm_traffic_h2o64_rand_sketch.json"
-    if traffic_file.exists() and args.config == "h2o64" and args.precision == "bf16":
-        measured = json.loads(traffic_file.read_text())["qrcp_fold"]
-        roofline["traffic"] = round(measured["traffic_bytes_per_launch"] / 1e9, 2)
-        roofline["traffic_unit"] = "GB per launch"
-        roofline["traffic_source"] = ("profiles/hbm_traffic_h2o64_rand_sketch.json (rocprofv3 --pmc FETCH_SIZE, "
-                                      "WRITE_SIZE; algorithmic "
-                                      f"{measured['algorithmic_bytes_per_launch'] / 1e9:.1f} GB per launch)")
+    fold_kind = int(d.get("fold_kind", 0))
+    fold_work = f"2*b*ell*n per fold, b = pivots folded (mean {k / fold_n:.1f}), ell={ell}, n={n}"
+    if fold_kind == 2:
+        ops = I8_PAIRS * fold_f
+        roofline_fold = {"kernel": "fold_i8_kernel (QRCP fold on INT8 digit planes of the FP32 sketch)",
+                         "bound": "mfma", "achieved": round(ops / fold_s / 1e12, 2) if fold_s else None,
+                         "peak": PEAK["int8_mfma_tops"], "unit": "TOP/s",
+                         "frac": round(ops / fold_s / 1e12 / PEAK["int8_mfma_tops"], 4) if fold_s else None,
+                         "fp64_equivalent_tflops": round(fold_f / fold_s / 1e12, 2) if fold_s else None,
+                         "launches": fold_n, "launch_ms": round(fold_s / fold_n * 1e3, 3),
+                         "ops_per_launch": ops / fold_n,
+                         "work": f"{I8_PAIRS} INT8 digit-pair products of " + fold_work}
+    else:
+        y_type = "FP64" if fold_kind == 0 else "FP32"
+        roofline_fold = {"kernel": f"tn_gemm_kernel fold (QRCP fold on {y_type} Y)", "bound": "mfma",
+                         "achieved": round(fold_f / fold_s / 1e12, 2) if fold_s else None,
+                         "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                         "frac": round(fold_f / fold_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if fold_s else None,
+                         "launches": fold_n, "launch_ms": round(fold_s / fold_n * 1e3, 3),
+                         "flops_per_launch": fold_f / fold_n, "work": fold_work}
+    fit_s, fit_f = d.get("fit_gemm_seconds", 0.0), d.get("fit_gemm_flops", 0.0)
+    fit_n = 3 * ((n + FIT_BATCH - 1) // FIT_BATCH)
+    roofline_fit = {"kernel": "tn_gemm_kernel (fit grid pass: S = (Phi_a Phi^T)^2, Z = L^-1 S, theta = L^-T Z)",
+                    "bound": "mfma", "achieved": round(fit_f / fit_s / 1e12, 2) if fit_s else None,
+                    "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                    "frac": round(fit_f / fit_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if fit_s else None,
+                    "launches": fit_n, "launch_ms": round(fit_s / fit_n * 1e3, 3), "flops_per_launch": fit_f / fit_n,
+                    "work": f"(2*k*d + 2*k*(k+1)) per grid point, k={k}, d={mol.nao}, n={n}, {FIT_BATCH}-point batches"}
     sk_s, sk_f, sk_n = d.get("sketch_kernel_seconds", 0.0), d.get("sketch_flops", 0.0), max(1, int(d.get("sketch_launches", 0)))
     peak_sk = PEAK["fp64_mfma_tflops" if args.precision == "fp64" else
                    ("bf16_mfma_tflops" if args.precision == "bf16" else "fp8_mfma_tflops")]
@@ -183,6 +200,23 @@ def main():
                        "unit": "TFLOP/s", "frac": round(sk_f / sk_s / 1e12 / peak_sk, 4) if sk_s else None,
                        "launches": sk_n, "launch_ms": round(sk_s / sk_n * 1e3, 3),
                        "work": f"4*ell*d*n = 4*{ell}*{mol.nao}*{n} over {sk_n} grid batches"}
+    # HBM bytes per launch from the committed counter passes (FETCH_SIZE and
+    # WRITE_SIZE, calibrated on a known-size copy) when they cover this workload.
+    traffic_file = ROOT / "profiles" / "hbm_traffic_h2o64_rand_sketch.json"
+    if traffic_file.exists() and args.config == "h2o64" and args.precision == "bf16":
+        measured = json.loads(traffic_file.read_text())
+        for key, entry in (("qrcp_fold", roofline_fold), ("fit_grid_gemm", roofline_fit), ("bf16_sketch", roofline_sketch)):
+            if key in measured and measured[key].get("kernel_tag") in entry["kernel"]:
+                entry["traffic"] = round(measured[key]["traffic_bytes_per_launch"] / 1e9, 3)
+                entry["traffic_unit"] = "GB per launch"
+                entry["traffic_source"] = (f"{traffic_file.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
+                                           f"algorithmic {measured[key]['algorithmic_bytes_per_launch'] / 1e9:.3f} "
+                                           "GB per launch)")
+    for entry in (roofline_fold, roofline_fit, roofline_sketch):
+        entry.setdefault("traffic", None)
+    # The dominant kernel is the one with the largest summed duration in the step.
+    dominant = max(((fold_s, roofline_fold), (fit_s, roofline_fit), (sk_s, roofline_sketch)), key=lambda t: t[0])
+    roofline = dict(dominant[1], seconds_in_step=round(dominant[0], 3))
     u_h = np.empty((mol.nao, ell))
     v_h = np.empty((mol.nao, ell))
     capi.call("mpisdf_omega", mol.nao, ell, 7, u_h, v_h)
@@ -222,7 +256,9 @@ def main():
                         "gram_condition_estimate": d.get("gram_condition_estimate")},
         "fp64_form": fp64_form,
         "roofline": roofline,
-        "roofline_sketch": roofline_sketch,
+        "roofline_fold": dict(roofline_fold, seconds_in_step=round(fold_s, 3)),
+        "roofline_fit": dict(roofline_fit, seconds_in_step=round(fit_s, 3)),
+        "roofline_sketch": dict(roofline_sketch, seconds_in_step=round(sk_s, 3)),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -120,6 +120,30 @@ int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t e
 int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, int y_f32,
                             int64_t ldy, double* proj2, double* energy, void* stream);
 
+/* In place: FP32 column j of y (pitch 4 P bytes = P floats, P a multiple of
+ * 16 >= m, m <= 36864) becomes four signed base-128 digit planes of P bytes,
+ * y_hat_j = scale[j] sum_r d_r 128^-(r+1) with scale[j] a power of two
+ * >= 2 ||y_j||; ynorm2[j] holds ||y_j||^2 on entry and ||y_hat_j||^2 on
+ * return.  The digit form of the sketch that the reduced-precision selection
+ * factorises (no reference counterpart: the reference factorises an FP64
+ * sketch, cpp/src/anchor_select_randomized_id.cpp:320-326). */
+int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
+                                   void* stream);
+
+/* Column-pivoted QR pivots of a device-resident sketch (replaces qrcp_cpu /
+ * qrcp_magma, cpp/src/anchor_select_qrcp.cpp:36-90, as called from
+ * cpp/src/anchor_select_randomized_id.cpp:320-326): dgeqp3's greedy pivot
+ * order (max residual norm, lowest index on ties) of Y[m x n] held as
+ * ytype 0 FP64 or 1 FP32 column-major (ldy elements; ynorm2 and yscale may
+ * be NULL), or 4 digit planes from mpisdf_digitize_columns_device (ldy = P
+ * bytes per plane; ynorm2 and yscale as it wrote them; folds on INT8 MFMA).
+ * Stops before a pivot with |R_ii| <= stop_rel |R_00| or after kmax pivots.
+ * Writes 0-based pivots and |R_ii| (host arrays of kmax entries; rdiag may
+ * be NULL) and their count. */
+int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n, int64_t ldy, const double* ynorm2,
+                       const double* yscale, int64_t kmax, double stop_rel, int64_t* pivots, double* rdiag,
+                       int64_t* count, void* stream);
+
 /* ---- ISDF fit on selected anchors ------------------------------------------- */
 /* Replaces the driver refit and pair-space error of the reference
  * (python/isdf.py:238-263 _physical_pair_refit, :176-191 pair error) with the
@@ -130,7 +154,9 @@ int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, c
  * nonzero weight).  theta (optional) receives row-major [k][n]; tsum
  * (optional) sum_g theta[:, g] w_g.  results[8]: 0 l2 error, 1 relative error,
  * 2 condition estimate of G2, 3 max diagonal of G2, 4 seconds (factor),
- * 5 seconds (grid pass). */
+ * 5 seconds (grid pass), 6 summed HIP-event seconds of the grid pass's FP64
+ * TN products (S, Z = L^-1 S and, when theta is formed, theta = L^-T Z),
+ * 7 their algorithmic flops (2 k d + 2 k (k+1) per point with theta). */
 int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w, int64_t k,
                     const double* xyz, const double* w, int64_t n, int64_t batch, double* theta, double* tsum,
                     double* results);

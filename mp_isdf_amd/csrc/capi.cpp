@@ -4,6 +4,7 @@
 #include "axis.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
+#include "qrcp.hpp"
 #include "util.hpp"
 
 #include <algorithm>
@@ -318,6 +319,43 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
     });
 }
 
+extern "C" int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2,
+                                              double* scale, void* stream)
+{
+    return guard([&] {
+        if (m <= 0 || n < 0 || !y || !ynorm2 || !scale) throw std::invalid_argument("invalid digitise arguments");
+        mpisdf::digitize_columns(y, m, n, P, ynorm2, scale, static_cast<hipStream_t>(stream));
+        MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
+extern "C" int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n, int64_t ldy, const double* ynorm2,
+                                  const double* yscale, int64_t kmax, double stop_rel, int64_t* pivots,
+                                  double* rdiag, int64_t* count, void* stream)
+{
+    return guard([&] {
+        if (!y || !pivots || !count || m <= 0 || n <= 0 || kmax <= 0 || !(stop_rel >= 0))
+            throw std::invalid_argument("invalid QRCP arguments");
+        if (ytype != mpisdf::kF64 && ytype != mpisdf::kF32 && ytype != mpisdf::kI8Digits)
+            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32) or 4 (digit planes)");
+        mpisdf::QrcpOptions opt;
+        opt.kmax = kmax;
+        opt.stop_rel = stop_rel;
+        opt.candidates = std::min<int64_t>(n, 2048);
+        opt.ynorm2 = ynorm2;
+        if (ytype == mpisdf::kI8Digits) {
+            opt.yscale = yscale;
+            opt.margin = mpisdf::kFoldI8Margin;
+        }
+        const auto r = mpisdf::qrcp_device(y, static_cast<mpisdf::DType>(ytype), m, n, ldy, opt,
+                                           static_cast<hipStream_t>(stream));
+        const int64_t k = std::min<int64_t>(kmax, (int64_t)r.pivots.size());
+        std::copy(r.pivots.begin(), r.pivots.begin() + k, pivots);
+        if (rdiag) std::copy(r.rdiag.begin(), r.rdiag.begin() + k, rdiag);
+        *count = k;
+    });
+}
+
 namespace {
 
 void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,
@@ -334,7 +372,8 @@ void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anc
     mpisdf::fit_factor(c, anchor_xyz_dev, k, f, s);
     const auto t1 = std::chrono::steady_clock::now();
     double err[2] = {0, 0};
-    if (n > 0) mpisdf::fit_stream(c, f, anchor_w, xyz_dev, w_dev, n, batch, theta, err, tsum, s);
+    double gemm_stats[3] = {0, 0, 0};
+    if (n > 0) mpisdf::fit_stream(c, f, anchor_w, xyz_dev, w_dev, n, batch, theta, err, tsum, s, gemm_stats);
     MPISDF_HIP(hipStreamSynchronize(s));
     const auto t2 = std::chrono::steady_clock::now();
     if (results) {
@@ -344,7 +383,8 @@ void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anc
         results[3] = f.gram_diag_max;
         results[4] = std::chrono::duration<double>(t1 - t0).count();
         results[5] = std::chrono::duration<double>(t2 - t1).count();
-        results[6] = results[7] = 0;
+        results[6] = gemm_stats[0];
+        results[7] = gemm_stats[1];
     }
 }
 

@@ -278,9 +278,12 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
 }
 
 void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const double* xyz, const double* w, int64_t n,
-                int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s)
+                int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s,
+                double* gemm_stats)
 {
     const int64_t k = f.k, d = f.d;
+    LaunchTimer timer;
+    double gemm_flops = 0;
     batch = std::max<int64_t>(1, std::min(batch, n));
     DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
     DeviceBuffer<double> th, thr, tsum;
@@ -297,17 +300,26 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         const int64_t nb = std::min(batch, n - start);
         launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
         // S = (Phi_a Phi_b^T)^2, k x nb column-major
+        timer.begin(s);
         gemm_tn_f64(k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, 0.0, sm.ptr, k, s);
+        timer.end(s);
+        gemm_flops += 2.0 * (double)k * (double)d * (double)nb;
         hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * nb)), dim3(256), 0, s, sm.ptr, k * nb);
         // Z = L^-1 S (lower triangular A)
+        timer.begin(s);
         gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s, 1);
+        timer.end(s);
+        gemm_flops += (double)k * (double)(k + 1) * (double)nb;
         if (err_out)
             hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
                                w + start, nb, acc.ptr);
         if (theta_host || tsum_out) {
             // theta_phys = L^-T Z (upper triangular A), column-major k x nb;
             // theta = theta_phys / w_anchor
+            timer.begin(s);
             gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, th.ptr, k, s, 2);
+            timer.end(s);
+            gemm_flops += (double)k * (double)(k + 1) * (double)nb;
             if (tsum_out)  // t_phys += theta_phys w_grid (divided by w_anchor at the end)
                 gemm_f64(k, 1, nb, 1.0, MatView{th.ptr, 1, k}, MatView{w + start, 1, 1}, 1.0, MutView{tsum.ptr, 1, 1},
                          s);
@@ -337,6 +349,12 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         MPISDF_HIP(hipMemcpyAsync(tsum_out, tsum.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
         MPISDF_HIP(hipStreamSynchronize(s));
         for (int64_t t = 0; t < k; ++t) tsum_out[t] /= anchor_w[t];
+    }
+    if (gemm_stats) {
+        MPISDF_HIP(hipStreamSynchronize(s));
+        gemm_stats[0] = timer.seconds();
+        gemm_stats[1] = gemm_flops;
+        gemm_stats[2] = (double)timer.launches();
     }
 }
 

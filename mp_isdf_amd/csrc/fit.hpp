@@ -23,8 +23,13 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
 // err_out (host, 2 doubles): sum |w| (||phi||^4 - ||z||^2), sum |w| ||phi||^4.
 // theta_host (optional): row-major [k][n].  tsum_out (optional host [k]):
 // sum_g theta[:, g] w_g.
+// gemm_stats (optional, host double[3]): summed HIP-event seconds of the grid
+// pass's TN products (S, Z = L^-1 S, theta = L^-T Z), their algorithmic flops
+// (2 k d + 2 k (k+1) per point, the latter only when theta is formed) and
+// launch count.
 void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const double* xyz, const double* w, int64_t n,
-                int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s);
+                int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s,
+                double* gemm_stats = nullptr);
 
 // Pair-reconstruction loss L = sum_g |w_g| ||phi_g phi_g^T - sum_k c_gk phi_k phi_k^T||_F^2
 // at the fitted coefficients (= the squared l2 pair-space error) and its

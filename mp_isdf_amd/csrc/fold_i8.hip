@@ -24,9 +24,10 @@
 // Lane layout of v_mfma_i32_16x16x64_i8: A row l % 16 and B column l % 16,
 // each supplying 16 consecutive K bytes at offset 16 (l / 16); result element
 // e of lane l is D[4 (l / 16) + e][l % 16].  Workgroup: 4 waves x 16 columns
-// x 128 rows (8 row tiles; grid.y walks further row groups).  The Q digit
-// slice (5 planes x 128 rows x 64 K) is double-buffered in LDS; Y digit
-// fragments go straight from HBM to registers, one slice ahead.
+// x up to 128 rows (one launch per row group).  The Q digit slice (5 planes x
+// 128 rows x 64 K) is double-buffered in LDS; Y digit fragments go straight
+// from HBM to registers through a four-slice register ring (three slices of
+// prefetch cover HBM latency at one wave per SIMD).
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -43,6 +44,8 @@ constexpr int FBM = 128;         // rows per workgroup
 constexpr int FBN = 64;          // columns per workgroup
 constexpr int FBK = 64;          // K per slice (one MFMA)
 constexpr int FPITCH = FBK + 16; // LDS row pitch (bytes): 80 -> conflict-free 16-byte reads
+// Word 3 of a raw buffer descriptor on gfx9-class GPUs (32-bit data format).
+constexpr int kBufferWord3 = 0x00020000;
 
 // Q digit planes: qd[(p * bpad + t) * ldq + k] = digit p of q_t[k] / u_t, qs[t] = u_t.
 __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int64_t m, int64_t b, int64_t ldq,
@@ -126,92 +129,123 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
     }
 }
 
-template <bool ENERGY>
-__global__ __launch_bounds__(256) void fold_i8_kernel(const int8_t* __restrict__ qd, const double* __restrict__ qs,
-                                                      int64_t ldq, int64_t bpad, int64_t b,
-                                                      const int8_t* __restrict__ yd, int64_t P,
-                                                      const double* __restrict__ yscale, int64_t m, int64_t n,
-                                                      double* __restrict__ proj2, double* __restrict__ energy)
+// RH (1 or 2): row halves of 64 in this launch's row group; the workgroup
+// has 4 RH waves, wave w computing rows 64 (w / 4) .. +64 of the group for
+// the 16 columns 16 (w % 4) .. +16.  Every wave runs the same four row tiles
+// (rows past b carry zero digits), so the MFMA sequence has no branches: a
+// branch would make the compiler drain every outstanding load at its join,
+// prefetches included.  Two waves per SIMD (RH = 2) let one wave's MFMAs
+// cover the other's waits.
+template <bool ENERGY, int RH>
+__global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_kernel(const int8_t* __restrict__ qd,
+                                                           const double* __restrict__ qs, int64_t ldq, int64_t bpad,
+                                                           int64_t b, int64_t m0, const int8_t* __restrict__ yd,
+                                                           int64_t P, const double* __restrict__ yscale, int64_t m,
+                                                           int64_t n, double* __restrict__ proj2,
+                                                           double* __restrict__ energy)
 {
-    __shared__ __align__(16) int8_t sQ[2][ND][FBM * FPITCH];
-    __shared__ double red[4][FBM];
+    constexpr int ROWS = 64 * RH;  // Q rows staged per slice
+    constexpr int NT = 4;          // row tiles of 16 per wave
+    constexpr int NTHREADS = 256 * RH;
+    __shared__ __align__(16) int8_t sQ[2][ND][ROWS * FPITCH];
+    __shared__ double red[4 * RH][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int quarter = lane >> 4, l16 = lane & 15;
-    const int64_t m0 = (int64_t)blockIdx.y * FBM;
-    const int64_t col = (int64_t)blockIdx.x * FBN + wave * 16 + l16;  // this lane's Y column
-    const int R = (int)std::min<int64_t>(8, (b - m0 + 15) / 16);
-
-    i4v acc[NG][8];
+    const int rh = wave >> 2, cg = wave & 3;
+    const int64_t g0 = (int64_t)blockIdx.x * FBN;
+    const int64_t col = g0 + cg * 16 + l16;  // this lane's Y column
+    i4v acc[NG][NT];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int t = 0; t < 8; ++t) acc[g][t] = i4v{0, 0, 0, 0};
+        for (int t = 0; t < NT; ++t) acc[g][t] = i4v{0, 0, 0, 0};
 
     const bool cok = col < n;
     const double sj = cok ? yscale[col] : 0.0;
-    const int8_t* ycol = yd + (cok ? col : 0) * 4 * P;
-    // This lane's 16 K bytes of plane r at slice k0 (zero past the plane).
-    i4v bd[NY], bn[NY];
-#define MPISDF_LOAD_YD(k0, dst)                                                                    \
-    do {                                                                                          \
-        const int64_t k_ = (k0) + 16 * quarter;                                                   \
-        const bool in_ = cok && k_ < P;                                                           \
-        _Pragma("unroll") for (int r_ = 0; r_ < NY; ++r_) {                                       \
-            const i4v v_ = *reinterpret_cast<const i4v*>(ycol + r_ * P + (in_ ? k_ : 0));          \
-            dst[r_] = in_ ? v_ : i4v{0, 0, 0, 0};                                                 \
-        }                                                                                         \
+    // Raw buffer loads: an offset past a descriptor's range returns zeros.
+    // K offsets at or past a plane's pitch are sent out of range, so the
+    // slices past the last one (the loop runs in steps of four) read zeros
+    // and add nothing.  The host keeps every offset below 2^31.
+    const int64_t ycols = std::min<int64_t>(FBN, n - g0);
+    const __amdgpu_buffer_rsrc_t yrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
+    const __amdgpu_buffer_rsrc_t qrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t*>(qd), (short)0, (int)(ND * bpad * ldq), kBufferWord3);
+    const uint32_t ybase = (uint32_t)((cg * 16 + l16) * 4 * P);
+    // Y digits of this lane (16 K bytes of each plane) for four slices in flight.
+    i4v y0[NY], y1[NY], y2[NY], y3[NY];
+#define MPISDF_LOAD_YD(k0, dst)                                                                       \
+    do {                                                                                             \
+        const uint32_t k_ = (uint32_t)(k0) + 16u * (uint32_t)quarter;                                \
+        _Pragma("unroll") for (int r_ = 0; r_ < NY; ++r_) {                                          \
+            const uint32_t off_ = k_ < (uint32_t)P ? ybase + (uint32_t)(r_ * P) + k_ : 0x80000000u;  \
+            dst[r_] = __builtin_bit_cast(i4v, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
+        }                                                                                            \
     } while (0)
-    // Q digits: ND planes x 128 rows x 64 bytes per slice = 2560 chunks of 16
-    // bytes, ten per thread.
-    constexpr int QCHUNKS = ND * FBM * (FBK / 16) / 256;
+    // Q digits: ND planes x ROWS rows x 64 bytes per slice, five 16-byte
+    // chunks per thread.
+    constexpr int QCHUNKS = ND * ROWS * (FBK / 16) / NTHREADS;
+    static_assert(QCHUNKS * NTHREADS == ND * ROWS * (FBK / 16), "Q slice must split evenly over the threads");
     uint4 rq[QCHUNKS];
-#define MPISDF_LOAD_Q(k0)                                                                          \
-    do {                                                                                          \
-        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                  \
-            const int c_ = tid + 256 * r_;                                                        \
-            const int p_ = c_ / (FBM * 4), rem_ = c_ - p_ * FBM * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            rq[r_] = *reinterpret_cast<const uint4*>(qd + ((int64_t)p_ * bpad + m0 + row_) * ldq + (k0) + kc_); \
-        }                                                                                         \
+#define MPISDF_LOAD_Q(k0)                                                                             \
+    do {                                                                                             \
+        const uint32_t k_ = (uint32_t)(k0);                                                          \
+        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
+            const int c_ = tid + NTHREADS * r_;                                                      \
+            const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
+            const uint32_t off_ = k_ < (uint32_t)ldq                                                 \
+                                      ? (uint32_t)(((int64_t)p_ * bpad + m0 + row_) * ldq) + k_ + (uint32_t)kc_ \
+                                      : 0x80000000u;                                                 \
+            rq[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(qrsrc, off_, 0, 0)); \
+        }                                                                                            \
     } while (0)
-#define MPISDF_STASH_Q(buf)                                                                        \
-    do {                                                                                          \
-        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                  \
-            const int c_ = tid + 256 * r_;                                                        \
-            const int p_ = c_ / (FBM * 4), rem_ = c_ - p_ * FBM * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            *reinterpret_cast<uint4*>(&sQ[buf][p_][row_ * FPITCH + kc_]) = rq[r_];                \
-        }                                                                                         \
+#define MPISDF_STASH_Q(buf)                                                                           \
+    do {                                                                                             \
+        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
+            const int c_ = tid + NTHREADS * r_;                                                      \
+            const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
+            *reinterpret_cast<uint4*>(&sQ[buf][p_][row_ * FPITCH + kc_]) = rq[r_];                   \
+        }                                                                                            \
+    } while (0)
+#define MPISDF_MMA(buf, yv)                                                                           \
+    do {                                                                                             \
+        _Pragma("unroll") for (int t = 0; t < NT; ++t) {                                             \
+            _Pragma("unroll") for (int p = 0; p < ND; ++p) {                                         \
+                const i4v a = *reinterpret_cast<const i4v*>(                                         \
+                    &sQ[buf][p][(rh * 64 + t * 16 + l16) * FPITCH + 16 * quarter]);                  \
+                _Pragma("unroll") for (int r = 0; r + p < NG && r < NY; ++r)                         \
+                    acc[p + r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, yv[r], acc[p + r][t], 0, 0, 0); \
+            }                                                                                        \
+        }                                                                                            \
+    } while (0)
+    // One slice: the Q digits of the next slice are loaded before the Y
+    // digits three slices ahead, so staging them waits only on the former.
+#define MPISDF_STEP(s, buf, ycur, yfill)                                                              \
+    do {                                                                                             \
+        MPISDF_LOAD_Q(((s) + 1) * FBK);                                                              \
+        __builtin_amdgcn_sched_barrier(0);                                                           \
+        MPISDF_LOAD_YD(((s) + 3) * FBK, yfill);                                                      \
+        __builtin_amdgcn_sched_barrier(0);                                                           \
+        MPISDF_MMA(buf, ycur);                                                                       \
+        MPISDF_STASH_Q((buf) ^ 1);                                                                   \
+        __syncthreads();                                                                             \
     } while (0)
 
     const int64_t nsl = (m + FBK - 1) / FBK;
     MPISDF_LOAD_Q(0);
-    MPISDF_LOAD_YD(0, bd);
+    MPISDF_LOAD_YD(0, y0);
+    MPISDF_LOAD_YD(FBK, y1);
+    MPISDF_LOAD_YD(2 * FBK, y2);
     MPISDF_STASH_Q(0);
     __syncthreads();
-    for (int64_t sl = 0; sl < nsl; ++sl) {
-        const int buf = (int)(sl & 1);
-        const bool more = sl + 1 < nsl;
-        if (more) {
-            MPISDF_LOAD_Q((sl + 1) * FBK);
-            MPISDF_LOAD_YD((sl + 1) * FBK, bn);
-        }
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            if (t >= R) continue;
-#pragma unroll
-            for (int p = 0; p < ND; ++p) {
-                const i4v a = *reinterpret_cast<const i4v*>(&sQ[buf][p][(t * 16 + l16) * FPITCH + 16 * quarter]);
-#pragma unroll
-                for (int r = 0; r + p < NG && r < NY; ++r)
-                    acc[p + r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, bd[r], acc[p + r][t], 0, 0, 0);
-            }
-        }
-        if (more) {
-            MPISDF_STASH_Q(buf ^ 1);
-#pragma unroll
-            for (int r = 0; r < NY; ++r) bd[r] = bn[r];
-        }
-        __syncthreads();
+    for (int64_t sl = 0; sl < nsl; sl += 4) {
+        MPISDF_STEP(sl, 0, y0, y3);
+        MPISDF_STEP(sl + 1, 1, y1, y0);
+        MPISDF_STEP(sl + 2, 0, y2, y1);
+        MPISDF_STEP(sl + 3, 1, y3, y2);
     }
+#undef MPISDF_STEP
+#undef MPISDF_MMA
 #undef MPISDF_LOAD_YD
 #undef MPISDF_LOAD_Q
 #undef MPISDF_STASH_Q
@@ -219,14 +253,15 @@ __global__ __launch_bounds__(256) void fold_i8_kernel(const int8_t* __restrict__
     // W[t][j] = u_t s_j sum_g 128^-(g+2) acc_g; element e of lane l is row 4 quarter + e.
     double colsq = 0;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < NT; ++t) {
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             double v = 0;
 #pragma unroll
             for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / 128.0) + (double)acc[g][t][e];
-            const int64_t row = m0 + t * 16 + 4 * quarter + e;
-            const double wv = (t < R && row < b) ? v * (1.0 / 16384.0) * qs[row] * sj : 0.0;
+            const int lrow = rh * 64 + t * 16 + 4 * quarter + e;
+            const int64_t row = m0 + lrow;
+            const double wv = row < b ? v * (1.0 / 16384.0) * qs[row] * sj : 0.0;
             colsq += wv * wv;
             if (ENERGY) {
                 double s = wv * wv;
@@ -238,17 +273,43 @@ __global__ __launch_bounds__(256) void fold_i8_kernel(const int8_t* __restrict__
     }
     colsq += __shfl_xor(colsq, 16);
     colsq += __shfl_xor(colsq, 32);
+    // One lane per column and row half; row groups run in order.
     if (quarter == 0 && cok) {
-        if (gridDim.y == 1) proj2[col] += colsq;
+        if (RH == 1) proj2[col] += colsq;
         else atomicAdd(&proj2[col], colsq);
     }
     if (ENERGY) {
         __syncthreads();
-        if (tid < FBM) {
+        if (tid < ROWS) {
             const int64_t row = m0 + tid;
-            if (row < b) atomicAdd(&energy[row], red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid]);
+            const int h = tid >> 6, lr = tid & 63;
+            if (row < b)
+                atomicAdd(&energy[row], red[4 * h][lr] + red[4 * h + 1][lr] + red[4 * h + 2][lr] + red[4 * h + 3][lr]);
         }
     }
+}
+
+void launch_fold_i8(int R, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs, int64_t ldq,
+                    int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P, const double* yscale, int64_t m,
+                    int64_t n, double* proj2, double* en)
+{
+    if (R <= 0 || R > 8) throw std::logic_error("INT8 fold row tiles out of range");
+    if (R <= 4) {
+        if (energy)
+            hipLaunchKernelGGL((fold_i8_kernel<true, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
+                               yscale, m, n, proj2, en);
+        else
+            hipLaunchKernelGGL((fold_i8_kernel<false, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
+                               yscale, m, n, proj2, static_cast<double*>(nullptr));
+    } else {
+        if (energy)
+            hipLaunchKernelGGL((fold_i8_kernel<true, 2>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
+                               yscale, m, n, proj2, en);
+        else
+            hipLaunchKernelGGL((fold_i8_kernel<false, 2>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
+                               yscale, m, n, proj2, static_cast<double*>(nullptr));
+    }
+    MPISDF_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -276,21 +337,20 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     const int64_t bpad = (b + FBM - 1) / FBM * FBM;
     const int64_t ldq = (m + FBK - 1) / FBK * FBK;
     const size_t need = (size_t)(ND * bpad * ldq);
+    if (need >= (size_t(1) << 31)) throw std::invalid_argument("the INT8 fold's Q digits exceed 2 GiB");
     if (scratch.digits.count < need) scratch.digits.resize(need);
     if (scratch.scales.count < (size_t)bpad) scratch.scales.resize(bpad);
     hipLaunchKernelGGL(split_q_kernel, dim3((unsigned)bpad), dim3(256), 0, stream, Qb, ldqb, m, b, ldq, bpad,
                        scratch.digits.ptr, scratch.scales.ptr);
     MPISDF_HIP(hipGetLastError());
-    const int64_t gx = (n + FBN - 1) / FBN, gy = bpad / FBM;
-    if (gx > 0x7fffffff || gy > 65535) throw std::overflow_error("fold grid too large");
-    if (energy)
-        hipLaunchKernelGGL(fold_i8_kernel<true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream,
-                           scratch.digits.ptr, scratch.scales.ptr, ldq, bpad, b, Yd, P, yscale, m, n, proj2, energy);
-    else
-        hipLaunchKernelGGL(fold_i8_kernel<false>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream,
-                           scratch.digits.ptr, scratch.scales.ptr, ldq, bpad, b, Yd, P, yscale, m, n, proj2,
-                           static_cast<double*>(nullptr));
-    MPISDF_HIP(hipGetLastError());
+    const int64_t gx = (n + FBN - 1) / FBN;
+    if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
+    // One launch per group of 128 rows (Q^T rows), each with its own tile count.
+    for (int64_t m0 = 0; m0 < b; m0 += FBM) {
+        const int R = (int)std::min<int64_t>(8, (b - m0 + 15) / 16);
+        launch_fold_i8(R, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr, scratch.scales.ptr, ldq,
+                       bpad, b, m0, Yd, P, yscale, m, n, proj2, energy);
+    }
 }
 
 }  // namespace mpisdf

@@ -87,6 +87,7 @@ void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, 
 // digit planes y_hat = scale[j] sum_r d_r 128^-(r+1); ynorm2[j] (in: of the
 // FP32 column, out: of y_hat).  m <= 36864.
 void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale, hipStream_t stream);
+constexpr int64_t kDigitRowsMax = 36864;  // digitize_columns holds one FP32 column in LDS
 
 // QRCP fold on the digit planes with INT8 MFMA (digit-pair products in exact
 // INT32, combined in FP64); m < 65536.

@@ -223,7 +223,8 @@ public:
         static const char* names[] = {"rows", "columns", "numerical_rank", "rank_cap", "rank", "sketch_norm",
                                       "relative_tail", "blocks", "folds", "seconds_sketch", "seconds_qrcp",
                                       "seconds_total", "stalls", "refreshes", "fold_seconds", "fold_flops",
-                                      "fold_launches", "sketch_kernel_seconds", "sketch_flops", "sketch_launches"};
+                                      "fold_launches", "sketch_kernel_seconds", "sketch_flops", "sketch_launches",
+                                      "fold_kind"};
         py::dict d;
         for (int i = 0; i < 20; ++i) d[names[i]] = diag[i];
         return py::make_tuple(idx, d);

@@ -85,6 +85,38 @@ __global__ void gather_kernel(const T* __restrict__ y, int64_t ldy, int64_t m,
         rc[i + c * ldr] = (double)src[i];
 }
 
+// y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1), exact in FP64 (28 bits).
+__device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t P, int64_t i, double scale)
+{
+    double v = 0;
+#pragma unroll
+    for (int r = 3; r >= 0; --r) v = (v + (double)colp[r * P + i]) * (1.0 / 128.0);
+    return v * scale;
+}
+
+__global__ void gather_digits_kernel(const int8_t* __restrict__ yd, int64_t P, const double* __restrict__ yscale,
+                                     int64_t m, const int64_t* __restrict__ cand, int64_t C, double* __restrict__ rc,
+                                     int64_t ldr)
+{
+    const int64_t c = blockIdx.y;
+    if (c >= C) return;
+    const int64_t j = cand[c];
+    const int8_t* colp = yd + j * 4 * P;
+    const double sc = yscale[j];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        rc[i + c * ldr] = digit_value(colp, P, i, sc);
+}
+
+__global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, const double* __restrict__ yscale,
+                                    int64_t j0, int64_t m, int64_t nc, double* __restrict__ out)
+{
+    const int64_t total = m * nc;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = i / m, r = i - j * m;
+        out[i] = digit_value(yd + (j0 + j) * 4 * P, P, r, yscale[j0 + j]);
+    }
+}
+
 __global__ void cand_norm_kernel(const double* __restrict__ rc, int64_t m, int64_t ldr, int64_t C,
                                  double* __restrict__ rn2)
 {
@@ -255,7 +287,12 @@ int grid_for(int64_t work, int threads = 256, int64_t cap = 65536)
 QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t ldy, const QrcpOptions& opt,
                        hipStream_t s)
 {
-    if (ytype != kF64 && ytype != kF32) throw std::invalid_argument("QRCP input must be f64 or f32");
+    const bool digits = ytype == kI8Digits;
+    if (ytype != kF64 && ytype != kF32 && !digits) throw std::invalid_argument("QRCP input must be f64, f32 or digits");
+    if (digits && (!opt.ynorm2 || !opt.yscale || !(opt.margin > 0)))
+        throw std::invalid_argument("QRCP on digit planes needs device norms, scales and a positive margin");
+    if (digits && (ldy % 16 || m >= (int64_t(1) << 16)))
+        throw std::invalid_argument("digit planes need a pitch that is a multiple of 16 and fewer than 65536 rows");
     if (m <= 0 || n <= 0) return {};
     if (ldy < m) throw std::invalid_argument("QRCP leading dimension smaller than rows");
     const int64_t kmax = std::min({opt.kmax > 0 ? opt.kmax : std::min(m, n), m, n});
@@ -305,6 +342,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     double r00 = 0;
     bool done = false;
     FoldScratch fold_scratch;
+    FoldI8Scratch i8_scratch;
     const bool split = opt.split_bf16_fold && ytype == kF32;
     // Fold launches are bracketed by HIP events; durations are read at the end.
     LaunchTimer fold_timer;
@@ -313,7 +351,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         if (b <= 0) return;
         fold_timer.begin(s);
         res.fold_flops += 2.0 * (double)b * (double)m * (double)n;
-        if (split)
+        if (digits)
+            fold_projections_i8(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const int8_t*>(y), ldy, opt.yscale,
+                                proj2.ptr, energy_out ? energy_out + folded : nullptr, i8_scratch, s);
+        else if (split)
             fold_projections_bf16x3(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const float*>(y), ldy, proj2.ptr,
                                     energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
         else
@@ -348,7 +389,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         double bound2 = -1.0;
         if (Ceff < remaining)
             MPISDF_HIP(hipMemcpyAsync(&bound2, keys_sorted.ptr + Ceff, sizeof(double), hipMemcpyDeviceToHost, s));
-        if (ytype == kF64)
+        if (digits)
+            hipLaunchKernelGGL(gather_digits_kernel, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
+                               static_cast<const int8_t*>(y), ldy, opt.yscale, m, idx_sorted.ptr, Ceff, rc.ptr, ldr);
+        else if (ytype == kF64)
             hipLaunchKernelGGL(gather_kernel<double>, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
                                static_cast<const double*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr, ldr);
         else
@@ -443,7 +487,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         DeviceBuffer<double> Wt(k > 0 ? (size_t)(k * nc) : 1);
         for (int64_t j0 = 0; j0 < n; j0 += nc) {
             const int64_t cnt = std::min(nc, n - j0);
-            if (ytype == kF64)
+            if (digits)
+                hipLaunchKernelGGL(widen_digits_kernel, dim3(grid_for(m * cnt)), dim3(256), 0, s,
+                                   static_cast<const int8_t*>(y), ldy, opt.yscale, j0, m, cnt, R.ptr);
+            else if (ytype == kF64)
                 hipLaunchKernelGGL(widen_kernel<double>, dim3(grid_for(m * cnt)), dim3(256), 0, s,
                                    static_cast<const double*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);
             else

@@ -14,7 +14,8 @@ struct QrcpOptions {
     bool want_energy = false;    // row energies sum_j (q_i^T y_j)^2 (randomized tail rule)
     bool want_tail = false;      // exact residual Frobenius^2 after the last pivot
     bool debug = false;
-    const double* ynorm2 = nullptr;  // optional device ||y_j||^2 (skips the norm pass)
+    const double* ynorm2 = nullptr;  // optional device ||y_j||^2 (skips the norm pass; required for kI8Digits)
+    const double* yscale = nullptr;  // kI8Digits: device per-column scales of the digit planes
     bool split_bf16_fold = false;    // FP32 Y: fold with split-BF16 MFMA (needs margin > 0)
 };
 
@@ -29,8 +30,11 @@ struct QrcpResult {
     int64_t blocks = 0, folds = 0, stalls = 0, refreshes = 0;
 };
 
-// Y is m x n column-major with leading dimension ldy, element type f64 or f32.
-// Y is read only.  Pivot decisions use exact FP64 residual norms of the
+// Y is m x n column-major with leading dimension ldy, element type f64 or f32;
+// or, for kI8Digits, column j at byte offset 4 ldy j holding four digit planes
+// of ldy bytes (y_hat_j = yscale[j] sum_r d_r 128^-(r+1)), folded with INT8
+// MFMA and factorised as y_hat (candidate columns are rebuilt from the same
+// digits; opt.margin must cover the fold's error).  Y is read only.  Pivot decisions use exact FP64 residual norms of the
 // candidate columns (LAPACK dgeqp3 pivot semantics: max norm, lowest index on
 // ties).
 QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t ldy,

@@ -213,7 +213,11 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
     DeviceBuffer<double> y64;
     DeviceBuffer<float> y32;
-    const int64_t ldy32 = (ell + 3) / 4 * 4;  // 16-byte aligned FP32 columns
+    // The reduced-precision forms fold on INT8 digit planes of the FP32 sketch
+    // (four planes of ldy32 bytes in place of each column), which needs a
+    // pitch that is a multiple of 16 and ell within the digit kernels' limits.
+    const bool use_digits = lowp && ell <= kDigitRowsMax;
+    const int64_t ldy32 = use_digits ? (ell + 15) / 16 * 16 : (ell + 3) / 4 * 4;  // 16-byte aligned FP32 columns
     const int64_t ldy64 = (ell + 3) / 4 * 4;  // 32-byte aligned FP64 columns
     DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;
@@ -281,6 +285,11 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         sketch_launches = (double)timer.launches();
     }
     if (bad.download(s)[0]) throw std::overflow_error("non-finite randomized ID sketch");
+    DeviceBuffer<double> yscale;
+    if (use_digits) {
+        yscale.resize(n);
+        digitize_columns(y32.ptr, ell, n, ldy32, ynorm2.ptr, yscale.ptr, s);
+    }
     const double t_sketch = seconds_since(started);
 
     const int64_t steps = std::min(ell, n);
@@ -295,12 +304,19 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     opt.debug = a.debug;
     const auto tq = Clock::now();
     opt.ynorm2 = ynorm2.ptr;
-    // The reduced-precision forms fold with exact FP64 products of the FP32
-    // sketch: certifying greedy pivots late in the factorisation needs folded
-    // norms accurate to ~1e-9 of ||y_j||^2, beyond what split-BF16 products
-    // deliver (the split fold stays available through QrcpOptions).
-    const QrcpResult r = lowp ? qrcp_device(y32.ptr, kF32, ell, n, ldy32, opt, s)
-                              : qrcp_device(y64.ptr, kF64, ell, n, ldy64, opt, s);
+    // Certifying greedy pivots late in the factorisation needs folded norms
+    // accurate to ~1e-9 of ||y_j||^2.  The FP64 form folds with FP64 MFMA; the
+    // reduced-precision forms fold the digit planes of the FP32 sketch with
+    // INT8 MFMA (exact digit products, ~1e-11 of ||y_j||^2 per fold, covered
+    // by the margin) and factorise y_hat, which equals the FP32 sketch to
+    // 2^-29 of each column's scale.
+    if (use_digits) {
+        opt.yscale = yscale.ptr;
+        opt.margin = kFoldI8Margin;
+    }
+    const QrcpResult r = use_digits ? qrcp_device(y32.ptr, kI8Digits, ell, n, ldy32, opt, s)
+                         : lowp     ? qrcp_device(y32.ptr, kF32, ell, n, ldy32, opt, s)
+                                    : qrcp_device(y64.ptr, kF64, ell, n, ldy64, opt, s);
     const double t_qrcp = seconds_since(tq);
     const int64_t K = (int64_t)r.pivots.size();
     // numerical_rank: first i with |R_ii| <= floor; the engine stops there.
@@ -355,6 +371,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         diag[kDiagSketchSeconds] = t_sketch_kernel;
         diag[kDiagSketchFlops] = 4.0 * (double)ell * (double)d * (double)n;
         diag[kDiagSketchLaunches] = sketch_launches;
+        diag[kDiagFoldKind] = use_digits ? 2.0 : (lowp ? 1.0 : 0.0);
     }
     return out;
 }

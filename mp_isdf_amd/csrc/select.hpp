@@ -51,6 +51,7 @@ enum Diag : int {
     kDiagSketchSeconds = 17,  // summed duration of the sketch kernel launches (HIP events)
     kDiagSketchFlops = 18,    // their algorithmic flops: sum 4 ell d count
     kDiagSketchLaunches = 19,
+    kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 INT8 MFMA on digit planes
     kDiagCount = 24
 };
 

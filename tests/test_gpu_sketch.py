@@ -126,3 +126,63 @@ def test_qrcp_fold_modes(mp_isdf, mode):
     tol = {0: 1e-12, 1: 1e-12, 2: 1e-5, 3: 1e-10}[mode]
     assert np.max(np.abs(proj.cpu().numpy() - want_cols) / ynorm2) < tol
     np.testing.assert_allclose(energy.cpu().numpy(), want_rows, rtol=max(tol, 1e-12))
+
+
+def _digit_model(y, ynorm2):
+    """NumPy model of the digit planes: s = 2^(e+1) with ||y|| < 2^e, and
+    four signed base-128 digits of y / s, rounded to nearest in FP32 steps."""
+    scale = np.exp2(np.frexp(np.sqrt(ynorm2))[1] + 1.0)
+    x = (y / scale[:, None]).astype(np.float32)
+    digits, y_hat, weight = [], np.zeros_like(y), 1.0
+    for _ in range(4):
+        x = (x * np.float32(128)).astype(np.float32)
+        dgt = np.rint(x)
+        x = (x - dgt).astype(np.float32)
+        weight /= 128.0
+        digits.append(dgt.astype(np.int8))
+        y_hat += dgt.astype(np.float64) * weight
+    return scale, digits, y_hat * scale[:, None]
+
+
+def test_qrcp_digit_planes_match_oracle(mp_isdf):
+    """The reduced-precision selection's QRCP: FP32 sketch columns -> digit
+    planes (checked digit for digit against the NumPy model) -> INT8-fold
+    QRCP, whose pivots must equal the oracle's dgeqp3-semantics QRCP on the
+    same y_hat; the FP32 input path must equal it on y."""
+    import torch
+    rng = np.random.default_rng(11)
+    m, n, kmax = 257, 4000, 120
+    y = (rng.normal(size=(n, 60)) @ rng.normal(size=(60, m)) * rng.uniform(0.2, 2.0, size=(n, 1))
+         + 1e-3 * rng.normal(size=(n, m))).astype(np.float32).astype(np.float64)
+    ynorm2 = np.sum(y ** 2, axis=1)
+    P = (m + 15) // 16 * 16
+    buf = np.zeros((n, P), dtype=np.float32)
+    buf[:, :m] = y
+    Y = torch.tensor(buf, device="cuda")
+    yn = torch.tensor(ynorm2, device="cuda")
+    sc = torch.zeros(n, dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
+    scale, digits, y_hat = _digit_model(y, ynorm2)
+    planes = Y.cpu().numpy().view(np.int8).reshape(n, 4, P)
+    np.testing.assert_array_equal(sc.cpu().numpy(), scale)
+    for r in range(4):
+        np.testing.assert_array_equal(planes[:, r, :m], digits[r])
+        assert not planes[:, r, m:].any()
+    np.testing.assert_allclose(yn.cpu().numpy(), np.sum(y_hat ** 2, axis=1), rtol=1e-14)
+
+    def gpu_qrcp(buf_dev, ytype, ld, norms, scales):
+        piv = np.zeros(kmax, dtype=np.int64)
+        rd = np.zeros(kmax)
+        cnt = np.zeros(1, dtype=np.int64)
+        capi.call("mpisdf_qrcp_device", buf_dev, ytype, m, n, ld, norms, scales, kmax, 0.0, piv, rd, cnt, None)
+        return piv[:cnt[0]], rd[:cnt[0]]
+
+    piv, rd = gpu_qrcp(Y, 4, P, yn, sc)
+    f, want, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
+    np.testing.assert_array_equal(piv, want[:kmax])
+    np.testing.assert_allclose(rd, np.abs(np.diag(f)[:kmax]), rtol=1e-9)
+
+    y32 = torch.tensor(np.ascontiguousarray(y.astype(np.float32)), device="cuda")
+    piv32, _ = gpu_qrcp(y32, 1, m, None, None)
+    _, want32, _ = oracle.geqp3(np.ascontiguousarray(y.T))
+    np.testing.assert_array_equal(piv32, want32[:kmax])
