@@ -206,7 +206,8 @@ def main():
     if traffic_file.exists() and args.config == "h2o64" and args.precision == "bf16":
         measured = json.loads(traffic_file.read_text())
         for key, entry in (("qrcp_fold", roofline_fold), ("fit_grid_gemm", roofline_fit), ("bf16_sketch", roofline_sketch)):
-            if key in measured and measured[key].get("kernel_tag") in entry["kernel"]:
+            tag = measured.get(key, {}).get("kernel_tag")
+            if tag and tag in entry["kernel"]:
                 entry["traffic"] = round(measured[key]["traffic_bytes_per_launch"] / 1e9, 3)
                 entry["traffic_unit"] = "GB per launch"
                 entry["traffic_source"] = (f"{traffic_file.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "

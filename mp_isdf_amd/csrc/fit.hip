@@ -30,12 +30,6 @@ int grid_for(int64_t work, int64_t cap = 65536)
     return (int)std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, cap));
 }
 
-__global__ void square_kernel(double* __restrict__ a, int64_t count)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-        a[i] = a[i] * a[i];
-}
-
 __global__ void diagonal_kernel(const double* __restrict__ a, int64_t lda, int64_t k, double* __restrict__ out)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += (int64_t)gridDim.x * blockDim.x)
@@ -134,6 +128,25 @@ __global__ void pair_error_kernel(const double* __restrict__ z, int64_t k, const
     if (lane == 0 && g < nb) { atomicAdd(&out[0], e); atomicAdd(&out[1], t); }
 }
 
+// The same sums from precomputed ||z_g||^2 (zz): qn = ||phi_g||^4,
+// acc += |w_g| max(qn - zz_g, 0), tot += |w_g| qn; one wave per point.
+__global__ void pair_error_norms_kernel(const double* __restrict__ zz, const double* __restrict__ phi, int64_t d,
+                                        const double* __restrict__ w, int64_t nb, double* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (g >= nb) return;
+    double pp = 0;
+    for (int64_t a = lane; a < d; a += 64) { const double v = phi[g * d + a]; pp += v * v; }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) pp += __shfl_xor(pp, off);
+    if (lane == 0) {
+        const double qn = pp * pp, aw = fabs(w[g]);
+        atomicAdd(&out[0], aw * fmax(qn - zz[g], 0.0));
+        atomicAdd(&out[1], aw * qn);
+    }
+}
+
 // out[c * rows + r] = in[r * cols + c] / (div ? div[c] : 1)
 __global__ void transpose_kernel(const double* __restrict__ in, int64_t rows, int64_t cols,
                                  const double* __restrict__ div, double* __restrict__ out)
@@ -213,8 +226,8 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
     f.linv.resize((size_t)(k * k));
     DeviceBuffer<double> gdiag;
     DeviceBuffer<double> a((size_t)(k * k));
-    gemm_tn_f64(k, k, d, 1.0, f.phi_a.ptr, d, f.phi_a.ptr, d, 0.0, a.ptr, k, s);  // symmetric: layout-free
-    hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * k)), dim3(256), 0, s, a.ptr, k * k);
+    gemm_tn_f64_fused(TnFused::Square, k, k, d, 1.0, f.phi_a.ptr, d, f.phi_a.ptr, d, a.ptr, k, nullptr, nullptr,
+                      nullptr, s);  // symmetric: layout-free
     MPISDF_HIP(hipGetLastError());
     // Diagonal of G2 gives the pair column norms; keep the max for diagnostics.
     {
@@ -286,6 +299,7 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     double gemm_flops = 0;
     batch = std::max<int64_t>(1, std::min(batch, n));
     DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
+    DeviceBuffer<double> zz((size_t)batch);
     DeviceBuffer<double> th, thr, tsum;
     std::vector<double> stage;
     wa.upload(anchor_w, k, s);
@@ -299,30 +313,35 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     for (int64_t start = 0; start < n; start += batch) {
         const int64_t nb = std::min(batch, n - start);
         launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
-        // S = (Phi_a Phi_b^T)^2, k x nb column-major
+        // S = (Phi_a Phi_b^T)^2, k x nb column-major (squared in the epilogue)
         timer.begin(s);
-        gemm_tn_f64(k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, 0.0, sm.ptr, k, s);
+        gemm_tn_f64_fused(TnFused::Square, k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, sm.ptr, k, nullptr, nullptr,
+                          nullptr, s);
         timer.end(s);
         gemm_flops += 2.0 * (double)k * (double)d * (double)nb;
-        hipLaunchKernelGGL(square_kernel, dim3(grid_for(k * nb)), dim3(256), 0, s, sm.ptr, k * nb);
-        // Z = L^-1 S (lower triangular A)
+        // Z = L^-1 S (lower triangular A), with ||z_g||^2 summed in the epilogue
+        MPISDF_HIP(hipMemsetAsync(zz.ptr, 0, nb * sizeof(double), s));
         timer.begin(s);
-        gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s, 1);
+        gemm_tn_f64_fused(TnFused::ColumnSumSquares, k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, z.ptr, k, zz.ptr,
+                          nullptr, nullptr, s, 1);
         timer.end(s);
         gemm_flops += (double)k * (double)(k + 1) * (double)nb;
         if (err_out)
-            hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
-                               w + start, nb, acc.ptr);
+            hipLaunchKernelGGL(pair_error_norms_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, zz.ptr, phi.ptr,
+                               d, w + start, nb, acc.ptr);
         if (theta_host || tsum_out) {
             // theta_phys = L^-T Z (upper triangular A), column-major k x nb;
             // theta = theta_phys / w_anchor
+            // (with t_phys += theta_phys w_grid summed in the epilogue; divided by
+            // w_anchor at the end)
             timer.begin(s);
-            gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, th.ptr, k, s, 2);
+            if (tsum_out)
+                gemm_tn_f64_fused(TnFused::RowDot, k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, th.ptr, k, nullptr,
+                                  tsum.ptr, w + start, s, 2);
+            else
+                gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, th.ptr, k, s, 2);
             timer.end(s);
             gemm_flops += (double)k * (double)(k + 1) * (double)nb;
-            if (tsum_out)  // t_phys += theta_phys w_grid (divided by w_anchor at the end)
-                gemm_f64(k, 1, nb, 1.0, MatView{th.ptr, 1, k}, MatView{w + start, 1, 1}, 1.0, MutView{tsum.ptr, 1, 1},
-                         s);
             if (theta_host) {  // rows of theta[k][n] on the host, through a contiguous staging copy
                 transpose(th.ptr, nb, k, wa.ptr, thr.ptr, s);
                 if (nb == n) {

@@ -31,7 +31,8 @@ constexpr int TBN = 128;       // columns per workgroup
 constexpr int TBK = 32;        // K per slice
 constexpr int LDA_S = TBK + 2; // LDS row pitch (doubles): 68 dwords, conflict-free b128 reads
 
-enum TnEpilogue { kTnStore = 0, kTnFold = 1, kTnFoldEnergy = 2, kTnKhatriRao = 3 };
+enum TnEpilogue { kTnStore = 0, kTnFold = 1, kTnFoldEnergy = 2, kTnKhatriRao = 3,
+                  kTnStoreSquare = 4, kTnStoreColSq = 5, kTnStoreRowDot = 6 };
 
 template <class TB>
 struct BVec;
@@ -201,7 +202,8 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         if (nonfinite) atomicAdd(bad, nonfinite);
         return;
     }
-    if (EPI == kTnStore) {
+    if (EPI == kTnStore || EPI == kTnStoreSquare || EPI == kTnStoreColSq || EPI == kTnStoreRowDot) {
+        // Store C = alpha acc + beta C (kTnStoreSquare: C = acc^2).
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             if (t >= R) continue;
@@ -214,9 +216,50 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
                     const int64_t gm = m0 + t * 16 + quarter + 4 * i;
                     if (gm < M) {
                         double* p = C + gm + gn * ldc;
-                        *p = alpha * acc[t][c][i] + (beta == 0.0 ? 0.0 : beta * *p);
+                        if (EPI == kTnStoreSquare) *p = acc[t][c][i] * acc[t][c][i];
+                        else *p = alpha * acc[t][c][i] + (beta == 0.0 ? 0.0 : beta * *p);
                     }
                 }
+            }
+        }
+        if (EPI == kTnStoreColSq) {
+            // colacc[n] += sum over this row group of (alpha acc)^2; rows >= M hold zeros.
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                double s = 0;
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) s += acc[t][c][i] * acc[t][c][i];
+                s += __shfl_xor(s, 16);
+                s += __shfl_xor(s, 32);
+                const int64_t gn = n0 + 16 * c + l16;
+                if (quarter == 0 && gn < N) atomicAdd(&colacc[gn], alpha * alpha * s);
+            }
+        }
+        if (EPI == kTnStoreRowDot) {
+            // rowacc[m] += sum over the workgroup's columns of alpha acc * cscale[n]
+            // (columns >= N hold zeros and weigh zero).
+            double wc[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const int64_t gn = n0 + 16 * c + l16;
+                wc[c] = gn < N ? alpha * cscale[gn] : 0.0;
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    double s = acc[t][0][i] * wc[0] + acc[t][1][i] * wc[1];
+#pragma unroll
+                    for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
+                    if (l16 == 0) red[wave][t * 16 + quarter + 4 * i] = s;
+                }
+            __syncthreads();
+            if (tid < TBM) {
+                const int64_t gm = m0 + tid;
+                const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
+                if (gm < M && tid < R * 16) atomicAdd(&rowacc[gm], s);
             }
         }
         return;
@@ -287,6 +330,28 @@ void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A,
 {
     launch_tn<double, kTnStore>(M, N, K, A, lda, B, ldb, alpha, beta, C, ldc, nullptr, nullptr, stream, nullptr,
                                 nullptr, nullptr, triangular);
+}
+
+void gemm_tn_f64_fused(TnFused mode, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                       const double* B, int64_t ldb, double* C, int64_t ldc, double* colacc, double* rowacc,
+                       const double* colw, hipStream_t stream, int triangular)
+{
+    switch (mode) {
+    case TnFused::Square:
+        launch_tn<double, kTnStoreSquare>(M, N, K, A, lda, B, ldb, 1.0, 0.0, C, ldc, nullptr, nullptr, stream, nullptr,
+                                          nullptr, nullptr, triangular);
+        break;
+    case TnFused::ColumnSumSquares:
+        if (!colacc) throw std::invalid_argument("column sums need an accumulator");
+        launch_tn<double, kTnStoreColSq>(M, N, K, A, lda, B, ldb, alpha, 0.0, C, ldc, colacc, nullptr, stream, nullptr,
+                                         nullptr, nullptr, triangular);
+        break;
+    case TnFused::RowDot:
+        if (!rowacc || !colw) throw std::invalid_argument("row sums need an accumulator and column weights");
+        launch_tn<double, kTnStoreRowDot>(M, N, K, A, lda, B, ldb, alpha, 0.0, C, ldc, nullptr, rowacc, stream, nullptr,
+                                          colw, nullptr, triangular);
+        break;
+    }
 }
 
 void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldq, const void* Y, bool y_f32,

@@ -51,6 +51,14 @@ void gemm_f64_f32b(int64_t M, int64_t N, int64_t K, double alpha, MatView A, con
 // k < t) skips the zero part of K.
 void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
                  int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream, int triangular = 0);
+// gemm_tn_f64 with a fused epilogue (C column-major M x N, beta = 0):
+//   Square:            C = (A^T B)^2 elementwise (alpha ignored);
+//   ColumnSumSquares:  C = alpha A^T B and colacc[n] += sum_m C[m, n]^2;
+//   RowDot:            C = alpha A^T B and rowacc[m] += sum_n C[m, n] colw[n].
+enum class TnFused { Square, ColumnSumSquares, RowDot };
+void gemm_tn_f64_fused(TnFused mode, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
+                       const double* B, int64_t ldb, double* C, int64_t ldc, double* colacc, double* rowacc,
+                       const double* colw, hipStream_t stream, int triangular = 0);
 
 // FP64 Khatri-Rao sketch on the TN kernel: ut/vt are ell x d row-major
 // (ut[s * d + a]), phi[g * ldphi + a];  y[g * ldy + s] = scale[g] (ut_s . phi_g)

@@ -226,7 +226,7 @@ public:
                                       "fold_launches", "sketch_kernel_seconds", "sketch_flops", "sketch_launches",
                                       "fold_kind"};
         py::dict d;
-        for (int i = 0; i < 20; ++i) d[names[i]] = diag[i];
+        for (size_t i = 0; i < sizeof(names) / sizeof(names[0]); ++i) d[names[i]] = diag[i];
         return py::make_tuple(idx, d);
     }
 

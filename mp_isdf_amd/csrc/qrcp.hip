@@ -74,17 +74,6 @@ __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __r
     }
 }
 
-template <class T>
-__global__ void gather_kernel(const T* __restrict__ y, int64_t ldy, int64_t m,
-                              const int64_t* __restrict__ cand, int64_t C, double* __restrict__ rc, int64_t ldr)
-{
-    const int64_t c = blockIdx.y;
-    if (c >= C) return;
-    const T* src = y + cand[c] * ldy;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-        rc[i + c * ldr] = (double)src[i];
-}
-
 // y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1), exact in FP64 (28 bits).
 __device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t P, int64_t i, double scale)
 {
@@ -92,19 +81,6 @@ __device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, i
 #pragma unroll
     for (int r = 3; r >= 0; --r) v = (v + (double)colp[r * P + i]) * (1.0 / 128.0);
     return v * scale;
-}
-
-__global__ void gather_digits_kernel(const int8_t* __restrict__ yd, int64_t P, const double* __restrict__ yscale,
-                                     int64_t m, const int64_t* __restrict__ cand, int64_t C, double* __restrict__ rc,
-                                     int64_t ldr)
-{
-    const int64_t c = blockIdx.y;
-    if (c >= C) return;
-    const int64_t j = cand[c];
-    const int8_t* colp = yd + j * 4 * P;
-    const double sc = yscale[j];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-        rc[i + c * ldr] = digit_value(colp, P, i, sc);
 }
 
 __global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, const double* __restrict__ yscale,
@@ -115,6 +91,100 @@ __global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, co
         const int64_t j = i / m, r = i - j * m;
         out[i] = digit_value(yd + (j0 + j) * 4 * P, P, r, yscale[j0 + j]);
     }
+}
+
+// Candidates of the previous block whose residual is still live (not
+// selected) keep it: src[c] = their slot there, else -1.
+__global__ void classify_candidates_kernel(const int64_t* __restrict__ idx, int64_t C,
+                                           const int32_t* __restrict__ slot_of, const double* __restrict__ rn2_prev,
+                                           int32_t* __restrict__ src)
+{
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < C; c += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t os = slot_of[idx[c]];
+        src[c] = (os >= 0 && rn2_prev[os] >= 0) ? os : -1;
+    }
+}
+
+// Stable partition of the candidate list, new columns (src < 0) first, in one
+// workgroup of 1024 threads: cand/srcs in the new order, *nnew = new count.
+__global__ __launch_bounds__(1024) void order_candidates_kernel(const int64_t* __restrict__ idx,
+                                                                const int32_t* __restrict__ src, int64_t C,
+                                                                int64_t* __restrict__ cand,
+                                                                int32_t* __restrict__ srcs,
+                                                                int64_t* __restrict__ nnew)
+{
+    __shared__ int scan[1024];
+    __shared__ int64_t total_new;
+    const int tid = threadIdx.x;
+    // pass 1: number of new columns
+    int cnt = 0;
+    for (int64_t c = tid; c < C; c += 1024) cnt += src[c] < 0;
+    scan[tid] = cnt;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (tid < s) scan[tid] += scan[tid + s];
+        __syncthreads();
+    }
+    if (tid == 0) total_new = scan[0];
+    __syncthreads();
+    const int64_t nn = total_new;
+    __syncthreads();
+    // pass 2: chunks of 1024 in order; inclusive scan of the new flags
+    int64_t base_new = 0, base_old = nn;
+    for (int64_t c0 = 0; c0 < C; c0 += 1024) {
+        const int64_t c = c0 + tid;
+        const int f = (c < C && src[c] < 0) ? 1 : 0;
+        scan[tid] = f;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int v = tid >= off ? scan[tid - off] : 0;
+            __syncthreads();
+            scan[tid] += v;
+            __syncthreads();
+        }
+        const int incl = scan[tid];
+        const int chunk_new = scan[1023];
+        if (c < C) {
+            const int64_t pos = f ? base_new + incl - 1 : base_old + (tid - incl);
+            cand[pos] = idx[c];
+            srcs[pos] = src[c];
+        }
+        const int64_t chunk = std::min<int64_t>(1024, C - c0);
+        base_new += chunk_new;
+        base_old += chunk - chunk_new;
+        __syncthreads();
+    }
+    if (tid == 0) *nnew = nn;
+}
+
+// Residual columns of the new candidate list: the kept residual of a
+// persisting candidate, or the sketch column of a new one (FP64, FP32 or
+// digit planes).
+__global__ void fill_candidates_kernel(const void* __restrict__ y, int ytype, int64_t ldy,
+                                       const double* __restrict__ yscale, int64_t m,
+                                       const int64_t* __restrict__ cand, const int32_t* __restrict__ srcs, int64_t C,
+                                       const double* __restrict__ rc_prev, double* __restrict__ rc, int64_t ldr)
+{
+    const int64_t c = blockIdx.y;
+    if (c >= C) return;
+    const int32_t os = srcs[c];
+    const int64_t j = cand[c];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        double v;
+        if (os >= 0) v = rc_prev[i + (int64_t)os * ldr];
+        else if (ytype == kF64) v = static_cast<const double*>(y)[i + j * ldy];
+        else if (ytype == kF32) v = (double)static_cast<const float*>(y)[i + j * ldy];
+        else v = digit_value(static_cast<const int8_t*>(y) + j * 4 * ldy, ldy, i, yscale[j]);
+        rc[i + c * ldr] = v;
+    }
+}
+
+__global__ void set_slots_kernel(const int64_t* __restrict__ cand, int64_t C, int32_t value_base,
+                                 int32_t* __restrict__ slot_of)
+{
+    // value_base < 0: clear (slot_of = -1); else slot_of[cand[c]] = c
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < C; c += (int64_t)gridDim.x * blockDim.x)
+        slot_of[cand[c]] = value_base < 0 ? -1 : (int32_t)c;
 }
 
 __global__ void cand_norm_kernel(const double* __restrict__ rc, int64_t m, int64_t ldr, int64_t C,
@@ -305,7 +375,16 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     // Q (column-major) and its row-major copy Qt; leading dimensions padded to
     // four doubles so the TN products load 32-byte vectors.
     const int64_t ldq = (m + 3) / 4 * 4, ldqt = (std::max<int64_t>(kmax, 1) + 3) / 4 * 4, ldr = ldq;
-    DeviceBuffer<double> Q((size_t)ldq * std::max<int64_t>(kmax, 1)), Qt((size_t)m * ldqt), rc, rn2, H, qv(m), dv;
+    DeviceBuffer<double> Q((size_t)ldq * std::max<int64_t>(kmax, 1)), Qt((size_t)m * ldqt), H, qv(m), dv;
+    // Candidate residuals, norms and columns of this block and the previous
+    // one (a candidate that persists keeps its residual).
+    DeviceBuffer<double> rcb[2], rn2b[2];
+    DeviceBuffer<int64_t> candb[2];
+    DeviceBuffer<int32_t> slot_of(n), csrc, csrcs;
+    DeviceBuffer<int64_t> d_nnew(1);
+    MPISDF_HIP(hipMemsetAsync(slot_of.ptr, 0xff, n * sizeof(int32_t), s));
+    int cur = 0;
+    int64_t Cprev = 0;
     MPISDF_HIP(hipMemsetAsync(Qt.ptr, 0, Qt.count * sizeof(double), s));
     DeviceBuffer<double> energy(std::max<int64_t>(kmax, 1)), tail(1);
     DeviceBuffer<ArgMax> am(1);
@@ -385,30 +464,50 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                                                                 idx.ptr, idx_sorted.ptr, (int)n, 0, 64, s));
         const int64_t remaining = n - k;
         const int64_t Ceff = std::min(C, remaining);
-        if (rc.count < (size_t)(ldr * Ceff)) { rc.resize((size_t)(ldr * Ceff)); rn2.resize(Ceff); }
+        if (Ceff > std::numeric_limits<int32_t>::max()) throw std::overflow_error("QRCP candidate count exceeds int32");
+        cur ^= 1;
+        DeviceBuffer<double>& rc = rcb[cur];
+        DeviceBuffer<double>& rn2 = rn2b[cur];
+        DeviceBuffer<int64_t>& cand = candb[cur];
+        if (rc.count < (size_t)(ldr * Ceff)) rc.resize((size_t)(ldr * Ceff));
+        if (rn2.count < (size_t)Ceff) rn2.resize(Ceff);
+        if (cand.count < (size_t)Ceff) cand.resize(Ceff);
+        if (csrc.count < (size_t)Ceff) { csrc.resize(Ceff); csrcs.resize(Ceff); }
         double bound2 = -1.0;
         if (Ceff < remaining)
             MPISDF_HIP(hipMemcpyAsync(&bound2, keys_sorted.ptr + Ceff, sizeof(double), hipMemcpyDeviceToHost, s));
-        if (digits)
-            hipLaunchKernelGGL(gather_digits_kernel, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
-                               static_cast<const int8_t*>(y), ldy, opt.yscale, m, idx_sorted.ptr, Ceff, rc.ptr, ldr);
-        else if (ytype == kF64)
-            hipLaunchKernelGGL(gather_kernel<double>, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
-                               static_cast<const double*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr, ldr);
+        // New candidates first, then the ones whose residual persists from the
+        // previous block (already orthogonal to every q by the rank-1 updates).
+        if (Cprev > 0)
+            hipLaunchKernelGGL(classify_candidates_kernel, dim3(grid_for(Ceff)), dim3(256), 0, s, idx_sorted.ptr, Ceff,
+                               slot_of.ptr, rn2b[cur ^ 1].ptr, csrc.ptr);
         else
-            hipLaunchKernelGGL(gather_kernel<float>, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s,
-                               static_cast<const float*>(y), ldy, m, idx_sorted.ptr, Ceff, rc.ptr, ldr);
+            MPISDF_HIP(hipMemsetAsync(csrc.ptr, 0xff, Ceff * sizeof(int32_t), s));
+        hipLaunchKernelGGL(order_candidates_kernel, dim3(1), dim3(1024), 0, s, idx_sorted.ptr, csrc.ptr, Ceff, cand.ptr,
+                           csrcs.ptr, d_nnew.ptr);
+        hipLaunchKernelGGL(fill_candidates_kernel, dim3(std::min(grid_for(m), 64), (unsigned)Ceff), dim3(256), 0, s, y,
+                           (int)ytype, ldy, opt.yscale, m, cand.ptr, csrcs.ptr, Ceff,
+                           Cprev > 0 ? rcb[cur ^ 1].ptr : nullptr, rc.ptr, ldr);
+        if (Cprev > 0)
+            hipLaunchKernelGGL(set_slots_kernel, dim3(grid_for(Cprev)), dim3(256), 0, s, candb[cur ^ 1].ptr, Cprev, -1,
+                               slot_of.ptr);
+        hipLaunchKernelGGL(set_slots_kernel, dim3(grid_for(Ceff)), dim3(256), 0, s, cand.ptr, Ceff, 0, slot_of.ptr);
         MPISDF_HIP(hipGetLastError());
-        DeviceBuffer<int64_t> cand;
-        cand.resize(Ceff);
-        MPISDF_HIP(hipMemcpyAsync(cand.ptr, idx_sorted.ptr, Ceff * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+        int64_t nnew = Ceff;
+        MPISDF_HIP(hipMemcpyAsync(&nnew, d_nnew.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        MPISDF_HIP(hipStreamSynchronize(s));
+        Cprev = Ceff;
         if (k > 0) {
-            // Two passes of classical Gram-Schmidt: H = Q^T R, R -= Q H (via Qt).
+            // Classical Gram-Schmidt, H = Q^T R, R -= Q H (via Qt): one pass over
+            // every candidate, a second over the new ones (the persisting ones
+            // had their two passes when they entered).
             const int64_t ldh = (k + 3) / 4 * 4;
             if (H.count < (size_t)(ldh * Ceff)) H.resize((size_t)(ldh * Ceff));
             for (int pass = 0; pass < 2; ++pass) {
-                gemm_tn_f64(k, Ceff, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
-                gemm_tn_f64(m, Ceff, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
+                const int64_t cols = pass == 0 ? Ceff : nnew;
+                if (cols <= 0) continue;
+                gemm_tn_f64(k, cols, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
+                gemm_tn_f64(m, cols, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
             }
         }
         hipLaunchKernelGGL(cand_norm_kernel, dim3((unsigned)((Ceff + 3) / 4)), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
