@@ -128,10 +128,12 @@ __global__ void pair_error_kernel(const double* __restrict__ z, int64_t k, const
     if (lane == 0 && g < nb) { atomicAdd(&out[0], e); atomicAdd(&out[1], t); }
 }
 
-// The same sums from precomputed ||z_g||^2 (zz): qn = ||phi_g||^4,
-// acc += |w_g| max(qn - zz_g, 0), tot += |w_g| qn; one wave per point.
-__global__ void pair_error_norms_kernel(const double* __restrict__ zz, const double* __restrict__ phi, int64_t d,
-                                        const double* __restrict__ w, int64_t nb, double* __restrict__ out)
+// The same sums from the row-group partials of ||z_g||^2 (zpart[r * nb + g],
+// added in row-group order): qn = ||phi_g||^4, acc += |w_g| max(qn - zz_g, 0),
+// tot += |w_g| qn; one wave per point.
+__global__ void pair_error_norms_kernel(const double* __restrict__ zpart, int64_t groups,
+                                        const double* __restrict__ phi, int64_t d, const double* __restrict__ w,
+                                        int64_t nb, double* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
     const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
@@ -141,8 +143,10 @@ __global__ void pair_error_norms_kernel(const double* __restrict__ zz, const dou
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) pp += __shfl_xor(pp, off);
     if (lane == 0) {
+        double zz = 0;
+        for (int64_t r = 0; r < groups; ++r) zz += zpart[r * nb + g];
         const double qn = pp * pp, aw = fabs(w[g]);
-        atomicAdd(&out[0], aw * fmax(qn - zz[g], 0.0));
+        atomicAdd(&out[0], aw * fmax(qn - zz, 0.0));
         atomicAdd(&out[1], aw * qn);
     }
 }
@@ -299,7 +303,8 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     double gemm_flops = 0;
     batch = std::max<int64_t>(1, std::min(batch, n));
     DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
-    DeviceBuffer<double> zz((size_t)batch);
+    const int64_t zgroups = (k + 127) / 128;  // row groups of the Z product
+    DeviceBuffer<double> zz((size_t)(zgroups * batch));
     DeviceBuffer<double> th, thr, tsum;
     std::vector<double> stage;
     wa.upload(anchor_w, k, s);
@@ -320,15 +325,14 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         timer.end(s);
         gemm_flops += 2.0 * (double)k * (double)d * (double)nb;
         // Z = L^-1 S (lower triangular A), with ||z_g||^2 summed in the epilogue
-        MPISDF_HIP(hipMemsetAsync(zz.ptr, 0, nb * sizeof(double), s));
         timer.begin(s);
         gemm_tn_f64_fused(TnFused::ColumnSumSquares, k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, z.ptr, k, zz.ptr,
                           nullptr, nullptr, s, 1);
         timer.end(s);
         gemm_flops += (double)k * (double)(k + 1) * (double)nb;
         if (err_out)
-            hipLaunchKernelGGL(pair_error_norms_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, zz.ptr, phi.ptr,
-                               d, w + start, nb, acc.ptr);
+            hipLaunchKernelGGL(pair_error_norms_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, zz.ptr, zgroups,
+                               phi.ptr, d, w + start, nb, acc.ptr);
         if (theta_host || tsum_out) {
             // theta_phys = L^-T Z (upper triangular A), column-major k x nb;
             // theta = theta_phys / w_anchor

@@ -43,7 +43,10 @@ constexpr int NG = 6;            // accumulator groups, p + r = 0..5
 constexpr int FBM = 128;         // rows per workgroup
 constexpr int FBN = 64;          // columns per workgroup
 constexpr int FBK = 64;          // K per slice (one MFMA)
-constexpr int FPITCH = FBK + 16; // LDS row pitch (bytes): 80 -> conflict-free 16-byte reads
+// LDS row pitch (bytes).  Lanes read 16 bytes at row * FPITCH + 16 q; on
+// gfx950 a pitch of 32 mod 64 bytes keeps these ds_read_b128 free of bank
+// conflicts (measured with SQ_LDS_BANK_CONFLICT: 80-byte rows conflict two ways).
+constexpr int FPITCH = FBK + 32;
 // Word 3 of a raw buffer descriptor on gfx9-class GPUs (32-bit data format).
 constexpr int kBufferWord3 = 0x00020000;
 

@@ -8,9 +8,10 @@
 // v_mfma_f64_16x16x4_f64 takes A[l % 16][k(l)] and B[k(l)][l % 16] from lane
 // l, where the four K values of one step are spread over the lane quarters.
 // The reduction over K does not care which K index a quarter supplies, so
-// quarter g feeds K = 4 g + s at step s of a 16-wide K chunk: every lane then
-// consumes four consecutive K values of one row / column, which it loads as a
-// single 16- or 32-byte vector.  Result element i of lane l is
+// quarter g feeds K = 2 g + s (s = 0, 1) and 8 + 2 g + s - 2 (s = 2, 3) of a
+// 16-wide K chunk: every lane then consumes two pairs of consecutive K values
+// of one row / column, read as two 16-byte LDS vectors (A) and two 8- or
+// 16-byte global vectors (B).  Result element i of lane l is
 // D[l / 16 + 4 i][l % 16].
 //
 // Workgroup: 4 waves, 128 columns (32 per wave = 2 column tiles) x up to 128
@@ -29,10 +30,46 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int TBM = 128;       // rows per workgroup (row group)
 constexpr int TBN = 128;       // columns per workgroup
 constexpr int TBK = 32;        // K per slice
-constexpr int LDA_S = TBK + 2; // LDS row pitch (doubles): 68 dwords, conflict-free b128 reads
+// LDS row pitch (doubles): 288 bytes.  Each lane reads 16 bytes at row * 288 +
+// 16 q (+ 64): on gfx950 a pitch of 32 mod 64 bytes keeps these ds_read_b128
+// free of bank conflicts (measured with SQ_LDS_BANK_CONFLICT; 272-byte rows
+// and a 32-byte lane stride conflict two ways).
+constexpr int LDA_S = TBK + 4;
 
 enum TnEpilogue { kTnStore = 0, kTnFold = 1, kTnFoldEnergy = 2, kTnKhatriRao = 3,
                   kTnStoreSquare = 4, kTnStoreColSq = 5, kTnStoreRowDot = 6 };
+
+template <class TB>
+struct BVec2;
+template <>
+struct BVec2<float> {
+    typedef float2 type;
+};
+template <>
+struct BVec2<double> {
+    typedef double2 type;
+};
+
+// out[0..1] = p[ka], p[ka + 1]; out[2..3] = p[kb], p[kb + 1] (zero past K or
+// for an absent column).
+template <class TB, bool VEC>
+__device__ __forceinline__ void load2x2(const TB* __restrict__ p, int64_t ka, int64_t kb, int64_t K, bool col_ok,
+                                        TB out[4])
+{
+    const int64_t ks[2] = {ka, kb};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int64_t k = ks[h];
+        if (VEC && col_ok && k + 2 <= K) {
+            const typename BVec2<TB>::type v = *reinterpret_cast<const typename BVec2<TB>::type*>(p + k);
+            out[2 * h] = v.x;
+            out[2 * h + 1] = v.y;
+        } else {
+            out[2 * h] = (col_ok && k < K) ? p[k] : TB(0);
+            out[2 * h + 1] = (col_ok && k + 1 < K) ? p[k + 1] : TB(0);
+        }
+    }
+}
 
 template <class TB>
 struct BVec;
@@ -112,7 +149,8 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
             *reinterpret_cast<double2*>(dst + 2) = make_double2(qa[r][2], qa[r][3]);
         }
     };
-    // B: per column tile and 16-K half, four consecutive K of column l16.
+    // B: per column tile and 16-K half, the four K of this lane's quarter q:
+    // 2q, 2q + 1, 8 + 2q, 9 + 2q (the order the A reads from LDS supply).
     TB yb[2][2][4], yn[2][2][4];  // raw B values, widened at use
     const bool ok0 = n0 + l16 < N, ok1 = n0 + 16 + l16 < N;
     const TB* b0 = B + (ok0 ? (n0 + l16) : 0) * ldb;
@@ -120,9 +158,9 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
     auto load_b = [&](int64_t k0, TB dst[2][2][4]) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int64_t k = k0 + 16 * h + 4 * quarter;
-            load4<TB, VEC, TB>(b0, k, K, ok0, dst[0][h]);
-            load4<TB, VEC, TB>(b1, k, K, ok1, dst[1][h]);
+            const int64_t k = k0 + 16 * h + 2 * quarter;
+            load2x2<TB, VEC>(b0, k, k + 8, K, ok0, dst[0][h]);
+            load2x2<TB, VEC>(b1, k, k + 8, K, ok1, dst[1][h]);
         }
     };
 
@@ -150,9 +188,9 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 if (t < R) {
-                    const double* ap = As + (t * 16 + l16) * LDA_S + 16 * h + 4 * quarter;
+                    const double* ap = As + (t * 16 + l16) * LDA_S + 16 * h + 2 * quarter;
                     const double2 a01 = *reinterpret_cast<const double2*>(ap);
-                    const double2 a23 = *reinterpret_cast<const double2*>(ap + 2);
+                    const double2 a23 = *reinterpret_cast<const double2*>(ap + 8);
                     const double av[4] = {a01.x, a01.y, a23.x, a23.y};
 #pragma unroll
                     for (int s = 0; s < 4; ++s) {
@@ -223,7 +261,9 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
             }
         }
         if (EPI == kTnStoreColSq) {
-            // colacc[n] += sum over this row group of (alpha acc)^2; rows >= M hold zeros.
+            // colacc[g * N + n] = sum over row group g of (alpha acc)^2 (rows >=
+            // M hold zeros); a partial per row group keeps the total independent
+            // of the order the workgroups finish in.
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
                 double s = 0;
@@ -234,7 +274,7 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
                 s += __shfl_xor(s, 16);
                 s += __shfl_xor(s, 32);
                 const int64_t gn = n0 + 16 * c + l16;
-                if (quarter == 0 && gn < N) atomicAdd(&colacc[gn], alpha * alpha * s);
+                if (quarter == 0 && gn < N) colacc[(int64_t)blockIdx.y * N + gn] = alpha * alpha * s;
             }
         }
         if (EPI == kTnStoreRowDot) {

@@ -53,7 +53,8 @@ void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A,
                  int64_t ldb, double beta, double* C, int64_t ldc, hipStream_t stream, int triangular = 0);
 // gemm_tn_f64 with a fused epilogue (C column-major M x N, beta = 0):
 //   Square:            C = (A^T B)^2 elementwise (alpha ignored);
-//   ColumnSumSquares:  C = alpha A^T B and colacc[n] += sum_m C[m, n]^2;
+//   ColumnSumSquares:  C = alpha A^T B and colacc[g * N + n] = sum over the
+//                      rows m of row group g (128 rows each) of C[m, n]^2;
 //   RowDot:            C = alpha A^T B and rowacc[m] += sum_n C[m, n] colw[n].
 enum class TnFused { Square, ColumnSumSquares, RowDot };
 void gemm_tn_f64_fused(TnFused mode, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
