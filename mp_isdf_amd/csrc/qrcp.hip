@@ -41,6 +41,17 @@ struct ArgMax {
     int64_t column;
 };
 
+// Device-side state of the pivot loop: pivots accepted so far, the stop
+// reason of the current block (0 running, 1 block ends, 2 factorisation
+// done) and |R_00|.
+struct PivotState {
+    int64_t k;
+    int32_t stop;
+    int32_t pad;
+    double r00;
+};
+constexpr int kStepsPerSync = 16;  // pivot steps queued between host reads of the state
+
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -199,10 +210,22 @@ __global__ void cand_norm_kernel(const double* __restrict__ rc, int64_t m, int64
     if (lane == 0) rn2[c] = s;
 }
 
-__global__ __launch_bounds__(1024) void argmax_kernel(const double* __restrict__ rn2,
-                                                      const int64_t* __restrict__ cand, int64_t C,
-                                                      ArgMax* __restrict__ out)
+// Pivot of one step: the exact argmax of the candidate norms (lowest column
+// on ties), then the accept / stop rules (dgeqp3's |R_ii| stop, the numerical
+// rank floor, certification against the non-candidates' bound):
+//   no positive norm            -> stop 2 if every non-candidate is zero, else 1;
+//   |R_ii| <= stop_abs or <= stop_rel |R_00| -> stop 2;
+//   value <= bound (first pivot after a fold: < bound, or forced) -> stop 1;
+//   else record the pivot (piv[k], rdiag[k]) and k += 1.
+__global__ __launch_bounds__(1024) void argmax_select_kernel(const double* __restrict__ rn2,
+                                                             const int64_t* __restrict__ cand, int64_t C,
+                                                             ArgMax* __restrict__ out, PivotState* __restrict__ st,
+                                                             const double* __restrict__ bound_ptr,
+                                                             int64_t block_start, double stop_abs, double stop_rel,
+                                                             double margin, int force, int64_t* __restrict__ piv,
+                                                             double* __restrict__ rdiag)
 {
+    if (st->stop) return;
     __shared__ double sv[1024];
     __shared__ int64_t ss[1024], sc[1024];
     double best = -2.0;
@@ -223,13 +246,32 @@ __global__ __launch_bounds__(1024) void argmax_kernel(const double* __restrict__
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) { out->value = sv[0]; out->slot = ss[0]; out->column = sc[0]; }
+    if (threadIdx.x == 0) {
+        const ArgMax b{sv[0], ss[0], sc[0]};
+        *out = b;
+        const double bound2 = bound_ptr ? *bound_ptr : -1.0;
+        const int64_t k = st->k;
+        if (b.slot < 0 || !(b.value > 0)) { st->stop = bound2 <= 0 ? 2 : 1; return; }
+        const double norm = sqrt(b.value);
+        if (k == 0) st->r00 = norm;
+        const double r00 = k == 0 ? norm : st->r00;
+        if (norm <= stop_abs || norm <= stop_rel * r00) { st->stop = 2; return; }
+        const bool first = k == block_start;
+        if (!(b.value > bound2 || (first && b.value >= bound2 && margin == 0.0) || (first && force))) {
+            st->stop = 1;
+            return;
+        }
+        piv[k] = b.column;
+        rdiag[k] = norm;
+        st->k = k + 1;
+    }
 }
 
 // q <- r_c* / ||r_c*||  (the norm comes from the exact residual norm)
 __global__ void make_q_kernel(const double* __restrict__ rc, int64_t m, int64_t ldr, const ArgMax* __restrict__ am,
-                              double* __restrict__ q, uint8_t* __restrict__ selected)
+                              double* __restrict__ q, uint8_t* __restrict__ selected, const PivotState* __restrict__ st)
 {
+    if (st->stop) return;
     const double inv = 1.0 / sqrt(am->value);
     const double* src = rc + am->slot * ldr;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
@@ -239,8 +281,9 @@ __global__ void make_q_kernel(const double* __restrict__ rc, int64_t m, int64_t 
 
 // d[t] = Q[:, t0 + t]^T q, one workgroup per t.
 __global__ void dots_kernel(const double* __restrict__ Q, int64_t m, int64_t ldq, int64_t t0,
-                            const double* __restrict__ q, double* __restrict__ d)
+                            const double* __restrict__ q, double* __restrict__ d, const PivotState* __restrict__ st)
 {
+    if (st->stop) return;
     __shared__ double part[4];
     const double* col = Q + (t0 + blockIdx.x) * ldq;
     double s = 0;
@@ -253,8 +296,9 @@ __global__ void dots_kernel(const double* __restrict__ Q, int64_t m, int64_t ldq
 
 // q -= sum_t d[t] Q[:, t0 + t]; then written (unnormalised) to out
 __global__ void reorth_kernel(double* __restrict__ q, const double* __restrict__ Q, int64_t m, int64_t ldq,
-                              int64_t t0, int64_t nt, const double* __restrict__ d)
+                              int64_t t0, int64_t nt, const double* __restrict__ d, const PivotState* __restrict__ st)
 {
+    if (st->stop) return;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         double v = q[i];
         for (int64_t t = 0; t < nt; ++t) v -= d[t] * Q[(t0 + t) * ldq + i];
@@ -266,8 +310,9 @@ __global__ void reorth_kernel(double* __restrict__ q, const double* __restrict__
 // row-major copy Qt (qt_k[i * ldqt] = q_i).
 __global__ __launch_bounds__(1024) void store_q_kernel(const double* __restrict__ q, int64_t m,
                                                        double* __restrict__ qk, double* __restrict__ qt_k,
-                                                       int64_t ldqt)
+                                                       int64_t ldqt, const PivotState* __restrict__ st)
 {
+    if (st->stop) return;
     __shared__ double part[16];
     double s = 0;
     for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += q[i] * q[i];
@@ -292,8 +337,10 @@ __global__ __launch_bounds__(1024) void store_q_kernel(const double* __restrict_
 // pivot's own slot is retired (rn2 = -1).
 __global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ rc, int64_t m, int64_t ldr,
                                                           int64_t C, const double* __restrict__ q,
-                                                          double* __restrict__ rn2, const ArgMax* __restrict__ am)
+                                                          double* __restrict__ rn2, const ArgMax* __restrict__ am,
+                                                          const PivotState* __restrict__ st)
 {
+    if (st->stop) return;
     __shared__ double part[4];
     __shared__ double hsh;
     const int64_t c = blockIdx.x;
@@ -388,6 +435,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     MPISDF_HIP(hipMemsetAsync(Qt.ptr, 0, Qt.count * sizeof(double), s));
     DeviceBuffer<double> energy(std::max<int64_t>(kmax, 1)), tail(1);
     DeviceBuffer<ArgMax> am(1);
+    DeviceBuffer<PivotState> pstate(1);
+    DeviceBuffer<int64_t> piv_dev(std::max<int64_t>(kmax, 1));
+    DeviceBuffer<double> rdiag_dev(std::max<int64_t>(kmax, 1));
+    MPISDF_HIP(hipMemsetAsync(pstate.ptr, 0, sizeof(PivotState), s));
     MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
     MPISDF_HIP(hipMemsetAsync(err2.ptr, 0, n * sizeof(double), s));
     MPISDF_HIP(hipMemsetAsync(selected.ptr, 0, n, s));
@@ -418,7 +469,6 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
 
 
     int64_t k = 0, folded = 0;
-    double r00 = 0;
     bool done = false;
     FoldScratch fold_scratch;
     FoldI8Scratch i8_scratch;
@@ -516,43 +566,56 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         ++res.blocks;
         const int64_t block_start = k;
         if (dv.count < (size_t)std::max<int64_t>(kmax, 1)) dv.resize(std::max<int64_t>(kmax, 1));
-        while (k < kmax) {
-            hipLaunchKernelGGL(argmax_kernel, dim3(1), dim3(1024), 0, s, rn2.ptr, cand.ptr, Ceff, am.ptr);
+        // Pivot steps run on the device in groups of kStepsPerSync: each step's
+        // argmax kernel decides (certification against the non-candidates'
+        // bound: the first pivot after a fold may tie it, as candidates are
+        // the lowest-index columns among equal keys) and the step's other
+        // kernels return at once after a stop.  Step i of a group assumes the
+        // i pivots before it were accepted, which holds for every step that
+        // runs: a stop ends the block.
+        const double* bound_ptr = Ceff < remaining ? keys_sorted.ptr + Ceff : nullptr;
+        MPISDF_HIP(hipMemsetAsync(&pstate.ptr->stop, 0, sizeof(int32_t), s));
+        bool block_over = false;
+        while (!block_over && k < kmax) {
+            const int64_t steps = std::min<int64_t>(kStepsPerSync, kmax - k);
+            for (int64_t i = 0; i < steps; ++i) {
+                const int64_t kk = k + i, nt = kk - block_start;
+                hipLaunchKernelGGL(argmax_select_kernel, dim3(1), dim3(1024), 0, s, rn2.ptr, cand.ptr, Ceff, am.ptr,
+                                   pstate.ptr, bound_ptr, block_start, opt.stop_abs, opt.stop_rel, opt.margin,
+                                   force ? 1 : 0, piv_dev.ptr, rdiag_dev.ptr);
+                hipLaunchKernelGGL(make_q_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, rc.ptr, m, ldr, am.ptr,
+                                   qv.ptr, selected.ptr, pstate.ptr);
+                if (nt > 0) {
+                    hipLaunchKernelGGL(dots_kernel, dim3((unsigned)nt), dim3(256), 0, s, Q.ptr, m, ldq, block_start,
+                                       qv.ptr, dv.ptr, pstate.ptr);
+                    hipLaunchKernelGGL(reorth_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, qv.ptr, Q.ptr, m, ldq,
+                                       block_start, nt, dv.ptr, pstate.ptr);
+                }
+                hipLaunchKernelGGL(store_q_kernel, dim3(1), dim3(1024), 0, s, qv.ptr, m, Q.ptr + kk * ldq, Qt.ptr + kk,
+                                   ldqt, pstate.ptr);
+                hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
+                                   Q.ptr + kk * ldq, rn2.ptr, am.ptr, pstate.ptr);
+            }
             MPISDF_HIP(hipGetLastError());
-            ArgMax best{};
-            MPISDF_HIP(hipMemcpyAsync(&best, am.ptr, sizeof(ArgMax), hipMemcpyDeviceToHost, s));
+            PivotState h{};
+            MPISDF_HIP(hipMemcpyAsync(&h, pstate.ptr, sizeof(PivotState), hipMemcpyDeviceToHost, s));
             MPISDF_HIP(hipStreamSynchronize(s));
-            if (best.slot < 0 || !(best.value > 0)) {
-                if (bound2 <= 0) done = true;  // every remaining column is exactly zero
-                break;
+            if (h.k < k || h.k > k + steps) throw std::logic_error("QRCP pivot state out of range");
+            k = h.k;
+            if (h.stop) {
+                block_over = true;
+                if (h.stop == 2) done = true;
             }
-            const double norm = std::sqrt(best.value);
-            if (k == 0) r00 = norm;
-            if (norm <= opt.stop_abs || norm <= opt.stop_rel * r00) { done = true; break; }
-            // Certification against the stale bound of the non-candidates.  The
-            // first pivot after a fold may tie the bound: candidates are the
-            // lowest-index columns among equal keys (stable sort).
-            const bool first_after_fold = (k == block_start);
-            if (!(best.value > bound2 || (first_after_fold && best.value >= bound2 && opt.margin == 0.0) ||
-                  (first_after_fold && force)))
-                break;
-            res.pivots.push_back(best.column);
-            res.rdiag.push_back(norm);
-            hipLaunchKernelGGL(make_q_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, rc.ptr, m, ldr, am.ptr, qv.ptr,
-                               selected.ptr);
-            const int64_t nt = k - block_start;
-            if (nt > 0) {
-                hipLaunchKernelGGL(dots_kernel, dim3((unsigned)nt), dim3(256), 0, s, Q.ptr, m, ldq, block_start, qv.ptr,
-                                   dv.ptr);
-                hipLaunchKernelGGL(reorth_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, qv.ptr, Q.ptr, m, ldq,
-                                   block_start, nt, dv.ptr);
-            }
-            hipLaunchKernelGGL(store_q_kernel, dim3(1), dim3(1024), 0, s, qv.ptr, m, Q.ptr + k * ldq, Qt.ptr + k,
-                               ldqt);
-            hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
-                               Q.ptr + k * ldq, rn2.ptr, am.ptr);
-            MPISDF_HIP(hipGetLastError());
-            ++k;
+        }
+        if (k > block_start) {
+            const int64_t nb = k - block_start;
+            res.pivots.resize(k);
+            res.rdiag.resize(k);
+            MPISDF_HIP(hipMemcpyAsync(res.pivots.data() + block_start, piv_dev.ptr + block_start, nb * sizeof(int64_t),
+                                      hipMemcpyDeviceToHost, s));
+            MPISDF_HIP(hipMemcpyAsync(res.rdiag.data() + block_start, rdiag_dev.ptr + block_start, nb * sizeof(double),
+                                      hipMemcpyDeviceToHost, s));
+            MPISDF_HIP(hipStreamSynchronize(s));
         }
         if (opt.debug)
             fprintf(stderr, "[qrcp] block %lld: pivots %lld..%lld candidates %lld bound2 %.3e last_r %.3e folds %lld\n",
@@ -567,6 +630,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             ++res.stalls;
             if (Ceff < remaining && C < Cmax) C = std::min(2 * C, Cmax);
             else if (split && !refreshed) { refresh(); refreshed = true; }
+            else if (force) break;  // even a forced pivot has no positive norm: nothing is left to factorise
             else force = true;
         } else {
             refreshed = false;
