@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--config", default="h2o64", choices=sorted(CONFIGS))
     p.add_argument("--precision", default="bf16", choices=["bf16", "fp8", "fp64"])
     p.add_argument("--no-fp64-form", action="store_true", help="skip the all-FP64 comparison run")
-    p.add_argument("--cpu-points", type=int, default=768, help="grid points in the CPU baseline sample")
+    p.add_argument("--cpu-points", type=int, default=12288, help="grid points in the CPU baseline sample (~10-15 s of work)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="finish with three device-to-device copies of this many MiB (a known-size kernel for "
