@@ -334,11 +334,15 @@ __global__ __launch_bounds__(1024) void store_q_kernel(const double* __restrict_
 }
 
 // For every candidate c: h = q^T r_c; r_c -= h q; rn2[c] = ||r_c||^2.  The
-// pivot's own slot is retired (rn2 = -1).
+// pivot's own slot is retired (rn2 = -1), and so is a candidate whose norm
+// falls to the block's bound or below: norms only decrease and only the first
+// pivot after a fold may tie the bound, so it cannot be accepted again in
+// this block, and its column is not read or written again.
 __global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ rc, int64_t m, int64_t ldr,
                                                           int64_t C, const double* __restrict__ q,
                                                           double* __restrict__ rn2, const ArgMax* __restrict__ am,
-                                                          const PivotState* __restrict__ st)
+                                                          const PivotState* __restrict__ st,
+                                                          const double* __restrict__ bound_ptr)
 {
     if (st->stop) return;
     __shared__ double part[4];
@@ -368,7 +372,10 @@ __global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ r
     __syncthreads();
     if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = n2;
     __syncthreads();
-    if (threadIdx.x == 0) rn2[c] = part[0] + part[1] + part[2] + part[3];
+    if (threadIdx.x == 0) {
+        const double v = part[0] + part[1] + part[2] + part[3];
+        rn2[c] = (bound_ptr && v <= *bound_ptr) ? -1.0 : v;
+    }
 }
 
 // tail2 += sum (Y - Q W)^2 over a column chunk; R holds Y chunk - Q W (m x nc).
@@ -594,7 +601,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                 hipLaunchKernelGGL(store_q_kernel, dim3(1), dim3(1024), 0, s, qv.ptr, m, Q.ptr + kk * ldq, Qt.ptr + kk,
                                    ldqt, pstate.ptr);
                 hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
-                                   Q.ptr + kk * ldq, rn2.ptr, am.ptr, pstate.ptr);
+                                   Q.ptr + kk * ldq, rn2.ptr, am.ptr, pstate.ptr, bound_ptr);
             }
             MPISDF_HIP(hipGetLastError());
             PivotState h{};
