@@ -186,3 +186,63 @@ def test_qrcp_digit_planes_match_oracle(mp_isdf):
     piv32, _ = gpu_qrcp(y32, 1, m, None, None)
     _, want32, _ = oracle.geqp3(np.ascontiguousarray(y.T))
     np.testing.assert_array_equal(piv32, want32[:kmax])
+
+
+def test_qrcp_stop_rules_and_rank_exhaustion(mp_isdf):
+    """The QRCP's device-side stop rules: the tolerance stop |R_ii| <=
+    tol |R_00| ends at the oracle's index with the oracle's pivots (FP64, FP32
+    and digit-plane inputs), and a sketch with only 12 nonzero columns ends
+    after exactly those 12 pivots instead of running to kmax."""
+    import torch
+    rng = np.random.default_rng(17)
+    m, n = 96, 2000
+    base = rng.normal(size=(n, 12)) @ rng.normal(size=(12, m))
+    y = (base * np.exp(-np.linspace(0, 8, n))[:, None] + 1e-6 * rng.normal(size=(n, m)))
+    y = y.astype(np.float32).astype(np.float64)
+    f, order, _ = oracle.geqp3(np.ascontiguousarray(y.T))
+    rdiag = np.abs(np.diag(f))
+
+    def run(buf, ytype, ld, norms, scales, kmax, tol):
+        piv = np.zeros(kmax, dtype=np.int64)
+        rd = np.zeros(kmax)
+        cnt = np.zeros(1, dtype=np.int64)
+        capi.call("mpisdf_qrcp_device", buf, ytype, m, n, ld, norms, scales, kmax, tol, piv, rd, cnt, None)
+        return piv[:cnt[0]]
+
+    tol = 1e-3
+    stop = int(np.argmax(rdiag <= tol * rdiag[0]))
+    assert 0 < stop < m
+    got64 = run(torch.tensor(y, device="cuda"), 0, m, None, None, m, tol)
+    got32 = run(torch.tensor(y.astype(np.float32), device="cuda"), 1, m, None, None, m, tol)
+    np.testing.assert_array_equal(got64, order[:stop])
+    np.testing.assert_array_equal(got32, order[:stop])
+
+    P = (m + 15) // 16 * 16
+    buf = np.zeros((n, P), dtype=np.float32)
+    buf[:, :m] = y
+    Y = torch.tensor(buf, device="cuda")
+    yn = torch.tensor(np.sum(y ** 2, axis=1), device="cuda")
+    sc = torch.zeros(n, dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
+    _, _, y_hat = _digit_model(y, np.sum(y ** 2, axis=1))
+    fh, order_h, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
+    rh = np.abs(np.diag(fh))
+    stop_h = int(np.argmax(rh <= tol * rh[0]))
+    np.testing.assert_array_equal(run(Y, 4, P, yn, sc, m, tol), order_h[:stop_h])
+
+    # only 12 nonzero columns: every path stops after them
+    z = np.zeros_like(y)
+    live = rng.choice(n, size=12, replace=False)
+    z[live] = rng.normal(size=(12, m))
+    z = z.astype(np.float32).astype(np.float64)
+    _, order_z, _ = oracle.geqp3(np.ascontiguousarray(z.T))
+    for ytype in (0, 1):
+        t = torch.tensor(z if ytype == 0 else z.astype(np.float32), device="cuda")
+        np.testing.assert_array_equal(run(t, ytype, m, None, None, 40, 0.0), order_z[:12])
+    zbuf = np.zeros((n, P), dtype=np.float32)
+    zbuf[:, :m] = z
+    Z = torch.tensor(zbuf, device="cuda")
+    zn = torch.tensor(np.sum(z ** 2, axis=1), device="cuda")
+    zs = torch.zeros(n, dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_digitize_columns_device", Z, m, n, P, zn, zs, None)
+    assert sorted(run(Z, 4, P, zn, zs, 40, 0.0).tolist()) == sorted(live.tolist())
