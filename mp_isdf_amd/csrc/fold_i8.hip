@@ -21,13 +21,12 @@
 // The dropped pairs and the truncation of q leave ~1e-11 of ||y_j||^2 in the
 // folded sums (measured), covered by the QRCP's relative margin.
 //
-// Lane layout of v_mfma_i32_16x16x64_i8: A row l % 16 and B column l % 16,
-// each supplying 16 consecutive K bytes at offset 16 (l / 16); result element
-// e of lane l is D[4 (l / 16) + e][l % 16].  Workgroup: 4 waves x 16 columns
-// x up to 128 rows (one launch per row group).  The Q digit slice (5 planes x
-// 128 rows x 64 K) is double-buffered in LDS; Y digit fragments go straight
-// from HBM to registers through a four-slice register ring (three slices of
-// prefetch cover HBM latency at one wave per SIMD).
+// Two kernels share the digit formats.  Row groups of more than 64 rows run
+// on v_mfma_i32_32x32x32_i8 (on gfx950 about 1.7x the rate of the 16x16x64
+// form), with both Q and Y digit slices staged through double-buffered LDS and
+// loads issued two slices ahead.  Groups of up to 64 rows run on
+// v_mfma_i32_16x16x64_i8 with Y digits straight from HBM to registers.  Lane
+// layouts and workgroup shapes are described at each kernel.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -41,12 +40,17 @@ constexpr int ND = 5;            // digits of q
 constexpr int NY = 4;            // digits of y (planes per column)
 constexpr int NG = 6;            // accumulator groups, p + r = 0..5
 constexpr int FBM = 128;         // rows per workgroup
-constexpr int FBN = 64;          // columns per workgroup
-constexpr int FBK = 64;          // K per slice (one MFMA)
-// LDS row pitch (bytes).  Lanes read 16 bytes at row * FPITCH + 16 q; on
-// gfx950 a pitch of 32 mod 64 bytes keeps these ds_read_b128 free of bank
-// conflicts (measured with SQ_LDS_BANK_CONFLICT: 80-byte rows conflict two ways).
-constexpr int FPITCH = FBK + 32;
+constexpr int FBN = 64;          // columns per workgroup (two tiles of 32)
+constexpr int FBK = 64;          // K bytes per slice (two MFMA K-steps)
+// LDS row pitches (bytes) of the digit slices, chosen so the operand reads
+// are free of bank conflicts on gfx950 (measured with SQ_LDS_BANK_CONFLICT):
+// * 32x32x32 kernel: lanes read 16 bytes at row * P + 32 h + 16 s with
+//   row = lane % 32, h = lane / 32; 80 bytes is conflict-free (64, 96 and 128
+//   conflict);
+// * 16x16x64 kernel: lanes read 16 bytes at row * P + 16 q with row = lane %
+//   16, q = lane / 16; a pitch of 32 mod 64 bytes is conflict-free (80 conflicts).
+constexpr int FPITCH32 = FBK + 16;
+constexpr int FPITCH16 = FBK + 32;
 // Word 3 of a raw buffer descriptor on gfx9-class GPUs (32-bit data format).
 constexpr int kBufferWord3 = 0x00020000;
 
@@ -132,6 +136,188 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
     }
 }
 
+typedef int i16v __attribute__((ext_vector_type(16)));
+
+// NR (1, 2 or 4): row tiles of 32 in this launch's row group.  The
+// workgroup has 2 NR waves; wave w computes rows 32 (w / 2) .. +32 of the
+// group for the 32 columns 32 (w % 2) .. +32, all six digit levels at once
+// (v_mfma_i32_32x32x32_i8: A row l % 32 and B column l % 32 take 16 K bytes
+// at offset 16 (l / 32); result element i of lane l is
+// D[8 (i / 4) + 4 (l / 32) + i % 4][l % 32]).  A K slice is 64 bytes, two
+// MFMA K-steps: lane half h supplies K bytes 32 h + 16 s at step s, so each
+// lane reads 32 contiguous bytes of a plane per slice.  Rows past b carry
+// zero digits; there are no branches in the MFMA sequence (a branch would
+// make the compiler drain every outstanding load at its join).
+template <bool ENERGY, int NR>
+__global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
+                                                                   const double* __restrict__ qs, int64_t ldq,
+                                                                   int64_t bpad, int64_t b, int64_t m0,
+                                                                   const int8_t* __restrict__ yd, int64_t P,
+                                                                   const double* __restrict__ yscale, int64_t m,
+                                                                   int64_t n, double* __restrict__ proj2,
+                                                                   double* __restrict__ energy)
+{
+    constexpr int ROWS = 32 * NR;  // Q rows staged per slice
+    constexpr int NTHREADS = 128 * NR;
+    __shared__ __align__(16) int8_t sQ[2][ND][ROWS * FPITCH32];
+    __shared__ __align__(16) int8_t sY[2][NY][FBN * FPITCH32];
+    __shared__ double colred[NR][FBN];
+    __shared__ double rowred[2][ROWS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int half = lane >> 5, l32 = lane & 31;
+    const int rt = wave >> 1, ct = wave & 1;
+    const int64_t g0 = (int64_t)blockIdx.x * FBN;
+    const int64_t col = g0 + ct * 32 + l32;  // this lane's Y column
+    i16v acc[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[g][e] = 0;
+
+    const bool cok = col < n;
+    const double sj = cok ? yscale[col] : 0.0;
+    // Raw buffer loads: an offset past a descriptor's range returns zeros.
+    // K offsets at or past a plane's pitch are sent out of range, so slices
+    // past the last one read zeros and add nothing.  The host keeps every
+    // offset below 2^31.
+    const int64_t ycols = std::min<int64_t>(FBN, n - g0);
+    const __amdgpu_buffer_rsrc_t yrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
+    const __amdgpu_buffer_rsrc_t qrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<int8_t*>(qd), (short)0, (int)(ND * bpad * ldq), kBufferWord3);
+    // Y digits: NY planes x FBN columns x 64 bytes per slice, staged through
+    // LDS once per workgroup and shared by the NR row-tile waves.
+    constexpr int YCHUNKS = NY * FBN * (FBK / 16) / NTHREADS;
+    static_assert(YCHUNKS * NTHREADS == NY * FBN * (FBK / 16), "Y slice must split evenly over the threads");
+    uint4 ryA[YCHUNKS], ryB[YCHUNKS];
+#define MPISDF_LOAD_YD(k0, ry)                                                                        \
+    do {                                                                                             \
+        _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
+            const int c_ = tid + NTHREADS * r_;                                                      \
+            const int p_ = c_ / (FBN * 4), rem_ = c_ - p_ * FBN * 4, col_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
+            const uint32_t k_ = (uint32_t)(k0) + (uint32_t)kc_;                                      \
+            const uint32_t off_ = k_ < (uint32_t)P ? (uint32_t)(col_ * 4 * P + p_ * P) + k_ : 0x80000000u; \
+            ry[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
+        }                                                                                            \
+    } while (0)
+#define MPISDF_STASH_Y(buf, ry)                                                                       \
+    do {                                                                                             \
+        _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
+            const int c_ = tid + NTHREADS * r_;                                                      \
+            const int p_ = c_ / (FBN * 4), rem_ = c_ - p_ * FBN * 4, col_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
+            *reinterpret_cast<uint4*>(&sY[buf][p_][col_ * FPITCH32 + kc_]) = ry[r_];                   \
+        }                                                                                            \
+    } while (0)
+    // Q digits: ND planes x ROWS rows x 64 bytes per slice, five 16-byte
+    // chunks per thread.
+    constexpr int QCHUNKS = ND * ROWS * (FBK / 16) / NTHREADS;
+    static_assert(QCHUNKS * NTHREADS == ND * ROWS * (FBK / 16), "Q slice must split evenly over the threads");
+    uint4 rqA[QCHUNKS], rqB[QCHUNKS];
+#define MPISDF_LOAD_Q(k0, rq)                                                                         \
+    do {                                                                                             \
+        const uint32_t k_ = (uint32_t)(k0);                                                          \
+        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
+            const int c_ = tid + NTHREADS * r_;                                                      \
+            const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
+            const uint32_t off_ = k_ < (uint32_t)ldq                                                 \
+                                      ? (uint32_t)(((int64_t)p_ * bpad + m0 + row_) * ldq) + k_ + (uint32_t)kc_ \
+                                      : 0x80000000u;                                                 \
+            rq[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(qrsrc, off_, 0, 0)); \
+        }                                                                                            \
+    } while (0)
+#define MPISDF_STASH_Q(buf, rq)                                                                       \
+    do {                                                                                             \
+        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
+            const int c_ = tid + NTHREADS * r_;                                                      \
+            const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
+            *reinterpret_cast<uint4*>(&sQ[buf][p_][row_ * FPITCH32 + kc_]) = rq[r_];                   \
+        }                                                                                            \
+    } while (0)
+#define MPISDF_MMA(buf)                                                                               \
+    do {                                                                                             \
+        _Pragma("unroll") for (int s = 0; s < 2; ++s) {                                              \
+            i4v yv[NY];                                                                              \
+            _Pragma("unroll") for (int r = 0; r < NY; ++r)                                           \
+                yv[r] = *reinterpret_cast<const i4v*>(                                               \
+                    &sY[buf][r][(ct * 32 + l32) * FPITCH32 + 32 * half + 16 * s]);                     \
+            _Pragma("unroll") for (int p = 0; p < ND; ++p) {                                         \
+                const i4v a = *reinterpret_cast<const i4v*>(                                         \
+                    &sQ[buf][p][(rt * 32 + l32) * FPITCH32 + 32 * half + 16 * s]);                     \
+                _Pragma("unroll") for (int r = 0; r + p < NG && r < NY; ++r)                         \
+                    acc[p + r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, yv[r], acc[p + r], 0, 0, 0); \
+            }                                                                                        \
+        }                                                                                            \
+    } while (0)
+    // One slice: the digits of the slice two ahead are loaded into one
+    // register set, this slice is multiplied from LDS, and the other register
+    // set (the next slice, loaded one step earlier) is staged into the other
+    // buffer; each load has two slices of compute to arrive in.
+#define MPISDF_STEP(sl, buf, rqL, ryL, rqS, ryS)                                                      \
+    do {                                                                                             \
+        MPISDF_LOAD_Q(((sl) + 2) * FBK, rqL);                                                        \
+        MPISDF_LOAD_YD(((sl) + 2) * FBK, ryL);                                                       \
+        __builtin_amdgcn_sched_barrier(0);                                                           \
+        MPISDF_MMA(buf);                                                                             \
+        MPISDF_STASH_Q((buf) ^ 1, rqS);                                                              \
+        MPISDF_STASH_Y((buf) ^ 1, ryS);                                                              \
+        __syncthreads();                                                                             \
+    } while (0)
+
+    const int64_t nsl = (m + FBK - 1) / FBK;
+    MPISDF_LOAD_Q(0, rqA);
+    MPISDF_LOAD_YD(0, ryA);
+    MPISDF_LOAD_Q(FBK, rqB);
+    MPISDF_LOAD_YD(FBK, ryB);
+    MPISDF_STASH_Q(0, rqA);
+    MPISDF_STASH_Y(0, ryA);
+    __syncthreads();
+    for (int64_t sl = 0; sl < nsl; sl += 2) {
+        MPISDF_STEP(sl, 0, rqA, ryA, rqB, ryB);
+        MPISDF_STEP(sl + 1, 1, rqB, ryB, rqA, ryA);
+    }
+#undef MPISDF_STEP
+#undef MPISDF_STASH_Y
+#undef MPISDF_MMA
+#undef MPISDF_LOAD_YD
+#undef MPISDF_LOAD_Q
+#undef MPISDF_STASH_Q
+
+    // W[t][j] = u_t s_j sum_g 128^-(g+2) acc_g.
+    double colsq = 0;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        double v = 0;
+#pragma unroll
+        for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / 128.0) + (double)acc[g][e];
+        const int lrow = rt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
+        const int64_t row = m0 + lrow;
+        const double wv = row < b ? v * (1.0 / 16384.0) * qs[row] * sj : 0.0;
+        colsq += wv * wv;
+        if (ENERGY) {
+            double s = wv * wv;
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) s += __shfl_xor(s, off);
+            if (l32 == 0) rowred[ct][lrow] = s;
+        }
+    }
+    colsq += __shfl_xor(colsq, 32);
+    // Column sums over the row tiles in tile order (one partial per wave),
+    // so the folded sums do not depend on the order the waves finish in.
+    if (half == 0) colred[rt][ct * 32 + l32] = colsq;
+    __syncthreads();
+    if (tid < FBN) {
+        const int64_t c = g0 + tid;
+        double sum = 0;
+#pragma unroll
+        for (int t = 0; t < NR; ++t) sum += colred[t][tid];
+        if (c < n) proj2[c] += sum;  // row groups run in order
+    }
+    if (ENERGY && tid < ROWS) {
+        const int64_t row = m0 + tid;
+        if (row < b) atomicAdd(&energy[row], rowred[0][tid] + rowred[1][tid]);
+    }
+}
+
 // RH (1 or 2): row halves of 64 in this launch's row group; the workgroup
 // has 4 RH waves, wave w computing rows 64 (w / 4) .. +64 of the group for
 // the 16 columns 16 (w % 4) .. +16.  Every wave runs the same four row tiles
@@ -140,7 +326,7 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
 // prefetches included.  Two waves per SIMD (RH = 2) let one wave's MFMAs
 // cover the other's waits.
 template <bool ENERGY, int RH>
-__global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_kernel(const int8_t* __restrict__ qd,
+__global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_16x16_kernel(const int8_t* __restrict__ qd,
                                                            const double* __restrict__ qs, int64_t ldq, int64_t bpad,
                                                            int64_t b, int64_t m0, const int8_t* __restrict__ yd,
                                                            int64_t P, const double* __restrict__ yscale, int64_t m,
@@ -150,7 +336,7 @@ __global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_kernel(const int8_t*
     constexpr int ROWS = 64 * RH;  // Q rows staged per slice
     constexpr int NT = 4;          // row tiles of 16 per wave
     constexpr int NTHREADS = 256 * RH;
-    __shared__ __align__(16) int8_t sQ[2][ND][ROWS * FPITCH];
+    __shared__ __align__(16) int8_t sQ[2][ND][ROWS * FPITCH16];
     __shared__ double red[4 * RH][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int quarter = lane >> 4, l16 = lane & 15;
@@ -207,7 +393,7 @@ __global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_kernel(const int8_t*
         _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
             const int c_ = tid + NTHREADS * r_;                                                      \
             const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            *reinterpret_cast<uint4*>(&sQ[buf][p_][row_ * FPITCH + kc_]) = rq[r_];                   \
+            *reinterpret_cast<uint4*>(&sQ[buf][p_][row_ * FPITCH16 + kc_]) = rq[r_];                   \
         }                                                                                            \
     } while (0)
 #define MPISDF_MMA(buf, yv)                                                                           \
@@ -215,7 +401,7 @@ __global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_kernel(const int8_t*
         _Pragma("unroll") for (int t = 0; t < NT; ++t) {                                             \
             _Pragma("unroll") for (int p = 0; p < ND; ++p) {                                         \
                 const i4v a = *reinterpret_cast<const i4v*>(                                         \
-                    &sQ[buf][p][(rh * 64 + t * 16 + l16) * FPITCH + 16 * quarter]);                  \
+                    &sQ[buf][p][(rh * 64 + t * 16 + l16) * FPITCH16 + 16 * quarter]);                  \
                 _Pragma("unroll") for (int r = 0; r + p < NG && r < NY; ++r)                         \
                     acc[p + r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, yv[r], acc[p + r][t], 0, 0, 0); \
             }                                                                                        \
@@ -292,25 +478,30 @@ __global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_kernel(const int8_t*
     }
 }
 
-void launch_fold_i8(int R, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs, int64_t ldq,
-                    int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P, const double* yscale, int64_t m,
-                    int64_t n, double* proj2, double* en)
+// rows: Q^T rows of this launch's group (1..128).  Groups of more than 64
+// rows run on the 32x32x32 kernel (NR = 4, 8 waves); smaller groups on the
+// 16x16x64 kernel with one 64-row half (4 waves, two workgroups per CU),
+// which is faster there: the 32x32x32 kernel's Y slice staging does not
+// shrink with the rows.
+void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs,
+                    int64_t ldq, int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P,
+                    const double* yscale, int64_t m, int64_t n, double* proj2, double* en)
 {
-    if (R <= 0 || R > 8) throw std::logic_error("INT8 fold row tiles out of range");
-    if (R <= 4) {
+    if (rows <= 0 || rows > FBM) throw std::logic_error("INT8 fold row group out of range");
+    if (rows <= 64) {
         if (energy)
-            hipLaunchKernelGGL((fold_i8_kernel<true, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
+            hipLaunchKernelGGL((fold_i8_16x16_kernel<true, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
                                yscale, m, n, proj2, en);
         else
-            hipLaunchKernelGGL((fold_i8_kernel<false, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
-                               yscale, m, n, proj2, static_cast<double*>(nullptr));
+            hipLaunchKernelGGL((fold_i8_16x16_kernel<false, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd,
+                               P, yscale, m, n, proj2, static_cast<double*>(nullptr));
     } else {
         if (energy)
-            hipLaunchKernelGGL((fold_i8_kernel<true, 2>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
+            hipLaunchKernelGGL((fold_i8_32x32_kernel<true, 4>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
                                yscale, m, n, proj2, en);
         else
-            hipLaunchKernelGGL((fold_i8_kernel<false, 2>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
-                               yscale, m, n, proj2, static_cast<double*>(nullptr));
+            hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd,
+                               P, yscale, m, n, proj2, static_cast<double*>(nullptr));
     }
     MPISDF_HIP(hipGetLastError());
 }
@@ -349,11 +540,9 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
     // One launch per group of 128 rows (Q^T rows), each with its own tile count.
-    for (int64_t m0 = 0; m0 < b; m0 += FBM) {
-        const int R = (int)std::min<int64_t>(8, (b - m0 + 15) / 16);
-        launch_fold_i8(R, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr, scratch.scales.ptr, ldq,
-                       bpad, b, m0, Yd, P, yscale, m, n, proj2, energy);
-    }
+    for (int64_t m0 = 0; m0 < b; m0 += FBM)
+        launch_fold_i8(std::min<int64_t>(FBM, b - m0), energy != nullptr, dim3((unsigned)gx), stream,
+                       scratch.digits.ptr, scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, yscale, m, n, proj2, energy);
 }
 
 }  // namespace mpisdf
