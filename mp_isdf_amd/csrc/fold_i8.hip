@@ -14,19 +14,17 @@
 // Products use the splitting method of Ozaki, Ogita, Oishi and Rump (2012) in
 // the INT8 form of Ootomo, Ozaki and Yokota (2024): row t of Qb^T is written
 // as five digits with its own power of two u_t >= 2 max|q_t|; the 17 digit
-// pairs with p + r <= 5 run on v_mfma_i32_16x16x64_i8, pairs of equal p + r
+// pairs with p + r <= 5 run on v_mfma_i32_32x32x32_i8, pairs of equal p + r
 // sharing an exact INT32 accumulator (|sum| < 4 * 64^2 * m < 2^31 for
 // m < 2^16), and the six accumulators are combined in FP64:
 //     W = u_t s_j sum_g 128^-(g+2) acc_g.
 // The dropped pairs and the truncation of q leave ~1e-11 of ||y_j||^2 in the
 // folded sums (measured), covered by the QRCP's relative margin.
 //
-// Two kernels share the digit formats.  Row groups of more than 64 rows run
-// on v_mfma_i32_32x32x32_i8 (on gfx950 about 1.7x the rate of the 16x16x64
-// form), with both Q and Y digit slices staged through double-buffered LDS and
-// loads issued two slices ahead.  Groups of up to 64 rows run on
-// v_mfma_i32_16x16x64_i8 with Y digits straight from HBM to registers.  Lane
-// layouts and workgroup shapes are described at each kernel.
+// v_mfma_i32_32x32x32_i8 runs at about 1.7x the rate of the 16x16x64 form on
+// gfx950.  Q and Y digit slices are staged through double-buffered LDS, with
+// loads issued two slices ahead; the lane layout and workgroup shapes are
+// described at the kernel.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -42,15 +40,23 @@ constexpr int NG = 6;            // accumulator groups, p + r = 0..5
 constexpr int FBM = 128;         // rows per workgroup
 constexpr int FBN = 64;          // columns per workgroup (two tiles of 32)
 constexpr int FBK = 64;          // K bytes per slice (two MFMA K-steps)
-// LDS row pitches (bytes) of the digit slices, chosen so the operand reads
-// are free of bank conflicts on gfx950 (measured with SQ_LDS_BANK_CONFLICT):
-// * 32x32x32 kernel: lanes read 16 bytes at row * P + 32 h + 16 s with
-//   row = lane % 32, h = lane / 32; 80 bytes is conflict-free (64, 96 and 128
-//   conflict);
-// * 16x16x64 kernel: lanes read 16 bytes at row * P + 16 q with row = lane %
-//   16, q = lane / 16; a pitch of 32 mod 64 bytes is conflict-free (80 conflicts).
-constexpr int FPITCH32 = FBK + 16;
-constexpr int FPITCH16 = FBK + 32;
+// LDS layout of the digit slices: lanes read 16-byte chunk 2 h + s of row
+// lane % 32 (h = lane / 32).  Two layouts are free of bank conflicts on gfx950
+// (measured with SQ_LDS_BANK_CONFLICT; unswizzled 64-, 96- and 128-byte rows
+// conflict): rows padded to 80 bytes, and unpadded 64-byte rows whose chunk
+// index is XORed with (row >> 2) & 3.  The 8-wave workgroups use the padded
+// rows (measured about 4% faster); the 4-wave ones the swizzled rows, whose
+// smaller footprint fits two workgroups per CU.
+template <int NR>
+struct SliceLayout {
+    static constexpr bool kSwizzle = NR < 4;
+    static constexpr int kPitch = kSwizzle ? FBK : FBK + 16;
+    // Byte offset of 16-byte chunk c of row r.
+    static __device__ __forceinline__ int at(int r, int c)
+    {
+        return r * kPitch + 16 * (kSwizzle ? c ^ ((r >> 2) & 3) : c);
+    }
+};
 // Word 3 of a raw buffer descriptor on gfx9-class GPUs (32-bit data format).
 constexpr int kBufferWord3 = 0x00020000;
 
@@ -138,7 +144,7 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
 
 typedef int i16v __attribute__((ext_vector_type(16)));
 
-// NR (1, 2 or 4): row tiles of 32 in this launch's row group.  The
+// NR (2 or 4): row tiles of 32 in this launch's row group.  The
 // workgroup has 2 NR waves; wave w computes rows 32 (w / 2) .. +32 of the
 // group for the 32 columns 32 (w % 2) .. +32, all six digit levels at once
 // (v_mfma_i32_32x32x32_i8: A row l % 32 and B column l % 32 take 16 K bytes
@@ -159,8 +165,9 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 {
     constexpr int ROWS = 32 * NR;  // Q rows staged per slice
     constexpr int NTHREADS = 128 * NR;
-    __shared__ __align__(16) int8_t sQ[2][ND][ROWS * FPITCH32];
-    __shared__ __align__(16) int8_t sY[2][NY][FBN * FPITCH32];
+    using L = SliceLayout<NR>;
+    __shared__ __align__(16) int8_t sQ[2][ND][ROWS * L::kPitch];
+    __shared__ __align__(16) int8_t sY[2][NY][FBN * L::kPitch];
     __shared__ double colred[NR][FBN];
     __shared__ double rowred[2][ROWS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -205,7 +212,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
             const int c_ = tid + NTHREADS * r_;                                                      \
             const int p_ = c_ / (FBN * 4), rem_ = c_ - p_ * FBN * 4, col_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            *reinterpret_cast<uint4*>(&sY[buf][p_][col_ * FPITCH32 + kc_]) = ry[r_];                   \
+            *reinterpret_cast<uint4*>(&sY[buf][p_][L::at(col_, kc_ >> 4)]) = ry[r_];                    \
         }                                                                                            \
     } while (0)
     // Q digits: ND planes x ROWS rows x 64 bytes per slice, five 16-byte
@@ -230,7 +237,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
             const int c_ = tid + NTHREADS * r_;                                                      \
             const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            *reinterpret_cast<uint4*>(&sQ[buf][p_][row_ * FPITCH32 + kc_]) = rq[r_];                   \
+            *reinterpret_cast<uint4*>(&sQ[buf][p_][L::at(row_, kc_ >> 4)]) = rq[r_];                    \
         }                                                                                            \
     } while (0)
 #define MPISDF_MMA(buf)                                                                               \
@@ -239,10 +246,10 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
             i4v yv[NY];                                                                              \
             _Pragma("unroll") for (int r = 0; r < NY; ++r)                                           \
                 yv[r] = *reinterpret_cast<const i4v*>(                                               \
-                    &sY[buf][r][(ct * 32 + l32) * FPITCH32 + 32 * half + 16 * s]);                     \
+                    &sY[buf][r][L::at(ct * 32 + l32, 2 * half + s)]);                                   \
             _Pragma("unroll") for (int p = 0; p < ND; ++p) {                                         \
                 const i4v a = *reinterpret_cast<const i4v*>(                                         \
-                    &sQ[buf][p][(rt * 32 + l32) * FPITCH32 + 32 * half + 16 * s]);                     \
+                    &sQ[buf][p][L::at(rt * 32 + l32, 2 * half + s)]);                                   \
                 _Pragma("unroll") for (int r = 0; r + p < NG && r < NY; ++r)                         \
                     acc[p + r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, yv[r], acc[p + r], 0, 0, 0); \
             }                                                                                        \
@@ -318,191 +325,26 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     }
 }
 
-// RH (1 or 2): row halves of 64 in this launch's row group; the workgroup
-// has 4 RH waves, wave w computing rows 64 (w / 4) .. +64 of the group for
-// the 16 columns 16 (w % 4) .. +16.  Every wave runs the same four row tiles
-// (rows past b carry zero digits), so the MFMA sequence has no branches: a
-// branch would make the compiler drain every outstanding load at its join,
-// prefetches included.  Two waves per SIMD (RH = 2) let one wave's MFMAs
-// cover the other's waits.
-template <bool ENERGY, int RH>
-__global__ __launch_bounds__(256 * RH, 2 / RH) void fold_i8_16x16_kernel(const int8_t* __restrict__ qd,
-                                                           const double* __restrict__ qs, int64_t ldq, int64_t bpad,
-                                                           int64_t b, int64_t m0, const int8_t* __restrict__ yd,
-                                                           int64_t P, const double* __restrict__ yscale, int64_t m,
-                                                           int64_t n, double* __restrict__ proj2,
-                                                           double* __restrict__ energy)
-{
-    constexpr int ROWS = 64 * RH;  // Q rows staged per slice
-    constexpr int NT = 4;          // row tiles of 16 per wave
-    constexpr int NTHREADS = 256 * RH;
-    __shared__ __align__(16) int8_t sQ[2][ND][ROWS * FPITCH16];
-    __shared__ double red[4 * RH][64];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int quarter = lane >> 4, l16 = lane & 15;
-    const int rh = wave >> 2, cg = wave & 3;
-    const int64_t g0 = (int64_t)blockIdx.x * FBN;
-    const int64_t col = g0 + cg * 16 + l16;  // this lane's Y column
-    i4v acc[NG][NT];
-#pragma unroll
-    for (int g = 0; g < NG; ++g)
-#pragma unroll
-        for (int t = 0; t < NT; ++t) acc[g][t] = i4v{0, 0, 0, 0};
-
-    const bool cok = col < n;
-    const double sj = cok ? yscale[col] : 0.0;
-    // Raw buffer loads: an offset past a descriptor's range returns zeros.
-    // K offsets at or past a plane's pitch are sent out of range, so the
-    // slices past the last one (the loop runs in steps of four) read zeros
-    // and add nothing.  The host keeps every offset below 2^31.
-    const int64_t ycols = std::min<int64_t>(FBN, n - g0);
-    const __amdgpu_buffer_rsrc_t yrsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
-    const __amdgpu_buffer_rsrc_t qrsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<int8_t*>(qd), (short)0, (int)(ND * bpad * ldq), kBufferWord3);
-    const uint32_t ybase = (uint32_t)((cg * 16 + l16) * 4 * P);
-    // Y digits of this lane (16 K bytes of each plane) for four slices in flight.
-    i4v y0[NY], y1[NY], y2[NY], y3[NY];
-#define MPISDF_LOAD_YD(k0, dst)                                                                       \
-    do {                                                                                             \
-        const uint32_t k_ = (uint32_t)(k0) + 16u * (uint32_t)quarter;                                \
-        _Pragma("unroll") for (int r_ = 0; r_ < NY; ++r_) {                                          \
-            const uint32_t off_ = k_ < (uint32_t)P ? ybase + (uint32_t)(r_ * P) + k_ : 0x80000000u;  \
-            dst[r_] = __builtin_bit_cast(i4v, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
-        }                                                                                            \
-    } while (0)
-    // Q digits: ND planes x ROWS rows x 64 bytes per slice, five 16-byte
-    // chunks per thread.
-    constexpr int QCHUNKS = ND * ROWS * (FBK / 16) / NTHREADS;
-    static_assert(QCHUNKS * NTHREADS == ND * ROWS * (FBK / 16), "Q slice must split evenly over the threads");
-    uint4 rq[QCHUNKS];
-#define MPISDF_LOAD_Q(k0)                                                                             \
-    do {                                                                                             \
-        const uint32_t k_ = (uint32_t)(k0);                                                          \
-        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
-            const int c_ = tid + NTHREADS * r_;                                                      \
-            const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            const uint32_t off_ = k_ < (uint32_t)ldq                                                 \
-                                      ? (uint32_t)(((int64_t)p_ * bpad + m0 + row_) * ldq) + k_ + (uint32_t)kc_ \
-                                      : 0x80000000u;                                                 \
-            rq[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(qrsrc, off_, 0, 0)); \
-        }                                                                                            \
-    } while (0)
-#define MPISDF_STASH_Q(buf)                                                                           \
-    do {                                                                                             \
-        _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
-            const int c_ = tid + NTHREADS * r_;                                                      \
-            const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            *reinterpret_cast<uint4*>(&sQ[buf][p_][row_ * FPITCH16 + kc_]) = rq[r_];                   \
-        }                                                                                            \
-    } while (0)
-#define MPISDF_MMA(buf, yv)                                                                           \
-    do {                                                                                             \
-        _Pragma("unroll") for (int t = 0; t < NT; ++t) {                                             \
-            _Pragma("unroll") for (int p = 0; p < ND; ++p) {                                         \
-                const i4v a = *reinterpret_cast<const i4v*>(                                         \
-                    &sQ[buf][p][(rh * 64 + t * 16 + l16) * FPITCH16 + 16 * quarter]);                  \
-                _Pragma("unroll") for (int r = 0; r + p < NG && r < NY; ++r)                         \
-                    acc[p + r][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, yv[r], acc[p + r][t], 0, 0, 0); \
-            }                                                                                        \
-        }                                                                                            \
-    } while (0)
-    // One slice: the Q digits of the next slice are loaded before the Y
-    // digits three slices ahead, so staging them waits only on the former.
-#define MPISDF_STEP(s, buf, ycur, yfill)                                                              \
-    do {                                                                                             \
-        MPISDF_LOAD_Q(((s) + 1) * FBK);                                                              \
-        __builtin_amdgcn_sched_barrier(0);                                                           \
-        MPISDF_LOAD_YD(((s) + 3) * FBK, yfill);                                                      \
-        __builtin_amdgcn_sched_barrier(0);                                                           \
-        MPISDF_MMA(buf, ycur);                                                                       \
-        MPISDF_STASH_Q((buf) ^ 1);                                                                   \
-        __syncthreads();                                                                             \
-    } while (0)
-
-    const int64_t nsl = (m + FBK - 1) / FBK;
-    MPISDF_LOAD_Q(0);
-    MPISDF_LOAD_YD(0, y0);
-    MPISDF_LOAD_YD(FBK, y1);
-    MPISDF_LOAD_YD(2 * FBK, y2);
-    MPISDF_STASH_Q(0);
-    __syncthreads();
-    for (int64_t sl = 0; sl < nsl; sl += 4) {
-        MPISDF_STEP(sl, 0, y0, y3);
-        MPISDF_STEP(sl + 1, 1, y1, y0);
-        MPISDF_STEP(sl + 2, 0, y2, y1);
-        MPISDF_STEP(sl + 3, 1, y3, y2);
-    }
-#undef MPISDF_STEP
-#undef MPISDF_MMA
-#undef MPISDF_LOAD_YD
-#undef MPISDF_LOAD_Q
-#undef MPISDF_STASH_Q
-
-    // W[t][j] = u_t s_j sum_g 128^-(g+2) acc_g; element e of lane l is row 4 quarter + e.
-    double colsq = 0;
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            double v = 0;
-#pragma unroll
-            for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / 128.0) + (double)acc[g][t][e];
-            const int lrow = rh * 64 + t * 16 + 4 * quarter + e;
-            const int64_t row = m0 + lrow;
-            const double wv = row < b ? v * (1.0 / 16384.0) * qs[row] * sj : 0.0;
-            colsq += wv * wv;
-            if (ENERGY) {
-                double s = wv * wv;
-#pragma unroll
-                for (int off = 1; off < 16; off <<= 1) s += __shfl_xor(s, off);
-                if (l16 == 0) red[wave][t * 16 + 4 * quarter + e] = s;
-            }
-        }
-    }
-    colsq += __shfl_xor(colsq, 16);
-    colsq += __shfl_xor(colsq, 32);
-    // One lane per column and row half; row groups run in order.
-    if (quarter == 0 && cok) {
-        if (RH == 1) proj2[col] += colsq;
-        else atomicAdd(&proj2[col], colsq);
-    }
-    if (ENERGY) {
-        __syncthreads();
-        if (tid < ROWS) {
-            const int64_t row = m0 + tid;
-            const int h = tid >> 6, lr = tid & 63;
-            if (row < b)
-                atomicAdd(&energy[row], red[4 * h][lr] + red[4 * h + 1][lr] + red[4 * h + 2][lr] + red[4 * h + 3][lr]);
-        }
-    }
-}
-
 // rows: Q^T rows of this launch's group (1..128).  Groups of more than 64
-// rows run on the 32x32x32 kernel (NR = 4, 8 waves); smaller groups on the
-// 16x16x64 kernel with one 64-row half (4 waves, two workgroups per CU),
-// which is faster there: the 32x32x32 kernel's Y slice staging does not
-// shrink with the rows.
+// rows run on 8-wave workgroups (NR = 4, one per CU); smaller groups on
+// 4-wave workgroups (NR = 2, two per CU), which multiply no zero rows.
 void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs,
                     int64_t ldq, int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P,
                     const double* yscale, int64_t m, int64_t n, double* proj2, double* en)
 {
     if (rows <= 0 || rows > FBM) throw std::logic_error("INT8 fold row group out of range");
-    if (rows <= 64) {
-        if (energy)
-            hipLaunchKernelGGL((fold_i8_16x16_kernel<true, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
-                               yscale, m, n, proj2, en);
-        else
-            hipLaunchKernelGGL((fold_i8_16x16_kernel<false, 1>), grid, dim3(256), 0, s, qd, qs, ldq, bpad, b, m0, yd,
-                               P, yscale, m, n, proj2, static_cast<double*>(nullptr));
+    double* const e = energy ? en : nullptr;
+#define MPISDF_FOLD(E, NR)                                                                                    \
+    hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
+                       yscale, m, n, proj2, e)
+    if (rows > 64) {
+        if (energy) MPISDF_FOLD(true, 4);
+        else MPISDF_FOLD(false, 4);
     } else {
-        if (energy)
-            hipLaunchKernelGGL((fold_i8_32x32_kernel<true, 4>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd, P,
-                               yscale, m, n, proj2, en);
-        else
-            hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4>), grid, dim3(512), 0, s, qd, qs, ldq, bpad, b, m0, yd,
-                               P, yscale, m, n, proj2, static_cast<double*>(nullptr));
+        if (energy) MPISDF_FOLD(true, 2);
+        else MPISDF_FOLD(false, 2);
     }
+#undef MPISDF_FOLD
     MPISDF_HIP(hipGetLastError());
 }
 
