@@ -121,10 +121,11 @@ int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, c
                             int64_t ldy, double* proj2, double* energy, void* stream);
 
 /* In place: FP32 column j of y (pitch 4 P bytes = P floats, P a multiple of
- * 16 >= m, m <= 36864) becomes four signed base-128 digit planes of P bytes,
- * y_hat_j = scale[j] sum_r d_r 128^-(r+1) with scale[j] a power of two
- * >= 2 ||y_j||; ynorm2[j] holds ||y_j||^2 on entry and ||y_hat_j||^2 on
- * return.  The digit form of the sketch that the reduced-precision selection
+ * 64 >= m, m <= 36864) becomes four signed base-128 digits per entry,
+ * y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1) with scale[j] a power of two
+ * >= 2 ||y_j||, digit r of entry i at byte 256 (i / 64) + 64 r + i % 64 of
+ * the column (zero past m); ynorm2[j] holds ||y_j||^2 on entry and
+ * ||y_hat_j||^2 on return.  The digit form of the sketch that the reduced-precision selection
  * factorises (no reference counterpart: the reference factorises an FP64
  * sketch, cpp/src/anchor_select_randomized_id.cpp:320-326). */
 int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
@@ -135,8 +136,9 @@ int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, do
  * cpp/src/anchor_select_randomized_id.cpp:320-326): dgeqp3's greedy pivot
  * order (max residual norm, lowest index on ties) of Y[m x n] held as
  * ytype 0 FP64 or 1 FP32 column-major (ldy elements; ynorm2 and yscale may
- * be NULL), or 4 digit planes from mpisdf_digitize_columns_device (ldy = P
- * bytes per plane; ynorm2 and yscale as it wrote them; folds on INT8 MFMA).
+ * be NULL), or 4 digits from mpisdf_digitize_columns_device (ldy = P, the
+ * column pitch in floats; ynorm2 and yscale as it wrote them; folds on INT8
+ * MFMA).
  * Stops before a pivot with |R_ii| <= stop_rel |R_00| or after kmax pivots.
  * Writes 0-based pivots and |R_ii| (host arrays of kmax entries; rdiag may
  * be NULL) and their count. */

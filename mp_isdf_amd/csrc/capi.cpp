@@ -295,10 +295,10 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
         if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 3)
             throw std::invalid_argument("invalid fold arguments");
         if (y_f32 == 3) {
-            // Digit planes of a copy of Y (pitch P = m rounded up to 16), then the INT8 fold.
+            // Digits of a copy of Y (pitch P = m rounded up to 64), then the INT8 fold.
             mpisdf::FoldI8Scratch scratch;
             auto st = static_cast<hipStream_t>(stream);
-            const int64_t P = (m + 15) / 16 * 16;
+            const int64_t P = (m + mpisdf::kDigitSlice - 1) / mpisdf::kDigitSlice * mpisdf::kDigitSlice;
             mpisdf::DeviceBuffer<double> yn(std::max<int64_t>(n, 1)), sc(std::max<int64_t>(n, 1));
             mpisdf::DeviceBuffer<float> yd((size_t)std::max<int64_t>(n * P, 1));
             mpisdf::launch_column_norms2_f32(static_cast<const float*>(Y), m, n, ldy, yn.ptr, st);

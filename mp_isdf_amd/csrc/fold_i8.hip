@@ -60,7 +60,9 @@ struct SliceLayout {
 // Word 3 of a raw buffer descriptor on gfx9-class GPUs (32-bit data format).
 constexpr int kBufferWord3 = 0x00020000;
 
-// Q digit planes: qd[(p * bpad + t) * ldq + k] = digit p of q_t[k] / u_t, qs[t] = u_t.
+// Q digits, slice-major: digit p of q_t[k] / u_t at
+// qd[((k / 64 * ND + p) * bpad + t) * 64 + k % 64] (one slice of every plane
+// and row is 64 ND bpad contiguous bytes), qs[t] = u_t.
 __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int64_t m, int64_t b, int64_t ldq,
                                int64_t bpad, int8_t* __restrict__ qd, double* __restrict__ qs)
 {
@@ -86,16 +88,16 @@ __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int6
         for (int p = 0; p < ND; ++p) {
             x *= 128.0;
             const double dgt = rint(x);
-            qd[((int64_t)p * bpad + t) * ldq + k] = (int8_t)dgt;
+            qd[(((k / FBK) * ND + p) * bpad + t) * FBK + k % FBK] = (int8_t)dgt;
             x -= dgt;  // exact: |x - dgt| <= 1/2
         }
     }
 }
 
 // In place: column j of y (FP32, m entries, column pitch 4 P bytes with P a
-// multiple of 16 >= m) becomes four digit planes of P bytes (plane r at byte
-// offset r P, zero past m); scale[j] = s_j and ynorm2[j] = ||y_hat_j||^2.
-// One workgroup per column; the column is first read into LDS.
+// multiple of 64 >= m) becomes four digits per entry at digit_offset (zero
+// past m); scale[j] = s_j and ynorm2[j] = ||y_hat_j||^2.  One workgroup per
+// column; the column is first read into LDS.
 __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict__ y, int64_t m, int64_t P,
                                                                double* __restrict__ ynorm2,
                                                                double* __restrict__ scale)
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
             x *= 128.0f;
             const float dgt = rintf(x);
             x -= dgt;  // exact (Sterbenz)
-            planes[r * P + i] = (int8_t)dgt;
+            planes[digit_offset(i, r)] = (int8_t)dgt;
         }
     }
     __syncthreads();
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
     for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
         double v = 0;
 #pragma unroll
-        for (int r = NY - 1; r >= 0; --r) v = (v + (double)planes[r * P + i]) * (1.0 / 128.0);
+        for (int r = NY - 1; r >= 0; --r) v = (v + (double)planes[digit_offset(i, r)]) * (1.0 / 128.0);
         v *= s;
         acc += v * v;
     }
@@ -167,7 +169,11 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     constexpr int NTHREADS = 128 * NR;
     using L = SliceLayout<NR>;
     __shared__ __align__(16) int8_t sQ[2][ND][ROWS * L::kPitch];
-    __shared__ __align__(16) int8_t sY[2][NY][FBN * L::kPitch];
+    // Each plane's Y slice is padded by 64 bytes: a stash instruction's
+    // 8-lane groups write two planes of one column, which then fall on
+    // different banks.
+    constexpr int YPLANE = FBN * L::kPitch + 64;
+    __shared__ __align__(16) int8_t sY[2][NY][YPLANE];
     __shared__ double colred[NR][FBN];
     __shared__ double rowred[2][ROWS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -184,16 +190,16 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     const bool cok = col < n;
     const double sj = cok ? yscale[col] : 0.0;
     // Raw buffer loads: an offset past a descriptor's range returns zeros.
-    // K offsets at or past a plane's pitch are sent out of range, so slices
-    // past the last one read zeros and add nothing.  The host keeps every
-    // offset below 2^31.
+    // Slices at or past a column's last one are sent out of range, so they
+    // read zeros and add nothing.  The host keeps every offset below 2^31.
     const int64_t ycols = std::min<int64_t>(FBN, n - g0);
     const __amdgpu_buffer_rsrc_t yrsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
     const __amdgpu_buffer_rsrc_t qrsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(qd), (short)0, (int)(ND * bpad * ldq), kBufferWord3);
-    // Y digits: NY planes x FBN columns x 64 bytes per slice, staged through
-    // LDS once per workgroup and shared by the NR row-tile waves.
+    // Y digits: FBN columns x 256 contiguous bytes (NY planes x 64) per
+    // slice, 16 lanes per column; staged through LDS once per workgroup and
+    // shared by the NR row-tile waves.
     constexpr int YCHUNKS = NY * FBN * (FBK / 16) / NTHREADS;
     static_assert(YCHUNKS * NTHREADS == NY * FBN * (FBK / 16), "Y slice must split evenly over the threads");
     uint4 ryA[YCHUNKS], ryB[YCHUNKS];
@@ -201,9 +207,10 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     do {                                                                                             \
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
             const int c_ = tid + NTHREADS * r_;                                                      \
-            const int p_ = c_ / (FBN * 4), rem_ = c_ - p_ * FBN * 4, col_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            const uint32_t k_ = (uint32_t)(k0) + (uint32_t)kc_;                                      \
-            const uint32_t off_ = k_ < (uint32_t)P ? (uint32_t)(col_ * 4 * P + p_ * P) + k_ : 0x80000000u; \
+            const int col_ = c_ >> 4;                                                                \
+            const uint32_t off_ = (uint32_t)(k0) < (uint32_t)P                                       \
+                                      ? (uint32_t)(col_ * 4 * P) + 4u * (uint32_t)(k0) + 16u * (uint32_t)(c_ & 15) \
+                                      : 0x80000000u;                                                 \
             ry[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
         }                                                                                            \
     } while (0)
@@ -211,12 +218,12 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     do {                                                                                             \
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
             const int c_ = tid + NTHREADS * r_;                                                      \
-            const int p_ = c_ / (FBN * 4), rem_ = c_ - p_ * FBN * 4, col_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            *reinterpret_cast<uint4*>(&sY[buf][p_][L::at(col_, kc_ >> 4)]) = ry[r_];                    \
+            const int col_ = c_ >> 4, p_ = (c_ >> 2) & 3;                                            \
+            *reinterpret_cast<uint4*>(&sY[buf][p_][L::at(col_, c_ & 3)]) = ry[r_];                    \
         }                                                                                            \
     } while (0)
-    // Q digits: ND planes x ROWS rows x 64 bytes per slice, five 16-byte
-    // chunks per thread.
+    // Q digits: ND planes x ROWS rows x 64 bytes per slice (rows of one plane
+    // contiguous), QCHUNKS 16-byte chunks per thread.
     constexpr int QCHUNKS = ND * ROWS * (FBK / 16) / NTHREADS;
     static_assert(QCHUNKS * NTHREADS == ND * ROWS * (FBK / 16), "Q slice must split evenly over the threads");
     uint4 rqA[QCHUNKS], rqB[QCHUNKS];
@@ -227,7 +234,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
             const int c_ = tid + NTHREADS * r_;                                                      \
             const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
             const uint32_t off_ = k_ < (uint32_t)ldq                                                 \
-                                      ? (uint32_t)(((int64_t)p_ * bpad + m0 + row_) * ldq) + k_ + (uint32_t)kc_ \
+                                      ? (uint32_t)((((int64_t)(k_ / FBK) * ND + p_) * bpad + m0 + row_) * FBK) + (uint32_t)kc_ \
                                       : 0x80000000u;                                                 \
             rq[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(qrsrc, off_, 0, 0)); \
         }                                                                                            \
@@ -353,7 +360,7 @@ void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const i
 void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale, hipStream_t stream)
 {
     if (n <= 0) return;
-    if (P % 16 || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 16 and >= m");
+    if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     const size_t lds = (size_t)m * sizeof(float);
     if (lds > 144 * 1024) throw std::invalid_argument("digitize_columns holds one column in LDS (m <= 36864)");
     if (lds > 64 * 1024)
@@ -370,6 +377,7 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
 {
     if (b <= 0 || n <= 0) return;
     if (m >= (int64_t(1) << 16)) throw std::invalid_argument("the INT8 fold needs fewer than 65536 sketch rows");
+    if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     const int64_t bpad = (b + FBM - 1) / FBM * FBM;
     const int64_t ldq = (m + FBK - 1) / FBK * FBK;
     const size_t need = (size_t)(ND * bpad * ldq);

@@ -92,11 +92,20 @@ void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, 
                              const double* ynorm2 = nullptr, double* err2 = nullptr);
 
 // Sketch in digit-plane form (fold_i8.hip): in place, FP32 column j
-// (pitch 4 P bytes, P a multiple of 16 >= m) becomes four signed base-128
-// digit planes y_hat = scale[j] sum_r d_r 128^-(r+1); ynorm2[j] (in: of the
-// FP32 column, out: of y_hat).  m <= 36864.
+// (pitch 4 P bytes, P a multiple of 64 >= m) becomes four signed base-128
+// digits per entry, y_hat = scale[j] sum_r d_r 128^-(r+1), stored slice by
+// slice (digit_offset); ynorm2[j] (in: of the FP32 column, out: of y_hat).
+// m <= 36864.
 void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale, hipStream_t stream);
 constexpr int64_t kDigitRowsMax = 36864;  // digitize_columns holds one FP32 column in LDS
+constexpr int64_t kDigitSlice = 64;       // entries per slice; P is a multiple of it
+// Byte offset, within its column's 4 P bytes, of digit r of entry i: each
+// slice of 64 entries keeps its four digit planes in 256 contiguous bytes, so
+// the fold streams every column slice as one 256-byte run.
+__host__ __device__ inline int64_t digit_offset(int64_t i, int r)
+{
+    return (i / kDigitSlice) * (4 * kDigitSlice) + r * kDigitSlice + (i % kDigitSlice);
+}
 
 // QRCP fold on the digit planes with INT8 MFMA (digit-pair products in exact
 // INT32, combined in FP64); m < 65536.

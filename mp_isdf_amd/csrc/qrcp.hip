@@ -85,12 +85,13 @@ __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __r
     }
 }
 
-// y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1), exact in FP64 (28 bits).
-__device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t P, int64_t i, double scale)
+// y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1), exact in FP64 (28 bits);
+// colp is column j's first byte.
+__device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t i, double scale)
 {
     double v = 0;
 #pragma unroll
-    for (int r = 3; r >= 0; --r) v = (v + (double)colp[r * P + i]) * (1.0 / 128.0);
+    for (int r = 3; r >= 0; --r) v = (v + (double)colp[digit_offset(i, r)]) * (1.0 / 128.0);
     return v * scale;
 }
 
@@ -100,7 +101,7 @@ __global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, co
     const int64_t total = m * nc;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = i / m, r = i - j * m;
-        out[i] = digit_value(yd + (j0 + j) * 4 * P, P, r, yscale[j0 + j]);
+        out[i] = digit_value(yd + (j0 + j) * 4 * P, r, yscale[j0 + j]);
     }
 }
 
@@ -185,7 +186,7 @@ __global__ void fill_candidates_kernel(const void* __restrict__ y, int ytype, in
         if (os >= 0) v = rc_prev[i + (int64_t)os * ldr];
         else if (ytype == kF64) v = static_cast<const double*>(y)[i + j * ldy];
         else if (ytype == kF32) v = (double)static_cast<const float*>(y)[i + j * ldy];
-        else v = digit_value(static_cast<const int8_t*>(y) + j * 4 * ldy, ldy, i, yscale[j]);
+        else v = digit_value(static_cast<const int8_t*>(y) + j * 4 * ldy, i, yscale[j]);
         rc[i + c * ldr] = v;
     }
 }
@@ -415,8 +416,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     if (ytype != kF64 && ytype != kF32 && !digits) throw std::invalid_argument("QRCP input must be f64, f32 or digits");
     if (digits && (!opt.ynorm2 || !opt.yscale || !(opt.margin > 0)))
         throw std::invalid_argument("QRCP on digit planes needs device norms, scales and a positive margin");
-    if (digits && (ldy % 16 || m >= (int64_t(1) << 16)))
-        throw std::invalid_argument("digit planes need a pitch that is a multiple of 16 and fewer than 65536 rows");
+    if (digits && (ldy % kDigitSlice || m >= (int64_t(1) << 16)))
+        throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and fewer than 65536 rows");
     if (m <= 0 || n <= 0) return {};
     if (ldy < m) throw std::invalid_argument("QRCP leading dimension smaller than rows");
     const int64_t kmax = std::min({opt.kmax > 0 ? opt.kmax : std::min(m, n), m, n});

@@ -214,10 +214,11 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     DeviceBuffer<double> y64;
     DeviceBuffer<float> y32;
     // The reduced-precision forms fold on INT8 digit planes of the FP32 sketch
-    // (four planes of ldy32 bytes in place of each column), which needs a
-    // pitch that is a multiple of 16 and ell within the digit kernels' limits.
+    // (four digits per entry in place of each column), which needs a pitch
+    // that is a multiple of 64 and ell within the digit kernels' limits.
     const bool use_digits = lowp && ell <= kDigitRowsMax;
-    const int64_t ldy32 = use_digits ? (ell + 15) / 16 * 16 : (ell + 3) / 4 * 4;  // 16-byte aligned FP32 columns
+    const int64_t ldy32 = use_digits ? (ell + kDigitSlice - 1) / kDigitSlice * kDigitSlice
+                                     : (ell + 3) / 4 * 4;  // 256- or 16-byte aligned FP32 columns
     const int64_t ldy64 = (ell + 3) / 4 * 4;  // 32-byte aligned FP64 columns
     DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;

@@ -144,6 +144,13 @@ def _digit_model(y, ynorm2):
     return scale, digits, y_hat * scale[:, None]
 
 
+def _planes(Y, n, P):
+    """Digits as written in place over the FP32 columns: each slice of 64
+    entries holds its four planes in 256 contiguous bytes; returns [n][4][P]."""
+    raw = Y.cpu().numpy().view(np.int8).reshape(n, P // 64, 4, 64)
+    return raw.transpose(0, 2, 1, 3).reshape(n, 4, P)
+
+
 def test_qrcp_digit_planes_match_oracle(mp_isdf):
     """The reduced-precision selection's QRCP: FP32 sketch columns -> digit
     planes (checked digit for digit against the NumPy model) -> INT8-fold
@@ -155,7 +162,7 @@ def test_qrcp_digit_planes_match_oracle(mp_isdf):
     y = (rng.normal(size=(n, 60)) @ rng.normal(size=(60, m)) * rng.uniform(0.2, 2.0, size=(n, 1))
          + 1e-3 * rng.normal(size=(n, m))).astype(np.float32).astype(np.float64)
     ynorm2 = np.sum(y ** 2, axis=1)
-    P = (m + 15) // 16 * 16
+    P = (m + 63) // 64 * 64
     buf = np.zeros((n, P), dtype=np.float32)
     buf[:, :m] = y
     Y = torch.tensor(buf, device="cuda")
@@ -163,7 +170,7 @@ def test_qrcp_digit_planes_match_oracle(mp_isdf):
     sc = torch.zeros(n, dtype=torch.float64, device="cuda")
     capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
     scale, digits, y_hat = _digit_model(y, ynorm2)
-    planes = Y.cpu().numpy().view(np.int8).reshape(n, 4, P)
+    planes = _planes(Y, n, P)
     np.testing.assert_array_equal(sc.cpu().numpy(), scale)
     for r in range(4):
         np.testing.assert_array_equal(planes[:, r, :m], digits[r])
@@ -217,7 +224,7 @@ def test_qrcp_stop_rules_and_rank_exhaustion(mp_isdf):
     np.testing.assert_array_equal(got64, order[:stop])
     np.testing.assert_array_equal(got32, order[:stop])
 
-    P = (m + 15) // 16 * 16
+    P = (m + 63) // 64 * 64
     buf = np.zeros((n, P), dtype=np.float32)
     buf[:, :m] = y
     Y = torch.tensor(buf, device="cuda")
