@@ -190,8 +190,10 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     const bool cok = col < n;
     const double sj = cok ? yscale[col] : 0.0;
     // Raw buffer loads: an offset past a descriptor's range returns zeros.
-    // Slices at or past a column's last one are sent out of range, so they
-    // read zeros and add nothing.  The host keeps every offset below 2^31.
+    // Each load's offset is a per-lane constant plus a wave-uniform slice
+    // offset; slices at or past the last one get 2^31 added instead (every
+    // constant and range is below 2^31, so the sum is out of range) and read
+    // zeros, adding nothing.
     const int64_t ycols = std::min<int64_t>(FBN, n - g0);
     const __amdgpu_buffer_rsrc_t yrsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
@@ -207,10 +209,8 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     do {                                                                                             \
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
             const int c_ = tid + NTHREADS * r_;                                                      \
-            const int col_ = c_ >> 4;                                                                \
-            const uint32_t off_ = (uint32_t)(k0) < (uint32_t)P                                       \
-                                      ? (uint32_t)(col_ * 4 * P) + 4u * (uint32_t)(k0) + 16u * (uint32_t)(c_ & 15) \
-                                      : 0x80000000u;                                                 \
+            const uint32_t lane_ = (uint32_t)((c_ >> 4) * 4 * P) + 16u * (uint32_t)(c_ & 15);       \
+            const uint32_t off_ = lane_ + ((uint32_t)(k0) < (uint32_t)P ? 4u * (uint32_t)(k0) : 0x80000000u); \
             ry[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
         }                                                                                            \
     } while (0)
@@ -233,9 +233,8 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         _Pragma("unroll") for (int r_ = 0; r_ < QCHUNKS; ++r_) {                                     \
             const int c_ = tid + NTHREADS * r_;                                                      \
             const int p_ = c_ / (ROWS * 4), rem_ = c_ - p_ * ROWS * 4, row_ = rem_ >> 2, kc_ = (rem_ & 3) * 16; \
-            const uint32_t off_ = k_ < (uint32_t)ldq                                                 \
-                                      ? (uint32_t)((((int64_t)(k_ / FBK) * ND + p_) * bpad + m0 + row_) * FBK) + (uint32_t)kc_ \
-                                      : 0x80000000u;                                                 \
+            const uint32_t lane_ = (uint32_t)(((int64_t)p_ * bpad + m0 + row_) * FBK) + (uint32_t)kc_; \
+            const uint32_t off_ = lane_ + (k_ < (uint32_t)ldq ? (k_ / FBK) * (uint32_t)(ND * bpad * FBK) : 0x80000000u); \
             rq[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(qrsrc, off_, 0, 0)); \
         }                                                                                            \
     } while (0)
@@ -247,19 +246,16 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
             *reinterpret_cast<uint4*>(&sQ[buf][p_][L::at(row_, kc_ >> 4)]) = rq[r_];                    \
         }                                                                                            \
     } while (0)
-#define MPISDF_MMA(buf)                                                                               \
+    // MFMA K-step s (0 or 1) of a slice: 32 K bytes at 32 h + 16 s per lane.
+#define MPISDF_MMA(buf, s)                                                                            \
     do {                                                                                             \
-        _Pragma("unroll") for (int s = 0; s < 2; ++s) {                                              \
-            i4v yv[NY];                                                                              \
-            _Pragma("unroll") for (int r = 0; r < NY; ++r)                                           \
-                yv[r] = *reinterpret_cast<const i4v*>(                                               \
-                    &sY[buf][r][L::at(ct * 32 + l32, 2 * half + s)]);                                   \
-            _Pragma("unroll") for (int p = 0; p < ND; ++p) {                                         \
-                const i4v a = *reinterpret_cast<const i4v*>(                                         \
-                    &sQ[buf][p][L::at(rt * 32 + l32, 2 * half + s)]);                                   \
-                _Pragma("unroll") for (int r = 0; r + p < NG && r < NY; ++r)                         \
-                    acc[p + r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, yv[r], acc[p + r], 0, 0, 0); \
-            }                                                                                        \
+        i4v yv[NY];                                                                                  \
+        _Pragma("unroll") for (int r = 0; r < NY; ++r)                                               \
+            yv[r] = *reinterpret_cast<const i4v*>(&sY[buf][r][L::at(ct * 32 + l32, 2 * half + (s))]); \
+        _Pragma("unroll") for (int p = 0; p < ND; ++p) {                                             \
+            const i4v a = *reinterpret_cast<const i4v*>(&sQ[buf][p][L::at(rt * 32 + l32, 2 * half + (s))]); \
+            _Pragma("unroll") for (int r = 0; r + p < NG && r < NY; ++r)                             \
+                acc[p + r] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, yv[r], acc[p + r], 0, 0, 0);   \
         }                                                                                            \
     } while (0)
     // One slice: the digits of the slice two ahead are loaded into one
@@ -271,7 +267,8 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         MPISDF_LOAD_Q(((sl) + 2) * FBK, rqL);                                                        \
         MPISDF_LOAD_YD(((sl) + 2) * FBK, ryL);                                                       \
         __builtin_amdgcn_sched_barrier(0);                                                           \
-        MPISDF_MMA(buf);                                                                             \
+        MPISDF_MMA(buf, 0);                                                                          \
+        MPISDF_MMA(buf, 1);                                                                          \
         MPISDF_STASH_Q((buf) ^ 1, rqS);                                                              \
         MPISDF_STASH_Y((buf) ^ 1, ryS);                                                              \
         __syncthreads();                                                                             \

@@ -268,66 +268,100 @@ __global__ __launch_bounds__(1024) void argmax_select_kernel(const double* __res
     }
 }
 
-// q <- r_c* / ||r_c*||  (the norm comes from the exact residual norm)
-__global__ void make_q_kernel(const double* __restrict__ rc, int64_t m, int64_t ldr, const ArgMax* __restrict__ am,
-                              double* __restrict__ q, uint8_t* __restrict__ selected, const PivotState* __restrict__ st)
-{
-    if (st->stop) return;
-    const double inv = 1.0 / sqrt(am->value);
-    const double* src = rc + am->slot * ldr;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
-        q[i] = src[i] * inv;
-    if (blockIdx.x == 0 && threadIdx.x == 0) selected[am->column] = 1;
-}
+// The new q of a pivot step, in three kernels that keep the work spread over
+// the GPU (r_p the pivot's residual, already orthogonal to every earlier q up
+// to rounding; the block's earlier q's are Q[:, t0 .. t0 + nt)):
+//   pivot_dots:    dpart[t][s] = chunk s of Q[:, t0 + t]^T r_p;
+//   pivot_vector:  q = (r_p - sum_t d_t Q[:, t0 + t]) / ||r_p||, d_t = sum_s dpart[t][s]
+//                  (the second Gram-Schmidt pass against the block), with
+//                  per-workgroup partial sums of q_i^2;
+//   store_q:       q / ||q|| into column k of Q and of the row-major copy Qt.
+// Every sum runs in a fixed order, so the result does not depend on timing.
+constexpr int kDotSplit = 8;   // m-chunks per column in pivot_dots
+constexpr int kVecRows = 64;   // rows per workgroup of pivot_vector
 
-// d[t] = Q[:, t0 + t]^T q, one workgroup per t.
-__global__ void dots_kernel(const double* __restrict__ Q, int64_t m, int64_t ldq, int64_t t0,
-                            const double* __restrict__ q, double* __restrict__ d, const PivotState* __restrict__ st)
+__global__ __launch_bounds__(256) void pivot_dots_kernel(const double* __restrict__ Q, int64_t m, int64_t ldq,
+                                                         int64_t t0, const double* __restrict__ rc, int64_t ldr,
+                                                         const ArgMax* __restrict__ am, double* __restrict__ dpart,
+                                                         const PivotState* __restrict__ st)
 {
     if (st->stop) return;
     __shared__ double part[4];
-    const double* col = Q + (t0 + blockIdx.x) * ldq;
+    const int64_t t = blockIdx.x, chunk = (m + kDotSplit - 1) / kDotSplit;
+    const int64_t i0 = (int64_t)blockIdx.y * chunk, i1 = std::min<int64_t>(m, i0 + chunk);
+    const double* col = Q + (t0 + t) * ldq;
+    const double* r = rc + am->slot * ldr;
     double s = 0;
-    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += col[i] * q[i];
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += col[i] * r[i];
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
     __syncthreads();
-    if (threadIdx.x == 0) d[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+    if (threadIdx.x == 0) dpart[t * kDotSplit + blockIdx.y] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
-// q -= sum_t d[t] Q[:, t0 + t]; then written (unnormalised) to out
-__global__ void reorth_kernel(double* __restrict__ q, const double* __restrict__ Q, int64_t m, int64_t ldq,
-                              int64_t t0, int64_t nt, const double* __restrict__ d, const PivotState* __restrict__ st)
+// Rows kVecRows b .. +kVecRows per workgroup; thread (g, l) sums the terms
+// t = g mod 4 of row l, the four partials are added in order.
+__global__ __launch_bounds__(256) void pivot_vector_kernel(const double* __restrict__ rc, int64_t m, int64_t ldr,
+                                                           const ArgMax* __restrict__ am,
+                                                           const double* __restrict__ Q, int64_t ldq, int64_t t0,
+                                                           int64_t nt, const double* __restrict__ dpart,
+                                                           double* __restrict__ q, double* __restrict__ npart,
+                                                           uint8_t* __restrict__ selected,
+                                                           const PivotState* __restrict__ st)
 {
     if (st->stop) return;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        double v = q[i];
-        for (int64_t t = 0; t < nt; ++t) v -= d[t] * Q[(t0 + t) * ldq + i];
-        q[i] = v;
+    constexpr int kChunk = 1024;
+    __shared__ double dsh[kChunk];
+    __shared__ double red[4][kVecRows];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int64_t i = (int64_t)blockIdx.x * kVecRows + lane;
+    double v = 0;
+    for (int64_t c0 = 0; c0 < nt; c0 += kChunk) {
+        const int64_t cn = std::min<int64_t>(kChunk, nt - c0);
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < cn; t += blockDim.x) {
+            const double* p = dpart + (c0 + t) * kDotSplit;
+            double d = 0;
+#pragma unroll
+            for (int c = 0; c < kDotSplit; ++c) d += p[c];
+            dsh[t] = d;
+        }
+        __syncthreads();
+        if (i < m)
+            for (int64_t t = g; t < cn; t += 4) v += dsh[t] * Q[(t0 + c0 + t) * ldq + i];
     }
+    red[g][lane] = v;
+    __syncthreads();
+    if (g == 0) {
+        double qi = 0;
+        if (i < m) {
+            const double proj = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+            qi = (rc[am->slot * ldr + i] - proj) / sqrt(am->value);
+            q[i] = qi;
+        }
+        const double s2 = wave_sum(qi * qi);
+        if (lane == 0) npart[blockIdx.x] = s2;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) selected[am->column] = 1;
 }
 
-// Normalise q in one workgroup and store it as column k of Q and of the
-// row-major copy Qt (qt_k[i * ldqt] = q_i).
-__global__ __launch_bounds__(1024) void store_q_kernel(const double* __restrict__ q, int64_t m,
-                                                       double* __restrict__ qk, double* __restrict__ qt_k,
-                                                       int64_t ldqt, const PivotState* __restrict__ st)
+__global__ __launch_bounds__(256) void store_q_kernel(const double* __restrict__ q, int64_t m,
+                                                      const double* __restrict__ npart, int64_t nparts,
+                                                      double* __restrict__ qk, double* __restrict__ qt_k,
+                                                      int64_t ldqt, const PivotState* __restrict__ st)
 {
     if (st->stop) return;
-    __shared__ double part[16];
-    double s = 0;
-    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) s += q[i] * q[i];
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
-    __syncthreads();
+    __shared__ double total;
     if (threadIdx.x < 64) {
-        double v = threadIdx.x < blockDim.x / 64 ? part[threadIdx.x] : 0.0;
-        v = wave_sum(v);
-        if (threadIdx.x == 0) part[0] = v;
+        double s = 0;
+        for (int64_t b = threadIdx.x; b < nparts; b += 64) s += npart[b];
+        s = wave_sum(s);
+        if (threadIdx.x == 0) total = s;
     }
     __syncthreads();
-    const double inv = 1.0 / sqrt(part[0]);
-    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
+    const double inv = 1.0 / sqrt(total);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) {
         const double v = q[i] * inv;
         qk[i] = v;
         qt_k[i * ldqt] = v;
@@ -430,7 +464,9 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     // Q (column-major) and its row-major copy Qt; leading dimensions padded to
     // four doubles so the TN products load 32-byte vectors.
     const int64_t ldq = (m + 3) / 4 * 4, ldqt = (std::max<int64_t>(kmax, 1) + 3) / 4 * 4, ldr = ldq;
-    DeviceBuffer<double> Q((size_t)ldq * std::max<int64_t>(kmax, 1)), Qt((size_t)m * ldqt), H, qv(m), dv;
+    DeviceBuffer<double> Q((size_t)ldq * std::max<int64_t>(kmax, 1)), Qt((size_t)m * ldqt), H, qv(m), dpart;
+    const int64_t nvec = (m + kVecRows - 1) / kVecRows;
+    DeviceBuffer<double> npart(nvec);
     // Candidate residuals, norms and columns of this block and the previous
     // one (a candidate that persists keeps its residual).
     DeviceBuffer<double> rcb[2], rn2b[2];
@@ -573,7 +609,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         MPISDF_HIP(hipGetLastError());
         ++res.blocks;
         const int64_t block_start = k;
-        if (dv.count < (size_t)std::max<int64_t>(kmax, 1)) dv.resize(std::max<int64_t>(kmax, 1));
+        if (dpart.count < (size_t)(kDotSplit * std::max<int64_t>(kmax, 1)))
+            dpart.resize((size_t)(kDotSplit * std::max<int64_t>(kmax, 1)));
         // Pivot steps run on the device in groups of kStepsPerSync: each step's
         // argmax kernel decides (certification against the non-candidates'
         // bound: the first pivot after a fold may tie it, as candidates are
@@ -591,16 +628,13 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                 hipLaunchKernelGGL(argmax_select_kernel, dim3(1), dim3(1024), 0, s, rn2.ptr, cand.ptr, Ceff, am.ptr,
                                    pstate.ptr, bound_ptr, block_start, opt.stop_abs, opt.stop_rel, opt.margin,
                                    force ? 1 : 0, piv_dev.ptr, rdiag_dev.ptr);
-                hipLaunchKernelGGL(make_q_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, rc.ptr, m, ldr, am.ptr,
-                                   qv.ptr, selected.ptr, pstate.ptr);
-                if (nt > 0) {
-                    hipLaunchKernelGGL(dots_kernel, dim3((unsigned)nt), dim3(256), 0, s, Q.ptr, m, ldq, block_start,
-                                       qv.ptr, dv.ptr, pstate.ptr);
-                    hipLaunchKernelGGL(reorth_kernel, dim3(grid_for(m, 256, 64)), dim3(256), 0, s, qv.ptr, Q.ptr, m, ldq,
-                                       block_start, nt, dv.ptr, pstate.ptr);
-                }
-                hipLaunchKernelGGL(store_q_kernel, dim3(1), dim3(1024), 0, s, qv.ptr, m, Q.ptr + kk * ldq, Qt.ptr + kk,
-                                   ldqt, pstate.ptr);
+                if (nt > 0)
+                    hipLaunchKernelGGL(pivot_dots_kernel, dim3((unsigned)nt, kDotSplit), dim3(256), 0, s, Q.ptr, m, ldq,
+                                       block_start, rc.ptr, ldr, am.ptr, dpart.ptr, pstate.ptr);
+                hipLaunchKernelGGL(pivot_vector_kernel, dim3((unsigned)nvec), dim3(256), 0, s, rc.ptr, m, ldr, am.ptr,
+                                   Q.ptr, ldq, block_start, nt, dpart.ptr, qv.ptr, npart.ptr, selected.ptr, pstate.ptr);
+                hipLaunchKernelGGL(store_q_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, qv.ptr, m,
+                                   npart.ptr, nvec, Q.ptr + kk * ldq, Qt.ptr + kk, ldqt, pstate.ptr);
                 hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
                                    Q.ptr + kk * ldq, rn2.ptr, am.ptr, pstate.ptr, bound_ptr);
             }
