@@ -413,6 +413,68 @@ __global__ __launch_bounds__(256) void cand_update_kernel(double* __restrict__ r
     }
 }
 
+// cand_update_kernel with the column held in registers (E entries per
+// thread, m <= T E): one read and one write of the column instead of two
+// reads and a write; q is read twice from L2.
+template <int T, int E>
+__global__ __launch_bounds__(T) void cand_update_reg_kernel(double* __restrict__ rc, int64_t m, int64_t ldr,
+                                                              int64_t C, const double* __restrict__ q,
+                                                              double* __restrict__ rn2, const ArgMax* __restrict__ am,
+                                                              const PivotState* __restrict__ st,
+                                                              const double* __restrict__ bound_ptr)
+{
+    if (st->stop) return;
+    constexpr int W = T / 64;
+    __shared__ double part[W];
+    __shared__ double hsh;
+    const int64_t c = blockIdx.x;
+    if (c >= C) return;
+    if (c == am->slot || rn2[c] < 0) {
+        if (threadIdx.x == 0) rn2[c] = -1.0;
+        return;
+    }
+    double* col = rc + c * ldr;
+    double v[E];
+    double s = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int64_t i = threadIdx.x + T * e;
+        v[e] = i < m ? col[i] : 0.0;
+        s += i < m ? q[i] * v[e] : 0.0;
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) t += part[w];
+        hsh = t;
+    }
+    __syncthreads();
+    const double h = hsh;
+    double n2 = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int64_t i = threadIdx.x + T * e;
+        if (i < m) {
+            const double x = v[e] - h * q[i];
+            col[i] = x;
+            n2 += x * x;
+        }
+    }
+    n2 = wave_sum(n2);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = n2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) x += part[w];
+        rn2[c] = (bound_ptr && x <= *bound_ptr) ? -1.0 : x;
+    }
+}
+
 // tail2 += sum (Y - Q W)^2 over a column chunk; R holds Y chunk - Q W (m x nc).
 __global__ void sumsq_kernel(const double* __restrict__ r, int64_t count, double* __restrict__ out)
 {
@@ -434,6 +496,25 @@ __global__ void widen_kernel(const T* __restrict__ y, int64_t ldy, int64_t m, in
         const int64_t j = i / m, r = i - j * m;
         out[i] = (double)y[r + j * ldy];
     }
+}
+
+// Candidate update of one pivot step: the register-resident kernel (16
+// entries per thread, 256 to 1024 threads) when the column fits
+// (m <= 16384), else the two-pass one.
+void launch_cand_update(dim3 grid, hipStream_t s, double* rc, int64_t m, int64_t ldr, int64_t C, const double* q,
+                        double* rn2, const ArgMax* am, const PivotState* st, const double* bound_ptr)
+{
+    if (m <= 256 * 16)
+        hipLaunchKernelGGL((cand_update_reg_kernel<256, 16>), grid, dim3(256), 0, s, rc, m, ldr, C, q, rn2, am, st,
+                           bound_ptr);
+    else if (m <= 512 * 16)
+        hipLaunchKernelGGL((cand_update_reg_kernel<512, 16>), grid, dim3(512), 0, s, rc, m, ldr, C, q, rn2, am, st,
+                           bound_ptr);
+    else if (m <= 1024 * 16)
+        hipLaunchKernelGGL((cand_update_reg_kernel<1024, 16>), grid, dim3(1024), 0, s, rc, m, ldr, C, q, rn2, am, st,
+                           bound_ptr);
+    else
+        hipLaunchKernelGGL(cand_update_kernel, grid, dim3(256), 0, s, rc, m, ldr, C, q, rn2, am, st, bound_ptr);
 }
 
 int grid_for(int64_t work, int threads = 256, int64_t cap = 65536)
@@ -635,8 +716,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                                    Q.ptr, ldq, block_start, nt, dpart.ptr, qv.ptr, npart.ptr, selected.ptr, pstate.ptr);
                 hipLaunchKernelGGL(store_q_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, qv.ptr, m,
                                    npart.ptr, nvec, Q.ptr + kk * ldq, Qt.ptr + kk, ldqt, pstate.ptr);
-                hipLaunchKernelGGL(cand_update_kernel, dim3((unsigned)Ceff), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
-                                   Q.ptr + kk * ldq, rn2.ptr, am.ptr, pstate.ptr, bound_ptr);
+                launch_cand_update(dim3((unsigned)Ceff), s, rc.ptr, m, ldr, Ceff, Q.ptr + kk * ldq, rn2.ptr, am.ptr,
+                                   pstate.ptr, bound_ptr);
             }
             MPISDF_HIP(hipGetLastError());
             PivotState h{};
