@@ -600,8 +600,19 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     const bool split = opt.split_bf16_fold && ytype == kF32;
     // Fold launches are bracketed by HIP events; durations are read at the end.
     LaunchTimer fold_timer;
-    auto fold = [&](double* energy_out) {
-        const int64_t b = k - folded;
+    // A fold streams all of Y once per group of up to 128 pivots, so a block's
+    // pivots are folded in whole groups of 128 and the rest (fewer than 128)
+    // are carried into the next fold: the keys then miss those projections,
+    // which leaves them valid upper bounds (residual norms only decrease),
+    // and certification stays exact.  The looser keys end some blocks sooner
+    // ((H2O)64: 95 blocks instead of 69), but the fold, which dominates a
+    // block, loses its partial launches (groups of 64 cost more blocks than
+    // they save).  whole = true folds every pending pivot (row energies
+    // wanted, or the previous block could certify nothing).
+    const int64_t fold_group = 128;
+    auto fold = [&](double* energy_out, bool whole) {
+        int64_t b = k - folded;
+        if (!whole && b >= fold_group) b = b / fold_group * fold_group;
         if (b <= 0) return;
         fold_timer.begin(s);
         res.fold_flops += 2.0 * (double)b * (double)m * (double)n;
@@ -615,7 +626,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             fold_projections(b, n, m, Q.ptr + folded * ldq, ldq, y, ytype == kF32, ldy, proj2.ptr,
                              energy_out ? energy_out + folded : nullptr, s);
         fold_timer.end(s);
-        folded = k;
+        folded += b;
         ++res.folds;
     };
 
@@ -625,13 +636,15 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
         MPISDF_HIP(hipMemsetAsync(err2.ptr, 0, n * sizeof(double), s));
         if (k > 0) fold_projections(k, n, m, Q.ptr, ldq, y, ytype == kF32, ldy, proj2.ptr, nullptr, s);
+        folded = k;
         ++res.refreshes;
     };
     int64_t C = C0;
     bool refreshed = false, force = false;
     const int64_t Cmax = std::max<int64_t>(C0, std::min<int64_t>(n, (int64_t(1) << 30) / std::max<int64_t>(m, 1)));
+    bool stalled = false;
     while (!done && k < kmax) {
-        fold(opt.want_energy ? energy.ptr : nullptr);
+        fold(opt.want_energy ? energy.ptr : nullptr, opt.want_energy || stalled);
         hipLaunchKernelGGL(keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, ynorm2.ptr, proj2.ptr, err2.ptr,
                            selected.ptr, opt.margin, n, keys.ptr, idx.ptr);
         MPISDF_HIP(hipGetLastError());
@@ -751,17 +764,18 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             // grow, make the bounds exact; if exact bounds still tie (FP64
             // rounding of ||y||^2 - ||Q^T y||^2), take the best candidate.
             ++res.stalls;
+            stalled = true;
+            if (folded < k) continue;  // first make every key exact to the pivots taken
             if (Ceff < remaining && C < Cmax) C = std::min(2 * C, Cmax);
             else if (split && !refreshed) { refresh(); refreshed = true; }
             else if (force) break;  // even a forced pivot has no positive norm: nothing is left to factorise
             else force = true;
         } else {
-            refreshed = false;
-            force = false;
+            stalled = refreshed = force = false;
         }
     }
     if (opt.want_energy) {
-        fold(energy.ptr);
+        fold(energy.ptr, true);
         res.energy.resize(k);
         if (k) MPISDF_HIP(hipMemcpyAsync(res.energy.data(), energy.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
     }
