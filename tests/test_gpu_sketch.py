@@ -253,3 +253,38 @@ def test_qrcp_stop_rules_and_rank_exhaustion(mp_isdf):
     zs = torch.zeros(n, dtype=torch.float64, device="cuda")
     capi.call("mpisdf_digitize_columns_device", Z, m, n, P, zn, zs, None)
     assert sorted(run(Z, 4, P, zn, zs, 40, 0.0).tolist()) == sorted(live.tolist())
+
+
+def test_qrcp_blocks_past_one_fold_group(mp_isdf):
+    """Blocks of more than 128 pivots, whose folds take whole groups of 128
+    and carry the rest into the next fold (the keys between folds are then
+    looser upper bounds): the pivots still equal the oracle's dgeqp3-semantics
+    QRCP on FP64, FP32 and digit-plane inputs.  Columns of slowly decaying
+    norm keep the top 2048 candidates above the bound for about 130 pivots
+    in the first block."""
+    import torch
+    rng = np.random.default_rng(23)
+    m, n, kmax = 400, 6000, 360
+    y = (rng.normal(size=(n, m)) * (0.9999 ** np.arange(n))[:, None]).astype(np.float32).astype(np.float64)
+    _, order, _ = oracle.geqp3(np.ascontiguousarray(y.T))
+
+    def run(buf, ytype, ld, norms, scales):
+        piv = np.zeros(kmax, dtype=np.int64)
+        rd = np.zeros(kmax)
+        cnt = np.zeros(1, dtype=np.int64)
+        capi.call("mpisdf_qrcp_device", buf, ytype, m, n, ld, norms, scales, kmax, 0.0, piv, rd, cnt, None)
+        return piv[:cnt[0]]
+
+    np.testing.assert_array_equal(run(torch.tensor(y, device="cuda"), 0, m, None, None), order[:kmax])
+    np.testing.assert_array_equal(run(torch.tensor(y.astype(np.float32), device="cuda"), 1, m, None, None),
+                                  order[:kmax])
+    P = (m + 63) // 64 * 64
+    buf = np.zeros((n, P), dtype=np.float32)
+    buf[:, :m] = y
+    Y = torch.tensor(buf, device="cuda")
+    yn = torch.tensor(np.sum(y ** 2, axis=1), device="cuda")
+    sc = torch.zeros(n, dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
+    _, _, y_hat = _digit_model(y, np.sum(y ** 2, axis=1))
+    _, order_h, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
+    np.testing.assert_array_equal(run(Y, 4, P, yn, sc), order_h[:kmax])
