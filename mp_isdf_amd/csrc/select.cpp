@@ -299,7 +299,10 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     opt.kmax = a.max_rank ? std::min(a.max_rank, structural) : structural;
     // Numerical-rank floor eps * max(ell, n) * max|R_ii| (|R_00| is the max).
     opt.stop_rel = kEps * (double)std::max(ell, n);
-    opt.candidates = std::min<int64_t>(n, 2048);
+    // Candidates per block: fewer make each block's Gram-Schmidt passes and
+    // per-pivot updates cheaper but end blocks sooner; 1024 is the measured
+    // optimum on (H2O)64 (QRCP 7.7 s; 8.4 s with 2048, 8.8 s with 512).
+    opt.candidates = std::min<int64_t>(n, 1024);
     opt.want_energy = a.max_rank == 0;
     opt.want_tail = a.max_rank == 0;
     opt.debug = a.debug;
