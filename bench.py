@@ -34,9 +34,9 @@ CONFIGS = {
 }
 PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "int8_mfma_tops": 5000.0,
         "hbm_gbs": 8000.0}
-# The INT8 fold splits q into 5 and y into 4 signed base-128 digits and runs
-# the 17 digit pairs with p + r <= 5 as INT8 products.
-I8_PAIRS = 17
+# The INT8 fold splits q into 5 and y into 3 signed base-128 digits and runs
+# the 14 digit pairs with p + r <= 5 as INT8 products.
+I8_PAIRS = 14
 FIT_BATCH = 8192  # grid points per fit batch
 
 
@@ -168,7 +168,7 @@ def main():
     fold_work = f"2*b*ell*n per fold, b = pivots folded (mean {k / fold_n:.1f}), ell={ell}, n={n}"
     if fold_kind == 2:
         ops = I8_PAIRS * fold_f
-        roofline_fold = {"kernel": "fold_i8_kernel (QRCP fold on INT8 digit planes of the FP32 sketch)",
+        roofline_fold = {"kernel": "fold_i8_32x32_kernel (QRCP fold on INT8 digits of the FP32 sketch)",
                          "bound": "mfma", "achieved": round(ops / fold_s / 1e12, 2) if fold_s else None,
                          "peak": PEAK["int8_mfma_tops"], "unit": "TOP/s",
                          "frac": round(ops / fold_s / 1e12 / PEAK["int8_mfma_tops"], 4) if fold_s else None,

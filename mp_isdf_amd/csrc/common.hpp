@@ -44,7 +44,7 @@ const char* last_error();
 void ensure_device();  // throws NoDevice when no HIP device is present
 
 // Output element types of the AO and sketch kernels.
-// kI8Digits: a sketch held as four signed base-128 digit planes per column
+// kI8Digits: a sketch held as three signed base-128 digit planes per column
 // (digitize_columns), with a per-column power-of-two scale.
 enum DType : int { kF64 = 0, kF32 = 1, kBF16 = 2, kFP8 = 3, kI8Digits = 4 };
 

@@ -3,23 +3,23 @@
 //
 //     W = Qb^T Y,  proj2[j] += sum_t W[t,j]^2  (energy[t] += sum_j W[t,j]^2)
 //
-// Y is held as four digit planes per column (digitize_columns, in place over
+// Y is held as three digit planes per column (digitize_columns, in place over
 // the FP32 sketch, same footprint): with the power of two s_j >= 2 ||y_j||,
-//     y_hat[i] = s_j sum_{r<4} d_r[i] 128^-(r+1),  |d_r| <= 64,
-// which equals the FP32 entry exactly when it lies within 2^-4 of s_j and
-// differs by at most 2^-29 s_j otherwise (a perturbation of ~1e-7 relative,
-// FP32-level).  The QRCP then factorises y_hat: candidate columns are
-// rebuilt from the same digits, and ||y_hat_j||^2 replaces ||y_j||^2.
+//     y_hat[i] = s_j sum_{r<3} d_r[i] 128^-(r+1),  |d_r| <= 64,
+// which rounds each entry to within 2^-22 s_j (about 3e-5 of ||y_j|| in RMS
+// over 13,200 rows, well below the rounding noise of a BF16 or FP8 sketch).
+// The QRCP then factorises y_hat: candidate columns are rebuilt from the same
+// digits, and ||y_hat_j||^2 replaces ||y_j||^2.
 //
 // Products use the splitting method of Ozaki, Ogita, Oishi and Rump (2012) in
 // the INT8 form of Ootomo, Ozaki and Yokota (2024): row t of Qb^T is written
-// as five digits with its own power of two u_t >= 2 max|q_t|; the 17 digit
+// as five digits with its own power of two u_t >= 2 max|q_t|; the 14 digit
 // pairs with p + r <= 5 run on v_mfma_i32_32x32x32_i8, pairs of equal p + r
 // sharing an exact INT32 accumulator (|sum| < 4 * 64^2 * m < 2^31 for
 // m < 2^16), and the six accumulators are combined in FP64:
 //     W = u_t s_j sum_g 128^-(g+2) acc_g.
-// The dropped pairs and the truncation of q leave ~1e-11 of ||y_j||^2 in the
-// folded sums (measured), covered by the QRCP's relative margin.
+// The dropped pair (4, 2) and the truncation of q leave ~1e-11 of ||y_j||^2
+// in the folded sums (measured), covered by the QRCP's relative margin.
 //
 // v_mfma_i32_32x32x32_i8 runs at about 1.7x the rate of the 16x16x64 form on
 // gfx950.  Q and Y digit slices are staged through double-buffered LDS, with
@@ -35,7 +35,7 @@ namespace {
 typedef int i4v __attribute__((ext_vector_type(4)));
 
 constexpr int ND = 5;            // digits of q
-constexpr int NY = 4;            // digits of y (planes per column)
+constexpr int NY = kDigitPlanes;  // digits of y (planes per column)
 constexpr int NG = 6;            // accumulator groups, p + r = 0..5
 constexpr int FBM = 128;         // rows per workgroup
 constexpr int FBN = 64;          // columns per workgroup (two tiles of 32)
@@ -95,8 +95,9 @@ __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int6
 }
 
 // In place: column j of y (FP32, m entries, column pitch 4 P bytes with P a
-// multiple of 64 >= m) becomes four digits per entry at digit_offset (zero
-// past m); scale[j] = s_j and ynorm2[j] = ||y_hat_j||^2.  One workgroup per
+// multiple of 64 >= m) becomes NY digits per entry at digit_offset (zero
+// past m and in the unused plane slots); scale[j] = s_j and
+// ynorm2[j] = ||y_hat_j||^2.  One workgroup per
 // column; the column is first read into LDS.
 __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict__ y, int64_t m, int64_t P,
                                                                double* __restrict__ ynorm2,
@@ -124,6 +125,7 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
             x -= dgt;  // exact (Sterbenz)
             planes[digit_offset(i, r)] = (int8_t)dgt;
         }
+        for (int r = NY; r < 4; ++r) planes[digit_offset(i, r)] = 0;  // the unused plane slots
     }
     __syncthreads();
     // ||y_hat||^2 from the stored digits, in FP64
@@ -199,17 +201,20 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
     const __amdgpu_buffer_rsrc_t qrsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(qd), (short)0, (int)(ND * bpad * ldq), kBufferWord3);
-    // Y digits: FBN columns x 256 contiguous bytes (NY planes x 64) per
-    // slice, 16 lanes per column; staged through LDS once per workgroup and
-    // shared by the NR row-tile waves.
-    constexpr int YCHUNKS = NY * FBN * (FBK / 16) / NTHREADS;
-    static_assert(YCHUNKS * NTHREADS == NY * FBN * (FBK / 16), "Y slice must split evenly over the threads");
+    // Y digits: FBN columns x NY x 64 contiguous bytes per slice, 4 NY
+    // chunks of 16 bytes per column; staged through LDS once per workgroup
+    // and shared by the NR row-tile waves.  When the chunks do not split
+    // evenly over the threads, the threads past the last chunk load and
+    // store a copy of an earlier one (the same bytes to the same place).
+    constexpr int YTOTAL = NY * FBN * (FBK / 16);
+    constexpr int YCHUNKS = (YTOTAL + NTHREADS - 1) / NTHREADS;
+    static_assert(YCHUNKS * NTHREADS - YTOTAL < NTHREADS, "Y slice chunk count");
     uint4 ryA[YCHUNKS], ryB[YCHUNKS];
 #define MPISDF_LOAD_YD(k0, ry)                                                                        \
     do {                                                                                             \
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
-            const int c_ = tid + NTHREADS * r_;                                                      \
-            const uint32_t lane_ = (uint32_t)((c_ >> 4) * 4 * P) + 16u * (uint32_t)(c_ & 15);       \
+            const int cr_ = tid + NTHREADS * r_, c_ = cr_ < YTOTAL ? cr_ : cr_ - NTHREADS;          \
+            const uint32_t lane_ = (uint32_t)((c_ / (4 * NY)) * 4 * P) + 16u * (uint32_t)(c_ % (4 * NY)); \
             const uint32_t off_ = lane_ + ((uint32_t)(k0) < (uint32_t)P ? 4u * (uint32_t)(k0) : 0x80000000u); \
             ry[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
         }                                                                                            \
@@ -217,8 +222,8 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 #define MPISDF_STASH_Y(buf, ry)                                                                       \
     do {                                                                                             \
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
-            const int c_ = tid + NTHREADS * r_;                                                      \
-            const int col_ = c_ >> 4, p_ = (c_ >> 2) & 3;                                            \
+            const int cr_ = tid + NTHREADS * r_, c_ = cr_ < YTOTAL ? cr_ : cr_ - NTHREADS;          \
+            const int col_ = c_ / (4 * NY), p_ = (c_ % (4 * NY)) >> 2;                               \
             *reinterpret_cast<uint4*>(&sY[buf][p_][L::at(col_, c_ & 3)]) = ry[r_];                    \
         }                                                                                            \
     } while (0)

@@ -85,13 +85,13 @@ __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __r
     }
 }
 
-// y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1), exact in FP64 (28 bits);
-// colp is column j's first byte.
+// y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1), exact in FP64; colp is
+// column j's first byte.
 __device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t i, double scale)
 {
     double v = 0;
 #pragma unroll
-    for (int r = 3; r >= 0; --r) v = (v + (double)colp[digit_offset(i, r)]) * (1.0 / 128.0);
+    for (int r = kDigitPlanes - 1; r >= 0; --r) v = (v + (double)colp[digit_offset(i, r)]) * (1.0 / 128.0);
     return v * scale;
 }
 

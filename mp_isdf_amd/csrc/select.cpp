@@ -214,7 +214,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     DeviceBuffer<double> y64;
     DeviceBuffer<float> y32;
     // The reduced-precision forms fold on INT8 digit planes of the FP32 sketch
-    // (four digits per entry in place of each column), which needs a pitch
+    // (three digits per entry in place of each column), which needs a pitch
     // that is a multiple of 64 and ell within the digit kernels' limits.
     const bool use_digits = lowp && ell <= kDigitRowsMax;
     const int64_t ldy32 = use_digits ? (ell + kDigitSlice - 1) / kDigitSlice * kDigitSlice

@@ -86,7 +86,7 @@ def test_lowp_selection_is_valid(mp_isdf):
 def test_qrcp_fold_modes(mp_isdf, mode):
     """Fold = column / row sums of (Q^T Y)^2 against NumPy; the split-BF16
     mode is checked against its stated bound (1e-5 of ||y_j||^2), the INT8
-    mode (3) against the fold of the four-digit columns it factorises."""
+    mode (3) against the fold of the three-digit columns it factorises."""
     import torch
     rng = np.random.default_rng(3)
     m, n, b = 1003, 3001, 150
@@ -107,20 +107,11 @@ def test_qrcp_fold_modes(mp_isdf, mode):
     torch.cuda.synchronize()
     ynorm2 = np.sum(y ** 2, axis=1)
     if mode == 3:
-        # The INT8 fold works on the four-digit form of each FP32 column:
-        # s = 2^(e+1) with ||y|| < 2^e, y_hat = s sum_r d_r 128^-(r+1).
-        scale = np.exp2(np.frexp(np.sqrt(ynorm2))[1] + 1.0)
-        x = (y / scale[:, None]).astype(np.float32)
-        y_hat = np.zeros_like(y)
-        weight = 1.0
-        for _ in range(4):
-            x = (x * np.float32(128)).astype(np.float32)
-            dgt = np.rint(x)
-            x = (x - dgt).astype(np.float32)
-            weight /= 128.0
-            y_hat += dgt.astype(np.float64) * weight
-        y_hat *= scale[:, None]
-        assert np.max(np.abs(y_hat - y)) <= 2.0 ** -29 * scale.max()
+        # The INT8 fold works on the three-digit form of each FP32 column:
+        # s = 2^(e+1) with ||y|| < 2^e, y_hat = s sum_r d_r 128^-(r+1),
+        # every entry within half a step (2^-22 s) of y.
+        scale, _, y_hat = _digit_model(y, ynorm2)
+        assert np.all(np.abs(y_hat - y) <= 2.0 ** -22 * scale[:, None])
         w_hat = q.T @ y_hat.T
         want_cols, want_rows = np.sum(w_hat ** 2, axis=0), np.sum(w_hat ** 2, axis=1)
     tol = {0: 1e-12, 1: 1e-12, 2: 1e-5, 3: 1e-10}[mode]
@@ -130,11 +121,11 @@ def test_qrcp_fold_modes(mp_isdf, mode):
 
 def _digit_model(y, ynorm2):
     """NumPy model of the digit planes: s = 2^(e+1) with ||y|| < 2^e, and
-    four signed base-128 digits of y / s, rounded to nearest in FP32 steps."""
+    three signed base-128 digits of y / s, rounded to nearest in FP32 steps."""
     scale = np.exp2(np.frexp(np.sqrt(ynorm2))[1] + 1.0)
     x = (y / scale[:, None]).astype(np.float32)
     digits, y_hat, weight = [], np.zeros_like(y), 1.0
-    for _ in range(4):
+    for _ in range(3):
         x = (x * np.float32(128)).astype(np.float32)
         dgt = np.rint(x)
         x = (x - dgt).astype(np.float32)
@@ -146,7 +137,8 @@ def _digit_model(y, ynorm2):
 
 def _planes(Y, n, P):
     """Digits as written in place over the FP32 columns: each slice of 64
-    entries holds its four planes in 256 contiguous bytes; returns [n][4][P]."""
+    entries holds its planes in 256 contiguous bytes (three planes, then a
+    zero slot); returns [n][4][P]."""
     raw = Y.cpu().numpy().view(np.int8).reshape(n, P // 64, 4, 64)
     return raw.transpose(0, 2, 1, 3).reshape(n, 4, P)
 
@@ -172,9 +164,10 @@ def test_qrcp_digit_planes_match_oracle(mp_isdf):
     scale, digits, y_hat = _digit_model(y, ynorm2)
     planes = _planes(Y, n, P)
     np.testing.assert_array_equal(sc.cpu().numpy(), scale)
-    for r in range(4):
+    for r in range(3):
         np.testing.assert_array_equal(planes[:, r, :m], digits[r])
         assert not planes[:, r, m:].any()
+    assert not planes[:, 3, :].any()
     np.testing.assert_allclose(yn.cpu().numpy(), np.sum(y_hat ** 2, axis=1), rtol=1e-14)
 
     def gpu_qrcp(buf_dev, ytype, ld, norms, scales):
