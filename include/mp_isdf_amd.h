@@ -124,8 +124,9 @@ int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, c
  * 64 >= m, m <= 36864) becomes three signed base-128 digits per entry,
  * y_hat_j[i] = scale[j] sum_{r<3} d_r[i] 128^-(r+1) with scale[j] a power of
  * two >= 2 ||y_j|| (each entry within 2^-22 scale[j] of y), digit r of entry
- * i at byte 256 (i / 64) + 64 r + i % 64 of the column (zero past m and at
- * r = 3); ynorm2[j] holds ||y_j||^2 on entry and ||y_hat_j||^2 on return.  The digit form of the sketch that the reduced-precision selection
+ * i at byte 192 (i / 64) + 64 r + i % 64 of the column (zero past m and in
+ * the column's last P bytes); ynorm2[j] holds ||y_j||^2 on entry and
+ * ||y_hat_j||^2 on return.  The digit form of the sketch that the reduced-precision selection
  * factorises (no reference counterpart: the reference factorises an FP64
  * sketch, cpp/src/anchor_select_randomized_id.cpp:320-326). */
 int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,

@@ -96,7 +96,7 @@ __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int6
 
 // In place: column j of y (FP32, m entries, column pitch 4 P bytes with P a
 // multiple of 64 >= m) becomes NY digits per entry at digit_offset (zero
-// past m and in the unused plane slots); scale[j] = s_j and
+// past m and in the column's last P bytes); scale[j] = s_j and
 // ynorm2[j] = ||y_hat_j||^2.  One workgroup per
 // column; the column is first read into LDS.
 __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict__ y, int64_t m, int64_t P,
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict
             x -= dgt;  // exact (Sterbenz)
             planes[digit_offset(i, r)] = (int8_t)dgt;
         }
-        for (int r = NY; r < 4; ++r) planes[digit_offset(i, r)] = 0;  // the unused plane slots
+        planes[NY * P + i] = 0;  // the column's unused last P bytes
     }
     __syncthreads();
     // ||y_hat||^2 from the stored digits, in FP64
@@ -201,8 +201,9 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
     const __amdgpu_buffer_rsrc_t qrsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(qd), (short)0, (int)(ND * bpad * ldq), kBufferWord3);
-    // Y digits: FBN columns x NY x 64 contiguous bytes per slice, 4 NY
-    // chunks of 16 bytes per column; staged through LDS once per workgroup
+    // Y digits: FBN columns x NY x 64 contiguous bytes per slice (slices of
+    // a column back to back), 4 NY chunks of 16 bytes per column; staged
+    // through LDS once per workgroup
     // and shared by the NR row-tile waves.  When the chunks do not split
     // evenly over the threads, the threads past the last chunk load and
     // store a copy of an earlier one (the same bytes to the same place).
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
             const int cr_ = tid + NTHREADS * r_, c_ = cr_ < YTOTAL ? cr_ : cr_ - NTHREADS;          \
             const uint32_t lane_ = (uint32_t)((c_ / (4 * NY)) * 4 * P) + 16u * (uint32_t)(c_ % (4 * NY)); \
-            const uint32_t off_ = lane_ + ((uint32_t)(k0) < (uint32_t)P ? 4u * (uint32_t)(k0) : 0x80000000u); \
+            const uint32_t off_ = lane_ + ((uint32_t)(k0) < (uint32_t)P ? (uint32_t)NY * (uint32_t)(k0) : 0x80000000u); \
             ry[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
         }                                                                                            \
     } while (0)

@@ -104,11 +104,12 @@ constexpr int64_t kDigitSlice = 64;       // entries per slice; P is a multiple 
 // BF16 or FP8 sketch.
 constexpr int kDigitPlanes = 3;
 // Byte offset, within its column's 4 P bytes, of digit r of entry i: each
-// slice of 64 entries keeps its digit planes in one 256-byte run (planes 0-2,
-// then 64 zero bytes), so the fold streams every column slice contiguously.
+// slice of 64 entries keeps its digit planes in one 192-byte run and the runs
+// follow each other, so a column's digits are its first 3 P bytes (the last
+// P are zero) and the fold streams them contiguously.
 __host__ __device__ inline int64_t digit_offset(int64_t i, int r)
 {
-    return (i / kDigitSlice) * (4 * kDigitSlice) + r * kDigitSlice + (i % kDigitSlice);
+    return (i / kDigitSlice) * (kDigitPlanes * kDigitSlice) + r * kDigitSlice + (i % kDigitSlice);
 }
 
 // QRCP fold on the digit planes with INT8 MFMA (digit-pair products in exact

@@ -137,10 +137,11 @@ def _digit_model(y, ynorm2):
 
 def _planes(Y, n, P):
     """Digits as written in place over the FP32 columns: each slice of 64
-    entries holds its planes in 256 contiguous bytes (three planes, then a
-    zero slot); returns [n][4][P]."""
-    raw = Y.cpu().numpy().view(np.int8).reshape(n, P // 64, 4, 64)
-    return raw.transpose(0, 2, 1, 3).reshape(n, 4, P)
+    entries holds its three planes in 192 contiguous bytes, slice after
+    slice; returns [n][3][P] and the column's unused last P bytes."""
+    raw = Y.cpu().numpy().view(np.int8).reshape(n, 4 * P)
+    planes = raw[:, :3 * P].reshape(n, P // 64, 3, 64).transpose(0, 2, 1, 3).reshape(n, 3, P)
+    return planes, raw[:, 3 * P:]
 
 
 def test_qrcp_digit_planes_match_oracle(mp_isdf):
@@ -162,12 +163,12 @@ def test_qrcp_digit_planes_match_oracle(mp_isdf):
     sc = torch.zeros(n, dtype=torch.float64, device="cuda")
     capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
     scale, digits, y_hat = _digit_model(y, ynorm2)
-    planes = _planes(Y, n, P)
+    planes, rest = _planes(Y, n, P)
     np.testing.assert_array_equal(sc.cpu().numpy(), scale)
     for r in range(3):
         np.testing.assert_array_equal(planes[:, r, :m], digits[r])
         assert not planes[:, r, m:].any()
-    assert not planes[:, 3, :].any()
+    assert not rest.any()
     np.testing.assert_allclose(yn.cpu().numpy(), np.sum(y_hat ** 2, axis=1), rtol=1e-14)
 
     def gpu_qrcp(buf_dev, ytype, ld, norms, scales):
