@@ -97,52 +97,63 @@ __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int6
 // In place: column j of y (FP32, m entries, column pitch 4 P bytes with P a
 // multiple of 64 >= m) becomes NY digits per entry at digit_offset (zero
 // past m and in the column's last P bytes); scale[j] = s_j and
-// ynorm2[j] = ||y_hat_j||^2.  One workgroup per
-// column; the column is first read into LDS.
-__global__ __launch_bounds__(256) void digitize_columns_kernel(float* __restrict__ y, int64_t m, int64_t P,
-                                                               double* __restrict__ ynorm2,
-                                                               double* __restrict__ scale)
+// ynorm2[j] = ||y_hat_j||^2.  One workgroup of kDigitizeThreads per column;
+// each thread holds its entries in registers between the read of the whole
+// column and the in-place writes (one barrier apart).
+constexpr int kDigitizeThreads = 1024;
+constexpr int kDigitizeEntries = (int)(kDigitRowsMax / kDigitizeThreads);  // entries per thread, P <= 36864
+__global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_kernel(float* __restrict__ y, int64_t m,
+                                                                            int64_t P, double* __restrict__ ynorm2,
+                                                                            double* __restrict__ scale)
 {
-    extern __shared__ float col[];
-    __shared__ double part[4];
+    __shared__ double part[kDigitizeThreads / 64];
     const int64_t j = blockIdx.x;
     float* src = y + j * P;  // 4 P bytes = P floats
-    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) col[i] = src[i];
+    float x[kDigitizeEntries];
+#pragma unroll
+    for (int e = 0; e < kDigitizeEntries; ++e) {
+        const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
+        x[e] = i < m ? src[i] : 0.f;
+    }
     const double nrm = sqrt(ynorm2[j]);  // of the FP32 entries, >= max |y_ij|
-    int e = 0;
-    if (nrm > 0) frexp(nrm, &e);
-    const double s = nrm > 0 ? ldexp(1.0, e + 1) : 0.0;
-    const float inv = nrm > 0 ? ldexpf(1.0f, -(e + 1)) : 0.f;
-    __syncthreads();
+    int ex = 0;
+    if (nrm > 0) frexp(nrm, &ex);
+    const double sc = nrm > 0 ? ldexp(1.0, ex + 1) : 0.0;
+    const float inv = nrm > 0 ? ldexpf(1.0f, -(ex + 1)) : 0.f;
+    __syncthreads();  // every entry of the column is read before its bytes are rewritten
     int8_t* planes = reinterpret_cast<int8_t*>(src);
     double acc = 0;
-    for (int64_t i = threadIdx.x; i < P; i += blockDim.x) {
-        float x = i < m ? col[i] * inv : 0.f;  // exact: power-of-two scaling, |x| < 1/2
 #pragma unroll
-        for (int r = 0; r < NY; ++r) {
-            x *= 128.0f;
-            const float dgt = rintf(x);
-            x -= dgt;  // exact (Sterbenz)
-            planes[digit_offset(i, r)] = (int8_t)dgt;
+    for (int e = 0; e < kDigitizeEntries; ++e) {
+        const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
+        if (i < P) {
+            float v = x[e] * inv;  // exact: power-of-two scaling, |v| < 1/2
+            double yh = 0, w = 1.0;
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                v *= 128.0f;
+                const float dgt = rintf(v);
+                v -= dgt;  // exact (Sterbenz)
+                planes[digit_offset(i, r)] = (int8_t)dgt;
+                w *= 1.0 / 128.0;
+                yh += (double)dgt * w;
+            }
+            planes[NY * P + i] = 0;  // the column's unused last P bytes
+            yh *= sc;
+            acc += yh * yh;
         }
-        planes[NY * P + i] = 0;  // the column's unused last P bytes
     }
-    __syncthreads();
-    // ||y_hat||^2 from the stored digits, in FP64
-    for (int64_t i = threadIdx.x; i < m; i += blockDim.x) {
-        double v = 0;
-#pragma unroll
-        for (int r = NY - 1; r >= 0; --r) v = (v + (double)planes[digit_offset(i, r)]) * (1.0 / 128.0);
-        v *= s;
-        acc += v * v;
-    }
+    // ||y_hat||^2 in FP64 from the digits, summed in a fixed order
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = acc;
     __syncthreads();
     if (threadIdx.x == 0) {
-        ynorm2[j] = part[0] + part[1] + part[2] + part[3];
-        scale[j] = s;
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < kDigitizeThreads / 64; ++w) t += part[w];
+        ynorm2[j] = t;
+        scale[j] = sc;
     }
 }
 
@@ -364,13 +375,10 @@ void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2,
 {
     if (n <= 0) return;
     if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
-    const size_t lds = (size_t)m * sizeof(float);
-    if (lds > 144 * 1024) throw std::invalid_argument("digitize_columns holds one column in LDS (m <= 36864)");
-    if (lds > 64 * 1024)
-        MPISDF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(digitize_columns_kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (P > kDigitRowsMax) throw std::invalid_argument("digitize_columns holds one column in registers (pitch <= 36864)");
     if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
-    hipLaunchKernelGGL(digitize_columns_kernel, dim3((unsigned)n), dim3(256), lds, stream, y, m, P, ynorm2, scale);
+    hipLaunchKernelGGL(digitize_columns_kernel, dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream, y, m, P, ynorm2,
+                       scale);
     MPISDF_HIP(hipGetLastError());
 }
 
