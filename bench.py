@@ -37,7 +37,7 @@ PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops":
 # The INT8 fold splits q into 5 and y into 3 signed base-128 digits and runs
 # the 14 digit pairs with p + r <= 5 as INT8 products.
 I8_PAIRS = 14
-FIT_BATCH = 8192  # grid points per fit batch
+FIT_BATCH = 32768  # grid points per fit batch (measured: 11.8 s fit against 12.1 s at 8192)
 
 
 def log(msg):
