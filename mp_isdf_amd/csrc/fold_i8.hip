@@ -6,8 +6,9 @@
 // Y is held as three digit planes per column (digitize_columns, in place over
 // the FP32 sketch, same footprint): with the power of two s_j >= 2 ||y_j||,
 //     y_hat[i] = s_j sum_{r<3} d_r[i] 128^-(r+1),  |d_r| <= 64,
-// which rounds each entry to within 2^-22 s_j (about 3e-5 of ||y_j|| in RMS
-// over 13,200 rows, well below the rounding noise of a BF16 or FP8 sketch).
+// which rounds each entry to within 2^-22 s_j (3e-5 to 6e-5 of ||y_j|| in RMS
+// over 13,200 rows, as s_j < 4 ||y_j||; well below the rounding noise of a
+// BF16 or FP8 sketch).
 // The QRCP then factorises y_hat: candidate columns are rebuilt from the same
 // digits, and ||y_hat_j||^2 replaces ||y_j||^2.
 //
@@ -45,8 +46,8 @@ constexpr int FBK = 64;          // K bytes per slice (two MFMA K-steps)
 // (measured with SQ_LDS_BANK_CONFLICT; unswizzled 64-, 96- and 128-byte rows
 // conflict): rows padded to 80 bytes, and unpadded 64-byte rows whose chunk
 // index is XORed with (row >> 2) & 3.  The 8-wave workgroups use the padded
-// rows (measured about 4% faster); the 4-wave ones the swizzled rows, whose
-// smaller footprint fits two workgroups per CU.
+// rows (the two measured within 2% of each other there); the 4-wave ones the
+// swizzled rows, whose smaller footprint fits two workgroups per CU.
 template <int NR>
 struct SliceLayout {
     static constexpr bool kSwizzle = NR < 4;

@@ -99,9 +99,9 @@ void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, 
 void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale, hipStream_t stream);
 constexpr int64_t kDigitRowsMax = 36864;  // digitize_columns holds one FP32 column in registers
 constexpr int64_t kDigitSlice = 64;       // entries per slice; P is a multiple of it
-// Digits per entry: three base-128 digits round an entry to 2^-22 s_j, about
-// 3e-5 of the column norm in RMS over 13,200 rows, far below the noise of a
-// BF16 or FP8 sketch.
+// Digits per entry: three base-128 digits round an entry to 2^-22 s_j, 3e-5
+// to 6e-5 of the column norm in RMS over 13,200 rows, far below the noise of
+// a BF16 or FP8 sketch.
 constexpr int kDigitPlanes = 3;
 // Byte offset, within its column's 4 P bytes, of digit r of entry i: each
 // slice of 64 entries keeps its digit planes in one 192-byte run and the runs
