@@ -99,13 +99,6 @@ __global__ __launch_bounds__(256) void potrf_block_kernel(double* __restrict__ a
     }
 }
 
-// dst[r][c] += 1 for the identity block inside R (jb x width, row-major).
-__global__ void add_identity_kernel(double* __restrict__ r, int64_t ld, int jb, int64_t col0)
-{
-    const int i = threadIdx.x;
-    if (i < jb) r[i * ld + col0 + i] += 1.0;
-}
-
 // err2 accumulation: per grid column g of Z (k x nb, column-major ld k):
 //   qn = (sum_a phi_ga^2)^2 ; e = max(qn - ||z_g||^2, 0) ; acc += |w_g| e, tot += |w_g| qn
 __global__ void pair_error_kernel(const double* __restrict__ z, int64_t k, const double* __restrict__ phi, int64_t d,
@@ -210,6 +203,30 @@ __global__ void anchor_adjoint_kernel(const double* __restrict__ m1, const doubl
         out[i] = -4.0 * (m1[i] - m2[i]);
 }
 
+// out[c * ldo + r] = in[r * ldi + c] for r < rows, c < cols.
+__global__ void transpose_into_kernel(const double* __restrict__ in, int64_t rows, int64_t cols, int64_t ldi,
+                                      double* __restrict__ out, int64_t ldo)
+{
+    __shared__ double tile[32][33];
+    const int64_t r0 = (int64_t)blockIdx.y * 32, c0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t rr = r0 + r, cc = c0 + tx;
+        tile[r][tx] = (rr < rows && cc < cols) ? in[rr * ldi + cc] : 0.0;
+    }
+    __syncthreads();
+    for (int c = ty; c < 32; c += 8) {
+        const int64_t cc = c0 + c, rr = r0 + tx;
+        if (rr < rows && cc < cols) out[cc * ldo + rr] = tile[tx][c];
+    }
+}
+
+__global__ void set_identity_kernel(double* __restrict__ x, int64_t k)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < k) x[i * k + i] = 1.0;
+}
+
 void transpose(const double* in, int64_t rows, int64_t cols, const double* div, double* out, hipStream_t s)
 {
     const dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
@@ -270,24 +287,35 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
         throw std::runtime_error("ISDF fit Gram (Phi_a Phi_a^T)^2 is not positive definite at row " +
                                  std::to_string(hinfo) + " (pivot below 1e-14 of its diagonal): the anchor pair "
                                  "columns are numerically dependent");
-    // L^-1 by blocks: Linv[j, :] = inv(L_jj) (I_j - L[j, :j] Linv[:j, :])
+    // L^-1 right-looking, in place on X = I (row-major, lower triangular):
+    // for each block J of NB rows (w = j0 + jb),
+    //   X[J, :w] <- inv(L_JJ) X[J, :w],
+    //   X[r, :w] -= L[r, J] X[J, :w] for every row r below J (one TN product, K = jb),
+    // which is forward substitution of L X = I by blocks.  The transpose of each
+    // finished block row goes to linvt: the TN product's K-contiguous operand
+    // here, and L^-T for the fit pass.
+    f.linvt.resize((size_t)(k * k));
     MPISDF_HIP(hipMemsetAsync(f.linv.ptr, 0, (size_t)(k * k) * sizeof(double), s));
-    DeviceBuffer<double> r((size_t)(NB * k));
-    std::vector<double> ldiag(k);
+    MPISDF_HIP(hipMemsetAsync(f.linvt.ptr, 0, (size_t)(k * k) * sizeof(double), s));
+    hipLaunchKernelGGL(set_identity_kernel, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, s, f.linv.ptr, k);
+    MPISDF_HIP(hipGetLastError());
+    DeviceBuffer<double> xs((size_t)(NB * k));
     for (int64_t j0 = 0; j0 < k; j0 += NB) {
         const int jb = (int)std::min<int64_t>(NB, k - j0);
-        const int64_t width = j0 + jb;
-        MPISDF_HIP(hipMemsetAsync(r.ptr, 0, (size_t)(jb * width) * sizeof(double), s));
-        if (j0 > 0)
-            gemm_f64(jb, j0, j0, -1.0, MatView{a.ptr + j0 * k, k, 1}, MatView{f.linv.ptr, k, 1}, 0.0,
-                     MutView{r.ptr, width, 1}, s);
-        hipLaunchKernelGGL(add_identity_kernel, dim3(1), dim3(64), 0, s, r.ptr, width, jb, j0);
-        gemm_f64(jb, width, jb, 1.0, MatView{invd.ptr + (j0 / NB) * NB * NB, jb, 1}, MatView{r.ptr, width, 1}, 0.0,
-                 MutView{f.linv.ptr + j0 * k, k, 1}, s);
+        const int64_t w = j0 + jb;
+        double* xj = f.linv.ptr + j0 * k;
+        gemm_f64(jb, w, jb, 1.0, MatView{invd.ptr + (j0 / NB) * NB * NB, jb, 1}, MatView{xj, k, 1}, 0.0,
+                 MutView{xs.ptr, w, 1}, s);
+        MPISDF_HIP(hipMemcpy2DAsync(xj, k * sizeof(double), xs.ptr, w * sizeof(double), w * sizeof(double), jb,
+                                    hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(transpose_into_kernel, dim3((unsigned)((w + 31) / 32), (unsigned)((jb + 31) / 32)),
+                           dim3(256), 0, s, xs.ptr, (int64_t)jb, w, w, f.linvt.ptr + j0, k);
+        MPISDF_HIP(hipGetLastError());
+        const int64_t rem = k - w;
+        if (rem > 0)
+            gemm_tn_f64(w, rem, jb, -1.0, f.linvt.ptr + j0, k, a.ptr + w * k + j0, k, 1.0, f.linv.ptr + w * k, k, s);
     }
-    f.linvt.resize((size_t)(k * k));
-    transpose(f.linv.ptr, k, k, nullptr, f.linvt.ptr, s);
-    ldiag = diagonal(a.ptr, k, s);
+    const std::vector<double> ldiag = diagonal(a.ptr, k, s);
     double lo = ldiag[0], hi = ldiag[0];
     for (double v : ldiag) { lo = std::min(lo, v); hi = std::max(hi, v); }
     // cond(G2) estimate from the factor's diagonal (a lower bound of the true condition)
