@@ -285,10 +285,16 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         MPISDF_LOAD_Q(((sl) + 2) * FBK, rqL);                                                        \
         MPISDF_LOAD_YD(((sl) + 2) * FBK, ryL);                                                       \
         __builtin_amdgcn_sched_barrier(0);                                                           \
+        if (NR == 4) { /* stash first: measured 2.5% faster with 8 waves, 0.7% slower with 4 */      \
+            MPISDF_STASH_Q((buf) ^ 1, rqS);                                                          \
+            MPISDF_STASH_Y((buf) ^ 1, ryS);                                                          \
+        }                                                                                            \
         MPISDF_MMA(buf, 0);                                                                          \
         MPISDF_MMA(buf, 1);                                                                          \
-        MPISDF_STASH_Q((buf) ^ 1, rqS);                                                              \
-        MPISDF_STASH_Y((buf) ^ 1, ryS);                                                              \
+        if (NR != 4) {                                                                               \
+            MPISDF_STASH_Q((buf) ^ 1, rqS);                                                          \
+            MPISDF_STASH_Y((buf) ^ 1, ryS);                                                          \
+        }                                                                                            \
         __syncthreads();                                                                             \
     } while (0)
 
