@@ -73,3 +73,26 @@ def test_build_isdf_driver(mp_isdf):
     assert res["overlap_relative_error"] < 1e-6
     with pytest.raises(ValueError, match="mutually exclusive"):
         build_isdf(mol, anchor_algorithm="rand_sketch", anchor_rank=2, anchor_tol=1e-6, selection_only=True)
+
+
+def test_fit_factor_over_several_blocks(mp_isdf):
+    """150 anchors: the Cholesky factor and L^-1 run over three 64-row blocks
+    (the last one partial) and k is no multiple of 4, so the blocked L^-1
+    products take their unaligned path; theta and the pair-space error must
+    still equal the oracle's SVD refit."""
+    mol = molecule.single_water("cc-pvdz")
+    rng = np.random.default_rng(8)
+    pts = rng.normal(scale=1.6, size=(3000, 3))
+    w = rng.uniform(0.05, 1.0, size=3000)
+    anchors = np.asarray(oracle.select(mol.atm, mol.bas, mol.env, pts, w, "ao", max_rank=150))
+    assert len(anchors) == 150
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    fit = mp_isdf.isdf_fit(ctx, pts[anchors], w[anchors], pts, w, batch_size=1000, interp_vecs=True)
+    ao = oracle.ao_eval(mol.atm, mol.bas, mol.env, pts)
+    aa = ao[anchors]
+    pairs = (aa[:, :, None] * aa[:, None, :]).reshape(len(anchors), -1)
+    theta, _, diag = oracle.physical_pair_refit(pairs, w[anchors], ao, 512)
+    assert diag["retained_rank"] == len(anchors)
+    np.testing.assert_allclose(fit["interp_vecs"], theta, rtol=1e-7, atol=1e-9 * np.abs(theta).max())
+    err = oracle.stream_pair_space_error_norms(ao, ao, pairs, w[anchors], theta, w)
+    np.testing.assert_allclose(fit["pair_space_error"]["l2"], err["l2"], rtol=1e-6)
