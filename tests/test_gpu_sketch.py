@@ -282,3 +282,21 @@ def test_qrcp_blocks_past_one_fold_group(mp_isdf):
     _, _, y_hat = _digit_model(y, np.sum(y ** 2, axis=1))
     _, order_h, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
     np.testing.assert_array_equal(run(Y, 4, P, yn, sc), order_h[:kmax])
+
+
+@pytest.mark.parametrize("m", [6000, 12000])
+def test_qrcp_tall_sketch_matches_oracle(mp_isdf, m):
+    """Tall sketches (the candidate update holds each column in registers
+    with 512 and 1024 threads at these heights): FP64 pivots equal the
+    oracle's dgeqp3-semantics QRCP."""
+    import torch
+    rng = np.random.default_rng(m)
+    n, kmax = 1500, 40
+    y = rng.normal(size=(n, 50)) @ rng.normal(size=(50, m)) * rng.uniform(0.3, 2.0, size=(n, 1))
+    _, order, _ = oracle.geqp3(np.ascontiguousarray(y.T))
+    piv = np.zeros(kmax, dtype=np.int64)
+    rd = np.zeros(kmax)
+    cnt = np.zeros(1, dtype=np.int64)
+    capi.call("mpisdf_qrcp_device", torch.tensor(y, device="cuda"), 0, m, n, m, None, None, kmax, 0.0, piv, rd, cnt,
+              None)
+    np.testing.assert_array_equal(piv[:cnt[0]], order[:kmax])
