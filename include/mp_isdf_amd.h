@@ -188,6 +188,35 @@ int mpisdf_isdf_fit_gradient(const mpisdf_context* ctx, const double* anchor_xyz
                              const double* xyz, const double* w, int64_t n, int64_t batch, double* loss,
                              double* grad_ao, double* grad_atoms);
 
+/* ---- Coulomb metric and J/K ---------------------------------------------- */
+/* Grid Coulomb metric, python/isdf.py:48-67 (grid_coulomb_metric) and :76-92
+ * (grid_cross_metric):  metric[a*kr + b] = sum_ij left[a][i] w_i K_ij w_j
+ * right[b][j] over n grid points, K_ij = 1/|r_i - r_j| (0 for distinct
+ * coincident points), K_ii = (32/15) pi^2 (3/(4 pi))^(5/3) |w_i|^(-1/3).
+ * Host arrays: left[kl*n], right[kr*n] (interpolation vectors, not weight
+ * scaled), xyz[n*3], w[n] (finite, nonzero), metric[kl*kr] overwritten.
+ * symmetrize != 0 (kl == kr) returns (M + M^T)/2 as grid_coulomb_metric does;
+ * with right == left this is grid_coulomb_metric(left, xyz, w). */
+int mpisdf_grid_coulomb_metric(const double* left, int64_t kl, const double* right, int64_t kr, const double* xyz,
+                               const double* w, int64_t n, int symmetrize, double* metric);
+/* Device form: left/right rows at stride ldl/ldr (>= n; multiples of 4 take
+ * the vector load path), device xyz, w and metric.  Values are not checked;
+ * a zero weight makes the self term infinite.  Returns after the stream has
+ * drained. */
+int mpisdf_grid_coulomb_metric_device(const double* left, int64_t kl, int64_t ldl, const double* right, int64_t kr,
+                                      int64_t ldr, const double* xyz, const double* w, int64_t n, int symmetrize,
+                                      double* metric, void* stream);
+/* ISDF J and K, python/isdf.py:130-151 (evaluate_isdf_jk): x[k*nao] anchor AO
+ * values, z[k*k] the anchor metric (both anchor weights included), dm[nao*nao]
+ * a finite symmetric density; P = x dm x^T, J = x^T diag(z diag(P)) x,
+ * K = x^T (z o P) x, both symmetrised into j[nao*nao], kmat[nao*nao].
+ * Invalid dm raises ValueError with the reference's messages. */
+int mpisdf_isdf_jk(const double* x, int64_t k, int64_t nao, const double* z, const double* dm, double* j,
+                   double* kmat);
+/* Device form (no value checks; returns after the stream has drained). */
+int mpisdf_isdf_jk_device(const double* x, int64_t k, int64_t nao, const double* z, const double* dm, double* j,
+                          double* kmat, void* stream);
+
 /* ---- Grid front end (no reference counterpart: the reference takes its grid
  * from PySCF's dft.gen_grid.Grids, python/isdf.py:488-496) ------------------ */
 /* Becke-partitioned atom-centred grid, Treutler radial nodes, product angular

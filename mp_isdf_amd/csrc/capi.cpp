@@ -3,6 +3,7 @@
 
 #include "axis.hpp"
 #include "context.hpp"
+#include "coulomb.hpp"
 #include "kernels.hpp"
 #include "qrcp.hpp"
 #include "util.hpp"
@@ -478,5 +479,109 @@ extern "C" int mpisdf_ao_adjoint_host(const mpisdf_context* ctx, const double* x
             mpisdf::launch_ao_adjoint(c, dxyz.ptr, n, dab.ptr, c.nao, dg.ptr, nullptr);
         }
         MPISDF_HIP(hipMemcpy(gout, dg.ptr, 3 * c.nao * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+namespace {
+
+// Copies rows x n host values into a device buffer with rows at a stride of
+// round-up(n, 4) doubles (the TN product's vector load path).
+int64_t upload_rows(const double* host, int64_t rows, int64_t n, mpisdf::DeviceBuffer<double>& dev)
+{
+    const int64_t ld = std::max<int64_t>((n + 3) / 4 * 4, 4);
+    dev.resize((size_t)(std::max<int64_t>(rows, 1) * ld));
+    if (rows > 0 && n > 0)
+        MPISDF_HIP(hipMemcpy2D(dev.ptr, ld * sizeof(double), host, n * sizeof(double), n * sizeof(double),
+                               (size_t)rows, hipMemcpyHostToDevice));
+    return ld;
+}
+
+void require_finite(const double* p, int64_t count, const char* msg)
+{
+    for (int64_t i = 0; i < count; ++i)
+        if (!std::isfinite(p[i])) throw std::invalid_argument(msg);
+}
+
+}  // namespace
+
+extern "C" int mpisdf_grid_coulomb_metric(const double* left, int64_t kl, const double* right, int64_t kr,
+                                          const double* xyz, const double* w, int64_t n, int symmetrize,
+                                          double* metric)
+{
+    return guard([&] {
+        mpisdf::ensure_device();
+        if (kl < 0 || kr < 0 || n < 0) throw std::invalid_argument("negative metric sizes");
+        if (symmetrize && kl != kr) throw std::invalid_argument("a symmetrised metric needs as many left as right rows");
+        if (kl == 0 || kr == 0) return;
+        if (!metric || (n > 0 && (!left || !right || !xyz || !w)))
+            throw std::invalid_argument("null metric argument");
+        require_finite(xyz, 3 * n, "grid coordinates must be finite");
+        for (int64_t i = 0; i < n; ++i)
+            if (!std::isfinite(w[i]) || w[i] == 0.0)
+                throw std::invalid_argument("grid weights must be finite and nonzero (the self-cell term is "
+                                            "|w|^(-1/3))");
+        require_finite(left, kl * n, "interpolation vectors must be finite");
+        if (right != left) require_finite(right, kr * n, "interpolation vectors must be finite");
+        mpisdf::DeviceBuffer<double> dl, dr, dxyz, dw, dm((size_t)(kl * kr));
+        const int64_t ldl = upload_rows(left, kl, n, dl);
+        int64_t ldr = ldl;
+        const double* rp = dl.ptr;
+        if (right != left) {
+            ldr = upload_rows(right, kr, n, dr);
+            rp = dr.ptr;
+        }
+        dxyz.upload(xyz, 3 * n, nullptr);
+        dw.upload(w, n, nullptr);
+        mpisdf::grid_coulomb_metric(dl.ptr, kl, ldl, rp, kr, ldr, dxyz.ptr, dw.ptr, n, symmetrize != 0, dm.ptr,
+                                    nullptr);
+        MPISDF_HIP(hipMemcpy(metric, dm.ptr, kl * kr * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" int mpisdf_grid_coulomb_metric_device(const double* left, int64_t kl, int64_t ldl, const double* right,
+                                                 int64_t kr, int64_t ldr, const double* xyz, const double* w,
+                                                 int64_t n, int symmetrize, double* metric, void* stream)
+{
+    return guard([&] {
+        if (kl > 0 && kr > 0 && (!metric || (n > 0 && (!left || !right || !xyz || !w))))
+            throw std::invalid_argument("null metric argument");
+        mpisdf::grid_coulomb_metric(left, kl, ldl, right, kr, ldr, xyz, w, n, symmetrize != 0, metric,
+                                    static_cast<hipStream_t>(stream));
+        MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
+extern "C" int mpisdf_isdf_jk(const double* x, int64_t k, int64_t nao, const double* z, const double* dm,
+                              double* j, double* kmat)
+{
+    return guard([&] {
+        mpisdf::ensure_device();
+        if (k < 0 || nao < 0) throw std::invalid_argument("negative J/K sizes");
+        if (!j || !kmat || (nao > 0 && !dm) || (k > 0 && (!x || !z))) throw std::invalid_argument("null J/K argument");
+        require_finite(dm, nao * nao, "dm must be a finite AO density matrix of shape (nao,nao)");
+        // numpy.allclose(dm, dm.T, rtol=1e-12, atol=1e-12) (python/isdf.py:143)
+        for (int64_t a = 0; a < nao; ++a)
+            for (int64_t b = 0; b < nao; ++b)
+                if (std::fabs(dm[a * nao + b] - dm[b * nao + a]) > 1e-12 + 1e-12 * std::fabs(dm[b * nao + a]))
+                    throw std::invalid_argument("dm must be symmetric");
+        require_finite(x, k * nao, "anchor AO values must be finite");
+        require_finite(z, k * k, "the Coulomb metric must be finite");
+        if (nao == 0) return;
+        mpisdf::DeviceBuffer<double> dx, dz, ddm, dj((size_t)(nao * nao)), dk((size_t)(nao * nao));
+        dx.upload(x, k * nao, nullptr);
+        dz.upload(z, k * k, nullptr);
+        ddm.upload(dm, nao * nao, nullptr);
+        mpisdf::isdf_jk(dx.ptr, k, nao, dz.ptr, ddm.ptr, dj.ptr, dk.ptr, nullptr);
+        MPISDF_HIP(hipMemcpy(j, dj.ptr, nao * nao * sizeof(double), hipMemcpyDeviceToHost));
+        MPISDF_HIP(hipMemcpy(kmat, dk.ptr, nao * nao * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" int mpisdf_isdf_jk_device(const double* x, int64_t k, int64_t nao, const double* z, const double* dm,
+                                     double* j, double* kmat, void* stream)
+{
+    return guard([&] {
+        if (nao > 0 && (!j || !kmat || !dm || (k > 0 && (!x || !z)))) throw std::invalid_argument("null J/K argument");
+        mpisdf::isdf_jk(x, k, nao, z, dm, j, kmat, static_cast<hipStream_t>(stream));
     });
 }

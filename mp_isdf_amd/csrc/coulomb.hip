@@ -1,0 +1,163 @@
+// Grid Coulomb metric of ISDF interpolation vectors and the J/K contractions
+// that consume it (python/isdf.py:48-67 grid_coulomb_metric, :76-92
+// grid_cross_metric, :130-151 evaluate_isdf_jk).
+//
+// The metric is M = L diag(w) K diag(w) R^T over n grid points, K the
+// point-to-point Coulomb kernel with a spherical-cell self term.  Its cost is
+// 2 n^2 kr + 2 n kl kr flops: an N-body product.  Rows of the weighted kernel
+// K'_ij = w_i K_ij w_j are generated in blocks of R rows (rows x n FP64, at
+// most 2 GiB) by one VALU kernel, then two FP64 MFMA TN products consume the
+// block:  V = K'_blk R^T (rows x kr),  M += L_blk V.  The kernel block is
+// written once and read by the product once per 128 columns of R, so for
+// kr >= 128 the block costs 8 (1 + kr/128) bytes against 2 kr flops per
+// element: FP64 MFMA bound above kr ~ 40 at 8 TB/s.
+#include "coulomb.hpp"
+#include "kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace mpisdf {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// K'[r * ldk + j] = w_i w_j K_ij for grid point i = i0 + r.
+__global__ __launch_bounds__(256) void coulomb_rows_kernel(const double* __restrict__ xyz,
+                                                           const double* __restrict__ w, int64_t n, int64_t i0,
+                                                           int64_t rows, int64_t ldk, double csphere,
+                                                           double* __restrict__ kb)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const double xj = xyz[3 * j], yj = xyz[3 * j + 1], zj = xyz[3 * j + 2], wj = w[j];
+    for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
+        const int64_t i = i0 + r;
+        double v;
+        if (i == j) {
+            v = csphere * pow(fabs(wj), -1.0 / 3.0) * wj * wj;
+        } else {
+            const double dx = xyz[3 * i] - xj, dy = xyz[3 * i + 1] - yj, dz = xyz[3 * i + 2] - zj;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            v = d2 > 0.0 ? w[i] * wj / sqrt(d2) : 0.0;
+        }
+        kb[r * ldk + j] = v;
+    }
+}
+
+// In place a <- (a + a^T) / 2 for a row-major m x m matrix.
+__global__ void symmetrize_kernel(double* __restrict__ a, int64_t m)
+{
+    const int64_t r = (int64_t)blockIdx.y * 16 + threadIdx.y, c = (int64_t)blockIdx.x * 16 + threadIdx.x;
+    if (r >= m || c >= m || c <= r) return;
+    const double s = 0.5 * (a[r * m + c] + a[c * m + r]);
+    a[r * m + c] = s;
+    a[c * m + r] = s;
+}
+
+// out[p * cols + c] = u[p] * x[p * cols + c]  (rows x cols)
+__global__ void scale_rows_kernel(const double* __restrict__ x, const double* __restrict__ u, int64_t rows,
+                                  int64_t cols, double* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= rows * cols) return;
+    out[t] = u[t / cols] * x[t];
+}
+
+__global__ void hadamard_kernel(const double* __restrict__ a, const double* __restrict__ b, int64_t count,
+                                double* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < count) out[t] = a[t] * b[t];
+}
+
+__global__ void fill_kernel(double* __restrict__ a, int64_t count, double value)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < count) a[t] = value;
+}
+
+void symmetrize(double* a, int64_t m, hipStream_t s)
+{
+    if (m <= 1) return;
+    const unsigned g = (unsigned)((m + 15) / 16);
+    hipLaunchKernelGGL(symmetrize_kernel, dim3(g, g), dim3(16, 16), 0, s, a, m);
+    MPISDF_HIP(hipGetLastError());
+}
+
+unsigned blocks_of(int64_t count) { return (unsigned)((count + 255) / 256); }
+
+}  // namespace
+
+int64_t coulomb_rows_per_block(int64_t n)
+{
+    const int64_t ldk = (std::max<int64_t>(n, 1) + 3) / 4 * 4;
+    int64_t rows = (int64_t{1} << 31) / (8 * ldk);
+    rows = std::max<int64_t>(128, rows / 128 * 128);
+    return std::min(rows, (n + 127) / 128 * 128);
+}
+
+void grid_coulomb_metric(const double* left, int64_t kl, int64_t ldl, const double* right, int64_t kr, int64_t ldr,
+                         const double* xyz, const double* w, int64_t n, bool symmetrize_out, double* metric,
+                         hipStream_t s)
+{
+    if (kl < 0 || kr < 0 || n < 0) throw std::invalid_argument("negative metric sizes");
+    if (symmetrize_out && kl != kr) throw std::invalid_argument("a symmetrised metric needs kl == kr");
+    if (kl == 0 || kr == 0) return;
+    if (n == 0) {
+        hipLaunchKernelGGL(fill_kernel, dim3(blocks_of(kl * kr)), dim3(256), 0, s, metric, kl * kr, 0.0);
+        MPISDF_HIP(hipGetLastError());
+        return;
+    }
+    if (ldl < n || ldr < n) throw std::invalid_argument("metric operand leading dimensions are below n");
+    const double csphere = (32.0 / 15.0) * kPi * kPi * std::pow(3.0 / (4.0 * kPi), 5.0 / 3.0);
+    const int64_t ldk = (n + 3) / 4 * 4;
+    const int64_t rblk = coulomb_rows_per_block(n);
+    DeviceBuffer<double> kb((size_t)(rblk * ldk)), v((size_t)(rblk * kr));
+    const unsigned gx = blocks_of(n);
+    for (int64_t i0 = 0; i0 < n; i0 += rblk) {
+        const int64_t rows = std::min(rblk, n - i0);
+        const unsigned gy = (unsigned)std::min<int64_t>(rows, 65535);
+        hipLaunchKernelGGL(coulomb_rows_kernel, dim3(gx, gy), dim3(256), 0, s, xyz, w, n, i0, rows, ldk, csphere,
+                           kb.ptr);
+        MPISDF_HIP(hipGetLastError());
+        // V[r + b * rblk] = sum_j K'[r][j] right[b][j]
+        gemm_tn_f64(rows, kr, n, 1.0, kb.ptr, ldk, right, ldr, 0.0, v.ptr, rblk, s);
+        // metric[a * kr + b] (+)= sum_r V[r + b * rblk] left[a][i0 + r]
+        gemm_tn_f64(kr, kl, rows, 1.0, v.ptr, rblk, left + i0, ldl, i0 == 0 ? 0.0 : 1.0, metric, kr, s);
+    }
+    if (symmetrize_out) symmetrize(metric, kl, s);
+    MPISDF_HIP(hipStreamSynchronize(s));  // before the block buffers are freed
+}
+
+void isdf_jk(const double* x, int64_t k, int64_t nao, const double* z, const double* dm, double* j, double* kmat,
+             hipStream_t s)
+{
+    if (k < 0 || nao < 0) throw std::invalid_argument("negative J/K sizes");
+    if (nao == 0) return;
+    if (k == 0) {
+        hipLaunchKernelGGL(fill_kernel, dim3(blocks_of(nao * nao)), dim3(256), 0, s, j, nao * nao, 0.0);
+        hipLaunchKernelGGL(fill_kernel, dim3(blocks_of(nao * nao)), dim3(256), 0, s, kmat, nao * nao, 0.0);
+        MPISDF_HIP(hipGetLastError());
+        return;
+    }
+    DeviceBuffer<double> t((size_t)(k * nao)), p((size_t)(k * k)), u((size_t)k), zp((size_t)(k * k));
+    // T = x dm (k x nao), P = T x^T (k x k)
+    gemm_f64(k, nao, nao, 1.0, MatView{x, nao, 1}, MatView{dm, nao, 1}, 0.0, MutView{t.ptr, nao, 1}, s);
+    gemm_f64(k, k, nao, 1.0, MatView{t.ptr, nao, 1}, MatView{x, 1, nao}, 0.0, MutView{p.ptr, k, 1}, s);
+    // u = z diag(P); J = x^T diag(u) x  (T is reused for diag(u) x)
+    gemm_f64(k, 1, k, 1.0, MatView{z, k, 1}, MatView{p.ptr, k + 1, 1}, 0.0, MutView{u.ptr, 1, 1}, s);
+    hipLaunchKernelGGL(scale_rows_kernel, dim3(blocks_of(k * nao)), dim3(256), 0, s, x, u.ptr, k, nao, t.ptr);
+    MPISDF_HIP(hipGetLastError());
+    gemm_f64(nao, nao, k, 1.0, MatView{x, 1, nao}, MatView{t.ptr, nao, 1}, 0.0, MutView{j, nao, 1}, s);
+    // K = x^T (z o P) x  (T is reused for (z o P) x)
+    hipLaunchKernelGGL(hadamard_kernel, dim3(blocks_of(k * k)), dim3(256), 0, s, z, p.ptr, k * k, zp.ptr);
+    MPISDF_HIP(hipGetLastError());
+    gemm_f64(k, nao, k, 1.0, MatView{zp.ptr, k, 1}, MatView{x, nao, 1}, 0.0, MutView{t.ptr, nao, 1}, s);
+    gemm_f64(nao, nao, k, 1.0, MatView{x, 1, nao}, MatView{t.ptr, nao, 1}, 0.0, MutView{kmat, nao, 1}, s);
+    symmetrize(j, nao, s);
+    symmetrize(kmat, nao, s);
+    MPISDF_HIP(hipStreamSynchronize(s));  // before the work buffers are freed
+}
+
+}  // namespace mpisdf

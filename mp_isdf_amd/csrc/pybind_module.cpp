@@ -379,6 +379,55 @@ PYBIND11_MODULE(mp_isdf, m)
        "Pair-reconstruction loss of the FP64 fit (sum_g |w_g| ||pair_g - fit_g||^2) and its gradient with "
        "respect to the nuclear coordinates (grid and anchor points fixed), by the hand-written adjoint.");
 
+    // Grid Coulomb metric (python/isdf.py:48-67, :76-92) on the GPU.
+    auto metric = [](const FloatArray& left, const FloatArray& right, const FloatArray& coords,
+                     const FloatArray& weights, bool same, bool symmetrize) {
+        if (coords.ndim() != 2 || coords.shape(1) != 3 || weights.ndim() != 1 || weights.shape(0) != coords.shape(0))
+            throw py::value_error("expected coords[n,3] and weights[n]");
+        const py::ssize_t n = coords.shape(0);
+        if (left.ndim() != 2 || left.shape(1) != n || right.ndim() != 2 || right.shape(1) != n)
+            throw py::value_error("interpolation vectors must have shape (nvec, ngrid)");
+        const py::ssize_t kl = left.shape(0), kr = right.shape(0);
+        py::array_t<double> out({kl, kr});
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_grid_coulomb_metric(left.data(), kl, same ? left.data() : right.data(), kr,
+                                                coords.data(), weights.data(), n, symmetrize ? 1 : 0,
+                                                out.mutable_data());
+        }
+        check(status);
+        return out;
+    };
+    m.def("grid_coulomb_metric", [metric](const FloatArray& interp_vecs, const FloatArray& coords,
+                                          const FloatArray& weights) {
+        return metric(interp_vecs, interp_vecs, coords, weights, true, true);
+    }, py::arg("interp_vecs"), py::arg("coords"), py::arg("weights"),
+       "Spherical-cell grid Coulomb metric of interp_vecs[nvec, ngrid] (not weight scaled), symmetrised.");
+    m.def("grid_cross_metric", [metric](const FloatArray& left, const FloatArray& right, const FloatArray& coords,
+                                        const FloatArray& weights) {
+        return metric(left, right, coords, weights, false, false);
+    }, py::arg("left"), py::arg("right"), py::arg("coords"), py::arg("weights"),
+       "Grid Coulomb metric between two sets of interpolation vectors.");
+
+    m.def("isdf_jk", [](const FloatArray& x, const FloatArray& z, const FloatArray& dm) {
+        if (x.ndim() != 2) throw py::value_error("expected anchor AO values X[k,nao]");
+        const py::ssize_t k = x.shape(0), nao = x.shape(1);
+        if (dm.ndim() != 2 || dm.shape(0) != nao || dm.shape(1) != nao)
+            throw py::value_error("dm must be a finite AO density matrix of shape (nao,nao)");
+        if (z.ndim() != 2 || z.shape(0) != k || z.shape(1) != k)
+            throw py::value_error("Coulomb metric has the wrong anchor dimensions");
+        py::array_t<double> j({nao, nao}), kmat({nao, nao});
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_isdf_jk(x.data(), k, nao, z.data(), dm.data(), j.mutable_data(), kmat.mutable_data());
+        }
+        check(status);
+        return py::make_tuple(j, kmat);
+    }, py::arg("x"), py::arg("metric"), py::arg("dm"),
+       "ISDF J and K of a symmetric AO density from anchor AO values X and the anchor Coulomb metric.");
+
     m.def("ao_adjoint", [](const Context& ctx, const FloatArray& coords, const FloatArray& abar) {
         if (coords.ndim() != 2 || coords.shape(1) != 3) throw py::value_error("expected coords[n,3]");
         int64_t nao = 0;

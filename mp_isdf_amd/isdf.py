@@ -3,12 +3,14 @@
 
 Stages and their timing keys follow the reference (test_isdf.py:62-65):
 ``grid, context, anchor_selection, ao_evaluation, isdf_fit, pair_space_check,
-overlap_check``.  The refit is the same least-squares problem as the
+overlap_check, coulomb_metric``.  The refit is the same least-squares problem as the
 reference's SVD refit (isdf.py:238-263) solved through the anchor-pair Gram
 on the GPU; the overlap check compares with the grid-quadrature AO overlap
 (the reference compares with PySCF's analytic int1e_ovlp, isdf.py:665-671,
-which needs PySCF).  Coulomb metrics, J/K and density sketches are outside
-this build.
+which needs PySCF).  The Coulomb metric is the reference's spherical-cell grid
+metric (isdf.py:48-67) on the GPU, and ``evaluate_isdf_jk`` contracts it into J
+and K on the GPU (isdf.py:130-151).  The integral-direct Cholesky metrics and
+the density sketches are outside this build.
 """
 from __future__ import annotations
 
@@ -24,6 +26,7 @@ ALGORITHMS = {
     "qrcp_online": "OnlineQRColumnPivoting",
     "rand_sketch": "RandomizedQuadratureKhatriRao",
 }
+METRIC_ALGORITHMS = ("grid", "unblocked_greedy", "unblocked_uniform")
 AXES = {
     "ao": "AtomicOrbital",
     "dx": "AtomicOrbitalGradX",
@@ -38,8 +41,15 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                anchor_batch_size: int = 128, fit_batch_size: int = 4096, anchor_sketch_size: int = 128,
                anchor_oversampling: int = 16, anchor_seed: int = 7, mo_coeff=None, nocc: int = 0,
                selection_only: bool = False, pair_space_error: bool = False, interp_vecs: bool = True,
+               compute_metric: bool = False, metric_algorithm: str = "grid",
                report_timings: bool = False, debug: bool = False):
-    """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU."""
+    """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU.
+
+    ``compute_metric=True`` adds the anchor Coulomb metric
+    ``anchor_weights[:, None] * grid_coulomb_metric(theta) * anchor_weights[None, :]``
+    as ``result["coulomb_metric"]`` (python/isdf.py:674-689); only the reference's
+    ``metric_algorithm="grid"`` is built.
+    """
     from . import mp_isdf
 
     started = time.perf_counter()
@@ -68,6 +78,15 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         raise ValueError("fit_batch_size must be a positive integer")
     if pair_space_error and selection_only:
         raise ValueError("pair_space_error requires fitting (selection_only=False)")
+    if metric_algorithm not in METRIC_ALGORITHMS:
+        raise ValueError(f"unknown metric algorithm: {metric_algorithm!r}")
+    if selection_only and compute_metric:
+        raise ValueError("selection_only requires compute_metric=False")
+    if compute_metric and metric_algorithm != "grid":
+        raise NotImplementedError(f"metric_algorithm={metric_algorithm!r} (integral-direct ERI Cholesky) is not "
+                                  "built; use metric_algorithm='grid'")
+    if compute_metric and not interp_vecs:
+        raise ValueError("compute_metric needs the interpolation vectors (interp_vecs=True)")
     tolerance = 1e-10 if anchor_tol is None else float(anchor_tol)
 
     if grid is None:
@@ -107,6 +126,7 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         "overlap_reconstruction": None,
         "overlap_relative_error": None,
         "X1": None,
+        "coulomb_metric": None,
         "timings": timings,
     }
     if selection_only:
@@ -133,5 +153,49 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     result["overlap_relative_error"] = float(np.linalg.norm(overlap - quadrature) /
                                              max(np.linalg.norm(quadrature), 1e-30))
     record("overlap_check")
+    if compute_metric:
+        raw = grid_coulomb_metric(result["interp_vecs"], coords, weights)
+        result["coulomb_metric"] = anchor_weights[:, None] * raw * anchor_weights[None, :]
+        record("coulomb_metric")
+    else:
+        timings["coulomb_metric"] = 0.0
     result["elapsed"] = last - started
     return result
+
+
+def grid_coulomb_metric(interp_vecs, coords, weights):
+    """Spherical-cell grid Coulomb metric of ``interp_vecs[nvec, ngrid]`` (not
+    weight scaled), symmetrised: the GPU form of python/isdf.py:48-67.  Weights
+    must be nonzero (the self-cell term is c |w|^(-1/3))."""
+    from . import mp_isdf
+    return mp_isdf.grid_coulomb_metric(interp_vecs, coords, weights)
+
+
+def grid_cross_metric(left, right, coords, weights):
+    """Grid Coulomb metric between two sets of interpolation vectors
+    (python/isdf.py:76-92), on the GPU."""
+    from . import mp_isdf
+    return mp_isdf.grid_cross_metric(left, right, coords, weights)
+
+
+def evaluate_isdf_jk(result, dm):
+    """Return (J, K) for a real symmetric AO density matrix (python/isdf.py:130-151).
+
+    ``result["X1"]`` holds the anchor AO values X[p, mu] and
+    ``result["coulomb_metric"]`` the anchor metric Z (both anchor weights
+    included); (mu nu|lambda sigma) ~ sum_pq X[p,mu] X[p,nu] Z[p,q] X[q,lambda] X[q,sigma].
+    The contractions run on the GPU.
+    """
+    from . import mp_isdf
+    x = np.asarray(result["X1"], dtype=np.float64)
+    z = result["coulomb_metric"]
+    if z is None:
+        raise ValueError("a Coulomb metric is required to evaluate J and K")
+    density = np.asarray(dm, dtype=np.float64)
+    if density.shape != (x.shape[1], x.shape[1]) or not np.isfinite(density).all():
+        raise ValueError("dm must be a finite AO density matrix of shape (nao,nao)")
+    if not np.allclose(density, density.T, rtol=1e-12, atol=1e-12):
+        raise ValueError("dm must be symmetric")
+    if np.shape(z) != (len(x), len(x)):
+        raise ValueError("Coulomb metric has the wrong anchor dimensions")
+    return mp_isdf.isdf_jk(x, np.asarray(z, dtype=np.float64), density)

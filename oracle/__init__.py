@@ -258,3 +258,108 @@ def threads() -> int:
     if env and env.isdigit():
         return int(env)
     return len(os.sched_getaffinity(0))
+
+
+# --------------------------------------------------------------------------
+# Grid Coulomb metric and J/K (python/isdf.py:48-67, :76-92, :130-151)
+# --------------------------------------------------------------------------
+
+def grid_cross_metric(left, right, coords, weights):
+    """python/isdf.py:76-92 (C loop in coulomb_oracle.cpp)."""
+    left = np.ascontiguousarray(left, dtype=np.float64)
+    right = np.ascontiguousarray(right, dtype=np.float64)
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    weights = np.ascontiguousarray(weights, dtype=np.float64)
+    n = len(weights)
+    assert left.shape[1] == n and right.shape[1] == n and coords.shape == (n, 3)
+    out = np.zeros((left.shape[0], right.shape[0]))
+    rc = lib().oracle_grid_cross_metric(_p(left), ctypes.c_int64(left.shape[0]), _p(right),
+                                        ctypes.c_int64(right.shape[0]), _p(coords), _p(weights),
+                                        ctypes.c_int64(n), _p(out))
+    if rc != 0:
+        raise ValueError("invalid grid metric sizes")
+    return out
+
+
+def grid_coulomb_metric(interp_vecs, coords, weights):
+    """python/isdf.py:48-67: the cross metric of interp_vecs with itself, symmetrised."""
+    metric = grid_cross_metric(interp_vecs, interp_vecs, coords, weights)
+    return 0.5 * (metric + metric.T)
+
+
+def evaluate_isdf_jk(x, z, dm):
+    """python/isdf.py:130-151 (validation left to the callers' tests)."""
+    projected_density = x @ dm @ x.T
+    j = x.T @ ((z @ np.diag(projected_density))[:, None] * x)
+    k = x.T @ (z * projected_density) @ x
+    return 0.5 * (j + j.T), 0.5 * (k + k.T)
+
+
+def s_functions(atm, bas, env):
+    """Contracted s functions of an s-only basis table (PySCF/libcint slots:
+    bas = [atom, l, nprim, nctr, kappa, ptr_exp, ptr_coeff, 0], atm =
+    [charge, ptr_coord, ...]).  Returns [(centre[3], exps[nprim], coefs[nprim])]
+    with the s angular factor 1/sqrt(4 pi) folded into the coefficients, so
+    phi(r) = sum_p coefs[p] exp(-exps[p] |r - centre|^2)."""
+    out = []
+    for row in np.asarray(bas):
+        atom, l, nprim, nctr, _, pexp, pcoef = (int(v) for v in row[:7])
+        if l != 0:
+            raise ValueError("s_functions handles s shells only")
+        centre = np.asarray(env[int(atm[atom][1]):int(atm[atom][1]) + 3], dtype=np.float64)
+        exps = np.asarray(env[pexp:pexp + nprim], dtype=np.float64)
+        for c in range(nctr):
+            coefs = np.asarray(env[pcoef + c * nprim:pcoef + (c + 1) * nprim]) / np.sqrt(4.0 * np.pi)
+            out.append((centre, exps, coefs))
+    return out
+
+
+def _boys0(t):
+    """F0(t) = (1/2) sqrt(pi / t) erf(sqrt t), F0(0) = 1."""
+    import math
+    return 1.0 - t / 3.0 if t < 1e-12 else 0.5 * math.sqrt(math.pi / t) * math.erf(math.sqrt(t))
+
+
+def s_overlap(funcs):
+    """Analytic overlap of contracted s Gaussians: (pi/p)^(3/2) exp(-ab/p |AB|^2)."""
+    nao = len(funcs)
+    s = np.zeros((nao, nao))
+    for u, (ca, ea, wa) in enumerate(funcs):
+        for v, (cb, eb, wb) in enumerate(funcs):
+            r2 = float(np.sum((ca - cb) ** 2))
+            for a, x in zip(ea, wa):
+                for b, y in zip(eb, wb):
+                    p = a + b
+                    s[u, v] += x * y * (np.pi / p) ** 1.5 * np.exp(-a * b / p * r2)
+    return s
+
+
+def s_eri(funcs):
+    """Analytic two-electron integrals (uv|wx) of contracted s Gaussians:
+    primitive (ab|cd) = 2 pi^(5/2) / (p q sqrt(p + q)) K_ab K_cd F0(pq/(p+q) |P - Q|^2)
+    (Boys 1950; Szabo and Ostlund eq. A.41)."""
+    prims = []  # per pair (u, v): list of (p, P, prefactor)
+    nao = len(funcs)
+    for u, (ca, ea, wa) in enumerate(funcs):
+        row = []
+        for v, (cb, eb, wb) in enumerate(funcs):
+            r2 = float(np.sum((ca - cb) ** 2))
+            lst = []
+            for a, x in zip(ea, wa):
+                for b, y in zip(eb, wb):
+                    p = a + b
+                    lst.append((p, (a * ca + b * cb) / p, x * y * np.exp(-a * b / p * r2)))
+            row.append(lst)
+        prims.append(row)
+    eri = np.zeros((nao,) * 4)
+    for u in range(nao):
+        for v in range(nao):
+            for w in range(nao):
+                for x in range(nao):
+                    total = 0.0
+                    for p, pc, kp in prims[u][v]:
+                        for q, qc, kq in prims[w][x]:
+                            t = p * q / (p + q) * float(np.sum((pc - qc) ** 2))
+                            total += 2 * np.pi ** 2.5 / (p * q * np.sqrt(p + q)) * kp * kq * _boys0(t)
+                    eri[u, v, w, x] = total
+    return eri

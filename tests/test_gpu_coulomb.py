@@ -1,0 +1,123 @@
+"""GPU grid Coulomb metric (python/isdf.py:48-67, :76-92) and J/K
+(:130-151) against the oracle, and the C1 case (H2 / STO-3G, qrcp_dense,
+3 anchors) against the analytic integrals.
+
+Tolerances: the metric sums n^2 FP64 products in a different order than the
+oracle (MFMA partial sums per 4-wide K step, row blocks), so it is compared
+at 1e-11 of the largest entry; the J/K contractions at 1e-12."""
+import numpy as np
+import pytest
+
+import oracle
+from mp_isdf_amd import isdf as driver
+from mp_isdf_amd.molecule import hydrogen_molecule
+
+pytestmark = pytest.mark.gpu
+
+
+def random_case(n, kl, kr, seed):
+    rng = np.random.default_rng(seed)
+    xyz = rng.normal(scale=2.0, size=(n, 3))
+    xyz[7] = xyz[3]  # two distinct grid points at one position: kernel 0
+    w = rng.uniform(0.01, 1.0, size=n)
+    w[::11] *= -1.0  # negative quadrature weights
+    left = rng.normal(size=(kl, n))
+    right = rng.normal(size=(kr, n))
+    return xyz, w, left, right
+
+
+def close(got, want, rel):
+    scale = np.abs(want).max()
+    assert np.abs(got - want).max() <= rel * scale, np.abs(got - want).max() / scale
+
+
+def test_metric_reference_fixture(mp_isdf):
+    # test_isdf.py:205-214
+    coords = np.array([[0., 0., 0.], [1., 0., 0.], [0., 1., 0.]])
+    weights = np.array([0.3, -0.2, 0.5])
+    left = np.array([[1., 2., -1.]])
+    right = np.array([[0., 1., 3.], [2., 0., 1.]])
+    stacked = mp_isdf.grid_coulomb_metric(np.concatenate((left, right)), coords, weights)
+    close(stacked, oracle.grid_coulomb_metric(np.concatenate((left, right)), coords, weights), 1e-14)
+    close(mp_isdf.grid_cross_metric(left, right, coords, weights), stacked[:1, 1:], 1e-14)
+    close(mp_isdf.grid_cross_metric(right, left, coords, weights), stacked[1:, :1], 1e-14)
+
+
+@pytest.mark.parametrize("n,k", [(3001, 37), (1024, 130)])
+def test_symmetric_metric_matches_oracle(mp_isdf, n, k):
+    xyz, w, theta, _ = random_case(n, k, 1, seed=n)
+    got = mp_isdf.grid_coulomb_metric(theta, xyz, w)
+    want = oracle.grid_coulomb_metric(theta, xyz, w)
+    np.testing.assert_array_equal(got, got.T)
+    close(got, want, 1e-11)
+
+
+def test_cross_metric_over_two_row_blocks_matches_oracle(mp_isdf):
+    # 20011 points: the kernel rows are generated in two blocks (13312 + 6699)
+    # and n is not a multiple of 4 (scalar load path of the product).
+    xyz, w, left, right = random_case(20011, 5, 130, seed=4)
+    got = mp_isdf.grid_cross_metric(left, right, xyz, w)
+    want = oracle.grid_cross_metric(left, right, xyz, w)
+    close(got, want, 1e-11)
+
+
+def test_metric_rejects_zero_weights_and_bad_shapes(mp_isdf):
+    xyz, w, theta, _ = random_case(50, 3, 1, seed=1)
+    w[4] = 0.0
+    with pytest.raises(ValueError, match="nonzero"):
+        mp_isdf.grid_coulomb_metric(theta, xyz, w)
+    with pytest.raises(ValueError):
+        mp_isdf.grid_coulomb_metric(theta[:, :10], xyz, np.abs(w) + 1)
+    empty = mp_isdf.grid_coulomb_metric(np.zeros((3, 0)), np.zeros((0, 3)), np.zeros(0))
+    np.testing.assert_array_equal(empty, np.zeros((3, 3)))
+
+
+@pytest.mark.parametrize("k,nao", [(50, 37), (3, 2), (260, 140)])
+def test_jk_matches_oracle(mp_isdf, k, nao):
+    rng = np.random.default_rng(k)
+    x = rng.normal(size=(k, nao))
+    z = rng.normal(size=(k, k))
+    z = z + z.T
+    dm = rng.normal(size=(nao, nao))
+    dm = dm + dm.T
+    j, kk = mp_isdf.isdf_jk(x, z, dm)
+    jo, ko = oracle.evaluate_isdf_jk(x, z, dm)
+    close(j, jo, 1e-12)
+    close(kk, ko, 1e-12)
+    np.testing.assert_array_equal(j, j.T)
+    np.testing.assert_array_equal(kk, kk.T)
+
+
+def test_c1_jk_against_oracle_and_exact_integrals(mp_isdf):
+    mol = hydrogen_molecule("sto-3g")
+    result = driver.build_isdf(mol, level=3, anchor_rank=3, anchor_algorithm="qrcp_dense", compute_metric=True)
+    assert len(result["anchor_indices"]) == 3
+    assert result["timings"]["coulomb_metric"] >= 0
+    theta, coords, weights = result["interp_vecs"], result["coordinates"], result["weights"]
+    aw = result["anchor_weights"]
+    want_metric = aw[:, None] * oracle.grid_coulomb_metric(theta, coords, weights) * aw[None, :]
+    close(result["coulomb_metric"], want_metric, 1e-11)
+
+    funcs = oracle.s_functions(mol.atm, mol.bas, mol.env)
+    s12 = oracle.s_overlap(funcs)[0, 1]
+    c = np.array([1.0, 1.0]) / np.sqrt(2 * (1 + s12))
+    dm = 2 * np.outer(c, c)
+    j, k = driver.evaluate_isdf_jk(result, dm)
+    jo, ko = oracle.evaluate_isdf_jk(result["X1"], result["coulomb_metric"], dm)
+    close(j, jo, 1e-12)
+    close(k, ko, 1e-12)
+    eri = oracle.s_eri(funcs)
+    dj = np.abs(j - np.einsum("uvwx,wx->uv", eri, dm)).max()
+    dk = np.abs(k - np.einsum("uwvx,wx->uv", eri, dm)).max()
+    print(f"C1 H2/STO-3G level 3: max|dJ| = {dj:.3e}, max|dK| = {dk:.3e}")
+    assert dj < 1e-3 and dk < 1e-3
+
+    # Validation messages of python/isdf.py:140-147
+    with pytest.raises(ValueError, match="density"):
+        driver.evaluate_isdf_jk(result, np.ones((3, 3)))
+    with pytest.raises(ValueError, match="symmetric"):
+        driver.evaluate_isdf_jk(result, np.array([[1., 1.], [0., 1.]]))
+    with pytest.raises(ValueError, match="metric"):
+        driver.evaluate_isdf_jk({**result, "coulomb_metric": None}, dm)
+    with pytest.raises(ValueError, match="symmetric"):
+        mp_isdf.isdf_jk(result["X1"], result["coulomb_metric"], np.array([[1., 1.], [0., 1.]]))
