@@ -54,6 +54,10 @@ def parse():
     p.add_argument("--no-fp64-form", action="store_true", help="skip the all-FP64 comparison run")
     p.add_argument("--cpu-points", type=int, default=12288, help="grid points in the CPU baseline sample (~10-15 s of work)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--metric-points", type=int, default=262144,
+                   help="grid points of the Coulomb-metric leg (0 skips it)")
+    p.add_argument("--metric-vectors", type=int, default=1536, help="interpolation vectors of the metric leg")
+    p.add_argument("--metric-cpu-points", type=int, default=4096, help="grid points of the metric's CPU sample")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="finish with three device-to-device copies of this many MiB (a known-size kernel for "
                         "calibrating HBM byte counters under rocprofv3 --pmc)")
@@ -239,6 +243,46 @@ def main():
                          f"infeasible at this size (dgeqp3 on {ell}x{n})"}
         log(f"cpu baseline: {m} points in {tc:.1f} s")
 
+    # ---- grid Coulomb metric leg (python/isdf.py:48-67 on the GPU) ------------
+    metric_leg = None
+    if args.metric_points > 0:
+        nm, km = min(args.metric_points, n), args.metric_vectors
+        gen = torch.Generator(device="cuda").manual_seed(7)
+        theta_m = torch.randn((km, nm), generator=gen, dtype=torch.float64, device="cuda")
+        out_m = torch.empty((km, km), dtype=torch.float64, device="cuda")
+        xm, wm = xyz_dev[:nm].contiguous(), w_dev[:nm].contiguous()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        capi.call("mpisdf_grid_coulomb_metric_device", theta_m, km, nm, theta_m, km, nm, xm, wm, nm, True, out_m, 0,
+                  None)
+        tm = time.perf_counter() - t
+        flops_m = 2.0 * nm * nm * km + 2.0 * nm * km * km
+        log(f"coulomb metric: {nm} points x {km} vectors in {tm:.2f} s ({flops_m / tm / 1e12:.1f} TFLOP/s)")
+        metric_leg = {
+            "workload": f"grid Coulomb metric of {km} synthetic interpolation vectors over the first {nm} kept grid "
+                        f"points (the (H2O)8 cc-pVDZ size: ~2.5e5 points, 8 anchors per AO)",
+            "seconds": round(tm, 4),
+            "roofline": {"kernel": "coulomb_rows_kernel + tn_gemm_kernel (V = K' Theta^T, M += Theta V)",
+                         "bound": "mfma", "achieved": round(flops_m / tm / 1e12, 2),
+                         "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                         "frac": round(flops_m / tm / 1e12 / PEAK["fp64_mfma_tflops"], 4), "traffic": None,
+                         "work": f"2*n^2*k + 2*n*k^2 = {flops_m:.3e} flops, n={nm}, k={km}, one call "
+                                 "(wall clock of the synchronous entry point)"},
+        }
+        del theta_m
+        if not args.no_cpu:
+            import oracle
+            mc = min(args.metric_cpu_points, nm)
+            th_c = np.random.default_rng(7).normal(size=(km, mc))
+            t = time.perf_counter()
+            oracle.grid_coulomb_metric(th_c, xyz[keep][:mc], w[keep][:mc])
+            tc = time.perf_counter() - t
+            fc = 2.0 * mc * mc * km + 2.0 * mc * km * km
+            metric_leg["cpu_baseline"] = {
+                "value": round(fc / tc / 1e9, 2), "unit": "GFLOP/s", "cores": oracle.threads(), "kind": "port",
+                "sample": f"oracle grid metric (C loops, FP64) of {km} vectors over {mc} grid points in {tc:.1f} s"}
+            log(f"coulomb metric cpu sample: {mc} points in {tc:.1f} s")
+
     value = n / t_step
     out = {
         "metric": "anchor-select+fit s and grid-pts/s, BF16/FP8 vs FP64 GPU at equal fit error",
@@ -261,6 +305,7 @@ def main():
         "roofline_fit": dict(roofline_fit, seconds_in_step=round(fit_s, 3)),
         "roofline_sketch": dict(roofline_sketch, seconds_in_step=round(sk_s, 3)),
         "cpu_baseline": cpu,
+        "coulomb_metric": metric_leg,
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:

@@ -200,12 +200,14 @@ int mpisdf_isdf_fit_gradient(const mpisdf_context* ctx, const double* anchor_xyz
 int mpisdf_grid_coulomb_metric(const double* left, int64_t kl, const double* right, int64_t kr, const double* xyz,
                                const double* w, int64_t n, int symmetrize, double* metric);
 /* Device form: left/right rows at stride ldl/ldr (>= n; multiples of 4 take
- * the vector load path), device xyz, w and metric.  Values are not checked;
- * a zero weight makes the self term infinite.  Returns after the stream has
- * drained. */
+ * the vector load path), device xyz, w and metric.  block_rows (a multiple
+ * of 128, or 0 for the automatic size) sets how many kernel rows one block
+ * materialises in device memory (block_rows * n doubles), the role of the
+ * reference's metric_batch_size.  Values are not checked; a zero weight makes
+ * the self term infinite.  Returns after the stream has drained. */
 int mpisdf_grid_coulomb_metric_device(const double* left, int64_t kl, int64_t ldl, const double* right, int64_t kr,
                                       int64_t ldr, const double* xyz, const double* w, int64_t n, int symmetrize,
-                                      double* metric, void* stream);
+                                      double* metric, int64_t block_rows, void* stream);
 /* ISDF J and K, python/isdf.py:130-151 (evaluate_isdf_jk): x[k*nao] anchor AO
  * values, z[k*k] the anchor metric (both anchor weights included), dm[nao*nao]
  * a finite symmetric density; P = x dm x^T, J = x^T diag(z diag(P)) x,

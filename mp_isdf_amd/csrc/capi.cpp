@@ -532,7 +532,7 @@ extern "C" int mpisdf_grid_coulomb_metric(const double* left, int64_t kl, const 
         }
         dxyz.upload(xyz, 3 * n, nullptr);
         dw.upload(w, n, nullptr);
-        mpisdf::grid_coulomb_metric(dl.ptr, kl, ldl, rp, kr, ldr, dxyz.ptr, dw.ptr, n, symmetrize != 0, dm.ptr,
+        mpisdf::grid_coulomb_metric(dl.ptr, kl, ldl, rp, kr, ldr, dxyz.ptr, dw.ptr, n, symmetrize != 0, dm.ptr, 0,
                                     nullptr);
         MPISDF_HIP(hipMemcpy(metric, dm.ptr, kl * kr * sizeof(double), hipMemcpyDeviceToHost));
     });
@@ -540,12 +540,13 @@ extern "C" int mpisdf_grid_coulomb_metric(const double* left, int64_t kl, const 
 
 extern "C" int mpisdf_grid_coulomb_metric_device(const double* left, int64_t kl, int64_t ldl, const double* right,
                                                  int64_t kr, int64_t ldr, const double* xyz, const double* w,
-                                                 int64_t n, int symmetrize, double* metric, void* stream)
+                                                 int64_t n, int symmetrize, double* metric, int64_t block_rows,
+                                                 void* stream)
 {
     return guard([&] {
         if (kl > 0 && kr > 0 && (!metric || (n > 0 && (!left || !right || !xyz || !w))))
             throw std::invalid_argument("null metric argument");
-        mpisdf::grid_coulomb_metric(left, kl, ldl, right, kr, ldr, xyz, w, n, symmetrize != 0, metric,
+        mpisdf::grid_coulomb_metric(left, kl, ldl, right, kr, ldr, xyz, w, n, symmetrize != 0, metric, block_rows,
                                     static_cast<hipStream_t>(stream));
         MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     });

@@ -5,8 +5,8 @@
 // The metric is M = L diag(w) K diag(w) R^T over n grid points, K the
 // point-to-point Coulomb kernel with a spherical-cell self term.  Its cost is
 // 2 n^2 kr + 2 n kl kr flops: an N-body product.  Rows of the weighted kernel
-// K'_ij = w_i K_ij w_j are generated in blocks of R rows (rows x n FP64, at
-// most 2 GiB) by one VALU kernel, then two FP64 MFMA TN products consume the
+// K'_ij = w_i K_ij w_j are generated in blocks of R rows (R x n FP64, sized
+// so that the product below runs ~1024 workgroups) by one VALU kernel, then two FP64 MFMA TN products consume the
 // block:  V = K'_blk R^T (rows x kr),  M += L_blk V.  The kernel block is
 // written once and read by the product once per 128 columns of R, so for
 // kr >= 128 the block costs 8 (1 + kr/128) bytes against 2 kr flops per
@@ -22,6 +22,10 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
+// Kernel rows one thread writes for its column j (its point's coordinates and
+// weight stay in registers; row i's are workgroup-uniform scalar loads).
+constexpr int kRowsPerThread = 16;
+
 // K'[r * ldk + j] = w_i w_j K_ij for grid point i = i0 + r.
 __global__ __launch_bounds__(256) void coulomb_rows_kernel(const double* __restrict__ xyz,
                                                            const double* __restrict__ w, int64_t n, int64_t i0,
@@ -31,17 +35,20 @@ __global__ __launch_bounds__(256) void coulomb_rows_kernel(const double* __restr
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const double xj = xyz[3 * j], yj = xyz[3 * j + 1], zj = xyz[3 * j + 2], wj = w[j];
-    for (int64_t r = blockIdx.y; r < rows; r += gridDim.y) {
-        const int64_t i = i0 + r;
-        double v;
-        if (i == j) {
-            v = csphere * pow(fabs(wj), -1.0 / 3.0) * wj * wj;
-        } else {
-            const double dx = xyz[3 * i] - xj, dy = xyz[3 * i + 1] - yj, dz = xyz[3 * i + 2] - zj;
-            const double d2 = dx * dx + dy * dy + dz * dz;
-            v = d2 > 0.0 ? w[i] * wj / sqrt(d2) : 0.0;
+    for (int64_t r0 = (int64_t)blockIdx.y * kRowsPerThread; r0 < rows; r0 += (int64_t)gridDim.y * kRowsPerThread) {
+        const int64_t rend = r0 + kRowsPerThread < rows ? r0 + kRowsPerThread : rows;
+        for (int64_t r = r0; r < rend; ++r) {
+            const int64_t i = i0 + r;
+            double v;
+            if (i == j) {
+                v = csphere * pow(fabs(wj), -1.0 / 3.0) * wj * wj;
+            } else {
+                const double dx = xyz[3 * i] - xj, dy = xyz[3 * i + 1] - yj, dz = xyz[3 * i + 2] - zj;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                v = d2 > 0.0 ? w[i] * wj / sqrt(d2) : 0.0;
+            }
+            kb[r * ldk + j] = v;
         }
-        kb[r * ldk + j] = v;
     }
 }
 
@@ -89,17 +96,25 @@ unsigned blocks_of(int64_t count) { return (unsigned)((count + 255) / 256); }
 
 }  // namespace
 
-int64_t coulomb_rows_per_block(int64_t n)
+int64_t coulomb_rows_per_block(int64_t n, int64_t kr)
 {
     const int64_t ldk = (std::max<int64_t>(n, 1) + 3) / 4 * 4;
-    int64_t rows = (int64_t{1} << 31) / (8 * ldk);
-    rows = std::max<int64_t>(128, rows / 128 * 128);
-    return std::min(rows, (n + 127) / 128 * 128);
+    // The V = K'_blk R^T product runs one 256-thread workgroup per 128 x 128
+    // tile, two per CU (73.7 KB of LDS each): 1024 tiles fill the 256 CUs in
+    // exactly two rounds.
+    const int64_t colgroups = std::max<int64_t>(1, (kr + 127) / 128);
+    int64_t rows = std::max<int64_t>(1, 1024 / colgroups) * 128;
+    // The block takes at most a third of the free device memory and 32 GiB.
+    size_t free_bytes = 0, total_bytes = 0;
+    MPISDF_HIP(hipMemGetInfo(&free_bytes, &total_bytes));
+    const int64_t budget = std::min<int64_t>(int64_t{32} << 30, (int64_t)(free_bytes / 3));
+    rows = std::min(rows, std::max<int64_t>(128, budget / (8 * ldk) / 128 * 128));
+    return std::max<int64_t>(128, std::min(rows, (n + 127) / 128 * 128));
 }
 
 void grid_coulomb_metric(const double* left, int64_t kl, int64_t ldl, const double* right, int64_t kr, int64_t ldr,
                          const double* xyz, const double* w, int64_t n, bool symmetrize_out, double* metric,
-                         hipStream_t s)
+                         int64_t block_rows, hipStream_t s)
 {
     if (kl < 0 || kr < 0 || n < 0) throw std::invalid_argument("negative metric sizes");
     if (symmetrize_out && kl != kr) throw std::invalid_argument("a symmetrised metric needs kl == kr");
@@ -112,12 +127,14 @@ void grid_coulomb_metric(const double* left, int64_t kl, int64_t ldl, const doub
     if (ldl < n || ldr < n) throw std::invalid_argument("metric operand leading dimensions are below n");
     const double csphere = (32.0 / 15.0) * kPi * kPi * std::pow(3.0 / (4.0 * kPi), 5.0 / 3.0);
     const int64_t ldk = (n + 3) / 4 * 4;
-    const int64_t rblk = coulomb_rows_per_block(n);
+    if (block_rows < 0 || block_rows % 128 != 0)
+        throw std::invalid_argument("block_rows must be 0 (automatic) or a positive multiple of 128");
+    const int64_t rblk = block_rows ? block_rows : coulomb_rows_per_block(n, kr);
     DeviceBuffer<double> kb((size_t)(rblk * ldk)), v((size_t)(rblk * kr));
     const unsigned gx = blocks_of(n);
     for (int64_t i0 = 0; i0 < n; i0 += rblk) {
         const int64_t rows = std::min(rblk, n - i0);
-        const unsigned gy = (unsigned)std::min<int64_t>(rows, 65535);
+        const unsigned gy = (unsigned)std::min<int64_t>((rows + kRowsPerThread - 1) / kRowsPerThread, 65535);
         hipLaunchKernelGGL(coulomb_rows_kernel, dim3(gx, gy), dim3(256), 0, s, xyz, w, n, i0, rows, ldk, csphere,
                            kb.ptr);
         MPISDF_HIP(hipGetLastError());

@@ -5,9 +5,10 @@
 
 namespace mpisdf {
 
-// Grid points whose kernel rows one block of the metric materialises: the
-// block (rows x n FP64) stays within 2 GiB and holds a multiple of 128 rows.
-int64_t coulomb_rows_per_block(int64_t n);
+// Grid points whose kernel rows one block of the metric materialises for kr
+// right-hand vectors: a multiple of 128 that gives the block's product about
+// 1024 128 x 128 tiles, within a third of the free device memory and 32 GiB.
+int64_t coulomb_rows_per_block(int64_t n, int64_t kr);
 
 // metric[a * kr + b] = sum_ij left[a * ldl + i] w_i K_ij w_j right[b * ldr + j]
 // over the n grid points (device xyz[n][3], w[n]; left kl x n and right
@@ -16,11 +17,12 @@ int64_t coulomb_rows_per_block(int64_t n);
 //   K_ij = 0                                (i != j, coincident points),
 //   K_ii = c_sphere |w_i|^(-1/3),  c_sphere = (32/15) pi^2 (3 / (4 pi))^(5/3).
 // symmetrize (kl == kr) replaces the result by (M + M^T) / 2.  Weights must
-// be nonzero.  Rows of K are generated block by block into HBM and consumed
-// by the FP64 MFMA TN product.
+// be nonzero.  Rows of K are generated block by block (block_rows rows, a
+// multiple of 128; 0 = coulomb_rows_per_block) into HBM and consumed by the
+// FP64 MFMA TN product.
 void grid_coulomb_metric(const double* left, int64_t kl, int64_t ldl, const double* right, int64_t kr, int64_t ldr,
                          const double* xyz, const double* w, int64_t n, bool symmetrize, double* metric,
-                         hipStream_t s);
+                         int64_t block_rows, hipStream_t s);
 
 // ISDF Coulomb (J) and exchange (K) matrices of a symmetric AO density dm
 // (nao x nao) from anchor AO values x (k x nao row-major) and the anchor

@@ -52,13 +52,33 @@ def test_symmetric_metric_matches_oracle(mp_isdf, n, k):
     close(got, want, 1e-11)
 
 
-def test_cross_metric_over_two_row_blocks_matches_oracle(mp_isdf):
-    # 20011 points: the kernel rows are generated in two blocks (13312 + 6699)
-    # and n is not a multiple of 4 (scalar load path of the product).
+def test_cross_metric_matches_oracle(mp_isdf):
+    # 20011 points (not a multiple of 4: scalar loads in the product), 130
+    # right-hand vectors (two column tiles)
     xyz, w, left, right = random_case(20011, 5, 130, seed=4)
-    got = mp_isdf.grid_cross_metric(left, right, xyz, w)
-    want = oracle.grid_cross_metric(left, right, xyz, w)
-    close(got, want, 1e-11)
+    close(mp_isdf.grid_cross_metric(left, right, xyz, w), oracle.grid_cross_metric(left, right, xyz, w), 1e-11)
+
+
+@pytest.mark.parametrize("block_rows,ld", [(1024, 3001), (128, 3004), (4096, 3004), (0, 3008)])
+def test_device_metric_row_blocks_match_oracle(mp_isdf, block_rows, ld):
+    """The device entry point with explicit row blocks (3001 points: three
+    1024-row blocks, 24 blocks of 128, one partial 4096-row block, automatic)
+    and padded operand rows."""
+    import torch
+    from mp_isdf_amd import capi
+    n, kl, kr = 3001, 7, 130
+    xyz, w, left, right = random_case(n, kl, kr, seed=9)
+    lpad = np.zeros((kl, ld))
+    rpad = np.zeros((kr, ld))
+    lpad[:, :n] = left
+    rpad[:, :n] = right
+    dl, dr = torch.tensor(lpad, device="cuda"), torch.tensor(rpad, device="cuda")
+    dx, dw = torch.tensor(xyz, device="cuda"), torch.tensor(w, device="cuda")
+    out = torch.full((kl, kr), float("nan"), dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_grid_coulomb_metric_device", dl, kl, ld, dr, kr, ld, dx, dw, n, False, out, block_rows, None)
+    close(out.cpu().numpy(), oracle.grid_cross_metric(left, right, xyz, w), 1e-11)
+    with pytest.raises(ValueError, match="block_rows"):
+        capi.call("mpisdf_grid_coulomb_metric_device", dl, kl, ld, dr, kr, ld, dx, dw, n, False, out, 100, None)
 
 
 def test_metric_rejects_zero_weights_and_bad_shapes(mp_isdf):
