@@ -6,8 +6,11 @@
 // point-to-point Coulomb kernel with a spherical-cell self term.  Its cost is
 // 2 n^2 kr + 2 n kl kr flops: an N-body product.  Rows of the weighted kernel
 // K'_ij = w_i K_ij w_j are generated in blocks of R rows (R x n FP64, sized
-// so that the product below runs ~1024 workgroups) by one VALU kernel, then two FP64 MFMA TN products consume the
-// block:  V = K'_blk R^T (rows x kr),  M += L_blk V.  The kernel block is
+// so that the product below runs ~1024 workgroups) by one VALU kernel, then
+// two FP64 MFMA TN products consume the block:  V = K'_blk R^T (rows x kr),
+// M += L_blk V.  For the symmetric metric of one vector set only the R x R
+// blocks on and above the diagonal are generated and multiplied (off-diagonal
+// ones twice weighted), which halves the dominant product.  The kernel block is
 // written once and read by the product once per 128 columns of R, so for
 // kr >= 128 the block costs 8 (1 + kr/128) bytes against 2 kr flops per
 // element: FP64 MFMA bound above kr ~ 40 at 8 TB/s.
@@ -26,14 +29,16 @@ constexpr double kPi = 3.14159265358979323846;
 // weight stay in registers; row i's are workgroup-uniform scalar loads).
 constexpr int kRowsPerThread = 16;
 
-// K'[r * ldk + j] = w_i w_j K_ij for grid point i = i0 + r.
+// K'[r * ldk + c] = w_i w_j K_ij for grid points i = i0 + r (r < rows) and
+// j = j0 + c (c < cols).
 __global__ __launch_bounds__(256) void coulomb_rows_kernel(const double* __restrict__ xyz,
-                                                           const double* __restrict__ w, int64_t n, int64_t i0,
-                                                           int64_t rows, int64_t ldk, double csphere,
+                                                           const double* __restrict__ w, int64_t i0, int64_t rows,
+                                                           int64_t j0, int64_t cols, int64_t ldk, double csphere,
                                                            double* __restrict__ kb)
 {
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= n) return;
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= cols) return;
+    const int64_t j = j0 + c;
     const double xj = xyz[3 * j], yj = xyz[3 * j + 1], zj = xyz[3 * j + 2], wj = w[j];
     for (int64_t r0 = (int64_t)blockIdx.y * kRowsPerThread; r0 < rows; r0 += (int64_t)gridDim.y * kRowsPerThread) {
         const int64_t rend = r0 + kRowsPerThread < rows ? r0 + kRowsPerThread : rows;
@@ -47,9 +52,19 @@ __global__ __launch_bounds__(256) void coulomb_rows_kernel(const double* __restr
                 const double d2 = dx * dx + dy * dy + dz * dz;
                 v = d2 > 0.0 ? w[i] * wj / sqrt(d2) : 0.0;
             }
-            kb[r * ldk + j] = v;
+            kb[r * ldk + c] = v;
         }
     }
+}
+
+void launch_rows(const double* xyz, const double* w, int64_t i0, int64_t rows, int64_t j0, int64_t cols, int64_t ldk,
+                 double csphere, double* kb, hipStream_t s)
+{
+    const unsigned gx = (unsigned)((cols + 255) / 256);
+    const unsigned gy = (unsigned)std::min<int64_t>((rows + kRowsPerThread - 1) / kRowsPerThread, 65535);
+    hipLaunchKernelGGL(coulomb_rows_kernel, dim3(gx, gy), dim3(256), 0, s, xyz, w, i0, rows, j0, cols, ldk, csphere,
+                       kb);
+    MPISDF_HIP(hipGetLastError());
 }
 
 // In place a <- (a + a^T) / 2 for a row-major m x m matrix.
@@ -130,18 +145,32 @@ void grid_coulomb_metric(const double* left, int64_t kl, int64_t ldl, const doub
     if (block_rows < 0 || block_rows % 128 != 0)
         throw std::invalid_argument("block_rows must be 0 (automatic) or a positive multiple of 128");
     const int64_t rblk = block_rows ? block_rows : coulomb_rows_per_block(n, kr);
-    DeviceBuffer<double> kb((size_t)(rblk * ldk)), v((size_t)(rblk * kr));
-    const unsigned gx = blocks_of(n);
-    for (int64_t i0 = 0; i0 < n; i0 += rblk) {
-        const int64_t rows = std::min(rblk, n - i0);
-        const unsigned gy = (unsigned)std::min<int64_t>((rows + kRowsPerThread - 1) / kRowsPerThread, 65535);
-        hipLaunchKernelGGL(coulomb_rows_kernel, dim3(gx, gy), dim3(256), 0, s, xyz, w, n, i0, rows, ldk, csphere,
-                           kb.ptr);
-        MPISDF_HIP(hipGetLastError());
-        // V[r + b * rblk] = sum_j K'[r][j] right[b][j]
-        gemm_tn_f64(rows, kr, n, 1.0, kb.ptr, ldk, right, ldr, 0.0, v.ptr, rblk, s);
-        // metric[a * kr + b] (+)= sum_r V[r + b * rblk] left[a][i0 + r]
-        gemm_tn_f64(kr, kl, rows, 1.0, v.ptr, rblk, left + i0, ldl, i0 == 0 ? 0.0 : 1.0, metric, kr, s);
+    if (symmetrize_out && left == right && ldl == ldr) {
+        // One vector set: K' is symmetric, so only the block pairs J >= I are
+        // generated.  V_I = K'_II T_I^T + 2 sum_{J>I} K'_IJ T_J^T and
+        // M += T_I V_I give M + M^T of the full sum after symmetrisation.
+        DeviceBuffer<double> kb((size_t)(rblk * rblk)), v((size_t)(rblk * kr));
+        for (int64_t i0 = 0; i0 < n; i0 += rblk) {
+            const int64_t rows = std::min(rblk, n - i0);
+            for (int64_t j0 = i0; j0 < n; j0 += rblk) {
+                const int64_t cols = std::min(rblk, n - j0);
+                launch_rows(xyz, w, i0, rows, j0, cols, rblk, csphere, kb.ptr, s);
+                // V[r + b * rblk] (+)= alpha sum_c K'[r][c] right[b][j0 + c]
+                gemm_tn_f64(rows, kr, cols, j0 == i0 ? 1.0 : 2.0, kb.ptr, rblk, right + j0, ldr,
+                            j0 == i0 ? 0.0 : 1.0, v.ptr, rblk, s);
+            }
+            gemm_tn_f64(kr, kl, rows, 1.0, v.ptr, rblk, left + i0, ldl, i0 == 0 ? 0.0 : 1.0, metric, kr, s);
+        }
+    } else {
+        DeviceBuffer<double> kb((size_t)(rblk * ldk)), v((size_t)(rblk * kr));
+        for (int64_t i0 = 0; i0 < n; i0 += rblk) {
+            const int64_t rows = std::min(rblk, n - i0);
+            launch_rows(xyz, w, i0, rows, 0, n, ldk, csphere, kb.ptr, s);
+            // V[r + b * rblk] = sum_j K'[r][j] right[b][j]
+            gemm_tn_f64(rows, kr, n, 1.0, kb.ptr, ldk, right, ldr, 0.0, v.ptr, rblk, s);
+            // metric[a * kr + b] (+)= sum_r V[r + b * rblk] left[a][i0 + r]
+            gemm_tn_f64(kr, kl, rows, 1.0, v.ptr, rblk, left + i0, ldl, i0 == 0 ? 0.0 : 1.0, metric, kr, s);
+        }
     }
     if (symmetrize_out) symmetrize(metric, kl, s);
     MPISDF_HIP(hipStreamSynchronize(s));  // before the block buffers are freed

@@ -81,6 +81,26 @@ def test_device_metric_row_blocks_match_oracle(mp_isdf, block_rows, ld):
         capi.call("mpisdf_grid_coulomb_metric_device", dl, kl, ld, dr, kr, ld, dx, dw, n, False, out, 100, None)
 
 
+@pytest.mark.parametrize("block_rows", [128, 1024, 0])
+def test_device_symmetric_metric_block_pairs_match_oracle(mp_isdf, block_rows):
+    """One vector set, symmetrised: only the kernel blocks on and above the
+    diagonal are generated (3001 points: 24 x 24 blocks of 128, 3 x 3 of 1024,
+    one automatic block)."""
+    import torch
+    from mp_isdf_amd import capi
+    n, k, ld = 3001, 130, 3004
+    xyz, w, theta, _ = random_case(n, k, 1, seed=12)
+    tpad = np.zeros((k, ld))
+    tpad[:, :n] = theta
+    dt = torch.tensor(tpad, device="cuda")
+    dx, dw = torch.tensor(xyz, device="cuda"), torch.tensor(w, device="cuda")
+    out = torch.full((k, k), float("nan"), dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_grid_coulomb_metric_device", dt, k, ld, dt, k, ld, dx, dw, n, True, out, block_rows, None)
+    got = out.cpu().numpy()
+    np.testing.assert_array_equal(got, got.T)
+    close(got, oracle.grid_coulomb_metric(theta, xyz, w), 1e-11)
+
+
 def test_metric_rejects_zero_weights_and_bad_shapes(mp_isdf):
     xyz, w, theta, _ = random_case(50, 3, 1, seed=1)
     w[4] = 0.0
