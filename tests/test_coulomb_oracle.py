@@ -142,3 +142,19 @@ def test_metric_and_jk_fail_loudly_without_a_device():
         module.grid_coulomb_metric(np.ones((1, 2)), np.zeros((2, 3)), np.ones(2))
     with pytest.raises(RuntimeError):
         module.isdf_jk(np.ones((1, 2)), np.ones((1, 1)), np.eye(2))
+
+
+def test_driver_metric_argument_validation():
+    """build_isdf's metric options are checked before any device work
+    (python/isdf.py:463-468)."""
+    from mp_isdf_amd import isdf as driver
+    mol = hydrogen_molecule("sto-3g")
+    grid = (np.zeros((4, 3)), np.ones(4))
+    with pytest.raises(ValueError, match="unknown metric algorithm"):
+        driver.build_isdf(mol, grid=grid, compute_metric=True, metric_algorithm="bogus")
+    with pytest.raises(ValueError, match="selection_only"):
+        driver.build_isdf(mol, grid=grid, selection_only=True, compute_metric=True)
+    with pytest.raises(NotImplementedError, match="grid"):
+        driver.build_isdf(mol, grid=grid, compute_metric=True, metric_algorithm="unblocked_greedy")
+    with pytest.raises(ValueError, match="metric"):
+        driver.evaluate_isdf_jk({"X1": np.ones((1, 2)), "coulomb_metric": None}, np.eye(2))
