@@ -41,6 +41,12 @@ int mpisdf_context_destroy(mpisdf_context* ctx);
 /* LibcintContext.nao (cpp/src/ao_eval_pybind.cpp:91). */
 int mpisdf_context_nao(const mpisdf_context* ctx, int64_t* nao);
 
+/* Analytic AO overlap s[nao*nao] (host array) of the context's functions:
+ * replaces PySCF's mol.intor("int1e_ovlp") in the reference's overlap check
+ * (python/isdf.py:665-671).  Obara-Saika cartesian overlaps, contracted and
+ * transformed exactly as the AO evaluation does. */
+int mpisdf_overlap(const mpisdf_context* ctx, double* s);
+
 /* ---- AO evaluation --------------------------------------------------------- */
 /* LibcintContext.eval_ao / eval_ao_gradient (cpp/src/ao_eval_pybind.cpp:48-64,
  * :92-101 -> cpp/src/ao_eval.cpp:274-310).  Host buffers, FP64.

@@ -91,6 +91,15 @@ int mpisdf_context_nao(const mpisdf_context* ctx, int64_t* nao)
     return guard([&] { *nao = unwrap(ctx).nao; });
 }
 
+int mpisdf_overlap(const mpisdf_context* ctx, double* s)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (!s) throw std::invalid_argument("null overlap output");
+        mpisdf::overlap_matrix(c, s);
+    });
+}
+
 int mpisdf_eval_ao(const mpisdf_context* ctx, const double* coords, int64_t n, int gradient,
                    double* out)
 {

@@ -48,4 +48,7 @@ std::unique_ptr<Context> make_context(const int32_t* atm, int64_t natm, const in
 // Real-solid-harmonic transform (ncomp x ncart) for one angular momentum.
 std::vector<double> transform_table(int l, bool cart);
 
+// Analytic overlap S[mu * nao + nu] of the context's AOs (overlap.cpp).
+void overlap_matrix(const Context& ctx, double* S);
+
 }  // namespace mpisdf

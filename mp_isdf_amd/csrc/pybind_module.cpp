@@ -256,7 +256,13 @@ PYBIND11_MODULE(mp_isdf, m)
         .def("eval_ao_gradient", [](const Context& c, const FloatArray& coords, const std::string& space) {
             return c.evaluate(coords, true, space);
         }, py::arg("coords"), py::arg("space") = "auto",
-           "Return x,y,z AO gradients with shape (3, ngrid, nao).");
+           "Return x,y,z AO gradients with shape (3, ngrid, nao).")
+        .def("overlap", [](const Context& c) {
+            py::array_t<double> s({(py::ssize_t)c.nao(), (py::ssize_t)c.nao()});
+            check(mpisdf_overlap(c.get(), s.mutable_data()));
+            return s;
+        }, "Analytic AO overlap matrix (nao, nao) of the evaluated functions (the int1e_ovlp the reference's "
+           "overlap check compares with).");
 
     m.def("eval_ao", [](const IntArray& atm, const IntArray& bas, const FloatArray& env,
                         const FloatArray& coords, const std::string& space) {

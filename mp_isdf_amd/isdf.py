@@ -5,9 +5,9 @@ Stages and their timing keys follow the reference (test_isdf.py:62-65):
 ``grid, context, anchor_selection, ao_evaluation, isdf_fit, pair_space_check,
 overlap_check, coulomb_metric``.  The refit is the same least-squares problem as the
 reference's SVD refit (isdf.py:238-263) solved through the anchor-pair Gram
-on the GPU; the overlap check compares with the grid-quadrature AO overlap
-(the reference compares with PySCF's analytic int1e_ovlp, isdf.py:665-671,
-which needs PySCF).  The Coulomb metric is the reference's spherical-cell grid
+on the GPU; the overlap check compares with the analytic AO overlap, as the
+reference does with PySCF's int1e_ovlp (isdf.py:665-671), and also with the
+grid-quadrature overlap.  The Coulomb metric is the reference's spherical-cell grid
 metric (isdf.py:48-67) on the GPU, and ``evaluate_isdf_jk`` contracts it into J
 and K on the GPU (isdf.py:130-151).  The integral-direct Cholesky metrics and
 the density sketches are outside this build.
@@ -125,6 +125,7 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         "pair_space_errors": None,
         "overlap_reconstruction": None,
         "overlap_relative_error": None,
+        "overlap_quadrature_relative_error": None,
         "X1": None,
         "coulomb_metric": None,
         "timings": timings,
@@ -145,13 +146,18 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         result["pair_space_errors"] = {"scalar": fit["pair_space_error"]}
         timings["pair_space_check"] = 0.0  # evaluated inside the fit's grid pass
     # Overlap check (isdf.py:665-671): S~ = sum_k phi_k phi_k^T w_k (theta_k . w_grid)
+    # against the analytic overlap (the reference's mol.intor("int1e_ovlp")).
+    # The error against the grid-quadrature overlap isolates the fit's part.
     tsum = fit["theta_weight_sums"]
     overlap = np.einsum("ku,kv,k->uv", anchor_ao, anchor_ao, anchor_weights * tsum, optimize=True)
+    exact = context.overlap()
     ao = context.eval_ao(coords)
     quadrature = ao.T @ (ao * weights[:, None])
     result["overlap_reconstruction"] = overlap
-    result["overlap_relative_error"] = float(np.linalg.norm(overlap - quadrature) /
-                                             max(np.linalg.norm(quadrature), 1e-30))
+    result["overlap_relative_error"] = float(np.linalg.norm(overlap - exact) /
+                                             max(np.linalg.norm(exact), 1e-30))
+    result["overlap_quadrature_relative_error"] = float(np.linalg.norm(overlap - quadrature) /
+                                                        max(np.linalg.norm(quadrature), 1e-30))
     record("overlap_check")
     if compute_metric:
         raw = grid_coulomb_metric(result["interp_vecs"], coords, weights)
