@@ -73,8 +73,9 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
 /* AnchorSelection.select (cpp/src/anchor_select_pybind.cpp:162-210 ->
  * cpp/src/anchor_select.cpp:79-173).  points[n*3], weights[n] on the host.
  * algorithm: 0 QRColumnPivoting (qrcp_dense), 1 OnlineQRColumnPivoting
- * (qrcp_online), 2 RandomizedQuadratureKhatriRao (rand_sketch); the other
- * reference values (CVT, density sketches) return status 1.
+ * (qrcp_online), 2 RandomizedQuadratureKhatriRao (rand_sketch); 3 (CVT)
+ * returns status 1; 4 and 5 (density sketches) need
+ * mpisdf_select_importance.
  * precision (rand_sketch only): 0 fp64, 1 fp32, 2 bf16, 3 fp8 e4m3.
  * axis_weights: NULL or naxis_weights == naxes finite positive factors.
  * Writes original grid indices in pivot order to out[0..*count).
@@ -85,6 +86,22 @@ int mpisdf_select(const mpisdf_context* ctx, const double* points, const double*
                   int64_t max_rank, int32_t algorithm, int64_t batch_size, int32_t debug,
                   int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
                   int64_t* out, int64_t capacity, int64_t* count, double* diagnostics);
+
+/* AnchorSelection.select with point_importance and ao_screening_threshold
+ * (cpp/src/anchor_select_pybind.cpp:186-209 -> cpp/src/anchor_select_randomized_id.cpp:92-374):
+ * algorithm 4 RandomizedAODensityKhatriRao, 5 RandomizedAODensityNormalizedKhatriRao
+ * (AO axis only).  point_importance[n] (finite, nonnegative) replaces |w| as
+ * the sketch column weight; points with zero weight or zero importance, and
+ * points whose AO squared norm is <= ao_screening_threshold, are not
+ * candidates; algorithm 5 divides each pair column by that squared norm.
+ * Other algorithms ignore point_importance, as the reference does. */
+int mpisdf_select_importance(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
+                             const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc,
+                             const double* axis_weights, int64_t naxis_weights, double tolerance,
+                             int64_t max_rank, int32_t algorithm, int64_t batch_size, int32_t debug,
+                             int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
+                             const double* point_importance, double ao_screening_threshold,
+                             int64_t* out, int64_t capacity, int64_t* count, double* diagnostics);
 
 /* ---- Khatri-Rao sketch stages (device pointers) ---------------------------- */
 /* Gaussian factors u, v [d][ell] row-major from std::mt19937_64(seed) and

@@ -177,20 +177,21 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
 #include "sketch_mfma.hpp"
 #include "select.hpp"
 
-extern "C" int mpisdf_select(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
-                             const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc,
-                             const double* axis_weights, int64_t naxis_weights, double tolerance,
-                             int64_t max_rank, int32_t algorithm, int64_t batch_size, int32_t debug,
-                             int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
-                             int64_t* out, int64_t capacity, int64_t* count, double* diagnostics)
+namespace {
+
+int select_impl(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
+                const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc, const double* axis_weights,
+                int64_t naxis_weights, double tolerance, int64_t max_rank, int32_t algorithm, int64_t batch_size,
+                int32_t debug, int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
+                const double* point_importance, double ao_screening_threshold, int64_t* out, int64_t capacity,
+                int64_t* count, double* diagnostics)
 {
     return guard([&] {
         auto& c = unwrap(ctx);
         if (n < 0 || !count) throw std::invalid_argument("expected points[ngrid,3] and weights[ngrid]");
         if (naxes <= 0 || !axes) throw std::invalid_argument("axis list must not be empty");
         if (algorithm < 0 || algorithm > 5) throw std::invalid_argument("unknown anchor selection algorithm");
-        if (algorithm > 2)
-            throw std::invalid_argument("CVT and density Khatri-Rao selection are not part of this build");
+        if (algorithm == 3) throw std::invalid_argument("CVT selection is not part of this build");
         if (precision < 0 || precision > 3) throw std::invalid_argument("precision must be 0..3");
         if (batch_size < 0 || sketch_size < 0 || oversampling < 0 || max_rank < 0)
             throw std::invalid_argument("sizes must be non-negative");
@@ -211,11 +212,41 @@ extern "C" int mpisdf_select(const mpisdf_context* ctx, const double* points, co
         a.oversampling = oversampling;
         a.seed = seed;
         a.precision = precision;
+        a.importance = point_importance;
+        a.ao_screening_threshold = ao_screening_threshold;
         const auto sel = mpisdf::select_anchors(c, a, diagnostics);
         if ((int64_t)sel.size() > capacity) throw std::invalid_argument("output capacity is too small");
         std::copy(sel.begin(), sel.end(), out);
         *count = (int64_t)sel.size();
     });
+}
+
+}  // namespace
+
+extern "C" int mpisdf_select(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
+                             const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc,
+                             const double* axis_weights, int64_t naxis_weights, double tolerance,
+                             int64_t max_rank, int32_t algorithm, int64_t batch_size, int32_t debug,
+                             int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
+                             int64_t* out, int64_t capacity, int64_t* count, double* diagnostics)
+{
+    return select_impl(ctx, points, weights, n, axes, naxes, mo_coeff, nocc, axis_weights, naxis_weights, tolerance,
+                       max_rank, algorithm, batch_size, debug, sketch_size, oversampling, seed, precision, nullptr,
+                       1e-24, out, capacity, count, diagnostics);
+}
+
+extern "C" int mpisdf_select_importance(const mpisdf_context* ctx, const double* points, const double* weights,
+                                        int64_t n, const int32_t* axes, int32_t naxes, const double* mo_coeff,
+                                        int64_t nocc, const double* axis_weights, int64_t naxis_weights,
+                                        double tolerance, int64_t max_rank, int32_t algorithm, int64_t batch_size,
+                                        int32_t debug, int64_t sketch_size, int64_t oversampling, uint64_t seed,
+                                        int32_t precision, const double* point_importance,
+                                        double ao_screening_threshold, int64_t* out, int64_t capacity,
+                                        int64_t* count, double* diagnostics)
+{
+    return select_impl(ctx, points, weights, n, axes, naxes, mo_coeff, nocc, axis_weights, naxis_weights, tolerance,
+                       max_rank, algorithm, batch_size, debug, sketch_size, oversampling, seed, precision,
+                       point_importance, ao_screening_threshold, out, capacity, count, diagnostics);
 }
 
 extern "C" int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level,

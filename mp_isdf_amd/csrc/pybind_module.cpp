@@ -171,19 +171,25 @@ public:
                                 py::object point_importance, double ao_screening_threshold,
                                 const std::string& precision, double* diagnostics = nullptr) const
     {
-        (void)cvt_max_iter; (void)cvt_tol; (void)ao_screening_threshold;
+        (void)cvt_max_iter; (void)cvt_tol;
         parse_space(space);
         const double* xyz = read_points(points);
         if (weights.ndim() != 1 || weights.shape(0) != points.shape(0))
             throw py::value_error("expected weights[ngrid]");
         const auto codes = read_axes(axes);
         const auto coeff = read_coefficients(ctx_, mo_coeff);
-        if (!cvt_weights.is_none() || !point_importance.is_none() ||
-            algorithm == LocalAlgorithm::CentroidalVoronoi ||
-            algorithm == LocalAlgorithm::RandomizedAODensityKhatriRao ||
-            algorithm == LocalAlgorithm::RandomizedAODensityNormalizedKhatriRao)
-            throw py::value_error("CVT and density Khatri-Rao selection are not part of this build; use "
-                                  "QRColumnPivoting, OnlineQRColumnPivoting or RandomizedQuadratureKhatriRao");
+        if (!cvt_weights.is_none() || algorithm == LocalAlgorithm::CentroidalVoronoi)
+            throw py::value_error("CVT selection is not part of this build; use QRColumnPivoting, "
+                                  "OnlineQRColumnPivoting or a randomized Khatri-Rao sketch");
+        // anchor_select_pybind.cpp:186-191: the importance, when given, must
+        // match the grid; only the density sketches use it.
+        std::vector<double> importance;
+        if (!point_importance.is_none()) {
+            const auto input = point_importance.cast<FloatArray>();
+            if (input.ndim() != 1 || input.shape(0) != points.shape(0))
+                throw py::value_error("point_importance must have shape (ngrid,)");
+            importance.assign(input.data(), input.data() + input.size());
+        }
         if (audit) throw py::value_error("audit=True is not part of this build");
         if (batch_size < 0 || sketch_size < 0 || oversampling < 0 || max_rank < 0 || nocc < 0)
             throw py::value_error("sizes must be non-negative");
@@ -198,12 +204,12 @@ public:
         int status;
         {
             py::gil_scoped_release release;
-            status = mpisdf_select(ctx_.get(), xyz, weights.data(), points.shape(0), codes.data(),
-                                   (int32_t)codes.size(), coeff.empty() ? nullptr : coeff.data(), nocc,
-                                   axis_weights.empty() ? nullptr : axis_weights.data(),
-                                   (int64_t)axis_weights.size(), tolerance, max_rank,
-                                   static_cast<int32_t>(algorithm), batch_size, debug ? 1 : 0, sketch_size,
-                                   oversampling, seed, prec, out.data(), (int64_t)out.size(), &count, diagnostics);
+            status = mpisdf_select_importance(
+                ctx_.get(), xyz, weights.data(), points.shape(0), codes.data(), (int32_t)codes.size(),
+                coeff.empty() ? nullptr : coeff.data(), nocc, axis_weights.empty() ? nullptr : axis_weights.data(),
+                (int64_t)axis_weights.size(), tolerance, max_rank, static_cast<int32_t>(algorithm), batch_size,
+                debug ? 1 : 0, sketch_size, oversampling, seed, prec, importance.empty() ? nullptr : importance.data(),
+                ao_screening_threshold, out.data(), (int64_t)out.size(), &count, diagnostics);
         }
         check(status);
         out.resize(count);

@@ -36,24 +36,35 @@ double seconds_since(Clock::time_point t) { return std::chrono::duration<double>
 constexpr double kEps = std::numeric_limits<double>::epsilon();
 
 struct Kept {
-    std::vector<double> xyz, w;
+    std::vector<double> xyz, w, importance;
     std::vector<int64_t> original;
 };
 
-Kept filter_points(const SelectArgs& a)
+Kept filter_points(const SelectArgs& a, const double* importance = nullptr)
 {
     Kept k;
     k.xyz.reserve(3 * a.n);
     for (int64_t i = 0; i < a.n; ++i) {
         if (!std::isfinite(a.weights[i])) throw std::invalid_argument("grid weights must be finite");
-        if (a.weights[i] == 0) continue;
+        if (a.weights[i] == 0 || (importance && importance[i] == 0)) continue;
         for (int c = 0; c < 3; ++c)
             if (!std::isfinite(a.points[3 * i + c])) throw std::invalid_argument("grid coordinates must be finite");
         k.xyz.insert(k.xyz.end(), a.points + 3 * i, a.points + 3 * i + 3);
         k.w.push_back(a.weights[i]);
         k.original.push_back(i);
+        if (importance) k.importance.push_back(importance[i]);
     }
     return k;
+}
+
+// q[g] = sum_a f[g * d + a]^2 for the rows of a feature batch.
+__global__ void row_norm2_kernel(const double* __restrict__ f, int64_t rows, int64_t d, double* __restrict__ q)
+{
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= rows) return;
+    double sum = 0;
+    for (int64_t a = 0; a < d; ++a) sum += f[g * d + a] * f[g * d + a];
+    q[g] = sum;
 }
 
 // Device batch for feature evaluation (independent of the reference's
@@ -163,7 +174,8 @@ std::vector<int64_t> select_dense(Context& ctx, const SelectArgs& a, const AxisS
 }
 
 // ---------------------------------------------------------------------------
-// rand_sketch, quadrature policy (cpp/src/anchor_select_randomized_id.cpp:92-374)
+// rand_sketch with the quadrature and density policies
+// (cpp/src/anchor_select_randomized_id.cpp:92-374)
 // ---------------------------------------------------------------------------
 std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const AxisSpec& spec,
                                        const std::vector<double>& scales, double* diag, hipStream_t s)
@@ -177,9 +189,29 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         throw std::overflow_error("randomized ID rank + oversampling overflows size_t");
     const int64_t requested = a.max_rank ? a.max_rank + a.oversampling : a.sketch_size;
     const int64_t d = spec.width;
-    Kept kept = filter_points(a);
-    const int64_t n = (int64_t)kept.original.size();
-    if (n == 0) return {};
+    // Density policies (randomized_id.cpp:102-115, :139, :231-260, :290-293):
+    // importance replaces |w| as the column weight, points with zero
+    // importance or an AO squared norm at or below the screening threshold
+    // are not candidates, and the normalized policy divides each pair
+    // column by that squared norm.
+    const bool density = a.algorithm == kDensity || a.algorithm == kDensityNormalized;
+    const bool normalized = a.algorithm == kDensityNormalized;
+    if (density) {
+        if (!a.importance) throw std::invalid_argument("point_importance must have shape (ngrid,)");
+        for (int64_t i = 0; i < a.n; ++i)
+            if (!std::isfinite(a.importance[i]) || a.importance[i] < 0)
+                throw std::invalid_argument("point_importance must be finite and nonnegative");
+        if (!std::isfinite(a.ao_screening_threshold) || a.ao_screening_threshold < 0)
+            throw std::invalid_argument("AO screening threshold must be finite and nonnegative");
+        if (a.axes.size() != 1 || a.axes[0] != 0)
+            throw std::invalid_argument("density Khatri-Rao sketches with non-AO axis features are NYI");
+    }
+    Kept kept = filter_points(a, density ? a.importance : nullptr);
+    int64_t n = (int64_t)kept.original.size();
+    if (n == 0) {
+        if (density && a.max_rank) throw std::invalid_argument("requested anchor_rank exceeds eligible candidates");
+        return {};
+    }
     const int64_t physical_rank = d * (d + 1) / 2;
     const int64_t physical_limit = physical_rank + a.oversampling;
     const int64_t backend_limit = std::numeric_limits<int64_t>::max() / n;
@@ -189,8 +221,39 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         throw std::invalid_argument(
             "randomized ID sketch/rank exceeds the ILP64 QRCP backend limit for this grid; "
             "reduce the sketch/rank or use a tiled QRCP backend");
+    std::vector<double> ao_norm2;
+    if (density) {
+        // Screening pass: only O(n) squared norms come back to the host.
+        DeviceBuffer<double> d_all, q(n);
+        d_all.upload(kept.xyz.data(), kept.xyz.size(), s);
+        const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
+        DeviceBuffer<double> feat((size_t)batch * d);
+        AxisWorkspace ws0;
+        for (int64_t start = 0; start < n; start += batch) {
+            const int64_t count = std::min(batch, n - start);
+            evaluate_axis_device(ctx, spec, d_all.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
+                                 a.mo_coeff, a.nocc, feat.ptr, ws0, s);
+            hipLaunchKernelGGL(row_norm2_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, feat.ptr,
+                               count, d, q.ptr + start);
+            MPISDF_HIP(hipGetLastError());
+        }
+        const auto hq = q.download(s);
+        Kept screened;
+        for (int64_t g = 0; g < n; ++g) {
+            if (!std::isfinite(hq[g])) throw std::overflow_error("non-finite AO squared norm");
+            if (hq[g] <= a.ao_screening_threshold) continue;
+            screened.xyz.insert(screened.xyz.end(), kept.xyz.begin() + 3 * g, kept.xyz.begin() + 3 * g + 3);
+            screened.w.push_back(kept.w[g]);
+            screened.importance.push_back(kept.importance[g]);
+            screened.original.push_back(kept.original[g]);
+            ao_norm2.push_back(hq[g]);
+        }
+        kept = std::move(screened);
+        n = (int64_t)kept.original.size();
+    }
     if (a.max_rank > std::min(n, physical_rank))
         throw std::invalid_argument("requested anchor_rank exceeds eligible candidates or physical pair rank");
+    if (n == 0) return {};
     if (a.precision == kPrecF32)
         throw std::invalid_argument("precision 'fp32' is not provided; use fp64, bf16 or fp8");
     const bool lowp = a.precision != kPrecF64;
@@ -204,11 +267,13 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
 
     DeviceBuffer<double> d_xyz, sw(n);
     d_xyz.upload(kept.xyz.data(), kept.xyz.size(), s);
-    {
-        std::vector<double> hsw(n);
-        for (int64_t g = 0; g < n; ++g) hsw[g] = std::sqrt(std::abs(kept.w[g]));
-        sw.upload(hsw.data(), n, s);
+    std::vector<double> hsw(n);
+    for (int64_t g = 0; g < n; ++g) {
+        // (phi/sqrt(q) . u)(phi/sqrt(q) . v) = (phi . u)(phi . v) / q
+        hsw[g] = density ? std::sqrt(kept.importance[g]) : std::sqrt(std::abs(kept.w[g]));
+        if (normalized) hsw[g] /= ao_norm2[g];
     }
+    sw.upload(hsw.data(), n, s);
     DeviceBuffer<unsigned long long> bad(1);
     MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
     DeviceBuffer<double> y64;
@@ -501,7 +566,9 @@ std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& a, double* d
         out = select_online(ctx, a, spec, scales, diag, s);
         break;
     }
-    case kRandomized: {
+    case kRandomized:
+    case kDensity:
+    case kDensityNormalized: {
         const auto scales = axis_scales(ctx, a, spec, s);
         out = select_randomized(ctx, a, spec, scales, diag, s);
         break;

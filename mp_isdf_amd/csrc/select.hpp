@@ -7,7 +7,8 @@
 
 namespace mpisdf {
 
-enum Algorithm : int { kDense = 0, kOnline = 1, kRandomized = 2 };
+// LocalAnchorSelectionAlgorithm values (cpp/include/enums.hpp:23-30); 3 is CVT.
+enum Algorithm : int { kDense = 0, kOnline = 1, kRandomized = 2, kDensity = 4, kDensityNormalized = 5 };
 enum Precision : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecFP8 = 3 };
 
 struct SelectArgs {
@@ -27,6 +28,10 @@ struct SelectArgs {
     int64_t oversampling = 16;
     uint64_t seed = 7;
     int precision = kPrecF64;
+    // Density Khatri-Rao sketches (kDensity, kDensityNormalized): host [n]
+    // nonnegative importance, and the AO squared-norm screening threshold.
+    const double* importance = nullptr;
+    double ao_screening_threshold = 1e-24;
 };
 
 // Diagnostics slots (double[16]) written when requested.

@@ -7,10 +7,14 @@ overlap_check, coulomb_metric``.  The refit is the same least-squares problem as
 reference's SVD refit (isdf.py:238-263) solved through the anchor-pair Gram
 on the GPU; the overlap check compares with the analytic AO overlap, as the
 reference does with PySCF's int1e_ovlp (isdf.py:665-671), and also with the
-grid-quadrature overlap.  The Coulomb metric is the reference's spherical-cell grid
-metric (isdf.py:48-67) on the GPU, and ``evaluate_isdf_jk`` contracts it into J
+grid-quadrature overlap.  rand_sketch takes the reference's three Khatri-Rao
+sketch techniques (quadrature, density, density_col_norm); the density
+importance is an input here.  The Coulomb metric is the reference's
+spherical-cell grid metric (isdf.py:48-67) on the GPU, and
+``evaluate_isdf_jk`` contracts it into J
 and K on the GPU (isdf.py:130-151).  The integral-direct Cholesky metrics and
-the density sketches are outside this build.
+the PySCF RKS density behind the default density importance are outside this
+build.
 """
 from __future__ import annotations
 
@@ -27,6 +31,13 @@ ALGORITHMS = {
     "rand_sketch": "RandomizedQuadratureKhatriRao",
 }
 METRIC_ALGORITHMS = ("grid", "unblocked_greedy", "unblocked_uniform")
+# Khatri-Rao sketch techniques of rand_sketch (python/isdf.py:30-36)
+KHATRIRAO_BACKENDS = {
+    "quadrature": "RandomizedQuadratureKhatriRao",
+    "density": "RandomizedAODensityKhatriRao",
+    "density_col_norm": "RandomizedAODensityNormalizedKhatriRao",
+}
+DENSITY_SKETCHES = ("density", "density_col_norm")
 AXES = {
     "ao": "AtomicOrbital",
     "dx": "AtomicOrbitalGradX",
@@ -41,14 +52,21 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                anchor_batch_size: int = 128, fit_batch_size: int = 4096, anchor_sketch_size: int = 128,
                anchor_oversampling: int = 16, anchor_seed: int = 7, mo_coeff=None, nocc: int = 0,
                selection_only: bool = False, pair_space_error: bool = False, interp_vecs: bool = True,
-               compute_metric: bool = False, metric_algorithm: str = "grid",
-               report_timings: bool = False, debug: bool = False):
+               compute_metric: bool = False, metric_algorithm: str = "grid", sketch: str = "quadrature",
+               point_importance=None, density_weight: str = "electron_density",
+               ao_screening_threshold: float = 1e-24, report_timings: bool = False, debug: bool = False):
     """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU.
 
     ``compute_metric=True`` adds the anchor Coulomb metric
     ``anchor_weights[:, None] * grid_coulomb_metric(theta) * anchor_weights[None, :]``
     as ``result["coulomb_metric"]`` (python/isdf.py:674-689); only the reference's
     ``metric_algorithm="grid"`` is built.
+
+    ``sketch="density" | "density_col_norm"`` (rand_sketch only) selects with
+    the AO-density Khatri-Rao sketches weighted by ``point_importance`` (ngrid,
+    finite, nonnegative; python/isdf.py:231-235).  Without it only
+    ``density_weight="quadrature_weights"`` (|w|) is available: the reference
+    computes the electron density with a PySCF RKS calculation.
     """
     from . import mp_isdf
 
@@ -72,6 +90,9 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         raise ValueError(f"unknown axis: {exc.args[0]!r}; choose from {list(AXES)}") from exc
     if anchor_algorithm not in ALGORITHMS:
         raise ValueError(f"unknown anchor algorithm: {anchor_algorithm!r}")
+    if sketch not in KHATRIRAO_BACKENDS:
+        raise ValueError(f"unknown Khatri-Rao sketch technique: {sketch!r}")
+    density_sketch = anchor_algorithm == "rand_sketch" and sketch in DENSITY_SKETCHES
     if anchor_rank and anchor_tol is not None:
         raise ValueError("anchor_rank and anchor_tol are mutually exclusive")
     if not isinstance(fit_batch_size, (int, np.integer)) or fit_batch_size < 1:
@@ -98,15 +119,29 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         raise ValueError("grid has no nonzero weights")
     record("grid")
 
+    importance = None
+    if density_sketch:
+        if point_importance is None:
+            if density_weight != "quadrature_weights":
+                raise NotImplementedError("the electron-density importance needs a PySCF RKS density; pass "
+                                          "point_importance or use density_weight='quadrature_weights'")
+            point_importance = np.abs(all_weights)
+        importance = np.asarray(point_importance, dtype=np.float64)
+        if importance.shape != (len(all_coords),) or not np.isfinite(importance).all() or np.any(importance < 0):
+            raise ValueError("point_importance must be a finite nonnegative array of shape (ngrid,)")
+        importance = np.ascontiguousarray(importance)
+
     context = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env, cart=mol.cart)
     selector = mp_isdf.AnchorSelection(context)
     record("context")
-    algorithm = getattr(mp_isdf.LocalAnchorSelectionAlgorithm, ALGORITHMS[anchor_algorithm])
+    algorithm = getattr(mp_isdf.LocalAnchorSelectionAlgorithm,
+                        KHATRIRAO_BACKENDS[sketch] if anchor_algorithm == "rand_sketch" else ALGORITHMS[anchor_algorithm])
     anchor_indices = np.asarray(selector.select(
         all_coords, all_weights, axis_types, tolerance=tolerance, max_rank=anchor_rank, algorithm=algorithm,
         mo_coeff=mo_coeff, nocc=nocc, batch_size=anchor_batch_size, debug=debug,
         sketch_size=anchor_sketch_size, oversampling=anchor_oversampling, seed=anchor_seed,
-        axis_weights=list(axis_weights) if axis_weights is not None else [], precision=precision), dtype=np.intp)
+        axis_weights=list(axis_weights) if axis_weights is not None else [], precision=precision,
+        point_importance=importance, ao_screening_threshold=ao_screening_threshold), dtype=np.intp)
     if len(anchor_indices) == 0:
         raise ValueError("anchor selection found no anchors; adjust settings or grid")
     record("anchor_selection")

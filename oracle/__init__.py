@@ -31,7 +31,7 @@ LIB_PATH = HERE / "_build" / "liboracle.so"
 _lib = None
 
 AXIS_CODES = {"ao": 0, "dx": 1, "dy": 2, "dz": 3, "occ": 4}
-ALGORITHMS = {"qrcp_dense": 0, "qrcp_online": 1, "rand_sketch": 2}
+ALGORITHMS = {"qrcp_dense": 0, "qrcp_online": 1, "rand_sketch": 2, "density": 4, "density_col_norm": 5}
 
 
 def build(force: bool = False) -> Path:
@@ -163,7 +163,7 @@ def sketch(phi, u, v, sqrt_weights):
 def select(atm, bas, env, points, weights, axes="ao", tolerance=1e-10, max_rank=0,
            algorithm="qrcp_dense", mo_coeff=None, nocc=0, axis_weights=None,
            batch_size=128, sketch_size=128, oversampling=16, seed=7, cart=False,
-           return_stats=False):
+           return_stats=False, point_importance=None, ao_screening_threshold=1e-24):
     atm, bas, env = _tables(atm, bas, env)
     pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     w = np.ascontiguousarray(weights, dtype=np.float64)
@@ -178,13 +178,17 @@ def select(atm, bas, env, points, weights, axes="ao", tolerance=1e-10, max_rank=
     count = ctypes.c_int64()
     stats = np.zeros(4, dtype=np.int64)
     alg = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    imp = None if point_importance is None else np.ascontiguousarray(point_importance, dtype=np.float64)
+    if imp is not None and imp.shape != (len(pts),):
+        raise ValueError("point_importance must have shape (ngrid,)")
     _check(lib().oracle_select(
         _p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)), _p(env),
         ctypes.c_int64(len(env)), ctypes.c_int(int(cart)), _p(pts), _p(w), ctypes.c_int64(len(pts)),
         _p(codes), ctypes.c_int(len(codes)), _p(mo), ctypes.c_int64(nocc), _p(aw),
         ctypes.c_double(tolerance), ctypes.c_int64(max_rank), ctypes.c_int(alg),
         ctypes.c_int64(batch_size), ctypes.c_int64(sketch_size), ctypes.c_int64(oversampling),
-        ctypes.c_uint64(seed), _p(out), ctypes.c_int64(cap), ctypes.byref(count), _p(stats)),
+        ctypes.c_uint64(seed), _p(imp), ctypes.c_double(ao_screening_threshold),
+        _p(out), ctypes.c_int64(cap), ctypes.byref(count), _p(stats)),
         selection=True)
     result = out[:count.value].tolist()
     return (result, stats) if return_stats else result

@@ -153,21 +153,24 @@ void geqp3(int64_t m, int64_t n, double* a, int64_t lda, int64_t* jpvt, double* 
 }
 
 struct Kept {
-    std::vector<double> xyz, w;
+    std::vector<double> xyz, w, importance;
     std::vector<int64_t> original;
 };
 
-Kept filter(const double* points, const double* weights, int64_t n)
+// randomized_id.cpp:137-145: zero-weight points, and for the density
+// sketches zero-importance points, are not candidates.
+Kept filter(const double* points, const double* weights, int64_t n, const double* importance = nullptr)
 {
     Kept k;
     for (int64_t i = 0; i < n; ++i) {
         if (!std::isfinite(weights[i])) throw std::invalid_argument("grid weights must be finite");
-        if (weights[i] == 0) continue;
+        if (weights[i] == 0 || (importance && importance[i] == 0)) continue;
         for (int c = 0; c < 3; ++c)
             if (!std::isfinite(points[3 * i + c])) throw std::invalid_argument("grid coordinates must be finite");
         k.xyz.insert(k.xyz.end(), points + 3 * i, points + 3 * i + 3);
         k.w.push_back(weights[i]);
         k.original.push_back(i);
+        if (importance) k.importance.push_back(importance[i]);
     }
     return k;
 }
@@ -450,38 +453,87 @@ struct RandResult {
     int64_t numerical_rank = 0, cap = 0;
 };
 
-// cpp/src/anchor_select_randomized_id.cpp:92-208 and :210-374 (quadrature)
+// cpp/src/anchor_select_randomized_id.cpp:92-208 and :210-374.  policy 0:
+// quadrature (column weight sqrt|w|); 1: AO density (sqrt of the point
+// importance, after the AO squared-norm screening of :231-260); 2: the same
+// with each AO row divided by its norm before the projections (:290-293,
+// :301-304).
 RandResult select_rand(const Tables& t, const double* points, const double* weights, int64_t npts,
                        double tolerance, int64_t max_rank, int64_t batch_size, int64_t sketch_size,
-                       int64_t oversampling, uint64_t seed, const std::vector<double>& scales)
+                       int64_t oversampling, uint64_t seed, const std::vector<double>& scales, int policy = 0,
+                       const double* importance = nullptr, double screening = 1e-24)
 {
+    const bool density = policy != 0, normalized = policy == 2;
+    if (density) {
+        if (!importance) throw std::invalid_argument("point_importance must have shape (ngrid,)");
+        for (int64_t i = 0; i < npts; ++i)
+            if (!std::isfinite(importance[i]) || importance[i] < 0)
+                throw std::invalid_argument("point_importance must be finite and nonnegative");
+        if (!std::isfinite(screening) || screening < 0)
+            throw std::invalid_argument("AO screening threshold must be finite and nonnegative");
+        if (t.naxes != 1 || t.axes[0] != 0)
+            throw std::invalid_argument("density Khatri-Rao sketches with non-AO axis features are NYI");
+    }
     if (!std::isfinite(tolerance) || tolerance < 0 || batch_size <= 0 || oversampling <= 0 ||
         (!max_rank && sketch_size <= oversampling))
         throw std::invalid_argument("invalid randomized ID grid, tolerance, batch, sketch or rank/oversampling");
     int64_t d = 0;
     features(t, nullptr, 0, {}, d);
     const int64_t requested = max_rank ? max_rank + oversampling : sketch_size;
-    Kept kept = filter(points, weights, npts);
-    const int64_t n = (int64_t)kept.original.size();
+    Kept kept = filter(points, weights, npts, density ? importance : nullptr);
+    int64_t n = (int64_t)kept.original.size();
     RandResult res;
-    if (n == 0) return res;
+    if (n == 0) {
+        if (density && max_rank) throw std::invalid_argument("requested anchor_rank exceeds eligible candidates");
+        return res;
+    }
     const int64_t physical_rank = d * (d + 1) / 2;
     const int64_t backend_limit = std::numeric_limits<int64_t>::max() / n;
     const int64_t ell = requested <= backend_limit ? requested
         : std::min({requested, physical_rank + oversampling, backend_limit});
     if (ell <= oversampling || (max_rank && std::min(max_rank, physical_rank) > ell - oversampling))
         throw std::invalid_argument("randomized ID sketch/rank exceeds the ILP64 QRCP backend limit");
+    std::vector<double> ao_norms;
+    if (density) {
+        Kept screened;
+        for (int64_t start = 0; start < n; start += std::min(batch_size, n - start)) {
+            const int64_t count = std::min(batch_size, n - start);
+            int64_t w = 0;
+            const auto phi = features(t, kept.xyz.data() + 3 * start, count, scales, w);
+            for (int64_t g = 0; g < count; ++g) {
+                double q = 0;
+                for (int64_t a = 0; a < d; ++a) q += phi[g * d + a] * phi[g * d + a];
+                if (!std::isfinite(q)) throw std::overflow_error("non-finite AO squared norm");
+                if (q <= screening) continue;
+                const int64_t id = start + g;
+                screened.xyz.insert(screened.xyz.end(), kept.xyz.begin() + 3 * id, kept.xyz.begin() + 3 * id + 3);
+                screened.w.push_back(kept.w[id]);
+                screened.importance.push_back(kept.importance[id]);
+                screened.original.push_back(kept.original[id]);
+                ao_norms.push_back(q);
+            }
+        }
+        kept = std::move(screened);
+        n = (int64_t)kept.original.size();
+    }
     if (max_rank > std::min(n, physical_rank))
         throw std::invalid_argument("requested anchor_rank exceeds eligible candidates or physical pair rank");
+    if (n == 0) return res;
     std::vector<double> u((size_t)(d * ell)), v((size_t)(d * ell));
     omega(d, ell, seed, u.data(), v.data());
     std::vector<double> y((size_t)(ell * n));
     for (int64_t start = 0; start < n; start += std::min(batch_size, n - start)) {
         const int64_t count = std::min(batch_size, n - start);
         int64_t w = 0;
-        const auto phi = features(t, kept.xyz.data() + 3 * start, count, scales, w);
+        auto phi = features(t, kept.xyz.data() + 3 * start, count, scales, w);
         std::vector<double> sw(count);
-        for (int64_t g = 0; g < count; ++g) sw[g] = std::sqrt(std::abs(kept.w[start + g]));
+        for (int64_t g = 0; g < count; ++g) {
+            sw[g] = std::sqrt(density ? kept.importance[start + g] : std::abs(kept.w[start + g]));
+            if (normalized) {
+                const double inv = 1 / std::sqrt(ao_norms[start + g]);
+                for (int64_t a = 0; a < d; ++a) phi[g * d + a] *= inv;
+            }
+        }
         sketch(phi.data(), count, d, u.data(), v.data(), ell, sw.data(), y.data() + start * ell);
     }
     for (double x : y) if (!std::isfinite(x)) throw std::overflow_error("non-finite randomized ID sketch");
@@ -552,14 +604,16 @@ int oracle_sketch(const double* phi, int64_t count, int64_t d, const double* u, 
     return guarded([&] { sketch(phi, count, d, u, v, ell, sw, y); });
 }
 
-// algorithm: 0 dense QRCP, 1 online QRCP, 2 randomized quadrature KR sketch
-// (the first three LocalAnchorSelectionAlgorithm values, enums.hpp:23-30).
+// algorithm: 0 dense QRCP, 1 online QRCP, 2 randomized quadrature KR sketch,
+// 4 / 5 AO-density KR sketches, raw / normalized (LocalAnchorSelectionAlgorithm
+// values, enums.hpp:23-30); importance[n] is read by 4 and 5 only.
 int oracle_select(const int32_t* atm, int natm, const int32_t* bas, int nbas,
                   const double* env, int64_t nenv, int cart,
                   const double* points, const double* weights, int64_t n,
                   const int32_t* axes, int naxes, const double* mo_coeff, int64_t nocc,
                   const double* axis_weights, double tolerance, int64_t max_rank, int algorithm,
                   int64_t batch_size, int64_t sketch_size, int64_t oversampling, uint64_t seed,
+                  const double* importance, double screening,
                   int64_t* out, int64_t capacity, int64_t* count, int64_t* stats)
 {
     return guarded([&] {
@@ -578,9 +632,10 @@ int oracle_select(const int32_t* atm, int natm, const int32_t* bas, int nbas,
         std::vector<int64_t> sel;
         if (algorithm == 0) sel = select_dense(t, points, weights, n, tolerance, max_rank, scales);
         else if (algorithm == 1) sel = select_online(t, points, weights, n, tolerance, max_rank, batch_size, scales, stats);
-        else if (algorithm == 2) sel = select_rand(t, points, weights, n, tolerance, max_rank, batch_size,
-                                                   sketch_size, oversampling, seed, scales).selected;
-        else throw std::invalid_argument("the oracle restates only qrcp_dense, qrcp_online and rand_sketch");
+        else if (algorithm == 2 || algorithm == 4 || algorithm == 5)
+            sel = select_rand(t, points, weights, n, tolerance, max_rank, batch_size, sketch_size, oversampling, seed,
+                              scales, algorithm == 2 ? 0 : algorithm - 3, importance, screening).selected;
+        else throw std::invalid_argument("the oracle restates qrcp_dense, qrcp_online and the randomized sketches");
         if ((int64_t)sel.size() > capacity) throw std::invalid_argument("output capacity too small");
         std::copy(sel.begin(), sel.end(), out);
         *count = (int64_t)sel.size();
