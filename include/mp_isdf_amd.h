@@ -74,8 +74,8 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
  * cpp/src/anchor_select.cpp:79-173).  points[n*3], weights[n] on the host.
  * algorithm: 0 QRColumnPivoting (qrcp_dense), 1 OnlineQRColumnPivoting
  * (qrcp_online), 2 RandomizedQuadratureKhatriRao (rand_sketch); 3 (CVT)
- * returns status 1; 4 and 5 (density sketches) need
- * mpisdf_select_importance.
+ * returns status 1 (see mpisdf_select_cvt); 4 and 5 (density sketches)
+ * need mpisdf_select_importance.
  * precision (rand_sketch only): 0 fp64, 1 fp32, 2 bf16, 3 fp8 e4m3.
  * axis_weights: NULL or naxis_weights == naxes finite positive factors.
  * Writes original grid indices in pivot order to out[0..*count).
@@ -102,6 +102,15 @@ int mpisdf_select_importance(const mpisdf_context* ctx, const double* points, co
                              int64_t sketch_size, int64_t oversampling, uint64_t seed, int32_t precision,
                              const double* point_importance, double ao_screening_threshold,
                              int64_t* out, int64_t capacity, int64_t* count, double* diagnostics);
+
+/* AnchorSelection.select with algorithm=CentroidalVoronoi
+ * (cpp/src/anchor_select_pybind.cpp:162-210 -> cpp/src/anchor_select_cvt.cpp:172-204):
+ * points[n*3], weights[n], cvt_weights[n] on the host (finite; cvt_weights
+ * nonnegative; a point's mass is cvt_weights where weights != 0).  Writes the
+ * rank selected original indices, sorted and distinct, to out. */
+int mpisdf_select_cvt(const double* points, const double* weights, const double* cvt_weights, int64_t n,
+                      int64_t rank, uint64_t seed, int64_t max_iter, double tol, int64_t* out, int64_t capacity,
+                      int64_t* count);
 
 /* ---- Khatri-Rao sketch stages (device pointers) ---------------------------- */
 /* Gaussian factors u, v [d][ell] row-major from std::mt19937_64(seed) and

@@ -4,6 +4,7 @@
 #include "axis.hpp"
 #include "context.hpp"
 #include "coulomb.hpp"
+#include "cvt.hpp"
 #include "kernels.hpp"
 #include "qrcp.hpp"
 #include "util.hpp"
@@ -624,5 +625,19 @@ extern "C" int mpisdf_isdf_jk_device(const double* x, int64_t k, int64_t nao, co
     return guard([&] {
         if (nao > 0 && (!j || !kmat || !dm || (k > 0 && (!x || !z)))) throw std::invalid_argument("null J/K argument");
         mpisdf::isdf_jk(x, k, nao, z, dm, j, kmat, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int mpisdf_select_cvt(const double* points, const double* weights, const double* cvt_weights, int64_t n,
+                                 int64_t rank, uint64_t seed, int64_t max_iter, double tol, int64_t* out,
+                                 int64_t capacity, int64_t* count)
+{
+    return guard([&] {
+        if (n < 0 || !count || (n > 0 && (!points || !weights || !cvt_weights)))
+            throw std::invalid_argument("CVT points, quadrature weights and CVT weights must have equal lengths");
+        const auto sel = mpisdf::select_cvt(points, weights, cvt_weights, n, rank, seed, max_iter, tol, nullptr);
+        if ((int64_t)sel.size() > capacity) throw std::invalid_argument("output capacity is too small");
+        std::copy(sel.begin(), sel.end(), out);
+        *count = (int64_t)sel.size();
     });
 }

@@ -171,16 +171,32 @@ public:
                                 py::object point_importance, double ao_screening_threshold,
                                 const std::string& precision, double* diagnostics = nullptr) const
     {
-        (void)cvt_max_iter; (void)cvt_tol;
         parse_space(space);
         const double* xyz = read_points(points);
         if (weights.ndim() != 1 || weights.shape(0) != points.shape(0))
             throw py::value_error("expected weights[ngrid]");
         const auto codes = read_axes(axes);
         const auto coeff = read_coefficients(ctx_, mo_coeff);
-        if (!cvt_weights.is_none() || algorithm == LocalAlgorithm::CentroidalVoronoi)
-            throw py::value_error("CVT selection is not part of this build; use QRColumnPivoting, "
-                                  "OnlineQRColumnPivoting or a randomized Khatri-Rao sketch");
+        if (algorithm == LocalAlgorithm::CentroidalVoronoi) {
+            // anchor_select.cpp:167-169: CVT reads the grid, the quadrature
+            // weights and cvt_weights only.
+            if (cvt_weights.is_none()) throw py::value_error("CVT requires cvt_weights (ngrid,)");
+            const auto cw = cvt_weights.cast<FloatArray>();
+            if (cw.ndim() != 1 || cw.shape(0) != points.shape(0))
+                throw py::value_error("CVT points, quadrature weights and CVT weights must have equal lengths");
+            if (max_rank < 0 || cvt_max_iter < 0) throw py::value_error("sizes must be non-negative");
+            std::vector<int64_t> out(std::max<int64_t>(max_rank, 1));
+            int64_t count = 0;
+            int status;
+            {
+                py::gil_scoped_release release;
+                status = mpisdf_select_cvt(xyz, weights.data(), cw.data(), points.shape(0), max_rank, seed,
+                                           cvt_max_iter, cvt_tol, out.data(), (int64_t)out.size(), &count);
+            }
+            check(status);
+            out.resize(count);
+            return out;
+        }
         // anchor_select_pybind.cpp:186-191: the importance, when given, must
         // match the grid; only the density sketches use it.
         std::vector<double> importance;

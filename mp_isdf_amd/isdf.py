@@ -29,6 +29,7 @@ ALGORITHMS = {
     "qrcp_dense": "QRColumnPivoting",
     "qrcp_online": "OnlineQRColumnPivoting",
     "rand_sketch": "RandomizedQuadratureKhatriRao",
+    "cvt": "CentroidalVoronoi",
 }
 METRIC_ALGORITHMS = ("grid", "unblocked_greedy", "unblocked_uniform")
 # Khatri-Rao sketch techniques of rand_sketch (python/isdf.py:30-36)
@@ -54,7 +55,8 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                selection_only: bool = False, pair_space_error: bool = False, interp_vecs: bool = True,
                compute_metric: bool = False, metric_algorithm: str = "grid", sketch: str = "quadrature",
                point_importance=None, density_weight: str = "electron_density",
-               ao_screening_threshold: float = 1e-24, report_timings: bool = False, debug: bool = False):
+               ao_screening_threshold: float = 1e-24, cvt_weight: str = "electron_density",
+               cvt_max_iter: int = 100, cvt_tol: float = 1e-5, report_timings: bool = False, debug: bool = False):
     """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU.
 
     ``compute_metric=True`` adds the anchor Coulomb metric
@@ -62,6 +64,8 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     as ``result["coulomb_metric"]`` (python/isdf.py:674-689); only the reference's
     ``metric_algorithm="grid"`` is built.
 
+    ``anchor_algorithm="cvt"`` selects centroidal Voronoi sites weighted by
+    ``point_importance`` (or |w| with ``cvt_weight="quadrature_weights"``).
     ``sketch="density" | "density_col_norm"`` (rand_sketch only) selects with
     the AO-density Khatri-Rao sketches weighted by ``point_importance`` (ngrid,
     finite, nonnegative; python/isdf.py:231-235).  Without it only
@@ -93,6 +97,10 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     if sketch not in KHATRIRAO_BACKENDS:
         raise ValueError(f"unknown Khatri-Rao sketch technique: {sketch!r}")
     density_sketch = anchor_algorithm == "rand_sketch" and sketch in DENSITY_SKETCHES
+    if cvt_weight not in ("electron_density", "quadrature_weights"):
+        raise ValueError("cvt_weight must be 'electron_density' or 'quadrature_weights'")
+    if anchor_algorithm == "cvt" and anchor_rank < 1:
+        raise ValueError("CVT requires a positive anchor_rank")
     if anchor_rank and anchor_tol is not None:
         raise ValueError("anchor_rank and anchor_tol are mutually exclusive")
     if not isinstance(fit_batch_size, (int, np.integer)) or fit_batch_size < 1:
@@ -120,11 +128,12 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     record("grid")
 
     importance = None
-    if density_sketch:
+    if density_sketch or anchor_algorithm == "cvt":
+        convention = cvt_weight if anchor_algorithm == "cvt" else density_weight
         if point_importance is None:
-            if density_weight != "quadrature_weights":
+            if convention != "quadrature_weights":
                 raise NotImplementedError("the electron-density importance needs a PySCF RKS density; pass "
-                                          "point_importance or use density_weight='quadrature_weights'")
+                                          "point_importance or use the 'quadrature_weights' convention")
             point_importance = np.abs(all_weights)
         importance = np.asarray(point_importance, dtype=np.float64)
         if importance.shape != (len(all_coords),) or not np.isfinite(importance).all() or np.any(importance < 0):
@@ -141,7 +150,9 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         mo_coeff=mo_coeff, nocc=nocc, batch_size=anchor_batch_size, debug=debug,
         sketch_size=anchor_sketch_size, oversampling=anchor_oversampling, seed=anchor_seed,
         axis_weights=list(axis_weights) if axis_weights is not None else [], precision=precision,
-        point_importance=importance, ao_screening_threshold=ao_screening_threshold), dtype=np.intp)
+        point_importance=importance if density_sketch else None, ao_screening_threshold=ao_screening_threshold,
+        cvt_weights=importance if anchor_algorithm == "cvt" else None, cvt_max_iter=cvt_max_iter,
+        cvt_tol=cvt_tol), dtype=np.intp)
     if len(anchor_indices) == 0:
         raise ValueError("anchor selection found no anchors; adjust settings or grid")
     record("anchor_selection")

@@ -367,3 +367,18 @@ def s_eri(funcs):
                             total += 2 * np.pi ** 2.5 / (p * q * np.sqrt(p + q)) * kp * kq * _boys0(t)
                     eri[u, v, w, x] = total
     return eri
+
+
+def select_cvt(points, weights, cvt_weights, max_rank, seed=7, cvt_max_iter=100, cvt_tol=1e-5):
+    """cpp/src/anchor_select_cvt.cpp:17-204: sorted distinct grid indices."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    imp = np.ascontiguousarray(cvt_weights, dtype=np.float64)
+    if len(w) != len(pts) or len(imp) != len(pts):
+        raise ValueError("CVT points, quadrature weights and CVT weights must have equal lengths")
+    out = np.empty(max(1, int(max_rank)), dtype=np.int64)
+    count = ctypes.c_int64()
+    _check(lib().oracle_select_cvt(_p(pts), _p(w), _p(imp), ctypes.c_int64(len(pts)), ctypes.c_int64(max_rank),
+                                   ctypes.c_uint64(seed), ctypes.c_int64(cvt_max_iter), ctypes.c_double(cvt_tol),
+                                   _p(out), ctypes.byref(count)), selection=True)
+    return out[:count.value].tolist()

@@ -23,6 +23,7 @@
 // external vector: parity unpinned beyond the Gaussian moment checks.
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -643,3 +644,160 @@ int oracle_select(const int32_t* atm, int natm, const int32_t* bas, int nbas,
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// CVT anchors: cpp/src/anchor_select_cvt.cpp:17-204 (host order of operations:
+// mass-weighted k-means++ seeding with long-double prefix sampling, Lloyd
+// iterations with centroid sums in point order, empty clusters reseeded at
+// the largest residual, snapping to distinct sites, sorted indices).
+// ---------------------------------------------------------------------------
+namespace {
+
+std::vector<int64_t> cvt_run(const double* points, const std::vector<int64_t>& active,
+                             const std::vector<double>& mass, int64_t rank, uint64_t seed, int64_t max_iter,
+                             double tol)
+{
+    const int64_t n = (int64_t)active.size();
+    auto P = [&](int64_t i, int d) { return points[3 * active[i] + d]; };
+    std::mt19937_64 rng(seed);
+    auto sample = [&](const std::vector<double>& scores) -> int64_t {
+        long double total = 0;
+        for (double v : scores) total += v;
+        if (!(total > 0)) throw std::invalid_argument("CVT importance has zero total mass");
+        std::uniform_real_distribution<long double> draw(0, total);
+        const long double target = draw(rng);
+        long double prefix = 0;
+        int64_t last = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            if (scores[i] <= 0) continue;
+            prefix += scores[i];
+            last = i;
+            if (target < prefix) return i;
+        }
+        return last;
+    };
+    std::vector<std::array<double, 3>> c(rank);
+    const int64_t first = sample(mass);
+    for (int d = 0; d < 3; ++d) c[0][d] = P(first, d);
+    std::vector<double> nearest(n), scores(n);
+    for (int64_t k = 1; k < rank; ++k) {
+        for (int64_t i = 0; i < n; ++i) {
+            double d2 = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double delta = P(i, d) - c[k - 1][d];
+                d2 += delta * delta;
+            }
+            nearest[i] = k == 1 ? d2 : (d2 < nearest[i] ? d2 : nearest[i]);
+        }
+        long double total = 0;
+        for (int64_t i = 0; i < n; ++i) {
+            scores[i] = mass[i] * nearest[i];
+            if (!std::isfinite(scores[i])) throw std::invalid_argument("CVT weighted distance overflow");
+            total += scores[i];
+        }
+        const int64_t pick = sample(total > 0 ? scores : mass);
+        for (int d = 0; d < 3; ++d) c[k][d] = P(pick, d);
+    }
+    std::array<double, 3> lo, hi;
+    for (int d = 0; d < 3; ++d) lo[d] = hi[d] = P(0, d);
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = std::min(lo[d], P(i, d));
+            hi[d] = std::max(hi[d], P(i, d));
+        }
+    double scale2 = 0;
+    for (int d = 0; d < 3; ++d) scale2 += (hi[d] - lo[d]) * (hi[d] - lo[d]);
+    const double scale = std::max(std::sqrt(scale2), 1.0);
+    std::vector<double> sums(4 * rank), residual(n);
+    for (int64_t iteration = 0; iteration < max_iter; ++iteration) {
+        std::fill(sums.begin(), sums.end(), 0.0);
+        for (int64_t i = 0; i < n; ++i) {
+            double best = 1.7976931348623157e308;
+            int64_t label = 0;
+            for (int64_t k = 0; k < rank; ++k) {
+                double d2 = 0;
+                for (int d = 0; d < 3; ++d) {
+                    const double delta = P(i, d) - c[k][d];
+                    d2 += delta * delta;
+                }
+                if (d2 < best) { best = d2; label = k; }
+            }
+            const double m = mass[i];
+            sums[4 * label] += m;
+            for (int d = 0; d < 3; ++d) sums[4 * label + 1 + d] += m * P(i, d);
+            residual[i] = m * best;
+        }
+        std::vector<std::array<double, 3>> next(rank);
+        std::vector<int64_t> empty;
+        for (int64_t k = 0; k < rank; ++k) {
+            if (sums[4 * k] > 0) {
+                for (int d = 0; d < 3; ++d) next[k][d] = sums[4 * k + 1 + d] / sums[4 * k];
+            } else {
+                empty.push_back(k);
+            }
+        }
+        std::vector<unsigned char> reseeded(n, 0);
+        for (auto k : empty) {
+            int64_t site = n;
+            double best = -1;
+            for (int64_t i = 0; i < n; ++i)
+                if (!reseeded[i] && residual[i] > best) { best = residual[i]; site = i; }
+            reseeded[site] = 1;
+            for (int d = 0; d < 3; ++d) next[k][d] = P(site, d);
+        }
+        double shift2 = 0;
+        for (int64_t k = 0; k < rank; ++k)
+            for (int d = 0; d < 3; ++d) {
+                const double delta = next[k][d] - c[k][d];
+                shift2 += delta * delta;
+            }
+        c.swap(next);
+        if (std::sqrt(shift2) <= tol * scale) break;
+    }
+    std::vector<unsigned char> used(n, 0);
+    std::vector<int64_t> indices;
+    for (const auto& center : c) {
+        int64_t site = n;
+        double best = std::numeric_limits<double>::infinity();
+        for (int64_t i = 0; i < n; ++i) {
+            if (used[i]) continue;
+            double d2 = 0;
+            for (int d = 0; d < 3; ++d) {
+                const double delta = P(i, d) - center[d];
+                d2 += delta * delta;
+            }
+            if (d2 < best || (d2 == best && (site == n || active[i] < active[site]))) { best = d2; site = i; }
+        }
+        used[site] = 1;
+        indices.push_back(active[site]);
+    }
+    std::sort(indices.begin(), indices.end());
+    return indices;
+}
+
+}  // namespace
+
+extern "C" int oracle_select_cvt(const double* points, const double* weights, const double* importance, int64_t npts,
+                                 int64_t rank, uint64_t seed, int64_t max_iter, double tol, int64_t* out,
+                                 int64_t* count)
+{
+    return guarded([&] {
+        if (rank <= 0 || max_iter <= 0 || !std::isfinite(tol) || tol < 0)
+            throw std::invalid_argument("CVT requires positive rank and max_iter, finite nonnegative tol");
+        std::vector<int64_t> active;
+        std::vector<double> mass;
+        for (int64_t i = 0; i < npts; ++i) {
+            for (int d = 0; d < 3; ++d)
+                if (!std::isfinite(points[3 * i + d])) throw std::invalid_argument("CVT coordinates must be finite");
+            if (!std::isfinite(weights[i]) || !std::isfinite(importance[i]) || importance[i] < 0)
+                throw std::invalid_argument("CVT weights must be finite and nonnegative");
+            const double m = weights[i] == 0 ? 0.0 : importance[i];
+            if (m > 0) { active.push_back(i); mass.push_back(m); }
+        }
+        if (rank > (int64_t)active.size())
+            throw std::invalid_argument("CVT requires at least rank positive-importance sites");
+        const auto sel = cvt_run(points, active, mass, rank, seed, max_iter, tol);
+        std::copy(sel.begin(), sel.end(), out);
+        *count = (int64_t)sel.size();
+    });
+}
