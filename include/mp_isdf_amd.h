@@ -112,6 +112,30 @@ int mpisdf_select_cvt(const double* points, const double* weights, const double*
                       int64_t rank, uint64_t seed, int64_t max_iter, double tol, int64_t* out, int64_t capacity,
                       int64_t* count);
 
+/* AO-only robust blockwise random pivoting with the optional interpolative
+ * decomposition: mp_isdf.rbrp_ao_cpu (cpp/src/anchor_select_pybind.cpp:95-115 ->
+ * cpp/src/anchor_select_rbrp.cpp:186-318), with the pair-row products on the
+ * GPU.  ao[n*nao] row-major (nonzero-weight grid rows), weights[n] nonzero.
+ * tau_block NaN selects 1/block_size.  Outputs (capacity >= the rank cap
+ * min(nao(nao+1)/2, n, max_rank or n)): indices (rows, selection order),
+ * interp_vecs[k*n] and transform[k*nao*nao] when compute_id (may be NULL),
+ * history[4 per block] (may be NULL), *nblocks, scalars[3] = relative
+ * residual, condition, retained rank; *status 0 rank_cap, 1 tolerance,
+ * 2 no_candidates, 3 numerical_rank, 4 roundoff_limited. */
+int mpisdf_rbrp_ao(const double* ao, const double* weights, int64_t n, int64_t nao, int64_t batch_size,
+                   int64_t block_size, double tolerance, int64_t max_rank, uint64_t seed, double tau_block,
+                   double rank_rtol, int64_t refresh_every, double svd_rtol, int compute_id, int64_t* indices,
+                   int64_t capacity, int64_t* count, double* interp_vecs, double* transform, double* history,
+                   int64_t* nblocks, double* scalars, int32_t* status);
+/* The same on grid points: mp_isdf.rbrp_ao_device
+ * (cpp/src/anchor_select_pybind.cpp:117-140 -> cpp/src/anchor_select_rbrp_device.cpp:327-491);
+ * AO values come from the context's evaluation kernel. */
+int mpisdf_rbrp_points(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
+                       int64_t batch_size, int64_t block_size, double tolerance, int64_t max_rank, uint64_t seed,
+                       double tau_block, double rank_rtol, int64_t refresh_every, double svd_rtol, int compute_id,
+                       int64_t* indices, int64_t capacity, int64_t* count, double* interp_vecs, double* transform,
+                       double* history, int64_t* nblocks, double* scalars, int32_t* status);
+
 /* ---- Khatri-Rao sketch stages (device pointers) ---------------------------- */
 /* Gaussian factors u, v [d][ell] row-major from std::mt19937_64(seed) and
  * (seed ^ 0x9e3779b97f4a7c15) with std::normal_distribution<double>(0, 1),

@@ -5,6 +5,7 @@
 #include "context.hpp"
 #include "coulomb.hpp"
 #include "cvt.hpp"
+#include "rbrp.hpp"
 #include "kernels.hpp"
 #include "qrcp.hpp"
 #include "util.hpp"
@@ -640,4 +641,101 @@ extern "C" int mpisdf_select_cvt(const double* points, const double* weights, co
         std::copy(sel.begin(), sel.end(), out);
         *count = (int64_t)sel.size();
     });
+}
+
+namespace {
+
+int rbrp_impl(const mpisdf_context* ctx, const double* points, const double* ao_in, const double* weights,
+              int64_t n, int64_t nao_in, int64_t batch_size, int64_t block_size, double tolerance, int64_t max_rank,
+              uint64_t seed, double tau_block, double rank_rtol, int64_t refresh_every, double svd_rtol,
+              int compute_id, int64_t* indices, int64_t capacity, int64_t* count, double* interp_vecs,
+              double* transform, double* history, int64_t* nblocks, double* scalars, int32_t* status)
+{
+    return guard([&] {
+        if (n < 0 || !count || !indices || !nblocks || !scalars || !status || (n > 0 && !weights))
+            throw std::invalid_argument("invalid RBRP arguments");
+        if (batch_size < 0 || block_size < 0 || max_rank < 0 || refresh_every < 0)
+            throw std::invalid_argument("invalid RBRP dimensions, batch size or tolerances");
+        std::vector<double> ao;
+        int64_t nao = nao_in;
+        const double* aop = ao_in;
+        if (ctx) {
+            auto& c = unwrap(ctx);
+            nao = c.nao;
+            for (int64_t g = 0; g < n; ++g) {
+                if (!std::isfinite(weights[g]) || weights[g] == 0)
+                    throw std::invalid_argument("device RBRP requires finite nonzero weights");
+                for (int d = 0; d < 3; ++d)
+                    if (!std::isfinite(points[3 * g + d]))
+                        throw std::invalid_argument("device RBRP requires finite coordinates");
+            }
+            ao.resize((size_t)(n * nao));
+            if (n > 0) {
+                mpisdf::DeviceBuffer<double> dxyz, dao((size_t)(n * nao));
+                dxyz.upload(points, 3 * n, nullptr);
+                mpisdf::launch_ao_eval(c, dxyz.ptr, n, mpisdf::kValues, mpisdf::kF64, dao.ptr, nao, n * nao, nullptr);
+                MPISDF_HIP(hipMemcpy(ao.data(), dao.ptr, ao.size() * sizeof(double), hipMemcpyDeviceToHost));
+            }
+            aop = ao.data();
+        } else if (n > 0 && !ao_in) {
+            throw std::invalid_argument("expected ao[grid,nao] and weights[grid]");
+        }
+        mpisdf::RbrpOptions o;
+        o.batch_size = batch_size;
+        o.block_size = block_size;
+        o.tolerance = tolerance;
+        o.max_rank = max_rank;
+        o.seed = seed;
+        o.tau_block = tau_block;
+        o.rank_rtol = rank_rtol;
+        o.refresh_every = refresh_every;
+        o.svd_rtol = svd_rtol;
+        o.compute_id = compute_id != 0;
+        const auto r = mpisdf::rbrp_ao(aop, nullptr, weights, n, nao, o, nullptr);
+        const int64_t k = (int64_t)r.indices.size();
+        if (k > capacity) throw std::invalid_argument("output capacity is too small");
+        std::copy(r.indices.begin(), r.indices.end(), indices);
+        *count = k;
+        *nblocks = (int64_t)r.history.size() / 4;
+        if (history) std::copy(r.history.begin(), r.history.end(), history);
+        if (compute_id && interp_vecs) std::copy(r.interp_vecs.begin(), r.interp_vecs.end(), interp_vecs);
+        if (compute_id && transform) std::copy(r.transform.begin(), r.transform.end(), transform);
+        scalars[0] = r.relative_residual;
+        scalars[1] = r.condition;
+        scalars[2] = (double)r.retained_rank;
+        static const char* names[] = {"rank_cap", "tolerance", "no_candidates", "numerical_rank", "roundoff_limited"};
+        *status = 0;
+        for (int32_t i = 0; i < 5; ++i)
+            if (r.status == names[i]) *status = i;
+    });
+}
+
+}  // namespace
+
+extern "C" int mpisdf_rbrp_ao(const double* ao, const double* weights, int64_t n, int64_t nao, int64_t batch_size,
+                              int64_t block_size, double tolerance, int64_t max_rank, uint64_t seed,
+                              double tau_block, double rank_rtol, int64_t refresh_every, double svd_rtol,
+                              int compute_id, int64_t* indices, int64_t capacity, int64_t* count,
+                              double* interp_vecs, double* transform, double* history, int64_t* nblocks,
+                              double* scalars, int32_t* status)
+{
+    return rbrp_impl(nullptr, nullptr, ao, weights, n, nao, batch_size, block_size, tolerance, max_rank, seed,
+                     tau_block, rank_rtol, refresh_every, svd_rtol, compute_id, indices, capacity, count, interp_vecs,
+                     transform, history, nblocks, scalars, status);
+}
+
+extern "C" int mpisdf_rbrp_points(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
+                                  int64_t batch_size, int64_t block_size, double tolerance, int64_t max_rank,
+                                  uint64_t seed, double tau_block, double rank_rtol, int64_t refresh_every,
+                                  double svd_rtol, int compute_id, int64_t* indices, int64_t capacity,
+                                  int64_t* count, double* interp_vecs, double* transform, double* history,
+                                  int64_t* nblocks, double* scalars, int32_t* status)
+{
+    if (!ctx) {
+        mpisdf::set_last_error("null mp_isdf context");
+        return mpisdf::kInvalid;
+    }
+    return rbrp_impl(ctx, points, nullptr, weights, n, 0, batch_size, block_size, tolerance, max_rank, seed,
+                     tau_block, rank_rtol, refresh_every, svd_rtol, compute_id, indices, capacity, count, interp_vecs,
+                     transform, history, nblocks, scalars, status);
 }

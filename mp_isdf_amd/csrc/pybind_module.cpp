@@ -12,7 +12,9 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <algorithm>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -138,6 +140,7 @@ const double* read_points(const FloatArray& points)
 class Selector {
 public:
     explicit Selector(const Context& ctx) : ctx_(ctx) {}
+    const Context& context() const { return ctx_; }
 
     py::array_t<double> evaluate_axis(const FloatArray& points, py::object axes, py::object mo_coeff,
                                       int64_t nocc, const std::string& space) const
@@ -455,6 +458,84 @@ PYBIND11_MODULE(mp_isdf, m)
         return py::make_tuple(j, kmat);
     }, py::arg("x"), py::arg("metric"), py::arg("dm"),
        "ISDF J and K of a symmetric AO density from anchor AO values X and the anchor Coulomb metric.");
+
+    // RBRP (cpp/src/anchor_select_pybind.cpp:68-140): the reference's result dict.
+    auto rbrp_call = [](const Context* ctx, const FloatArray& data, const FloatArray& weights, int64_t nao,
+                        int64_t batch_size, int64_t block_size, double tolerance, int64_t max_rank, uint64_t seed,
+                        py::object tau_block, double rank_rtol, int64_t refresh_every, double svd_rtol,
+                        bool compute_id) {
+        const py::ssize_t n = data.shape(0);
+        const double tb = tau_block.is_none() ? std::numeric_limits<double>::quiet_NaN() : tau_block.cast<double>();
+        const int64_t p = nao * (nao + 1) / 2;
+        const int64_t cap = std::max<int64_t>(1, std::min<int64_t>({p, (int64_t)n, max_rank ? max_rank : (int64_t)n}));
+        std::vector<int64_t> idx(cap);
+        std::vector<double> theta(compute_id ? (size_t)cap * n : 0), transform(compute_id ? (size_t)cap * nao * nao : 0),
+            history((size_t)cap * 4), scalars(3);
+        int64_t count = 0, nblocks = 0;
+        int32_t status_code = 0;
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = ctx ? mpisdf_rbrp_points(ctx->get(), data.data(), weights.data(), n, batch_size, block_size,
+                                               tolerance, max_rank, seed, tb, rank_rtol, refresh_every, svd_rtol,
+                                               compute_id ? 1 : 0, idx.data(), cap, &count,
+                                               compute_id ? theta.data() : nullptr,
+                                               compute_id ? transform.data() : nullptr, history.data(), &nblocks,
+                                               scalars.data(), &status_code)
+                         : mpisdf_rbrp_ao(data.data(), weights.data(), n, nao, batch_size, block_size, tolerance,
+                                          max_rank, seed, tb, rank_rtol, refresh_every, svd_rtol, compute_id ? 1 : 0,
+                                          idx.data(), cap, &count, compute_id ? theta.data() : nullptr,
+                                          compute_id ? transform.data() : nullptr, history.data(), &nblocks,
+                                          scalars.data(), &status_code);
+        }
+        check(status);
+        static const char* names[] = {"rank_cap", "tolerance", "no_candidates", "numerical_rank", "roundoff_limited"};
+        py::dict out;
+        idx.resize(count);
+        out["indices"] = py::cast(idx);
+        out["relative_residual"] = scalars[0];
+        out["status"] = std::string(names[status_code]);
+        out["condition"] = scalars[1];
+        out["retained_rank"] = (int64_t)scalars[2];
+        py::array_t<double> hist({(py::ssize_t)nblocks, (py::ssize_t)4});
+        std::copy(history.begin(), history.begin() + 4 * nblocks, hist.mutable_data());
+        out["history"] = hist;
+        if (compute_id) {
+            py::array_t<double> th({(py::ssize_t)count, n}), tr({(py::ssize_t)count, (py::ssize_t)(nao * nao)});
+            std::copy(theta.begin(), theta.begin() + (size_t)count * n, th.mutable_data());
+            std::copy(transform.begin(), transform.begin() + (size_t)count * nao * nao, tr.mutable_data());
+            out["interp_vecs"] = th;
+            out["transform"] = tr;
+        }
+        return out;
+    };
+    m.def("rbrp_ao", [rbrp_call](const FloatArray& ao, const FloatArray& weights, int64_t batch_size,
+                                 int64_t block_size, double tolerance, int64_t max_rank, uint64_t seed,
+                                 py::object tau_block, double rank_rtol, int64_t refresh_every, double svd_rtol,
+                                 bool compute_id) {
+        if (ao.ndim() != 2 || weights.ndim() != 1 || weights.shape(0) != ao.shape(0))
+            throw py::value_error("expected ao[grid,nao] and weights[grid]");
+        return rbrp_call(nullptr, ao, weights, ao.shape(1), batch_size, block_size, tolerance, max_rank, seed,
+                         tau_block, rank_rtol, refresh_every, svd_rtol, compute_id);
+    }, py::arg("ao"), py::arg("weights"), py::arg("batch_size") = 128, py::arg("block_size") = 16,
+       py::arg("tolerance") = 1e-10, py::arg("max_rank") = 0, py::arg("seed") = 7, py::arg("tau_block") = py::none(),
+       py::arg("rank_rtol") = 1e-12, py::arg("refresh_every") = 8, py::arg("svd_rtol") = 1e-12,
+       py::arg("compute_id") = false,
+       "AO-only RBRP on nonzero grid rows with the optional Algorithm 5.1 ID (the reference's rbrp_ao_cpu "
+       "surface; pair-row products on the GPU).");
+    m.def("rbrp_ao_device", [rbrp_call](const Selector& selector, const FloatArray& points, const FloatArray& weights,
+                                        int64_t batch_size, int64_t block_size, double tolerance, int64_t max_rank,
+                                        uint64_t seed, py::object tau_block, double rank_rtol, int64_t refresh_every,
+                                        double svd_rtol, bool compute_id) {
+        if (points.ndim() != 2 || points.shape(1) != 3) throw py::value_error("expected points[ngrid,3]");
+        if (weights.ndim() != 1 || weights.shape(0) != points.shape(0)) throw py::value_error("expected weights[ngrid]");
+        return rbrp_call(&selector.context(), points, weights, selector.context().nao(), batch_size, block_size,
+                         tolerance, max_rank, seed, tau_block, rank_rtol, refresh_every, svd_rtol, compute_id);
+    }, py::arg("selector"), py::arg("points"), py::arg("weights"), py::arg("batch_size") = 128,
+       py::arg("block_size") = 16, py::arg("tolerance") = 1e-10, py::arg("max_rank") = 0, py::arg("seed") = 7,
+       py::arg("tau_block") = py::none(), py::arg("rank_rtol") = 1e-12, py::arg("refresh_every") = 8,
+       py::arg("svd_rtol") = 1e-12, py::arg("compute_id") = false,
+       "AO-only RBRP on grid points (AOs from the GPU kernel) with the optional ID.");
 
     m.def("ao_adjoint", [](const Context& ctx, const FloatArray& coords, const FloatArray& abar) {
         if (coords.ndim() != 2 || coords.shape(1) != 3) throw py::value_error("expected coords[n,3]");

@@ -382,3 +382,149 @@ def select_cvt(points, weights, cvt_weights, max_rank, seed=7, cvt_max_iter=100,
                                    ctypes.c_uint64(seed), ctypes.c_int64(cvt_max_iter), ctypes.c_double(cvt_tol),
                                    _p(out), ctypes.byref(count)), selection=True)
     return out[:count.value].tolist()
+
+
+def rbrp_ao(ao, weights, batch_size=128, block_size=16, tolerance=1e-10, max_rank=0, seed=7, tau_block=None,
+            rank_rtol=1e-12, refresh_every=8, svd_rtol=1e-12, compute_id=False):
+    """cpp/src/anchor_select_rbrp.cpp:186-318 (RBRP) and :135-184 (the ID),
+    with LAPACK's dgeqp3/dorgqr (scipy.linalg.qr) and gesdd (numpy.linalg.svd)
+    as the reference calls them, and libstdc++'s discrete_distribution."""
+    import scipy.linalg
+    ao = np.ascontiguousarray(ao, dtype=np.float64)
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    n, nao = ao.shape
+    if (not n or not nao or batch_size <= 0 or block_size <= 0 or not np.isfinite(tolerance) or tolerance < 0
+            or not np.isfinite(rank_rtol) or rank_rtol <= 0 or not np.isfinite(svd_rtol) or svd_rtol < 0
+            or (tau_block is not None and (not np.isfinite(tau_block) or not 0 <= tau_block < 1))):
+        raise ValueError("invalid RBRP dimensions, batch size or tolerances")
+    if not np.all(np.isfinite(w)) or np.any(w == 0):
+        raise ValueError("RBRP requires finite nonzero weights")
+    if not np.all(np.isfinite(ao)):
+        raise ValueError("RBRP requires finite real AOs")
+    p = nao * (nao + 1) // 2
+    cap = min(p, n, max_rank if max_rank else min(n, p))
+    tau = (1.0 / block_size if block_size > 1 else 0.0) if tau_block is None else tau_block
+    sw = np.sqrt(np.abs(w))
+    iu, ju = np.triu_indices(nao)
+    fac = np.where(iu == ju, 1.0, np.sqrt(2.0))
+    rows = sw[:, None] * ao[:, iu] * ao[:, ju] * fac
+    d = np.abs(w) * np.sum(ao * ao, axis=1) ** 2
+    total = float(d.sum())
+    target = tolerance * tolerance * total
+    q = np.zeros((0, p))
+    l = np.zeros((n, 0))
+    chosen = np.zeros(n, dtype=bool)
+    indices, history = [], []
+    status = "rank_cap"
+    lib().oracle_rng_create.restype = ctypes.c_void_p
+    lib().oracle_rng_discrete.restype = ctypes.c_int64
+    rng = lib().oracle_rng_create(ctypes.c_uint64(seed))
+
+    def residual():
+        e = rows - l @ q
+        r = np.sum(e * e, axis=1)
+        r[chosen] = 0
+        return r
+    try:
+        iteration = 0
+        while True:
+            old = len(indices)
+            if old and (d.sum() <= target or (refresh_every and iteration % refresh_every == 0)):
+                d = residual()
+            if d.sum() <= target:
+                status = "tolerance"
+                break
+            if old >= cap:
+                break
+            active = [i for i in range(n) if d[i] > 0 and not chosen[i]]
+            if not active:
+                status = "no_candidates"
+                break
+            b = min(block_size, cap - old, len(active))
+            sample = []
+            for _ in range(b):
+                wl = np.ascontiguousarray(d[active])
+                pos = lib().oracle_rng_discrete(ctypes.c_void_p(rng), _p(wl), ctypes.c_int64(len(wl)))
+                sample.append(active.pop(pos))
+            v = rows[sample].T.copy()
+            original2 = float(np.sum(v * v))
+            for _ in range(2):
+                for j in range(b):
+                    for t in range(old):
+                        v[:, j] -= (v[:, j] @ q[t]) * q[t]
+            qf, rf, piv = scipy.linalg.qr(v, mode="economic", pivoting=True)
+            tails = np.zeros(b)
+            for r in range(b - 1, -1, -1):
+                tails[r] = (tails[r + 1] if r + 1 < b else 0.0) + float(np.sum(rf[r, r:] ** 2))
+            if not (tails[0] > 0) or not np.isfinite(tails[0]):
+                status = "numerical_rank"
+                break
+            keep = 0
+            for i in range(b):
+                if tails[i] >= tau * tails[0]:
+                    keep = i + 1
+            cutoff = rank_rtol * np.sqrt(original2)
+            for i in range(keep):
+                if abs(rf[i, i]) <= cutoff:
+                    keep = i
+                    break
+            if not keep:
+                status = "numerical_rank"
+                break
+            qb = qf[:, :keep].T.copy()
+            for j in range(keep):
+                col = qb[j]
+                for _ in range(2):
+                    for t in range(old):
+                        col -= (q[t] @ col) * q[t]
+                    for t in range(j):
+                        col -= (qb[t] @ col) * qb[t]
+                norm = np.linalg.norm(col)
+                if not (norm > 0) or not np.isfinite(norm):
+                    raise RuntimeError("RBRP block lost orthogonality")
+                qb[j] = col / norm
+            dots = rows @ qb.T
+            dots[chosen] = 0
+            d = np.where(chosen, 0.0, np.maximum(0.0, d - np.sum(dots * dots, axis=1)))
+            l = np.hstack([l, dots])
+            q = np.vstack([q, qb])
+            for j in range(keep):
+                idx = sample[piv[j]]
+                indices.append(idx)
+                chosen[idx] = True
+                d[idx] = 0
+            history.append([len(indices), b, keep, np.sqrt(d.sum() / total) if total else 0.0])
+            iteration += 1
+    finally:
+        lib().oracle_rng_destroy(ctypes.c_void_p(rng))
+    d = residual()
+    relative = float(np.sqrt(d.sum() / total)) if total else 0.0
+    if relative <= tolerance:
+        status = "tolerance"
+    elif status == "tolerance":
+        status = "roundoff_limited"
+    out = {"indices": indices, "relative_residual": relative, "status": status, "condition": 0.0,
+           "retained_rank": 0, "history": np.array(history).reshape(-1, 4)}
+    k = len(indices)
+    if not compute_id or not k:
+        return out
+    a = l[indices]
+    u, s, vt = np.linalg.svd(a)
+    out["condition"] = float(s[0] / s[-1]) if s[-1] else float("inf")
+    r = int(np.count_nonzero(s > svd_rtol * s[0]))
+    if not r:
+        raise RuntimeError("RBRP anchor block has zero numerical rank")
+    out["retained_rank"] = r
+    inv = (vt[:r].T / s[:r]) @ u[:, :r].T
+    j = np.asarray(indices)
+    theta = ((l @ inv) * (sw[j] / w[j])[None, :] / sw[:, None]).T
+    theta[:, j] = np.diag(1.0 / w[j])
+    packed = inv.T @ q  # [a, packed]
+    packed = packed * (sw[j] / w[j])[:, None]
+    packed = packed / fac[None, :]
+    transform = np.zeros((k, nao, nao))
+    transform[:, iu, ju] = packed
+    transform[:, ju, iu] = packed
+    out["interp_vecs"] = theta
+    out["transform"] = transform.reshape(k, nao * nao)
+    return out

@@ -801,3 +801,14 @@ extern "C" int oracle_select_cvt(const double* points, const double* weights, co
         *count = (int64_t)sel.size();
     });
 }
+
+// libstdc++ std::discrete_distribution draws on std::mt19937_64, the sampler of
+// the reference's RBRP blocks (cpp/src/anchor_select_rbrp.cpp:246-256), for
+// the NumPy restatement in oracle/__init__.py.
+extern "C" void* oracle_rng_create(uint64_t seed) { return new std::mt19937_64(seed); }
+extern "C" void oracle_rng_destroy(void* rng) { delete static_cast<std::mt19937_64*>(rng); }
+extern "C" int64_t oracle_rng_discrete(void* rng, const double* weights, int64_t n)
+{
+    std::discrete_distribution<size_t> dist(weights, weights + n);
+    return (int64_t)dist(*static_cast<std::mt19937_64*>(rng));
+}
