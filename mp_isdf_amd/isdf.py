@@ -30,6 +30,8 @@ ALGORITHMS = {
     "qrcp_online": "OnlineQRColumnPivoting",
     "rand_sketch": "RandomizedQuadratureKhatriRao",
     "cvt": "CentroidalVoronoi",
+    "rbrp": None,      # mp_isdf.rbrp_ao_device (python/isdf.py:564-590)
+    "rbrp_id": None,   # the same, with its interpolative decomposition as the fit
 }
 METRIC_ALGORITHMS = ("grid", "unblocked_greedy", "unblocked_uniform")
 # Khatri-Rao sketch techniques of rand_sketch (python/isdf.py:30-36)
@@ -56,7 +58,9 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                compute_metric: bool = False, metric_algorithm: str = "grid", sketch: str = "quadrature",
                point_importance=None, density_weight: str = "electron_density",
                ao_screening_threshold: float = 1e-24, cvt_weight: str = "electron_density",
-               cvt_max_iter: int = 100, cvt_tol: float = 1e-5, report_timings: bool = False, debug: bool = False):
+               cvt_max_iter: int = 100, cvt_tol: float = 1e-5, rbrp_block_size: int = 16, rbrp_tau_block=None,
+               rbrp_rank_rtol: float = 1e-12, rbrp_refresh_every: int = 8, rbrp_svd_rtol: float = 1e-12,
+               report_timings: bool = False, debug: bool = False):
     """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU.
 
     ``compute_metric=True`` adds the anchor Coulomb metric
@@ -64,6 +68,9 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     as ``result["coulomb_metric"]`` (python/isdf.py:674-689); only the reference's
     ``metric_algorithm="grid"`` is built.
 
+    ``anchor_algorithm="rbrp"`` selects by robust blockwise random pivoting on
+    the kept grid points; ``"rbrp_id"`` also takes RBRP's interpolative
+    decomposition as the interpolation vectors instead of the refit.
     ``anchor_algorithm="cvt"`` selects centroidal Voronoi sites weighted by
     ``point_importance`` (or |w| with ``cvt_weight="quadrature_weights"``).
     ``sketch="density" | "density_col_norm"`` (rand_sketch only) selects with
@@ -143,16 +150,26 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     context = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env, cart=mol.cart)
     selector = mp_isdf.AnchorSelection(context)
     record("context")
-    algorithm = getattr(mp_isdf.LocalAnchorSelectionAlgorithm,
-                        KHATRIRAO_BACKENDS[sketch] if anchor_algorithm == "rand_sketch" else ALGORITHMS[anchor_algorithm])
-    anchor_indices = np.asarray(selector.select(
-        all_coords, all_weights, axis_types, tolerance=tolerance, max_rank=anchor_rank, algorithm=algorithm,
-        mo_coeff=mo_coeff, nocc=nocc, batch_size=anchor_batch_size, debug=debug,
-        sketch_size=anchor_sketch_size, oversampling=anchor_oversampling, seed=anchor_seed,
-        axis_weights=list(axis_weights) if axis_weights is not None else [], precision=precision,
-        point_importance=importance if density_sketch else None, ao_screening_threshold=ao_screening_threshold,
-        cvt_weights=importance if anchor_algorithm == "cvt" else None, cvt_max_iter=cvt_max_iter,
-        cvt_tol=cvt_tol), dtype=np.intp)
+    rbrp_result = None
+    if anchor_algorithm in ("rbrp", "rbrp_id"):
+        rbrp_result = mp_isdf.rbrp_ao_device(
+            selector, all_coords[keep], all_weights[keep], batch_size=anchor_batch_size, block_size=rbrp_block_size,
+            tolerance=tolerance, max_rank=anchor_rank, seed=anchor_seed, tau_block=rbrp_tau_block,
+            rank_rtol=rbrp_rank_rtol, refresh_every=rbrp_refresh_every, svd_rtol=rbrp_svd_rtol,
+            compute_id=anchor_algorithm == "rbrp_id" and not selection_only)
+        anchor_indices = np.flatnonzero(keep)[np.asarray(rbrp_result["indices"], dtype=np.intp)]
+    else:
+        algorithm = getattr(mp_isdf.LocalAnchorSelectionAlgorithm,
+                            KHATRIRAO_BACKENDS[sketch] if anchor_algorithm == "rand_sketch"
+                            else ALGORITHMS[anchor_algorithm])
+        anchor_indices = np.asarray(selector.select(
+            all_coords, all_weights, axis_types, tolerance=tolerance, max_rank=anchor_rank, algorithm=algorithm,
+            mo_coeff=mo_coeff, nocc=nocc, batch_size=anchor_batch_size, debug=debug,
+            sketch_size=anchor_sketch_size, oversampling=anchor_oversampling, seed=anchor_seed,
+            axis_weights=list(axis_weights) if axis_weights is not None else [], precision=precision,
+            point_importance=importance if density_sketch else None, ao_screening_threshold=ao_screening_threshold,
+            cvt_weights=importance if anchor_algorithm == "cvt" else None, cvt_max_iter=cvt_max_iter,
+            cvt_tol=cvt_tol), dtype=np.intp)
     if len(anchor_indices) == 0:
         raise ValueError("anchor selection found no anchors; adjust settings or grid")
     record("anchor_selection")
@@ -183,8 +200,21 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     anchor_ao = context.eval_ao(anchor_coords)
     result["X1"] = anchor_ao
     record("ao_evaluation")
-    fit = mp_isdf.isdf_fit(context, anchor_coords, anchor_weights, coords, weights, batch_size=fit_batch_size,
-                           interp_vecs=interp_vecs, weight_sums=True)
+    if anchor_algorithm == "rbrp_id":
+        # RBRP's ID is the fit (python/isdf.py:606-620); its relative residual
+        # is the pair-space error of that ID.
+        theta = rbrp_result["interp_vecs"]
+        fit = {"interp_vecs": theta, "theta_weight_sums": theta @ weights,
+               "fit_diagnostics": {"retained_rank": rbrp_result["retained_rank"],
+                                   "condition": rbrp_result["condition"], "status": rbrp_result["status"]}}
+        if pair_space_error:
+            ao_kept = context.eval_ao(coords)
+            norm = float(np.sqrt(np.sum(np.abs(weights) * np.sum(ao_kept * ao_kept, axis=1) ** 2)))
+            fit["pair_space_error"] = {"l2": rbrp_result["relative_residual"] * norm,
+                                       "relative": rbrp_result["relative_residual"]}
+    else:
+        fit = mp_isdf.isdf_fit(context, anchor_coords, anchor_weights, coords, weights, batch_size=fit_batch_size,
+                               interp_vecs=interp_vecs, weight_sums=True)
     result["interp_vecs"] = fit["interp_vecs"]
     result["fit_diagnostics"] = fit["fit_diagnostics"]
     record("isdf_fit")

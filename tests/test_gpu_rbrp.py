@@ -44,3 +44,28 @@ def test_device_points_match_oracle(mp_isdf):
     assert np.abs(got["interp_vecs"] - want["interp_vecs"]).max() < 1e-8 * scale
     with pytest.raises(ValueError, match="nonzero"):
         mp_isdf.rbrp_ao_device(sel, xyz, np.where(np.arange(len(w)) == 3, 0.0, w), max_rank=4)
+
+
+def test_driver_rbrp_modes(mp_isdf):
+    """build_isdf(anchor_algorithm="rbrp" | "rbrp_id") (python/isdf.py:564-620):
+    rbrp selects with rbrp_ao_device on the kept points and refits; rbrp_id
+    keeps RBRP's ID, whose pair-space error is its relative residual and is
+    no smaller than the least-squares refit's at the same anchors."""
+    from mp_isdf_amd.isdf import build_isdf
+    mol = molecule.single_water("sto-3g")
+    xyz, w = gridmod.make_grid(mol, 1)
+    keep = w != 0
+    sel = mp_isdf.AnchorSelection(mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env))
+    direct = mp_isdf.rbrp_ao_device(sel, xyz[keep], w[keep], max_rank=20, seed=7)
+    res = build_isdf(mol, level=1, anchor_algorithm="rbrp", anchor_rank=20, pair_space_error=True)
+    np.testing.assert_array_equal(res["anchor_indices"], np.flatnonzero(keep)[direct["indices"]])
+    rid = build_isdf(mol, level=1, anchor_algorithm="rbrp_id", anchor_rank=20, pair_space_error=True,
+                     compute_metric=True)
+    np.testing.assert_array_equal(rid["anchor_indices"], res["anchor_indices"])
+    e_id = rid["pair_space_errors"]["scalar"]["relative"]
+    e_fit = res["pair_space_errors"]["scalar"]["relative"]
+    assert e_fit <= e_id * (1 + 1e-9)
+    theta = rid["interp_vecs"]
+    assert theta.shape == (20, int(keep.sum()))
+    assert rid["coulomb_metric"].shape == (20, 20)
+    print(f"rbrp (H2O/STO-3G, 20 anchors): ID pair error {e_id:.4e}, refit {e_fit:.4e}")
