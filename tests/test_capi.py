@@ -29,8 +29,11 @@ def test_library_exports_every_declared_symbol():
 def test_module_surface_matches_reference():
     m = mp_isdf_amd.mp_isdf
     for name in ("LibcintContext", "AnchorSelection", "DistributedAnchorSelection", "eval_ao",
-                 "eval_ao_gradient", "cuda_enabled", "device_enabled"):
+                 "eval_ao_gradient", "cuda_enabled", "device_enabled", "rbrp_ao_device",
+                 # the driver's GPU counterparts (python/isdf.py:48-151) and the rbrp_ao_cpu surface
+                 "grid_coulomb_metric", "grid_cross_metric", "isdf_jk", "rbrp_ao", "isdf_fit"):
         assert hasattr(m, name), name
+    assert hasattr(m.LibcintContext, "overlap")
     assert [v for v in m.AxisType.__members__] == [
         "AtomicOrbital", "AtomicOrbitalGradX", "AtomicOrbitalGradY", "AtomicOrbitalGradZ", "OccupiedOrbital"]
     assert list(m.LocalAnchorSelectionAlgorithm.__members__) == [
