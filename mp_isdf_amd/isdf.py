@@ -60,7 +60,7 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                ao_screening_threshold: float = 1e-24, cvt_weight: str = "electron_density",
                cvt_max_iter: int = 100, cvt_tol: float = 1e-5, rbrp_block_size: int = 16, rbrp_tau_block=None,
                rbrp_rank_rtol: float = 1e-12, rbrp_refresh_every: int = 8, rbrp_svd_rtol: float = 1e-12,
-               report_timings: bool = False, debug: bool = False):
+               space: str = "auto", report_timings: bool = False, debug: bool = False):
     """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU.
 
     ``compute_metric=True`` adds the anchor Coulomb metric
@@ -93,6 +93,10 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         if report_timings and not debug:
             print(f"  {stage:18s} {timings[stage]:10.6f} s  (elapsed {last - started:.6f} s)", flush=True)
 
+    if space not in ("auto", "device", "host"):
+        raise ValueError("space must be 'auto', 'host', or 'device'")
+    if space == "host":
+        raise ValueError("this build runs on the GPU only; space='host' is not available (use 'auto' or 'device')")
     if isinstance(axes, str):
         axes = (axes,)
     try:

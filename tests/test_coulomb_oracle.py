@@ -158,3 +158,15 @@ def test_driver_metric_argument_validation():
         driver.build_isdf(mol, grid=grid, compute_metric=True, metric_algorithm="unblocked_greedy")
     with pytest.raises(ValueError, match="metric"):
         driver.evaluate_isdf_jk({"X1": np.ones((1, 2)), "coulomb_metric": None}, np.eye(2))
+
+
+def test_driver_space_argument():
+    """build_isdf(space=...) takes the reference's values; the host space is
+    not part of this build (python/isdf.py:288)."""
+    from mp_isdf_amd import isdf as driver
+    mol = hydrogen_molecule("sto-3g")
+    grid = (np.zeros((4, 3)), np.ones(4))
+    with pytest.raises(ValueError, match="host"):
+        driver.build_isdf(mol, grid=grid, space="host")
+    with pytest.raises(ValueError, match="space"):
+        driver.build_isdf(mol, grid=grid, space="gpu")
