@@ -342,8 +342,11 @@ RbrpResult rbrp_ao(const double* ao_host, const double* ao_dev, const double* we
         std::vector<double> cols((size_t)(n * keep));
         for (int64_t g = 0; g < n; ++g)
             for (int64_t t = 0; t < keep; ++t) cols[g * keep + t] = l[g * (old + keep) + old + t];
-        MPISDF_HIP(hipMemcpy2DAsync(dl.ptr + old, cap * sizeof(double), cols.data(), keep * sizeof(double),
-                                    keep * sizeof(double), (size_t)n, hipMemcpyHostToDevice, s));
+        // One contiguous upload, then a device-side strided copy (a 2-D copy
+        // straight from pageable host memory goes row by row).
+        MPISDF_HIP(hipMemcpyAsync(dots.ptr, cols.data(), cols.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        MPISDF_HIP(hipMemcpy2DAsync(dl.ptr + old, cap * sizeof(double), dots.ptr, keep * sizeof(double),
+                                    keep * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, s));
         MPISDF_HIP(hipStreamSynchronize(s));
         for (int64_t j = 0; j < keep; ++j) {
             const int64_t id = sample[piv[j]];
