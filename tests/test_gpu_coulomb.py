@@ -10,6 +10,7 @@ import pytest
 
 import oracle
 from mp_isdf_amd import isdf as driver
+from mp_isdf_amd import molecule
 from mp_isdf_amd.molecule import hydrogen_molecule
 
 pytestmark = pytest.mark.gpu
@@ -161,3 +162,22 @@ def test_c1_jk_against_oracle_and_exact_integrals(mp_isdf):
         driver.evaluate_isdf_jk({**result, "coulomb_metric": None}, dm)
     with pytest.raises(ValueError, match="symmetric"):
         mp_isdf.isdf_jk(result["X1"], result["coulomb_metric"], np.array([[1., 1.], [0., 1.]]))
+
+
+def test_water_driver_metric_and_jk_match_oracle(mp_isdf):
+    """Water cc-pVDZ (s, p and d shells), level-2 grid (20,272 points), 60
+    qrcp_dense anchors: the driver's metric over the real interpolation
+    vectors and J/K of a symmetric density equal the oracle's."""
+    mol = molecule.single_water("cc-pvdz")
+    result = driver.build_isdf(mol, level=2, anchor_rank=60, anchor_algorithm="qrcp_dense", compute_metric=True)
+    theta, coords, weights = result["interp_vecs"], result["coordinates"], result["weights"]
+    aw = result["anchor_weights"]
+    want = aw[:, None] * oracle.grid_coulomb_metric(theta, coords, weights) * aw[None, :]
+    close(result["coulomb_metric"], want, 1e-11)
+    rng = np.random.default_rng(3)
+    c = rng.normal(size=(mol.nao, 5))
+    dm = 2 * c @ c.T
+    j, k = driver.evaluate_isdf_jk(result, dm)
+    jo, ko = oracle.evaluate_isdf_jk(result["X1"], result["coulomb_metric"], dm)
+    close(j, jo, 1e-12)
+    close(k, ko, 1e-12)
