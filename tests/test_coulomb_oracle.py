@@ -137,11 +137,23 @@ def test_c1_jk_of_the_oracle_pipeline_is_close_to_exact():
 
 
 @pytest.mark.skipif(module.device_enabled, reason="checks the no-device path")
-def test_metric_and_jk_fail_loudly_without_a_device():
+def test_gpu_entry_points_fail_loudly_without_a_device():
     with pytest.raises(RuntimeError):
         module.grid_coulomb_metric(np.ones((1, 2)), np.zeros((2, 3)), np.ones(2))
     with pytest.raises(RuntimeError):
         module.isdf_jk(np.ones((1, 2)), np.ones((1, 1)), np.eye(2))
+    with pytest.raises(RuntimeError):
+        module.rbrp_ao(np.ones((4, 2)), np.ones(4), max_rank=1)
+    mol = hydrogen_molecule("sto-3g")
+    with pytest.raises(RuntimeError):
+        module.LibcintContext(mol.atm, mol.bas, mol.env)
+    from mp_isdf_amd import capi
+    import ctypes
+    out = np.zeros(2, dtype=np.int64)
+    count = ctypes.c_int64()
+    with pytest.raises(RuntimeError):
+        capi.call("mpisdf_select_cvt", np.zeros((4, 3)), np.ones(4), np.ones(4), 4, 2, 7, 10, 1e-5, out, 2,
+                  ctypes.byref(count))
 
 
 def test_driver_metric_argument_validation():
