@@ -18,9 +18,11 @@
 // (LAPACK dgeqp3) and a brute-force greedy residual check in tests/.
 //
 // Omega uses libstdc++'s std::mt19937_64 and std::normal_distribution<double>
-// exactly as randomized_id.cpp:269-279 does; parity of that stream with the
-// reference build is by construction (same library algorithm), not by an
-// external vector: parity unpinned beyond the Gaussian moment checks.
+// exactly as randomized_id.cpp:269-279 does.  tests/test_omega_pin.py pins it
+// bit for bit against an independent Python restatement of MT19937-64 (itself
+// pinned by the C++ standard's required 10000th draw, 9981545732273789042),
+// of generate_canonical<double, 53> and of Marsaglia's polar method, with the
+// seed mix and the u[a*ell + s] fill order.
 
 #include <algorithm>
 #include <array>
