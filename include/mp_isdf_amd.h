@@ -278,8 +278,10 @@ int mpisdf_isdf_jk_device(const double* x, int64_t k, int64_t nao, const double*
 /* ---- Grid front end (no reference counterpart: the reference takes its grid
  * from PySCF's dft.gen_grid.Grids, python/isdf.py:488-496) ------------------ */
 /* Becke-partitioned atom-centred grid, Treutler radial nodes, product angular
- * rules.  Call with xyz == NULL to get *count; then xyz[3*count], w[count]. */
-int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level,
+ * rules.  Call with xyz == NULL to get *count; then xyz[3*count], w[count].
+ * device != 0 runs the partition on the GPU when one is present; the weights
+ * equal the host loop's bit for bit. */
+int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level, int32_t device,
                       double* xyz, double* weights, int64_t capacity, int64_t* count);
 
 #ifdef __cplusplus

@@ -17,6 +17,7 @@
 //     transform tables live in LDS.
 #include "context.hpp"
 #include "kernels.hpp"
+#include "util.hpp"
 
 #include <algorithm>
 
@@ -111,9 +112,11 @@ __device__ __forceinline__ void eval_shell(const DevShell& sh, const double* __r
 }
 
 // ADJOINT = true turns the kernel into the adjoint of gradient evaluation:
-// nothing is stored; instead gout[c * nao + mu] += sum_p abar[p * ldab + mu]
+// nothing is stored; instead each tile writes its partial
+// gout[tile * NC * nao + c * nao + mu] = sum_{p in tile} abar[p * ldab + mu]
 // d_c phi_mu(r_p) (NC = 3), i.e. the AO gradients are recomputed tile by tile
-// and contracted with the incoming adjoint without ever reaching HBM.
+// and contracted with the incoming adjoint without ever reaching HBM; the
+// launcher adds the tiles' partials in tile order.
 template <class C, class OutT, int NC, bool ADJOINT = false>
 __global__ __launch_bounds__(kThreads) void ao_eval_kernel(
     const DevShell* __restrict__ shells, const double* __restrict__ env,
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(kThreads) void ao_eval_kernel(
                 const double* arow = abar + g0 * ldab + ck.a0 + j;
                 double s = 0;
                 for (int p = 0; p < np; ++p) s += (double)plane[p * width + j] * arow[(int64_t)p * ldab];
-                if (s != 0.0) atomicAdd(&gout[comp * nao + ck.a0 + j], s);
+                gout[(int64_t)blockIdx.x * NC * nao + comp * nao + ck.a0 + j] = s;
             }
             __syncthreads();
             continue;
@@ -259,7 +262,7 @@ void launch_mode(Context& ctx, const double* xyz, int64_t n, AoMode mode, void* 
 }  // namespace
 
 void launch_ao_adjoint(Context& ctx, const double* xyz, int64_t n, const double* abar, int64_t ldab, double* gout,
-                       hipStream_t stream)
+                       DeviceBuffer<double>& part, hipStream_t stream)
 {
     if (n <= 0 || ctx.nao == 0) return;
     if (ldab < ctx.nao) throw std::invalid_argument("AO adjoint leading dimension is smaller than nao");
@@ -272,17 +275,27 @@ void launch_ao_adjoint(Context& ctx, const double* xyz, int64_t n, const double*
     const size_t lds = (size_t)NC * kTile * width * sizeof(double) + (size_t)ntab * sizeof(double);
     for (const auto& ck : table.first)
         if (ck.s1 - ck.s0 > kMaxChunkShells) throw std::logic_error("AO chunk holds too many shells");
-    const int64_t blocks = (n + kTile - 1) / kTile;
-    if (blocks > 0x7fffffff) throw std::overflow_error("AO grid is too large for one launch");
     auto kernel = ao_eval_kernel<double, double, NC, true>;
     if (lds > 64 * 1024)
         MPISDF_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kThreads), lds, stream, ctx.d_shells.ptr, ctx.d_env.ptr,
-                       ctx.d_tabs.ptr, ntab, table.second->ptr, (int)table.first.size(), xyz, n,
-                       static_cast<double*>(nullptr), (int64_t)0, (int64_t)0, width, abar, ldab, gout,
-                       (int64_t)ctx.nao);
-    MPISDF_HIP(hipGetLastError());
+    // Launches of at most 4096 tiles, each tile's partial written to ``part``
+    // and the tiles added in order, so the sum is the same on every run.
+    constexpr int64_t kMaxTiles = 4096;
+    const int64_t per = NC * ctx.nao;
+    const int64_t tiles_total = (n + kTile - 1) / kTile;
+    const int64_t need = std::min(tiles_total, kMaxTiles) * per;
+    if (part.count < (size_t)need) part.resize((size_t)need);
+    for (int64_t start = 0; start < n; start += kMaxTiles * kTile) {
+        const int64_t cnt = std::min(n - start, kMaxTiles * kTile);
+        const int64_t blocks = (cnt + kTile - 1) / kTile;
+        hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kThreads), lds, stream, ctx.d_shells.ptr, ctx.d_env.ptr,
+                           ctx.d_tabs.ptr, ntab, table.second->ptr, (int)table.first.size(), xyz + 3 * start, cnt,
+                           static_cast<double*>(nullptr), (int64_t)0, (int64_t)0, width, abar + start * ldab, ldab,
+                           part.ptr, (int64_t)ctx.nao);
+        MPISDF_HIP(hipGetLastError());
+        launch_sum_partials(part.ptr, blocks, per, gout, true, stream);
+    }
 }
 
 void launch_ao_eval(Context& ctx, const double* xyz, int64_t n, AoMode mode, DType dtype,

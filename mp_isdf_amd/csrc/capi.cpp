@@ -252,16 +252,17 @@ extern "C" int mpisdf_select_importance(const mpisdf_context* ctx, const double*
 }
 
 extern "C" int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level,
-                                 double* xyz, double* weights, int64_t capacity, int64_t* count)
+                                 int32_t device, double* xyz, double* weights, int64_t capacity, int64_t* count)
 {
     return guard([&] {
         if (!charges || !coords || !count) throw std::invalid_argument("null grid argument");
         std::vector<int> z(charges, charges + natm);
-        auto grid = mpisdf::becke_grid(z.data(), coords, natm, level);
-        const int64_t n = (int64_t)grid.second.size();
+        const int64_t n = mpisdf::becke_grid_size(z.data(), natm, level);
         *count = n;
         if (!xyz) return;
+        if (!weights) throw std::invalid_argument("null grid argument");
         if (capacity < n) throw std::invalid_argument("grid output capacity is too small");
+        auto grid = mpisdf::becke_grid(z.data(), coords, natm, level, device != 0);
         std::copy(grid.first.begin(), grid.first.end(), xyz);
         std::copy(grid.second.begin(), grid.second.end(), weights);
     });
@@ -326,8 +327,10 @@ extern "C" int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt
 {
     return guard([&] {
         if (ldy < ell || ldphi < d || !ynorm2 || !bad) throw std::invalid_argument("invalid sketch arguments");
+        mpisdf::DeviceBuffer<double> part;
         mpisdf::kr_sketch_lowp(static_cast<mpisdf::DType>(dtype), ut, vt, ell, d, phi, ldphi, count, scale, y, ldy,
-                               ynorm2, bad, static_cast<hipStream_t>(stream));
+                               ynorm2, bad, part, static_cast<hipStream_t>(stream));
+        MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     });
 }
 
@@ -357,7 +360,10 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
                                             static_cast<hipStream_t>(stream));
             MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
         } else {
-            mpisdf::fold_projections(b, n, m, Qb, m, Y, y_f32 != 0, ldy, proj2, energy, static_cast<hipStream_t>(stream));
+            mpisdf::DeviceBuffer<double> part;
+            mpisdf::fold_projections(b, n, m, Qb, m, Y, y_f32 != 0, ldy, proj2, energy, part,
+                                     static_cast<hipStream_t>(stream));
+            MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
         }
     });
 }
@@ -467,7 +473,8 @@ extern "C" int mpisdf_ao_adjoint_device(const mpisdf_context* ctx, const double*
     return guard([&] {
         auto& c = unwrap(ctx);
         if (n < 0 || !gout || (n > 0 && (!xyz || !abar))) throw std::invalid_argument("invalid AO adjoint arguments");
-        mpisdf::launch_ao_adjoint(c, xyz, n, abar, ldab, gout, static_cast<hipStream_t>(stream));
+        mpisdf::DeviceBuffer<double> part;
+        mpisdf::launch_ao_adjoint(c, xyz, n, abar, ldab, gout, part, static_cast<hipStream_t>(stream));
         MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     });
 }
@@ -518,7 +525,9 @@ extern "C" int mpisdf_ao_adjoint_host(const mpisdf_context* ctx, const double* x
         if (n > 0) {
             dxyz.upload(xyz, 3 * n, nullptr);
             dab.upload(abar, n * c.nao, nullptr);
-            mpisdf::launch_ao_adjoint(c, dxyz.ptr, n, dab.ptr, c.nao, dg.ptr, nullptr);
+            mpisdf::DeviceBuffer<double> part;
+            mpisdf::launch_ao_adjoint(c, dxyz.ptr, n, dab.ptr, c.nao, dg.ptr, part, nullptr);
+            MPISDF_HIP(hipStreamSynchronize(nullptr));
         }
         MPISDF_HIP(hipMemcpy(gout, dg.ptr, 3 * c.nao * sizeof(double), hipMemcpyDeviceToHost));
     });

@@ -15,6 +15,7 @@
 #include "fit.hpp"
 
 #include "kernels.hpp"
+#include "util.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -99,10 +100,11 @@ __global__ __launch_bounds__(256) void potrf_block_kernel(double* __restrict__ a
     }
 }
 
-// err2 accumulation: per grid column g of Z (k x nb, column-major ld k):
-//   qn = (sum_a phi_ga^2)^2 ; e = max(qn - ||z_g||^2, 0) ; acc += |w_g| e, tot += |w_g| qn
+// Per-point pair errors from Z (k x nb, column-major ld k), one wave per point:
+//   qn = (sum_a phi_ga^2)^2 ; out[g] = |w_g| max(qn - ||z_g||^2, 0), out[ld + g] = |w_g| qn
+// (summed over the batch in a fixed order afterwards).
 __global__ void pair_error_kernel(const double* __restrict__ z, int64_t k, const double* __restrict__ phi, int64_t d,
-                                  const double* __restrict__ w, int64_t nb, double* __restrict__ out)
+                                  const double* __restrict__ w, int64_t nb, double* __restrict__ out, int64_t ld)
 {
     const int lane = threadIdx.x & 63;
     const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
@@ -118,15 +120,15 @@ __global__ void pair_error_kernel(const double* __restrict__ z, int64_t k, const
         e = aw * fmax(qn - zz, 0.0);
         t = aw * qn;
     }
-    if (lane == 0 && g < nb) { atomicAdd(&out[0], e); atomicAdd(&out[1], t); }
+    if (lane == 0 && g < nb) { out[g] = e; out[ld + g] = t; }
 }
 
-// The same sums from the row-group partials of ||z_g||^2 (zpart[r * nb + g],
-// added in row-group order): qn = ||phi_g||^4, acc += |w_g| max(qn - zz_g, 0),
-// tot += |w_g| qn; one wave per point.
+// The same per-point terms from the row-group partials of ||z_g||^2
+// (zpart[r * nb + g], added in row-group order): qn = ||phi_g||^4,
+// out[g] = |w_g| max(qn - zz_g, 0), out[ld + g] = |w_g| qn; one wave per point.
 __global__ void pair_error_norms_kernel(const double* __restrict__ zpart, int64_t groups,
                                         const double* __restrict__ phi, int64_t d, const double* __restrict__ w,
-                                        int64_t nb, double* __restrict__ out)
+                                        int64_t nb, double* __restrict__ out, int64_t ld)
 {
     const int lane = threadIdx.x & 63;
     const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
@@ -139,8 +141,8 @@ __global__ void pair_error_norms_kernel(const double* __restrict__ zpart, int64_
         double zz = 0;
         for (int64_t r = 0; r < groups; ++r) zz += zpart[r * nb + g];
         const double qn = pp * pp, aw = fabs(w[g]);
-        atomicAdd(&out[0], aw * fmax(qn - zz, 0.0));
-        atomicAdd(&out[1], aw * qn);
+        out[g] = aw * fmax(qn - zz, 0.0);
+        out[ld + g] = aw * qn;
     }
 }
 
@@ -332,14 +334,15 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     batch = std::max<int64_t>(1, std::min(batch, n));
     DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
     const int64_t zgroups = (k + 127) / 128;  // row groups of the Z product
-    DeviceBuffer<double> zz((size_t)(zgroups * batch));
-    DeviceBuffer<double> th, thr, tsum;
+    DeviceBuffer<double> zz((size_t)(zgroups * batch)), ept((size_t)(2 * batch));
+    DeviceBuffer<double> th, thr, tsum, tpart;
     std::vector<double> stage;
     wa.upload(anchor_w, k, s);
     if (tsum_out || theta_host) th.resize((size_t)(k * batch));
     if (theta_host) thr.resize((size_t)(k * batch));
     if (tsum_out) {
         tsum.resize(k);
+        tpart.resize((size_t)(tn_column_blocks(batch) * k));
         MPISDF_HIP(hipMemsetAsync(tsum.ptr, 0, k * sizeof(double), s));
     }
     MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
@@ -358,9 +361,11 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
                           nullptr, nullptr, s, 1);
         timer.end(s);
         gemm_flops += (double)k * (double)(k + 1) * (double)nb;
-        if (err_out)
+        if (err_out) {
             hipLaunchKernelGGL(pair_error_norms_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, zz.ptr, zgroups,
-                               phi.ptr, d, w + start, nb, acc.ptr);
+                               phi.ptr, d, w + start, nb, ept.ptr, batch);
+            launch_sum_fixed(ept.ptr, nb, batch, 2, acc.ptr, s);
+        }
         if (theta_host || tsum_out) {
             // theta_phys = L^-T Z (upper triangular A), column-major k x nb;
             // theta = theta_phys / w_anchor
@@ -369,10 +374,11 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
             timer.begin(s);
             if (tsum_out)
                 gemm_tn_f64_fused(TnFused::RowDot, k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, th.ptr, k, nullptr,
-                                  tsum.ptr, w + start, s, 2);
+                                  tpart.ptr, w + start, s, 2);
             else
                 gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, th.ptr, k, s, 2);
             timer.end(s);
+            if (tsum_out) launch_sum_partials(tpart.ptr, tn_column_blocks(nb), k, tsum.ptr, true, s);
             gemm_flops += (double)k * (double)(k + 1) * (double)nb;
             if (theta_host) {  // rows of theta[k][n] on the host, through a contiguous staging copy
                 transpose(th.ptr, nb, k, wa.ptr, thr.ptr, s);
@@ -431,7 +437,8 @@ void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xy
     DeviceBuffer<double> phi((size_t)(batch * d)), tm((size_t)(k * batch)), sm((size_t)(k * batch)),
         z((size_t)(k * batch)), cm((size_t)(k * batch)), um((size_t)(k * batch)), vt((size_t)(batch * d)),
         abar((size_t)(std::max(batch, k) * d)), acc(2), gao((size_t)(3 * nao)), m1((size_t)(k * d)),
-        cwc((size_t)(k * k)), gm((size_t)(k * k)), m2((size_t)(k * d)), phiat((size_t)(d * k));
+        cwc((size_t)(k * k)), gm((size_t)(k * k)), m2((size_t)(k * d)), phiat((size_t)(d * k)),
+        ept((size_t)(2 * batch)), adj_part;
     MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
     MPISDF_HIP(hipMemsetAsync(gao.ptr, 0, 3 * nao * sizeof(double), s));
     MPISDF_HIP(hipMemsetAsync(m1.ptr, 0, (size_t)(k * d) * sizeof(double), s));
@@ -447,7 +454,8 @@ void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xy
                            (const double*)nullptr, k, k * nb, sm.ptr);
         gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s, 1);
         hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
-                           wb, nb, acc.ptr);
+                           wb, nb, ept.ptr, batch);
+        launch_sum_fixed(ept.ptr, nb, batch, 2, acc.ptr, s);
         gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, cm.ptr, k, s, 2);
         // Grid adjoint: V = Phi_a^T (C o T) stored [g][mu]; abar_g = 4|w|(||phi||^2 phi - V)
         hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, cm.ptr, tm.ptr,
@@ -455,7 +463,7 @@ void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xy
         gemm_tn_f64(d, nb, k, 1.0, phiat.ptr, k, um.ptr, k, 0.0, vt.ptr, d, s);
         hipLaunchKernelGGL(grid_adjoint_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, phi.ptr, vt.ptr, d,
                            wb, nb, abar.ptr);
-        launch_ao_adjoint(ctx, xyz + 3 * start, nb, abar.ptr, d, gao.ptr, s);
+        launch_ao_adjoint(ctx, xyz + 3 * start, nb, abar.ptr, d, gao.ptr, adj_part, s);
         // Anchor terms: M1 += (U W) Phi_b, CWC^T += (C W) C^T
         hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, um.ptr,
                            (const double*)nullptr, wb, k, k * nb, um.ptr);
@@ -471,7 +479,7 @@ void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xy
     gemm_f64(k, d, k, 1.0, MatView{cwc.ptr, k, 1}, MatView{f.phi_a.ptr, d, 1}, 0.0, MutView{m2.ptr, d, 1}, s);
     hipLaunchKernelGGL(anchor_adjoint_kernel, blocks(k * d), dim3(256), 0, s, m1.ptr, m2.ptr, k * d, abar.ptr);
     MPISDF_HIP(hipGetLastError());
-    launch_ao_adjoint(ctx, anchor_xyz, k, abar.ptr, d, gao.ptr, s);
+    launch_ao_adjoint(ctx, anchor_xyz, k, abar.ptr, d, gao.ptr, adj_part, s);
     const auto h = acc.download(s);
     if (loss_out) *loss_out = h[0];
     if (gao_out) MPISDF_HIP(hipMemcpyAsync(gao_out, gao.ptr, 3 * nao * sizeof(double), hipMemcpyDeviceToHost, s));

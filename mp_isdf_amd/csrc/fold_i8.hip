@@ -27,6 +27,7 @@
 // loads issued two slices ahead; the lane layout and workgroup shapes are
 // described at the kernel.
 #include "kernels.hpp"
+#include "util.hpp"
 
 #include <algorithm>
 
@@ -348,8 +349,10 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         if (c < n) proj2[c] += sum;  // row groups run in order
     }
     if (ENERGY && tid < ROWS) {
-        const int64_t row = m0 + tid;
-        if (row < b) atomicAdd(&energy[row], rowred[0][tid] + rowred[1][tid]);
+        // energy holds this launch's per-column-block partials, [block][rows]
+        // with rows = min(ROWS, b - m0); the launcher adds the blocks in order.
+        const int64_t rows = b - m0 < ROWS ? b - m0 : ROWS;
+        if (tid < rows) energy[(int64_t)blockIdx.x * rows + tid] = rowred[0][tid] + rowred[1][tid];
     }
 }
 
@@ -407,10 +410,16 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     MPISDF_HIP(hipGetLastError());
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
-    // One launch per group of 128 rows (Q^T rows), each with its own tile count.
-    for (int64_t m0 = 0; m0 < b; m0 += FBM)
-        launch_fold_i8(std::min<int64_t>(FBM, b - m0), energy != nullptr, dim3((unsigned)gx), stream,
-                       scratch.digits.ptr, scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, yscale, m, n, proj2, energy);
+    if (energy && scratch.part.count < (size_t)(gx * FBM)) scratch.part.resize((size_t)(gx * FBM));
+    // One launch per group of 128 rows (Q^T rows), each with its own tile
+    // count; the row energies of a group are summed over column blocks in
+    // block order.
+    for (int64_t m0 = 0; m0 < b; m0 += FBM) {
+        const int64_t rows = std::min<int64_t>(FBM, b - m0);
+        launch_fold_i8(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr, scratch.scales.ptr, ldq,
+                       bpad, b, m0, Yd, P, yscale, m, n, proj2, scratch.part.ptr);
+        if (energy) launch_sum_partials(scratch.part.ptr, gx, rows, energy + m0, true, stream);
+    }
 }
 
 }  // namespace mpisdf

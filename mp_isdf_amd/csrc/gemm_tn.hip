@@ -19,6 +19,7 @@
 // x 32 K is staged in double-buffered LDS and shared by the four waves; B goes
 // straight from HBM to registers, one slice ahead.
 #include "kernels.hpp"
+#include "util.hpp"
 
 #include <algorithm>
 
@@ -213,7 +214,8 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
     }
 
     if (KR) {
-        // y[g * ldc + s] = cscale[g] (U phi)[s, g] (V phi)[s, g]; colacc[g] += sum_s y^2.
+        // y[g * ldc + s] = cscale[g] (U phi)[s, g] (V phi)[s, g]; colacc[r * N + g] = sum of y^2
+        // over the rows s of row group r (one writer per entry; added in row-group order later).
         unsigned long long nonfinite = 0;
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
                 }
             s2 += __shfl_xor(s2, 16);
             s2 += __shfl_xor(s2, 32);
-            if (quarter == 0 && gin && colacc) atomicAdd(&colacc[gn], s2);
+            if (quarter == 0 && gin && colacc) colacc[(int64_t)blockIdx.y * N + gn] = s2;
         }
         if (nonfinite) atomicAdd(bad, nonfinite);
         return;
@@ -278,8 +280,8 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
             }
         }
         if (EPI == kTnStoreRowDot) {
-            // rowacc[m] += sum over the workgroup's columns of alpha acc * cscale[n]
-            // (columns >= N hold zeros and weigh zero).
+            // rowacc[x * M + m] = sum over the columns of column block x of
+            // alpha acc * cscale[n] (columns >= N hold zeros and weigh zero).
             double wc[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
@@ -299,12 +301,13 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
             if (tid < TBM) {
                 const int64_t gm = m0 + tid;
                 const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-                if (gm < M && tid < R * 16) atomicAdd(&rowacc[gm], s);
+                if (gm < M && tid < R * 16) rowacc[(int64_t)blockIdx.x * M + gm] = s;
             }
         }
         return;
     }
-    // Column sums of W^2 over this row group: each column lives in one wave.
+    // Column sums of W^2 over the (single) row group: each column lives in one
+    // wave, and launches of successive row groups add in order.
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
         double s = 0;
@@ -315,14 +318,12 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         s += __shfl_xor(s, 16);
         s += __shfl_xor(s, 32);
         const int64_t gn = n0 + 16 * c + l16;
-        if (quarter == 0 && gn < N) {
-            if (gridDim.y == 1) colacc[gn] += s;
-            else atomicAdd(&colacc[gn], s);
-        }
+        if (quarter == 0 && gn < N) colacc[gn] += s;
     }
     if (EPI == kTnFoldEnergy) {
         // Row sums of W^2 over the workgroup's columns: lanes of one quarter
         // share rows; reduce over l16 and the two column tiles, then the waves.
+        // rowacc[x * M + m] is column block x's partial.
 #pragma unroll
         for (int t = 0; t < 8; ++t)
 #pragma unroll
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         if (tid < TBM) {
             const int64_t gm = m0 + tid;
             const double s = red[0][tid] + red[1][tid] + red[2][tid] + red[3][tid];
-            if (gm < M && tid < R * 16) atomicAdd(&rowacc[gm], s);
+            if (gm < M && tid < R * 16) rowacc[(int64_t)blockIdx.x * M + gm] = s;
         }
     }
 }
@@ -351,6 +352,8 @@ void launch_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, co
     const int64_t rows = EPI == kTnKhatriRao ? TBM / 2 : TBM;
     const int64_t gx = (N + TBN - 1) / TBN, gy = (M + rows - 1) / rows;
     if (gx > 0x7fffffff || gy > 65535) throw std::overflow_error("TN GEMM grid too large");
+    if ((EPI == kTnFold || EPI == kTnFoldEnergy) && gy != 1)
+        throw std::logic_error("a fold launch covers one row group of at most 128 rows");
     const bool vec = lda % 4 == 0 && ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
                      (reinterpret_cast<uintptr_t>(A2) % 32) == 0 &&
                      (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB))) == 0;
@@ -394,27 +397,44 @@ void gemm_tn_f64_fused(TnFused mode, int64_t M, int64_t N, int64_t K, double alp
     }
 }
 
+int64_t tn_column_blocks(int64_t N) { return (N + TBN - 1) / TBN; }
+
 void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldq, const void* Y, bool y_f32,
-                      int64_t ldy, double* proj2, double* energy, hipStream_t stream)
+                      int64_t ldy, double* proj2, double* energy, DeviceBuffer<double>& part, hipStream_t stream)
 {
-    // W (b x n) = Qb^T Y; proj2[j] += sum_t W[t,j]^2; energy[t] += sum_j W[t,j]^2
-    if (y_f32) {
-        const float* y = static_cast<const float*>(Y);
-        if (energy) launch_tn<float, kTnFoldEnergy>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
-        else launch_tn<float, kTnFold>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
-    } else {
-        const double* y = static_cast<const double*>(Y);
-        if (energy) launch_tn<double, kTnFoldEnergy>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, energy, stream);
-        else launch_tn<double, kTnFold>(b, n, m, Qb, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
+    // W (b x n) = Qb^T Y; proj2[j] += sum_t W[t,j]^2; energy[t] += sum_j W[t,j]^2.
+    // One launch per group of 128 rows, so every column's sum has one writer
+    // per launch and the groups add in order; the energies are written as
+    // per-column-block partials and added in block order.
+    if (b <= 0 || n <= 0) return;
+    const int64_t gx = tn_column_blocks(n);
+    if (energy && part.count < (size_t)(gx * TBM)) part.resize((size_t)(gx * TBM));
+    for (int64_t r0 = 0; r0 < b; r0 += TBM) {
+        const int64_t rows = std::min<int64_t>(TBM, b - r0);
+        const double* q = Qb + r0 * ldq;
+        if (y_f32) {
+            const float* y = static_cast<const float*>(Y);
+            if (energy) launch_tn<float, kTnFoldEnergy>(rows, n, m, q, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, part.ptr, stream);
+            else launch_tn<float, kTnFold>(rows, n, m, q, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
+        } else {
+            const double* y = static_cast<const double*>(Y);
+            if (energy) launch_tn<double, kTnFoldEnergy>(rows, n, m, q, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, part.ptr, stream);
+            else launch_tn<double, kTnFold>(rows, n, m, q, ldq, y, ldy, 1.0, 0.0, nullptr, 0, proj2, nullptr, stream);
+        }
+        if (energy) launch_sum_partials(part.ptr, gx, rows, energy + r0, true, stream);
     }
 }
 
 void kr_sketch_f64_tn(const double* ut, const double* vt, int64_t ell, int64_t d, const double* phi, int64_t ldphi,
                       int64_t count, const double* scale, double* y, int64_t ldy, double* ynorm2,
-                      unsigned long long* bad, hipStream_t stream)
+                      DeviceBuffer<double>& part, unsigned long long* bad, hipStream_t stream)
 {
-    launch_tn<double, kTnKhatriRao>(ell, count, d, ut, d, phi, ldphi, 1.0, 0.0, y, ldy, ynorm2, nullptr, stream, vt,
-                                    scale, bad);
+    // ||y_g||^2 per 64-row group of the launch, then added in group order.
+    const int64_t groups = (ell + TBM / 2 - 1) / (TBM / 2);
+    if (ynorm2 && part.count < (size_t)(groups * count)) part.resize((size_t)(groups * count));
+    launch_tn<double, kTnKhatriRao>(ell, count, d, ut, d, phi, ldphi, 1.0, 0.0, y, ldy, ynorm2 ? part.ptr : nullptr,
+                                    nullptr, stream, vt, scale, bad);
+    if (ynorm2) launch_sum_partials(part.ptr, groups, count, ynorm2, true, stream);
 }
 
 }  // namespace mpisdf

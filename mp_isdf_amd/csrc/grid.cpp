@@ -13,7 +13,12 @@
 //   sqrt(R_Bragg) size adjustment, a_ij clipped to [-1/2, 1/2].
 // * Order: atom, radial shell, polar ring, azimuth -- consecutive points are
 //   spatially close, which the AO kernel's tile culling exploits.
+// * The partition runs on the GPU when one is present (one thread per point)
+//   and on host threads otherwise; both evaluate the same function with no
+//   FMA contraction, so the weights are identical.
 #include "grid.hpp"
+
+#include "common.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -156,13 +161,6 @@ AtomGrid atom_grid(int z, int level)
     return g;
 }
 
-inline double becke_s(double mu)
-{
-    double g = mu;
-    for (int i = 0; i < 3; ++i) g = 1.5 * g - 0.5 * g * g * g;
-    return 0.5 * (1 - g);
-}
-
 template <class F>
 void parallel_for(int64_t n, F&& f)
 {
@@ -179,78 +177,169 @@ void parallel_for(int64_t n, F&& f)
 
 }  // namespace
 
-std::pair<std::vector<double>, std::vector<double>> becke_grid(const int* charges, const double* coords, int64_t natm,
-                                                               int level)
+namespace {
+
+struct Layout {
+    std::vector<AtomGrid> per_element;
+    std::vector<int64_t> offset;  // first point of each atom
+    std::vector<double> rpair, apair;
+};
+
+Layout layout(const int* charges, const double* coords, int64_t natm, int level)
 {
     if (natm <= 0) throw std::invalid_argument("grid needs at least one atom");
     if (level < 0 || level > 9) throw std::invalid_argument("grid level must be 0..9");
-    std::vector<AtomGrid> per_element(11);
-    std::vector<double> sqrt_r(natm), rpair((size_t)(natm * natm)), apair((size_t)(natm * natm));
+    Layout L;
+    L.per_element.resize(11);
+    std::vector<double> sqrt_r(natm);
+    L.rpair.resize((size_t)(natm * natm));
+    L.apair.resize((size_t)(natm * natm));
     for (int64_t i = 0; i < natm; ++i) {
         const int z = charges[i];
         bragg_radius_bohr(z);  // validates Z before indexing
-        if (per_element[z].w.empty()) per_element[z] = atom_grid(z, level);
+        if (L.per_element[z].w.empty()) L.per_element[z] = atom_grid(z, level);
         sqrt_r[i] = std::sqrt(bragg_radius_bohr(z));
     }
     for (int64_t i = 0; i < natm; ++i)
         for (int64_t j = 0; j < natm; ++j) {
             const double dx = coords[3 * i] - coords[3 * j], dy = coords[3 * i + 1] - coords[3 * j + 1],
                          dz = coords[3 * i + 2] - coords[3 * j + 2];
-            rpair[i * natm + j] = std::sqrt(dx * dx + dy * dy + dz * dz);
+            L.rpair[i * natm + j] = std::sqrt(dx * dx + dy * dy + dz * dz);
             const double rr = sqrt_r[i] / sqrt_r[j];
-            apair[i * natm + j] = std::clamp(0.25 * (1 / rr - rr), -0.5, 0.5);
+            L.apair[i * natm + j] = std::clamp(0.25 * (1 / rr - rr), -0.5, 0.5);
         }
     for (int64_t i = 0; i < natm; ++i)
         for (int64_t j = 0; j < i; ++j)
-            if (rpair[i * natm + j] < 1e-8) throw std::invalid_argument("atoms must not coincide");
-    std::vector<int64_t> offset(natm + 1, 0);
-    for (int64_t i = 0; i < natm; ++i) offset[i + 1] = offset[i] + (int64_t)per_element[charges[i]].w.size();
-    const int64_t total = offset[natm];
-    std::vector<double> xyz((size_t)(3 * total)), w((size_t)total);
-    parallel_for(natm * 64, [&](int64_t task) {
-        const int64_t atom = task / 64, part = task % 64;
-        const AtomGrid& g = per_element[charges[atom]];
-        const int64_t np = (int64_t)g.w.size();
-        std::vector<double> dist(natm);
-        std::vector<int64_t> near;
-        for (int64_t p = part; p < np; p += 64) {
-            const double px = coords[3 * atom] + g.xyz[3 * p], py = coords[3 * atom + 1] + g.xyz[3 * p + 1],
-                         pz = coords[3 * atom + 2] + g.xyz[3 * p + 2];
-            int64_t nearest = 0;
-            for (int64_t i = 0; i < natm; ++i) {
-                const double dx = px - coords[3 * i], dy = py - coords[3 * i + 1], dz = pz - coords[3 * i + 2];
-                dist[i] = std::sqrt(dx * dx + dy * dy + dz * dz);
-                if (dist[i] < dist[nearest]) nearest = i;
-            }
+            if (L.rpair[i * natm + j] < 1e-8) throw std::invalid_argument("atoms must not coincide");
+    L.offset.assign(natm + 1, 0);
+    for (int64_t i = 0; i < natm; ++i) L.offset[i + 1] = L.offset[i] + (int64_t)L.per_element[charges[i]].w.size();
+    return L;
+}
+
+// The Becke partition of one point, written once for the host loop and the
+// device kernel so both round every operation alike (no contraction into
+// FMAs): w = wq * P_atom / sum_i P_i over the cells that can contribute.
+#pragma clang fp contract(off)
+__host__ __device__ inline double becke_s_point(double mu)
+{
+    double g = mu;
+    for (int i = 0; i < 3; ++i) g = 1.5 * g - 0.5 * g * g * g;
+    return 0.5 * (1 - g);
+}
+
+__host__ __device__ inline double atom_distance(const double* coords, int64_t i, double px, double py, double pz)
+{
+    const double dx = px - coords[3 * i], dy = py - coords[3 * i + 1], dz = pz - coords[3 * i + 2];
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+__host__ __device__ inline double becke_point(const double* coords, const double* rpair, const double* apair,
+                                              int64_t natm, int64_t atom, double px, double py, double pz, double wq)
+{
+    int64_t nearest = 0;
+    double dn = atom_distance(coords, 0, px, py, pz);
+    for (int64_t i = 1; i < natm; ++i) {
+        const double di = atom_distance(coords, i, px, py, pz);
+        if (di < dn) { nearest = i; dn = di; }
+    }
+    double sum = 0, own = 0;
+    for (int64_t i = 0; i < natm; ++i) {
+        const double di = atom_distance(coords, i, px, py, pz);
+        if (i != nearest && i != atom) {
             // Cells whose factor against the nearest atom is below ~1e-15
             // (adjusted mu > 0.99) cannot contribute.
-            near.clear();
-            for (int64_t i = 0; i < natm; ++i) {
-                if (i == nearest || i == atom) { near.push_back(i); continue; }
-                const double r = rpair[i * natm + nearest];
-                double mu = (dist[i] - dist[nearest]) / r;
-                mu += apair[i * natm + nearest] * (1 - mu * mu);
-                if (mu < 0.99) near.push_back(i);
-            }
-            double sum = 0, own = 0;
-            for (int64_t i : near) {
-                double prod = 1;
-                for (int64_t j = 0; j < natm && prod > 0; ++j) {
-                    if (j == i) continue;
-                    double mu = (dist[i] - dist[j]) / rpair[i * natm + j];
-                    mu += apair[i * natm + j] * (1 - mu * mu);
-                    prod *= becke_s(mu);
-                }
-                sum += prod;
-                if (i == atom) own = prod;
-            }
-            const int64_t out = offset[atom] + p;
-            xyz[3 * out] = px;
-            xyz[3 * out + 1] = py;
-            xyz[3 * out + 2] = pz;
-            w[out] = sum > 0 ? g.w[p] * own / sum : 0.0;
+            double mu = (di - dn) / rpair[i * natm + nearest];
+            mu += apair[i * natm + nearest] * (1 - mu * mu);
+            if (!(mu < 0.99)) continue;
         }
-    });
+        double prod = 1;
+        for (int64_t j = 0; j < natm && prod > 0; ++j) {
+            if (j == i) continue;
+            double mu = (di - atom_distance(coords, j, px, py, pz)) / rpair[i * natm + j];
+            mu += apair[i * natm + j] * (1 - mu * mu);
+            prod *= becke_s_point(mu);
+        }
+        sum += prod;
+        if (i == atom) own = prod;
+    }
+    return sum > 0 ? wq * own / sum : 0.0;
+}
+
+__global__ void becke_partition_kernel(const double* __restrict__ xyz, const double* __restrict__ wq,
+                                       const int32_t* __restrict__ atom_of, const double* __restrict__ coords,
+                                       const double* __restrict__ rpair, const double* __restrict__ apair,
+                                       int64_t natm, int64_t n, double* __restrict__ w)
+{
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        w[p] = becke_point(coords, rpair, apair, natm, atom_of[p], xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2], wq[p]);
+}
+#pragma clang fp contract(on)
+
+bool have_device()
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+int64_t becke_grid_size(const int* charges, int64_t natm, int level)
+{
+    if (natm <= 0) throw std::invalid_argument("grid needs at least one atom");
+    if (level < 0 || level > 9) throw std::invalid_argument("grid level must be 0..9");
+    std::vector<int64_t> per(11, -1);
+    int64_t total = 0;
+    for (int64_t i = 0; i < natm; ++i) {
+        const int z = charges[i];
+        bragg_radius_bohr(z);
+        if (per[z] < 0) per[z] = (int64_t)atom_grid(z, level).w.size();
+        total += per[z];
+    }
+    return total;
+}
+
+std::pair<std::vector<double>, std::vector<double>> becke_grid(const int* charges, const double* coords, int64_t natm,
+                                                               int level, bool device)
+{
+    const Layout L = layout(charges, coords, natm, level);
+    const int64_t total = L.offset[natm];
+    std::vector<double> xyz((size_t)(3 * total)), wq((size_t)total), w((size_t)total);
+    std::vector<int32_t> atom_of((size_t)total);
+    for (int64_t atom = 0; atom < natm; ++atom) {
+        const AtomGrid& g = L.per_element[charges[atom]];
+        for (int64_t p = 0; p < (int64_t)g.w.size(); ++p) {
+            const int64_t out = L.offset[atom] + p;
+            for (int c = 0; c < 3; ++c) xyz[3 * out + c] = coords[3 * atom + c] + g.xyz[3 * p + c];
+            wq[out] = g.w[p];
+            atom_of[out] = (int32_t)atom;
+        }
+    }
+    if (device && have_device()) {
+        // One thread per point; the partition is O(n natm^2) in the worst
+        // case, seconds on the host at 2e6 points and 192 atoms.
+        DeviceBuffer<double> dx, dwq, dc, drp, dap, dw(total);
+        DeviceBuffer<int32_t> da;
+        dx.upload(xyz.data(), xyz.size(), nullptr);
+        dwq.upload(wq.data(), wq.size(), nullptr);
+        dc.upload(coords, 3 * natm, nullptr);
+        drp.upload(L.rpair.data(), L.rpair.size(), nullptr);
+        dap.upload(L.apair.data(), L.apair.size(), nullptr);
+        da.upload(atom_of.data(), atom_of.size(), nullptr);
+        const int64_t blocks = std::min<int64_t>((total + 255) / 256, 1 << 20);
+        hipLaunchKernelGGL(becke_partition_kernel, dim3((unsigned)std::max<int64_t>(blocks, 1)), dim3(256), 0, nullptr,
+                           dx.ptr, dwq.ptr, da.ptr, dc.ptr, drp.ptr, dap.ptr, natm, total, dw.ptr);
+        MPISDF_HIP(hipGetLastError());
+        MPISDF_HIP(hipMemcpy(w.data(), dw.ptr, total * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        parallel_for(total, [&](int64_t p) {
+            w[p] = becke_point(coords, L.rpair.data(), L.apair.data(), natm, atom_of[p], xyz[3 * p], xyz[3 * p + 1],
+                               xyz[3 * p + 2], wq[p]);
+        });
+    }
     return {std::move(xyz), std::move(w)};
 }
 

@@ -14,9 +14,10 @@ void launch_ao_eval(Context& ctx, const double* xyz, int64_t n, AoMode mode, DTy
                     void* out, int64_t ld, int64_t cstride, hipStream_t stream);
 
 // Adjoint of gradient evaluation (FP64): gout[c * nao + mu] += sum_p
-// abar[p * ldab + mu] * d phi_mu(r_p) / d r_c, gradients recomputed on the fly.
+// abar[p * ldab + mu] * d phi_mu(r_p) / d r_c, gradients recomputed on the fly;
+// per-tile partials (``part``, resized as needed) are added in tile order.
 void launch_ao_adjoint(Context& ctx, const double* xyz, int64_t n, const double* abar, int64_t ldab, double* gout,
-                       hipStream_t stream);
+                       DeviceBuffer<double>& part, hipStream_t stream);
 
 // One axis block of the concatenated feature matrix.
 struct AxisSegment {
@@ -55,26 +56,33 @@ void gemm_tn_f64(int64_t M, int64_t N, int64_t K, double alpha, const double* A,
 //   Square:            C = (A^T B)^2 elementwise (alpha ignored);
 //   ColumnSumSquares:  C = alpha A^T B and colacc[g * N + n] = sum over the
 //                      rows m of row group g (128 rows each) of C[m, n]^2;
-//   RowDot:            C = alpha A^T B and rowacc[m] += sum_n C[m, n] colw[n].
+//   RowDot:            C = alpha A^T B and rowacc[x * M + m] = sum over the
+//                      columns n of column block x (tn_column_blocks(N)
+//                      blocks of 128) of C[m, n] colw[n]; the caller adds
+//                      the blocks in order (launch_sum_partials).
 enum class TnFused { Square, ColumnSumSquares, RowDot };
 void gemm_tn_f64_fused(TnFused mode, int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda,
                        const double* B, int64_t ldb, double* C, int64_t ldc, double* colacc, double* rowacc,
                        const double* colw, hipStream_t stream, int triangular = 0);
 
+int64_t tn_column_blocks(int64_t N);
+
 // FP64 Khatri-Rao sketch on the TN kernel: ut/vt are ell x d row-major
 // (ut[s * d + a]), phi[g * ldphi + a];  y[g * ldy + s] = scale[g] (ut_s . phi_g)
-// (vt_s . phi_g) for g < count, ynorm2[g] += ||y_g||^2 (when given), non-finite
-// outputs counted into *bad.
+// (vt_s . phi_g) for g < count, ynorm2[g] += ||y_g||^2 (when given; summed
+// over row groups in a fixed order through ``part``), non-finite outputs
+// counted into *bad.
 void kr_sketch_f64_tn(const double* ut, const double* vt, int64_t ell, int64_t d, const double* phi, int64_t ldphi,
                       int64_t count, const double* scale, double* y, int64_t ldy, double* ynorm2,
-                      unsigned long long* bad, hipStream_t stream);
+                      DeviceBuffer<double>& part, unsigned long long* bad, hipStream_t stream);
 
 // QRCP fold: W = Qb^T Y (Qb: m x b column-major with leading dimension ldq,
 // Y: m x n with leading dimension ldy, FP32 or FP64); proj2[j] += sum_t
 // W[t,j]^2 and, when energy is given, energy[t] += sum_j W[t,j]^2.  FP64
-// MFMA, W never stored.
+// MFMA, W never stored; every sum is added in a fixed order (``part`` holds
+// the energies' per-column-block partials).
 void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldq, const void* Y, bool y_f32,
-                      int64_t ldy, double* proj2, double* energy, hipStream_t stream);
+                      int64_t ldy, double* proj2, double* energy, DeviceBuffer<double>& part, hipStream_t stream);
 
 // Same fold for FP32 Y with split-BF16 MFMA (hi*hi + hi*lo + lo*hi) and FP64
 // accumulation across 64-wide K chunks; error ~1e-6 ||q|| ||y_j|| per entry.
@@ -117,6 +125,7 @@ __host__ __device__ inline int64_t digit_offset(int64_t i, int r)
 struct FoldI8Scratch {
     DeviceBuffer<int8_t> digits;
     DeviceBuffer<double> scales;
+    DeviceBuffer<double> part;  // per-column-block row energies of one launch
 };
 // Relative bound (of ||y_j||^2) on the error a folded sum accumulates over a
 // whole factorisation with the INT8 fold; enters the keys as a margin.

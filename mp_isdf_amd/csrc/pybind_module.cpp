@@ -317,23 +317,26 @@ PYBIND11_MODULE(mp_isdf, m)
         .value("SVD", DistributedAlgorithm::SVD)
         .value("RandomizedQRColumnPivoting", DistributedAlgorithm::RandomizedQRColumnPivoting);
 
-    m.def("becke_grid", [](const IntArray& charges, const FloatArray& coords, int level) {
+    m.def("becke_grid", [](const IntArray& charges, const FloatArray& coords, int level, bool device) {
         if (charges.ndim() != 1 || coords.ndim() != 2 || coords.shape(1) != 3 || coords.shape(0) != charges.shape(0))
             throw py::value_error("expected charges[natm] and coords[natm,3] (Bohr)");
         int64_t count = 0;
-        check(mpisdf_becke_grid(charges.data(), coords.data(), charges.shape(0), level, nullptr, nullptr, 0, &count));
+        check(mpisdf_becke_grid(charges.data(), coords.data(), charges.shape(0), level, device, nullptr, nullptr, 0,
+                                &count));
         py::array_t<double> xyz({(py::ssize_t)count, py::ssize_t(3)}), w((py::ssize_t)count);
         int status;
         double* px = xyz.mutable_data();
         double* pw = w.mutable_data();
         {
             py::gil_scoped_release release;
-            status = mpisdf_becke_grid(charges.data(), coords.data(), charges.shape(0), level, px, pw, count, &count);
+            status = mpisdf_becke_grid(charges.data(), coords.data(), charges.shape(0), level, device, px, pw, count,
+                                       &count);
         }
         check(status);
         return py::make_tuple(xyz, w);
-    }, py::arg("charges"), py::arg("coords"), py::arg("level") = 3,
-       "Becke-partitioned molecular grid (Bohr): returns (coords[n,3], weights[n]).  Not part of the reference "
+    }, py::arg("charges"), py::arg("coords"), py::arg("level") = 3, py::arg("device") = true,
+       "Becke-partitioned molecular grid (Bohr): returns (coords[n,3], weights[n]); the partition runs on the GPU "
+       "when device is true and one is present (same weights as the host loop).  Not part of the reference "
        "extension, which receives PySCF's grid.");
 
     m.def("isdf_fit", [](const Context& ctx, const FloatArray& anchor_coords, const FloatArray& anchor_weights,

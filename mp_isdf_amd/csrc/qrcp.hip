@@ -24,6 +24,7 @@
 #include "qrcp.hpp"
 
 #include "kernels.hpp"
+#include "util.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -475,7 +476,8 @@ __global__ __launch_bounds__(T) void cand_update_reg_kernel(double* __restrict__
     }
 }
 
-// tail2 += sum (Y - Q W)^2 over a column chunk; R holds Y chunk - Q W (m x nc).
+// out[block] = this workgroup's part of sum (Y - Q W)^2 over a column chunk;
+// R holds Y chunk - Q W (m x nc).  The parts are added in block order.
 __global__ void sumsq_kernel(const double* __restrict__ r, int64_t count, double* __restrict__ out)
 {
     __shared__ double part[4];
@@ -485,7 +487,7 @@ __global__ void sumsq_kernel(const double* __restrict__ r, int64_t count, double
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = s;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+    if (threadIdx.x == 0) out[blockIdx.x] = (part[0] + part[1]) + (part[2] + part[3]);
 }
 
 template <class T>
@@ -597,6 +599,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     bool done = false;
     FoldScratch fold_scratch;
     FoldI8Scratch i8_scratch;
+    DeviceBuffer<double> fold_part;  // the FP64 fold's per-column-block energy partials
     const bool split = opt.split_bf16_fold && ytype == kF32;
     // Fold launches are bracketed by HIP events; durations are read at the end.
     LaunchTimer fold_timer;
@@ -624,7 +627,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                                     energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
         else
             fold_projections(b, n, m, Q.ptr + folded * ldq, ldq, y, ytype == kF32, ldy, proj2.ptr,
-                             energy_out ? energy_out + folded : nullptr, s);
+                             energy_out ? energy_out + folded : nullptr, fold_part, s);
         fold_timer.end(s);
         folded += b;
         ++res.folds;
@@ -635,7 +638,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     auto refresh = [&]() {
         MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
         MPISDF_HIP(hipMemsetAsync(err2.ptr, 0, n * sizeof(double), s));
-        if (k > 0) fold_projections(k, n, m, Q.ptr, ldq, y, ytype == kF32, ldy, proj2.ptr, nullptr, s);
+        if (k > 0) fold_projections(k, n, m, Q.ptr, ldq, y, ytype == kF32, ldy, proj2.ptr, nullptr, fold_part, s);
         folded = k;
         ++res.refreshes;
     };
@@ -783,7 +786,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         // Exact residual after the last pivot: Y - Q Q^T Y, column chunks.
         MPISDF_HIP(hipMemsetAsync(tail.ptr, 0, sizeof(double), s));
         const int64_t nc = std::max<int64_t>(1, std::min<int64_t>(n, (1ll << 26) / std::max<int64_t>(m, 1)));
-        DeviceBuffer<double> R((size_t)(m * nc));
+        DeviceBuffer<double> R((size_t)(m * nc)), tail_part(4096);
         DeviceBuffer<double> Wt(k > 0 ? (size_t)(k * nc) : 1);
         for (int64_t j0 = 0; j0 < n; j0 += nc) {
             const int64_t cnt = std::min(nc, n - j0);
@@ -803,9 +806,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                              s);
                 }
             }
-            hipLaunchKernelGGL(sumsq_kernel, dim3(grid_for(m * cnt, 256, 4096)), dim3(256), 0, s, R.ptr, m * cnt,
-                               tail.ptr);
+            const int blocks = grid_for(m * cnt, 256, 4096);
+            hipLaunchKernelGGL(sumsq_kernel, dim3(blocks), dim3(256), 0, s, R.ptr, m * cnt, tail_part.ptr);
             MPISDF_HIP(hipGetLastError());
+            launch_sum_fixed(tail_part.ptr, blocks, 0, 1, tail.ptr, s);
         }
         MPISDF_HIP(hipMemcpyAsync(&res.tail2, tail.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
     }

@@ -87,7 +87,7 @@ std::vector<double> axis_scales(Context& ctx, const SelectArgs& a, const AxisSpe
     const int64_t n = (int64_t)kept.original.size();
     if (n == 0) return scales;
     const int64_t batch = std::min(n, device_batch(spec.width, ctx.nao, 0));
-    DeviceBuffer<double> d_xyz(3 * batch), feat(batch * spec.width), sums(spec.width);
+    DeviceBuffer<double> d_xyz(3 * batch), feat(batch * spec.width), sums(spec.width), part;
     MPISDF_HIP(hipMemsetAsync(sums.ptr, 0, spec.width * sizeof(double), s));
     AxisWorkspace ws;
     for (int64_t start = 0; start < n; start += batch) {
@@ -95,7 +95,7 @@ std::vector<double> axis_scales(Context& ctx, const SelectArgs& a, const AxisSpe
         MPISDF_HIP(hipMemcpyAsync(d_xyz.ptr, kept.xyz.data() + 3 * start, 3 * count * sizeof(double),
                                   hipMemcpyHostToDevice, s));
         evaluate_axis_device(ctx, spec, d_xyz.ptr, count, nullptr, a.mo_coeff, a.nocc, feat.ptr, ws, s);
-        launch_colsumsq(feat.ptr, count, spec.width, sums.ptr, s);
+        launch_colsumsq(feat.ptr, count, spec.width, sums.ptr, part, s);
     }
     const auto h = sums.download(s);
     int64_t off = 0;
@@ -318,6 +318,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         DeviceBuffer<uint8_t> phi((size_t)batch * dp * esz);
         MPISDF_HIP(hipMemsetAsync(phi.ptr, 0, phi.count, s));  // padding columns stay zero
         DeviceBuffer<float> phi32, scale(batch);
+        DeviceBuffer<double> ynorm_part;
         DeviceBuffer<double> feat;
         if (!ao_only) feat.resize((size_t)batch * d);
         if (in_type == kFP8) phi32.resize((size_t)batch * d);
@@ -343,7 +344,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             }
             timer.begin(s);
             kr_sketch_lowp(in_type, ut.ptr, vt.ptr, ell, dp, phi.ptr, dp, count, scale.ptr, y32.ptr + start * ldy32,
-                           ldy32, ynorm2.ptr + start, bad.ptr, s);
+                           ldy32, ynorm2.ptr + start, bad.ptr, ynorm_part, s);
             timer.end(s);
         }
         MPISDF_HIP(hipStreamSynchronize(s));

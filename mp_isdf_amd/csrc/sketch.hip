@@ -3,8 +3,8 @@
 //
 // FP64 form: U^T and V^T (ell x d, K-contiguous) are formed once per Omega;
 // each batch is one TN MFMA launch whose epilogue multiplies the two
-// projections, scales by sqrt(h_g / ell), writes Y, accumulates ||y_g||^2 and
-// counts non-finite entries (the reference's id_check_sketch, :310-317).
+// projections, scales by sqrt(h_g / ell), writes Y, sums ||y_g||^2 per row
+// group (added in group order afterwards) and counts non-finite entries (the reference's id_check_sketch, :310-317).
 #include "kernels.hpp"
 #include "sketch.hpp"
 
@@ -63,8 +63,8 @@ void sketch_f64(const double* phi, int64_t count, int64_t d, const double* u, co
                        1.0 / std::sqrt((double)ell), scratch.scale.ptr);
     MPISDF_HIP(hipGetLastError());
     scratch.timer.begin(stream);
-    kr_sketch_f64_tn(scratch.ut.ptr, scratch.vt.ptr, ell, d, phi, d, count, scratch.scale.ptr, y, ldy, ynorm2, bad,
-                     stream);
+    kr_sketch_f64_tn(scratch.ut.ptr, scratch.vt.ptr, ell, d, phi, d, count, scratch.scale.ptr, y, ldy, ynorm2,
+                     scratch.part, bad, stream);
     scratch.timer.end(stream);
 }
 

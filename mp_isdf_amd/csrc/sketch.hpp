@@ -8,6 +8,7 @@ namespace mpisdf {
 // Transposed Omega (formed once per u, v) and per-batch point scales.
 struct SketchScratch {
     DeviceBuffer<double> ut, vt, scale;
+    DeviceBuffer<double> part;  // per-row-group partial sums of ||y_g||^2
     const double* u_src = nullptr;
     const double* v_src = nullptr;
     int64_t ell = 0, d = 0;

@@ -24,10 +24,11 @@ void quantize_fp8_rows(const float* phi, int64_t ld, int64_t count, int64_t d, c
 // out[g * ldo + a] = cast(in[g * d + a]) with zero padding (bf16 or f32).
 void cast_rows(const double* in, int64_t count, int64_t d, int64_t ldo, DType dtype, void* out, hipStream_t s);
 
-// Y[g * ldy + s] (FP32) and ynorm2[g] += ||y_g||^2 for g < count; d must be a
+// Y[g * ldy + s] (FP32) and ynorm2[g] += ||y_g||^2 for g < count (per Omega
+// row block partials in ``part``, added in block order); d must be a
 // multiple of 8 (bf16) or 16 (fp8) -- callers zero-pad.
 void kr_sketch_lowp(DType dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi, int64_t ldphi,
                     int64_t count, const float* scale, float* y, int64_t ldy, double* ynorm2, unsigned long long* bad,
-                    hipStream_t s);
+                    DeviceBuffer<double>& part, hipStream_t s);
 
 }  // namespace mpisdf

@@ -12,7 +12,9 @@ import numpy as np
 from .basis import CHARGES, Molecule
 
 
-def make_grid(mol: Molecule, level: int = 3):
-    from . import mp_isdf  # compiled extension (host-side generator)
+def make_grid(mol: Molecule, level: int = 3, device: bool = True):
+    """Becke-partitioned grid; ``device`` runs the partition on the GPU when one
+    is present (the same weights as the host loop)."""
+    from . import mp_isdf  # compiled extension
     charges = np.asarray([CHARGES[s] for s in mol.symbols], dtype=np.int32)
-    return mp_isdf.becke_grid(charges, np.ascontiguousarray(mol.coords), level)
+    return mp_isdf.becke_grid(charges, np.ascontiguousarray(mol.coords), level, device)

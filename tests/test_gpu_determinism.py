@@ -1,0 +1,47 @@
+"""Every FP64 sum over workgroups is added in a fixed order, so a selection
+and its fit are bit-identical from run to run: the FP64 rand_sketch anchors
+(the FP64 fold's column sums, the sketch's ||y||^2, the QRCP keys), the
+relative tail of the tolerance mode (row energies, exact tail), the
+multi-axis scales and the fit's pair-space error and theta . w."""
+import numpy as np
+import pytest
+
+from mp_isdf_amd import grid as gridmod
+from mp_isdf_amd import molecule
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def water8(mp_isdf):
+    mol = molecule.water_cluster(8, "cc-pvdz")
+    xyz, w = gridmod.make_grid(mol, 2)
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    return mol, xyz, w, ctx, mp_isdf.AnchorSelection(ctx)
+
+
+def test_fp64_rand_sketch_and_fit_repeat_bitwise(mp_isdf, water8):
+    mol, xyz, w, ctx, sel = water8
+    alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
+    ao = mp_isdf.AxisType.AtomicOrbital
+    runs = [sel.select_with_diagnostics(xyz, w, ao, algorithm=alg, max_rank=800, oversampling=80, seed=7)
+            for _ in range(2)]
+    assert runs[0][0] == runs[1][0]
+    assert runs[0][1]["sketch_norm"] == runs[1][1]["sketch_norm"]
+    a = np.asarray(runs[0][0])
+    fits = [mp_isdf.isdf_fit(ctx, xyz[a], w[a], xyz[w != 0], w[w != 0], batch_size=20000, weight_sums=True)
+            for _ in range(2)]
+    assert fits[0]["pair_space_error"]["l2"] == fits[1]["pair_space_error"]["l2"]
+    np.testing.assert_array_equal(fits[0]["theta_weight_sums"], fits[1]["theta_weight_sums"])
+
+
+def test_tolerance_mode_and_multi_axis_repeat_bitwise(mp_isdf, water8):
+    mol, xyz, w, ctx, sel = water8
+    alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
+    axes = [mp_isdf.AxisType.AtomicOrbital, mp_isdf.AxisType.AtomicOrbitalGradX]
+    sub = slice(0, 60000)
+    runs = [sel.select_with_diagnostics(xyz[sub], w[sub], axes, tolerance=1e-3, sketch_size=600, oversampling=60,
+                                        seed=3) for _ in range(2)]
+    assert runs[0][0] == runs[1][0]
+    assert runs[0][1]["relative_tail"] == runs[1][1]["relative_tail"]
+    assert 0 < len(runs[0][0]) < 540
