@@ -76,7 +76,9 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
  * (qrcp_online), 2 RandomizedQuadratureKhatriRao (rand_sketch); 3 (CVT)
  * returns status 1 (see mpisdf_select_cvt); 4 and 5 (density sketches)
  * need mpisdf_select_importance.
- * precision (rand_sketch only): 0 fp64, 1 fp32, 2 bf16, 3 fp8 e4m3.
+ * precision (rand_sketch only): 0 fp64, 1 fp32 (rejected), 2 bf16, 3 fp8 e4m3
+ * (two-term phi), 4 split bf16 (hi.hi + lo.hi + hi.lo, sketch digits in four
+ * planes).
  * axis_weights: NULL or naxis_weights == naxes finite positive factors.
  * Writes original grid indices in pivot order to out[0..*count).
  * diagnostics: NULL or double[24] (see mp_isdf_amd/csrc/select.hpp). */
@@ -150,16 +152,26 @@ int mpisdf_kr_sketch_f64_device(const double* phi, int64_t count, int64_t d, con
                                 int64_t ell, const double* sw, double* y, int64_t ldy,
                                 unsigned long long* bad, void* stream);
 
-/* Reduced-precision operands: ut/vt[s*ldo + a] from u/v (dtype 2 bf16, 3 fp8
- * scaled by 32), zero-padded to ldo (a multiple of 16). */
+/* Reduced-precision Omega operands from u/v, in segments of ldo values (ldo
+ * >= d, a multiple of 16, zero-padded past d): dtype 2 bf16 ut[s*ldo + a];
+ * 3 fp8 (e4m3 times 32) [u | u], row pitch 2 ldo; 4 split bf16 [hi | hi | lo]
+ * (u = hi + lo), row pitch 3 ldo. */
 int mpisdf_sketch_prepare_omega(const double* u, const double* v, int64_t d, int64_t ell, int64_t ldo, int dtype,
                                 void* ut, void* vt, void* stream);
+/* Split-BF16 rows out[g*3*ldo + {0, ldo, 2 ldo} + a] = {hi, lo, hi} of
+ * phi[g*ld + a] (FP64), so that mpisdf_kr_sketch_device (dtype 2, d = 3 ldo)
+ * with the dtype-4 Omega sums hi.hi + lo.hi + hi.lo. */
+int mpisdf_sketch_split_bf16x3(const double* phi, int64_t ld, int64_t count, int64_t d, void* out, int64_t ldo,
+                               void* stream);
 /* BF16 point scales scale[g] = sw[g] / sqrt(ell). */
 int mpisdf_sketch_scales_bf16(const double* sw, int64_t count, int64_t ell, float* scale, void* stream);
-/* FP8 rows with per-point power-of-two scales; scale[g] absorbs them. */
+/* FP8 rows [hi | lo] (row pitch 2 ldo; hi = e4m3(x), lo = e4m3(x - hi) on the
+ * point's one power-of-two scale); scale[g] absorbs the scales.  Entries past
+ * d of each segment are not written. */
 int mpisdf_sketch_quantize_fp8(const float* phi, int64_t ld, int64_t count, int64_t d, const double* sw,
                                int64_t ell, uint8_t* out, int64_t ldo, float* scale, void* stream);
-/* MFMA sketch (dtype 2 bf16, 3 fp8): FP32 y[g*ldy + s], ynorm2[g] += ||y_g||^2. */
+/* MFMA sketch (dtype 2 bf16, 3 fp8): FP32 y[g*ldy + s], ynorm2[g] += ||y_g||^2;
+ * d is the K length (all segments of the operands' rows). */
 int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi,
                             int64_t ldphi, int64_t count, const float* scale, float* y, int64_t ldy,
                             double* ynorm2, unsigned long long* bad, void* stream);
@@ -170,32 +182,34 @@ int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t e
  * W = Qb^T Y with Qb[m x b] column-major FP64 and Y[m x n] (ldy);
  * y_f32 = 0: FP64 Y, FP64 MFMA; 1: FP32 Y, FP64 MFMA (exact products);
  * 2: FP32 Y, split-BF16 MFMA with FP64 chunk accumulation;
- * 3: FP32 Y, INT8 MFMA on base-128 digit splits (~1e-10 relative; m < 65536).
+ * 3 / 4: FP32 Y, INT8 MFMA on three / four base-128 digit planes of it
+ * (~1e-10 relative; m < 65536).
  * proj2[j] += sum_t W[t,j]^2 and, if energy != NULL, energy[t] += sum_j W[t,j]^2.
  * W is never stored. */
 int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, int y_f32,
                             int64_t ldy, double* proj2, double* energy, void* stream);
 
 /* In place: FP32 column j of y (pitch 4 P bytes = P floats, P a multiple of
- * 64 >= m, m <= 36864) becomes three signed base-128 digits per entry,
- * y_hat_j[i] = scale[j] sum_{r<3} d_r[i] 128^-(r+1) with scale[j] a power of
- * two >= 2 ||y_j|| (each entry within 2^-22 scale[j] of y), digit r of entry
- * i at byte 192 (i / 64) + 64 r + i % 64 of the column (zero past m and in
- * the column's last P bytes); ynorm2[j] holds ||y_j||^2 on entry and
+ * 64 >= m, m <= 36864) becomes planes (3 or 4) signed base-128 digits per
+ * entry, y_hat_j[i] = scale[j] sum_{r<planes} d_r[i] 128^-(r+1) with scale[j]
+ * a power of two >= 2 ||y_j|| (each entry within 2^-(7 planes + 1) scale[j]
+ * of y), digit r of entry i at byte 64 planes (i / 64) + 64 r + i % 64 of the
+ * column (zero past m and in the column's last (4 - planes) P bytes);
+ * ynorm2[j] holds ||y_j||^2 on entry and
  * ||y_hat_j||^2 on return.  The digit form of the sketch that the reduced-precision selection
  * factorises (no reference counterpart: the reference factorises an FP64
  * sketch, cpp/src/anchor_select_randomized_id.cpp:320-326). */
-int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
-                                   void* stream);
+int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, int32_t planes, double* ynorm2,
+                                   double* scale, void* stream);
 
 /* Column-pivoted QR pivots of a device-resident sketch (replaces qrcp_cpu /
  * qrcp_magma, cpp/src/anchor_select_qrcp.cpp:36-90, as called from
  * cpp/src/anchor_select_randomized_id.cpp:320-326): dgeqp3's greedy pivot
  * order (max residual norm, lowest index on ties) of Y[m x n] held as
  * ytype 0 FP64 or 1 FP32 column-major (ldy elements; ynorm2 and yscale may
- * be NULL), or 4 digits from mpisdf_digitize_columns_device (ldy = P, the
- * column pitch in floats; ynorm2 and yscale as it wrote them; folds on INT8
- * MFMA).
+ * be NULL), or 4 / 5 digits (three / four planes) from
+ * mpisdf_digitize_columns_device (ldy = P, the column pitch in floats; ynorm2
+ * and yscale as it wrote them; folds on INT8 MFMA).
  * Stops before a pivot with |R_ii| <= stop_rel |R_00| or after kmax pivots.
  * Writes 0-based pivots and |R_ii| (host arrays of kmax entries; rdiag may
  * be NULL) and their count. */

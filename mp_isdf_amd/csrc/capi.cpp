@@ -194,7 +194,7 @@ int select_impl(const mpisdf_context* ctx, const double* points, const double* w
         if (naxes <= 0 || !axes) throw std::invalid_argument("axis list must not be empty");
         if (algorithm < 0 || algorithm > 5) throw std::invalid_argument("unknown anchor selection algorithm");
         if (algorithm == 3) throw std::invalid_argument("CVT selection is not part of this build");
-        if (precision < 0 || precision > 3) throw std::invalid_argument("precision must be 0..3");
+        if (precision < 0 || precision > 4) throw std::invalid_argument("precision must be 0..4");
         if (batch_size < 0 || sketch_size < 0 || oversampling < 0 || max_rank < 0)
             throw std::invalid_argument("sizes must be non-negative");
         mpisdf::SelectArgs a;
@@ -302,8 +302,16 @@ extern "C" int mpisdf_sketch_prepare_omega(const double* u, const double* v, int
 {
     return guard([&] {
         if (ldo < d || ldo % 16) throw std::invalid_argument("ldo must be >= d and a multiple of 16");
-        mpisdf::prepare_omega_lowp(u, v, d, ell, ldo, static_cast<mpisdf::DType>(dtype), ut, vt,
-                                   static_cast<hipStream_t>(stream));
+        mpisdf::prepare_omega_lowp(u, v, d, ell, ldo, dtype, ut, vt, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int mpisdf_sketch_split_bf16x3(const double* phi, int64_t ld, int64_t count, int64_t d, void* out,
+                                          int64_t ldo, void* stream)
+{
+    return guard([&] {
+        if (ldo < d || ldo % 16 || ld < d) throw std::invalid_argument("invalid split row strides");
+        mpisdf::split_rows_bf16x3(phi, ld, count, d, ldo, out, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -338,10 +346,11 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
                                        int64_t ldy, double* proj2, double* energy, void* stream)
 {
     return guard([&] {
-        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 3)
+        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 4)
             throw std::invalid_argument("invalid fold arguments");
-        if (y_f32 == 3) {
-            // Digits of a copy of Y (pitch P = m rounded up to 64), then the INT8 fold.
+        if (y_f32 == 3 || y_f32 == 4) {
+            // Digits (3 or 4 planes) of a copy of Y (pitch P = m rounded up to
+            // 64), then the INT8 fold.
             mpisdf::FoldI8Scratch scratch;
             auto st = static_cast<hipStream_t>(stream);
             const int64_t P = (m + mpisdf::kDigitSlice - 1) / mpisdf::kDigitSlice * mpisdf::kDigitSlice;
@@ -350,9 +359,10 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
             mpisdf::launch_column_norms2_f32(static_cast<const float*>(Y), m, n, ldy, yn.ptr, st);
             MPISDF_HIP(hipMemcpy2DAsync(yd.ptr, P * sizeof(float), Y, ldy * sizeof(float), m * sizeof(float), n,
                                         hipMemcpyDeviceToDevice, st));
-            mpisdf::digitize_columns(yd.ptr, m, n, P, yn.ptr, sc.ptr, st);
-            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, sc.ptr, proj2,
-                                        energy, scratch, st);
+            const int planes = y_f32 == 4 ? 4 : 3;
+            mpisdf::digitize_columns(yd.ptr, m, n, P, planes, yn.ptr, sc.ptr, st);
+            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, planes, sc.ptr,
+                                        proj2, energy, scratch, st);
             MPISDF_HIP(hipStreamSynchronize(st));
         } else if (y_f32 == 2) {
             mpisdf::FoldScratch scratch;
@@ -368,12 +378,12 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
     });
 }
 
-extern "C" int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2,
-                                              double* scale, void* stream)
+extern "C" int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, int32_t planes,
+                                              double* ynorm2, double* scale, void* stream)
 {
     return guard([&] {
         if (m <= 0 || n < 0 || !y || !ynorm2 || !scale) throw std::invalid_argument("invalid digitise arguments");
-        mpisdf::digitize_columns(y, m, n, P, ynorm2, scale, static_cast<hipStream_t>(stream));
+        mpisdf::digitize_columns(y, m, n, P, planes, ynorm2, scale, static_cast<hipStream_t>(stream));
         MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
     });
 }
@@ -385,14 +395,14 @@ extern "C" int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n
     return guard([&] {
         if (!y || !pivots || !count || m <= 0 || n <= 0 || kmax <= 0 || !(stop_rel >= 0))
             throw std::invalid_argument("invalid QRCP arguments");
-        if (ytype != mpisdf::kF64 && ytype != mpisdf::kF32 && ytype != mpisdf::kI8Digits)
-            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32) or 4 (digit planes)");
+        if (ytype != mpisdf::kF64 && ytype != mpisdf::kF32 && !mpisdf::is_digit_type((mpisdf::DType)ytype))
+            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32), 4 or 5 (3 or 4 digit planes)");
         mpisdf::QrcpOptions opt;
         opt.kmax = kmax;
         opt.stop_rel = stop_rel;
         opt.candidates = std::min<int64_t>(n, 2048);
         opt.ynorm2 = ynorm2;
-        if (ytype == mpisdf::kI8Digits) {
+        if (mpisdf::is_digit_type((mpisdf::DType)ytype)) {
             opt.yscale = yscale;
             opt.margin = mpisdf::kFoldI8Margin;
         }

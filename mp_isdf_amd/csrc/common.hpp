@@ -44,9 +44,11 @@ const char* last_error();
 void ensure_device();  // throws NoDevice when no HIP device is present
 
 // Output element types of the AO and sketch kernels.
-// kI8Digits: a sketch held as three signed base-128 digit planes per column
-// (digitize_columns), with a per-column power-of-two scale.
-enum DType : int { kF64 = 0, kF32 = 1, kBF16 = 2, kFP8 = 3, kI8Digits = 4 };
+// kI8Digits / kI8Digits4: a sketch held as three / four signed base-128 digit
+// planes per column (digitize_columns), with a per-column power-of-two scale.
+enum DType : int { kF64 = 0, kF32 = 1, kBF16 = 2, kFP8 = 3, kI8Digits = 4, kI8Digits4 = 5 };
+__host__ __device__ inline bool is_digit_type(DType t) { return t == kI8Digits || t == kI8Digits4; }
+__host__ __device__ inline int digit_planes_of(DType t) { return t == kI8Digits4 ? 4 : 3; }
 
 // Derivative modes of AO evaluation.
 enum AoMode : int { kValues = 0, kGradient = 1, kValuesGradient = 2 };

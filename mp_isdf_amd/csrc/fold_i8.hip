@@ -3,24 +3,27 @@
 //
 //     W = Qb^T Y,  proj2[j] += sum_t W[t,j]^2  (energy[t] += sum_j W[t,j]^2)
 //
-// Y is held as three digit planes per column (digitize_columns, in place over
-// the FP32 sketch, same footprint): with the power of two s_j >= 2 ||y_j||,
-//     y_hat[i] = s_j sum_{r<3} d_r[i] 128^-(r+1),  |d_r| <= 64,
-// which rounds each entry to within 2^-22 s_j (3e-5 to 6e-5 of ||y_j|| in RMS
-// over 13,200 rows, as s_j < 4 ||y_j||; well below the rounding noise of a
-// BF16 or FP8 sketch).
+// Y is held as NY = 3 or 4 digit planes per column (digitize_columns, in place
+// over the FP32 sketch, same footprint): with the power of two s_j >= 2 ||y_j||,
+//     y_hat[i] = s_j sum_{r<NY} d_r[i] 128^-(r+1),  |d_r| <= 64,
+// which rounds each entry to within 2^-(7 NY + 1) s_j: with three planes 3e-5
+// to 6e-5 of ||y_j|| in RMS over 13,200 rows (s_j < 4 ||y_j||; well below the
+// rounding noise of a BF16 or FP8 sketch), with four ~2e-7 (below that of a
+// split-BF16 sketch).
 // The QRCP then factorises y_hat: candidate columns are rebuilt from the same
 // digits, and ||y_hat_j||^2 replaces ||y_j||^2.
 //
 // Products use the splitting method of Ozaki, Ogita, Oishi and Rump (2012) in
 // the INT8 form of Ootomo, Ozaki and Yokota (2024): row t of Qb^T is written
-// as five digits with its own power of two u_t >= 2 max|q_t|; the 14 digit
-// pairs with p + r <= 5 run on v_mfma_i32_32x32x32_i8, pairs of equal p + r
-// sharing an exact INT32 accumulator (|sum| < 4 * 64^2 * m < 2^31 for
-// m < 2^16), and the six accumulators are combined in FP64:
+// as five digits with its own power of two u_t >= 2 max|q_t|; the digit
+// pairs with p + r <= 5 (14 with three Y planes, 17 with four) run on
+// v_mfma_i32_32x32x32_i8, pairs of equal p + r sharing an exact INT32
+// accumulator (at most four pairs: |sum| < 4 * 64^2 * m < 2^31 for m < 2^16),
+// and the six accumulators are combined in FP64:
 //     W = u_t s_j sum_g 128^-(g+2) acc_g.
-// The dropped pair (4, 2) and the truncation of q leave ~1e-11 of ||y_j||^2
-// in the folded sums (measured), covered by the QRCP's relative margin.
+// The dropped pairs (p + r >= 6) and the truncation of q leave ~1e-11 of
+// ||y_j||^2 in the folded sums (measured), covered by the QRCP's relative
+// margin.
 //
 // v_mfma_i32_32x32x32_i8 runs at about 1.7x the rate of the 16x16x64 form on
 // gfx950.  Q and Y digit slices are staged through double-buffered LDS, with
@@ -37,7 +40,6 @@ namespace {
 typedef int i4v __attribute__((ext_vector_type(4)));
 
 constexpr int ND = 5;            // digits of q
-constexpr int NY = kDigitPlanes;  // digits of y (planes per column)
 constexpr int NG = 6;            // accumulator groups, p + r = 0..5
 constexpr int FBM = 128;         // rows per workgroup
 constexpr int FBN = 64;          // columns per workgroup (two tiles of 32)
@@ -98,12 +100,13 @@ __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int6
 
 // In place: column j of y (FP32, m entries, column pitch 4 P bytes with P a
 // multiple of 64 >= m) becomes NY digits per entry at digit_offset (zero
-// past m and in the column's last P bytes); scale[j] = s_j and
+// past m and in the column's last (4 - NY) P bytes); scale[j] = s_j and
 // ynorm2[j] = ||y_hat_j||^2.  One workgroup of kDigitizeThreads per column;
 // each thread holds its entries in registers between the read of the whole
 // column and the in-place writes (one barrier apart).
 constexpr int kDigitizeThreads = 1024;
 constexpr int kDigitizeEntries = (int)(kDigitRowsMax / kDigitizeThreads);  // entries per thread, P <= 36864
+template <int NY>
 __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_kernel(float* __restrict__ y, int64_t m,
                                                                             int64_t P, double* __restrict__ ynorm2,
                                                                             double* __restrict__ scale)
@@ -136,11 +139,12 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_kernel(floa
                 v *= 128.0f;
                 const float dgt = rintf(v);
                 v -= dgt;  // exact (Sterbenz)
-                planes[digit_offset(i, r)] = (int8_t)dgt;
+                planes[digit_offset(i, r, NY)] = (int8_t)dgt;
                 w *= 1.0 / 128.0;
                 yh += (double)dgt * w;
             }
-            planes[NY * P + i] = 0;  // the column's unused last P bytes
+#pragma unroll
+            for (int r = NY; r < 4; ++r) planes[(int64_t)r * P + i] = 0;  // the column's unused last bytes
             yh *= sc;
             acc += yh * yh;
         }
@@ -171,7 +175,7 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // lane reads 32 contiguous bytes of a plane per slice.  Rows past b carry
 // zero digits; there are no branches in the MFMA sequence (a branch would
 // make the compiler drain every outstanding load at its join).
-template <bool ENERGY, int NR>
+template <bool ENERGY, int NR, int NY>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
                                                                    int64_t bpad, int64_t b, int64_t m0,
@@ -359,6 +363,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 // rows: Q^T rows of this launch's group (1..128).  Groups of more than 64
 // rows run on 8-wave workgroups (NR = 4, one per CU); smaller groups on
 // 4-wave workgroups (NR = 2, two per CU), which multiply no zero rows.
+template <int NY>
 void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs,
                     int64_t ldq, int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P,
                     const double* yscale, int64_t m, int64_t n, double* proj2, double* en)
@@ -366,7 +371,7 @@ void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const i
     if (rows <= 0 || rows > FBM) throw std::logic_error("INT8 fold row group out of range");
     double* const e = energy ? en : nullptr;
 #define MPISDF_FOLD(E, NR)                                                                                    \
-    hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
+    hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR, NY>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
                        yscale, m, n, proj2, e)
     if (rows > 64) {
         if (energy) MPISDF_FOLD(true, 4);
@@ -381,21 +386,28 @@ void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const i
 
 }  // namespace
 
-void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale, hipStream_t stream)
+void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, double* ynorm2, double* scale,
+                      hipStream_t stream)
 {
+    if (planes != 3 && planes != 4) throw std::invalid_argument("digit planes must be 3 or 4");
     if (n <= 0) return;
     if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     if (P > kDigitRowsMax) throw std::invalid_argument("digitize_columns holds one column in registers (pitch <= 36864)");
     if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
-    hipLaunchKernelGGL(digitize_columns_kernel, dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream, y, m, P, ynorm2,
-                       scale);
+    if (planes == 4)
+        hipLaunchKernelGGL(digitize_columns_kernel<4>, dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream, y, m, P,
+                           ynorm2, scale);
+    else
+        hipLaunchKernelGGL(digitize_columns_kernel<3>, dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream, y, m, P,
+                           ynorm2, scale);
     MPISDF_HIP(hipGetLastError());
 }
 
 void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const int8_t* Yd,
-                         int64_t P, const double* yscale, double* proj2, double* energy, FoldI8Scratch& scratch,
-                         hipStream_t stream)
+                         int64_t P, int planes, const double* yscale, double* proj2, double* energy,
+                         FoldI8Scratch& scratch, hipStream_t stream)
 {
+    if (planes != 3 && planes != 4) throw std::invalid_argument("digit planes must be 3 or 4");
     if (b <= 0 || n <= 0) return;
     if (m >= (int64_t(1) << 16)) throw std::invalid_argument("the INT8 fold needs fewer than 65536 sketch rows");
     if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
@@ -416,8 +428,12 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     // block order.
     for (int64_t m0 = 0; m0 < b; m0 += FBM) {
         const int64_t rows = std::min<int64_t>(FBM, b - m0);
-        launch_fold_i8(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr, scratch.scales.ptr, ldq,
-                       bpad, b, m0, Yd, P, yscale, m, n, proj2, scratch.part.ptr);
+        if (planes == 4)
+            launch_fold_i8<4>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
+                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, yscale, m, n, proj2, scratch.part.ptr);
+        else
+            launch_fold_i8<3>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
+                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, yscale, m, n, proj2, scratch.part.ptr);
         if (energy) launch_sum_partials(scratch.part.ptr, gx, rows, energy + m0, true, stream);
     }
 }

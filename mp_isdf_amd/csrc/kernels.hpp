@@ -100,24 +100,28 @@ void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, 
                              const double* ynorm2 = nullptr, double* err2 = nullptr);
 
 // Sketch in digit-plane form (fold_i8.hip): in place, FP32 column j
-// (pitch 4 P bytes, P a multiple of 64 >= m) becomes kDigitPlanes signed
-// base-128 digits per entry, y_hat = scale[j] sum_r d_r 128^-(r+1), stored
-// slice by slice (digit_offset); ynorm2[j] (in: of the FP32 column, out: of
-// y_hat).  m <= 36864.
-void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale, hipStream_t stream);
+// (pitch 4 P bytes, P a multiple of 64 >= m) becomes ``planes`` (3 or 4)
+// signed base-128 digits per entry, y_hat = scale[j] sum_r d_r 128^-(r+1),
+// stored slice by slice (digit_offset); ynorm2[j] (in: of the FP32 column,
+// out: of y_hat).  m <= 36864.
+void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, double* ynorm2, double* scale,
+                      hipStream_t stream);
 constexpr int64_t kDigitRowsMax = 36864;  // digitize_columns holds one FP32 column in registers
 constexpr int64_t kDigitSlice = 64;       // entries per slice; P is a multiple of it
-// Digits per entry: three base-128 digits round an entry to 2^-22 s_j, 3e-5
+// Digits per entry.  Three base-128 digits round an entry to 2^-22 s_j, 3e-5
 // to 6e-5 of the column norm in RMS over 13,200 rows, far below the noise of
-// a BF16 or FP8 sketch.
+// a BF16 or FP8 sketch.  Four (2^-29 s_j, ~2e-7 of the norm) keep a split-BF16
+// sketch (~1e-7) intact: its greedy pivots then agree with the FP64 form's on
+// >= 98% of the anchors, where three digits alone drop the agreement to ~93%.
 constexpr int kDigitPlanes = 3;
+constexpr int kDigitPlanesMax = 4;
 // Byte offset, within its column's 4 P bytes, of digit r of entry i: each
-// slice of 64 entries keeps its digit planes in one 192-byte run and the runs
-// follow each other, so a column's digits are its first 3 P bytes (the last
-// P are zero) and the fold streams them contiguously.
-__host__ __device__ inline int64_t digit_offset(int64_t i, int r)
+// slice of 64 entries keeps its digit planes in one run of 64 planes bytes and
+// the runs follow each other, so a column's digits are its first planes * P
+// bytes (the rest, if any, zero) and the fold streams them contiguously.
+__host__ __device__ inline int64_t digit_offset(int64_t i, int r, int planes)
 {
-    return (i / kDigitSlice) * (kDigitPlanes * kDigitSlice) + r * kDigitSlice + (i % kDigitSlice);
+    return (i / kDigitSlice) * (planes * kDigitSlice) + r * kDigitSlice + (i % kDigitSlice);
 }
 
 // QRCP fold on the digit planes with INT8 MFMA (digit-pair products in exact
@@ -131,7 +135,7 @@ struct FoldI8Scratch {
 // whole factorisation with the INT8 fold; enters the keys as a margin.
 constexpr double kFoldI8Margin = 1e-9;
 void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const int8_t* Yd,
-                         int64_t P, const double* yscale, double* proj2, double* energy, FoldI8Scratch& scratch,
-                         hipStream_t stream);
+                         int64_t P, int planes, const double* yscale, double* proj2, double* energy,
+                         FoldI8Scratch& scratch, hipStream_t stream);
 
 }  // namespace mpisdf

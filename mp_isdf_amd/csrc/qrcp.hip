@@ -86,23 +86,23 @@ __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __r
     }
 }
 
-// y_hat_j[i] = scale[j] sum_r d_r[i] 128^-(r+1), exact in FP64; colp is
-// column j's first byte.
-__device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t i, double scale)
+// y_hat_j[i] = scale[j] sum_{r < planes} d_r[i] 128^-(r+1), exact in FP64;
+// colp is column j's first byte.
+__device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t i, double scale, int planes)
 {
     double v = 0;
-#pragma unroll
-    for (int r = kDigitPlanes - 1; r >= 0; --r) v = (v + (double)colp[digit_offset(i, r)]) * (1.0 / 128.0);
+    for (int r = planes - 1; r >= 0; --r) v = (v + (double)colp[digit_offset(i, r, planes)]) * (1.0 / 128.0);
     return v * scale;
 }
 
-__global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, const double* __restrict__ yscale,
-                                    int64_t j0, int64_t m, int64_t nc, double* __restrict__ out)
+__global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, int planes,
+                                    const double* __restrict__ yscale, int64_t j0, int64_t m, int64_t nc,
+                                    double* __restrict__ out)
 {
     const int64_t total = m * nc;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = i / m, r = i - j * m;
-        out[i] = digit_value(yd + (j0 + j) * 4 * P, r, yscale[j0 + j]);
+        out[i] = digit_value(yd + (j0 + j) * 4 * P, r, yscale[j0 + j], planes);
     }
 }
 
@@ -187,7 +187,7 @@ __global__ void fill_candidates_kernel(const void* __restrict__ y, int ytype, in
         if (os >= 0) v = rc_prev[i + (int64_t)os * ldr];
         else if (ytype == kF64) v = static_cast<const double*>(y)[i + j * ldy];
         else if (ytype == kF32) v = (double)static_cast<const float*>(y)[i + j * ldy];
-        else v = digit_value(static_cast<const int8_t*>(y) + j * 4 * ldy, i, yscale[j]);
+        else v = digit_value(static_cast<const int8_t*>(y) + j * 4 * ldy, i, yscale[j], digit_planes_of((DType)ytype));
         rc[i + c * ldr] = v;
     }
 }
@@ -529,7 +529,8 @@ int grid_for(int64_t work, int threads = 256, int64_t cap = 65536)
 QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t ldy, const QrcpOptions& opt,
                        hipStream_t s)
 {
-    const bool digits = ytype == kI8Digits;
+    const bool digits = is_digit_type(ytype);
+    const int planes = digit_planes_of(ytype);
     if (ytype != kF64 && ytype != kF32 && !digits) throw std::invalid_argument("QRCP input must be f64, f32 or digits");
     if (digits && (!opt.ynorm2 || !opt.yscale || !(opt.margin > 0)))
         throw std::invalid_argument("QRCP on digit planes needs device norms, scales and a positive margin");
@@ -620,8 +621,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         fold_timer.begin(s);
         res.fold_flops += 2.0 * (double)b * (double)m * (double)n;
         if (digits)
-            fold_projections_i8(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const int8_t*>(y), ldy, opt.yscale,
-                                proj2.ptr, energy_out ? energy_out + folded : nullptr, i8_scratch, s);
+            fold_projections_i8(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const int8_t*>(y), ldy, planes,
+                                opt.yscale, proj2.ptr, energy_out ? energy_out + folded : nullptr, i8_scratch, s);
         else if (split)
             fold_projections_bf16x3(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const float*>(y), ldy, proj2.ptr,
                                     energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
@@ -792,7 +793,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             const int64_t cnt = std::min(nc, n - j0);
             if (digits)
                 hipLaunchKernelGGL(widen_digits_kernel, dim3(grid_for(m * cnt)), dim3(256), 0, s,
-                                   static_cast<const int8_t*>(y), ldy, opt.yscale, j0, m, cnt, R.ptr);
+                                   static_cast<const int8_t*>(y), ldy, planes, opt.yscale, j0, m, cnt, R.ptr);
             else if (ytype == kF64)
                 hipLaunchKernelGGL(widen_kernel<double>, dim3(grid_for(m * cnt)), dim3(256), 0, s,
                                    static_cast<const double*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);

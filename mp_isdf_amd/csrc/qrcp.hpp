@@ -31,8 +31,9 @@ struct QrcpResult {
 };
 
 // Y is m x n column-major with leading dimension ldy, element type f64 or f32;
-// or, for kI8Digits, column j at byte offset 4 ldy j holding three digits per
-// entry (digit_offset; y_hat_j = yscale[j] sum_r d_r 128^-(r+1)), folded with INT8
+// or, for kI8Digits / kI8Digits4, column j at byte offset 4 ldy j holding three
+// / four digits per entry (digit_offset; y_hat_j = yscale[j] sum_r d_r
+// 128^-(r+1)), folded with INT8
 // MFMA and factorised as y_hat (candidate columns are rebuilt from the same
 // digits; opt.margin must cover the fold's error).  Y is read only.  Pivot decisions use exact FP64 residual norms of the
 // candidate columns (LAPACK dgeqp3 pivot semantics: max norm, lowest index on

@@ -255,9 +255,15 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         throw std::invalid_argument("requested anchor_rank exceeds eligible candidates or physical pair rank");
     if (n == 0) return {};
     if (a.precision == kPrecF32)
-        throw std::invalid_argument("precision 'fp32' is not provided; use fp64, bf16 or fp8");
+        throw std::invalid_argument("precision 'fp32' is not provided; use fp64, bf16x3, bf16 or fp8");
     const bool lowp = a.precision != kPrecF64;
-    const DType in_type = a.precision == kPrecFP8 ? kFP8 : kBF16;
+    // MFMA operand form of the reduced-precision sketch (sketch_mfma.hpp):
+    // BF16, two-term FP8 or split BF16 (hi.hi + lo.hi + hi.lo).
+    const int form = a.precision == kPrecFP8 ? kSketchFP8 : a.precision == kPrecBF16x3 ? kSketchBF16x3 : kSketchBF16;
+    const DType in_type = form == kSketchFP8 ? kFP8 : kBF16;
+    // The split-BF16 sketch is accurate to ~1e-7 of each column, so its digit
+    // form keeps four planes (three would round it to ~3e-5).
+    const int planes = form == kSketchBF16x3 ? 4 : 3;
 
     std::vector<double> hu, hv;
     make_omega(d, ell, a.seed, hu, hv);
@@ -279,8 +285,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     DeviceBuffer<double> y64;
     DeviceBuffer<float> y32;
     // The reduced-precision forms fold on INT8 digit planes of the FP32 sketch
-    // (three digits per entry in place of each column), which needs a pitch
-    // that is a multiple of 64 and ell within the digit kernels' limits.
+    // (three or four digits per entry in place of each column), which needs a
+    // pitch that is a multiple of 64 and ell within the digit kernels' limits.
     const bool use_digits = lowp && ell <= kDigitRowsMax;
     const int64_t ldy32 = use_digits ? (ell + kDigitSlice - 1) / kDigitSlice * kDigitSlice
                                      : (ell + 3) / 4 * 4;  // 256- or 16-byte aligned FP32 columns
@@ -306,26 +312,35 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         t_sketch_kernel = scratch.timer.seconds();
         sketch_launches = (double)scratch.timer.launches();
     } else {
-        // Reduced precision: operands padded to a multiple of 16 features.
+        // Reduced precision: operand segments padded to a multiple of 16
+        // features, sketch_segments(form) of them along K.
         const int64_t dp = (d + 15) / 16 * 16;
+        const int64_t dk = sketch_segments(form) * dp;
         const size_t esz = in_type == kFP8 ? 1 : 2;
-        DeviceBuffer<uint8_t> ut((size_t)ell * dp * esz), vt((size_t)ell * dp * esz);
-        prepare_omega_lowp(u.ptr, v.ptr, d, ell, dp, in_type, ut.ptr, vt.ptr, s);
+        DeviceBuffer<uint8_t> ut((size_t)ell * dk * esz), vt((size_t)ell * dk * esz);
+        prepare_omega_lowp(u.ptr, v.ptr, d, ell, dp, form, ut.ptr, vt.ptr, s);
         y32.resize((size_t)ldy32 * n);
         const bool ao_only = spec.kinds.size() == 1 && spec.kinds[0] == 0;
         LaunchTimer timer;
         const int64_t batch = std::min<int64_t>(n, 1 << 16);
-        DeviceBuffer<uint8_t> phi((size_t)batch * dp * esz);
+        DeviceBuffer<uint8_t> phi((size_t)batch * dk * esz);
         MPISDF_HIP(hipMemsetAsync(phi.ptr, 0, phi.count, s));  // padding columns stay zero
         DeviceBuffer<float> phi32, scale(batch);
         DeviceBuffer<double> ynorm_part;
         DeviceBuffer<double> feat;
-        if (!ao_only) feat.resize((size_t)batch * d);
+        if (!ao_only || form == kSketchBF16x3) feat.resize((size_t)batch * d);
         if (in_type == kFP8) phi32.resize((size_t)batch * d);
         for (int64_t start = 0; start < n; start += batch) {
             const int64_t count = std::min(batch, n - start);
             const double* xyz = d_xyz.ptr + 3 * start;
-            if (in_type == kBF16) {
+            if (form == kSketchBF16x3) {
+                if (ao_only) launch_ao_eval(ctx, xyz, count, kValues, kF64, feat.ptr, d, count * d, s);
+                else
+                    evaluate_axis_device(ctx, spec, xyz, count, scales.empty() ? nullptr : scales.data(), a.mo_coeff,
+                                         a.nocc, feat.ptr, ws, s);
+                split_rows_bf16x3(feat.ptr, d, count, d, dp, phi.ptr, s);
+                bf16_point_scales(sw.ptr + start, count, ell, scale.ptr, s);
+            } else if (in_type == kBF16) {
                 if (ao_only) launch_ao_eval(ctx, xyz, count, kValues, kBF16, phi.ptr, dp, count * dp, s);
                 else {
                     evaluate_axis_device(ctx, spec, xyz, count, scales.empty() ? nullptr : scales.data(), a.mo_coeff,
@@ -343,7 +358,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
                 quantize_fp8_rows(phi32.ptr, d, count, d, sw.ptr + start, ell, phi.ptr, dp, scale.ptr, s);
             }
             timer.begin(s);
-            kr_sketch_lowp(in_type, ut.ptr, vt.ptr, ell, dp, phi.ptr, dp, count, scale.ptr, y32.ptr + start * ldy32,
+            kr_sketch_lowp(in_type, ut.ptr, vt.ptr, ell, dk, phi.ptr, dk, count, scale.ptr, y32.ptr + start * ldy32,
                            ldy32, ynorm2.ptr + start, bad.ptr, ynorm_part, s);
             timer.end(s);
         }
@@ -355,7 +370,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     DeviceBuffer<double> yscale;
     if (use_digits) {
         yscale.resize(n);
-        digitize_columns(y32.ptr, ell, n, ldy32, ynorm2.ptr, yscale.ptr, s);
+        digitize_columns(y32.ptr, ell, n, ldy32, planes, ynorm2.ptr, yscale.ptr, s);
     }
     const double t_sketch = seconds_since(started);
 
@@ -384,7 +399,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         opt.yscale = yscale.ptr;
         opt.margin = kFoldI8Margin;
     }
-    const QrcpResult r = use_digits ? qrcp_device(y32.ptr, kI8Digits, ell, n, ldy32, opt, s)
+    const QrcpResult r = use_digits ? qrcp_device(y32.ptr, planes == 4 ? kI8Digits4 : kI8Digits, ell, n, ldy32, opt, s)
                          : lowp     ? qrcp_device(y32.ptr, kF32, ell, n, ldy32, opt, s)
                                     : qrcp_device(y64.ptr, kF64, ell, n, ldy64, opt, s);
     const double t_qrcp = seconds_since(tq);
@@ -439,9 +454,11 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         diag[kDiagFoldFlops] = r.fold_flops;
         diag[kDiagFoldLaunches] = (double)r.folds;
         diag[kDiagSketchSeconds] = t_sketch_kernel;
-        diag[kDiagSketchFlops] = 4.0 * (double)ell * (double)d * (double)n;
+        // MFMA work of the sketch: 4 ell d n per product term (three for the
+        // split-BF16 form, two for the two-term FP8 form).
+        diag[kDiagSketchFlops] = 4.0 * (double)ell * (double)d * (double)n * (lowp ? sketch_segments(form) : 1);
         diag[kDiagSketchLaunches] = sketch_launches;
-        diag[kDiagFoldKind] = use_digits ? 2.0 : (lowp ? 1.0 : 0.0);
+        diag[kDiagFoldKind] = use_digits ? (planes == 4 ? 3.0 : 2.0) : (lowp ? 1.0 : 0.0);
     }
     return out;
 }

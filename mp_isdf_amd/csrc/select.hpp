@@ -9,7 +9,7 @@ namespace mpisdf {
 
 // LocalAnchorSelectionAlgorithm values (cpp/include/enums.hpp:23-30); 3 is CVT.
 enum Algorithm : int { kDense = 0, kOnline = 1, kRandomized = 2, kDensity = 4, kDensityNormalized = 5 };
-enum Precision : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecFP8 = 3 };
+enum Precision : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecFP8 = 3, kPrecBF16x3 = 4 };
 
 struct SelectArgs {
     const double* points = nullptr;   // host [n][3]
@@ -56,7 +56,7 @@ enum Diag : int {
     kDiagSketchSeconds = 17,  // summed duration of the sketch kernel launches (HIP events)
     kDiagSketchFlops = 18,    // their algorithmic flops: sum 4 ell d count
     kDiagSketchLaunches = 19,
-    kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 INT8 MFMA on digit planes
+    kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 / 3 INT8 MFMA on 3 / 4 digit planes
     kDiagCount = 24
 };
 

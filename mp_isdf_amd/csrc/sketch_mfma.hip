@@ -12,7 +12,8 @@
 // Operands are K-contiguous: Ut[s * d + a], Vt[s * d + a] and
 // phi[g * ldphi + a].  BF16 and FP8 e4m3 (OCP) inputs, FP32 accumulation;
 // FP8 phi rows carry a per-point power-of-two scale chosen on quantisation,
-// Ut/Vt a global 2^5.
+// Ut/Vt a global 2^5.  The split forms lay their terms side by side along K
+// (sketch_mfma.hpp), so one K sum forms every product term.
 #include "sketch_mfma.hpp"
 #include "util.hpp"
 
@@ -253,8 +254,8 @@ __global__ __launch_bounds__(256) void kr_sketch_kernel(
 // ---------------------------------------------------------------------------
 // out[s * ldo + a] = cast(scale * in[a * ell + s]) for a < d, zero for d <= a < ldo
 template <class T>
-__global__ void transpose_cast_kernel(const double* __restrict__ in, int64_t d, int64_t ell, int64_t ldo, float scale,
-                                      T* __restrict__ out)
+__global__ void transpose_cast_kernel(const double* __restrict__ in, int64_t d, int64_t ell, int64_t ldo,
+                                      int64_t pitch, float scale, T* __restrict__ out)
 {
     __shared__ float tile[32][33];
     const int64_t a0 = (int64_t)blockIdx.y * 32, s0 = (int64_t)blockIdx.x * 32;
@@ -269,9 +270,9 @@ __global__ void transpose_cast_kernel(const double* __restrict__ in, int64_t d, 
         if (s < ell && a < ldo) {
             if constexpr (sizeof(T) == 1) {
                 const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(tile[tx][r], 0.f, 0, false);
-                out[s * ldo + a] = (uint8_t)(packed & 0xff);
+                out[s * pitch + a] = (uint8_t)(packed & 0xff);
             } else {
-                out[s * ldo + a] = static_cast<T>(tile[tx][r]);
+                out[s * pitch + a] = static_cast<T>(tile[tx][r]);
             }
         }
     }
@@ -298,10 +299,16 @@ __global__ void quantize_fp8_rows_kernel(const float* __restrict__ phi, int64_t 
         e = 8 - ex;           // mx * 2^e in [128, 256)
     }
     const float mul = ldexpf(1.f, e);
-    uint8_t* dst = out + g * ldo;
+    // Two terms on the one scale: hi = e4m3(x), lo = e4m3(x - hi) (x - hi is
+    // exact in FP32), at offsets 0 and ldo of the 2 ldo row.
+    uint8_t* dst = out + g * 2 * ldo;
     for (int64_t a = lane; a < d; a += 64) {
-        const int packed = __builtin_amdgcn_cvt_pk_fp8_f32(row[a] * mul, 0.f, 0, false);
-        dst[a] = (uint8_t)(packed & 0xff);
+        const float x = row[a] * mul;
+        const int hi = __builtin_amdgcn_cvt_pk_fp8_f32(x, 0.f, 0, false);
+        const float r = x - __builtin_amdgcn_cvt_f32_fp8(hi, 0);
+        const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(r, 0.f, 0, false);
+        dst[a] = (uint8_t)(hi & 0xff);
+        dst[ldo + a] = (uint8_t)(lo & 0xff);
     }
     if (lane == 0) scale[g] = (float)(sw[g] * inv_sqrt_ell * post * ldexp(1.0, -2 * e));
 }
@@ -325,6 +332,56 @@ __global__ void cast_rows_kernel(const double* __restrict__ in, int64_t count, i
     }
 }
 
+// BF16 splits x = hi + lo of FP64 values: hi = bf16(x), lo = bf16(x - hi).
+__device__ __forceinline__ void split_bf16(double x, __bf16& hi, __bf16& lo)
+{
+    hi = static_cast<__bf16>((float)x);
+    lo = static_cast<__bf16>((float)(x - (double)(float)hi));
+}
+
+// Split-BF16 phi rows: out[g * 3 ldo + {0, ldo, 2 ldo} + a] = {hi, lo, hi} of
+// in[g * ld + a] (zero for d <= a < ldo).
+__global__ void split_rows_bf16x3_kernel(const double* __restrict__ in, int64_t ld, int64_t count, int64_t d,
+                                         int64_t ldo, __bf16* __restrict__ out)
+{
+    const int64_t total = count * ldo;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = i / ldo, a = i - g * ldo;
+        __bf16 hi, lo;
+        split_bf16(a < d ? in[g * ld + a] : 0.0, hi, lo);
+        __bf16* row = out + g * 3 * ldo;
+        row[a] = hi;
+        row[ldo + a] = lo;
+        row[2 * ldo + a] = hi;
+    }
+}
+
+// Split-BF16 Omega rows: out[s * 3 ldo + {0, ldo, 2 ldo} + a] = {hi, hi, lo}
+// of in[a * ell + s] (zero for d <= a < ldo).
+__global__ void split_omega_bf16x3_kernel(const double* __restrict__ in, int64_t d, int64_t ell, int64_t ldo,
+                                          __bf16* __restrict__ out)
+{
+    __shared__ double tile[32][33];
+    const int64_t a0 = (int64_t)blockIdx.y * 32, s0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t a = a0 + r, s = s0 + tx;
+        tile[r][tx] = (a < d && s < ell) ? in[a * ell + s] : 0.0;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int64_t s = s0 + r, a = a0 + tx;
+        if (s < ell && a < ldo) {
+            __bf16 hi, lo;
+            split_bf16(tile[tx][r], hi, lo);
+            __bf16* row = out + s * 3 * ldo;
+            row[a] = hi;
+            row[ldo + a] = hi;
+            row[2 * ldo + a] = lo;
+        }
+    }
+}
+
 int grid_for(int64_t work, int64_t cap = 65536)
 {
     return (int)std::max<int64_t>(1, std::min<int64_t>((work + 255) / 256, cap));
@@ -332,22 +389,29 @@ int grid_for(int64_t work, int64_t cap = 65536)
 
 }  // namespace
 
-void prepare_omega_lowp(const double* u, const double* v, int64_t d, int64_t ell, int64_t ldo, DType dtype,
+void prepare_omega_lowp(const double* u, const double* v, int64_t d, int64_t ell, int64_t ldo, int form,
                         void* ut, void* vt, hipStream_t s)
 {
+    if (ldo < d || ldo % 16) throw std::invalid_argument("omega segment width must be >= d and a multiple of 16");
     const dim3 grid((unsigned)((ell + 31) / 32), (unsigned)((ldo + 31) / 32));
-    if (dtype == kBF16) {
-        hipLaunchKernelGGL(transpose_cast_kernel<__bf16>, grid, dim3(256), 0, s, u, d, ell, ldo, 1.0f,
+    if (form == kSketchBF16) {
+        hipLaunchKernelGGL(transpose_cast_kernel<__bf16>, grid, dim3(256), 0, s, u, d, ell, ldo, ldo, 1.0f,
                            static_cast<__bf16*>(ut));
-        hipLaunchKernelGGL(transpose_cast_kernel<__bf16>, grid, dim3(256), 0, s, v, d, ell, ldo, 1.0f,
+        hipLaunchKernelGGL(transpose_cast_kernel<__bf16>, grid, dim3(256), 0, s, v, d, ell, ldo, ldo, 1.0f,
                            static_cast<__bf16*>(vt));
-    } else if (dtype == kFP8) {
-        hipLaunchKernelGGL(transpose_cast_kernel<uint8_t>, grid, dim3(256), 0, s, u, d, ell, ldo, kOmegaFp8Scale,
-                           static_cast<uint8_t*>(ut));
-        hipLaunchKernelGGL(transpose_cast_kernel<uint8_t>, grid, dim3(256), 0, s, v, d, ell, ldo, kOmegaFp8Scale,
-                           static_cast<uint8_t*>(vt));
+    } else if (form == kSketchFP8) {
+        // [u | u]: the same e4m3 Omega row meets phi's hi and lo terms.
+        for (int seg = 0; seg < 2; ++seg) {
+            hipLaunchKernelGGL(transpose_cast_kernel<uint8_t>, grid, dim3(256), 0, s, u, d, ell, ldo, 2 * ldo,
+                               kOmegaFp8Scale, static_cast<uint8_t*>(ut) + seg * ldo);
+            hipLaunchKernelGGL(transpose_cast_kernel<uint8_t>, grid, dim3(256), 0, s, v, d, ell, ldo, 2 * ldo,
+                               kOmegaFp8Scale, static_cast<uint8_t*>(vt) + seg * ldo);
+        }
+    } else if (form == kSketchBF16x3) {
+        hipLaunchKernelGGL(split_omega_bf16x3_kernel, grid, dim3(256), 0, s, u, d, ell, ldo, static_cast<__bf16*>(ut));
+        hipLaunchKernelGGL(split_omega_bf16x3_kernel, grid, dim3(256), 0, s, v, d, ell, ldo, static_cast<__bf16*>(vt));
     } else {
-        throw std::invalid_argument("reduced-precision omega must be bf16 or fp8");
+        throw std::invalid_argument("reduced-precision sketch form must be bf16, fp8 or bf16x3");
     }
     MPISDF_HIP(hipGetLastError());
 }
@@ -367,6 +431,15 @@ void quantize_fp8_rows(const float* phi, int64_t ld, int64_t count, int64_t d, c
     const double post = 1.0 / ((double)kOmegaFp8Scale * (double)kOmegaFp8Scale);
     hipLaunchKernelGGL(quantize_fp8_rows_kernel, dim3((unsigned)((count + 3) / 4)), dim3(256), 0, s, phi, ld, count,
                        d, sw, 1.0 / std::sqrt((double)ell), post, out, ldo, scale);
+    MPISDF_HIP(hipGetLastError());
+}
+
+void split_rows_bf16x3(const double* in, int64_t ld, int64_t count, int64_t d, int64_t ldo, void* out, hipStream_t s)
+{
+    if (count <= 0) return;
+    if (ldo < d || ld < d) throw std::invalid_argument("invalid split row strides");
+    hipLaunchKernelGGL(split_rows_bf16x3_kernel, dim3(grid_for(count * ldo)), dim3(256), 0, s, in, ld, count, d, ldo,
+                       static_cast<__bf16*>(out));
     MPISDF_HIP(hipGetLastError());
 }
 

@@ -40,8 +40,8 @@ def test_tolerance_mode_and_multi_axis_repeat_bitwise(mp_isdf, water8):
     alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
     axes = [mp_isdf.AxisType.AtomicOrbital, mp_isdf.AxisType.AtomicOrbitalGradX]
     sub = slice(0, 60000)
-    runs = [sel.select_with_diagnostics(xyz[sub], w[sub], axes, tolerance=1e-3, sketch_size=600, oversampling=60,
-                                        seed=3) for _ in range(2)]
+    runs = [sel.select_with_diagnostics(xyz[sub], w[sub], axes, tolerance=1e-2, algorithm=alg, sketch_size=600,
+                                        oversampling=60, seed=3) for _ in range(2)]
     assert runs[0][0] == runs[1][0]
     assert runs[0][1]["relative_tail"] == runs[1][1]["relative_tail"]
-    assert 0 < len(runs[0][0]) < 540
+    assert 0 < len(runs[0][0]) <= 540

@@ -1,0 +1,75 @@
+"""The precision forms of rand_sketch at the workloads they are defined on:
+(H2O)32 / cc-pVDZ (768 AOs, level-3 grid, k = 6144, ell = 6759) and
+(H2O)64 / cc-pVDZ (1536 AOs, level-3 grid, k = 12000, ell = 13200), seed 7,
+every form fitted on the kept grid in FP64
+(cpp/src/anchor_select_randomized_id.cpp:269-374; python/isdf.py:154-191,
+:238-263).
+
+* The all-FP64 form is bit-identical from run to run (anchors and the fit's
+  pair-space error): every sum over workgroups is added in a fixed order.
+* Every reduced form returns k distinct nonzero-weight anchors whose FP64 fit
+  reaches the FP64 form's relative pair-space error to within 1e-4.
+* The split-BF16 form (hi.hi + lo.hi + hi.lo, four digit planes) also agrees
+  with the FP64 form's anchor set on at least 98% of the anchors.  BF16 and
+  two-term FP8 inputs perturb the sketch by ~1e-3, far beyond the ~1e-7 the
+  greedy pivots tolerate before they start to differ (measured by perturbing
+  phi: identical anchors at 6e-9, 57% at 6e-5), so their overlap is printed,
+  not asserted.
+"""
+import numpy as np
+import pytest
+
+from mp_isdf_amd import capi, grid as gridmod, molecule
+
+pytestmark = pytest.mark.gpu
+
+WORKLOADS = {"h2o32": (32, 6144), "h2o64": (64, 12000)}
+FIT_BATCH = 32768
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("name", ["h2o32", "h2o64"])
+def test_precision_forms_at_equal_fit_error(mp_isdf, name):
+    import torch
+    nwater, k = WORKLOADS[name]
+    oversampling = (k + 9) // 10
+    mol = molecule.water_cluster(nwater, "cc-pvdz")
+    xyz, w = gridmod.make_grid(mol, 3)
+    keep = w != 0
+    n = int(keep.sum())
+    print(f"\n[{name}] nao {mol.nao} grid {len(w)} kept {n} k {k} ell {k + oversampling}", flush=True)
+    ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
+    sel = mp_isdf.AnchorSelection(ctx)
+    cctx = capi.Context(mol.atm, mol.bas, mol.env)
+    alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
+    ao = mp_isdf.AxisType.AtomicOrbital
+    xyz_dev = torch.tensor(xyz[keep], device="cuda")
+    w_dev = torch.tensor(w[keep], device="cuda")
+
+    def select(precision):
+        anchors = sel.select(xyz, w, ao, algorithm=alg, max_rank=k, oversampling=oversampling, seed=7,
+                             precision=precision)
+        assert len(anchors) == k and len(set(anchors)) == k
+        assert np.all(w[anchors] != 0)
+        return anchors
+
+    def fit(anchors):
+        res = np.zeros(8)
+        capi.call("mpisdf_isdf_fit_device", cctx.handle, torch.tensor(xyz[anchors], device="cuda"),
+                  np.ascontiguousarray(w[anchors]), k, xyz_dev, w_dev, n, FIT_BATCH, None, res, None)
+        return res[1]
+
+    ref = select("fp64")
+    assert select("fp64") == ref
+    e64 = fit(ref)
+    assert fit(ref) == e64
+    print(f"[{name}] fp64 relative pair-space error {e64:.6e}", flush=True)
+    for precision in ("bf16x3", "bf16", "fp8"):
+        got = select(precision)
+        e = fit(got)
+        overlap = len(set(got) & set(ref)) / k
+        print(f"[{name}] {precision}: overlap {overlap:.4f} pair error {e:.6e} (difference {e - e64:+.2e})",
+              flush=True)
+        assert abs(e - e64) <= 1e-4
+        if precision == "bf16x3":
+            assert overlap >= 0.98

@@ -33,35 +33,60 @@ def test_fp64_sketch_matches_oracle(case):
     np.testing.assert_allclose(y.cpu().numpy(), case["y"], rtol=1e-12, atol=1e-14 * np.abs(case["y"]).max())
 
 
-@pytest.mark.parametrize("dtype,tol", [(2, 2e-2), (3, 2e-1)])
-def test_lowp_sketch_matches_oracle(case, dtype, tol):
+@pytest.mark.parametrize("form,tol", [(2, 2e-2), (3, 6e-2), (4, 1e-4)])
+def test_lowp_sketch_matches_oracle(case, form, tol):
+    """BF16 (one term), FP8 (two e4m3 terms of phi on one power-of-two scale,
+    Omega [u | u]) and split BF16 (phi [hi | lo | hi] . Omega [hi | hi | lo])
+    against the FP64 oracle sketch on the same phi, Omega and weights; the
+    split form is within ~2^-16 of each product term.  The FP8 form rounds
+    Omega itself to e4m3 (~2^-5 per entry, a different Gaussian-like Omega),
+    so it is also compared, at 1e-2, with the oracle sketch of the e4m3
+    Omega it uses: that isolates the two-term phi (~2^-8)."""
     torch = case["torch"]
     dev = lambda a: torch.tensor(np.ascontiguousarray(a), device="cuda")
     n, d, ell = len(case["w"]), case["d"], case["ell"]
     dp = (d + 15) // 16 * 16
-    esz = torch.bfloat16 if dtype == 2 else torch.uint8
-    ut = torch.empty((ell, dp), dtype=esz, device="cuda")
-    vt = torch.empty((ell, dp), dtype=esz, device="cuda")
-    capi.call("mpisdf_sketch_prepare_omega", dev(case["u"]), dev(case["v"]), d, ell, dp, dtype, ut, vt, None)
+    seg = {2: 1, 3: 2, 4: 3}[form]
+    esz = torch.uint8 if form == 3 else torch.bfloat16
+    ut = torch.empty((ell, seg * dp), dtype=esz, device="cuda")
+    vt = torch.empty((ell, seg * dp), dtype=esz, device="cuda")
+    capi.call("mpisdf_sketch_prepare_omega", dev(case["u"]), dev(case["v"]), d, ell, dp, form, ut, vt, None)
     sw = dev(np.sqrt(case["w"]))
     scale = torch.empty(n, dtype=torch.float32, device="cuda")
-    if dtype == 2:
+    if form == 2:
         phi = torch.zeros((n, dp), dtype=torch.bfloat16, device="cuda")
         phi[:, :d] = dev(case["phi"]).to(torch.bfloat16)
         capi.call("mpisdf_sketch_scales_bf16", sw, n, ell, scale, None)
-    else:
-        phi = torch.zeros((n, dp), dtype=torch.uint8, device="cuda")
+    elif form == 3:
+        phi = torch.zeros((n, 2 * dp), dtype=torch.uint8, device="cuda")
         phi32 = dev(case["phi"]).float().contiguous()
         capi.call("mpisdf_sketch_quantize_fp8", phi32, d, n, d, sw, ell, phi, dp, scale, None)
+    else:
+        phi = torch.empty((n, 3 * dp), dtype=torch.bfloat16, device="cuda")
+        capi.call("mpisdf_sketch_split_bf16x3", dev(case["phi"]), d, n, d, phi, dp, None)
+        capi.call("mpisdf_sketch_scales_bf16", sw, n, ell, scale, None)
+        hi = phi[:, :dp].double()
+        lo = phi[:, dp:2 * dp].double()
+        assert torch.equal(phi[:, 2 * dp:], phi[:, :dp])
+        x = dev(case["phi"])
+        assert float((hi[:, :d] + lo[:, :d] - x).abs().max() / x.abs().max()) < 2.0 ** -15
     y = torch.empty((n, ell), dtype=torch.float32, device="cuda")
     ynorm2 = torch.zeros(n, dtype=torch.float64, device="cuda")
     bad = torch.zeros(1, dtype=torch.int64, device="cuda")
-    capi.call("mpisdf_kr_sketch_device", dtype, ut, vt, ell, dp, phi, dp, n, scale, y, ell, ynorm2, bad, None)
+    dtype = 3 if form == 3 else 2
+    capi.call("mpisdf_kr_sketch_device", dtype, ut, vt, ell, seg * dp, phi, seg * dp, n, scale, y, ell, ynorm2, bad,
+              None)
     torch.cuda.synchronize()
     got = y.double().cpu().numpy()
     ref = case["y"]
     rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
     assert rel < tol, rel
+    if form == 3:
+        f8 = lambda t: t[:, :d].contiguous().view(torch.float8_e4m3fn).double().cpu().numpy().T / 32.0
+        ref8 = oracle.sketch(case["phi"], np.ascontiguousarray(f8(ut)), np.ascontiguousarray(f8(vt)),
+                             np.sqrt(case["w"]))
+        rel8 = np.linalg.norm(got - ref8) / np.linalg.norm(ref8)
+        assert rel8 < 1e-2, rel8
     np.testing.assert_allclose(ynorm2.cpu().numpy(), np.sum(got ** 2, axis=1), rtol=1e-6)
     assert int(bad.item()) == 0
 
@@ -75,14 +100,14 @@ def test_lowp_selection_is_valid(mp_isdf):
     ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
     sel = mp_isdf.AnchorSelection(ctx)
     alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
-    for precision in ("bf16", "fp8"):
+    for precision in ("bf16", "fp8", "bf16x3"):
         got = sel.select(pts, w, mp_isdf.AxisType.AtomicOrbital, algorithm=alg, max_rank=120, oversampling=12,
                          seed=7, precision=precision)
         assert len(got) == 120 and len(set(got)) == 120
         assert not set(got) & set(np.flatnonzero(w == 0).tolist())
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
 def test_qrcp_fold_modes(mp_isdf, mode):
     """Fold = column / row sums of (Q^T Y)^2 against NumPy; the split-BF16
     mode is checked against its stated bound (1e-5 of ||y_j||^2), the INT8
@@ -106,26 +131,27 @@ def test_qrcp_fold_modes(mp_isdf, mode):
     capi.call("mpisdf_qrcp_fold_device", b, n, m, Qb, Y, mode, ld, proj, energy, None)
     torch.cuda.synchronize()
     ynorm2 = np.sum(y ** 2, axis=1)
-    if mode == 3:
-        # The INT8 fold works on the three-digit form of each FP32 column:
-        # s = 2^(e+1) with ||y|| < 2^e, y_hat = s sum_r d_r 128^-(r+1),
-        # every entry within half a step (2^-22 s) of y.
-        scale, _, y_hat = _digit_model(y, ynorm2)
-        assert np.all(np.abs(y_hat - y) <= 2.0 ** -22 * scale[:, None])
+    if mode in (3, 4):
+        # The INT8 fold works on the three- (four-) digit form of each FP32
+        # column: s = 2^(e+1) with ||y|| < 2^e, y_hat = s sum_r d_r 128^-(r+1),
+        # every entry within half a step (2^-22 s, 2^-29 s) of y.
+        planes = mode
+        scale, _, y_hat = _digit_model(y, ynorm2, planes)
+        assert np.all(np.abs(y_hat - y) <= 2.0 ** -(7 * planes + 1) * scale[:, None])
         w_hat = q.T @ y_hat.T
         want_cols, want_rows = np.sum(w_hat ** 2, axis=0), np.sum(w_hat ** 2, axis=1)
-    tol = {0: 1e-12, 1: 1e-12, 2: 1e-5, 3: 1e-10}[mode]
+    tol = {0: 1e-12, 1: 1e-12, 2: 1e-5, 3: 1e-10, 4: 1e-10}[mode]
     assert np.max(np.abs(proj.cpu().numpy() - want_cols) / ynorm2) < tol
     np.testing.assert_allclose(energy.cpu().numpy(), want_rows, rtol=max(tol, 1e-12))
 
 
-def _digit_model(y, ynorm2):
+def _digit_model(y, ynorm2, planes=3):
     """NumPy model of the digit planes: s = 2^(e+1) with ||y|| < 2^e, and
-    three signed base-128 digits of y / s, rounded to nearest in FP32 steps."""
+    ``planes`` signed base-128 digits of y / s, rounded to nearest in FP32 steps."""
     scale = np.exp2(np.frexp(np.sqrt(ynorm2))[1] + 1.0)
     x = (y / scale[:, None]).astype(np.float32)
     digits, y_hat, weight = [], np.zeros_like(y), 1.0
-    for _ in range(3):
+    for _ in range(planes):
         x = (x * np.float32(128)).astype(np.float32)
         dgt = np.rint(x)
         x = (x - dgt).astype(np.float32)
@@ -135,13 +161,13 @@ def _digit_model(y, ynorm2):
     return scale, digits, y_hat * scale[:, None]
 
 
-def _planes(Y, n, P):
+def _planes(Y, n, P, planes=3):
     """Digits as written in place over the FP32 columns: each slice of 64
     entries holds its three planes in 192 contiguous bytes, slice after
     slice; returns [n][3][P] and the column's unused last P bytes."""
     raw = Y.cpu().numpy().view(np.int8).reshape(n, 4 * P)
-    planes = raw[:, :3 * P].reshape(n, P // 64, 3, 64).transpose(0, 2, 1, 3).reshape(n, 3, P)
-    return planes, raw[:, 3 * P:]
+    got = raw[:, :planes * P].reshape(n, P // 64, planes, 64).transpose(0, 2, 1, 3).reshape(n, planes, P)
+    return got, raw[:, planes * P:]
 
 
 def test_qrcp_digit_planes_match_oracle(mp_isdf):
@@ -161,7 +187,7 @@ def test_qrcp_digit_planes_match_oracle(mp_isdf):
     Y = torch.tensor(buf, device="cuda")
     yn = torch.tensor(ynorm2, device="cuda")
     sc = torch.zeros(n, dtype=torch.float64, device="cuda")
-    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
+    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, 3, yn, sc, None)
     scale, digits, y_hat = _digit_model(y, ynorm2)
     planes, rest = _planes(Y, n, P)
     np.testing.assert_array_equal(sc.cpu().numpy(), scale)
@@ -187,6 +213,42 @@ def test_qrcp_digit_planes_match_oracle(mp_isdf):
     piv32, _ = gpu_qrcp(y32, 1, m, None, None)
     _, want32, _ = oracle.geqp3(np.ascontiguousarray(y.T))
     np.testing.assert_array_equal(piv32, want32[:kmax])
+
+
+def test_qrcp_four_digit_planes_match_oracle(mp_isdf):
+    """The split-BF16 selection's QRCP: four digit planes per FP32 column
+    (checked digit for digit against the NumPy model, no unused bytes) and
+    INT8-fold pivots equal to the oracle's dgeqp3-semantics QRCP on that
+    y_hat, which is within 2^-29 s of each entry."""
+    import torch
+    rng = np.random.default_rng(12)
+    m, n, kmax = 321, 3000, 150
+    y = (rng.normal(size=(n, 70)) @ rng.normal(size=(70, m)) * rng.uniform(0.2, 2.0, size=(n, 1))
+         + 1e-4 * rng.normal(size=(n, m))).astype(np.float32).astype(np.float64)
+    ynorm2 = np.sum(y ** 2, axis=1)
+    P = (m + 63) // 64 * 64
+    buf = np.zeros((n, P), dtype=np.float32)
+    buf[:, :m] = y
+    Y = torch.tensor(buf, device="cuda")
+    yn = torch.tensor(ynorm2, device="cuda")
+    sc = torch.zeros(n, dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, 4, yn, sc, None)
+    scale, digits, y_hat = _digit_model(y, ynorm2, 4)
+    assert np.all(np.abs(y_hat - y) <= 2.0 ** -29 * scale[:, None])
+    planes, rest = _planes(Y, n, P, 4)
+    assert rest.size == 0
+    np.testing.assert_array_equal(sc.cpu().numpy(), scale)
+    for r in range(4):
+        np.testing.assert_array_equal(planes[:, r, :m], digits[r])
+        assert not planes[:, r, m:].any()
+    np.testing.assert_allclose(yn.cpu().numpy(), np.sum(y_hat ** 2, axis=1), rtol=1e-14)
+    piv = np.zeros(kmax, dtype=np.int64)
+    rd = np.zeros(kmax)
+    cnt = np.zeros(1, dtype=np.int64)
+    capi.call("mpisdf_qrcp_device", Y, 5, m, n, P, yn, sc, kmax, 0.0, piv, rd, cnt, None)
+    f, want, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
+    np.testing.assert_array_equal(piv[:cnt[0]], want[:kmax])
+    np.testing.assert_allclose(rd[:cnt[0]], np.abs(np.diag(f)[:kmax]), rtol=1e-9)
 
 
 def test_qrcp_stop_rules_and_rank_exhaustion(mp_isdf):
@@ -224,7 +286,7 @@ def test_qrcp_stop_rules_and_rank_exhaustion(mp_isdf):
     Y = torch.tensor(buf, device="cuda")
     yn = torch.tensor(np.sum(y ** 2, axis=1), device="cuda")
     sc = torch.zeros(n, dtype=torch.float64, device="cuda")
-    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
+    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, 3, yn, sc, None)
     _, _, y_hat = _digit_model(y, np.sum(y ** 2, axis=1))
     fh, order_h, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
     rh = np.abs(np.diag(fh))
@@ -245,7 +307,7 @@ def test_qrcp_stop_rules_and_rank_exhaustion(mp_isdf):
     Z = torch.tensor(zbuf, device="cuda")
     zn = torch.tensor(np.sum(z ** 2, axis=1), device="cuda")
     zs = torch.zeros(n, dtype=torch.float64, device="cuda")
-    capi.call("mpisdf_digitize_columns_device", Z, m, n, P, zn, zs, None)
+    capi.call("mpisdf_digitize_columns_device", Z, m, n, P, 3, zn, zs, None)
     assert sorted(run(Z, 4, P, zn, zs, 40, 0.0).tolist()) == sorted(live.tolist())
 
 
@@ -278,7 +340,7 @@ def test_qrcp_blocks_past_one_fold_group(mp_isdf):
     Y = torch.tensor(buf, device="cuda")
     yn = torch.tensor(np.sum(y ** 2, axis=1), device="cuda")
     sc = torch.zeros(n, dtype=torch.float64, device="cuda")
-    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, yn, sc, None)
+    capi.call("mpisdf_digitize_columns_device", Y, m, n, P, 3, yn, sc, None)
     _, _, y_hat = _digit_model(y, np.sum(y ** 2, axis=1))
     _, order_h, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
     np.testing.assert_array_equal(run(Y, 4, P, yn, sc), order_h[:kmax])
