@@ -234,9 +234,11 @@ int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_xyz, const d
                     const double* xyz, const double* w, int64_t n, int64_t batch, double* theta, double* tsum,
                     double* results);
 /* Same with device-resident grid (xyz_dev[n*3], w_dev[n]) and anchors
- * (anchor_xyz_dev[k*3]); anchor_w stays on the host.  theta is formed batch
- * by batch on the device when tsum (host, k entries) is given and consumed
- * there: tsum = sum_g theta[:, g] w_g. */
+ * (anchor_xyz_dev[k*3]); anchor_w stays on the host.  tsum (host, k entries,
+ * optional) = sum_g theta[:, g] w_g, formed as L^-T (sum_g w_g z_g) / w_anchor
+ * with z_g = L^-1 (Phi_a phi_g)^2: theta itself is not formed, and the grid
+ * pass runs only the S and Z products (a consumer that needs theta forms
+ * L^-T Z per batch, or calls mpisdf_isdf_fit). */
 int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,
                            const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* tsum,
                            double* results, void* stream);

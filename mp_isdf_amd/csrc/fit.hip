@@ -146,6 +146,19 @@ __global__ void pair_error_norms_kernel(const double* __restrict__ zpart, int64_
     }
 }
 
+// part[y * k + r] = sum over the columns g of column block y (128 columns) of
+// w[g] z[r + g * k]: this batch's share of sum_g w_g z_g, added in block order.
+__global__ void zw_rows_kernel(const double* __restrict__ z, int64_t k, int64_t nb, const double* __restrict__ w,
+                               double* __restrict__ part)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (r >= k) return;
+    const int64_t g0 = (int64_t)blockIdx.y * 128, g1 = min(nb, g0 + 128);
+    double s = 0;
+    for (int64_t g = g0; g < g1; ++g) s += w[g] * z[r + g * k];
+    part[(int64_t)blockIdx.y * k + r] = s;
+}
+
 // out[c * rows + r] = in[r * cols + c] / (div ? div[c] : 1)
 __global__ void transpose_kernel(const double* __restrict__ in, int64_t rows, int64_t cols,
                                  const double* __restrict__ div, double* __restrict__ out)
@@ -335,15 +348,24 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
     const int64_t zgroups = (k + 127) / 128;  // row groups of the Z product
     DeviceBuffer<double> zz((size_t)(zgroups * batch)), ept((size_t)(2 * batch));
-    DeviceBuffer<double> th, thr, tsum, tpart;
+    DeviceBuffer<double> th, thr, tsum, tpart, zw;
+    // theta . w alone needs no theta: sum_g w_g theta_phys[:, g] =
+    // L^-T (sum_g w_g z_g), one k x k triangular product after the grid pass.
+    const bool theta_w_only = tsum_out && !theta_host;
     std::vector<double> stage;
     wa.upload(anchor_w, k, s);
-    if (tsum_out || theta_host) th.resize((size_t)(k * batch));
-    if (theta_host) thr.resize((size_t)(k * batch));
+    if (theta_host) {
+        th.resize((size_t)(k * batch));
+        thr.resize((size_t)(k * batch));
+    }
     if (tsum_out) {
         tsum.resize(k);
         tpart.resize((size_t)(tn_column_blocks(batch) * k));
         MPISDF_HIP(hipMemsetAsync(tsum.ptr, 0, k * sizeof(double), s));
+    }
+    if (theta_w_only) {
+        zw.resize(k);
+        MPISDF_HIP(hipMemsetAsync(zw.ptr, 0, k * sizeof(double), s));
     }
     MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
     for (int64_t start = 0; start < n; start += batch) {
@@ -366,11 +388,16 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
                                phi.ptr, d, w + start, nb, ept.ptr, batch);
             launch_sum_fixed(ept.ptr, nb, batch, 2, acc.ptr, s);
         }
-        if (theta_host || tsum_out) {
+        if (theta_w_only) {
+            const dim3 grid((unsigned)((k + 255) / 256), (unsigned)((nb + 127) / 128));
+            hipLaunchKernelGGL(zw_rows_kernel, grid, dim3(256), 0, s, z.ptr, k, nb, w + start, tpart.ptr);
+            MPISDF_HIP(hipGetLastError());
+            launch_sum_partials(tpart.ptr, grid.y, k, zw.ptr, true, s);
+        }
+        if (theta_host) {
             // theta_phys = L^-T Z (upper triangular A), column-major k x nb;
-            // theta = theta_phys / w_anchor
-            // (with t_phys += theta_phys w_grid summed in the epilogue; divided by
-            // w_anchor at the end)
+            // theta = theta_phys / w_anchor (with t_phys += theta_phys w_grid
+            // summed in the epilogue; divided by w_anchor at the end)
             timer.begin(s);
             if (tsum_out)
                 gemm_tn_f64_fused(TnFused::RowDot, k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, th.ptr, k, nullptr,
@@ -380,7 +407,7 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
             timer.end(s);
             if (tsum_out) launch_sum_partials(tpart.ptr, tn_column_blocks(nb), k, tsum.ptr, true, s);
             gemm_flops += (double)k * (double)(k + 1) * (double)nb;
-            if (theta_host) {  // rows of theta[k][n] on the host, through a contiguous staging copy
+            {  // rows of theta[k][n] on the host, through a contiguous staging copy
                 transpose(th.ptr, nb, k, wa.ptr, thr.ptr, s);
                 if (nb == n) {
                     MPISDF_HIP(hipMemcpyAsync(theta_host, thr.ptr, (size_t)(k * n) * sizeof(double),
@@ -402,6 +429,8 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         err_out[0] = h[0];
         err_out[1] = h[1];
     }
+    if (theta_w_only)  // t_phys = L^-T zw (linvt: row-major upper triangular)
+        gemm_f64(k, 1, k, 1.0, MatView{f.linvt.ptr, k, 1}, MatView{zw.ptr, 1, 1}, 0.0, MutView{tsum.ptr, 1, 1}, s);
     if (tsum_out) {
         MPISDF_HIP(hipMemcpyAsync(tsum_out, tsum.ptr, k * sizeof(double), hipMemcpyDeviceToHost, s));
         MPISDF_HIP(hipStreamSynchronize(s));

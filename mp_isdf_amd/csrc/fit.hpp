@@ -21,8 +21,10 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
 // Streams grid batches (device xyz[n][3], device w[n]) through
 //   S = (Phi_a Phi_b^T)^2, Z = L^-1 S, theta = L^-T Z / w_anchor (host anchor_w[k]).
 // err_out (host, 2 doubles): sum |w| (||phi||^4 - ||z||^2), sum |w| ||phi||^4.
-// theta_host (optional): row-major [k][n].  tsum_out (optional host [k]):
-// sum_g theta[:, g] w_g.
+// theta_host (optional): row-major [k][n], formed batch by batch as
+// L^-T Z / w_anchor.  tsum_out (optional host [k]): sum_g theta[:, g] w_g;
+// without theta_host it is L^-T (sum_g w_g z_g) / w_anchor, so the grid pass
+// then runs only the S and Z products.
 // gemm_stats (optional, host double[3]): summed HIP-event seconds of the grid
 // pass's TN products (S, Z = L^-1 S, theta = L^-T Z), their algorithmic flops
 // (2 k d + 2 k (k+1) per point, the latter only when theta is formed) and

@@ -54,7 +54,7 @@ enum Diag : int {
     kDiagFoldFlops = 15,    // their algorithmic FP64 flops: sum 2 b ell n
     kDiagFoldLaunches = 16,
     kDiagSketchSeconds = 17,  // summed duration of the sketch kernel launches (HIP events)
-    kDiagSketchFlops = 18,    // their algorithmic flops: sum 4 ell d count
+    kDiagSketchFlops = 18,    // their MFMA flops: sum 4 ell d count per product term (3 split BF16, 2 FP8)
     kDiagSketchLaunches = 19,
     kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 / 3 INT8 MFMA on 3 / 4 digit planes
     kDiagCount = 24

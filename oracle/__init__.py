@@ -127,14 +127,20 @@ def axis_eval(atm, bas, env, coords, axes, mo_coeff=None, nocc=0, scales=None, c
     return out
 
 
-def geqp3(a: np.ndarray):
-    """dgeqp3 semantics on a copy of ``a``: returns (packed QR, 0-based pivots, tau)."""
+def geqp3(a: np.ndarray, kmax: int | None = None):
+    """dgeqp3 semantics on a copy of ``a``: returns (packed QR, 0-based pivots, tau).
+    ``kmax`` stops after that many Householder steps (pivots 0..kmax-1 are the
+    full factorisation's)."""
     m, n = a.shape
     f = np.asfortranarray(a, dtype=np.float64).copy(order="F")
     jpvt = np.empty(n, dtype=np.int64)
     tau = np.empty(min(m, n))
-    _check(lib().oracle_geqp3(ctypes.c_int64(m), ctypes.c_int64(n), _p(f), ctypes.c_int64(m),
-                              _p(jpvt), _p(tau)), selection=True)
+    if kmax is None:
+        _check(lib().oracle_geqp3(ctypes.c_int64(m), ctypes.c_int64(n), _p(f), ctypes.c_int64(m),
+                                  _p(jpvt), _p(tau)), selection=True)
+    else:
+        _check(lib().oracle_geqp3_steps(ctypes.c_int64(m), ctypes.c_int64(n), _p(f), ctypes.c_int64(m),
+                                        _p(jpvt), _p(tau), ctypes.c_int64(kmax)), selection=True)
     return f, jpvt - 1, tau
 
 

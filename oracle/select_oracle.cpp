@@ -92,9 +92,11 @@ double nrm2(int64_t n, const double* x)
 
 // LAPACK dgeqp3 semantics (unblocked dlaqp2 form), column-major, in place.
 // jpvt receives 1-based pivots, all columns free.
-void geqp3(int64_t m, int64_t n, double* a, int64_t lda, int64_t* jpvt, double* tau)
+// kmax >= 0 stops after kmax Householder steps (the pivots and norms up to
+// there are those of the full factorisation).
+void geqp3(int64_t m, int64_t n, double* a, int64_t lda, int64_t* jpvt, double* tau, int64_t kmax = -1)
 {
-    const int64_t k = std::min(m, n);
+    const int64_t k = kmax >= 0 ? std::min({m, n, kmax}) : std::min(m, n);
     std::vector<double> vn1(n), vn2(n), work(n);
     for (int64_t j = 0; j < n; ++j) {
         jpvt[j] = j + 1;
@@ -594,6 +596,11 @@ const char* oracle_select_last_error() { return g_sel_error.c_str(); }
 int oracle_geqp3(int64_t m, int64_t n, double* a, int64_t lda, int64_t* jpvt, double* tau)
 {
     return guarded([&] { geqp3(m, n, a, lda, jpvt, tau); });
+}
+
+int oracle_geqp3_steps(int64_t m, int64_t n, double* a, int64_t lda, int64_t* jpvt, double* tau, int64_t kmax)
+{
+    return guarded([&] { geqp3(m, n, a, lda, jpvt, tau, kmax); });
 }
 
 int oracle_omega(int64_t d, int64_t ell, uint64_t seed, double* u, double* v)

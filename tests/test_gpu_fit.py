@@ -37,6 +37,12 @@ def test_fit_matches_oracle_refit(mp_isdf, nanchor):
     np.testing.assert_allclose(fit["pair_space_error"]["l2"], err["l2"], rtol=1e-6)
     np.testing.assert_allclose(fit["pair_space_error"]["relative"], err["relative"], rtol=1e-6)
     np.testing.assert_allclose(fit["theta_weight_sums"], theta @ w, rtol=1e-7, atol=1e-9 * np.abs(theta @ w).max())
+    # theta . w without forming theta: L^-T (sum_g w_g z_g) / w_anchor
+    lean = mp_isdf.isdf_fit(ctx, pts[anchors], w[anchors], pts, w, batch_size=128, weight_sums=True)
+    assert lean["interp_vecs"] is None
+    np.testing.assert_allclose(lean["theta_weight_sums"], fit["theta_weight_sums"], rtol=1e-10,
+                               atol=1e-12 * np.abs(theta @ w).max())
+    assert lean["pair_space_error"]["l2"] == fit["pair_space_error"]["l2"]
 
 
 def test_fit_batch_invariance(mp_isdf):
