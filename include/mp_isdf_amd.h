@@ -175,6 +175,18 @@ int mpisdf_sketch_quantize_fp8(const float* phi, int64_t ld, int64_t count, int6
 int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi,
                             int64_t ldphi, int64_t count, const float* scale, float* y, int64_t ldy,
                             double* ynorm2, unsigned long long* bad, void* stream);
+/* Allocation-free form of mpisdf_kr_sketch_device, with the same arguments
+ * and results bit for bit.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops).  It calls no hipMalloc/hipFree and does not
+ * synchronise the stream: the per-Omega-row-block partial sums of ||y_g||^2
+ * live in a static device array of 2^22 doubles (32 MiB) that each GPU holds
+ * until the process ends, so count * ceil(ell / 256) must not exceed 2^22
+ * (status 1 otherwise).  All calls on one GPU share that array: calls that
+ * overlap in time on one GPU race. */
+int mpisdf_kr_sketch_device_noalloc(int dtype, const void* ut, const void* vt, int64_t ell, int64_t d,
+                                    const void* phi, int64_t ldphi, int64_t count, const float* scale, float* y,
+                                    int64_t ldy, double* ynorm2, unsigned long long* bad, void* stream);
 
 /* ---- QRCP fold (device pointers) --------------------------------------------- */
 /* Block update of the column-pivoted QR's residual norms (replaces the

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <new>
 
 struct mpisdf_context {
@@ -56,6 +57,39 @@ mpisdf::Context& unwrap(const mpisdf_context* ctx)
 
 // Points per device batch for host-buffer evaluation (bounds device memory).
 constexpr int64_t kHostBatch = 1 << 18;
+
+}  // namespace
+
+// Device scratch of the allocation-free entry point (mpisdf_kr_sketch_device_noalloc):
+// 2^22 doubles (32 MiB) per GPU, held by each GPU the library runs on until
+// the process ends.  Host code never uses the symbol itself as an address:
+// the device address comes from hipGetSymbolAddress, once per device.
+constexpr int64_t kNoAllocScratchDoubles = int64_t(1) << 22;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kScratchDevices] = {};
+
+// The current device's copy of ``scratch`` (looked up on the first call on
+// each device, with that device current).
+double* scratch_on_current_device()
+{
+    int dev = -1;
+    MPISDF_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) throw std::runtime_error("device ordinal outside the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
 
 }  // namespace
 
@@ -339,6 +373,24 @@ extern "C" int mpisdf_kr_sketch_device(int dtype, const void* ut, const void* vt
         mpisdf::kr_sketch_lowp(static_cast<mpisdf::DType>(dtype), ut, vt, ell, d, phi, ldphi, count, scale, y, ldy,
                                ynorm2, bad, part, static_cast<hipStream_t>(stream));
         MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
+extern "C" int mpisdf_kr_sketch_device_noalloc(int dtype, const void* ut, const void* vt, int64_t ell, int64_t d,
+                                               const void* phi, int64_t ldphi, int64_t count, const float* scale,
+                                               float* y, int64_t ldy, double* ynorm2, unsigned long long* bad,
+                                               void* stream)
+{
+    return guard([&] {
+        if (ldy < ell || ldphi < d || !ynorm2 || !bad) throw std::invalid_argument("invalid sketch arguments");
+        const int64_t need = mpisdf::kr_sketch_lowp_partials(ell, count);
+        if (need > kNoAllocScratchDoubles)
+            throw std::invalid_argument("sketch too large for the allocation-free entry point: count * ceil(ell / 256) = " +
+                                        std::to_string(need) + " exceeds its scratch of " +
+                                        std::to_string(kNoAllocScratchDoubles) + " doubles");
+        double* part = scratch_on_current_device();
+        mpisdf::kr_sketch_lowp_into(static_cast<mpisdf::DType>(dtype), ut, vt, ell, d, phi, ldphi, count, scale, y,
+                                    ldy, ynorm2, bad, part, static_cast<hipStream_t>(stream));
     });
 }
 

@@ -457,14 +457,27 @@ void cast_rows(const double* in, int64_t count, int64_t d, int64_t ldo, DType dt
     MPISDF_HIP(hipGetLastError());
 }
 
+int64_t kr_sketch_lowp_partials(int64_t ell, int64_t count)
+{
+    return count > 0 ? (ell + SK_S - 1) / SK_S * count : 0;
+}
+
 void kr_sketch_lowp(DType dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi, int64_t ldphi,
                     int64_t count, const float* scale, float* y, int64_t ldy, double* ynorm2, unsigned long long* bad,
                     DeviceBuffer<double>& part, hipStream_t s)
 {
+    const int64_t need = kr_sketch_lowp_partials(ell, count);
+    if (part.count < (size_t)need) part.resize((size_t)need);
+    kr_sketch_lowp_into(dtype, ut, vt, ell, d, phi, ldphi, count, scale, y, ldy, ynorm2, bad, part.ptr, s);
+}
+
+void kr_sketch_lowp_into(DType dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi,
+                         int64_t ldphi, int64_t count, const float* scale, float* y, int64_t ldy, double* ynorm2,
+                         unsigned long long* bad, double* part, hipStream_t s)
+{
     if (count <= 0) return;
     const int64_t gx = (count + SK_G - 1) / SK_G, gy = (ell + SK_S - 1) / SK_S;
     if (gx > 0x7fffffff || gy > 65535) throw std::overflow_error("sketch grid too large");
-    if (part.count < (size_t)(gy * count)) part.resize((size_t)(gy * count));
     // Buffer offsets inside the kernel are 32-bit: Omega rows (padded to the
     // 256-row tile) and one 128-point phi tile must stay below 2^31 bytes.
     const int64_t dbytes = dtype == kBF16 ? 2 * d : d, pbytes = dtype == kBF16 ? 2 * ldphi : ldphi;
@@ -476,16 +489,16 @@ void kr_sketch_lowp(DType dtype, const void* ut, const void* vt, int64_t ell, in
     if (dtype == kBF16) {
         if (ldphi % 8 || d % 8) throw std::invalid_argument("bf16 sketch needs d and ldphi multiples of 8");
         hipLaunchKernelGGL(kr_sketch_kernel<0>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, u8, v8, ell, 2 * d,
-                           p8, 2 * ldphi, count, scale, y, ldy, part.ptr, bad);
+                           p8, 2 * ldphi, count, scale, y, ldy, part, bad);
     } else if (dtype == kFP8) {
         if (ldphi % 16 || d % 16) throw std::invalid_argument("fp8 sketch needs d and ldphi multiples of 16");
         hipLaunchKernelGGL(kr_sketch_kernel<1>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, u8, v8, ell, d, p8,
-                           ldphi, count, scale, y, ldy, part.ptr, bad);
+                           ldphi, count, scale, y, ldy, part, bad);
     } else {
         throw std::invalid_argument("reduced-precision sketch must be bf16 or fp8");
     }
     MPISDF_HIP(hipGetLastError());
-    launch_sum_partials(part.ptr, gy, count, ynorm2, true, s);
+    launch_sum_partials(part, gy, count, ynorm2, true, s);
 }
 
 }  // namespace mpisdf

@@ -47,5 +47,11 @@ void cast_rows(const double* in, int64_t count, int64_t d, int64_t ldo, DType dt
 void kr_sketch_lowp(DType dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi, int64_t ldphi,
                     int64_t count, const float* scale, float* y, int64_t ldy, double* ynorm2, unsigned long long* bad,
                     DeviceBuffer<double>& part, hipStream_t s);
+// The same with caller-provided partials (kr_sketch_lowp_partials(ell, count)
+// doubles, every one written before it is read); allocates nothing.
+void kr_sketch_lowp_into(DType dtype, const void* ut, const void* vt, int64_t ell, int64_t d, const void* phi,
+                         int64_t ldphi, int64_t count, const float* scale, float* y, int64_t ldy, double* ynorm2,
+                         unsigned long long* bad, double* part, hipStream_t s);
+int64_t kr_sketch_lowp_partials(int64_t ell, int64_t count);
 
 }  // namespace mpisdf
