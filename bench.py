@@ -131,7 +131,7 @@ def main():
         error and theta . w = L^-T (sum_g w_g z_g) / w_anchor."""
         ax = torch.tensor(xyz[anchors], device="cuda")
         aw = np.ascontiguousarray(w[anchors])
-        res = np.zeros(8)
+        res = np.zeros(10)
         tsum = np.zeros(len(anchors))
         capi.call("mpisdf_isdf_fit_device", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev, n, FIT_BATCH, tsum, res,
                   None)

@@ -237,11 +237,15 @@ int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n, int64_t l
  * error_g^2 = ||phi_g||^4 - ||L^-1 (Phi_a phi_g)^2||^2.
  * Host arrays: anchor_xyz[k*3], anchor_w[k], xyz[n*3], w[n] (grid points with
  * nonzero weight).  theta (optional) receives row-major [k][n]; tsum
- * (optional) sum_g theta[:, g] w_g.  results[8]: 0 l2 error, 1 relative error,
+ * (optional) sum_g theta[:, g] w_g.  results[10]: 0 l2 error, 1 relative error,
  * 2 condition estimate of G2, 3 max diagonal of G2, 4 seconds (factor),
  * 5 seconds (grid pass), 6 summed HIP-event seconds of the grid pass's FP64
  * TN products (S, Z = L^-1 S and, when theta is formed, theta = L^-T Z),
- * 7 their algorithmic flops (2 k d + 2 k (k+1) per point with theta). */
+ * 7 their algorithmic flops (2 k d + k (k+1) per point, + k (k+1) with
+ * theta), 8 retained rank, 9 1 when the anchor pairs were numerically
+ * dependent and the pseudo-inverse of G2 over its eigenvalues above 1e-14 of
+ * the largest replaced L^-1 (the reference's truncated-SVD refit, which keeps
+ * singular values above 1e-12 of the largest; k <= 2048), else 0. */
 int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w, int64_t k,
                     const double* xyz, const double* w, int64_t n, int64_t batch, double* theta, double* tsum,
                     double* results);

@@ -496,6 +496,8 @@ void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anc
         results[5] = std::chrono::duration<double>(t2 - t1).count();
         results[6] = gemm_stats[0];
         results[7] = gemm_stats[1];
+        results[8] = (double)f.retained_rank;
+        results[9] = f.triangular ? 0.0 : 1.0;
     }
 }
 

@@ -25,6 +25,59 @@ namespace {
 
 constexpr int NB = 64;  // diagonal block of the blocked Cholesky / inverse
 constexpr double kDependent = 1e-14;
+// Pseudo-inverse fallback (dependent anchor pairs): eigenvalues of G2 at or
+// below this fraction of the largest are dropped (singular values of the pair
+// matrix below ~1e-7 of the largest: the resolution of the normal equations
+// in FP64), and the host eigensolver takes at most this many anchors.
+constexpr double kEigenCut = 1e-14;
+constexpr int64_t kPinvMaxAnchors = 2048;
+
+// Eigen-decomposition of the symmetric n x n matrix a (row-major, destroyed)
+// by cyclic Jacobi rotations: on return a holds the eigenvalues on its
+// diagonal and v (row-major) the eigenvectors in its columns.  Rotations
+// zero a_pq with t = sgn(theta) / (|theta| + sqrt(theta^2 + 1)),
+// theta = (a_qq - a_pp) / (2 a_pq), c = 1 / sqrt(t^2 + 1), s = t c, applied as
+// A <- P^T A P with P_pp = P_qq = c, P_pq = s, P_qp = -s (the classical
+// Jacobi method, e.g. Golub and Van Loan, Matrix Computations, 8.5).
+void jacobi_eigen(std::vector<double>& a, std::vector<double>& v, int64_t n)
+{
+    v.assign((size_t)(n * n), 0.0);
+    for (int64_t i = 0; i < n; ++i) v[i * n + i] = 1.0;
+    auto A = [&](int64_t i, int64_t j) -> double& { return a[i * n + j]; };
+    double total = 0;
+    for (double x : a) total += x * x;
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = 0;
+        for (int64_t i = 0; i < n; ++i)
+            for (int64_t j = i + 1; j < n; ++j) off += A(i, j) * A(i, j);
+        if (!(off > 1e-32 * total)) return;
+        for (int64_t p = 0; p < n; ++p)
+            for (int64_t q = p + 1; q < n; ++q) {
+                const double apq = A(p, q);
+                if (apq == 0.0) continue;
+                const double theta = (A(q, q) - A(p, p)) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+                for (int64_t r = 0; r < n; ++r) {  // columns p, q
+                    const double arp = A(r, p), arq = A(r, q);
+                    A(r, p) = c * arp - sn * arq;
+                    A(r, q) = sn * arp + c * arq;
+                }
+                for (int64_t r = 0; r < n; ++r) {  // rows p, q
+                    const double apr = A(p, r), aqr = A(q, r);
+                    A(p, r) = c * apr - sn * aqr;
+                    A(q, r) = sn * apr + c * aqr;
+                }
+                A(p, q) = A(q, p) = 0.0;
+                for (int64_t r = 0; r < n; ++r) {
+                    const double vrp = v[r * n + p], vrq = v[r * n + q];
+                    v[r * n + p] = c * vrp - sn * vrq;
+                    v[r * n + q] = sn * vrp + c * vrq;
+                }
+            }
+    }
+    throw std::runtime_error("Jacobi eigensolver of the fit Gram did not converge");
+}
 
 int grid_for(int64_t work, int64_t cap = 65536)
 {
@@ -249,6 +302,45 @@ void transpose(const double* in, int64_t rows, int64_t cols, const double* div, 
     MPISDF_HIP(hipGetLastError());
 }
 
+// Pseudo-inverse factor of G2 (see fit_factor): eigen-decomposition of G2 on
+// the host, W = diag(lambda)^-1/2 V^T over the eigenvalues above kEigenCut of
+// the largest (largest first), as a k x k row-major linv padded with zero
+// rows and its transpose linvt.
+void fit_pseudo_inverse(FitFactor& f, int64_t k, int64_t d, hipStream_t s)
+{
+    DeviceBuffer<double> g((size_t)(k * k));
+    gemm_tn_f64_fused(TnFused::Square, k, k, d, 1.0, f.phi_a.ptr, d, f.phi_a.ptr, d, g.ptr, k, nullptr, nullptr,
+                      nullptr, s);
+    std::vector<double> a = g.download(s), v;
+    jacobi_eigen(a, v, k);
+    std::vector<int64_t> order((size_t)k);
+    for (int64_t i = 0; i < k; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return a[x * k + x] > a[y * k + y]; });
+    const double lmax = a[order[0] * k + order[0]];
+    if (!(lmax > 0)) throw std::runtime_error("ISDF fit Gram has zero numerical rank");
+    std::vector<double> w((size_t)(k * k), 0.0), wt((size_t)(k * k), 0.0);
+    int64_t r = 0;
+    double lmin = lmax;
+    for (int64_t t = 0; t < k; ++t) {
+        const int64_t j = order[t];
+        const double lam = a[j * k + j];
+        if (!(lam > kEigenCut * lmax)) break;
+        const double sc = 1.0 / std::sqrt(lam);
+        for (int64_t i = 0; i < k; ++i) {
+            w[t * k + i] = v[i * k + j] * sc;
+            wt[i * k + t] = v[i * k + j] * sc;
+        }
+        lmin = lam;
+        ++r;
+    }
+    f.linv.upload(w.data(), w.size(), s);
+    f.linvt.upload(wt.data(), wt.size(), s);
+    MPISDF_HIP(hipStreamSynchronize(s));
+    f.triangular = false;
+    f.retained_rank = r;
+    f.condition_estimate = lmax / lmin;
+}
+
 }  // namespace
 
 void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f, hipStream_t s)
@@ -298,10 +390,23 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
     int hinfo = 0;
     MPISDF_HIP(hipMemcpyAsync(&hinfo, info.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
     MPISDF_HIP(hipStreamSynchronize(s));
-    if (hinfo)
-        throw std::runtime_error("ISDF fit Gram (Phi_a Phi_a^T)^2 is not positive definite at row " +
-                                 std::to_string(hinfo) + " (pivot below 1e-14 of its diagonal): the anchor pair "
-                                 "columns are numerically dependent");
+    f.triangular = true;
+    f.retained_rank = k;
+    if (hinfo) {
+        // Dependent anchor pairs (a Schur pivot below 1e-14 of its diagonal,
+        // e.g. more anchors than independent pairs).  The reference's SVD
+        // refit keeps the directions above its cutoff (python/isdf.py:238-263);
+        // here G2 = V diag(lambda) V^T, and the rows of W = diag(lambda)^-1/2 V^T
+        // over the kept eigenvalues replace L^-1: ||W s||^2 = s^T G2^+ s and
+        // W^T W s = G2^+ s, the minimum-norm least-squares coefficients.
+        if (k > kPinvMaxAnchors)
+            throw std::runtime_error("ISDF fit Gram (Phi_a Phi_a^T)^2 is not positive definite at row " +
+                                     std::to_string(hinfo) + " (pivot below 1e-14 of its diagonal): the anchor "
+                                     "pair columns are numerically dependent, and the pseudo-inverse fallback "
+                                     "takes at most " + std::to_string(kPinvMaxAnchors) + " anchors");
+        fit_pseudo_inverse(f, k, d, s);
+        return;
+    }
     // L^-1 right-looking, in place on X = I (row-major, lower triangular):
     // for each block J of NB rows (w = j0 + jb),
     //   X[J, :w] <- inv(L_JJ) X[J, :w],
@@ -380,7 +485,7 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         // Z = L^-1 S (lower triangular A), with ||z_g||^2 summed in the epilogue
         timer.begin(s);
         gemm_tn_f64_fused(TnFused::ColumnSumSquares, k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, z.ptr, k, zz.ptr,
-                          nullptr, nullptr, s, 1);
+                          nullptr, nullptr, s, f.triangular ? 1 : 0);
         timer.end(s);
         gemm_flops += (double)k * (double)(k + 1) * (double)nb;
         if (err_out) {
@@ -401,9 +506,9 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
             timer.begin(s);
             if (tsum_out)
                 gemm_tn_f64_fused(TnFused::RowDot, k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, th.ptr, k, nullptr,
-                                  tpart.ptr, w + start, s, 2);
+                                  tpart.ptr, w + start, s, f.triangular ? 2 : 0);
             else
-                gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, th.ptr, k, s, 2);
+                gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, th.ptr, k, s, f.triangular ? 2 : 0);
             timer.end(s);
             if (tsum_out) launch_sum_partials(tpart.ptr, tn_column_blocks(nb), k, tsum.ptr, true, s);
             gemm_flops += (double)k * (double)(k + 1) * (double)nb;
@@ -481,11 +586,11 @@ void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xy
         gemm_tn_f64(k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, 0.0, tm.ptr, k, s);
         hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, tm.ptr, tm.ptr,
                            (const double*)nullptr, k, k * nb, sm.ptr);
-        gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s, 1);
+        gemm_tn_f64(k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, 0.0, z.ptr, k, s, f.triangular ? 1 : 0);
         hipLaunchKernelGGL(pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, z.ptr, k, phi.ptr, d,
                            wb, nb, ept.ptr, batch);
         launch_sum_fixed(ept.ptr, nb, batch, 2, acc.ptr, s);
-        gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, cm.ptr, k, s, 2);
+        gemm_tn_f64(k, nb, k, 1.0, f.linvt.ptr, k, z.ptr, k, 0.0, cm.ptr, k, s, f.triangular ? 2 : 0);
         // Grid adjoint: V = Phi_a^T (C o T) stored [g][mu]; abar_g = 4|w|(||phi||^2 phi - V)
         hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, cm.ptr, tm.ptr,
                            (const double*)nullptr, k, k * nb, um.ptr);

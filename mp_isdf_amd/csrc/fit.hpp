@@ -11,11 +11,17 @@ struct FitFactor {
     DeviceBuffer<double> linv;   // [k][k] row-major inverse of the Cholesky factor of G2
     DeviceBuffer<double> linvt;  // [k][k] row-major transpose of linv (upper triangular)
     double condition_estimate = 0, gram_diag_max = 0;
+    // false: linv holds W = diag(lambda)^-1/2 V^T of the kept eigenpairs of G2
+    // (retained_rank rows, then zero rows) in place of L^-1, a full matrix.
+    bool triangular = true;
+    int64_t retained_rank = 0;
 };
 
 // Anchor AOs, G2 = (Phi_a Phi_a^T)^2, Cholesky and triangular inverse.
-// anchor_xyz is a device pointer [k][3].  Throws std::runtime_error when G2
-// is not positive definite.
+// anchor_xyz is a device pointer [k][3].  When the Cholesky finds the anchor
+// pairs numerically dependent, G2 is eigen-decomposed instead and its
+// pseudo-inverse over the eigenvalues above 1e-14 of the largest is used
+// (k <= 2048; std::runtime_error beyond).
 void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f, hipStream_t s);
 
 // Streams grid batches (device xyz[n][3], device w[n]) through

@@ -354,7 +354,7 @@ PYBIND11_MODULE(mp_isdf, m)
         double* ps = nullptr;
         if (interp_vecs) { theta = py::array_t<double>({k, n}); pt = theta.mutable_data(); }
         if (weight_sums) { tsum = py::array_t<double>(k); ps = tsum.mutable_data(); }
-        std::vector<double> res(8, 0.0);
+        std::vector<double> res(10, 0.0);
         int status;
         {
             py::gil_scoped_release release;
@@ -372,6 +372,8 @@ PYBIND11_MODULE(mp_isdf, m)
         diag["gram_diag_max"] = res[3];
         diag["seconds_factor"] = res[4];
         diag["seconds_grid_pass"] = res[5];
+        diag["retained_rank"] = (int64_t)res[8];
+        diag["factorization"] = res[9] != 0 ? "pseudo-inverse" : "cholesky";
         out["fit_diagnostics"] = diag;
         out["interp_vecs"] = interp_vecs ? py::object(theta) : py::object(py::none());
         out["theta_weight_sums"] = weight_sums ? py::object(tsum) : py::object(py::none());

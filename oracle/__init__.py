@@ -391,10 +391,20 @@ def select_cvt(points, weights, cvt_weights, max_rank, seed=7, cvt_max_iter=100,
 
 
 def rbrp_ao(ao, weights, batch_size=128, block_size=16, tolerance=1e-10, max_rank=0, seed=7, tau_block=None,
-            rank_rtol=1e-12, refresh_every=8, svd_rtol=1e-12, compute_id=False):
+            rank_rtol=1e-12, refresh_every=8, svd_rtol=1e-12, compute_id=False, branch="host"):
     """cpp/src/anchor_select_rbrp.cpp:186-318 (RBRP) and :135-184 (the ID),
     with LAPACK's dgeqp3/dorgqr (scipy.linalg.qr) and gesdd (numpy.linalg.svd)
-    as the reference calls them, and libstdc++'s discrete_distribution."""
+    as the reference calls them, and libstdc++'s discrete_distribution.
+
+    branch="device" restates the reference's device branch instead
+    (cpp/src/anchor_select_rbrp_device.cpp:327-491), which the driver takes on
+    a GPU build (python/isdf.py:572-576): each draw is
+    uniform_real_distribution(0, total) over the prefix sums of the unmasked
+    residuals and takes the first positive, unmasked index whose prefix exceeds
+    it (the first positive one if rounding leaves none, :129-160); a block asks
+    for min(block, cap - rank) draws and records how many it got; the new
+    basis is an unpivoted QR of the accepted rows, projected and factorised
+    twice (:431-451).  Prefix sums run left to right here."""
     import scipy.linalg
     ao = np.ascontiguousarray(ao, dtype=np.float64)
     w = np.ascontiguousarray(weights, dtype=np.float64)
@@ -422,8 +432,26 @@ def rbrp_ao(ao, weights, batch_size=128, block_size=16, tolerance=1e-10, max_ran
     chosen = np.zeros(n, dtype=bool)
     indices, history = [], []
     status = "rank_cap"
+    if branch not in ("host", "device"):
+        raise ValueError("branch must be 'host' or 'device'")
     lib().oracle_rng_create.restype = ctypes.c_void_p
     lib().oracle_rng_discrete.restype = ctypes.c_int64
+    lib().oracle_rng_uniform.restype = ctypes.c_double
+    lib().oracle_rng_uniform.argtypes = [ctypes.c_void_p, ctypes.c_double]
+
+    def device_draw(mask):
+        live = np.where(mask, 0.0, d)
+        prefix = np.cumsum(live)  # sequential left-to-right sums
+        tot = float(prefix[-1])
+        if not (tot > 0) or not np.isfinite(tot):
+            return n
+        target = lib().oracle_rng_uniform(ctypes.c_void_p(rng), tot)
+        ok = (~mask) & (d > 0)
+        hit = np.flatnonzero(ok & (prefix > target))
+        if hit.size:
+            return int(hit[0])
+        first = np.flatnonzero(ok)
+        return int(first[0]) if first.size else n
     rng = lib().oracle_rng_create(ctypes.c_uint64(seed))
 
     def residual():
@@ -442,16 +470,30 @@ def rbrp_ao(ao, weights, batch_size=128, block_size=16, tolerance=1e-10, max_ran
                 break
             if old >= cap:
                 break
-            active = [i for i in range(n) if d[i] > 0 and not chosen[i]]
-            if not active:
-                status = "no_candidates"
-                break
-            b = min(block_size, cap - old, len(active))
-            sample = []
-            for _ in range(b):
-                wl = np.ascontiguousarray(d[active])
-                pos = lib().oracle_rng_discrete(ctypes.c_void_p(rng), _p(wl), ctypes.c_int64(len(wl)))
-                sample.append(active.pop(pos))
+            if branch == "device":
+                mask = np.zeros(n, dtype=bool)
+                sample = []
+                for _ in range(min(block_size, cap - old)):
+                    idx = device_draw(mask)
+                    if idx == n:
+                        break
+                    mask[idx] = True
+                    sample.append(idx)
+                if not sample:
+                    status = "no_candidates"
+                    break
+                b = len(sample)
+            else:
+                active = [i for i in range(n) if d[i] > 0 and not chosen[i]]
+                if not active:
+                    status = "no_candidates"
+                    break
+                b = min(block_size, cap - old, len(active))
+                sample = []
+                for _ in range(b):
+                    wl = np.ascontiguousarray(d[active])
+                    pos = lib().oracle_rng_discrete(ctypes.c_void_p(rng), _p(wl), ctypes.c_int64(len(wl)))
+                    sample.append(active.pop(pos))
             v = rows[sample].T.copy()
             original2 = float(np.sum(v * v))
             for _ in range(2):
@@ -477,18 +519,28 @@ def rbrp_ao(ao, weights, batch_size=128, block_size=16, tolerance=1e-10, max_ran
             if not keep:
                 status = "numerical_rank"
                 break
-            qb = qf[:, :keep].T.copy()
-            for j in range(keep):
-                col = qb[j]
+            if branch == "device":
+                # fresh rows of the accepted samples; (project twice, unpivoted QR) twice
+                acc = rows[[sample[piv[j]] for j in range(keep)]].T.copy()
                 for _ in range(2):
-                    for t in range(old):
-                        col -= (q[t] @ col) * q[t]
-                    for t in range(j):
-                        col -= (qb[t] @ col) * qb[t]
-                norm = np.linalg.norm(col)
-                if not (norm > 0) or not np.isfinite(norm):
-                    raise RuntimeError("RBRP block lost orthogonality")
-                qb[j] = col / norm
+                    for _ in range(2):
+                        if old:
+                            acc -= q.T @ (q @ acc)
+                    acc = scipy.linalg.qr(acc, mode="economic")[0]
+                qb = acc.T.copy()
+            else:
+                qb = qf[:, :keep].T.copy()
+                for j in range(keep):
+                    col = qb[j]
+                    for _ in range(2):
+                        for t in range(old):
+                            col -= (q[t] @ col) * q[t]
+                        for t in range(j):
+                            col -= (qb[t] @ col) * qb[t]
+                    norm = np.linalg.norm(col)
+                    if not (norm > 0) or not np.isfinite(norm):
+                        raise RuntimeError("RBRP block lost orthogonality")
+                    qb[j] = col / norm
             dots = rows @ qb.T
             dots[chosen] = 0
             d = np.where(chosen, 0.0, np.maximum(0.0, d - np.sum(dots * dots, axis=1)))

@@ -821,3 +821,11 @@ extern "C" int64_t oracle_rng_discrete(void* rng, const double* weights, int64_t
     std::discrete_distribution<size_t> dist(weights, weights + n);
     return (int64_t)dist(*static_cast<std::mt19937_64*>(rng));
 }
+// std::uniform_real_distribution<double>(0, total) on the same engine: the
+// draw of the reference's device RBRP sampler
+// (cpp/src/anchor_select_rbrp_device.cpp:129-160).
+extern "C" double oracle_rng_uniform(void* rng, double total)
+{
+    std::uniform_real_distribution<double> dist(0.0, total);
+    return dist(*static_cast<std::mt19937_64*>(rng));
+}

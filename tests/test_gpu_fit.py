@@ -55,12 +55,43 @@ def test_fit_batch_invariance(mp_isdf):
         np.testing.assert_allclose(got["pair_space_error"]["l2"], ref["pair_space_error"]["l2"], rtol=1e-10)
 
 
-def test_dependent_anchors_raise(mp_isdf):
-    mol, pts, w, anchors = water_case(6)
+def test_dependent_anchors_take_the_pseudo_inverse(mp_isdf):
+    """Numerically dependent anchor pairs (python/isdf.py:238-263 keeps the
+    singular values above 1e-12 of the largest): more anchors than the
+    nao(nao+1)/2 = 28 independent pairs of water STO-3G, and a duplicated
+    anchor.  The fit falls back to the pseudo-inverse of the pair Gram, with
+    the reference's retained rank, its minimum-norm theta and its pair error."""
+    mol = molecule.single_water("sto-3g")
+    rng = np.random.default_rng(3)
+    pts = rng.normal(scale=1.4, size=(900, 3))
+    w = rng.uniform(0.05, 1.0, size=900)
     ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
-    dup = np.concatenate([anchors, anchors[:1]])
-    with pytest.raises(RuntimeError, match="positive definite"):
-        mp_isdf.isdf_fit(ctx, pts[dup], w[dup], pts, w)
+    ao = oracle.ao_eval(mol.atm, mol.bas, mol.env, pts)
+    _, _, _, six = water_case(6)
+    for anchors in (rng.choice(900, size=40, replace=False), np.concatenate([six, six[:1]])):
+        fit = mp_isdf.isdf_fit(ctx, pts[anchors], w[anchors], pts, w, batch_size=128, interp_vecs=True,
+                               weight_sums=True)
+        aa = ao[anchors]
+        pairs = (aa[:, :, None] * aa[:, None, :]).reshape(len(anchors), -1)
+        theta, _, diag = oracle.physical_pair_refit(pairs, w[anchors], ao, 256)
+        assert fit["fit_diagnostics"]["factorization"] == "pseudo-inverse"
+        assert fit["fit_diagnostics"]["retained_rank"] == diag["retained_rank"] < len(anchors)
+        cond = fit["fit_diagnostics"]["condition_estimate"]  # of G2 on the kept eigenvalues
+        rel = np.linalg.norm(fit["interp_vecs"] - theta) / np.linalg.norm(theta)
+        rel_w = np.linalg.norm(fit["theta_weight_sums"] - theta @ w) / np.linalg.norm(theta @ w)
+        err = oracle.stream_pair_space_error_norms(ao, ao, pairs, w[anchors], theta, w)
+        print(f"k {len(anchors)}: retained {diag['retained_rank']}, cond(G2) {cond:.3e}, theta {rel:.3e}, "
+              f"theta.w {rel_w:.3e}, pair error {fit['pair_space_error']['relative']:.6e} vs {err['relative']:.6e}")
+        # The normal equations square the pair matrix's condition: theta is
+        # accurate to about eps cond(G2).
+        assert rel <= max(1e-7, 100 * np.finfo(float).eps * cond)
+        assert rel_w <= max(1e-7, 100 * np.finfo(float).eps * cond)
+        # The error is evaluated as ||phi||^4 - ||z||^2 per point: its cancellation
+        # leaves ~sqrt(eps) of relative error resolvable (this fit is exact to 1e-10).
+        np.testing.assert_allclose(fit["pair_space_error"]["relative"], err["relative"], rtol=1e-6, atol=1e-7)
+    many = rng.choice(900, size=2049, replace=True)
+    with pytest.raises(RuntimeError, match="at most 2048"):
+        mp_isdf.isdf_fit(ctx, pts[many], w[many], pts[:50], w[:50])
 
 
 def test_build_isdf_driver(mp_isdf):

@@ -54,7 +54,7 @@ def test_precision_forms_at_equal_fit_error(mp_isdf, name):
         return anchors
 
     def fit(anchors):
-        res = np.zeros(8)
+        res = np.zeros(10)
         capi.call("mpisdf_isdf_fit_device", cctx.handle, torch.tensor(xyz[anchors], device="cuda"),
                   np.ascontiguousarray(w[anchors]), k, xyz_dev, w_dev, n, FIT_BATCH, None, res, None)
         return res[1]

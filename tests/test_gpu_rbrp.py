@@ -42,6 +42,11 @@ def test_device_points_match_oracle(mp_isdf):
     np.testing.assert_allclose(got["history"], want["history"], rtol=1e-9, atol=1e-12)
     scale = np.abs(want["interp_vecs"]).max()
     assert np.abs(got["interp_vecs"] - want["interp_vecs"]).max() < 1e-8 * scale
+    # The reference driver calls the device branch on a GPU build (isdf.py:572-576):
+    # its sampler and basis, restated, select the same anchors.
+    dev = oracle.rbrp_ao(ao, w, max_rank=40, seed=7, tolerance=0.0, compute_id=True, branch="device")
+    assert got["indices"] == dev["indices"] and got["status"] == dev["status"]
+    assert np.abs(got["interp_vecs"] - dev["interp_vecs"]).max() < 1e-8 * scale
     with pytest.raises(ValueError, match="nonzero"):
         mp_isdf.rbrp_ao_device(sel, xyz, np.where(np.arange(len(w)) == 3, 0.0, w), max_rank=4)
 
