@@ -121,9 +121,12 @@ int mpisdf_select_cvt(const double* points, const double* weights, const double*
  * tau_block NaN selects 1/block_size.  Outputs (capacity >= the rank cap
  * min(nao(nao+1)/2, n, max_rank or n)): indices (rows, selection order),
  * interp_vecs[k*n] and transform[k*nao*nao] when compute_id (may be NULL),
- * history[4 per block] (may be NULL), *nblocks, scalars[3] = relative
+ * history[4 per block] (may be NULL; every block adds at least one anchor,
+ * so 4*capacity doubles always suffice), *nblocks, scalars[3] = relative
  * residual, condition, retained rank; *status 0 rank_cap, 1 tolerance,
- * 2 no_candidates, 3 numerical_rank, 4 roundoff_limited. */
+ * 2 no_candidates, 3 numerical_rank, 4 roundoff_limited.  batch_size must be
+ * positive, as in the reference; the pair rows are formed once in one device
+ * buffer (n * nao(nao+1)/2 doubles), so it does not bound memory here. */
 int mpisdf_rbrp_ao(const double* ao, const double* weights, int64_t n, int64_t nao, int64_t batch_size,
                    int64_t block_size, double tolerance, int64_t max_rank, uint64_t seed, double tau_block,
                    double rank_rtol, int64_t refresh_every, double svd_rtol, int compute_id, int64_t* indices,
@@ -258,6 +261,20 @@ int mpisdf_isdf_fit(const mpisdf_context* ctx, const double* anchor_xyz, const d
 int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,
                            const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* tsum,
                            double* results, void* stream);
+
+/* Pair-space error of given interpolation vectors theta (host, row-major
+ * [k][n]) over the grid points xyz[n*3], w[n] (host), as the reference's
+ * _stream_pair_space_error_norms (python/isdf.py:176-191) evaluates it, in
+ * O(k) per point: ||phi_g||^4 - 2 c.s_g + c.(G2 c), c = anchor_w o theta[:, g],
+ * s_g = (Phi_a phi_g)^2.  results[2]: 0 l2 error, 1 relative error. */
+int mpisdf_pair_space_error(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w, int64_t k,
+                            const double* xyz, const double* w, int64_t n, const double* theta, int64_t batch,
+                            double* results);
+/* Grid-quadrature AO overlap out[nao*nao] = sum_g w_g phi_g phi_g^T (host
+ * arrays; the AO values never leave the GPU), the quadrature side of the
+ * reference's overlap check (python/isdf.py:665-671). */
+int mpisdf_quadrature_overlap(const mpisdf_context* ctx, const double* xyz, const double* w, int64_t n, int64_t batch,
+                              double* out);
 
 /* ---- Adjoint (no reference counterpart: the reference has no nuclear
  * derivatives; it provides the dx/dy/dz AO derivatives the adjoint is checked

@@ -531,6 +531,44 @@ extern "C" int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* a
     });
 }
 
+extern "C" int mpisdf_pair_space_error(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w,
+                                       int64_t k, const double* xyz, const double* w, int64_t n, const double* theta,
+                                       int64_t batch, double* results)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (k <= 0 || n < 0 || batch <= 0 || !anchor_xyz || !anchor_w || !results || (n > 0 && (!xyz || !w || !theta)))
+            throw std::invalid_argument("invalid pair-space error arguments");
+        mpisdf::DeviceBuffer<double> dax, dxyz, dw;
+        dax.upload(anchor_xyz, 3 * k, nullptr);
+        double err[2] = {0, 0};
+        if (n > 0) {
+            dxyz.upload(xyz, 3 * n, nullptr);
+            dw.upload(w, n, nullptr);
+            mpisdf::theta_pair_error(c, dax.ptr, anchor_w, k, dxyz.ptr, dw.ptr, n, theta, batch, err, nullptr);
+        }
+        results[0] = std::sqrt(err[0]);
+        results[1] = err[1] > 0 ? std::sqrt(err[0] / err[1]) : (err[0] == 0 ? 0.0 : INFINITY);
+    });
+}
+
+extern "C" int mpisdf_quadrature_overlap(const mpisdf_context* ctx, const double* xyz, const double* w, int64_t n,
+                                         int64_t batch, double* out)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0 || batch <= 0 || !out || (n > 0 && (!xyz || !w)))
+            throw std::invalid_argument("invalid quadrature overlap arguments");
+        mpisdf::DeviceBuffer<double> dxyz, dw, ds((size_t)(c.nao * c.nao));
+        if (n > 0) {
+            dxyz.upload(xyz, 3 * n, nullptr);
+            dw.upload(w, n, nullptr);
+        }
+        mpisdf::quadrature_overlap(c, dxyz.ptr, dw.ptr, n, batch, ds.ptr, nullptr);
+        MPISDF_HIP(hipMemcpy(out, ds.ptr, (size_t)(c.nao * c.nao) * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
 extern "C" int mpisdf_ao_adjoint_device(const mpisdf_context* ctx, const double* xyz, int64_t n, const double* abar,
                                         int64_t ldab, double* gout, void* stream)
 {
@@ -711,6 +749,7 @@ extern "C" int mpisdf_select_cvt(const double* points, const double* weights, co
             throw std::invalid_argument("CVT points, quadrature weights and CVT weights must have equal lengths");
         const auto sel = mpisdf::select_cvt(points, weights, cvt_weights, n, rank, seed, max_iter, tol, nullptr);
         if ((int64_t)sel.size() > capacity) throw std::invalid_argument("output capacity is too small");
+        if (!sel.empty() && !out) throw std::invalid_argument("CVT output buffer is NULL");
         std::copy(sel.begin(), sel.end(), out);
         *count = (int64_t)sel.size();
     });
@@ -735,6 +774,7 @@ int rbrp_impl(const mpisdf_context* ctx, const double* points, const double* ao_
         if (ctx) {
             auto& c = unwrap(ctx);
             nao = c.nao;
+            if (n > 0 && !points) throw std::invalid_argument("expected points[grid,3] and weights[grid]");
             for (int64_t g = 0; g < n; ++g) {
                 if (!std::isfinite(weights[g]) || weights[g] == 0)
                     throw std::invalid_argument("device RBRP requires finite nonzero weights");

@@ -302,6 +302,46 @@ void transpose(const double* in, int64_t rows, int64_t cols, const double* div, 
     MPISDF_HIP(hipGetLastError());
 }
 
+// Pair error of given coefficients, one wave per point g of the batch:
+// c = column g of C (k x nb), s and t = columns of S = (Phi_a phi_g)^2 and
+// T = G2 C;  out[g] = |w_g| (||phi_g||^4 - 2 c.s + c.t), out[ld + g] = |w_g| ||phi_g||^4.
+__global__ void theta_pair_error_kernel(const double* __restrict__ c, const double* __restrict__ sm,
+                                        const double* __restrict__ t, int64_t k, const double* __restrict__ phi,
+                                        int64_t d, const double* __restrict__ w, int64_t nb, double* __restrict__ out,
+                                        int64_t ld)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (g >= nb) return;
+    double cs = 0, ct = 0, pp = 0;
+    for (int64_t i = lane; i < k; i += 64) {
+        const double ci = c[i + g * k];
+        cs += ci * sm[i + g * k];
+        ct += ci * t[i + g * k];
+    }
+    for (int64_t a = lane; a < d; a += 64) { const double v = phi[g * d + a]; pp += v * v; }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        cs += __shfl_xor(cs, off);
+        ct += __shfl_xor(ct, off);
+        pp += __shfl_xor(pp, off);
+    }
+    if (lane == 0) {
+        const double qn = pp * pp, aw = fabs(w[g]);
+        out[g] = aw * fmax(qn - 2.0 * cs + ct, 0.0);
+        out[ld + g] = aw * qn;
+    }
+}
+
+// out[g * d + a] = w[g] in[g * d + a]
+__global__ void scale_rows_kernel(const double* __restrict__ in, int64_t count, int64_t d,
+                                  const double* __restrict__ w, double* __restrict__ out)
+{
+    const int64_t total = count * d;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = w[i / d] * in[i];
+}
+
 // Pseudo-inverse factor of G2 (see fit_factor): eigen-decomposition of G2 on
 // the host, W = diag(lambda)^-1/2 V^T over the eigenvalues above kEigenCut of
 // the largest (largest first), as a k x k row-major linv padded with zero
@@ -546,6 +586,62 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         gemm_stats[0] = timer.seconds();
         gemm_stats[1] = gemm_flops;
         gemm_stats[2] = (double)timer.launches();
+    }
+}
+
+void theta_pair_error(Context& ctx, const double* anchor_xyz, const double* anchor_w, int64_t k, const double* xyz,
+                      const double* w, int64_t n, const double* theta_host, int64_t batch, double* err_out,
+                      hipStream_t s)
+{
+    const int64_t d = ctx.nao;
+    batch = std::max<int64_t>(1, std::min(batch, std::max<int64_t>(n, 1)));
+    DeviceBuffer<double> phi_a((size_t)(k * d)), g2((size_t)(k * k)), phi((size_t)(batch * d)),
+        sm((size_t)(k * batch)), cm((size_t)(k * batch)), tm((size_t)(k * batch)), ept((size_t)(2 * batch)), acc(2);
+    launch_ao_eval(ctx, anchor_xyz, k, kValues, kF64, phi_a.ptr, d, k * d, s);
+    gemm_tn_f64_fused(TnFused::Square, k, k, d, 1.0, phi_a.ptr, d, phi_a.ptr, d, g2.ptr, k, nullptr, nullptr, nullptr,
+                      s);
+    MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
+    std::vector<double> stage((size_t)(k * batch));
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t nb = std::min(batch, n - start);
+        // c[t + g k] = w_anchor[t] theta[t][start + g] (the physical
+        // coefficients); the previous batch's copy out of ``stage`` and its
+        // reads of cm are finished first.
+        MPISDF_HIP(hipStreamSynchronize(s));
+        for (int64_t t = 0; t < k; ++t) {
+            const double* row = theta_host + t * n + start;
+            for (int64_t g = 0; g < nb; ++g) stage[(size_t)(t + g * k)] = anchor_w[t] * row[g];
+        }
+        MPISDF_HIP(hipMemcpyAsync(cm.ptr, stage.data(), (size_t)(k * nb) * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
+        gemm_tn_f64_fused(TnFused::Square, k, nb, d, 1.0, phi_a.ptr, d, phi.ptr, d, sm.ptr, k, nullptr, nullptr,
+                          nullptr, s);
+        gemm_tn_f64(k, nb, k, 1.0, g2.ptr, k, cm.ptr, k, 0.0, tm.ptr, k, s);  // G2 symmetric
+        hipLaunchKernelGGL(theta_pair_error_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, cm.ptr, sm.ptr,
+                           tm.ptr, k, phi.ptr, d, w + start, nb, ept.ptr, batch);
+        MPISDF_HIP(hipGetLastError());
+        launch_sum_fixed(ept.ptr, nb, batch, 2, acc.ptr, s);
+    }
+    const auto h = acc.download(s);
+    err_out[0] = h[0];
+    err_out[1] = h[1];
+}
+
+void quadrature_overlap(Context& ctx, const double* xyz, const double* w, int64_t n, int64_t batch, double* out,
+                        hipStream_t s)
+{
+    const int64_t d = ctx.nao;
+    batch = std::max<int64_t>(1, std::min(batch, std::max<int64_t>(n, 1)));
+    DeviceBuffer<double> phi((size_t)(batch * d)), wphi((size_t)(batch * d));
+    MPISDF_HIP(hipMemsetAsync(out, 0, (size_t)(d * d) * sizeof(double), s));
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t nb = std::min(batch, n - start);
+        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
+        hipLaunchKernelGGL(scale_rows_kernel, dim3((unsigned)grid_for(nb * d)), dim3(256), 0, s, phi.ptr, nb, d,
+                           w + start, wphi.ptr);
+        MPISDF_HIP(hipGetLastError());
+        // out += Phi_b^T diag(w) Phi_b, batches in order
+        gemm_f64(d, d, nb, 1.0, MatView{phi.ptr, 1, d}, MatView{wphi.ptr, d, 1}, 1.0, MutView{out, d, 1}, s);
     }
 }
 

@@ -39,6 +39,20 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
                 int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s,
                 double* gemm_stats = nullptr);
 
+// Pair-space error of given interpolation vectors (theta_host: row-major
+// [k][n] on the host), as python/isdf.py:176-191 evaluates it: per point
+// ||phi_g||^4 - 2 c.s_g + c.(G2 c) with c = w_anchor o theta[:, g] and
+// s_g = (Phi_a phi_g)^2.  err_out (host, 2): sum |w| err^2, sum |w| ||phi||^4.
+// anchor_xyz, xyz, w: device; anchor_w: host.
+void theta_pair_error(Context& ctx, const double* anchor_xyz, const double* anchor_w, int64_t k, const double* xyz,
+                      const double* w, int64_t n, const double* theta_host, int64_t batch, double* err_out,
+                      hipStream_t s);
+
+// Grid-quadrature AO overlap out[nao][nao] (device) = sum_g w_g phi_g phi_g^T
+// over device xyz[n][3], w[n], in batches added in order.
+void quadrature_overlap(Context& ctx, const double* xyz, const double* w, int64_t n, int64_t batch, double* out,
+                        hipStream_t s);
+
 // Pair-reconstruction loss L = sum_g |w_g| ||phi_g phi_g^T - sum_k c_gk phi_k phi_k^T||_F^2
 // at the fitted coefficients (= the squared l2 pair-space error) and its
 // adjoint contracted with AO gradients: gao_out (host, [3][nao]) =

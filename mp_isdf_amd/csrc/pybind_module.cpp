@@ -283,6 +283,22 @@ PYBIND11_MODULE(mp_isdf, m)
             return c.evaluate(coords, true, space);
         }, py::arg("coords"), py::arg("space") = "auto",
            "Return x,y,z AO gradients with shape (3, ngrid, nao).")
+        .def("quadrature_overlap", [](const Context& c, const FloatArray& coords, const FloatArray& weights,
+                                      int64_t batch_size) {
+            if (coords.ndim() != 2 || coords.shape(1) != 3 || weights.ndim() != 1 || weights.shape(0) != coords.shape(0))
+                throw py::value_error("expected coords[n,3] and weights[n]");
+            py::array_t<double> s({(py::ssize_t)c.nao(), (py::ssize_t)c.nao()});
+            int status;
+            double* ps = s.mutable_data();
+            {
+                py::gil_scoped_release release;
+                status = mpisdf_quadrature_overlap(c.get(), coords.data(), weights.data(), coords.shape(0), batch_size,
+                                                   ps);
+            }
+            check(status);
+            return s;
+        }, py::arg("coords"), py::arg("weights"), py::arg("batch_size") = 65536,
+           "Grid-quadrature AO overlap sum_g w_g phi_g phi_g^T (nao, nao), evaluated on the GPU.")
         .def("overlap", [](const Context& c) {
             py::array_t<double> s({(py::ssize_t)c.nao(), (py::ssize_t)c.nao()});
             check(mpisdf_overlap(c.get(), s.mutable_data()));
@@ -383,6 +399,32 @@ PYBIND11_MODULE(mp_isdf, m)
        py::arg("weight_sums") = false,
        "GPU ISDF fit on anchors (normal equations of the anchor-pair least squares, FP64 Cholesky): "
        "pair-space error, optional interpolation vectors theta[k,n] and sum_g theta[:,g] w_g.");
+
+    m.def("pair_space_error", [](const Context& ctx, const FloatArray& anchor_coords, const FloatArray& anchor_weights,
+                                 const FloatArray& coords, const FloatArray& weights, const FloatArray& interp_vecs,
+                                 int64_t batch_size) {
+        const py::ssize_t k = anchor_coords.shape(0), n = coords.shape(0);
+        if (anchor_coords.ndim() != 2 || anchor_coords.shape(1) != 3 || anchor_weights.ndim() != 1 ||
+            anchor_weights.shape(0) != k || coords.ndim() != 2 || coords.shape(1) != 3 || weights.ndim() != 1 ||
+            weights.shape(0) != n || interp_vecs.ndim() != 2 || interp_vecs.shape(0) != k ||
+            interp_vecs.shape(1) != n)
+            throw py::value_error("expected anchor_coords[k,3], anchor_weights[k], coords[n,3], weights[n] and "
+                                  "interp_vecs[k,n]");
+        double res[2] = {0, 0};
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_pair_space_error(ctx.get(), anchor_coords.data(), anchor_weights.data(), k, coords.data(),
+                                             weights.data(), n, interp_vecs.data(), batch_size, res);
+        }
+        check(status);
+        py::dict out;
+        out["l2"] = res[0];
+        out["relative"] = res[1];
+        return out;
+    }, py::arg("context"), py::arg("anchor_coords"), py::arg("anchor_weights"), py::arg("coords"), py::arg("weights"),
+       py::arg("interp_vecs"), py::arg("batch_size") = 4096,
+       "Pair-space error {l2, relative} of given interpolation vectors (python/isdf.py:176-191), on the GPU.");
 
     m.def("isdf_fit_gradient", [](const Context& ctx, const FloatArray& anchor_coords,
                                   const FloatArray& anchor_weights, const FloatArray& coords,

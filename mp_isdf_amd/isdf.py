@@ -41,6 +41,9 @@ KHATRIRAO_BACKENDS = {
     "density_col_norm": "RandomizedAODensityNormalizedKhatriRao",
 }
 DENSITY_SKETCHES = ("density", "density_col_norm")
+# interp_vecs=None returns theta[k, n] only up to this size; larger fits keep
+# theta on the device (the error and theta . w come from the grid pass).
+INTERP_VECS_AUTO_BYTES = 8 << 30
 AXES = {
     "ao": "AtomicOrbital",
     "dx": "AtomicOrbitalGradX",
@@ -54,7 +57,7 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                axes=("ao",), axis_weights=None, anchor_algorithm: str = "qrcp_dense", precision: str = "fp64",
                anchor_batch_size: int = 128, fit_batch_size: int = 4096, anchor_sketch_size: int = 128,
                anchor_oversampling: int = 16, anchor_seed: int = 7, mo_coeff=None, nocc: int = 0,
-               selection_only: bool = False, pair_space_error: bool = False, interp_vecs: bool = True,
+               selection_only: bool = False, pair_space_error: bool = False, interp_vecs=None,
                compute_metric: bool = False, metric_algorithm: str = "grid", sketch: str = "quadrature",
                point_importance=None, density_weight: str = "electron_density",
                ao_screening_threshold: float = 1e-24, cvt_weight: str = "electron_density",
@@ -62,6 +65,11 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                rbrp_rank_rtol: float = 1e-12, rbrp_refresh_every: int = 8, rbrp_svd_rtol: float = 1e-12,
                space: str = "auto", report_timings: bool = False, debug: bool = False):
     """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU.
+
+    ``interp_vecs``: True returns theta[k, n] (the reference always does);
+    None (default) returns it when it takes at most 8 GiB -- at (H2O)64 with
+    12,000 anchors it would be 221 GB -- and False never.  The pair-space
+    error and the overlap check need no theta on the host.
 
     ``compute_metric=True`` adds the anchor Coulomb metric
     ``anchor_weights[:, None] * grid_coulomb_metric(theta) * anchor_weights[None, :]``
@@ -125,7 +133,7 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     if compute_metric and metric_algorithm != "grid":
         raise NotImplementedError(f"metric_algorithm={metric_algorithm!r} (integral-direct ERI Cholesky) is not "
                                   "built; use metric_algorithm='grid'")
-    if compute_metric and not interp_vecs:
+    if compute_metric and interp_vecs is False:
         raise ValueError("compute_metric needs the interpolation vectors (interp_vecs=True)")
     tolerance = 1e-10 if anchor_tol is None else float(anchor_tol)
 
@@ -183,6 +191,9 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     weights = all_weights[keep]
     result = {
         "anchor_indices": anchor_indices,
+        "point_importance": importance,
+        "rbrp_diagnostics": ({key: rbrp_result[key] for key in ("relative_residual", "status", "history")}
+                             if rbrp_result is not None else None),
         "anchor_coords": anchor_coords,
         "anchor_weights": anchor_weights,
         "coordinates": coords,
@@ -193,8 +204,10 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         "overlap_reconstruction": None,
         "overlap_relative_error": None,
         "overlap_quadrature_relative_error": None,
-        "X1": None,
+        "X1": None, "X2": None, "X3": None, "X4": None,
         "coulomb_metric": None,
+        "derivative_integrals": {},
+        "target_factors": {},
         "timings": timings,
     }
     if selection_only:
@@ -202,37 +215,46 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         return result
 
     anchor_ao = context.eval_ao(anchor_coords)
-    result["X1"] = anchor_ao
+    result["X1"] = result["X2"] = result["X3"] = result["X4"] = anchor_ao
     record("ao_evaluation")
+    want_theta = (bool(interp_vecs) if interp_vecs is not None
+                  else len(anchor_indices) * len(coords) * 8 <= INTERP_VECS_AUTO_BYTES)
+    if compute_metric and not want_theta:
+        raise ValueError("compute_metric needs the interpolation vectors, which exceed 8 GiB here; pass "
+                         "interp_vecs=True to form them on the host")
     if anchor_algorithm == "rbrp_id":
-        # RBRP's ID is the fit (python/isdf.py:606-620); its relative residual
-        # is the pair-space error of that ID.
+        # RBRP's ID is the fit (python/isdf.py:606-620): the reference reports
+        # no fit diagnostics and keeps the ID's under rbrp_diagnostics["id"].
         theta = rbrp_result["interp_vecs"]
-        fit = {"interp_vecs": theta, "theta_weight_sums": theta @ weights,
-               "fit_diagnostics": {"retained_rank": rbrp_result["retained_rank"],
-                                   "condition": rbrp_result["condition"], "status": rbrp_result["status"]}}
-        if pair_space_error:
-            ao_kept = context.eval_ao(coords)
-            norm = float(np.sqrt(np.sum(np.abs(weights) * np.sum(ao_kept * ao_kept, axis=1) ** 2)))
-            fit["pair_space_error"] = {"l2": rbrp_result["relative_residual"] * norm,
-                                       "relative": rbrp_result["relative_residual"]}
+        fit = {"interp_vecs": theta, "theta_weight_sums": theta @ weights, "fit_diagnostics": None}
+        result["rbrp_diagnostics"]["id"] = {key: rbrp_result[key] for key in
+                                            ("retained_rank", "condition", "relative_residual", "status")}
     else:
         fit = mp_isdf.isdf_fit(context, anchor_coords, anchor_weights, coords, weights, batch_size=fit_batch_size,
-                               interp_vecs=interp_vecs, weight_sums=True)
+                               interp_vecs=want_theta, weight_sums=True)
+        if result["rbrp_diagnostics"] is not None:
+            result["rbrp_diagnostics"]["id"] = None
     result["interp_vecs"] = fit["interp_vecs"]
     result["fit_diagnostics"] = fit["fit_diagnostics"]
     record("isdf_fit")
     if pair_space_error:
-        result["pair_space_errors"] = {"scalar": fit["pair_space_error"]}
-        timings["pair_space_check"] = 0.0  # evaluated inside the fit's grid pass
+        if anchor_algorithm == "rbrp_id":
+            # the error of the returned ID, evaluated as the reference does
+            # (python/isdf.py:176-191), on the GPU
+            err = mp_isdf.pair_space_error(context, anchor_coords, anchor_weights, coords, weights,
+                                           np.ascontiguousarray(fit["interp_vecs"]), batch_size=fit_batch_size)
+            result["pair_space_errors"] = {"scalar": err}
+            record("pair_space_check")
+        else:
+            result["pair_space_errors"] = {"scalar": fit["pair_space_error"]}
+            timings["pair_space_check"] = 0.0  # evaluated inside the fit's grid pass
     # Overlap check (isdf.py:665-671): S~ = sum_k phi_k phi_k^T w_k (theta_k . w_grid)
     # against the analytic overlap (the reference's mol.intor("int1e_ovlp")).
     # The error against the grid-quadrature overlap isolates the fit's part.
     tsum = fit["theta_weight_sums"]
     overlap = np.einsum("ku,kv,k->uv", anchor_ao, anchor_ao, anchor_weights * tsum, optimize=True)
     exact = context.overlap()
-    ao = context.eval_ao(coords)
-    quadrature = ao.T @ (ao * weights[:, None])
+    quadrature = context.quadrature_overlap(coords, weights)  # on the GPU, in grid batches
     result["overlap_reconstruction"] = overlap
     result["overlap_relative_error"] = float(np.linalg.norm(overlap - exact) /
                                              max(np.linalg.norm(exact), 1e-30))
