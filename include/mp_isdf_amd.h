@@ -80,8 +80,15 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
  * (two-term phi), 4 split bf16 (hi.hi + lo.hi + hi.lo, sketch digits in four
  * planes).
  * axis_weights: NULL or naxis_weights == naxes finite positive factors.
+ * debug: bit 0 prints the reference's progress lines; bit 1 (value 2) runs
+ * the dense audit of rand_sketch and density-sketch pivots
+ * (anchor_select_randomized_id.cpp:19-88: projection residuals of the
+ * selected and of equally many dense QRCP pivots on the actual weighted pair
+ * matrix, printed to stderr; limited to 2 million packed pair-matrix
+ * elements, else status 1); other algorithms ignore it, as the reference does.
  * Writes original grid indices in pivot order to out[0..*count).
- * diagnostics: NULL or double[24] (see mp_isdf_amd/csrc/select.hpp). */
+ * diagnostics: NULL or double[28] (see mp_isdf_amd/csrc/select.hpp; slots
+ * 24-27 hold the audit residuals, -1 when no audit ran). */
 int mpisdf_select(const mpisdf_context* ctx, const double* points, const double* weights, int64_t n,
                   const int32_t* axes, int32_t naxes, const double* mo_coeff, int64_t nocc,
                   const double* axis_weights, int64_t naxis_weights, double tolerance,

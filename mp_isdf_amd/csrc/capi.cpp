@@ -243,7 +243,8 @@ int select_impl(const mpisdf_context* ctx, const double* points, const double* w
         a.max_rank = max_rank;
         a.algorithm = algorithm;
         a.batch_size = batch_size;
-        a.debug = debug != 0;
+        a.debug = (debug & 1) != 0;
+        a.audit = (debug & 2) != 0;
         a.sketch_size = sketch_size;
         a.oversampling = oversampling;
         a.seed = seed;

@@ -209,7 +209,6 @@ public:
                 throw py::value_error("point_importance must have shape (ngrid,)");
             importance.assign(input.data(), input.data() + input.size());
         }
-        if (audit) throw py::value_error("audit=True is not part of this build");
         if (batch_size < 0 || sketch_size < 0 || oversampling < 0 || max_rank < 0 || nocc < 0)
             throw py::value_error("sizes must be non-negative");
         int prec = 0;
@@ -228,7 +227,7 @@ public:
                 ctx_.get(), xyz, weights.data(), points.shape(0), codes.data(), (int32_t)codes.size(),
                 coeff.empty() ? nullptr : coeff.data(), nocc, axis_weights.empty() ? nullptr : axis_weights.data(),
                 (int64_t)axis_weights.size(), tolerance, max_rank, static_cast<int32_t>(algorithm), batch_size,
-                debug ? 1 : 0, sketch_size, oversampling, seed, prec, importance.empty() ? nullptr : importance.data(),
+                (debug ? 1 : 0) | (audit ? 2 : 0), sketch_size, oversampling, seed, prec, importance.empty() ? nullptr : importance.data(),
                 ao_screening_threshold, out.data(), (int64_t)out.size(), &count, diagnostics);
         }
         check(status);
@@ -240,12 +239,13 @@ public:
                                       double tolerance, int64_t max_rank, LocalAlgorithm algorithm,
                                       py::object mo_coeff, int64_t nocc, const std::string& space, int64_t batch_size,
                                       bool debug, int64_t sketch_size, int64_t oversampling, uint64_t seed,
-                                      const std::vector<double>& axis_weights, const std::string& precision) const
+                                      const std::vector<double>& axis_weights, const std::string& precision,
+                                      bool audit, py::object point_importance, double ao_screening_threshold) const
     {
-        std::vector<double> diag(24, 0.0);
+        std::vector<double> diag(28, 0.0);
         auto idx = select(points, weights, axes, tolerance, max_rank, algorithm, mo_coeff, nocc, space, batch_size,
-                          debug, sketch_size, oversampling, seed, false, py::none(), 100, 1e-5, axis_weights,
-                          py::none(), 1e-24, precision, diag.data());
+                          debug, sketch_size, oversampling, seed, audit, py::none(), 100, 1e-5, axis_weights,
+                          point_importance, ao_screening_threshold, precision, diag.data());
         static const char* names[] = {"rows", "columns", "numerical_rank", "rank_cap", "rank", "sketch_norm",
                                       "relative_tail", "blocks", "folds", "seconds_sketch", "seconds_qrcp",
                                       "seconds_total", "stalls", "refreshes", "fold_seconds", "fold_flops",
@@ -253,6 +253,9 @@ public:
                                       "fold_kind"};
         py::dict d;
         for (size_t i = 0; i < sizeof(names) / sizeof(names[0]); ++i) d[names[i]] = diag[i];
+        // The audit's residuals (selected, dense QRCP) per objective, when it ran.
+        if (diag[24] >= 0) d["audit"] = py::make_tuple(diag[24], diag[25]);
+        if (diag[26] >= 0) d["audit_density"] = py::make_tuple(diag[26], diag[27]);
         return py::make_tuple(idx, d);
     }
 
@@ -631,7 +634,11 @@ PYBIND11_MODULE(mp_isdf, m)
              py::arg("nocc") = 0, py::arg("space") = "auto", py::arg("batch_size") = 128, py::arg("debug") = false,
              py::arg("sketch_size") = 128, py::arg("oversampling") = 16, py::arg("seed") = 7,
              py::arg("axis_weights") = std::vector<double>{}, py::arg("precision") = "fp64",
-             "select() plus a dict of stage timings and QRCP block statistics (not in the reference).")
+             py::arg("audit") = false, py::arg("point_importance") = py::none(),
+             py::arg("ao_screening_threshold") = 1e-24,
+             "select() plus a dict of stage timings and QRCP block statistics (not in the reference); with "
+             "audit=True, 'audit' (and 'audit_density' for the density sketches) holds the projection residuals "
+             "of the selected and of the dense QRCP pivots.")
         .def("evaluate_axis", &Selector::evaluate_axis, py::arg("points"), py::arg("axes"),
              py::arg("mo_coeff") = py::none(), py::arg("nocc") = 0, py::arg("space") = "auto",
              "Evaluate concatenated axis functions on the GPU.");

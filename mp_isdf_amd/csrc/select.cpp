@@ -9,6 +9,7 @@
 // index order (original grid indices, pivot order).  All grid work runs on the
 // device; only pivots, norms and small vectors return to the host.
 #include "select.hpp"
+#include "audit.hpp"
 
 #include "kernels.hpp"
 #include "online.hpp"
@@ -173,6 +174,47 @@ std::vector<int64_t> select_dense(Context& ctx, const SelectArgs& a, const AxisS
     return out;
 }
 
+// Dense audit of the selected pivots (cpp/src/anchor_select_randomized_id.cpp:19-88):
+// the weighted pair matrix of the retained grid (column g scaled by col[g];
+// the full d^2 rows have the packed matrix's inner products), its first
+// rank(selected) dense QRCP pivots, and the relative projection residual of
+// both pivot sets, printed as the reference prints them.  Returns
+// (selected, dense).
+std::pair<double, double> audit_projection(Context& ctx, const SelectArgs& a, const AxisSpec& spec,
+                                           const std::vector<double>& scales, const double* d_xyz, int64_t n,
+                                           const std::vector<double>& col, const std::vector<int64_t>& selected,
+                                           const char* objective, hipStream_t s)
+{
+    const int64_t d = spec.width, m = d * d;
+    DeviceBuffer<double> feat((size_t)(n * d)), sw, A((size_t)(m * n));
+    AxisWorkspace ws;
+    const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t count = std::min(batch, n - start);
+        evaluate_axis_device(ctx, spec, d_xyz + 3 * start, count, scales.empty() ? nullptr : scales.data(),
+                             a.mo_coeff, a.nocc, feat.ptr + start * d, ws, s);
+    }
+    sw.upload(col.data(), n, s);
+    DeviceBuffer<unsigned long long> bad(1);
+    MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
+    launch_pair_product(feat.ptr, n, d, sw.ptr, A.ptr, bad.ptr, s);
+    if (bad.download(s)[0]) throw std::overflow_error("non-finite weighted pair product in the audit");
+    const int64_t k = (int64_t)selected.size();
+    std::vector<int64_t> dense;
+    if (k > 0) {
+        QrcpOptions opt;
+        opt.kmax = std::min({k, m, n});
+        opt.candidates = std::min<int64_t>(n, 2048);
+        dense = qrcp_device(A.ptr, kF64, m, n, m, opt, s).pivots;
+    }
+    const double actual = projection_relative_error(A.ptr, m, n, selected, s);
+    const double ref = projection_relative_error(A.ptr, m, n, dense, s);
+    std::cerr << "[randomized id audit] objective=" << objective << " rank=" << k
+              << " actual_projection_relative_error=" << actual << " dense_qrcp_projection_relative_error=" << ref
+              << " (equal rank; not ID interpolation error)\n";
+    return {actual, ref};
+}
+
 // ---------------------------------------------------------------------------
 // rand_sketch with the quadrature and density policies
 // (cpp/src/anchor_select_randomized_id.cpp:92-374)
@@ -208,6 +250,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     }
     Kept kept = filter_points(a, density ? a.importance : nullptr);
     int64_t n = (int64_t)kept.original.size();
+    if (a.audit && d * (d + 1) / 2 > 2000000 / std::max<int64_t>(1, n))
+        throw std::invalid_argument("randomized ID dense audit is limited to 2 million pair-matrix elements");
     if (n == 0) {
         if (density && a.max_rank) throw std::invalid_argument("requested anchor_rank exceeds eligible candidates");
         return {};
@@ -430,6 +474,25 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             throw std::runtime_error("randomized ID QRCP returned invalid pivot");
         out.push_back(kept.original[r.pivots[i]]);
     }
+    if (a.audit) {
+        // Positions of the selected anchors within the retained grid.
+        const std::vector<int64_t> local(r.pivots.begin(), r.pivots.begin() + rank);
+        std::vector<double> quad(n);
+        for (int64_t g = 0; g < n; ++g) quad[g] = std::sqrt(std::abs(kept.w[g]));
+        const auto q = audit_projection(ctx, a, spec, scales, d_xyz.ptr, n, quad, local, "physical_quadrature", s);
+        if (diag) {
+            diag[kDiagAuditActual] = q.first;
+            diag[kDiagAuditDense] = q.second;
+        }
+        if (density) {
+            const auto e = audit_projection(ctx, a, spec, scales, d_xyz.ptr, n, hsw, local,
+                                            normalized ? "density_normalized" : "density_raw", s);
+            if (diag) {
+                diag[kDiagAuditDensityActual] = e.first;
+                diag[kDiagAuditDensityDense] = e.second;
+            }
+        }
+    }
     if (a.debug)
         std::cerr << "[rand_sketch:quadrature] rank=" << rank << " sketch_size=" << ell
                   << " oversampling=" << a.oversampling << " columns=" << n
@@ -564,7 +627,10 @@ void make_omega(int64_t d, int64_t ell, uint64_t seed, std::vector<double>& u, s
 std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& a, double* diag)
 {
     const auto started = Clock::now();
-    if (diag) std::fill(diag, diag + kDiagCount, 0.0);
+    if (diag) {
+        std::fill(diag, diag + kDiagCount, 0.0);
+        std::fill(diag + kDiagAuditActual, diag + kDiagAuditDensityDense + 1, -1.0);
+    }
     if (a.n < 0) throw std::invalid_argument("points and weights must have equal lengths");
     if (!a.axis_weights.empty() && a.axis_weights.size() != a.axes.size())
         throw std::invalid_argument("axis weights must match axes");

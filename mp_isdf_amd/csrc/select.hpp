@@ -32,6 +32,11 @@ struct SelectArgs {
     // nonnegative importance, and the AO squared-norm screening threshold.
     const double* importance = nullptr;
     double ao_screening_threshold = 1e-24;
+    // rand_sketch and the density sketches: after selection, compare the
+    // pivots' projection residual of the actual weighted pair matrix with
+    // that of equally many dense QRCP pivots
+    // (cpp/src/anchor_select_randomized_id.cpp:19-88, :164-165, :389-400).
+    bool audit = false;
 };
 
 // Diagnostics slots (double[16]) written when requested.
@@ -57,7 +62,14 @@ enum Diag : int {
     kDiagSketchFlops = 18,    // their MFMA flops: sum 4 ell d count per product term (3 split BF16, 2 FP8)
     kDiagSketchLaunches = 19,
     kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 / 3 INT8 MFMA on 3 / 4 digit planes
-    kDiagCount = 24
+    // Audit (SelectArgs::audit): relative projection residuals of the selected
+    // and of the dense QRCP pivots, quadrature columns sqrt|w| pair_g, then,
+    // for the density sketches, the sketch's own columns; -1 when not run.
+    kDiagAuditActual = 24,
+    kDiagAuditDense = 25,
+    kDiagAuditDensityActual = 26,
+    kDiagAuditDensityDense = 27,
+    kDiagCount = 28
 };
 
 std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& args, double* diagnostics);

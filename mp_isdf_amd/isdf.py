@@ -56,7 +56,8 @@ AXES = {
 def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0, anchor_tol=None,
                axes=("ao",), axis_weights=None, anchor_algorithm: str = "qrcp_dense", precision: str = "fp64",
                anchor_batch_size: int = 128, fit_batch_size: int = 4096, anchor_sketch_size: int = 128,
-               anchor_oversampling: int = 16, anchor_seed: int = 7, mo_coeff=None, nocc: int = 0,
+               anchor_oversampling: int = 16, anchor_seed: int = 7, anchor_audit: bool = False,
+               mo_coeff=None, nocc: int = 0,
                selection_only: bool = False, pair_space_error: bool = False, interp_vecs=None,
                compute_metric: bool = False, metric_algorithm: str = "grid", sketch: str = "quadrature",
                point_importance=None, density_weight: str = "electron_density",
@@ -65,6 +66,10 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                rbrp_rank_rtol: float = 1e-12, rbrp_refresh_every: int = 8, rbrp_svd_rtol: float = 1e-12,
                space: str = "auto", report_timings: bool = False, debug: bool = False):
     """Select anchors on the molecular grid and fit the AO-pair ISDF on the GPU.
+
+    ``anchor_audit=True`` (rand_sketch only; python/isdf.py:358-359, :461-462)
+    prints the projection residuals of the selected and of equally many dense
+    QRCP pivots on the actual pair matrix; small problems only.
 
     ``interp_vecs``: True returns theta[k, n] (the reference always does);
     None (default) returns it when it takes at most 8 GiB -- at (H2O)64 with
@@ -118,6 +123,8 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     density_sketch = anchor_algorithm == "rand_sketch" and sketch in DENSITY_SKETCHES
     if cvt_weight not in ("electron_density", "quadrature_weights"):
         raise ValueError("cvt_weight must be 'electron_density' or 'quadrature_weights'")
+    if anchor_audit and anchor_algorithm != "rand_sketch":
+        raise ValueError("anchor_audit requires a rand sketch algorithm")
     if anchor_algorithm == "cvt" and anchor_rank < 1:
         raise ValueError("CVT requires a positive anchor_rank")
     if anchor_rank and anchor_tol is not None:
@@ -177,7 +184,7 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         anchor_indices = np.asarray(selector.select(
             all_coords, all_weights, axis_types, tolerance=tolerance, max_rank=anchor_rank, algorithm=algorithm,
             mo_coeff=mo_coeff, nocc=nocc, batch_size=anchor_batch_size, debug=debug,
-            sketch_size=anchor_sketch_size, oversampling=anchor_oversampling, seed=anchor_seed,
+            sketch_size=anchor_sketch_size, oversampling=anchor_oversampling, seed=anchor_seed, audit=anchor_audit,
             axis_weights=list(axis_weights) if axis_weights is not None else [], precision=precision,
             point_importance=importance if density_sketch else None, ao_screening_threshold=ao_screening_threshold,
             cvt_weights=importance if anchor_algorithm == "cvt" else None, cvt_max_iter=cvt_max_iter,

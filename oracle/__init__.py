@@ -12,6 +12,9 @@ driver maths:
 * ``physical_pair_refit``  python/isdf.py:238-263 (SVD solve, 1e-12 cutoff)
 * ``pair_space_error_norms``  python/isdf.py:154-173
 * ``stream_pair_space_error_norms``  python/isdf.py:176-191
+* ``audit_projection``  cpp/src/anchor_select_randomized_id.cpp:19-88 (pinned
+  against LAPACK's dgeqp3 pivots through SciPy and an orthogonal-projector
+  residual from NumPy's QR, tests/test_audit_oracle.py)
 
 Reference build: not possible here (CMake FetchContent of Kokkos, LAPACK++,
 libcint, pybind11 plus an ILP64 LAPACK), so there is no ``oracle/_ref``.
@@ -142,6 +145,51 @@ def geqp3(a: np.ndarray, kmax: int | None = None):
         _check(lib().oracle_geqp3_steps(ctypes.c_int64(m), ctypes.c_int64(n), _p(f), ctypes.c_int64(m),
                                         _p(jpvt), _p(tau), ctypes.c_int64(kmax)), selection=True)
     return f, jpvt - 1, tau
+
+
+def packed_pairs(features, col):
+    """The audit's packed weighted pair matrix (m = d(d+1)/2 rows, one column per
+    point): col_g f_mu f_nu for mu <= nu, off-diagonal rows times sqrt(2)
+    (cpp/src/anchor_select_randomized_id.cpp:35-46)."""
+    f = np.asarray(features, dtype=np.float64)
+    mu, nu = np.triu_indices(f.shape[1])
+    fac = np.where(mu == nu, 1.0, np.sqrt(2.0))
+    return (f[:, mu] * f[:, nu] * fac * np.asarray(col, dtype=np.float64)[:, None]).T.copy()
+
+
+def projection_error(a, pivots):
+    """Relative residual of every column of ``a`` after projection on the span
+    of the pivot columns (cpp/src/anchor_select_randomized_id.cpp:49-81): a
+    basis by two passes of modified Gram-Schmidt that skips a column left with
+    at most 32 eps of its norm, then two projection passes per column."""
+    a = np.asarray(a, dtype=np.float64)
+    m, n = a.shape
+    q = []
+    for g in pivots:
+        v = a[:, g].copy()
+        original = np.linalg.norm(v)
+        for _ in range(2):
+            for qj in q:
+                v -= (qj @ v) * qj
+        norm = np.linalg.norm(v)
+        if norm <= 32 * np.finfo(float).eps * original:
+            continue
+        q.append(v / norm)
+    total = float(np.sum(a * a))
+    r = a.copy()
+    for _ in range(2):
+        for qj in q:
+            r -= np.outer(qj, qj @ r)
+    return np.sqrt(float(np.sum(r * r)) / total) if total else 0.0
+
+
+def audit_projection(features, col, selected):
+    """Dense audit of sketch pivots (cpp/src/anchor_select_randomized_id.cpp:19-88):
+    (residual of the selected pivots, residual of the first len(selected)
+    dense QRCP pivots) on the packed weighted pair matrix."""
+    a = packed_pairs(features, col)
+    _, piv, _ = geqp3(a)
+    return projection_error(a, list(selected)), projection_error(a, list(piv[:len(selected)]))
 
 
 def omega(d: int, ell: int, seed: int = 7):
