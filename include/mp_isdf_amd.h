@@ -300,6 +300,19 @@ int mpisdf_isdf_fit_gradient(const mpisdf_context* ctx, const double* anchor_xyz
                              const double* xyz, const double* w, int64_t n, int64_t batch, double* loss,
                              double* grad_ao, double* grad_atoms);
 
+/* Coulomb energy of the fitted density on fixed anchor and grid points (host
+ * arrays) and its gradient with respect to the nuclear coordinates of the
+ * context: E_J = tr(J dm) / 2 for the ISDF J of evaluate_isdf_jk
+ * (python/isdf.py:130-151) with the grid metric of the FP64 refit
+ * (isdf.py:48-67, :238-263, :674-689), evaluated as u^T W K W u / 2 with
+ * u = C^T rho the fitted density on the grid (no theta, no metric formed).
+ * dm[nao*nao] symmetric; w nonzero.  *energy, grad_ao[3*nao] and
+ * grad_atoms[natm*3] (each optional; a gradient needs independent anchor
+ * pairs).  No counterpart in the reference, which has no nuclear gradients. */
+int mpisdf_coulomb_energy_gradient(const mpisdf_context* ctx, const double* anchor_xyz, int64_t k, const double* xyz,
+                                   const double* w, int64_t n, const double* dm, int64_t batch, double* energy,
+                                   double* grad_ao, double* grad_atoms);
+
 /* ---- Coulomb metric and J/K ---------------------------------------------- */
 /* Grid Coulomb metric, python/isdf.py:48-67 (grid_coulomb_metric) and :76-92
  * (grid_cross_metric):  metric[a*kr + b] = sum_ij left[a][i] w_i K_ij w_j

@@ -582,6 +582,21 @@ extern "C" int mpisdf_ao_adjoint_device(const mpisdf_context* ctx, const double*
     });
 }
 
+namespace {
+// grad_atoms[3 A + c] = -sum_{mu on A} gao[c nao + mu]: moving a nucleus moves its AOs.
+void atoms_from_ao_gradient(const mpisdf::Context& c, const std::vector<double>& gao, double* grad_atoms)
+{
+    const int64_t natm = (int64_t)c.atm.size() / 6;
+    std::fill(grad_atoms, grad_atoms + 3 * natm, 0.0);
+    for (size_t sidx = 0; sidx < c.shells.size(); ++sidx) {
+        const int atom = c.bas[sidx * 8 + 0];
+        const auto& sh = c.shells[sidx];
+        for (int mu = sh.ao_off; mu < sh.ao_off + sh.nctr * sh.ncomp; ++mu)
+            for (int comp = 0; comp < 3; ++comp) grad_atoms[3 * atom + comp] -= gao[comp * c.nao + mu];
+    }
+}
+}  // namespace
+
 extern "C" int mpisdf_isdf_fit_gradient(const mpisdf_context* ctx, const double* anchor_xyz, const double* anchor_w,
                                         int64_t k, const double* xyz, const double* w, int64_t n, int64_t batch,
                                         double* loss, double* grad_ao, double* grad_atoms)
@@ -603,17 +618,44 @@ extern "C" int mpisdf_isdf_fit_gradient(const mpisdf_context* ctx, const double*
         mpisdf::fit_loss_gradient(c, f, dax.ptr, dxyz.ptr, dw.ptr, n, batch, &l, gao.data(), nullptr);
         if (loss) *loss = l;
         if (grad_ao) std::copy(gao.begin(), gao.end(), grad_ao);
-        if (grad_atoms) {
-            // dL/dR_A = -sum_{mu on A} gao[:, mu]: moving a nucleus moves its AOs.
-            const int64_t natm = (int64_t)c.atm.size() / 6;
-            std::fill(grad_atoms, grad_atoms + 3 * natm, 0.0);
-            for (size_t sidx = 0; sidx < c.shells.size(); ++sidx) {
-                const int atom = c.bas[sidx * 8 + 0];
-                const auto& sh = c.shells[sidx];
-                for (int mu = sh.ao_off; mu < sh.ao_off + sh.nctr * sh.ncomp; ++mu)
-                    for (int comp = 0; comp < 3; ++comp) grad_atoms[3 * atom + comp] -= gao[comp * c.nao + mu];
+        if (grad_atoms) atoms_from_ao_gradient(c, gao, grad_atoms);
+    });
+}
+
+extern "C" int mpisdf_coulomb_energy_gradient(const mpisdf_context* ctx, const double* anchor_xyz, int64_t k,
+                                              const double* xyz, const double* w, int64_t n, const double* dm,
+                                              int64_t batch, double* energy, double* grad_ao, double* grad_atoms)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (n < 0 || k <= 0 || batch <= 0 || !anchor_xyz || !dm || (n > 0 && (!xyz || !w)))
+            throw std::invalid_argument("invalid Coulomb-energy arguments");
+        for (int64_t g = 0; g < n; ++g)
+            if (!std::isfinite(w[g]) || w[g] == 0)
+                throw std::invalid_argument("grid weights must be finite and nonzero (the metric's self term)");
+        const int64_t nao = c.nao;
+        for (int64_t i = 0; i < nao; ++i)
+            for (int64_t j = 0; j < nao; ++j) {
+                const double a = dm[i * nao + j], b = dm[j * nao + i];
+                if (!std::isfinite(a)) throw std::invalid_argument("dm must be a finite AO density matrix");
+                if (std::abs(a - b) > 1e-12 + 1e-12 * std::abs(b)) throw std::invalid_argument("dm must be symmetric");
             }
-        }
+        mpisdf::DeviceBuffer<double> dxyz, dw, dax, ddm;
+        dxyz.upload(xyz, 3 * n, nullptr);
+        dw.upload(w, n, nullptr);
+        dax.upload(anchor_xyz, 3 * k, nullptr);
+        ddm.upload(dm, nao * nao, nullptr);
+        mpisdf::FitFactor f;
+        mpisdf::fit_factor(c, dax.ptr, k, f, nullptr);
+        const bool want = grad_ao || grad_atoms;
+        std::vector<double> gao(want ? 3 * nao : 0, 0.0);
+        double e = 0;
+        mpisdf::coulomb_energy_gradient(c, f, dax.ptr, dxyz.ptr, dw.ptr, n, ddm.ptr, batch, &e,
+                                        want ? gao.data() : nullptr, nullptr);
+        MPISDF_HIP(hipStreamSynchronize(nullptr));
+        if (energy) *energy = e;
+        if (grad_ao) std::copy(gao.begin(), gao.end(), grad_ao);
+        if (grad_atoms) atoms_from_ao_gradient(c, gao, grad_atoms);
     });
 }
 

@@ -57,6 +57,72 @@ __global__ __launch_bounds__(256) void coulomb_rows_kernel(const double* __restr
     }
 }
 
+// 1/sqrt(x) for finite x > 0: the hardware estimate v_rsq_f64 refined by two
+// Newton-Raphson steps y <- y + y (1 - x y^2) / 2, each of which squares the
+// relative error, to within a few ulp of the rounded quotient.
+__device__ inline double inv_sqrt(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// v[i] = w_i sum_j K_ij w_j u_j, the kernel of grid_coulomb_metric applied to
+// one weighted vector.  Each thread owns two target points; source points
+// stream through LDS 256 at a time, and every target sums its sources in
+// index order, so the result is the same on every run.
+constexpr int kPotTargets = 2;
+__global__ __launch_bounds__(256) void coulomb_potential_kernel(const double* __restrict__ xyz,
+                                                                const double* __restrict__ w,
+                                                                const double* __restrict__ u, int64_t n,
+                                                                double csphere, double* __restrict__ v)
+{
+    __shared__ double sx[256], sy[256], sz[256], sq[256];
+    double xi[kPotTargets], yi[kPotTargets], zi[kPotTargets], acc[kPotTargets];
+    int64_t idx[kPotTargets];
+#pragma unroll
+    for (int t = 0; t < kPotTargets; ++t) {
+        idx[t] = ((int64_t)blockIdx.x * kPotTargets + t) * 256 + threadIdx.x;
+        const bool ok = idx[t] < n;
+        xi[t] = ok ? xyz[3 * idx[t]] : 0.0;
+        yi[t] = ok ? xyz[3 * idx[t] + 1] : 0.0;
+        zi[t] = ok ? xyz[3 * idx[t] + 2] : 0.0;
+        acc[t] = 0.0;
+    }
+    for (int64_t j0 = 0; j0 < n; j0 += 256) {
+        const int64_t j = j0 + threadIdx.x;
+        __syncthreads();
+        if (j < n) {
+            sx[threadIdx.x] = xyz[3 * j];
+            sy[threadIdx.x] = xyz[3 * j + 1];
+            sz[threadIdx.x] = xyz[3 * j + 2];
+            sq[threadIdx.x] = w[j] * u[j];
+        }
+        __syncthreads();
+        const int cnt = (int)(n - j0 < 256 ? n - j0 : 256);
+        for (int c = 0; c < cnt; ++c) {
+            const double px = sx[c], py = sy[c], pz = sz[c], q = sq[c];
+#pragma unroll
+            for (int t = 0; t < kPotTargets; ++t) {
+                const double dx = xi[t] - px, dy = yi[t] - py, dz = zi[t] - pz;
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                // the point itself and coincident distinct points add nothing here
+                acc[t] += d2 > 0.0 ? q * inv_sqrt(d2) : 0.0;
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kPotTargets; ++t) {
+        const int64_t i = idx[t];
+        if (i < n) {
+            const double wi = w[i];
+            v[i] = wi * (acc[t] + csphere * pow(fabs(wi), -1.0 / 3.0) * wi * u[i]);
+        }
+    }
+}
+
 void launch_rows(const double* xyz, const double* w, int64_t i0, int64_t rows, int64_t j0, int64_t cols, int64_t ldk,
                  double csphere, double* kb, hipStream_t s)
 {
@@ -125,6 +191,16 @@ int64_t coulomb_rows_per_block(int64_t n, int64_t kr)
     const int64_t budget = std::min<int64_t>(int64_t{32} << 30, (int64_t)(free_bytes / 3));
     rows = std::min(rows, std::max<int64_t>(128, budget / (8 * ldk) / 128 * 128));
     return std::max<int64_t>(128, std::min(rows, (n + 127) / 128 * 128));
+}
+
+void grid_coulomb_potential(const double* xyz, const double* w, const double* u, int64_t n, double* v, hipStream_t s)
+{
+    if (n <= 0) return;
+    const double csphere = (32.0 / 15.0) * kPi * kPi * std::pow(3.0 / (4.0 * kPi), 5.0 / 3.0);
+    const int64_t per_block = 256 * kPotTargets;
+    hipLaunchKernelGGL(coulomb_potential_kernel, dim3((unsigned)((n + per_block - 1) / per_block)), dim3(256), 0, s,
+                       xyz, w, u, n, csphere, v);
+    MPISDF_HIP(hipGetLastError());
 }
 
 void grid_coulomb_metric(const double* left, int64_t kl, int64_t ldl, const double* right, int64_t kr, int64_t ldr,

@@ -24,6 +24,11 @@ void grid_coulomb_metric(const double* left, int64_t kl, int64_t ldl, const doub
                          const double* xyz, const double* w, int64_t n, bool symmetrize, double* metric,
                          int64_t block_rows, hipStream_t s);
 
+// v[i] = w_i sum_j K_ij w_j u_j over the n grid points (device xyz[n][3],
+// w[n], u[n], v[n]), K as above: the metric's kernel applied to one weighted
+// vector by a direct sum, O(n^2) without materialising K.
+void grid_coulomb_potential(const double* xyz, const double* w, const double* u, int64_t n, double* v, hipStream_t s);
+
 // ISDF Coulomb (J) and exchange (K) matrices of a symmetric AO density dm
 // (nao x nao) from anchor AO values x (k x nao row-major) and the anchor
 // metric z (k x k):  P = x dm x^T,  J = x^T diag(z diag(P)) x,

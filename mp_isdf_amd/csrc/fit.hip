@@ -13,6 +13,7 @@
 // Gram, RHS and triangular products are FP64 MFMA GEMMs; the diagonal
 // blocks of the Cholesky factor are factorised and inverted in LDS.
 #include "fit.hpp"
+#include "coulomb.hpp"
 
 #include "kernels.hpp"
 #include "util.hpp"
@@ -713,6 +714,153 @@ void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xy
     const auto h = acc.download(s);
     if (loss_out) *loss_out = h[0];
     if (gao_out) MPISDF_HIP(hipMemcpyAsync(gao_out, gao.ptr, 3 * nao * sizeof(double), hipMemcpyDeviceToHost, s));
+    MPISDF_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------------------
+// Coulomb energy of the fitted density and its nuclear adjoint
+// ---------------------------------------------------------------------------
+namespace {
+
+// out[i] = alpha a[i] (rowf ? rowf[i % rows] : 1) (colf ? colf[i / rows] : 1)
+// for a column-major rows x (total / rows) matrix a.
+__global__ void scale_rc_kernel(const double* __restrict__ a, int64_t rows, int64_t total,
+                                const double* __restrict__ rowf, const double* __restrict__ colf, double alpha,
+                                double* __restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = alpha * a[i] * (rowf ? rowf[i % rows] : 1.0) * (colf ? colf[i / rows] : 1.0);
+}
+
+// out[r] = sum_c a[r * d + c] b[r * d + c], one wavefront per row.
+__global__ void row_dot_kernel(const double* __restrict__ a, const double* __restrict__ b, int64_t rows, int64_t d,
+                               double* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (r >= rows) return;
+    double s = 0;
+    for (int64_t c = lane; c < d; c += 64) s += a[r * d + c] * b[r * d + c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) out[r] = s;
+}
+
+// h[i k + j] = g[i k + j] (x_i y_j + y_i x_j)
+__global__ void sym_outer_hadamard_kernel(const double* __restrict__ g, const double* __restrict__ x,
+                                          const double* __restrict__ y, int64_t k, double* __restrict__ h)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k * k; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / k, j = e % k;
+        h[e] = g[e] * (x[i] * y[j] + y[i] * x[j]);
+    }
+}
+
+// abar[p][mu] = 2 (lam_p m1[p][mu] + beta_p y[p][mu] - m2[p][mu])
+__global__ void anchor_energy_adjoint_kernel(const double* __restrict__ m1, const double* __restrict__ lam,
+                                             const double* __restrict__ y, const double* __restrict__ beta,
+                                             const double* __restrict__ m2, int64_t k, int64_t d,
+                                             double* __restrict__ out)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < k * d; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = e / d;
+        out[e] = 2.0 * (lam[p] * m1[e] + beta[p] * y[e] - m2[e]);
+    }
+}
+
+// x <- G2^-1 x through the explicit factor inverse: L^-T (L^-1 x).
+void solve_g2(const FitFactor& f, double* x, double* tmp, hipStream_t s)
+{
+    const int64_t k = f.k;
+    gemm_tn_f64(k, 1, k, 1.0, f.linv.ptr, k, x, k, 0.0, tmp, k, s, f.triangular ? 1 : 0);
+    gemm_tn_f64(k, 1, k, 1.0, f.linvt.ptr, k, tmp, k, 0.0, x, k, s, f.triangular ? 2 : 0);
+}
+
+}  // namespace
+
+void coulomb_energy_gradient(Context& ctx, const FitFactor& f, const double* anchor_xyz, const double* xyz,
+                             const double* w, int64_t n, const double* dm, int64_t batch, double* energy_out,
+                             double* gao_out, hipStream_t s)
+{
+    const int64_t k = f.k, d = f.d, nao = ctx.nao;
+    if (gao_out && !f.triangular)
+        throw std::runtime_error("the Coulomb-energy gradient needs linearly independent anchor pairs; the fit fell "
+                                 "back to the pseudo-inverse of the pair Gram");
+    batch = std::max<int64_t>(1, std::min(batch, std::max<int64_t>(n, 1)));
+    auto blocks = [](int64_t work) { return dim3((unsigned)grid_for(work)); };
+    DeviceBuffer<double> y((size_t)(k * d)), rho(k), lam(k), tmp(k), u(std::max<int64_t>(n, 1)),
+        v(std::max<int64_t>(n, 1)), uv(std::max<int64_t>(n, 1)), acc(1), phi((size_t)(batch * d)),
+        tm((size_t)(k * batch)), sm((size_t)(k * batch));
+    // rho_p = phi_p^T D phi_p (the density at the anchors), lam = G2^-1 rho.
+    gemm_f64(k, d, d, 1.0, MatView{f.phi_a.ptr, d, 1}, MatView{dm, d, 1}, 0.0, MutView{y.ptr, d, 1}, s);
+    hipLaunchKernelGGL(row_dot_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, s, y.ptr, f.phi_a.ptr, k, d,
+                       rho.ptr);
+    MPISDF_HIP(hipMemcpyAsync(lam.ptr, rho.ptr, k * sizeof(double), hipMemcpyDeviceToDevice, s));
+    solve_g2(f, lam.ptr, tmp.ptr, s);
+    // u_g = s_g^T lam = (C^T rho)_g, the fitted density at grid point g.
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t nb = std::min(batch, n - start);
+        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
+        gemm_tn_f64_fused(TnFused::Square, k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, sm.ptr, k, nullptr, nullptr,
+                          nullptr, s);
+        gemm_tn_f64(nb, 1, k, 1.0, sm.ptr, k, lam.ptr, k, 0.0, u.ptr + start, nb, s);
+    }
+    // v = W K W u; E_J = u^T v / 2.
+    grid_coulomb_potential(xyz, w, u.ptr, n, v.ptr, s);
+    hipLaunchKernelGGL(hadamard_weight_kernel, blocks(n), dim3(256), 0, s, u.ptr, v.ptr, (const double*)nullptr,
+                       n, n, uv.ptr);
+    MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, sizeof(double), s));
+    if (n > 0) launch_sum_fixed(uv.ptr, n, n, 1, acc.ptr, s);
+    MPISDF_HIP(hipGetLastError());
+    const double e = 0.5 * acc.download(s)[0];
+    if (energy_out) *energy_out = e;
+    if (!gao_out) return;
+
+    // Adjoint.  dE = v^T du with u_g = s_g^T lam, lam = G2^-1 rho:
+    //   dE = sum_g v_g lam^T ds_g + beta^T (d rho - dG2 lam),  beta = G2^-1 S v.
+    DeviceBuffer<double> lt((size_t)(k * batch)), vt((size_t)(batch * d)), abar((size_t)(std::max(batch, k) * d)),
+        m1((size_t)(k * d)), m2((size_t)(k * d)), sigma(k), gm((size_t)(k * k)), gao((size_t)(3 * nao)), spart,
+        adj_part, phiat((size_t)(d * k));
+    MPISDF_HIP(hipMemsetAsync(m1.ptr, 0, (size_t)(k * d) * sizeof(double), s));
+    MPISDF_HIP(hipMemsetAsync(sigma.ptr, 0, k * sizeof(double), s));
+    MPISDF_HIP(hipMemsetAsync(gao.ptr, 0, 3 * nao * sizeof(double), s));
+    spart.resize((size_t)(k * ((batch + 127) / 128)));
+    transpose(f.phi_a.ptr, k, d, nullptr, phiat.ptr, s);  // Phi_a^T, row-major d x k
+    for (int64_t start = 0; start < n; start += batch) {
+        const int64_t nb = std::min(batch, n - start);
+        const double* vb = v.ptr + start;
+        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
+        gemm_tn_f64(k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, 0.0, tm.ptr, k, s);  // T = Phi_a Phi_b^T
+        hipLaunchKernelGGL(hadamard_weight_kernel, blocks(k * nb), dim3(256), 0, s, tm.ptr, tm.ptr,
+                           (const double*)nullptr, k, k * nb, sm.ptr);
+        // sigma += S v_b, per 128-point block then in block order.
+        const dim3 grid((unsigned)((k + 255) / 256), (unsigned)((nb + 127) / 128));
+        hipLaunchKernelGGL(zw_rows_kernel, grid, dim3(256), 0, s, sm.ptr, k, nb, vb, spart.ptr);
+        launch_sum_partials(spart.ptr, grid.y, k, sigma.ptr, true, s);
+        // Grid AOs: abar_g = 2 v_g Phi_a^T (lam o t_g).
+        hipLaunchKernelGGL(scale_rc_kernel, blocks(k * nb), dim3(256), 0, s, tm.ptr, k, k * nb, lam.ptr,
+                           (const double*)nullptr, 1.0, lt.ptr);
+        gemm_tn_f64(d, nb, k, 1.0, phiat.ptr, k, lt.ptr, k, 0.0, vt.ptr, d, s);
+        hipLaunchKernelGGL(scale_rc_kernel, blocks(nb * d), dim3(256), 0, s, vt.ptr, d, nb * d,
+                           (const double*)nullptr, vb, 2.0, abar.ptr);
+        launch_ao_adjoint(ctx, xyz + 3 * start, nb, abar.ptr, d, gao.ptr, adj_part, s);
+        // Anchor AOs (s_g part): M1 += (T o v) Phi_b, i.e. sum_g v_g t_gk phi_g.
+        hipLaunchKernelGGL(scale_rc_kernel, blocks(k * nb), dim3(256), 0, s, tm.ptr, k, k * nb,
+                           (const double*)nullptr, vb, 1.0, lt.ptr);
+        gemm_f64(k, d, nb, 1.0, MatView{lt.ptr, 1, k}, MatView{phi.ptr, d, 1}, 1.0, MutView{m1.ptr, d, 1}, s);
+        MPISDF_HIP(hipGetLastError());
+    }
+    // beta = G2^-1 sigma; anchor adjoint 2 (lam_p M1_p + beta_p (D phi_p) - [(G o (beta lam^T + lam beta^T)) Phi_a]_p).
+    solve_g2(f, sigma.ptr, tmp.ptr, s);
+    gemm_tn_f64(k, k, d, 1.0, f.phi_a.ptr, d, f.phi_a.ptr, d, 0.0, gm.ptr, k, s);  // G, symmetric
+    hipLaunchKernelGGL(sym_outer_hadamard_kernel, blocks(k * k), dim3(256), 0, s, gm.ptr, sigma.ptr, lam.ptr, k,
+                       gm.ptr);
+    gemm_f64(k, d, k, 1.0, MatView{gm.ptr, k, 1}, MatView{f.phi_a.ptr, d, 1}, 0.0, MutView{m2.ptr, d, 1}, s);
+    hipLaunchKernelGGL(anchor_energy_adjoint_kernel, blocks(k * d), dim3(256), 0, s, m1.ptr, lam.ptr, y.ptr,
+                       sigma.ptr, m2.ptr, k, d, abar.ptr);
+    MPISDF_HIP(hipGetLastError());
+    launch_ao_adjoint(ctx, anchor_xyz, k, abar.ptr, d, gao.ptr, adj_part, s);
+    MPISDF_HIP(hipMemcpyAsync(gao_out, gao.ptr, 3 * nao * sizeof(double), hipMemcpyDeviceToHost, s));
     MPISDF_HIP(hipStreamSynchronize(s));
 }
 

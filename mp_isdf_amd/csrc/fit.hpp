@@ -62,4 +62,18 @@ void quadrature_overlap(Context& ctx, const double* xyz, const double* w, int64_
 void fit_loss_gradient(Context& ctx, const FitFactor& f, const double* anchor_xyz, const double* xyz,
                        const double* w, int64_t n, int64_t batch, double* loss_out, double* gao_out, hipStream_t s);
 
+// Coulomb energy of the fitted density, the ISDF J energy
+// E_J = tr(J D) / 2 = rho^T Z rho / 2 of python/isdf.py:130-151 with the grid
+// metric Z = W_a grid_coulomb_metric(theta) W_a of isdf.py:48-67, :674-689,
+// evaluated without theta or Z: with rho_p = phi_p^T D phi_p at the anchors
+// and u_g = s_g^T G2^-1 rho (the fitted density at grid point g),
+// E_J = u^T W K W u / 2, one O(n^2) potential sum.  dm is a device
+// nao x nao symmetric matrix; anchor_xyz, xyz, w are device pointers (w
+// nonzero).  gao_out (host [3][nao], optional) receives the adjoint contracted
+// with AO gradients as for fit_loss_gradient; it needs the Cholesky factor
+// (independent anchor pairs): the gradient of a pseudo-inverse fit raises.
+void coulomb_energy_gradient(Context& ctx, const FitFactor& f, const double* anchor_xyz, const double* xyz,
+                             const double* w, int64_t n, const double* dm, int64_t batch, double* energy_out,
+                             double* gao_out, hipStream_t s);
+
 }  // namespace mpisdf

@@ -461,6 +461,42 @@ PYBIND11_MODULE(mp_isdf, m)
        "Pair-reconstruction loss of the FP64 fit (sum_g |w_g| ||pair_g - fit_g||^2) and its gradient with "
        "respect to the nuclear coordinates (grid and anchor points fixed), by the hand-written adjoint.");
 
+    m.def("coulomb_energy_gradient", [](const Context& ctx, const FloatArray& anchor_coords, const FloatArray& coords,
+                                        const FloatArray& weights, const FloatArray& dm, int64_t batch_size,
+                                        bool gradient) {
+        if (anchor_coords.ndim() != 2 || anchor_coords.shape(1) != 3)
+            throw py::value_error("expected anchor_coords[k,3]");
+        if (coords.ndim() != 2 || coords.shape(1) != 3 || weights.ndim() != 1 || weights.shape(0) != coords.shape(0))
+            throw py::value_error("expected coords[n,3] and weights[n]");
+        int64_t nao = 0;
+        check(mpisdf_context_nao(ctx.get(), &nao));
+        if (dm.ndim() != 2 || dm.shape(0) != nao || dm.shape(1) != nao)
+            throw py::value_error("dm must be a finite AO density matrix of shape (nao,nao)");
+        const py::ssize_t k = anchor_coords.shape(0), n = coords.shape(0);
+        const py::ssize_t natm = (py::ssize_t)ctx.natm();
+        py::array_t<double> gao({(py::ssize_t)3, (py::ssize_t)nao}), gat({natm, (py::ssize_t)3});
+        double energy = 0;
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_coulomb_energy_gradient(ctx.get(), anchor_coords.data(), k, coords.data(), weights.data(),
+                                                    n, dm.data(), batch_size, &energy,
+                                                    gradient ? gao.mutable_data() : nullptr,
+                                                    gradient ? gat.mutable_data() : nullptr);
+        }
+        check(status);
+        py::dict out;
+        out["energy"] = energy;
+        if (gradient) {
+            out["grad_ao"] = gao;
+            out["grad_atoms"] = gat;
+        }
+        return out;
+    }, py::arg("context"), py::arg("anchor_coords"), py::arg("coords"), py::arg("weights"), py::arg("dm"),
+       py::arg("batch_size") = 4096, py::arg("gradient") = true,
+       "Coulomb energy E_J = tr(J dm)/2 of the ISDF fit on fixed anchors with the grid Coulomb metric, and "
+       "(gradient=True) its gradient with respect to the nuclear coordinates, by the hand-written adjoint.");
+
     // Grid Coulomb metric (python/isdf.py:48-67, :76-92) on the GPU.
     auto metric = [](const FloatArray& left, const FloatArray& right, const FloatArray& coords,
                      const FloatArray& weights, bool same, bool symmetrize) {
