@@ -346,10 +346,16 @@ int mpisdf_isdf_jk_device(const double* x, int64_t k, int64_t nao, const double*
 
 /* ---- Grid front end (no reference counterpart: the reference takes its grid
  * from PySCF's dft.gen_grid.Grids, python/isdf.py:488-496) ------------------ */
-/* Becke-partitioned atom-centred grid, Treutler radial nodes, product angular
- * rules.  Call with xyz == NULL to get *count; then xyz[3*count], w[count].
- * device != 0 runs the partition on the GPU when one is present; the weights
- * equal the host loop's bit for bit. */
+/* Becke-partitioned atom-centred grid with PySCF's defaults: Treutler radial
+ * nodes, Lebedev angular rules of PySCF's order per level and period,
+ * NWChem pruning.  Levels 0..3 (and higher levels whose rules are
+ * tabulated; status 1 otherwise).  Call with xyz == NULL to get *count; then
+ * xyz[3*count], w[count].  device != 0 runs the partition on the GPU when one
+ * is present; the weights equal the host loop's bit for bit. */
+/* The npoints-point Lebedev-Laikov rule (PySCF's dft.gen_grid angular
+ * grids): xyz[3*npoints] unit vectors, weights[npoints] summing to 1.
+ * Status 1 when no rule of that size is tabulated. */
+int mpisdf_lebedev(int32_t npoints, double* xyz, double* weights);
 int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level, int32_t device,
                       double* xyz, double* weights, int64_t capacity, int64_t* count);
 

@@ -209,6 +209,7 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
 
 #include "fit.hpp"
 #include "grid.hpp"
+#include "lebedev.hpp"
 #include "sketch.hpp"
 #include "sketch_mfma.hpp"
 #include "select.hpp"
@@ -300,6 +301,19 @@ extern "C" int mpisdf_becke_grid(const int32_t* charges, const double* coords, i
         auto grid = mpisdf::becke_grid(z.data(), coords, natm, level, device != 0);
         std::copy(grid.first.begin(), grid.first.end(), xyz);
         std::copy(grid.second.begin(), grid.second.end(), weights);
+    });
+}
+
+extern "C" int mpisdf_lebedev(int32_t npoints, double* xyz, double* weights)
+{
+    return guard([&] {
+        const mpisdf::LebedevRule* rule = mpisdf::lebedev_rule(npoints);
+        if (!rule) throw std::invalid_argument("no tabulated Lebedev rule has " + std::to_string(npoints) + " points");
+        if (!xyz || !weights) throw std::invalid_argument("null Lebedev output");
+        std::vector<double> x, w;
+        mpisdf::lebedev_points(*rule, x, w);
+        std::copy(x.begin(), x.end(), xyz);
+        std::copy(w.begin(), w.end(), weights);
     });
 }
 

@@ -1,17 +1,18 @@
 // Molecular integration grid (host, multithreaded): the input the reference
 // takes from PySCF's dft.gen_grid.Grids (python/isdf.py:488-496), built here
-// without PySCF.
+// without PySCF, with PySCF's defaults:
 //
 // * Radial: Treutler-Ahlrichs M4 mapping of Gauss-Chebyshev (2nd kind)
-//   nodes, r = xi/ln2 (1+x)^0.6 ln(2/(1-x)), xi_H = 0.8, xi_O = 0.9.
-// * Angular: product Gauss-Legendre(cos theta) x trapezoid(phi) rules whose
-//   point counts match the Lebedev sizes PySCF prunes between
-//   (NWChem-style regions on r / R_Bragg).  Lebedev generator tables are not
-//   reproduced; a product rule with nt polar and 2nt azimuthal nodes is exact
-//   for spherical harmonics up to degree 2nt-1.
+//   nodes, r = xi/ln2 (1+x)^0.6 ln(2/(1-x)), xi_H = 0.8, xi_O = 0.9; the
+//   number of shells per level and period (RAD_GRIDS).
+// * Angular: Lebedev-Laikov rules (lebedev.cpp) of the order PySCF assigns
+//   per level and period (ANG_ORDER), pruned NWChem-style: five regions on
+//   r / R_Bragg take the rules [1, 3, i-1, i, i-1] of the Lebedev size list
+//   from 38 points up, i being the full rule's position ([1, 2, 2, 2, 1] for
+//   the 50-point rule).
 // * Partition: Becke fuzzy cells (three smoothing iterations) with Treutler's
 //   sqrt(R_Bragg) size adjustment, a_ij clipped to [-1/2, 1/2].
-// * Order: atom, radial shell, polar ring, azimuth -- consecutive points are
+// * Order: atom, radial shell, Lebedev orbit -- consecutive points are
 //   spatially close, which the AO kernel's tile culling exploits.
 // * The partition runs on the GPU when one is present (one thread per point)
 //   and on host threads otherwise; both evaluate the same function with no
@@ -19,10 +20,12 @@
 #include "grid.hpp"
 
 #include "common.hpp"
+#include "lebedev.hpp"
 
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -55,62 +58,50 @@ int radial_points(int z, int level)
     return table[period(z)][std::clamp(level, 0, 9)];
 }
 
-// Polar node count of the product rule standing in for a Lebedev size.
-int polar_nodes_for(int lebedev)
+// Lebedev sizes from 38 points up (PySCF's LEBEDEV_NGRID[4:]); the pruning
+// regions index into this list.
+constexpr int kLebedevFrom38[] = {38,   50,   74,   86,   110,  146,  170,  194,  230,  266,
+                                  302,  350,  434,  590,  770,  974,  1202, 1454, 1730, 2030,
+                                  2354, 2702, 3074, 3470, 3890, 4334, 4802, 5294, 5810};
+
+// Angular rule size of the full (unpruned) shells: ANG_ORDER[level][period]
+// mapped through the Lebedev order -> size table.
+int angular_size(int z, int level)
 {
-    return std::max(2, (int)std::lround(std::sqrt(lebedev / 2.0)));
+    static const int sizes[2][10] = {{50, 110, 194, 302, 434, 590, 770, 974, 1202, 1454},
+                                     {86, 194, 302, 302, 590, 770, 974, 1202, 1202, 1454}};
+    return sizes[period(z)][std::clamp(level, 0, 9)];
 }
 
-int lebedev_size(int level)
+// Rule sizes of the five NWChem pruning regions for a full size n.
+void pruned_sizes(int n, int out[5])
 {
-    static const int table[10] = {50, 110, 194, 302, 434, 590, 770, 974, 1202, 1454};
-    return table[std::clamp(level, 0, 9)];
+    if (n < 50) {
+        std::fill(out, out + 5, n);
+        return;
+    }
+    int idx[5] = {1, 2, 2, 2, 1};
+    if (n != 50) {
+        int i = 0;
+        while (kLebedevFrom38[i] != n) ++i;
+        idx[0] = 1, idx[1] = 3, idx[2] = i - 1, idx[3] = i, idx[4] = i - 1;
+    }
+    for (int r = 0; r < 5; ++r) out[r] = kLebedevFrom38[idx[r]];
 }
 
 struct Angular {
-    std::vector<double> x, y, z, w;
+    std::vector<double> xyz, w;  // unit vectors; weights sum to 4 pi
 };
 
-Angular product_rule(int nt)
+Angular angular_rule(int n, int level)
 {
-    // Gauss-Legendre nodes by Newton iteration on P_nt.
-    std::vector<double> t(nt), wt(nt);
-    for (int i = 0; i < nt; ++i) {
-        double x = std::cos(kPi * (i + 0.75) / (nt + 0.5));
-        for (int it = 0; it < 100; ++it) {
-            double p0 = 1, p1 = x;
-            for (int k = 2; k <= nt; ++k) {
-                const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
-                p0 = p1;
-                p1 = p2;
-            }
-            const double dp = nt * (x * p1 - p0) / (x * x - 1);
-            const double dx = p1 / dp;
-            x -= dx;
-            if (std::abs(dx) < 1e-16) break;
-        }
-        double p0 = 1, p1 = x;
-        for (int k = 2; k <= nt; ++k) {
-            const double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
-            p0 = p1;
-            p1 = p2;
-        }
-        const double dp = nt * (x * p1 - p0) / (x * x - 1);
-        t[i] = x;
-        wt[i] = 2.0 / ((1 - x * x) * dp * dp);
-    }
-    const int np = 2 * nt;
+    const LebedevRule* rule = lebedev_rule(n);
+    if (!rule)
+        throw std::invalid_argument("grid level " + std::to_string(level) + " needs the " + std::to_string(n) +
+                                    "-point Lebedev rule, which is not tabulated");
     Angular a;
-    for (int i = 0; i < nt; ++i) {
-        const double st = std::sqrt(std::max(0.0, 1 - t[i] * t[i]));
-        for (int j = 0; j < np; ++j) {
-            const double ph = 2 * kPi * (j + 0.5 * (i & 1)) / np;
-            a.x.push_back(st * std::cos(ph));
-            a.y.push_back(st * std::sin(ph));
-            a.z.push_back(t[i]);
-            a.w.push_back(wt[i] * 2 * kPi / np);
-        }
-    }
+    lebedev_points(*rule, a.xyz, a.w);
+    for (double& x : a.w) x *= 4 * kPi;
     return a;
 }
 
@@ -123,22 +114,13 @@ AtomGrid atom_grid(int z, int level)
     const int nr = radial_points(z, level);
     const double xi = treutler_xi(z);
     const double rb = bragg_radius_bohr(z);
-    const int target = lebedev_size(level);
     // NWChem-style pruning regions (r / R_Bragg) and Lebedev sizes per region.
     static const double alphas[2][4] = {{0.25, 0.5, 1.0, 4.5}, {0.1667, 0.5, 0.9, 3.5}};
     int sizes[5];
-    if (target <= 50) {
-        std::fill(sizes, sizes + 5, target);
-    } else {
-        sizes[0] = 50;
-        sizes[1] = std::min(target, 86);
-        sizes[2] = std::min(target, 266);
-        sizes[3] = target;
-        sizes[4] = std::min(target, 266);
-    }
+    pruned_sizes(angular_size(z, level), sizes);
     AtomGrid g;
     std::vector<Angular> rules(5);
-    for (int r = 0; r < 5; ++r) rules[r] = product_rule(polar_nodes_for(sizes[r]));
+    for (int r = 0; r < 5; ++r) rules[r] = angular_rule(sizes[r], level);
     for (int i = 1; i <= nr; ++i) {
         const double th = i * kPi / (nr + 1);
         const double x = std::cos(th);
@@ -152,9 +134,7 @@ AtomGrid atom_grid(int z, int level)
             if (r / rb > alphas[period(z)][k]) ++place;
         const Angular& a = rules[place];
         for (size_t j = 0; j < a.w.size(); ++j) {
-            g.xyz.push_back(r * a.x[j]);
-            g.xyz.push_back(r * a.y[j]);
-            g.xyz.push_back(r * a.z[j]);
+            for (int c = 0; c < 3; ++c) g.xyz.push_back(r * a.xyz[3 * j + c]);
             g.w.push_back(wr * a.w[j]);
         }
     }

@@ -337,6 +337,12 @@ PYBIND11_MODULE(mp_isdf, m)
         .value("SVD", DistributedAlgorithm::SVD)
         .value("RandomizedQRColumnPivoting", DistributedAlgorithm::RandomizedQRColumnPivoting);
 
+    m.def("lebedev", [](int npoints) {
+        py::array_t<double> xyz({(py::ssize_t)std::max(npoints, 0), (py::ssize_t)3}), w((py::ssize_t)std::max(npoints, 0));
+        check(mpisdf_lebedev(npoints, xyz.mutable_data(), w.mutable_data()));
+        return py::make_tuple(xyz, w);
+    }, py::arg("npoints"),
+       "The npoints-point Lebedev-Laikov rule: (unit vectors [n,3], weights [n] summing to 1).");
     m.def("becke_grid", [](const IntArray& charges, const FloatArray& coords, int level, bool device) {
         if (charges.ndim() != 1 || coords.ndim() != 2 || coords.shape(1) != 3 || coords.shape(0) != charges.shape(0))
             throw py::value_error("expected charges[natm] and coords[natm,3] (Bohr)");
