@@ -229,7 +229,7 @@ int select_impl(const mpisdf_context* ctx, const double* points, const double* w
         if (naxes <= 0 || !axes) throw std::invalid_argument("axis list must not be empty");
         if (algorithm < 0 || algorithm > 5) throw std::invalid_argument("unknown anchor selection algorithm");
         if (algorithm == 3) throw std::invalid_argument("CVT selection is not part of this build");
-        if (precision < 0 || precision > 4) throw std::invalid_argument("precision must be 0..4");
+        if (precision < 0 || precision > 5) throw std::invalid_argument("precision must be 0..5");
         if (batch_size < 0 || sketch_size < 0 || oversampling < 0 || max_rank < 0)
             throw std::invalid_argument("sizes must be non-negative");
         mpisdf::SelectArgs a;
@@ -413,9 +413,24 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
                                        int64_t ldy, double* proj2, double* energy, void* stream)
 {
     return guard([&] {
-        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 4)
+        if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 5)
             throw std::invalid_argument("invalid fold arguments");
-        if (y_f32 == 3 || y_f32 == 4) {
+        if (y_f32 == 5) {
+            // Five digit planes of a copy of FP64 Y (pitch P = m rounded up
+            // to 64), then the INT8 fold.
+            mpisdf::FoldI8Scratch scratch;
+            auto st = static_cast<hipStream_t>(stream);
+            const int64_t P = (m + mpisdf::kDigitSlice - 1) / mpisdf::kDigitSlice * mpisdf::kDigitSlice;
+            mpisdf::DeviceBuffer<double> yn(std::max<int64_t>(n, 1)), sc(std::max<int64_t>(n, 1));
+            mpisdf::DeviceBuffer<double> yd((size_t)std::max<int64_t>(n * P, 1));
+            mpisdf::launch_column_norms2_f64(static_cast<const double*>(Y), m, n, ldy, yn.ptr, st);
+            MPISDF_HIP(hipMemcpy2DAsync(yd.ptr, P * sizeof(double), Y, ldy * sizeof(double), m * sizeof(double), n,
+                                        hipMemcpyDeviceToDevice, st));
+            mpisdf::digitize_columns_f64(yd.ptr, m, n, P, yn.ptr, sc.ptr, st);
+            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, 5, 8, sc.ptr,
+                                        proj2, energy, scratch, st);
+            MPISDF_HIP(hipStreamSynchronize(st));
+        } else if (y_f32 == 3 || y_f32 == 4) {
             // Digits (3 or 4 planes) of a copy of Y (pitch P = m rounded up to
             // 64), then the INT8 fold.
             mpisdf::FoldI8Scratch scratch;
@@ -428,7 +443,7 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
                                         hipMemcpyDeviceToDevice, st));
             const int planes = y_f32 == 4 ? 4 : 3;
             mpisdf::digitize_columns(yd.ptr, m, n, P, planes, yn.ptr, sc.ptr, st);
-            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, planes, sc.ptr,
+            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, planes, 4, sc.ptr,
                                         proj2, energy, scratch, st);
             MPISDF_HIP(hipStreamSynchronize(st));
         } else if (y_f32 == 2) {
@@ -455,6 +470,16 @@ extern "C" int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, in
     });
 }
 
+extern "C" int mpisdf_digitize_columns_f64_device(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2,
+                                                  double* scale, void* stream)
+{
+    return guard([&] {
+        if (m <= 0 || n < 0 || !y || !ynorm2 || !scale) throw std::invalid_argument("invalid digitise arguments");
+        mpisdf::digitize_columns_f64(y, m, n, P, ynorm2, scale, static_cast<hipStream_t>(stream));
+        MPISDF_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)));
+    });
+}
+
 extern "C" int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n, int64_t ldy, const double* ynorm2,
                                   const double* yscale, int64_t kmax, double stop_rel, int64_t* pivots,
                                   double* rdiag, int64_t* count, void* stream)
@@ -463,7 +488,7 @@ extern "C" int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n
         if (!y || !pivots || !count || m <= 0 || n <= 0 || kmax <= 0 || !(stop_rel >= 0))
             throw std::invalid_argument("invalid QRCP arguments");
         if (ytype != mpisdf::kF64 && ytype != mpisdf::kF32 && !mpisdf::is_digit_type((mpisdf::DType)ytype))
-            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32), 4 or 5 (3 or 4 digit planes)");
+            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32), 4, 5 or 6 (3, 4 or 5 digit planes)");
         mpisdf::QrcpOptions opt;
         opt.kmax = kmax;
         opt.stop_rel = stop_rel;

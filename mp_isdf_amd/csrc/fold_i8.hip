@@ -163,6 +163,67 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_kernel(floa
     }
 }
 
+// Five planes in place of FP64 column j (pitch 8 P bytes): as above with
+// the digits taken from the FP64 entries, y_hat within 2^-36 s_j of y; the
+// column's bytes past its 5 P digit bytes are zeroed.
+constexpr int kDigitizeEntriesF64 = (int)(kDigitRowsMaxF64 / kDigitizeThreads);  // P <= 16384
+__global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(double* __restrict__ y, int64_t m,
+                                                                                int64_t P,
+                                                                                double* __restrict__ ynorm2,
+                                                                                double* __restrict__ scale)
+{
+    constexpr int NY = 5;
+    __shared__ double part[kDigitizeThreads / 64];
+    const int64_t j = blockIdx.x;
+    double* src = y + j * P;  // 8 P bytes
+    double x[kDigitizeEntriesF64];
+#pragma unroll
+    for (int e = 0; e < kDigitizeEntriesF64; ++e) {
+        const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
+        x[e] = i < m ? src[i] : 0.0;
+    }
+    const double nrm = sqrt(ynorm2[j]);  // >= max |y_ij|
+    int ex = 0;
+    if (nrm > 0) frexp(nrm, &ex);
+    const double sc = nrm > 0 ? ldexp(1.0, ex + 1) : 0.0;
+    const double inv = nrm > 0 ? ldexp(1.0, -(ex + 1)) : 0.0;
+    __syncthreads();  // every entry of the column is read before its bytes are rewritten
+    int8_t* planes = reinterpret_cast<int8_t*>(src);
+    double acc = 0;
+#pragma unroll
+    for (int e = 0; e < kDigitizeEntriesF64; ++e) {
+        const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
+        if (i < P) {
+            double v = x[e] * inv;  // exact: power-of-two scaling, |v| < 1/2
+            double yh = 0, w = 1.0;
+#pragma unroll
+            for (int r = 0; r < NY; ++r) {
+                v *= 128.0;
+                const double dgt = rint(v);
+                v -= dgt;  // exact
+                planes[digit_offset(i, r, NY)] = (int8_t)dgt;
+                w *= 1.0 / 128.0;
+                yh += dgt * w;
+            }
+#pragma unroll
+            for (int r = NY; r < 8; ++r) planes[(int64_t)r * P + i] = 0;  // the column's unused last bytes
+            yh *= sc;
+            acc += yh * yh;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0;
+#pragma unroll
+        for (int w = 0; w < kDigitizeThreads / 64; ++w) t += part[w];
+        ynorm2[j] = t;
+        scale[j] = sc;
+    }
+}
+
 typedef int i16v __attribute__((ext_vector_type(16)));
 
 // NR (2 or 4): row tiles of 32 in this launch's row group.  The
@@ -180,6 +241,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                                                                    const double* __restrict__ qs, int64_t ldq,
                                                                    int64_t bpad, int64_t b, int64_t m0,
                                                                    const int8_t* __restrict__ yd, int64_t P,
+                                                                   int colbytes,
                                                                    const double* __restrict__ yscale, int64_t m,
                                                                    int64_t n, double* __restrict__ proj2,
                                                                    double* __restrict__ energy)
@@ -215,7 +277,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     // zeros, adding nothing.
     const int64_t ycols = std::min<int64_t>(FBN, n - g0);
     const __amdgpu_buffer_rsrc_t yrsrc = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<int8_t*>(yd + g0 * 4 * P), (short)0, (int)(ycols * 4 * P), kBufferWord3);
+        const_cast<int8_t*>(yd + g0 * colbytes * P), (short)0, (int)(ycols * colbytes * P), kBufferWord3);
     const __amdgpu_buffer_rsrc_t qrsrc = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<int8_t*>(qd), (short)0, (int)(ND * bpad * ldq), kBufferWord3);
     // Y digits: FBN columns x NY x 64 contiguous bytes per slice (slices of
@@ -232,7 +294,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     do {                                                                                             \
         _Pragma("unroll") for (int r_ = 0; r_ < YCHUNKS; ++r_) {                                     \
             const int cr_ = tid + NTHREADS * r_, c_ = cr_ < YTOTAL ? cr_ : cr_ - NTHREADS;          \
-            const uint32_t lane_ = (uint32_t)((c_ / (4 * NY)) * 4 * P) + 16u * (uint32_t)(c_ % (4 * NY)); \
+            const uint32_t lane_ = (uint32_t)((c_ / (4 * NY)) * colbytes * P) + 16u * (uint32_t)(c_ % (4 * NY)); \
             const uint32_t off_ = lane_ + ((uint32_t)(k0) < (uint32_t)P ? (uint32_t)NY * (uint32_t)(k0) : 0x80000000u); \
             ry[r_] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(yrsrc, off_, 0, 0)); \
         }                                                                                            \
@@ -365,14 +427,14 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 // 4-wave workgroups (NR = 2, two per CU), which multiply no zero rows.
 template <int NY>
 void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs,
-                    int64_t ldq, int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P,
+                    int64_t ldq, int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P, int colbytes,
                     const double* yscale, int64_t m, int64_t n, double* proj2, double* en)
 {
     if (rows <= 0 || rows > FBM) throw std::logic_error("INT8 fold row group out of range");
     double* const e = energy ? en : nullptr;
 #define MPISDF_FOLD(E, NR)                                                                                    \
     hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR, NY>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
-                       yscale, m, n, proj2, e)
+                       colbytes, yscale, m, n, proj2, e)
     if (rows > 64) {
         if (energy) MPISDF_FOLD(true, 4);
         else MPISDF_FOLD(false, 4);
@@ -385,6 +447,19 @@ void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const i
 }
 
 }  // namespace
+
+void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
+                          hipStream_t stream)
+{
+    if (n <= 0) return;
+    if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
+    if (P > kDigitRowsMaxF64)
+        throw std::invalid_argument("digitize_columns_f64 holds one column in registers (pitch <= 16384)");
+    if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
+    hipLaunchKernelGGL(digitize_columns_f64_kernel, dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream, y, m, P,
+                       ynorm2, scale);
+    MPISDF_HIP(hipGetLastError());
+}
 
 void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, double* ynorm2, double* scale,
                       hipStream_t stream)
@@ -404,10 +479,12 @@ void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, dou
 }
 
 void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const int8_t* Yd,
-                         int64_t P, int planes, const double* yscale, double* proj2, double* energy,
+                         int64_t P, int planes, int colbytes, const double* yscale, double* proj2, double* energy,
                          FoldI8Scratch& scratch, hipStream_t stream)
 {
-    if (planes != 3 && planes != 4) throw std::invalid_argument("digit planes must be 3 or 4");
+    if (planes < 3 || planes > 5) throw std::invalid_argument("digit planes must be 3, 4 or 5");
+    if ((colbytes != 4 && colbytes != 8) || planes > colbytes)
+        throw std::invalid_argument("digit columns take the 4 or 8 bytes per entry of the entries they replace");
     if (b <= 0 || n <= 0) return;
     if (m >= (int64_t(1) << 16)) throw std::invalid_argument("the INT8 fold needs fewer than 65536 sketch rows");
     if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
@@ -428,12 +505,18 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     // block order.
     for (int64_t m0 = 0; m0 < b; m0 += FBM) {
         const int64_t rows = std::min<int64_t>(FBM, b - m0);
-        if (planes == 4)
+        if (planes == 5)
+            launch_fold_i8<5>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
+                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
+                              scratch.part.ptr);
+        else if (planes == 4)
             launch_fold_i8<4>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
-                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, yscale, m, n, proj2, scratch.part.ptr);
+                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
+                              scratch.part.ptr);
         else
             launch_fold_i8<3>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
-                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, yscale, m, n, proj2, scratch.part.ptr);
+                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
+                              scratch.part.ptr);
         if (energy) launch_sum_partials(scratch.part.ptr, gx, rows, energy + m0, true, stream);
     }
 }

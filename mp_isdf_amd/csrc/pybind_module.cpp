@@ -217,7 +217,8 @@ public:
         else if (precision == "bf16") prec = 2;
         else if (precision == "fp8") prec = 3;
         else if (precision == "bf16x3") prec = 4;
-        else throw py::value_error("precision must be 'fp64', 'fp32', 'bf16x3', 'bf16' or 'fp8'");
+        else if (precision == "fp64i8") prec = 5;
+        else throw py::value_error("precision must be 'fp64', 'fp64i8', 'fp32', 'bf16x3', 'bf16' or 'fp8'");
         std::vector<int64_t> out(std::max<py::ssize_t>(points.shape(0), 1));
         int64_t count = 0;
         int status;

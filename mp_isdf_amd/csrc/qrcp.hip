@@ -95,14 +95,14 @@ __device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, i
     return v * scale;
 }
 
-__global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, int planes,
+__global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, int planes, int colbytes,
                                     const double* __restrict__ yscale, int64_t j0, int64_t m, int64_t nc,
                                     double* __restrict__ out)
 {
     const int64_t total = m * nc;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = i / m, r = i - j * m;
-        out[i] = digit_value(yd + (j0 + j) * 4 * P, r, yscale[j0 + j], planes);
+        out[i] = digit_value(yd + (j0 + j) * colbytes * P, r, yscale[j0 + j], planes);
     }
 }
 
@@ -187,7 +187,9 @@ __global__ void fill_candidates_kernel(const void* __restrict__ y, int ytype, in
         if (os >= 0) v = rc_prev[i + (int64_t)os * ldr];
         else if (ytype == kF64) v = static_cast<const double*>(y)[i + j * ldy];
         else if (ytype == kF32) v = (double)static_cast<const float*>(y)[i + j * ldy];
-        else v = digit_value(static_cast<const int8_t*>(y) + j * 4 * ldy, i, yscale[j], digit_planes_of((DType)ytype));
+        else
+            v = digit_value(static_cast<const int8_t*>(y) + j * digit_column_bytes((DType)ytype) * ldy, i, yscale[j],
+                            digit_planes_of((DType)ytype));
         rc[i + c * ldr] = v;
     }
 }
@@ -622,7 +624,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         res.fold_flops += 2.0 * (double)b * (double)m * (double)n;
         if (digits)
             fold_projections_i8(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const int8_t*>(y), ldy, planes,
-                                opt.yscale, proj2.ptr, energy_out ? energy_out + folded : nullptr, i8_scratch, s);
+                                digit_column_bytes(ytype), opt.yscale, proj2.ptr,
+                                energy_out ? energy_out + folded : nullptr, i8_scratch, s);
         else if (split)
             fold_projections_bf16x3(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const float*>(y), ldy, proj2.ptr,
                                     energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
@@ -793,7 +796,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             const int64_t cnt = std::min(nc, n - j0);
             if (digits)
                 hipLaunchKernelGGL(widen_digits_kernel, dim3(grid_for(m * cnt)), dim3(256), 0, s,
-                                   static_cast<const int8_t*>(y), ldy, planes, opt.yscale, j0, m, cnt, R.ptr);
+                                   static_cast<const int8_t*>(y), ldy, planes, digit_column_bytes(ytype), opt.yscale,
+                                   j0, m, cnt, R.ptr);
             else if (ytype == kF64)
                 hipLaunchKernelGGL(widen_kernel<double>, dim3(grid_for(m * cnt)), dim3(256), 0, s,
                                    static_cast<const double*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);

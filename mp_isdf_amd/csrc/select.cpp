@@ -300,7 +300,12 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     if (n == 0) return {};
     if (a.precision == kPrecF32)
         throw std::invalid_argument("precision 'fp32' is not provided; use fp64, bf16x3, bf16 or fp8");
-    const bool lowp = a.precision != kPrecF64;
+    const bool lowp = a.precision != kPrecF64 && a.precision != kPrecF64I8;
+    // The FP64 sketch on five digit planes: pitch a multiple of 64, ell within
+    // the FP64 digitiser's register budget.
+    const bool digits5 = a.precision == kPrecF64I8;
+    if (digits5 && ell > kDigitRowsMaxF64)
+        throw std::invalid_argument("precision 'fp64i8' supports sketches of at most 16384 rows");
     // MFMA operand form of the reduced-precision sketch (sketch_mfma.hpp):
     // BF16, two-term FP8 or split BF16 (hi.hi + lo.hi + hi.lo).
     const int form = a.precision == kPrecFP8 ? kSketchFP8 : a.precision == kPrecBF16x3 ? kSketchBF16x3 : kSketchBF16;
@@ -334,7 +339,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     const bool use_digits = lowp && ell <= kDigitRowsMax;
     const int64_t ldy32 = use_digits ? (ell + kDigitSlice - 1) / kDigitSlice * kDigitSlice
                                      : (ell + 3) / 4 * 4;  // 256- or 16-byte aligned FP32 columns
-    const int64_t ldy64 = (ell + 3) / 4 * 4;  // 32-byte aligned FP64 columns
+    const int64_t ldy64 = digits5 ? (ell + kDigitSlice - 1) / kDigitSlice * kDigitSlice
+                                  : (ell + 3) / 4 * 4;  // 32-byte aligned FP64 columns
     DeviceBuffer<double> ynorm2;
     AxisWorkspace ws;
     double t_sketch_kernel = 0, sketch_launches = 0;
@@ -415,6 +421,9 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     if (use_digits) {
         yscale.resize(n);
         digitize_columns(y32.ptr, ell, n, ldy32, planes, ynorm2.ptr, yscale.ptr, s);
+    } else if (digits5) {
+        yscale.resize(n);
+        digitize_columns_f64(y64.ptr, ell, n, ldy64, ynorm2.ptr, yscale.ptr, s);
     }
     const double t_sketch = seconds_since(started);
 
@@ -439,11 +448,12 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     // INT8 MFMA (exact digit products, ~1e-11 of ||y_j||^2 per fold, covered
     // by the margin) and factorise y_hat, which equals the FP32 sketch to
     // 2^-29 of each column's scale.
-    if (use_digits) {
+    if (use_digits || digits5) {
         opt.yscale = yscale.ptr;
         opt.margin = kFoldI8Margin;
     }
     const QrcpResult r = use_digits ? qrcp_device(y32.ptr, planes == 4 ? kI8Digits4 : kI8Digits, ell, n, ldy32, opt, s)
+                         : digits5  ? qrcp_device(y64.ptr, kI8Digits5, ell, n, ldy64, opt, s)
                          : lowp     ? qrcp_device(y32.ptr, kF32, ell, n, ldy32, opt, s)
                                     : qrcp_device(y64.ptr, kF64, ell, n, ldy64, opt, s);
     const double t_qrcp = seconds_since(tq);
@@ -521,7 +531,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         // split-BF16 form, two for the two-term FP8 form).
         diag[kDiagSketchFlops] = 4.0 * (double)ell * (double)d * (double)n * (lowp ? sketch_segments(form) : 1);
         diag[kDiagSketchLaunches] = sketch_launches;
-        diag[kDiagFoldKind] = use_digits ? (planes == 4 ? 3.0 : 2.0) : (lowp ? 1.0 : 0.0);
+        diag[kDiagFoldKind] = digits5 ? 4.0 : use_digits ? (planes == 4 ? 3.0 : 2.0) : (lowp ? 1.0 : 0.0);
     }
     return out;
 }

@@ -100,8 +100,9 @@ void launch_colsumsq(const double* f, int64_t n, int64_t w, double* out, DeviceB
 }
 
 namespace {
-__global__ void column_norms2_f32_kernel(const float* __restrict__ y, int64_t m, int64_t n, int64_t ldy,
-                                         double* __restrict__ out)
+template <class T>
+__global__ void column_norms2_kernel(const T* __restrict__ y, int64_t m, int64_t n, int64_t ldy,
+                                     double* __restrict__ out)
 {
     const int lane = threadIdx.x & 63;
     const int64_t j = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
@@ -120,7 +121,15 @@ __global__ void column_norms2_f32_kernel(const float* __restrict__ y, int64_t m,
 void launch_column_norms2_f32(const float* y, int64_t m, int64_t n, int64_t ldy, double* out, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(column_norms2_f32_kernel, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, y, m, n, ldy, out);
+    hipLaunchKernelGGL(column_norms2_kernel<float>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, y, m, n, ldy, out);
+    MPISDF_HIP(hipGetLastError());
+}
+
+void launch_column_norms2_f64(const double* y, int64_t m, int64_t n, int64_t ldy, double* out, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(column_norms2_kernel<double>, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, s, y, m, n, ldy,
+                       out);
     MPISDF_HIP(hipGetLastError());
 }
 

@@ -25,5 +25,7 @@ void launch_sum_fixed(const double* x, int64_t n, int64_t ld, int64_t nvec, doub
 
 // out[j] = ||y_j||^2 (FP64) of the FP32 columns y[j * ldy + i], i < m.
 void launch_column_norms2_f32(const float* y, int64_t m, int64_t n, int64_t ldy, double* out, hipStream_t s);
+// The same for FP64 columns.
+void launch_column_norms2_f64(const double* y, int64_t m, int64_t n, int64_t ldy, double* out, hipStream_t s);
 
 }  // namespace mpisdf

@@ -100,14 +100,14 @@ def test_lowp_selection_is_valid(mp_isdf):
     ctx = mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env)
     sel = mp_isdf.AnchorSelection(ctx)
     alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
-    for precision in ("bf16", "fp8", "bf16x3"):
+    for precision in ("bf16", "fp8", "bf16x3", "fp64i8"):
         got = sel.select(pts, w, mp_isdf.AxisType.AtomicOrbital, algorithm=alg, max_rank=120, oversampling=12,
                          seed=7, precision=precision)
         assert len(got) == 120 and len(set(got)) == 120
         assert not set(got) & set(np.flatnonzero(w == 0).tolist())
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 4, 5])
 def test_qrcp_fold_modes(mp_isdf, mode):
     """Fold = column / row sums of (Q^T Y)^2 against NumPy; the split-BF16
     mode is checked against its stated bound (1e-5 of ||y_j||^2), the INT8
@@ -121,8 +121,12 @@ def test_qrcp_fold_modes(mp_isdf, mode):
     w = q.T @ y.T
     want_cols = np.sum(w ** 2, axis=0)
     want_rows = np.sum(w ** 2, axis=1)
-    ld = m + 1 if mode == 0 else (m + 3) // 4 * 4
-    ybuf = np.zeros((n, ld), dtype=np.float64 if mode == 0 else np.float32)
+    if mode == 5:  # FP64 entries
+        y = rng.normal(size=(n, m)) * rng.uniform(0.1, 3.0, size=(n, 1))
+        w = q.T @ y.T
+        want_cols, want_rows = np.sum(w ** 2, axis=0), np.sum(w ** 2, axis=1)
+    ld = m + 1 if mode in (0, 5) else (m + 3) // 4 * 4
+    ybuf = np.zeros((n, ld), dtype=np.float64 if mode in (0, 5) else np.float32)
     ybuf[:, :m] = y
     Y = torch.tensor(ybuf, device="cuda")
     Qb = torch.tensor(np.ascontiguousarray(q.T), device="cuda")  # column-major m x b
@@ -140,7 +144,14 @@ def test_qrcp_fold_modes(mp_isdf, mode):
         assert np.all(np.abs(y_hat - y) <= 2.0 ** -(7 * planes + 1) * scale[:, None])
         w_hat = q.T @ y_hat.T
         want_cols, want_rows = np.sum(w_hat ** 2, axis=0), np.sum(w_hat ** 2, axis=1)
-    tol = {0: 1e-12, 1: 1e-12, 2: 1e-5, 3: 1e-10, 4: 1e-10}[mode]
+    if mode == 5:
+        # five digits of each FP64 column (model in FP64), every entry within
+        # 2^-36 s of y; the fold works on that y_hat
+        scale, _, y_hat = _digit_model_f64(y, ynorm2)
+        assert np.all(np.abs(y_hat - y) <= 2.0 ** -36 * scale[:, None])
+        w_hat = q.T @ y_hat.T
+        want_cols, want_rows = np.sum(w_hat ** 2, axis=0), np.sum(w_hat ** 2, axis=1)
+    tol = {0: 1e-12, 1: 1e-12, 2: 1e-5, 3: 1e-10, 4: 1e-10, 5: 1e-10}[mode]
     assert np.max(np.abs(proj.cpu().numpy() - want_cols) / ynorm2) < tol
     np.testing.assert_allclose(energy.cpu().numpy(), want_rows, rtol=max(tol, 1e-12))
 
@@ -158,6 +169,21 @@ def _digit_model(y, ynorm2, planes=3):
         weight /= 128.0
         digits.append(dgt.astype(np.int8))
         y_hat += dgt.astype(np.float64) * weight
+    return scale, digits, y_hat * scale[:, None]
+
+
+def _digit_model_f64(y, ynorm2, planes=5):
+    """The same model for FP64 columns: digits of y / s in FP64 steps."""
+    scale = np.exp2(np.frexp(np.sqrt(ynorm2))[1] + 1.0)
+    x = y / scale[:, None]
+    digits, y_hat, weight = [], np.zeros_like(y), 1.0
+    for _ in range(planes):
+        x = x * 128.0
+        dgt = np.rint(x)
+        x = x - dgt
+        weight /= 128.0
+        digits.append(dgt.astype(np.int8))
+        y_hat += dgt * weight
     return scale, digits, y_hat * scale[:, None]
 
 
@@ -362,3 +388,56 @@ def test_qrcp_tall_sketch_matches_oracle(mp_isdf, m):
     capi.call("mpisdf_qrcp_device", torch.tensor(y, device="cuda"), 0, m, n, m, None, None, kmax, 0.0, piv, rd, cnt,
               None)
     np.testing.assert_array_equal(piv[:cnt[0]], order[:kmax])
+
+
+def test_qrcp_five_digit_planes_of_fp64_match_oracle(mp_isdf):
+    """The fp64i8 selection's QRCP: five digit planes in place of each FP64
+    column (checked digit for digit against the NumPy model, unused bytes
+    zero), y_hat within 2^-36 s of y, and INT8-fold pivots equal to the
+    oracle's dgeqp3-semantics QRCP on that y_hat and on y itself."""
+    import torch
+    rng = np.random.default_rng(13)
+    m, n, kmax = 385, 3000, 160
+    y = (rng.normal(size=(n, 80)) @ rng.normal(size=(80, m)) * rng.uniform(0.2, 2.0, size=(n, 1))
+         + 1e-5 * rng.normal(size=(n, m)))
+    ynorm2 = np.sum(y ** 2, axis=1)
+    P = (m + 63) // 64 * 64
+    buf = np.zeros((n, P), dtype=np.float64)
+    buf[:, :m] = y
+    Y = torch.tensor(buf, device="cuda")
+    yn = torch.tensor(ynorm2, device="cuda")
+    sc = torch.zeros(n, dtype=torch.float64, device="cuda")
+    capi.call("mpisdf_digitize_columns_f64_device", Y, m, n, P, yn, sc, None)
+    scale, digits, y_hat = _digit_model_f64(y, ynorm2)
+    assert np.all(np.abs(y_hat - y) <= 2.0 ** -36 * scale[:, None])
+    raw = Y.cpu().numpy().view(np.int8).reshape(n, 8 * P)
+    planes = raw[:, :5 * P].reshape(n, P // 64, 5, 64).transpose(0, 2, 1, 3).reshape(n, 5, P)
+    assert not raw[:, 5 * P:].any()
+    np.testing.assert_array_equal(sc.cpu().numpy(), scale)
+    for r in range(5):
+        np.testing.assert_array_equal(planes[:, r, :m], digits[r])
+        assert not planes[:, r, m:].any()
+    np.testing.assert_allclose(yn.cpu().numpy(), np.sum(y_hat ** 2, axis=1), rtol=1e-14)
+    piv = np.zeros(kmax, dtype=np.int64)
+    rd = np.zeros(kmax)
+    cnt = np.zeros(1, dtype=np.int64)
+    capi.call("mpisdf_qrcp_device", Y, 6, m, n, P, yn, sc, kmax, 0.0, piv, rd, cnt, None)
+    _, want, _ = oracle.geqp3(np.ascontiguousarray(y_hat.T))
+    np.testing.assert_array_equal(piv[:cnt[0]], want[:kmax])
+    _, exact, _ = oracle.geqp3(np.ascontiguousarray(y.T))
+    np.testing.assert_array_equal(piv[:cnt[0]], exact[:kmax])
+
+
+def test_fp64i8_selects_the_fp64_anchors(mp_isdf):
+    """(H2O)8 / cc-pVDZ, level-2 grid, 1,536 rand_sketch anchors: the FP64
+    sketch on five digit planes pivots like the FP64 form (identical anchors
+    in the same order)."""
+    from mp_isdf_amd import grid as gridmod
+    mol = molecule.water_cluster(8, "cc-pvdz")
+    xyz, w = gridmod.make_grid(mol, 2)
+    sel = mp_isdf.AnchorSelection(mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env))
+    alg = mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao
+    got = {p: sel.select_with_diagnostics(xyz, w, mp_isdf.AxisType.AtomicOrbital, algorithm=alg, max_rank=1536,
+                                          oversampling=154, seed=7, precision=p) for p in ("fp64", "fp64i8")}
+    assert got["fp64i8"][1]["fold_kind"] == 4 and got["fp64"][1]["fold_kind"] == 0
+    assert got["fp64i8"][0] == got["fp64"][0]
