@@ -13,8 +13,9 @@ weights already resident in HBM.
 The metric compares reduced-precision forms with the all-FP64 form at equal
 fit error: relative pair-space error within 1e-4 of the FP64 form's and at
 least 98% of its anchors.  Every form's criteria are checked in the run; the
-headline ``value`` is the value form's throughput when it meets both, else
-the FP64 form's.
+headline ``value`` is the throughput of the fastest candidate form (by
+default split-BF16 and the FP64 sketch on INT8 digit planes, each timed over
+the same steps) that meets both, else the FP64 form's.
 
 Printed: one JSON line (metric, grid-pts/s value, per-stage seconds, the
 FP64 form and the other precision forms for comparison, the anchor-overlap
@@ -43,9 +44,10 @@ CONFIGS = {
 }
 PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "int8_mfma_tops": 5000.0,
         "hbm_gbs": 8000.0}
-# The INT8 fold splits q into 5 and y into 3 (4) signed base-128 digits and
-# runs the 14 (17) digit pairs with p + r <= 5 as INT8 products.
-I8_PAIRS = {2: 14, 3: 17}
+# The INT8 fold splits q into 5 and y into 3 (4, 5) signed base-128 digits and
+# runs the 14 (17, 19) digit pairs with p + r <= 5 as INT8 products; keyed by
+# the selection's fold_kind.
+I8_PAIRS = {2: 14, 3: 17, 4: 19}
 FIT_BATCH = 32768  # grid points per fit batch
 EQUAL_PAIR_ERROR = 1e-4  # |relative pair-space error - FP64 form's|
 EQUAL_OVERLAP = 0.98     # fraction of the FP64 form's anchors
@@ -61,8 +63,8 @@ def parse():
     p.add_argument("--steps", type=int, default=2)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--config", default="h2o64", choices=sorted(CONFIGS))
-    p.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16", "fp8", "fp64"],
-                   help="the value form")
+    p.add_argument("--precision", default="auto", choices=["auto", "bf16x3", "fp64i8", "bf16", "fp8", "fp64"],
+                   help="the value form; auto times bf16x3 and fp64i8 and takes the faster one at equal fit error")
     p.add_argument("--other-forms", default="bf16,fp8",
                    help="reduced forms run once each for the equal-fit-error table ('' for none)")
     p.add_argument("--no-fp64-form", action="store_true", help="skip the all-FP64 comparison runs")
@@ -177,14 +179,17 @@ def main():
             describe(i, label, times[-1], d)
         return max(times), diags[-1], anchors
 
-    # ---- the value form: W warmups, then exactly K timed steps ---------------
-    t_step, d, anchors = timed(args.precision, args.steps, args.warmup, args.precision)
+    # ---- the candidate value forms: W warmups, then exactly K timed steps each
+    # (fp64i8 first: the reduced forms that follow reuse buffers of the sizes
+    # it freed or smaller)
+    candidates = ["fp64i8", "bf16x3"] if args.precision == "auto" else [args.precision]
+    timed_forms = {c: timed(c, args.steps, args.warmup, c) for c in candidates}
 
     # ---- the other reduced forms, once each (right after the value form, so
     # their sketch buffers follow buffers of the same size) --------------------
     others = {}
     if not args.no_fp64_form:
-        for other in [f for f in args.other_forms.split(",") if f and f != args.precision and f != "fp64"]:
+        for other in [f for f in args.other_forms.split(",") if f and f not in candidates and f != "fp64"]:
             torch.cuda.synchronize()
             t = time.perf_counter()
             a_o, d_o = step(other)
@@ -194,7 +199,7 @@ def main():
 
     # ---- the all-FP64 form over the same number of steps ----------------------
     fp64_form, ref, e64 = None, None, None
-    if not args.no_fp64_form and args.precision != "fp64":
+    if not args.no_fp64_form and candidates != ["fp64"]:
         t64, d64, ref = timed("fp64", args.steps, 0, "fp64")
         e64 = d64["pair_space_relative_error"]
         fp64_form = {"seconds": round(t64, 4), "ms_per_step": round(t64 * 1e3, 2), "steps": args.steps,
@@ -224,12 +229,24 @@ def main():
                 "pair_error_ok": abs(diff) <= EQUAL_PAIR_ERROR, "overlap_ok": overlap >= EQUAL_OVERLAP,
                 "equal_fit_error": abs(diff) <= EQUAL_PAIR_ERROR and overlap >= EQUAL_OVERLAP}
 
-    value_check = criteria(anchors, d["pair_space_relative_error"]) if args.precision != "fp64" else None
     forms = {}
-    if value_check is not None:
-        forms[args.precision] = dict(value_check, seconds=round(t_step, 4), value=round(n / t_step, 1),
-                                     speedup_vs_fp64=round(fp64_form["seconds"] / t_step, 3),
-                                     pair_space_relative_error=d["pair_space_relative_error"], steps=args.steps)
+    for c, (t_c, d_c, a_c) in timed_forms.items():
+        check = criteria(a_c, d_c["pair_space_relative_error"]) if c != "fp64" else None
+        if check is not None:
+            forms[c] = dict(check, seconds=round(t_c, 4), value=round(n / t_c, 1),
+                            speedup_vs_fp64=round(fp64_form["seconds"] / t_c, 3),
+                            pair_space_relative_error=d_c["pair_space_relative_error"], steps=args.steps,
+                            seconds_sketch=round(d_c["seconds_sketch"], 4), seconds_qrcp=round(d_c["seconds_qrcp"], 4))
+    # ---- headline: the fastest candidate at equal fit error, else FP64 -------
+    # (without the FP64 comparison runs nothing can be checked: the first candidate)
+    passing = candidates[:1] if ref is None else [
+        c for c in candidates if c == "fp64" or (c in forms and forms[c]["equal_fit_error"])]
+    if passing:
+        value_form = min(passing, key=lambda c: timed_forms[c][0])
+        t_step, d, anchors = timed_forms[value_form]
+    else:
+        value_form, t_step, d, anchors = "fp64", t64, d64, ref
+        log(f"no candidate form is at equal fit error: reporting the FP64 form as the value")
     for other, (t_o, a_o, d_o) in others.items():
         if ref is None:
             break
@@ -248,8 +265,9 @@ def main():
     if fold_kind in I8_PAIRS:
         pairs = I8_PAIRS[fold_kind]
         ops = pairs * fold_f
+        sketch_type = "FP64" if fold_kind == 4 else "FP32"
         roofline_fold = {"kernel": f"fold_i8_32x32_kernel (QRCP fold on {fold_kind + 1} INT8 digit planes of the "
-                                   f"FP32 sketch)",
+                                   f"{sketch_type} sketch)",
                          "bound": "mfma", "achieved": round(ops / fold_s / 1e12, 2) if fold_s else None,
                          "peak": PEAK["int8_mfma_tops"], "unit": "TOP/s",
                          "frac": round(ops / fold_s / 1e12 / PEAK["int8_mfma_tops"], 4) if fold_s else None,
@@ -274,10 +292,10 @@ def main():
                     "launches": fit_n, "launch_ms": round(fit_s / fit_n * 1e3, 3), "flops_per_launch": fit_f / fit_n,
                     "work": f"(2*k*d + k*(k+1)) per grid point, k={k}, d={mol.nao}, n={n}, {FIT_BATCH}-point batches"}
     sk_s, sk_f, sk_n = d.get("sketch_kernel_seconds", 0.0), d.get("sketch_flops", 0.0), max(1, int(d.get("sketch_launches", 0)))
-    terms = {"bf16x3": 3, "fp8": 2}.get(args.precision, 1)
-    peak_sk = PEAK["fp64_mfma_tflops" if args.precision == "fp64" else
-                   ("fp8_mfma_tflops" if args.precision == "fp8" else "bf16_mfma_tflops")]
-    roofline_sketch = {"kernel": f"kr_sketch ({args.precision})", "bound": "mfma",
+    terms = {"bf16x3": 3, "fp8": 2}.get(value_form, 1)
+    peak_sk = PEAK["fp64_mfma_tflops" if value_form in ("fp64", "fp64i8") else
+                   ("fp8_mfma_tflops" if value_form == "fp8" else "bf16_mfma_tflops")]
+    roofline_sketch = {"kernel": f"kr_sketch ({value_form})", "bound": "mfma",
                        "achieved": round(sk_f / sk_s / 1e12, 2) if sk_s else None, "peak": peak_sk,
                        "unit": "TFLOP/s", "frac": round(sk_f / sk_s / 1e12 / peak_sk, 4) if sk_s else None,
                        "launches": sk_n, "launch_ms": round(sk_s / sk_n * 1e3, 3),
@@ -289,7 +307,7 @@ def main():
     if traffic_file.exists() and args.config == "h2o64":
         measured = json.loads(traffic_file.read_text())
         for key, entry in (("qrcp_fold", roofline_fold), ("fit_grid_gemm", roofline_fit), ("sketch", roofline_sketch)):
-            rec = measured.get(args.precision, {}).get(key, {})
+            rec = measured.get(value_form, {}).get(key, {})
             tag = rec.get("kernel_tag")
             if tag and tag in entry["kernel"]:
                 entry["traffic"] = round(rec["traffic_bytes_per_launch"] / 1e9, 3)
@@ -415,11 +433,7 @@ def main():
                 "sample": f"oracle grid metric (C loops, FP64) of {km} vectors over {mc} grid points in {tc:.1f} s"}
             log(f"coulomb metric cpu sample: {mc} points in {tc:.1f} s")
 
-    # ---- headline: the value form if it is at equal fit error, else FP64 -----
-    value_form, t_value = args.precision, t_step
-    if value_check is not None and not value_check["equal_fit_error"]:
-        value_form, t_value = "fp64", fp64_form["seconds"]
-        log(f"{args.precision} form is not at equal fit error: reporting the FP64 form as the value")
+    t_value = t_step
     value = n / t_value
     out = {
         "metric": "anchor-select+fit s and grid-pts/s, BF16/FP8 vs FP64 GPU at equal fit error",
@@ -438,7 +452,7 @@ def main():
                    "seconds_qrcp": round(d.get("seconds_qrcp", 0), 4), "qrcp_blocks": int(d.get("blocks", 0)),
                    "seconds_fit_factor": round(d.get("seconds_fit_factor", 0), 4),
                    "seconds_fit_grid": round(d.get("seconds_fit_grid", 0), 4),
-                   "of_form": args.precision},
+                   "of_form": value_form},
         "fit_quality": {"pair_space_relative_error": d.get("pair_space_relative_error"),
                         "gram_condition_estimate": d.get("gram_condition_estimate")},
         "fp64_form": fp64_form,
