@@ -44,10 +44,11 @@ CONFIGS = {
 }
 PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops": 5000.0, "int8_mfma_tops": 5000.0,
         "hbm_gbs": 8000.0}
-# The INT8 fold splits q into 5 and y into 3 (4, 5) signed base-128 digits and
-# runs the 14 (17, 19) digit pairs with p + r <= 5 as INT8 products; keyed by
-# the selection's fold_kind.
-I8_PAIRS = {2: 14, 3: 17, 4: 19}
+# The INT8 fold splits q into 5 and y into 3 (4) signed base-128 digits and
+# runs the 14 (17) digit pairs with p + r <= 5 as INT8 products; for the FP64
+# sketch, q into 5 and y into 4 base-254 digits, the 14 pairs p + r <= 4.
+# Keyed by the selection's fold_kind.
+I8_PAIRS = {2: 14, 3: 17, 4: 14}
 FIT_BATCH = 32768  # grid points per fit batch
 EQUAL_PAIR_ERROR = 1e-4  # |relative pair-space error - FP64 form's|
 EQUAL_OVERLAP = 0.98     # fraction of the FP64 form's anchors
@@ -265,9 +266,9 @@ def main():
     if fold_kind in I8_PAIRS:
         pairs = I8_PAIRS[fold_kind]
         ops = pairs * fold_f
-        sketch_type = "FP64" if fold_kind == 4 else "FP32"
-        roofline_fold = {"kernel": f"fold_i8_32x32_kernel (QRCP fold on {fold_kind + 1} INT8 digit planes of the "
-                                   f"{sketch_type} sketch)",
+        planes_desc = "4 base-254 INT8 digit planes of the FP64" if fold_kind == 4 else \
+            f"{fold_kind + 1} base-128 INT8 digit planes of the FP32"
+        roofline_fold = {"kernel": f"fold_i8_32x32_kernel (QRCP fold on {planes_desc} sketch)",
                          "bound": "mfma", "achieved": round(ops / fold_s / 1e12, 2) if fold_s else None,
                          "peak": PEAK["int8_mfma_tops"], "unit": "TOP/s",
                          "frac": round(ops / fold_s / 1e12 / PEAK["int8_mfma_tops"], 4) if fold_s else None,

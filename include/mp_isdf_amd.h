@@ -78,8 +78,8 @@ int mpisdf_evaluate_axis(const mpisdf_context* ctx, const double* points, int64_
  * need mpisdf_select_importance.
  * precision (rand_sketch only): 0 fp64, 1 fp32 (rejected), 2 bf16, 3 fp8 e4m3
  * (two-term phi), 4 split bf16 (hi.hi + lo.hi + hi.lo, sketch digits in four
- * planes), 5 fp64i8 (the FP64 sketch on five digit planes with the INT8 fold;
- * at most 16384 sketch rows).
+ * planes), 5 fp64i8 (the FP64 sketch on four base-254 digit planes with the
+ * INT8 fold; at most 16384 sketch rows).
  * axis_weights: NULL or naxis_weights == naxes finite positive factors.
  * debug: bit 0 prints the reference's progress lines; bit 1 (value 2) runs
  * the dense audit of rand_sketch and density-sketch pivots
@@ -206,8 +206,8 @@ int mpisdf_kr_sketch_device_noalloc(int dtype, const void* ut, const void* vt, i
  * y_f32 = 0: FP64 Y, FP64 MFMA; 1: FP32 Y, FP64 MFMA (exact products);
  * 2: FP32 Y, split-BF16 MFMA with FP64 chunk accumulation;
  * 3 / 4: FP32 Y, INT8 MFMA on three / four base-128 digit planes of it
- * (~1e-10 relative; m < 65536); 5: FP64 Y, INT8 MFMA on five digit planes
- * of it (m <= 16384).
+ * (~1e-10 relative; m < 65536); 5: FP64 Y, INT8 MFMA on four base-254
+ * digit planes of it (m <= 16384).
  * proj2[j] += sum_t W[t,j]^2 and, if energy != NULL, energy[t] += sum_j W[t,j]^2.
  * W is never stored. */
 int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, const void* Y, int y_f32,
@@ -226,10 +226,12 @@ int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const double* Qb, c
 int mpisdf_digitize_columns_device(float* y, int64_t m, int64_t n, int64_t P, int32_t planes, double* ynorm2,
                                    double* scale, void* stream);
 /* The same in place of FP64 column j (pitch 8 P bytes = P doubles, P a
- * multiple of 64, m <= P <= 16384): five planes, each entry within 2^-36
- * scale[j] of y, digit r of entry i at byte 320 (i / 64) + 64 r + i % 64 of
- * the column (zero past m and in its last 3 P bytes).  The digit form of the
- * FP64 sketch that precision fp64i8 factorises (no reference counterpart). */
+ * multiple of 64, m <= P <= 16384): four signed base-254 planes (|d| <= 127),
+ * y_hat_j[i] = scale[j] sum_r d_r[i] 254^-(r+1), each entry within
+ * scale[j] 254^-4 / 2 of y, digit r of entry i at byte 256 (i / 64) + 64 r +
+ * i % 64 of the column (zero past m and in its last 4 P bytes).  The digit
+ * form of the FP64 sketch that precision fp64i8 factorises (no reference
+ * counterpart). */
 int mpisdf_digitize_columns_f64_device(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
                                        void* stream);
 
@@ -240,8 +242,8 @@ int mpisdf_digitize_columns_f64_device(double* y, int64_t m, int64_t n, int64_t 
  * ytype 0 FP64 or 1 FP32 column-major (ldy elements; ynorm2 and yscale may
  * be NULL), or 4 / 5 digits (three / four planes) from
  * mpisdf_digitize_columns_device (ldy = P, the column pitch in floats; ynorm2
- * and yscale as it wrote them; folds on INT8 MFMA), or 6 (five planes) from
- * mpisdf_digitize_columns_f64_device (ldy = P in doubles).
+ * and yscale as it wrote them; folds on INT8 MFMA), or 6 (four base-254
+ * planes) from mpisdf_digitize_columns_f64_device (ldy = P in doubles).
  * Stops before a pivot with |R_ii| <= stop_rel |R_00| or after kmax pivots.
  * Writes 0-based pivots and |R_ii| (host arrays of kmax entries; rdiag may
  * be NULL) and their count. */

@@ -416,8 +416,8 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
         if (b < 0 || n < 0 || m <= 0 || ldy < m || !proj2 || y_f32 < 0 || y_f32 > 5)
             throw std::invalid_argument("invalid fold arguments");
         if (y_f32 == 5) {
-            // Five digit planes of a copy of FP64 Y (pitch P = m rounded up
-            // to 64), then the INT8 fold.
+            // Four base-254 digit planes of a copy of FP64 Y (pitch P = m
+            // rounded up to 64), then the INT8 fold.
             mpisdf::FoldI8Scratch scratch;
             auto st = static_cast<hipStream_t>(stream);
             const int64_t P = (m + mpisdf::kDigitSlice - 1) / mpisdf::kDigitSlice * mpisdf::kDigitSlice;
@@ -427,7 +427,7 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
             MPISDF_HIP(hipMemcpy2DAsync(yd.ptr, P * sizeof(double), Y, ldy * sizeof(double), m * sizeof(double), n,
                                         hipMemcpyDeviceToDevice, st));
             mpisdf::digitize_columns_f64(yd.ptr, m, n, P, yn.ptr, sc.ptr, st);
-            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, 5, 8, sc.ptr,
+            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, 4, 254, 8, sc.ptr,
                                         proj2, energy, scratch, st);
             MPISDF_HIP(hipStreamSynchronize(st));
         } else if (y_f32 == 3 || y_f32 == 4) {
@@ -443,8 +443,8 @@ extern "C" int mpisdf_qrcp_fold_device(int64_t b, int64_t n, int64_t m, const do
                                         hipMemcpyDeviceToDevice, st));
             const int planes = y_f32 == 4 ? 4 : 3;
             mpisdf::digitize_columns(yd.ptr, m, n, P, planes, yn.ptr, sc.ptr, st);
-            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, planes, 4, sc.ptr,
-                                        proj2, energy, scratch, st);
+            mpisdf::fold_projections_i8(b, n, m, Qb, m, reinterpret_cast<const int8_t*>(yd.ptr), P, planes, 128, 4,
+                                        sc.ptr, proj2, energy, scratch, st);
             MPISDF_HIP(hipStreamSynchronize(st));
         } else if (y_f32 == 2) {
             mpisdf::FoldScratch scratch;
@@ -488,7 +488,7 @@ extern "C" int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n
         if (!y || !pivots || !count || m <= 0 || n <= 0 || kmax <= 0 || !(stop_rel >= 0))
             throw std::invalid_argument("invalid QRCP arguments");
         if (ytype != mpisdf::kF64 && ytype != mpisdf::kF32 && !mpisdf::is_digit_type((mpisdf::DType)ytype))
-            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32), 4, 5 or 6 (3, 4 or 5 digit planes)");
+            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32), 4 or 5 (3 or 4 base-128 planes) or 6 (4 base-254 planes)");
         mpisdf::QrcpOptions opt;
         opt.kmax = kmax;
         opt.stop_rel = stop_rel;

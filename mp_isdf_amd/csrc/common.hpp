@@ -46,13 +46,14 @@ void ensure_device();  // throws NoDevice when no HIP device is present
 // Output element types of the AO and sketch kernels.
 // kI8Digits / kI8Digits4: a sketch held as three / four signed base-128 digit
 // planes per column in place of its FP32 entries (digitize_columns), with a
-// per-column power-of-two scale; kI8Digits5: five planes in place of FP64
-// entries (digitize_columns_f64).
-enum DType : int { kF64 = 0, kF32 = 1, kBF16 = 2, kFP8 = 3, kI8Digits = 4, kI8Digits4 = 5, kI8Digits5 = 6 };
-__host__ __device__ inline bool is_digit_type(DType t) { return t == kI8Digits || t == kI8Digits4 || t == kI8Digits5; }
-__host__ __device__ inline int digit_planes_of(DType t) { return t == kI8Digits5 ? 5 : t == kI8Digits4 ? 4 : 3; }
+// per-column power-of-two scale; kI8Digits254: four signed base-254 planes in
+// place of FP64 entries (digitize_columns_f64).
+enum DType : int { kF64 = 0, kF32 = 1, kBF16 = 2, kFP8 = 3, kI8Digits = 4, kI8Digits4 = 5, kI8Digits254 = 6 };
+__host__ __device__ inline bool is_digit_type(DType t) { return t == kI8Digits || t == kI8Digits4 || t == kI8Digits254; }
+__host__ __device__ inline int digit_planes_of(DType t) { return t == kI8Digits254 || t == kI8Digits4 ? 4 : 3; }
+__host__ __device__ inline int digit_base_of(DType t) { return t == kI8Digits254 ? 254 : 128; }
 // Bytes per pitch entry of a digit-plane column: the entries it replaced.
-__host__ __device__ inline int digit_column_bytes(DType t) { return t == kI8Digits5 ? 8 : 4; }
+__host__ __device__ inline int digit_column_bytes(DType t) { return t == kI8Digits254 ? 8 : 4; }
 
 // Derivative modes of AO evaluation.
 enum AoMode : int { kValues = 0, kGradient = 1, kValuesGradient = 2 };

@@ -9,7 +9,9 @@
 // which rounds each entry to within 2^-(7 NY + 1) s_j: with three planes 3e-5
 // to 6e-5 of ||y_j|| in RMS over 13,200 rows (s_j < 4 ||y_j||; well below the
 // rounding noise of a BF16 or FP8 sketch), with four ~2e-7 (below that of a
-// split-BF16 sketch).
+// split-BF16 sketch).  An FP64 sketch is held as four base-254 planes
+// (digitize_columns_f64, |d_r| <= 127, within s_j 254^-4 / 2 ~ 1.2e-10 s_j:
+// the FP64 sketch's pivots), in place over its FP64 entries.
 // The QRCP then factorises y_hat: candidate columns are rebuilt from the same
 // digits, and ||y_hat_j||^2 replaces ||y_j||^2.
 //
@@ -21,7 +23,9 @@
 // accumulator (at most four pairs: |sum| < 4 * 64^2 * m < 2^31 for m < 2^16),
 // and the six accumulators are combined in FP64:
 //     W = u_t s_j sum_g 128^-(g+2) acc_g.
-// The dropped pairs (p + r >= 6) and the truncation of q leave ~1e-11 of
+// In base 254 the 14 pairs p + r <= 4 run in five groups (at most four
+// pairs: |sum| < 4 * 127^2 * m < 2^31 for m < 2^15).
+// The dropped pairs (p + r >= 6 in base 128, >= 5 in base 254) and the truncation of q leave ~1e-11 of
 // ||y_j||^2 in the folded sums (measured), covered by the QRCP's relative
 // margin.
 //
@@ -39,8 +43,21 @@ namespace {
 
 typedef int i4v __attribute__((ext_vector_type(4)));
 
-constexpr int ND = 5;            // digits of q
-constexpr int NG = 6;            // accumulator groups, p + r = 0..5
+// Digit schemes.  Base 128 (|d| <= 64): q in five digits, Y in three or four
+// planes, the pairs p + r <= 5 in six accumulator groups.  Base 254
+// (|d| <= 127): q in five digits, Y in four planes (an FP64 sketch), the
+// pairs p + r <= 4 (14) in five groups; the dropped pairs are 254^-7 of
+// u_t s_j.
+template <int BASE>
+struct DigitScheme;
+template <>
+struct DigitScheme<128> {
+    static constexpr int ND = 5, NG = 6;
+};
+template <>
+struct DigitScheme<254> {
+    static constexpr int ND = 5, NG = 5;
+};
 constexpr int FBM = 128;         // rows per workgroup
 constexpr int FBN = 64;          // columns per workgroup (two tiles of 32)
 constexpr int FBK = 64;          // K bytes per slice (two MFMA K-steps)
@@ -67,9 +84,11 @@ constexpr int kBufferWord3 = 0x00020000;
 // Q digits, slice-major: digit p of q_t[k] / u_t at
 // qd[((k / 64 * ND + p) * bpad + t) * 64 + k % 64] (one slice of every plane
 // and row is 64 ND bpad contiguous bytes), qs[t] = u_t.
+template <int BASE>
 __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int64_t m, int64_t b, int64_t ldq,
                                int64_t bpad, int8_t* __restrict__ qd, double* __restrict__ qs)
 {
+    constexpr int ND = DigitScheme<BASE>::ND;
     __shared__ double red[256];
     const int64_t t = blockIdx.x;
     double mx = 0;
@@ -90,7 +109,7 @@ __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int6
         double x = (t < b && k < m) ? qb[k + t * ldqb] * inv : 0.0;  // |x| < 1/2, exact scaling
 #pragma unroll
         for (int p = 0; p < ND; ++p) {
-            x *= 128.0;
+            x *= (double)BASE;
             const double dgt = rint(x);
             qd[(((k / FBK) * ND + p) * bpad + t) * FBK + k % FBK] = (int8_t)dgt;
             x -= dgt;  // exact: |x - dgt| <= 1/2
@@ -163,16 +182,16 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_kernel(floa
     }
 }
 
-// Five planes in place of FP64 column j (pitch 8 P bytes): as above with
-// the digits taken from the FP64 entries, y_hat within 2^-36 s_j of y; the
-// column's bytes past its 5 P digit bytes are zeroed.
+// Four base-254 planes (|d| <= 127) in place of FP64 column j (pitch 8 P
+// bytes): y_hat[i] = s_j sum_r d_r 254^-(r+1), within s_j 254^-4 / 2
+// (~1.2e-10 s_j) of y; the column's bytes past its 4 P digit bytes are zeroed.
 constexpr int kDigitizeEntriesF64 = (int)(kDigitRowsMaxF64 / kDigitizeThreads);  // P <= 16384
 __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(double* __restrict__ y, int64_t m,
                                                                                 int64_t P,
                                                                                 double* __restrict__ ynorm2,
                                                                                 double* __restrict__ scale)
 {
-    constexpr int NY = 5;
+    constexpr int NY = 4;
     __shared__ double part[kDigitizeThreads / 64];
     const int64_t j = blockIdx.x;
     double* src = y + j * P;  // 8 P bytes
@@ -195,16 +214,18 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(
         const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
         if (i < P) {
             double v = x[e] * inv;  // exact: power-of-two scaling, |v| < 1/2
-            double yh = 0, w = 1.0;
+            double dg[NY];
 #pragma unroll
             for (int r = 0; r < NY; ++r) {
-                v *= 128.0;
-                const double dgt = rint(v);
-                v -= dgt;  // exact
-                planes[digit_offset(i, r, NY)] = (int8_t)dgt;
-                w *= 1.0 / 128.0;
-                yh += dgt * w;
+                v *= 254.0;  // |v| <= 127
+                dg[r] = rint(v);
+                v -= dg[r];  // exact
+                planes[digit_offset(i, r, NY)] = (int8_t)dg[r];
             }
+            // y_hat as the QRCP rebuilds it (Horner, lowest digit first)
+            double yh = 0;
+#pragma unroll
+            for (int r = NY - 1; r >= 0; --r) yh = (yh + dg[r]) * (1.0 / 254.0);
 #pragma unroll
             for (int r = NY; r < 8; ++r) planes[(int64_t)r * P + i] = 0;  // the column's unused last bytes
             yh *= sc;
@@ -236,7 +257,7 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // lane reads 32 contiguous bytes of a plane per slice.  Rows past b carry
 // zero digits; there are no branches in the MFMA sequence (a branch would
 // make the compiler drain every outstanding load at its join).
-template <bool ENERGY, int NR, int NY>
+template <bool ENERGY, int NR, int NY, int BASE>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
                                                                    int64_t bpad, int64_t b, int64_t m0,
@@ -246,6 +267,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                                                                    int64_t n, double* __restrict__ proj2,
                                                                    double* __restrict__ energy)
 {
+    constexpr int ND = DigitScheme<BASE>::ND, NG = DigitScheme<BASE>::NG;
     constexpr int ROWS = 32 * NR;  // Q rows staged per slice
     constexpr int NTHREADS = 128 * NR;
     using L = SliceLayout<NR>;
@@ -384,16 +406,16 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 #undef MPISDF_LOAD_Q
 #undef MPISDF_STASH_Q
 
-    // W[t][j] = u_t s_j sum_g 128^-(g+2) acc_g.
+    // W[t][j] = u_t s_j sum_g BASE^-(g+2) acc_g.
     double colsq = 0;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
         double v = 0;
 #pragma unroll
-        for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / 128.0) + (double)acc[g][e];
+        for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / BASE) + (double)acc[g][e];
         const int lrow = rt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
         const int64_t row = m0 + lrow;
-        const double wv = row < b ? v * (1.0 / 16384.0) * qs[row] * sj : 0.0;
+        const double wv = row < b ? v * (1.0 / ((double)BASE * BASE)) * qs[row] * sj : 0.0;
         colsq += wv * wv;
         if (ENERGY) {
             double s = wv * wv;
@@ -425,7 +447,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 // rows: Q^T rows of this launch's group (1..128).  Groups of more than 64
 // rows run on 8-wave workgroups (NR = 4, one per CU); smaller groups on
 // 4-wave workgroups (NR = 2, two per CU), which multiply no zero rows.
-template <int NY>
+template <int NY, int BASE>
 void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs,
                     int64_t ldq, int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P, int colbytes,
                     const double* yscale, int64_t m, int64_t n, double* proj2, double* en)
@@ -433,7 +455,7 @@ void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const i
     if (rows <= 0 || rows > FBM) throw std::logic_error("INT8 fold row group out of range");
     double* const e = energy ? en : nullptr;
 #define MPISDF_FOLD(E, NR)                                                                                    \
-    hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR, NY>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
+    hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR, NY, BASE>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
                        colbytes, yscale, m, n, proj2, e)
     if (rows > 64) {
         if (energy) MPISDF_FOLD(true, 4);
@@ -479,23 +501,30 @@ void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, dou
 }
 
 void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const int8_t* Yd,
-                         int64_t P, int planes, int colbytes, const double* yscale, double* proj2, double* energy,
-                         FoldI8Scratch& scratch, hipStream_t stream)
+                         int64_t P, int planes, int base, int colbytes, const double* yscale, double* proj2,
+                         double* energy, FoldI8Scratch& scratch, hipStream_t stream)
 {
-    if (planes < 3 || planes > 5) throw std::invalid_argument("digit planes must be 3, 4 or 5");
-    if ((colbytes != 4 && colbytes != 8) || planes > colbytes)
+    if (base == 128 ? (planes != 3 && planes != 4) : (base != 254 || planes != 4))
+        throw std::invalid_argument("digit planes must be 3 or 4 of base 128, or 4 of base 254");
+    if (colbytes != 4 && colbytes != 8)
         throw std::invalid_argument("digit columns take the 4 or 8 bytes per entry of the entries they replace");
+    if (base == 254 && m > 32767) throw std::invalid_argument("the base-254 fold needs at most 32767 sketch rows");
     if (b <= 0 || n <= 0) return;
     if (m >= (int64_t(1) << 16)) throw std::invalid_argument("the INT8 fold needs fewer than 65536 sketch rows");
     if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     const int64_t bpad = (b + FBM - 1) / FBM * FBM;
     const int64_t ldq = (m + FBK - 1) / FBK * FBK;
-    const size_t need = (size_t)(ND * bpad * ldq);
+    const int nd = base == 254 ? DigitScheme<254>::ND : DigitScheme<128>::ND;
+    const size_t need = (size_t)(nd * bpad * ldq);
     if (need >= (size_t(1) << 31)) throw std::invalid_argument("the INT8 fold's Q digits exceed 2 GiB");
     if (scratch.digits.count < need) scratch.digits.resize(need);
     if (scratch.scales.count < (size_t)bpad) scratch.scales.resize(bpad);
-    hipLaunchKernelGGL(split_q_kernel, dim3((unsigned)bpad), dim3(256), 0, stream, Qb, ldqb, m, b, ldq, bpad,
-                       scratch.digits.ptr, scratch.scales.ptr);
+    if (base == 254)
+        hipLaunchKernelGGL(split_q_kernel<254>, dim3((unsigned)bpad), dim3(256), 0, stream, Qb, ldqb, m, b, ldq, bpad,
+                           scratch.digits.ptr, scratch.scales.ptr);
+    else
+        hipLaunchKernelGGL(split_q_kernel<128>, dim3((unsigned)bpad), dim3(256), 0, stream, Qb, ldqb, m, b, ldq, bpad,
+                           scratch.digits.ptr, scratch.scales.ptr);
     MPISDF_HIP(hipGetLastError());
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
@@ -505,16 +534,16 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     // block order.
     for (int64_t m0 = 0; m0 < b; m0 += FBM) {
         const int64_t rows = std::min<int64_t>(FBM, b - m0);
-        if (planes == 5)
-            launch_fold_i8<5>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
-                              scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
-                              scratch.part.ptr);
+        if (base == 254)
+            launch_fold_i8<4, 254>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
+                                   scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
+                                   scratch.part.ptr);
         else if (planes == 4)
-            launch_fold_i8<4>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
+            launch_fold_i8<4, 128>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
                               scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
                               scratch.part.ptr);
         else
-            launch_fold_i8<3>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
+            launch_fold_i8<3, 128>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
                               scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
                               scratch.part.ptr);
         if (energy) launch_sum_partials(scratch.part.ptr, gx, rows, energy + m0, true, stream);

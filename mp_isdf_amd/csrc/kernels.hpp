@@ -107,9 +107,10 @@ void fold_projections_bf16x3(int64_t b, int64_t n, int64_t m, const double* Qb, 
 void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, double* ynorm2, double* scale,
                       hipStream_t stream);
 // The same for an FP64 sketch (column pitch 8 P bytes, m <= P <= 16384):
-// five planes, y_hat rounds each entry to 2^-36 s_j (~1e-11 of the column
-// norm), so its greedy pivots are the FP64 sketch's; ynorm2[j] (in: of the
-// FP64 column, out: of y_hat).
+// four signed base-254 planes (|d| <= 127), y_hat = scale[j] sum_r d_r
+// 254^-(r+1) within scale[j] 254^-4 / 2 (~1e-10 s_j) of each entry, so its
+// greedy pivots are the FP64 sketch's; ynorm2[j] (in: of the FP64 column,
+// out: of y_hat).
 void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
                           hipStream_t stream);
 constexpr int64_t kDigitRowsMaxF64 = 16384;  // digitize_columns_f64 holds one FP64 column in registers
@@ -121,7 +122,7 @@ constexpr int64_t kDigitSlice = 64;       // entries per slice; P is a multiple 
 // sketch (~1e-7) intact: its greedy pivots then agree with the FP64 form's on
 // >= 98% of the anchors, where three digits alone drop the agreement to ~93%.
 constexpr int kDigitPlanes = 3;
-constexpr int kDigitPlanesMax = 5;
+constexpr int kDigitPlanesMax = 4;
 // Byte offset, within its column's 4 P bytes, of digit r of entry i: each
 // slice of 64 entries keeps its digit planes in one run of 64 planes bytes and
 // the runs follow each other, so a column's digits are its first planes * P
@@ -141,10 +142,11 @@ struct FoldI8Scratch {
 // Relative bound (of ||y_j||^2) on the error a folded sum accumulates over a
 // whole factorisation with the INT8 fold; enters the keys as a margin.
 constexpr double kFoldI8Margin = 1e-9;
+// base: 128 (three or four planes) or 254 (four planes, m <= 32767);
 // colbytes: bytes per pitch entry of a digit column (4 for planes held in
 // place of FP32 entries, 8 in place of FP64 ones).
 void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const int8_t* Yd,
-                         int64_t P, int planes, int colbytes, const double* yscale, double* proj2, double* energy,
-                         FoldI8Scratch& scratch, hipStream_t stream);
+                         int64_t P, int planes, int base, int colbytes, const double* yscale, double* proj2,
+                         double* energy, FoldI8Scratch& scratch, hipStream_t stream);
 
 }  // namespace mpisdf

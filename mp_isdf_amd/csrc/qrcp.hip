@@ -86,23 +86,27 @@ __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __r
     }
 }
 
-// y_hat_j[i] = scale[j] sum_{r < planes} d_r[i] 128^-(r+1), exact in FP64;
-// colp is column j's first byte.
-__device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t i, double scale, int planes)
+// y_hat_j[i] = scale[j] sum_{r < planes} d_r[i] base^-(r+1) by Horner's rule
+// (exact in FP64 for base 128); colp is column j's first byte.
+__device__ __forceinline__ double digit_value(const int8_t* __restrict__ colp, int64_t i, double scale, int planes,
+                                              double inv_base)
 {
     double v = 0;
-    for (int r = planes - 1; r >= 0; --r) v = (v + (double)colp[digit_offset(i, r, planes)]) * (1.0 / 128.0);
+    for (int r = planes - 1; r >= 0; --r) v = (v + (double)colp[digit_offset(i, r, planes)]) * inv_base;
     return v * scale;
 }
 
-__global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, int planes, int colbytes,
+__global__ void widen_digits_kernel(const int8_t* __restrict__ yd, int64_t P, int ytype,
                                     const double* __restrict__ yscale, int64_t j0, int64_t m, int64_t nc,
                                     double* __restrict__ out)
 {
+    const DType t = (DType)ytype;
+    const int planes = digit_planes_of(t), colbytes = digit_column_bytes(t);
+    const double inv = 1.0 / digit_base_of(t);
     const int64_t total = m * nc;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t j = i / m, r = i - j * m;
-        out[i] = digit_value(yd + (j0 + j) * colbytes * P, r, yscale[j0 + j], planes);
+        out[i] = digit_value(yd + (j0 + j) * colbytes * P, r, yscale[j0 + j], planes, inv);
     }
 }
 
@@ -189,7 +193,7 @@ __global__ void fill_candidates_kernel(const void* __restrict__ y, int ytype, in
         else if (ytype == kF32) v = (double)static_cast<const float*>(y)[i + j * ldy];
         else
             v = digit_value(static_cast<const int8_t*>(y) + j * digit_column_bytes((DType)ytype) * ldy, i, yscale[j],
-                            digit_planes_of((DType)ytype));
+                            digit_planes_of((DType)ytype), 1.0 / digit_base_of((DType)ytype));
         rc[i + c * ldr] = v;
     }
 }
@@ -624,7 +628,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         res.fold_flops += 2.0 * (double)b * (double)m * (double)n;
         if (digits)
             fold_projections_i8(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const int8_t*>(y), ldy, planes,
-                                digit_column_bytes(ytype), opt.yscale, proj2.ptr,
+                                digit_base_of(ytype), digit_column_bytes(ytype), opt.yscale, proj2.ptr,
                                 energy_out ? energy_out + folded : nullptr, i8_scratch, s);
         else if (split)
             fold_projections_bf16x3(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const float*>(y), ldy, proj2.ptr,
@@ -796,8 +800,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             const int64_t cnt = std::min(nc, n - j0);
             if (digits)
                 hipLaunchKernelGGL(widen_digits_kernel, dim3(grid_for(m * cnt)), dim3(256), 0, s,
-                                   static_cast<const int8_t*>(y), ldy, planes, digit_column_bytes(ytype), opt.yscale,
-                                   j0, m, cnt, R.ptr);
+                                   static_cast<const int8_t*>(y), ldy, (int)ytype, opt.yscale, j0, m, cnt, R.ptr);
             else if (ytype == kF64)
                 hipLaunchKernelGGL(widen_kernel<double>, dim3(grid_for(m * cnt)), dim3(256), 0, s,
                                    static_cast<const double*>(y) + j0 * ldy, ldy, m, cnt, R.ptr);

@@ -301,8 +301,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     if (a.precision == kPrecF32)
         throw std::invalid_argument("precision 'fp32' is not provided; use fp64, bf16x3, bf16 or fp8");
     const bool lowp = a.precision != kPrecF64 && a.precision != kPrecF64I8;
-    // The FP64 sketch on five digit planes: pitch a multiple of 64, ell within
-    // the FP64 digitiser's register budget.
+    // The FP64 sketch on four base-254 digit planes: pitch a multiple of 64,
+    // ell within the FP64 digitiser's register budget.
     const bool digits5 = a.precision == kPrecF64I8;
     if (digits5 && ell > kDigitRowsMaxF64)
         throw std::invalid_argument("precision 'fp64i8' supports sketches of at most 16384 rows");
@@ -453,7 +453,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         opt.margin = kFoldI8Margin;
     }
     const QrcpResult r = use_digits ? qrcp_device(y32.ptr, planes == 4 ? kI8Digits4 : kI8Digits, ell, n, ldy32, opt, s)
-                         : digits5  ? qrcp_device(y64.ptr, kI8Digits5, ell, n, ldy64, opt, s)
+                         : digits5  ? qrcp_device(y64.ptr, kI8Digits254, ell, n, ldy64, opt, s)
                          : lowp     ? qrcp_device(y32.ptr, kF32, ell, n, ldy32, opt, s)
                                     : qrcp_device(y64.ptr, kF64, ell, n, ldy64, opt, s);
     const double t_qrcp = seconds_since(tq);

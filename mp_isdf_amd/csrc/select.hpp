@@ -9,9 +9,9 @@ namespace mpisdf {
 
 // LocalAnchorSelectionAlgorithm values (cpp/include/enums.hpp:23-30); 3 is CVT.
 enum Algorithm : int { kDense = 0, kOnline = 1, kRandomized = 2, kDensity = 4, kDensityNormalized = 5 };
-// kPrecF64I8: the FP64 sketch, factorised as five INT8 digit planes (the
-// INT8 fold in place of the FP64 one; the same pivots as kPrecF64 unless two
-// candidates tie to ~1e-11).
+// kPrecF64I8: the FP64 sketch, factorised as four base-254 INT8 digit planes
+// (the INT8 fold in place of the FP64 one; the same pivots as kPrecF64 unless
+// two candidates tie to ~1e-10).
 enum Precision : int { kPrecF64 = 0, kPrecF32 = 1, kPrecBF16 = 2, kPrecFP8 = 3, kPrecBF16x3 = 4, kPrecF64I8 = 5 };
 
 struct SelectArgs {
@@ -64,7 +64,7 @@ enum Diag : int {
     kDiagSketchSeconds = 17,  // summed duration of the sketch kernel launches (HIP events)
     kDiagSketchFlops = 18,    // their MFMA flops: sum 4 ell d count per product term (3 split BF16, 2 FP8)
     kDiagSketchLaunches = 19,
-    kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 / 3 / 4 INT8 MFMA on 3 / 4 / 5 planes
+    kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 / 3 INT8 MFMA on 3 / 4 base-128 planes, 4 on 4 base-254 planes
     // Audit (SelectArgs::audit): relative projection residuals of the selected
     // and of the dense QRCP pivots, quadrature columns sqrt|w| pair_g, then,
     // for the density sketches, the sketch's own columns; -1 when not run.
