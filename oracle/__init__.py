@@ -52,6 +52,7 @@ def lib():
         _lib = ctypes.CDLL(str(LIB_PATH))
         _lib.oracle_last_error.restype = ctypes.c_char_p
         _lib.oracle_select_last_error.restype = ctypes.c_char_p
+        _lib.oracle_eri_last_error.restype = ctypes.c_char_p
     return _lib
 
 
@@ -62,7 +63,10 @@ def _p(a):
 def _check(rc, selection=False):
     if rc == 0:
         return
-    msg = (lib().oracle_select_last_error() if selection else lib().oracle_last_error()).decode()
+    if selection == "eri":
+        msg = lib().oracle_eri_last_error().decode()
+    else:
+        msg = (lib().oracle_select_last_error() if selection else lib().oracle_last_error()).decode()
     if rc == 1:
         raise ValueError(msg)
     if rc == 2:
@@ -633,4 +637,83 @@ def rbrp_ao(ao, weights, batch_size=128, block_size=16, tolerance=1e-10, max_ran
     transform[:, ju, iu] = packed
     out["interp_vecs"] = theta
     out["transform"] = transform.reshape(k, nao * nao)
+    return out
+
+
+# ---- integral-direct Cholesky Coulomb metric (eri_oracle.cpp) -------------
+
+_CHOLESKY_ALGORITHMS = {"unblocked_greedy": 0, "unblocked_uniform": 1}
+
+
+def _table_args(atm, bas, env, cart):
+    atm, bas, env = _tables(atm, bas, env)
+    keep = (atm, bas, env)
+    return keep, (_p(atm), ctypes.c_int(len(atm)), _p(bas), ctypes.c_int(len(bas)), _p(env),
+                  ctypes.c_int64(len(env)), ctypes.c_int(int(cart)))
+
+
+def eri_prim(powers, exps, centres) -> float:
+    """Primitive cartesian ERI (ab|cd) of unnormalised Gaussians
+    (x - A_x)^{a_x} (y - A_y)^{a_y} (z - A_z)^{a_z} exp(-alpha |r - A|^2):
+    powers (4, 3), exps (4,), centres (4, 3)."""
+    pw = np.ascontiguousarray(powers, dtype=np.int32).reshape(12)
+    ex = np.ascontiguousarray(exps, dtype=np.float64).reshape(4)
+    xyz = np.ascontiguousarray(centres, dtype=np.float64).reshape(12)
+    out = ctypes.c_double()
+    _check(lib().oracle_eri_prim(_p(pw), _p(ex), _p(xyz), ctypes.byref(out)), "eri")
+    return out.value
+
+
+def boys(nmax: int, t: float) -> np.ndarray:
+    out = np.empty(nmax + 1)
+    _check(lib().oracle_boys(ctypes.c_int(nmax), ctypes.c_double(t), _p(out)), "eri")
+    return out
+
+
+def eri_packed(atm, bas, env, cart=False) -> np.ndarray:
+    """Packed ERI matrix (mu nu | la si) over mu >= nu, la >= si, pair index mu (mu + 1) / 2 + nu."""
+    keep, args = _table_args(atm, bas, env, cart)
+    m = nao(atm, bas, env, cart)
+    npair = m * (m + 1) // 2
+    out = np.zeros((npair, npair))
+    _check(lib().oracle_eri_packed(*args, _p(out)), "eri")
+    return out
+
+
+def cholesky_eri(atm, bas, env, tolerance=1e-12, max_rank=0, algorithm="unblocked_greedy", seed=0, cart=False):
+    """(factors[rank, npair], pivots, residual[npair]) as mp_isdf.cholesky_eri returns them
+    (cpp/src/cholesky_eri_greedy.cpp:53-175)."""
+    if algorithm not in _CHOLESKY_ALGORITHMS:
+        raise ValueError("unknown Cholesky algorithm")
+    keep, args = _table_args(atm, bas, env, cart)
+    m = nao(atm, bas, env, cart)
+    npair = m * (m + 1) // 2
+    capacity = max_rank if 0 < max_rank <= npair else npair
+    factors = np.zeros((capacity, npair))
+    pivots = np.zeros(capacity, dtype=np.int64)
+    residual = np.zeros(npair)
+    rank = ctypes.c_int64()
+    _check(lib().oracle_cholesky_eri(*args, ctypes.c_double(tolerance), ctypes.c_int64(max_rank),
+                                     ctypes.c_int(_CHOLESKY_ALGORITHMS[algorithm]), ctypes.c_uint64(seed),
+                                     ctypes.c_int64(capacity), _p(factors), _p(pivots), _p(residual),
+                                     ctypes.byref(rank)), "eri")
+    r = rank.value
+    return factors[:r].copy(), pivots[:r].copy(), residual
+
+
+def coulomb_metric(atm, bas, env, transform, algorithm="unblocked_greedy", tolerance=1e-12, max_rank=0,
+                   batch_size=64, seed=0, cart=False):
+    """transform[nvec, nao * nao] -> (nvec, nvec) (cpp/src/coulomb_metric.cpp:20-98)."""
+    if algorithm not in _CHOLESKY_ALGORITHMS:
+        raise ValueError("unknown Cholesky algorithm")
+    keep, args = _table_args(atm, bas, env, cart)
+    m = nao(atm, bas, env, cart)
+    t = np.ascontiguousarray(transform, dtype=np.float64)
+    if t.ndim != 2 or t.shape[1] != m * m:
+        raise ValueError("expected transform[nvec,nao*nao]")
+    out = np.zeros((t.shape[0], t.shape[0]))
+    _check(lib().oracle_coulomb_metric(*args, _p(t), ctypes.c_int64(t.shape[0]),
+                                       ctypes.c_int(_CHOLESKY_ALGORITHMS[algorithm]), ctypes.c_double(tolerance),
+                                       ctypes.c_int64(max_rank), ctypes.c_int64(batch_size), ctypes.c_uint64(seed),
+                                       _p(out)), "eri")
     return out
