@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 SOURCES = [
     "common.cpp", "context.cpp", "axis.cpp", "select.cpp", "online.cpp", "capi.cpp", "grid.cpp", "lebedev.cpp", "overlap.cpp", "rbrp.cpp",
-    "ao_eval.hip", "gemm.hip", "gemm_tn.hip", "qrcp.hip", "sketch.hip", "util.hip", "online_kernels.hip", "sketch_mfma.hip", "fold_bf16x3.hip", "fold_i8.hip", "fit.hip", "coulomb.hip", "cvt.hip", "audit.hip",
+    "ao_eval.hip", "gemm.hip", "gemm_tn.hip", "qrcp.hip", "sketch.hip", "util.hip", "online_kernels.hip", "sketch_mfma.hip", "fold_bf16x3.hip", "fold_i8.hip", "fit.hip", "coulomb.hip", "cvt.hip", "audit.hip", "eri.hip",
 ]
 
 
