@@ -26,6 +26,7 @@ a bounded sample).
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -49,6 +50,12 @@ PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops":
 # sketch, q into 5 and y into 4 base-254 digits, the 14 pairs p + r <= 4.
 # Keyed by the selection's fold_kind.
 I8_PAIRS = {2: 14, 3: 17, 4: 14}
+# Digit pairs p + r <= 4 of five base-254 planes on each operand (the fp64i8
+# sketch and the digit form of the fit's products).
+DIGIT_PAIRS_5 = 15
+# Arithmetic of the fit's Z = L^-1 S product per form: 0 FP64 MFMA, 1 base-254
+# INT8 digits of the FP64 operands (every form but the all-FP64 one).
+FIT_FORM = {"fp64": 0}
 FIT_BATCH = 32768  # grid points per fit batch
 EQUAL_PAIR_ERROR = 1e-4  # |relative pair-space error - FP64 form's|
 EQUAL_OVERLAP = 0.98     # fraction of the FP64 form's anchors
@@ -128,16 +135,17 @@ def main():
     xyz_dev = torch.tensor(xyz[keep], device="cuda")
     w_dev = torch.tensor(w[keep], device="cuda")
 
-    def fit(anchors):
-        """FP64 fit on the kept grid: Gram, Cholesky and L^-1 of the anchor pairs,
-        then the grid pass (S = (Phi_a Phi^T)^2, Z = L^-1 S) with the pair-space
-        error and theta . w = L^-T (sum_g w_g z_g) / w_anchor."""
+    def fit(anchors, form):
+        """The fit on the kept grid: Gram, Cholesky and L^-1 of the anchor pairs
+        in FP64, then the grid pass (S = (Phi_a Phi^T)^2 on FP64 MFMA, Z = L^-1 S
+        on FP64 MFMA for form 0 or on base-254 INT8 digits of S and L^-1 for
+        form 1) with the pair-space error and theta . w."""
         ax = torch.tensor(xyz[anchors], device="cuda")
         aw = np.ascontiguousarray(w[anchors])
-        res = np.zeros(10)
+        res = np.zeros(12)
         tsum = np.zeros(len(anchors))
-        capi.call("mpisdf_isdf_fit_device", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev, n, FIT_BATCH, tsum, res,
-                  None)
+        capi.call("mpisdf_isdf_fit_device_form", cctx.handle, ax, aw, len(anchors), xyz_dev, w_dev, n, FIT_BATCH,
+                  ctypes.c_int32(form), tsum, res, None)
         return res
 
     def select(precision, points=None):
@@ -148,13 +156,15 @@ def main():
         t = time.perf_counter()
         anchors, d = select(precision)
         t_sel = time.perf_counter() - t
-        res = fit(anchors)
+        res = fit(anchors, FIT_FORM.get(precision, 1))
         d = dict(d)
         d["seconds_select"] = t_sel
         d["seconds_fit_factor"] = res[4]
         d["seconds_fit_grid"] = res[5]
         d["fit_gemm_seconds"] = res[6]
         d["fit_gemm_flops"] = res[7]
+        d["fit_z_seconds"] = res[10]
+        d["fit_z_launches"] = res[11]
         d["pair_space_relative_error"] = float(res[1])
         d["gram_condition_estimate"] = float(res[2])
         return anchors, d
@@ -284,30 +294,87 @@ def main():
                          "frac": round(fold_f / fold_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if fold_s else None,
                          "launches": fold_n, "launch_ms": round(fold_s / fold_n * 1e3, 3),
                          "flops_per_launch": fold_f / fold_n, "work": fold_work}
-    fit_s, fit_f = d.get("fit_gemm_seconds", 0.0), d.get("fit_gemm_flops", 0.0)
-    fit_n = 2 * ((n + FIT_BATCH - 1) // FIT_BATCH)
-    roofline_fit = {"kernel": "tn_gemm_kernel (fit grid pass: S = (Phi_a Phi^T)^2, Z = L^-1 S)",
-                    "bound": "mfma", "achieved": round(fit_f / fit_s / 1e12, 2) if fit_s else None,
-                    "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
-                    "frac": round(fit_f / fit_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if fit_s else None,
-                    "launches": fit_n, "launch_ms": round(fit_s / fit_n * 1e3, 3), "flops_per_launch": fit_f / fit_n,
-                    "work": f"(2*k*d + k*(k+1)) per grid point, k={k}, d={mol.nao}, n={n}, {FIT_BATCH}-point batches"}
+    # The fit's grid pass: S on FP64 MFMA, then Z = L^-1 S (FP64 MFMA, or the
+    # INT8 fold on the base-254 digits of S and L^-1).
+    nbatch = (n + FIT_BATCH - 1) // FIT_BATCH
+    z_s, z_n = d.get("fit_z_seconds", 0.0), max(1, int(d.get("fit_z_launches", 0)))
+    s_s = d.get("fit_gemm_seconds", 0.0) - z_s
+    s_f, z_f = 2.0 * k * mol.nao * n, float(k) * (k + 1) * n
+    dpad = (mol.nao + 63) // 64 * 64
+    if FIT_FORM.get(value_form, 1) == 1:
+        s_ops = DIGIT_PAIRS_5 * 2.0 * k * dpad * n
+        roofline_fit_s = {"kernel": "fold_i8_32x32_kernel (fit S = (Phi_a Phi^T)^2 on base-254 INT8 digits)",
+                          "bound": "mfma", "achieved": round(s_ops / s_s / 1e12, 2) if s_s > 0 else None,
+                          "peak": PEAK["int8_mfma_tops"], "unit": "TOP/s",
+                          "frac": round(s_ops / s_s / 1e12 / PEAK["int8_mfma_tops"], 4) if s_s > 0 else None,
+                          "fp64_equivalent_tflops": round(s_f / s_s / 1e12, 2) if s_s > 0 else None,
+                          "launches": nbatch * ((k + 127) // 128), "launch_ms": None,
+                          "work": f"{DIGIT_PAIRS_5} INT8 digit-pair products of 2*k*d64 per grid point, k={k}, "
+                                  f"d64={dpad}, n={n}; FP64 equivalent 2*k*d"}
+        if s_s > 0:
+            roofline_fit_s["launch_ms"] = round(s_s / roofline_fit_s["launches"] * 1e3, 3)
+    else:
+        roofline_fit_s = {"kernel": "tn_gemm_kernel (fit S = (Phi_a Phi^T)^2)", "bound": "mfma",
+                          "achieved": round(s_f / s_s / 1e12, 2) if s_s > 0 else None,
+                          "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                          "frac": round(s_f / s_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if s_s > 0 else None,
+                          "launches": nbatch, "launch_ms": round(s_s / nbatch * 1e3, 3),
+                          "flops_per_launch": s_f / nbatch,
+                          "work": f"2*k*d per grid point, k={k}, d={mol.nao}, n={n}, {FIT_BATCH}-point batches"}
+    if FIT_FORM.get(value_form, 1) == 1:
+        # row groups of 128 rows of L^-1, each folding its first min(k, m0 + 128)
+        # entries (in 64-entry slices): the digit-pair work per grid point
+        kext = sum(min(128, k - m0) * (min(k, m0 + 128) + 63) // 64 * 64 for m0 in range(0, k, 128))
+        z_ops = DIGIT_PAIRS_5 * 2.0 * kext * n
+        roofline_fit = {"kernel": "fold_i8_32x32_kernel (fit Z = L^-1 S on base-254 INT8 digits)", "bound": "mfma",
+                        "achieved": round(z_ops / z_s / 1e12, 2) if z_s else None,
+                        "peak": PEAK["int8_mfma_tops"], "unit": "TOP/s",
+                        "frac": round(z_ops / z_s / 1e12 / PEAK["int8_mfma_tops"], 4) if z_s else None,
+                        "fp64_equivalent_tflops": round(z_f / z_s / 1e12, 2) if z_s else None,
+                        "launches": z_n, "launch_ms": round(z_s / z_n * 1e3, 3), "ops_per_launch": z_ops / z_n,
+                        "work": f"{DIGIT_PAIRS_5} INT8 digit-pair products of 2*sum_g rows_g*K_g = 2*{kext} per grid "
+                                f"point (triangular K of 128-row groups of L^-1), n={n}; FP64 equivalent "
+                                f"k*(k+1) per point, k={k}"}
+    else:
+        roofline_fit = {"kernel": "tn_gemm_kernel (fit Z = L^-1 S)", "bound": "mfma",
+                        "achieved": round(z_f / z_s / 1e12, 2) if z_s else None,
+                        "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                        "frac": round(z_f / z_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if z_s else None,
+                        "launches": z_n, "launch_ms": round(z_s / z_n * 1e3, 3), "flops_per_launch": z_f / z_n,
+                        "work": f"k*(k+1) per grid point, k={k}, n={n}, {FIT_BATCH}-point batches"}
+    fit_s = z_s
     sk_s, sk_f, sk_n = d.get("sketch_kernel_seconds", 0.0), d.get("sketch_flops", 0.0), max(1, int(d.get("sketch_launches", 0)))
-    terms = {"bf16x3": 3, "fp8": 2}.get(value_form, 1)
-    peak_sk = PEAK["fp64_mfma_tflops" if value_form in ("fp64", "fp64i8") else
-                   ("fp8_mfma_tflops" if value_form == "fp8" else "bf16_mfma_tflops")]
-    roofline_sketch = {"kernel": f"kr_sketch ({value_form})", "bound": "mfma",
-                       "achieved": round(sk_f / sk_s / 1e12, 2) if sk_s else None, "peak": peak_sk,
-                       "unit": "TFLOP/s", "frac": round(sk_f / sk_s / 1e12 / peak_sk, 4) if sk_s else None,
-                       "launches": sk_n, "launch_ms": round(sk_s / sk_n * 1e3, 3),
-                       "work": f"{terms} x 4*ell*d*n = {terms} x 4*{ell}*{mol.nao}*{n} over {sk_n} grid batches "
-                               f"({terms} product term{'s' if terms > 1 else ''} per entry)"}
+    if value_form == "fp64i8":
+        sk_ops = d.get("sketch_int8_ops", 0.0)
+        ell64 = (ell + 63) // 64 * 64
+        roofline_sketch = {"kernel": "fold_i8_32x32_kernel (Khatri-Rao sketch on base-254 INT8 digits)",
+                           "bound": "mfma", "achieved": round(sk_ops / sk_s / 1e12, 2) if sk_s else None,
+                           "peak": PEAK["int8_mfma_tops"], "unit": "TOP/s",
+                           "frac": round(sk_ops / sk_s / 1e12 / PEAK["int8_mfma_tops"], 4) if sk_s else None,
+                           "fp64_equivalent_tflops": round(sk_f / sk_s / 1e12, 2) if sk_s else None,
+                           "launches": sk_n * (2 * ell64 // 128), "launch_ms": None,
+                           "work": f"{DIGIT_PAIRS_5} INT8 digit-pair products of 2*(2*ell64)*d64 per grid point, "
+                                   f"ell64={ell64}, d64={dpad}, n={n} (the U and V rows stacked by 64); FP64 "
+                                   f"equivalent 4*ell*d*n"}
+        if sk_s:
+            roofline_sketch["launch_ms"] = round(sk_s / roofline_sketch["launches"] * 1e3, 3)
+    else:
+        terms = {"bf16x3": 3, "fp8": 2}.get(value_form, 1)
+        peak_sk = PEAK["fp64_mfma_tflops" if value_form == "fp64" else
+                       ("fp8_mfma_tflops" if value_form == "fp8" else "bf16_mfma_tflops")]
+        roofline_sketch = {"kernel": f"kr_sketch ({value_form})", "bound": "mfma",
+                           "achieved": round(sk_f / sk_s / 1e12, 2) if sk_s else None, "peak": peak_sk,
+                           "unit": "TFLOP/s", "frac": round(sk_f / sk_s / 1e12 / peak_sk, 4) if sk_s else None,
+                           "launches": sk_n, "launch_ms": round(sk_s / sk_n * 1e3, 3),
+                           "work": f"{terms} x 4*ell*d*n = {terms} x 4*{ell}*{mol.nao}*{n} over {sk_n} grid batches "
+                                   f"({terms} product term{'s' if terms > 1 else ''} per entry)"}
     # HBM bytes per launch from the committed counter passes (FETCH_SIZE and
     # WRITE_SIZE, calibrated on a known-size copy) when they cover this workload.
     traffic_file = ROOT / "profiles" / "hbm_traffic_h2o64_rand_sketch.json"
     if traffic_file.exists() and args.config == "h2o64":
         measured = json.loads(traffic_file.read_text())
-        for key, entry in (("qrcp_fold", roofline_fold), ("fit_grid_gemm", roofline_fit), ("sketch", roofline_sketch)):
+        for key, entry in (("qrcp_fold", roofline_fold), ("fit_grid_gemm", roofline_fit), ("fit_s", roofline_fit_s),
+                           ("sketch", roofline_sketch)):
             rec = measured.get(value_form, {}).get(key, {})
             tag = rec.get("kernel_tag")
             if tag and tag in entry["kernel"]:
@@ -316,10 +383,11 @@ def main():
                 entry["traffic_source"] = (f"{traffic_file.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
                                            f"algorithmic {rec['algorithmic_bytes_per_launch'] / 1e9:.3f} "
                                            "GB per launch)")
-    for entry in (roofline_fold, roofline_fit, roofline_sketch):
+    for entry in (roofline_fold, roofline_fit, roofline_fit_s, roofline_sketch):
         entry.setdefault("traffic", None)
     # The dominant kernel is the one with the largest summed duration in the step.
-    dominant = max(((fold_s, roofline_fold), (fit_s, roofline_fit), (sk_s, roofline_sketch)), key=lambda t: t[0])
+    dominant = max(((fold_s, roofline_fold), (fit_s, roofline_fit), (s_s, roofline_fit_s), (sk_s, roofline_sketch)),
+                   key=lambda t: t[0])
     roofline = dict(dominant[1], seconds_in_step=round(dominant[0], 3))
 
     # ---- AO evaluation on the grid (HBM bound) --------------------------------
@@ -443,8 +511,9 @@ def main():
         "dtype": value_form, "data": "synthetic: seeded water cluster, library Becke grid, seed-7 Gaussian sketch",
         "config": {"workload": f"(H2O){nwater}/{basis} level-{level} grid rand_sketch k={k}",
                    "grid_points": n, "nao": mol.nao, "anchors": k, "sketch_rows": ell,
-                   "step": "selection + FP64 fit (anchor-pair Cholesky; grid pass giving the pair-space error "
-                           "and theta . w)",
+                   "step": "selection + fit (FP64 anchor-pair Cholesky; grid pass giving the pair-space error "
+                           "and theta . w, its Z = L^-1 S product on base-254 INT8 digits of the FP64 operands "
+                           "for every form but fp64)",
                    "precision": value_form},
         "equal_fit_error": {"criteria": f"|relative pair-space error - FP64 form's| <= {EQUAL_PAIR_ERROR} and "
                                         f">= {EQUAL_OVERLAP:.0%} of the FP64 form's anchors",
@@ -460,6 +529,7 @@ def main():
         "roofline": roofline,
         "roofline_fold": dict(roofline_fold, seconds_in_step=round(fold_s, 3)),
         "roofline_fit": dict(roofline_fit, seconds_in_step=round(fit_s, 3)),
+        "roofline_fit_s": dict(roofline_fit_s, seconds_in_step=round(s_s, 3)),
         "roofline_sketch": dict(roofline_sketch, seconds_in_step=round(sk_s, 3)),
         "roofline_ao": roofline_ao,
         "cpu_baseline": cpu,

@@ -38,6 +38,11 @@ int mpisdf_device_count(int* count);
 int mpisdf_context_create(const int32_t* atm, int64_t natm, const int32_t* bas, int64_t nbas,
                           const double* env, int64_t nenv, int cart, mpisdf_context** out);
 int mpisdf_context_destroy(mpisdf_context* ctx);
+/* The randomized selectors keep their sketch storage (up to ell * n * 8
+ * bytes) in the context between calls, so that repeated selections do not
+ * free and reallocate it; this gives it back to the device (no reference
+ * counterpart: the reference allocates per call). */
+int mpisdf_context_release_workspace(mpisdf_context* ctx);
 /* LibcintContext.nao (cpp/src/ao_eval_pybind.cpp:91). */
 int mpisdf_context_nao(const mpisdf_context* ctx, int64_t* nao);
 
@@ -281,6 +286,21 @@ int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* anchor_xyz_d
                            const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* tsum,
                            double* results, void* stream);
 
+/* The fit of mpisdf_isdf_fit_device with a choice of arithmetic for the
+ * grid pass's Z = L^-1 (Phi_a phi_g)^2 product, the same refit and error as
+ * python/isdf.py:238-263 and :176-191.  form 0: FP64 MFMA (as
+ * mpisdf_isdf_fit_device).  form 1 (no reference counterpart): each column
+ * s_g and each row of L^-1 as five base-254 digit planes under a power of two
+ * (within ~4.8e-13 of its largest entry); ||L^-1 s_g||^2 comes from the 15
+ * digit-pair products on INT8 MFMA, exact INT32 sums combined in FP64,
+ * without forming Z (k <= 16384; a pseudo-inverse fit, results[9] = 1, keeps
+ * FP64 products).  results[12]: 0-9 as for
+ * mpisdf_isdf_fit, 10 HIP-event seconds of the Z product alone, 11 its
+ * launches. */
+int mpisdf_isdf_fit_device_form(const mpisdf_context* ctx, const double* anchor_xyz_dev, const double* anchor_w,
+                                int64_t k, const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch,
+                                int32_t form, double* tsum, double* results, void* stream);
+
 /* Pair-space error of given interpolation vectors theta (host, row-major
  * [k][n]) over the grid points xyz[n*3], w[n] (host), as the reference's
  * _stream_pair_space_error_norms (python/isdf.py:176-191) evaluates it, in
@@ -370,6 +390,37 @@ int mpisdf_isdf_jk_device(const double* x, int64_t k, int64_t nao, const double*
 int mpisdf_lebedev(int32_t npoints, double* xyz, double* weights);
 int mpisdf_becke_grid(const int32_t* charges, const double* coords, int64_t natm, int32_t level, int32_t device,
                       double* xyz, double* weights, int64_t capacity, int64_t* count);
+
+/* ---- Integral-direct Cholesky Coulomb metric ---------------------------- */
+
+/* Pivoted Cholesky of the packed AO-pair Coulomb matrix V[(xy),(zw)] =
+ * (xy|zw) over x >= y (index x(x+1)/2 + y), columns computed
+ * integral-direct per pivot on the GPU: mp_isdf.cholesky_eri
+ * (cpp/src/coulomb_metric_pybind.cpp:25-45 -> cpp/src/cholesky_eri_greedy.cpp:53-175,
+ * cpp/src/cholesky_eri_uniform.cpp:4-10).  algorithm 0 unblocked_greedy
+ * (largest residual, first index on ties), 1 unblocked_uniform (uniform
+ * among residuals above tolerance * the largest initial diagonal, drawn from
+ * std::mt19937_64(seed) as cpp/include/cholesky_pivot.hpp:16-36 draws);
+ * tolerance relative to the largest initial diagonal, max_rank 0 for no
+ * cap; s, p and d shells.  The handle holds the factors on the device until
+ * mpisdf_eri_cholesky_destroy. */
+typedef struct mpisdf_eri_cholesky mpisdf_eri_cholesky;
+int mpisdf_cholesky_eri(const mpisdf_context* ctx, double tolerance, int64_t max_rank, int32_t algorithm,
+                        uint64_t seed, int32_t debug, mpisdf_eri_cholesky** out);
+/* rank and npair = nao(nao+1)/2 of a decomposition. */
+int mpisdf_eri_cholesky_size(const mpisdf_eri_cholesky* h, int64_t* rank, int64_t* npair);
+/* factors[rank*npair] (row-major, one factor per row), pivots[rank] (packed
+ * pair indices in selection order), residual[npair]; any may be NULL. */
+int mpisdf_eri_cholesky_copy(const mpisdf_eri_cholesky* h, double* factors, int64_t* pivots, double* residual);
+int mpisdf_eri_cholesky_destroy(mpisdf_eri_cholesky* h);
+/* out[nvec*nvec] = P P^T, P = packed(transform) F^T with transform[nvec][nao*nao]
+ * row-major (packed folds T[v,y,x] into pair (x,y)): mp_isdf.coulomb_metric
+ * (cpp/src/coulomb_metric_pybind.cpp:73-92 -> cpp/src/coulomb_metric.cpp:20-98),
+ * without anchor weights.  batch_size must be positive (the reference's host
+ * batching; the products here run whole on the GPU). */
+int mpisdf_coulomb_metric(const mpisdf_context* ctx, const double* transform, int64_t nvec, int32_t algorithm,
+                          double tolerance, int64_t max_rank, int64_t batch_size, int32_t debug, uint64_t seed,
+                          double* out);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,7 @@
 #include "context.hpp"
 #include "coulomb.hpp"
 #include "cvt.hpp"
+#include "eri.hpp"
 #include "rbrp.hpp"
 #include "kernels.hpp"
 #include "qrcp.hpp"
@@ -18,6 +19,10 @@
 
 struct mpisdf_context {
     std::unique_ptr<mpisdf::Context> impl;
+};
+
+struct mpisdf_eri_cholesky {
+    mpisdf::EriCholesky impl;
 };
 
 namespace {
@@ -120,6 +125,14 @@ int mpisdf_context_create(const int32_t* atm, int64_t natm, const int32_t* bas, 
 int mpisdf_context_destroy(mpisdf_context* ctx)
 {
     return guard([&] { delete ctx; });
+}
+
+int mpisdf_context_release_workspace(mpisdf_context* ctx)
+{
+    return guard([&] {
+        if (!ctx || !ctx->impl) throw std::invalid_argument("null mp_isdf context");
+        ctx->impl->release_sketch_store();
+    });
 }
 
 int mpisdf_context_nao(const mpisdf_context* ctx, int64_t* nao)
@@ -511,7 +524,8 @@ namespace {
 
 void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,
              const double* xyz_dev, const double* w_dev, int64_t n, int64_t batch, double* theta, double* tsum,
-             double* results, hipStream_t s)
+             double* results, hipStream_t s, mpisdf::FitProducts products = mpisdf::FitProducts::F64,
+             bool extended = false)
 {
     if (k <= 0) throw std::invalid_argument("the fit needs at least one anchor");
     for (int64_t i = 0; i < k; ++i)
@@ -523,8 +537,9 @@ void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anc
     mpisdf::fit_factor(c, anchor_xyz_dev, k, f, s);
     const auto t1 = std::chrono::steady_clock::now();
     double err[2] = {0, 0};
-    double gemm_stats[3] = {0, 0, 0};
-    if (n > 0) mpisdf::fit_stream(c, f, anchor_w, xyz_dev, w_dev, n, batch, theta, err, tsum, s, gemm_stats);
+    double gemm_stats[5] = {0, 0, 0, 0, 0};
+    if (n > 0)
+        mpisdf::fit_stream(c, f, anchor_w, xyz_dev, w_dev, n, batch, theta, err, tsum, s, gemm_stats, products);
     MPISDF_HIP(hipStreamSynchronize(s));
     const auto t2 = std::chrono::steady_clock::now();
     if (results) {
@@ -538,6 +553,10 @@ void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anc
         results[7] = gemm_stats[1];
         results[8] = (double)f.retained_rank;
         results[9] = f.triangular ? 0.0 : 1.0;
+        if (extended) {
+            results[10] = gemm_stats[3];
+            results[11] = gemm_stats[4];
+        }
     }
 }
 
@@ -568,6 +587,22 @@ extern "C" int mpisdf_isdf_fit_device(const mpisdf_context* ctx, const double* a
         auto& c = unwrap(ctx);
         run_fit(c, anchor_xyz_dev, anchor_w, k, xyz_dev, w_dev, n, batch, nullptr, tsum, results,
                 static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" int mpisdf_isdf_fit_device_form(const mpisdf_context* ctx, const double* anchor_xyz_dev,
+                                           const double* anchor_w, int64_t k, const double* xyz_dev,
+                                           const double* w_dev, int64_t n, int64_t batch, int32_t form, double* tsum,
+                                           double* results, void* stream)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (form != 0 && form != 1) throw std::invalid_argument("fit form must be 0 (FP64) or 1 (INT8 digits)");
+        if (!anchor_xyz_dev || !anchor_w || (n > 0 && (!xyz_dev || !w_dev)))
+            throw std::invalid_argument("invalid fit arguments");
+        run_fit(c, anchor_xyz_dev, anchor_w, k, xyz_dev, w_dev, n, batch, nullptr, tsum, results,
+                static_cast<hipStream_t>(stream),
+                form == 1 ? mpisdf::FitProducts::Int8Digits : mpisdf::FitProducts::F64, true);
     });
 }
 
@@ -933,4 +968,72 @@ extern "C" int mpisdf_rbrp_points(const mpisdf_context* ctx, const double* point
     return rbrp_impl(ctx, points, nullptr, weights, n, 0, batch_size, block_size, tolerance, max_rank, seed,
                      tau_block, rank_rtol, refresh_every, svd_rtol, compute_id, indices, capacity, count, interp_vecs,
                      transform, history, nblocks, scalars, status);
+}
+
+/* ---- integral-direct Cholesky Coulomb metric ---------------------------- */
+
+namespace {
+
+mpisdf::CholeskyAlgorithm cholesky_algorithm(int32_t algorithm)
+{
+    if (algorithm != 0 && algorithm != 1) throw std::invalid_argument("unknown Cholesky algorithm");
+    return algorithm == 1 ? mpisdf::CholeskyAlgorithm::UnblockedUniform : mpisdf::CholeskyAlgorithm::UnblockedGreedy;
+}
+
+}  // namespace
+
+extern "C" int mpisdf_cholesky_eri(const mpisdf_context* ctx, double tolerance, int64_t max_rank, int32_t algorithm,
+                                   uint64_t seed, int32_t debug, mpisdf_eri_cholesky** out)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (!out) throw std::invalid_argument("null Cholesky handle pointer");
+        *out = nullptr;
+        auto h = std::make_unique<mpisdf_eri_cholesky>();
+        mpisdf::eri_cholesky(c, tolerance, max_rank, cholesky_algorithm(algorithm), seed, debug != 0, h->impl,
+                             nullptr);
+        MPISDF_HIP(hipStreamSynchronize(nullptr));
+        *out = h.release();
+    });
+}
+
+extern "C" int mpisdf_eri_cholesky_size(const mpisdf_eri_cholesky* h, int64_t* rank, int64_t* npair)
+{
+    return guard([&] {
+        if (!h || !rank || !npair) throw std::invalid_argument("null Cholesky handle or output");
+        *rank = h->impl.rank;
+        *npair = h->impl.npair;
+    });
+}
+
+extern "C" int mpisdf_eri_cholesky_copy(const mpisdf_eri_cholesky* h, double* factors, int64_t* pivots,
+                                        double* residual)
+{
+    return guard([&] {
+        if (!h) throw std::invalid_argument("null Cholesky handle");
+        const auto& f = h->impl;
+        if (factors && f.rank > 0)
+            MPISDF_HIP(hipMemcpy(factors, f.factors.ptr, (size_t)(f.rank * f.npair) * sizeof(double),
+                                 hipMemcpyDeviceToHost));
+        if (pivots) std::copy(f.pivots.begin(), f.pivots.end(), pivots);
+        if (residual)
+            MPISDF_HIP(hipMemcpy(residual, f.residual.ptr, (size_t)f.npair * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+extern "C" int mpisdf_eri_cholesky_destroy(mpisdf_eri_cholesky* h)
+{
+    return guard([&] { delete h; });
+}
+
+extern "C" int mpisdf_coulomb_metric(const mpisdf_context* ctx, const double* transform, int64_t nvec,
+                                     int32_t algorithm, double tolerance, int64_t max_rank, int64_t batch_size,
+                                     int32_t debug, uint64_t seed, double* out)
+{
+    return guard([&] {
+        auto& c = unwrap(ctx);
+        if (nvec < 0 || (nvec > 0 && (!transform || !out))) throw std::invalid_argument("invalid Coulomb metric arguments");
+        mpisdf::eri_coulomb_metric(c, transform, nvec, cholesky_algorithm(algorithm), tolerance, max_rank,
+                                   batch_size, debug != 0, seed, out, nullptr);
+    });
 }

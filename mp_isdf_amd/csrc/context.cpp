@@ -148,6 +148,21 @@ std::vector<double> transform_table(int l, bool cart)
     return t;
 }
 
+void* Context::reserve_sketch(size_t bytes)
+{
+    if (sketch_store.count < bytes) {
+        sketch_store.release();  // before the new allocation: both may not fit
+        sketch_store.resize(bytes);
+    }
+    return sketch_store.ptr;
+}
+
+void Context::release_sketch_store()
+{
+    std::lock_guard<std::mutex> lock(sketch_mutex);
+    sketch_store.release();
+}
+
 const std::pair<std::vector<AoChunk>, std::unique_ptr<DeviceBuffer<AoChunk>>>&
 Context::chunk_table(int width)
 {

@@ -33,6 +33,16 @@ struct Context {
     DeviceBuffer<double> d_env;
     DeviceBuffer<double> d_tabs;
 
+    // The randomized selectors' sketch storage, kept between selections and
+    // grown to the largest one asked for: a sketch of ~200 GB freed and
+    // allocated again would wait seconds for the driver to clear the freed
+    // memory first.  Selections on one context take sketch_mutex while they
+    // use it; release_sketch_store() gives the memory back.
+    std::mutex sketch_mutex;
+    DeviceBuffer<uint8_t> sketch_store;
+    void* reserve_sketch(size_t bytes);
+    void release_sketch_store();
+
     // Chunk tables per chunk width (built lazily, one per kernel variant).
     std::mutex chunk_mutex;
     std::map<int, std::pair<std::vector<AoChunk>, std::unique_ptr<DeviceBuffer<AoChunk>>>> chunks;

@@ -200,16 +200,43 @@ __global__ void pair_error_norms_kernel(const double* __restrict__ zpart, int64_
     }
 }
 
+// pp[g] = ||phi_g||^2 (rows of phi, leading dimension ld), one wave per point.
+__global__ void row_norms2_kernel(const double* __restrict__ phi, int64_t d, int64_t ld, int64_t nb,
+                                  double* __restrict__ pp)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t g = (int64_t)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    if (g >= nb) return;
+    double acc = 0;
+    for (int64_t a = lane; a < d; a += 64) { const double v = phi[g * ld + a]; acc += v * v; }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+    if (lane == 0) pp[g] = acc;
+}
+
+// Per-point pair-error terms from ||z_g||^2 and ||phi_g||^2:
+// out[g] = |w_g| max(pp^2 - zz, 0), out[ld + g] = |w_g| pp^2.
+__global__ void pair_error_from_norms_kernel(const double* __restrict__ zz, const double* __restrict__ pp,
+                                             const double* __restrict__ w, int64_t nb, double* __restrict__ out,
+                                             int64_t ld)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nb) return;
+    const double qn = pp[g] * pp[g], aw = fabs(w[g]);
+    out[g] = aw * fmax(qn - zz[g], 0.0);
+    out[ld + g] = aw * qn;
+}
+
 // part[y * k + r] = sum over the columns g of column block y (128 columns) of
 // w[g] z[r + g * k]: this batch's share of sum_g w_g z_g, added in block order.
-__global__ void zw_rows_kernel(const double* __restrict__ z, int64_t k, int64_t nb, const double* __restrict__ w,
-                               double* __restrict__ part)
+__global__ void zw_rows_kernel(const double* __restrict__ z, int64_t k, int64_t ld, int64_t nb,
+                               const double* __restrict__ w, double* __restrict__ part)
 {
     const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (r >= k) return;
     const int64_t g0 = (int64_t)blockIdx.y * 128, g1 = min(nb, g0 + 128);
     double s = 0;
-    for (int64_t g = g0; g < g1; ++g) s += w[g] * z[r + g * k];
+    for (int64_t g = g0; g < g1; ++g) s += w[g] * z[r + g * ld];
     part[(int64_t)blockIdx.y * k + r] = s;
 }
 
@@ -485,18 +512,45 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
 
 void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const double* xyz, const double* w, int64_t n,
                 int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s,
-                double* gemm_stats)
+                double* gemm_stats, FitProducts products)
 {
     const int64_t k = f.k, d = f.d;
-    LaunchTimer timer;
+    // The pseudo-inverse's rows carry the inverse square roots of eigenvalues
+    // down to 1e-14 of the largest, which would amplify the digits' rounding
+    // (~5e-13) to ~1e-6 of z: that (small, k <= 2048) case keeps FP64 products.
+    const bool digits = products == FitProducts::Int8Digits && f.triangular;
+    if (digits && theta_host)
+        throw std::invalid_argument("the INT8-digit grid pass gives the pair error and theta . w, not theta");
+    if (digits && k > kDigitRowsMaxF64)
+        throw std::invalid_argument("the INT8-digit grid pass takes at most 16384 anchors");
+    LaunchTimer timer, ztimer;
     double gemm_flops = 0;
+    int64_t zlaunches = 0;
     batch = std::max<int64_t>(1, std::min(batch, n));
-    DeviceBuffer<double> phi((size_t)(batch * d)), sm((size_t)(k * batch)), z((size_t)(k * batch)), acc(2), wa;
-    const int64_t zgroups = (k + 127) / 128;  // row groups of the Z product
+    // digit form: S is held with a column pitch of P (a multiple of 64) and
+    // digitised in place; the Z product folds it against the digits of L^-1
+    const int64_t P = digits ? (k + kDigitSlice - 1) / kDigitSlice * kDigitSlice : k;
+    // digit form: phi rows with a pitch of dpad (a multiple of 64), digitised
+    // in place after their norms are taken
+    const int64_t dpad = digits ? (d + kDigitSlice - 1) / kDigitSlice * kDigitSlice : d;
+    DeviceBuffer<double> phi((size_t)(batch * dpad)), sm((size_t)(P * batch)), z, acc(2), wa;
+    if (digits && dpad != d) MPISDF_HIP(hipMemsetAsync(phi.ptr, 0, phi.count * sizeof(double), s));
+    if (!digits) z.resize((size_t)(k * batch));
+    const int64_t zgroups = digits ? 1 : (k + 127) / 128;  // row groups of the Z product
     DeviceBuffer<double> zz((size_t)(zgroups * batch)), ept((size_t)(2 * batch));
-    DeviceBuffer<double> th, thr, tsum, tpart, zw;
+    DeviceBuffer<double> th, thr, tsum, tpart, zw, yscale, ynorm, pscale, pp;
+    FoldI8Rows lrows, arows;
+    if (digits) {
+        yscale.resize((size_t)batch);
+        ynorm.resize((size_t)batch);
+        pscale.resize((size_t)batch);
+        pp.resize((size_t)batch);
+        split_rows_i8(f.linv.ptr, k, k, k, lrows, s);
+        split_rows_i8(f.phi_a.ptr, d, k, d, arows, s);
+    }
     // theta . w alone needs no theta: sum_g w_g theta_phys[:, g] =
-    // L^-T (sum_g w_g z_g), one k x k triangular product after the grid pass.
+    // L^-T (sum_g w_g z_g), one k x k triangular product after the grid pass
+    // (digit form: L^-T L^-1 (sum_g w_g s_g), two).
     const bool theta_w_only = tsum_out && !theta_host;
     std::vector<double> stage;
     wa.upload(anchor_w, k, s);
@@ -516,7 +570,43 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
     MPISDF_HIP(hipMemsetAsync(acc.ptr, 0, 2 * sizeof(double), s));
     for (int64_t start = 0; start < n; start += batch) {
         const int64_t nb = std::min(batch, n - start);
-        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, d, nb * d, s);
+        launch_ao_eval(ctx, xyz + 3 * start, nb, kValues, kF64, phi.ptr, dpad, nb * dpad, s);
+        if (digits) {
+            // ||phi_g||^2, then phi_g and the anchor rows as five base-254
+            // digits; S = (Phi_a Phi_b^T)^2 (k x nb, pitch P) from the 15
+            // digit-pair products on INT8 MFMA
+            hipLaunchKernelGGL(row_norms2_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, phi.ptr, d, dpad,
+                               nb, pp.ptr);
+            digitize_columns_f64_max(phi.ptr, d, nb, dpad, ynorm.ptr, pscale.ptr, s);
+            timer.begin(s);
+            fold_rows_square_i8(arows, reinterpret_cast<const int8_t*>(phi.ptr), dpad, pscale.ptr, nb, sm.ptr, P, s);
+            timer.end(s);
+            gemm_flops += 2.0 * (double)k * (double)d * (double)nb;
+            if (theta_w_only) {  // sum_g w_g s_g, before S becomes digits
+                const dim3 grid((unsigned)((k + 255) / 256), (unsigned)((nb + 127) / 128));
+                hipLaunchKernelGGL(zw_rows_kernel, grid, dim3(256), 0, s, sm.ptr, k, P, nb, w + start, tpart.ptr);
+                MPISDF_HIP(hipGetLastError());
+                launch_sum_partials(tpart.ptr, grid.y, k, zw.ptr, true, s);
+            }
+            // five base-254 planes per entry under each column's power of two
+            // (~4.8e-13 of its largest entry), then ||L^-1 s_hat_g||^2 on INT8
+            digitize_columns_f64_max(sm.ptr, k, nb, P, ynorm.ptr, yscale.ptr, s);
+            MPISDF_HIP(hipMemsetAsync(zz.ptr, 0, (size_t)nb * sizeof(double), s));
+            timer.begin(s);
+            ztimer.begin(s);
+            fold_rows_i8(lrows, f.triangular, reinterpret_cast<const int8_t*>(sm.ptr), P, yscale.ptr, nb, zz.ptr, s);
+            ztimer.end(s);
+            timer.end(s);
+            zlaunches += (k + 127) / 128;
+            gemm_flops += (double)k * (double)(k + 1) * (double)nb;
+            if (err_out) {
+                hipLaunchKernelGGL(pair_error_from_norms_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s,
+                                   zz.ptr, pp.ptr, w + start, nb, ept.ptr, batch);
+                launch_sum_fixed(ept.ptr, nb, batch, 2, acc.ptr, s);
+            }
+            MPISDF_HIP(hipGetLastError());
+            continue;
+        }
         // S = (Phi_a Phi_b^T)^2, k x nb column-major (squared in the epilogue)
         timer.begin(s);
         gemm_tn_f64_fused(TnFused::Square, k, nb, d, 1.0, f.phi_a.ptr, d, phi.ptr, d, sm.ptr, k, nullptr, nullptr,
@@ -525,9 +615,12 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         gemm_flops += 2.0 * (double)k * (double)d * (double)nb;
         // Z = L^-1 S (lower triangular A), with ||z_g||^2 summed in the epilogue
         timer.begin(s);
+        ztimer.begin(s);
         gemm_tn_f64_fused(TnFused::ColumnSumSquares, k, nb, k, 1.0, f.linv.ptr, k, sm.ptr, k, z.ptr, k, zz.ptr,
                           nullptr, nullptr, s, f.triangular ? 1 : 0);
+        ztimer.end(s);
         timer.end(s);
+        ++zlaunches;
         gemm_flops += (double)k * (double)(k + 1) * (double)nb;
         if (err_out) {
             hipLaunchKernelGGL(pair_error_norms_kernel, dim3((unsigned)((nb + 3) / 4)), dim3(256), 0, s, zz.ptr, zgroups,
@@ -536,7 +629,7 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         }
         if (theta_w_only) {
             const dim3 grid((unsigned)((k + 255) / 256), (unsigned)((nb + 127) / 128));
-            hipLaunchKernelGGL(zw_rows_kernel, grid, dim3(256), 0, s, z.ptr, k, nb, w + start, tpart.ptr);
+            hipLaunchKernelGGL(zw_rows_kernel, grid, dim3(256), 0, s, z.ptr, k, k, nb, w + start, tpart.ptr);
             MPISDF_HIP(hipGetLastError());
             launch_sum_partials(tpart.ptr, grid.y, k, zw.ptr, true, s);
         }
@@ -575,6 +668,12 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         err_out[0] = h[0];
         err_out[1] = h[1];
     }
+    if (theta_w_only && digits) {  // zw = sum_g w_g s_g: first L^-1 zw (linv: row-major)
+        DeviceBuffer<double> tmp((size_t)k);
+        gemm_f64(k, 1, k, 1.0, MatView{f.linv.ptr, k, 1}, MatView{zw.ptr, 1, 1}, 0.0, MutView{tmp.ptr, 1, 1}, s);
+        MPISDF_HIP(hipMemcpyAsync(zw.ptr, tmp.ptr, (size_t)k * sizeof(double), hipMemcpyDeviceToDevice, s));
+        MPISDF_HIP(hipStreamSynchronize(s));
+    }
     if (theta_w_only)  // t_phys = L^-T zw (linvt: row-major upper triangular)
         gemm_f64(k, 1, k, 1.0, MatView{f.linvt.ptr, k, 1}, MatView{zw.ptr, 1, 1}, 0.0, MutView{tsum.ptr, 1, 1}, s);
     if (tsum_out) {
@@ -587,6 +686,8 @@ void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const 
         gemm_stats[0] = timer.seconds();
         gemm_stats[1] = gemm_flops;
         gemm_stats[2] = (double)timer.launches();
+        gemm_stats[3] = ztimer.seconds();
+        gemm_stats[4] = (double)zlaunches;
     }
 }
 
@@ -835,7 +936,7 @@ void coulomb_energy_gradient(Context& ctx, const FitFactor& f, const double* anc
                            (const double*)nullptr, k, k * nb, sm.ptr);
         // sigma += S v_b, per 128-point block then in block order.
         const dim3 grid((unsigned)((k + 255) / 256), (unsigned)((nb + 127) / 128));
-        hipLaunchKernelGGL(zw_rows_kernel, grid, dim3(256), 0, s, sm.ptr, k, nb, vb, spart.ptr);
+        hipLaunchKernelGGL(zw_rows_kernel, grid, dim3(256), 0, s, sm.ptr, k, k, nb, vb, spart.ptr);
         launch_sum_partials(spart.ptr, grid.y, k, sigma.ptr, true, s);
         // Grid AOs: abar_g = 2 v_g Phi_a^T (lam o t_g).
         hipLaunchKernelGGL(scale_rc_kernel, blocks(k * nb), dim3(256), 0, s, tm.ptr, k, k * nb, lam.ptr,

@@ -24,6 +24,16 @@ struct FitFactor {
 // (k <= 2048; std::runtime_error beyond).
 void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f, hipStream_t s);
 
+// Arithmetic of the grid pass's products.  F64: S and Z = L^-1 S on FP64
+// MFMA.  Int8Digits: both products on INT8 MFMA from five base-254 digits
+// per entry under a power of two per row or column (within ~4.8e-13 of its
+// largest entry), the 15 digit-pair products exact in INT32 and combined in
+// FP64: S from the digits of phi_g and of the anchor rows, then
+// ||L^-1 s_g||^2 from the digits of S's columns and of L^-1's rows, without
+// forming Z; no theta (k <= 16384).  A pseudo-inverse fit (dependent
+// anchors) keeps F64.
+enum class FitProducts { F64, Int8Digits };
+
 // Streams grid batches (device xyz[n][3], device w[n]) through
 //   S = (Phi_a Phi_b^T)^2, Z = L^-1 S, theta = L^-T Z / w_anchor (host anchor_w[k]).
 // err_out (host, 2 doubles): sum |w| (||phi||^4 - ||z||^2), sum |w| ||phi||^4.
@@ -31,13 +41,13 @@ void fit_factor(Context& ctx, const double* anchor_xyz, int64_t k, FitFactor& f,
 // L^-T Z / w_anchor.  tsum_out (optional host [k]): sum_g theta[:, g] w_g;
 // without theta_host it is L^-T (sum_g w_g z_g) / w_anchor, so the grid pass
 // then runs only the S and Z products.
-// gemm_stats (optional, host double[3]): summed HIP-event seconds of the grid
-// pass's TN products (S, Z = L^-1 S, theta = L^-T Z), their algorithmic flops
-// (2 k d + 2 k (k+1) per point, the latter only when theta is formed) and
-// launch count.
+// gemm_stats (optional, host double[5]): summed HIP-event seconds of the grid
+// pass's products (S, Z = L^-1 S, theta = L^-T Z), their algorithmic flops
+// (2 k d + k (k+1) per point, + k (k+1) when theta is formed), timed
+// intervals, then the seconds and launch count of the Z product alone.
 void fit_stream(Context& ctx, const FitFactor& f, const double* anchor_w, const double* xyz, const double* w, int64_t n,
                 int64_t batch, double* theta_host, double* err_out, double* tsum_out, hipStream_t s,
-                double* gemm_stats = nullptr);
+                double* gemm_stats = nullptr, FitProducts products = FitProducts::F64);
 
 // Pair-space error of given interpolation vectors (theta_host: row-major
 // [k][n] on the host), as python/isdf.py:176-191 evaluates it: per point

@@ -23,8 +23,9 @@
 // accumulator (at most four pairs: |sum| < 4 * 64^2 * m < 2^31 for m < 2^16),
 // and the six accumulators are combined in FP64:
 //     W = u_t s_j sum_g 128^-(g+2) acc_g.
-// In base 254 the 14 pairs p + r <= 4 run in five groups (at most four
-// pairs: |sum| < 4 * 127^2 * m < 2^31 for m < 2^15).
+// In base 254 the pairs p + r <= 4 run in five groups: 14 with four Y
+// planes (at most four pairs per group: |sum| < 4 * 127^2 * m < 2^31 for
+// m < 2^15), 15 with five (at most five: m <= 26624).
 // The dropped pairs (p + r >= 6 in base 128, >= 5 in base 254) and the truncation of q leave ~1e-11 of
 // ||y_j||^2 in the folded sums (measured), covered by the QRCP's relative
 // margin.
@@ -185,14 +186,18 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_kernel(floa
 // Four base-254 planes (|d| <= 127) in place of FP64 column j (pitch 8 P
 // bytes): y_hat[i] = s_j sum_r d_r 254^-(r+1), within s_j 254^-4 / 2
 // (~1.2e-10 s_j) of y; the column's bytes past its 4 P digit bytes are zeroed.
+// FROM_MAX: the scale comes from the column's largest |y_ij| (found here)
+// rather than from its norm, and ynorm2 is output only.  NY (4 or 5) planes:
+// within s_j 254^-NY / 2 of each entry.
 constexpr int kDigitizeEntriesF64 = (int)(kDigitRowsMaxF64 / kDigitizeThreads);  // P <= 16384
+template <bool FROM_MAX, int NY>
 __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(double* __restrict__ y, int64_t m,
                                                                                 int64_t P,
                                                                                 double* __restrict__ ynorm2,
                                                                                 double* __restrict__ scale)
 {
-    constexpr int NY = 4;
     __shared__ double part[kDigitizeThreads / 64];
+    __shared__ double pmax[kDigitizeThreads / 64];
     const int64_t j = blockIdx.x;
     double* src = y + j * P;  // 8 P bytes
     double x[kDigitizeEntriesF64];
@@ -201,7 +206,22 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(
         const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
         x[e] = i < m ? src[i] : 0.0;
     }
-    const double nrm = sqrt(ynorm2[j]);  // >= max |y_ij|
+    double nrm;  // >= max |y_ij|
+    if (FROM_MAX) {
+        double mx = 0;
+#pragma unroll
+        for (int e = 0; e < kDigitizeEntriesF64; ++e) mx = fmax(mx, fabs(x[e]));
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if ((threadIdx.x & 63) == 0) pmax[threadIdx.x / 64] = mx;
+        __syncthreads();
+        mx = 0;
+#pragma unroll
+        for (int w = 0; w < kDigitizeThreads / 64; ++w) mx = fmax(mx, pmax[w]);
+        nrm = mx;
+    } else {
+        nrm = sqrt(ynorm2[j]);
+    }
     int ex = 0;
     if (nrm > 0) frexp(nrm, &ex);
     const double sc = nrm > 0 ? ldexp(1.0, ex + 1) : 0.0;
@@ -257,7 +277,14 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // lane reads 32 contiguous bytes of a plane per slice.  Rows past b carry
 // zero digits; there are no branches in the MFMA sequence (a branch would
 // make the compiler drain every outstanding load at its join).
-template <bool ENERGY, int NR, int NY, int BASE>
+// Epilogues: kEpiFold sums W^2 per column into proj2 (and per row into
+// energy); kEpiSquare stores C[row + col * ldc] = W^2; kEpiKhatriRao takes
+// the group's rows 64..127 as the partners of rows 0..63 and stores
+// C[m0 / 2 + r + col * ldc] = cscale[col] W[r] W[64 + r] (r < 64), with
+// proj2[col] += the column's sum of squares and non-finite entries counted
+// into *bad.
+constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2;
+template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
                                                                    int64_t bpad, int64_t b, int64_t m0,
@@ -265,7 +292,11 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                                                                    int colbytes,
                                                                    const double* __restrict__ yscale, int64_t m,
                                                                    int64_t n, double* __restrict__ proj2,
-                                                                   double* __restrict__ energy)
+                                                                   double* __restrict__ energy,
+                                                                   double* __restrict__ cout = nullptr,
+                                                                   int64_t ldc = 0,
+                                                                   const double* __restrict__ cscale = nullptr,
+                                                                   unsigned long long* __restrict__ bad = nullptr)
 {
     constexpr int ND = DigitScheme<BASE>::ND, NG = DigitScheme<BASE>::NG;
     constexpr int ROWS = 32 * NR;  // Q rows staged per slice
@@ -407,6 +438,83 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 #undef MPISDF_STASH_Q
 
     // W[t][j] = u_t s_j sum_g BASE^-(g+2) acc_g.
+    if constexpr (EPI == kEpiSquare) {
+        // lane (l32, half) owns column col and, per e / 4, four consecutive rows
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+            double w4[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int e = 4 * q4 + i;
+                double v = 0;
+#pragma unroll
+                for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / BASE) + (double)acc[g][e];
+                const int64_t row = m0 + rt * 32 + 8 * q4 + 4 * half + i;
+                const double wv = row < b ? v * (1.0 / ((double)BASE * BASE)) * qs[row] * sj : 0.0;
+                w4[i] = wv * wv;
+            }
+            const int64_t row0 = m0 + rt * 32 + 8 * q4 + 4 * half;
+            if (cok && row0 < b) {
+                double* dst = cout + row0 + col * ldc;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (row0 + i < b) dst[i] = w4[i];
+            }
+        }
+        return;
+    }
+    if constexpr (EPI == kEpiKhatriRao) {
+        static_assert(NR == 4, "the Khatri-Rao epilogue pairs rows r and 64 + r of a 128-row group");
+        // waves rt = 2, 3 (rows 64..127) hand their W to waves rt = 0, 1
+        // through LDS (the operand buffers are free after the last barrier)
+        double* xw = reinterpret_cast<double*>(&sQ[0][0][0]);
+        double wv[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            double v = 0;
+#pragma unroll
+            for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / BASE) + (double)acc[g][e];
+            const int64_t row = m0 + rt * 32 + 8 * (e / 4) + 4 * half + (e % 4);
+            wv[e] = row < b ? v * (1.0 / ((double)BASE * BASE)) * qs[row] * sj : 0.0;
+        }
+        if (rt >= 2) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) xw[(((rt - 2) * 2 + ct) * 16 + e) * 64 + lane] = wv[e];
+        }
+        __syncthreads();
+        double colsq = 0;
+        if (rt < 2) {
+            const double cs = cok ? cscale[col] : 0.0;
+            unsigned long long nbad = 0;
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+                double y4[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int e = 4 * q4 + i;
+                    const double y = cs * wv[e] * xw[((rt * 2 + ct) * 16 + e) * 64 + lane];
+                    if (!isfinite(y)) ++nbad;
+                    y4[i] = y;
+                    colsq += y * y;
+                }
+                if (cok) {
+                    double* dst = cout + m0 / 2 + rt * 32 + 8 * q4 + 4 * half + col * ldc;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) dst[i] = y4[i];
+                }
+            }
+            if (nbad) atomicAdd(bad, nbad);
+            colsq += __shfl_xor(colsq, 32);
+            if (half == 0) colred[rt][ct * 32 + l32] = colsq;
+        }
+        __syncthreads();
+        if (tid < FBN) {
+            const int64_t c = g0 + tid;
+            const double sum = colred[0][tid] + colred[1][tid];
+            if (c < n) proj2[c] += sum;  // row groups run in order
+        }
+        return;
+    }
     double colsq = 0;
 #pragma unroll
     for (int e = 0; e < 16; ++e) {
@@ -478,9 +586,125 @@ void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* yn
     if (P > kDigitRowsMaxF64)
         throw std::invalid_argument("digitize_columns_f64 holds one column in registers (pitch <= 16384)");
     if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
-    hipLaunchKernelGGL(digitize_columns_f64_kernel, dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream, y, m, P,
-                       ynorm2, scale);
+    hipLaunchKernelGGL((digitize_columns_f64_kernel<false, 4>), dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream,
+                       y, m, P, ynorm2, scale);
     MPISDF_HIP(hipGetLastError());
+}
+
+void digitize_columns_f64_max(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
+                              hipStream_t stream)
+{
+    if (n <= 0) return;
+    if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
+    if (P > kDigitRowsMaxF64)
+        throw std::invalid_argument("digitize_columns_f64 holds one column in registers (pitch <= 16384)");
+    if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
+    hipLaunchKernelGGL((digitize_columns_f64_kernel<true, 5>), dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream,
+                       y, m, P, ynorm2, scale);
+    MPISDF_HIP(hipGetLastError());
+}
+
+void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream)
+{
+    if (rows <= 0 || kdim <= 0 || lda < kdim) throw std::invalid_argument("invalid row digit operand");
+    if (kdim > 32767) throw std::invalid_argument("the base-254 fold needs at most 32767 entries per row");
+    out.rows = rows;
+    out.kdim = kdim;
+    out.bpad = (rows + FBM - 1) / FBM * FBM;
+    out.ldq = (kdim + FBK - 1) / FBK * FBK;
+    const size_t need = (size_t)(DigitScheme<254>::ND * out.bpad * out.ldq);
+    if (need >= (size_t(1) << 31)) throw std::invalid_argument("the INT8 fold's row digits exceed 2 GiB");
+    if (out.digits.count < need) out.digits.resize(need);
+    if (out.scales.count < (size_t)out.bpad) out.scales.resize((size_t)out.bpad);
+    if (out.bpad > 0x7fffffff) throw std::overflow_error("too many rows to digitise");
+    hipLaunchKernelGGL(split_q_kernel<254>, dim3((unsigned)out.bpad), dim3(256), 0, stream, a, lda, kdim, rows, out.ldq,
+                       out.bpad, out.digits.ptr, out.scales.ptr);
+    MPISDF_HIP(hipGetLastError());
+}
+
+// out[(128 g + r) * d + a] = u[a * ell + 64 g + r], out[(128 g + 64 + r) * d + a]
+// = v[a * ell + 64 g + r] (r < 64; zero past ell).
+__global__ void stack_khatri_rao_kernel(const double* __restrict__ u, const double* __restrict__ v, int64_t d,
+                                        int64_t ell, int64_t rows, double* __restrict__ out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= rows * d) return;
+    const int64_t row = idx / d, a = idx % d;
+    const int64_t g = row / 128, r = row % 128, sk = 64 * g + (r & 63);
+    const double* src = r < 64 ? u : v;
+    out[idx] = sk < ell ? src[a * ell + sk] : 0.0;
+}
+
+void split_khatri_rao_rows_i8(const double* u, const double* v, int64_t d, int64_t ell, FoldI8Rows& out,
+                              hipStream_t stream)
+{
+    const int64_t rows = 2 * ((ell + 63) / 64 * 64);
+    DeviceBuffer<double> stacked((size_t)(rows * d));
+    hipLaunchKernelGGL(stack_khatri_rao_kernel, dim3((unsigned)((rows * d + 255) / 256)), dim3(256), 0, stream, u, v,
+                       d, ell, rows, stacked.ptr);
+    MPISDF_HIP(hipGetLastError());
+    split_rows_i8(stacked.ptr, d, rows, d, out, stream);
+    MPISDF_HIP(hipStreamSynchronize(stream));  // before the stacked rows are freed
+}
+
+void fold_rows_square_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n, double* c,
+                         int64_t ldc, hipStream_t stream)
+{
+    if (n <= 0) return;
+    if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
+    if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    if (ldc < q.rows) throw std::invalid_argument("the product's leading dimension is shorter than its rows");
+    const int64_t gx = (n + FBN - 1) / FBN;
+    if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
+    for (int64_t m0 = 0; m0 < q.rows; m0 += FBM) {
+        const int64_t rows = std::min<int64_t>(FBM, q.rows - m0);
+        if (rows > 64)
+            hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiSquare>), dim3((unsigned)gx), dim3(512), 0,
+                               stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, m0, Yd, P, 8, yscale, q.kdim,
+                               n, nullptr, nullptr, c, ldc, nullptr, nullptr);
+        else
+            hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 2, 5, 254, kEpiSquare>), dim3((unsigned)gx), dim3(256), 0,
+                               stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, m0, Yd, P, 8, yscale, q.kdim,
+                               n, nullptr, nullptr, c, ldc, nullptr, nullptr);
+        MPISDF_HIP(hipGetLastError());
+    }
+}
+
+void fold_rows_khatri_rao_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
+                             const double* cscale, double* y, int64_t ldy, double* ynorm2, unsigned long long* bad,
+                             hipStream_t stream)
+{
+    if (n <= 0) return;
+    if (q.rows % FBM) throw std::invalid_argument("Khatri-Rao row digits come in groups of 64 + 64 rows");
+    if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
+    if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    if (ldy < q.rows / 2) throw std::invalid_argument("the sketch's leading dimension is shorter than its rows");
+    const int64_t gx = (n + FBN - 1) / FBN;
+    if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
+    for (int64_t m0 = 0; m0 < q.rows; m0 += FBM) {
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiKhatriRao>), dim3((unsigned)gx), dim3(512), 0,
+                           stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, m0, Yd, P, 8, yscale, q.kdim, n,
+                           ynorm2, nullptr, y, ldy, cscale, bad);
+        MPISDF_HIP(hipGetLastError());
+    }
+}
+
+void fold_rows_i8(const FoldI8Rows& q, bool lower, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
+                  double* proj2, hipStream_t stream)
+{
+    if (n <= 0) return;
+    if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
+    // five Y planes: up to five pairs share an accumulator, 5 * 127^2 * ldq < 2^31
+    if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    const int64_t gx = (n + FBN - 1) / FBN;
+    if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
+    for (int64_t m0 = 0; m0 < q.rows; m0 += FBM) {
+        const int64_t rows = std::min<int64_t>(FBM, q.rows - m0);
+        // row t of a lower-triangular operand has no entries past t
+        const int64_t kext = lower ? std::min<int64_t>(q.kdim, m0 + rows) : q.kdim;
+        launch_fold_i8<5, 254>(rows, false, dim3((unsigned)gx), stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad,
+                               q.rows, m0, Yd, P, 8, yscale, kext, n, proj2, nullptr);
+    }
 }
 
 void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, double* ynorm2, double* scale,

@@ -113,6 +113,11 @@ void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, dou
 // out: of y_hat).
 void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
                           hipStream_t stream);
+// Five base-254 planes (within scale[j] 254^-5 / 2 ~ 4.8e-13 scale[j] of
+// each entry), the scale taken from each column's largest entry (found by the
+// kernel); ynorm2 (out) receives ||y_hat_j||^2.
+void digitize_columns_f64_max(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
+                              hipStream_t stream);
 constexpr int64_t kDigitRowsMaxF64 = 16384;  // digitize_columns_f64 holds one FP64 column in registers
 constexpr int64_t kDigitRowsMax = 36864;  // digitize_columns holds one FP32 column in registers
 constexpr int64_t kDigitSlice = 64;       // entries per slice; P is a multiple of it
@@ -148,5 +153,37 @@ constexpr double kFoldI8Margin = 1e-9;
 void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldqb, const int8_t* Yd,
                          int64_t P, int planes, int base, int colbytes, const double* yscale, double* proj2,
                          double* energy, FoldI8Scratch& scratch, hipStream_t stream);
+
+// The rows of a row-major FP64 matrix a (rows x kdim, leading dimension lda)
+// as the fold's Q operand: five base-254 digits per entry under one power of
+// two per row (split once, folded against many digit batches).
+struct FoldI8Rows {
+    DeviceBuffer<int8_t> digits;
+    DeviceBuffer<double> scales;
+    int64_t rows = 0, kdim = 0, bpad = 0, ldq = 0;
+};
+void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream);
+// proj2[j] += sum_t (a_t . y_hat_j)^2 over the rows t of q, for the n
+// base-254 digit columns of Yd (five planes in place of FP64 entries, from
+// digitize_columns_f64_max; pitch 8 P bytes, scales yscale[j]); the 15 digit
+// pairs p + r <= 4.  lower: a is lower triangular, and each 128-row group
+// folds only the entries up to its last row.  kdim <= 26624.
+void fold_rows_i8(const FoldI8Rows& q, bool lower, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
+                  double* proj2, hipStream_t stream);
+// c[t + j * ldc] = (a_t . y_hat_j)^2 for every row t of q (same digits and
+// limits as fold_rows_i8).
+void fold_rows_square_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n, double* c,
+                         int64_t ldc, hipStream_t stream);
+// Khatri-Rao products: q's rows come in groups of 128, rows 64 g + r of U
+// then the same rows of V (zero past the sketch), and
+//   y[s + j * ldy] = cscale[j] (u_s . y_hat_j)(v_s . y_hat_j),
+// ynorm2[j] += ||y_j||^2 (summed per group in group order), non-finite
+// entries counted into *bad.
+// The row digits fold_rows_khatri_rao_i8 takes, from u, v [d][ell] (u[a * ell + s]).
+void split_khatri_rao_rows_i8(const double* u, const double* v, int64_t d, int64_t ell, FoldI8Rows& out,
+                              hipStream_t stream);
+void fold_rows_khatri_rao_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
+                             const double* cscale, double* y, int64_t ldy, double* ynorm2, unsigned long long* bad,
+                             hipStream_t stream);
 
 }  // namespace mpisdf

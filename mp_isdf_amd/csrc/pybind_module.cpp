@@ -83,6 +83,7 @@ public:
     int64_t natm() const { return natm_; }
     bool cart() const { return cart_; }
     const mpisdf_context* get() const { return handle_.get(); }
+    mpisdf_context* get_mut() { return handle_.get(); }
 
     py::array_t<double> evaluate(const FloatArray& coords, bool gradient, const std::string& space) const
     {
@@ -251,7 +252,7 @@ public:
                                       "relative_tail", "blocks", "folds", "seconds_sketch", "seconds_qrcp",
                                       "seconds_total", "stalls", "refreshes", "fold_seconds", "fold_flops",
                                       "fold_launches", "sketch_kernel_seconds", "sketch_flops", "sketch_launches",
-                                      "fold_kind"};
+                                      "fold_kind", "sketch_int8_ops"};
         py::dict d;
         for (size_t i = 0; i < sizeof(names) / sizeof(names[0]); ++i) d[names[i]] = diag[i];
         // The audit's residuals (selected, dense QRCP) per objective, when it ran.
@@ -280,6 +281,8 @@ PYBIND11_MODULE(mp_isdf, m)
              py::arg("bas"), py::arg("env"), py::arg("cart") = false)
         .def_property_readonly("nao", &Context::nao)
         .def_property_readonly("cart", &Context::cart)
+        .def("release_workspace", [](Context& c) { check(mpisdf_context_release_workspace(c.get_mut())); },
+             "Give back the sketch storage the randomized selectors keep between calls.")
         .def("eval_ao", [](const Context& c, const FloatArray& coords, const std::string& space) {
             return c.evaluate(coords, false, space);
         }, py::arg("coords"), py::arg("space") = "auto", "Return AO values with shape (ngrid, nao).")
@@ -534,6 +537,57 @@ PYBIND11_MODULE(mp_isdf, m)
         return metric(left, right, coords, weights, false, false);
     }, py::arg("left"), py::arg("right"), py::arg("coords"), py::arg("weights"),
        "Grid Coulomb metric between two sets of interpolation vectors.");
+
+    // Integral-direct Cholesky metric (cpp/src/coulomb_metric_pybind.cpp:16-92).
+    auto cholesky_algorithm = [](const std::string& algorithm) -> int32_t {
+        if (algorithm == "unblocked_greedy") return 0;
+        if (algorithm == "unblocked_uniform") return 1;
+        throw py::value_error("unknown Cholesky algorithm");
+    };
+    m.def("cholesky_eri", [cholesky_algorithm](const Context& ctx, double tolerance, int64_t max_rank,
+                                               const std::string& algorithm, bool debug, uint64_t seed) {
+        const int32_t alg = cholesky_algorithm(algorithm);
+        mpisdf_eri_cholesky* h = nullptr;
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_cholesky_eri(ctx.get(), tolerance, max_rank, alg, seed, debug ? 1 : 0, &h);
+        }
+        check(status);
+        std::unique_ptr<mpisdf_eri_cholesky, int (*)(mpisdf_eri_cholesky*)> owned(h, mpisdf_eri_cholesky_destroy);
+        int64_t rank = 0, npair = 0;
+        check(mpisdf_eri_cholesky_size(h, &rank, &npair));
+        py::array_t<double> factor({(py::ssize_t)rank, (py::ssize_t)npair});
+        py::array_t<double> residual((py::ssize_t)npair);
+        py::array_t<py::ssize_t> pivots((py::ssize_t)rank);
+        std::vector<int64_t> piv((size_t)rank);
+        check(mpisdf_eri_cholesky_copy(h, factor.mutable_data(), piv.data(), residual.mutable_data()));
+        for (int64_t k = 0; k < rank; ++k) pivots.mutable_data()[k] = (py::ssize_t)piv[k];
+        return py::make_tuple(factor, pivots, residual);
+    }, py::arg("context"), py::arg("tolerance") = 1e-12, py::arg("max_rank") = 0,
+       py::arg("algorithm") = "unblocked_greedy", py::arg("debug") = false, py::arg("seed") = 0,
+       "Return (factors[rank,npair], zero-based pivots, residual diagonal) in packed AO-pair order.");
+    m.def("coulomb_metric", [cholesky_algorithm](const Context& ctx, const FloatArray& transform,
+                                                 const std::string& algorithm, double tolerance, int64_t max_rank,
+                                                 int64_t batch_size, bool debug, uint64_t seed) {
+        const int32_t alg = cholesky_algorithm(algorithm);
+        const int64_t nao = ctx.nao();
+        if (transform.ndim() != 2 || transform.shape(1) != nao * nao)
+            throw py::value_error("expected transform[nvec,nao*nao]");
+        const int64_t nvec = transform.shape(0);
+        py::array_t<double> result({(py::ssize_t)nvec, (py::ssize_t)nvec});
+        int status;
+        {
+            py::gil_scoped_release release;
+            status = mpisdf_coulomb_metric(ctx.get(), transform.data(), nvec, alg, tolerance, max_rank, batch_size,
+                                           debug ? 1 : 0, seed, result.mutable_data());
+        }
+        check(status);
+        return result;
+    }, py::arg("context"), py::arg("transform"), py::arg("algorithm") = "unblocked_greedy",
+       py::arg("tolerance") = 1e-12, py::arg("max_rank") = 0, py::arg("batch_size") = 64,
+       py::arg("debug") = false, py::arg("seed") = 0,
+       "AO-product Coulomb metric; transform uses ordered AO-pair columns. No anchor weights applied.");
 
     m.def("isdf_jk", [](const FloatArray& x, const FloatArray& z, const FloatArray& dm) {
         if (x.ndim() != 2) throw py::value_error("expected anchor AO values X[k,nao]");

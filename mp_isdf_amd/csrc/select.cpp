@@ -331,8 +331,10 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     sw.upload(hsw.data(), n, s);
     DeviceBuffer<unsigned long long> bad(1);
     MPISDF_HIP(hipMemsetAsync(bad.ptr, 0, sizeof(unsigned long long), s));
-    DeviceBuffer<double> y64;
-    DeviceBuffer<float> y32;
+    // Y lives in the context's sketch storage (held between selections).
+    std::unique_lock<std::mutex> store_lock(ctx.sketch_mutex);
+    struct { double* ptr = nullptr; } y64;
+    struct { float* ptr = nullptr; } y32;
     // The reduced-precision forms fold on INT8 digit planes of the FP32 sketch
     // (three or four digits per entry in place of each column), which needs a
     // pitch that is a multiple of 64 and ell within the digit kernels' limits.
@@ -346,8 +348,54 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     double t_sketch_kernel = 0, sketch_launches = 0;
     ynorm2.resize(n);
     MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
-    if (!lowp) {
-        y64.resize((size_t)ldy64 * n);
+    if (digits5) {
+        // The FP64 sketch from digit products: the rows of U and V (stacked,
+        // 64 of U then the same 64 of V) and each point's features as five
+        // base-254 digits under a power of two (within ~4.8e-13 of the row's
+        // or point's largest entry); the 15 digit-pair products p + r <= 4 run
+        // exactly in INT32 on INT8 MFMA and are combined in FP64, and the
+        // kernel's epilogue forms y = sqrt(h / ell) (u . z)(v . z), ||y||^2 and
+        // the non-finite count as the FP64 sketch does.
+        y64.ptr = static_cast<double*>(ctx.reserve_sketch((size_t)ldy64 * n * sizeof(double)));
+        const int64_t dpad = (d + kDigitSlice - 1) / kDigitSlice * kDigitSlice;
+        FoldI8Rows uv;
+        split_khatri_rao_rows_i8(u.ptr, v.ptr, d, ell, uv, s);
+        const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
+        DeviceBuffer<double> feat((size_t)batch * d), featp, fscale(batch), fnorm(batch), cscale(n);
+        if (dpad != d) {
+            featp.resize((size_t)batch * dpad);
+            MPISDF_HIP(hipMemsetAsync(featp.ptr, 0, featp.count * sizeof(double), s));  // pad columns stay zero
+        }
+        {
+            std::vector<double> hc(n);
+            const double inv = 1.0 / std::sqrt((double)ell);
+            for (int64_t g = 0; g < n; ++g) hc[g] = hsw[g] * inv;
+            cscale.upload(hc.data(), n, s);
+            MPISDF_HIP(hipStreamSynchronize(s));
+        }
+        LaunchTimer timer;
+        for (int64_t start = 0; start < n; start += batch) {
+            const int64_t count = std::min(batch, n - start);
+            evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
+                                 a.mo_coeff, a.nocc, feat.ptr, ws, s);
+            double* fp = feat.ptr;
+            if (dpad != d) {
+                MPISDF_HIP(hipMemcpy2DAsync(featp.ptr, dpad * sizeof(double), feat.ptr, d * sizeof(double),
+                                            d * sizeof(double), count, hipMemcpyDeviceToDevice, s));
+                fp = featp.ptr;
+            }
+            digitize_columns_f64_max(fp, d, count, dpad, fnorm.ptr, fscale.ptr, s);
+            timer.begin(s);
+            fold_rows_khatri_rao_i8(uv, reinterpret_cast<const int8_t*>(fp), dpad, fscale.ptr, count,
+                                    cscale.ptr + start, y64.ptr + start * ldy64, ldy64, ynorm2.ptr + start, bad.ptr,
+                                    s);
+            timer.end(s);
+        }
+        MPISDF_HIP(hipStreamSynchronize(s));
+        t_sketch_kernel = timer.seconds();
+        sketch_launches = (double)timer.launches();
+    } else if (!lowp) {
+        y64.ptr = static_cast<double*>(ctx.reserve_sketch((size_t)ldy64 * n * sizeof(double)));
         const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
         DeviceBuffer<double> feat((size_t)batch * d);
         SketchScratch scratch;
@@ -369,7 +417,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         const size_t esz = in_type == kFP8 ? 1 : 2;
         DeviceBuffer<uint8_t> ut((size_t)ell * dk * esz), vt((size_t)ell * dk * esz);
         prepare_omega_lowp(u.ptr, v.ptr, d, ell, dp, form, ut.ptr, vt.ptr, s);
-        y32.resize((size_t)ldy32 * n);
+        y32.ptr = static_cast<float*>(ctx.reserve_sketch((size_t)ldy32 * n * sizeof(float)));
         const bool ao_only = spec.kinds.size() == 1 && spec.kinds[0] == 0;
         LaunchTimer timer;
         const int64_t batch = std::min<int64_t>(n, 1 << 16);
@@ -530,6 +578,9 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         // MFMA work of the sketch: 4 ell d n per product term (three for the
         // split-BF16 form, two for the two-term FP8 form).
         diag[kDiagSketchFlops] = 4.0 * (double)ell * (double)d * (double)n * (lowp ? sketch_segments(form) : 1);
+        // fp64i8: the INT8 operations, 15 digit pairs x 2 (2 ell64) dpad per point
+        if (digits5)
+            diag[kDiagSketchOps] = 15.0 * 2.0 * (2.0 * (double)ldy64) * (double)((d + 63) / 64 * 64) * (double)n;
         diag[kDiagSketchLaunches] = sketch_launches;
         diag[kDiagFoldKind] = digits5 ? 4.0 : use_digits ? (planes == 4 ? 3.0 : 2.0) : (lowp ? 1.0 : 0.0);
     }

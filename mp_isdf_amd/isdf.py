@@ -59,7 +59,8 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                anchor_oversampling: int = 16, anchor_seed: int = 7, anchor_audit: bool = False,
                mo_coeff=None, nocc: int = 0,
                selection_only: bool = False, pair_space_error: bool = False, interp_vecs=None,
-               compute_metric: bool = False, metric_algorithm: str = "grid", sketch: str = "quadrature",
+               compute_metric: bool = False, metric_algorithm: str = "unblocked_greedy", metric_tol: float = 1e-12,
+               metric_rank: int = 0, metric_batch_size: int = 64, metric_seed: int = 0, sketch: str = "quadrature",
                point_importance=None, density_weight: str = "electron_density",
                ao_screening_threshold: float = 1e-24, cvt_weight: str = "electron_density",
                cvt_max_iter: int = 100, cvt_tol: float = 1e-5, rbrp_block_size: int = 16, rbrp_tau_block=None,
@@ -77,9 +78,14 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     error and the overlap check need no theta on the host.
 
     ``compute_metric=True`` adds the anchor Coulomb metric
-    ``anchor_weights[:, None] * grid_coulomb_metric(theta) * anchor_weights[None, :]``
-    as ``result["coulomb_metric"]`` (python/isdf.py:674-689); only the reference's
-    ``metric_algorithm="grid"`` is built.
+    ``anchor_weights[:, None] * raw * anchor_weights[None, :]`` as
+    ``result["coulomb_metric"]`` (python/isdf.py:674-689), with ``raw`` from
+    ``metric_algorithm``: ``"unblocked_greedy"`` (default) or
+    ``"unblocked_uniform"`` -- the integral-direct ERI Cholesky metric of the
+    fit's transform pinv(anchor pairs^T) / w_anchor (RBRP's transform for
+    ``rbrp_id``), ``mp_isdf.coulomb_metric`` with ``metric_tol``,
+    ``metric_rank``, ``metric_batch_size`` and ``metric_seed`` -- or
+    ``"grid"``, the grid Coulomb metric of theta.
 
     ``anchor_algorithm="rbrp"`` selects by robust blockwise random pivoting on
     the kept grid points; ``"rbrp_id"`` also takes RBRP's interpolative
@@ -137,10 +143,9 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
         raise ValueError(f"unknown metric algorithm: {metric_algorithm!r}")
     if selection_only and compute_metric:
         raise ValueError("selection_only requires compute_metric=False")
-    if compute_metric and metric_algorithm != "grid":
-        raise NotImplementedError(f"metric_algorithm={metric_algorithm!r} (integral-direct ERI Cholesky) is not "
-                                  "built; use metric_algorithm='grid'")
-    if compute_metric and interp_vecs is False:
+    if not isinstance(metric_seed, (int, np.integer)) or not 0 <= metric_seed < 2**64:
+        raise ValueError("metric_seed must be an unsigned 64-bit integer")
+    if compute_metric and metric_algorithm == "grid" and interp_vecs is False:
         raise ValueError("compute_metric needs the interpolation vectors (interp_vecs=True)")
     tolerance = 1e-10 if anchor_tol is None else float(anchor_tol)
 
@@ -226,7 +231,7 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
     record("ao_evaluation")
     want_theta = (bool(interp_vecs) if interp_vecs is not None
                   else len(anchor_indices) * len(coords) * 8 <= INTERP_VECS_AUTO_BYTES)
-    if compute_metric and not want_theta:
+    if compute_metric and metric_algorithm == "grid" and not want_theta:
         raise ValueError("compute_metric needs the interpolation vectors, which exceed 8 GiB here; pass "
                          "interp_vecs=True to form them on the host")
     if anchor_algorithm == "rbrp_id":
@@ -269,13 +274,32 @@ def build_isdf(mol: Molecule, *, level: int = 3, grid=None, anchor_rank: int = 0
                                                         max(np.linalg.norm(quadrature), 1e-30))
     record("overlap_check")
     if compute_metric:
-        raw = grid_coulomb_metric(result["interp_vecs"], coords, weights)
+        if metric_algorithm == "grid":
+            raw = grid_coulomb_metric(result["interp_vecs"], coords, weights)
+        else:
+            transform = (rbrp_result["transform"] if anchor_algorithm == "rbrp_id"
+                         else metric_transform(anchor_ao, anchor_weights))
+            raw = mp_isdf.coulomb_metric(context, np.ascontiguousarray(transform).reshape(len(anchor_ao), -1),
+                                         algorithm=metric_algorithm, tolerance=metric_tol, max_rank=metric_rank,
+                                         batch_size=metric_batch_size, debug=debug, seed=metric_seed)
         result["coulomb_metric"] = anchor_weights[:, None] * raw * anchor_weights[None, :]
         record("coulomb_metric")
     else:
         timings["coulomb_metric"] = 0.0
     result["elapsed"] = last - started
     return result
+
+
+def metric_transform(anchor_ao, anchor_weights):
+    """The refit's metric transform pinv(P_a^T) / w_anchor with P_a the anchor
+    AO pairs [k, nao * nao] (python/isdf.py:238-248: the truncated SVD, singular
+    values above 1e-12 of the largest), on the host as the reference forms it."""
+    a = np.asarray(anchor_ao, dtype=np.float64)
+    pairs = (a[:, :, None] * a[:, None, :]).reshape(len(a), -1)
+    u, s, vh = np.linalg.svd(pairs.T, full_matrices=False)
+    keep = s > (1e-12 * s[0] if len(s) else 0)
+    physical = (vh[keep].T / s[keep]) @ u[:, keep].T
+    return physical / np.asarray(anchor_weights, dtype=np.float64)[:, None]
 
 
 def grid_coulomb_metric(interp_vecs, coords, weights):

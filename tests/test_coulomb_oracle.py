@@ -169,8 +169,8 @@ def test_driver_metric_argument_validation():
         driver.build_isdf(mol, grid=grid, compute_metric=True, metric_algorithm="bogus")
     with pytest.raises(ValueError, match="selection_only"):
         driver.build_isdf(mol, grid=grid, selection_only=True, compute_metric=True)
-    with pytest.raises(NotImplementedError, match="grid"):
-        driver.build_isdf(mol, grid=grid, compute_metric=True, metric_algorithm="unblocked_greedy")
+    with pytest.raises(ValueError, match="metric_seed"):
+        driver.build_isdf(mol, grid=grid, compute_metric=True, metric_algorithm="unblocked_uniform", metric_seed=-1)
     with pytest.raises(ValueError, match="metric"):
         driver.evaluate_isdf_jk({"X1": np.ones((1, 2)), "coulomb_metric": None}, np.eye(2))
 

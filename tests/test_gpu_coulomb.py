@@ -131,7 +131,8 @@ def test_jk_matches_oracle(mp_isdf, k, nao):
 
 def test_c1_jk_against_oracle_and_exact_integrals(mp_isdf):
     mol = hydrogen_molecule("sto-3g")
-    result = driver.build_isdf(mol, level=3, anchor_rank=3, anchor_algorithm="qrcp_dense", compute_metric=True)
+    result = driver.build_isdf(mol, level=3, anchor_rank=3, anchor_algorithm="qrcp_dense", compute_metric=True,
+                               metric_algorithm="grid")
     assert len(result["anchor_indices"]) == 3
     assert result["timings"]["coulomb_metric"] >= 0
     theta, coords, weights = result["interp_vecs"], result["coordinates"], result["weights"]
@@ -169,7 +170,8 @@ def test_water_driver_metric_and_jk_match_oracle(mp_isdf):
     qrcp_dense anchors: the driver's metric over the real interpolation
     vectors and J/K of a symmetric density equal the oracle's."""
     mol = molecule.single_water("cc-pvdz")
-    result = driver.build_isdf(mol, level=2, anchor_rank=60, anchor_algorithm="qrcp_dense", compute_metric=True)
+    result = driver.build_isdf(mol, level=2, anchor_rank=60, anchor_algorithm="qrcp_dense", compute_metric=True,
+                               metric_algorithm="grid")
     theta, coords, weights = result["interp_vecs"], result["coordinates"], result["weights"]
     aw = result["anchor_weights"]
     want = aw[:, None] * oracle.grid_coulomb_metric(theta, coords, weights) * aw[None, :]

@@ -177,3 +177,23 @@ def test_h2o8_cholesky_scale(mp_isdf):
     # the factor rows reproduce the pivot columns' diagonals exactly at the pivots
     diag = np.sum(f[:, p] ** 2, axis=0)
     assert np.all(diag > 0)
+
+
+def test_driver_uses_ao_coefficients_and_anchor_weights(mp_isdf):
+    """build_isdf's default metric (unblocked_greedy) on H2 / STO-3G, level 0,
+    three anchors (python/test/test_coulomb_metric.py:156-170 restated): the
+    metric is W_a T V T^T W_a with T = pinv(W P W P^T) W P, W = diag(w_anchor),
+    P the anchor AO pairs, V the ERI tensor."""
+    from mp_isdf_amd import isdf as driver
+    mol = molecule.hydrogen_molecule("sto-3g")
+    result = driver.build_isdf(mol, level=0, anchor_rank=3, compute_metric=True)
+    anchor = result["X1"]
+    weighted = (anchor[:, :, None] * anchor[:, None, :]).reshape(len(anchor), -1)
+    weighted *= result["anchor_weights"][:, None]
+    transform = np.linalg.pinv(weighted @ weighted.T, rcond=1e-12) @ weighted
+    coeff = transform.reshape(len(anchor), mol.nao, mol.nao)
+    eri = packed_full(oracle.eri_packed(mol.atm, mol.bas, mol.env), mol.nao)
+    expected = np.einsum("aij,ijkl,bkl->ab", coeff, eri, coeff)
+    w = result["anchor_weights"]
+    np.testing.assert_allclose(result["coulomb_metric"], w[:, None] * expected * w[None, :], rtol=1e-10, atol=1e-10)
+    assert result["timings"]["coulomb_metric"] > 0

@@ -445,3 +445,29 @@ def test_fp64i8_selects_the_fp64_anchors(mp_isdf):
                                           oversampling=154, seed=7, precision=p) for p in ("fp64", "fp64i8")}
     assert got["fp64i8"][1]["fold_kind"] == 4 and got["fp64"][1]["fold_kind"] == 0
     assert got["fp64i8"][0] == got["fp64"][0]
+    # the digit-product sketch (five base-254 planes of U, V and phi) and its
+    # four-plane form for the QRCP stay within ~1e-10 of the FP64 sketch
+    n64, n8 = got["fp64"][1]["sketch_norm"], got["fp64i8"][1]["sketch_norm"]
+    print(f"\nsketch norm fp64 {n64:.15e}, fp64i8 {n8:.15e} ({n8 / n64 - 1:+.2e})")
+    assert abs(n8 / n64 - 1) <= 1e-9
+    assert got["fp64i8"][1]["sketch_int8_ops"] > 0 and got["fp64"][1]["sketch_int8_ops"] == 0
+
+
+@pytest.mark.parametrize("axes", [("ao",), ("ao", "dx")])
+def test_fp64i8_feature_widths_off_64(mp_isdf, axes):
+    """Feature widths that are not multiples of 64 (water cc-pVDZ: 24 AOs,
+    48 with dx): the digit sketch pads each point's features to 64 entries;
+    its anchors equal the oracle's FP64 rand_sketch."""
+    import oracle
+    mol = molecule.single_water("cc-pvdz")
+    rng = np.random.default_rng(5)
+    points = rng.normal(scale=1.5, size=(700, 3))
+    weights = rng.uniform(0.1, 1.0, size=700)
+    sel = mp_isdf.AnchorSelection(mp_isdf.LibcintContext(mol.atm, mol.bas, mol.env))
+    types = [getattr(mp_isdf.AxisType, {"ao": "AtomicOrbital", "dx": "AtomicOrbitalGradX"}[a]) for a in axes]
+    got = sel.select(points, weights, types if len(types) > 1 else types[0],
+                     algorithm=mp_isdf.LocalAnchorSelectionAlgorithm.RandomizedQuadratureKhatriRao, max_rank=40,
+                     oversampling=8, seed=7, precision="fp64i8")
+    want = oracle.select(mol.atm, mol.bas, mol.env, points, weights, list(axes), max_rank=40,
+                         algorithm="rand_sketch", oversampling=8, seed=7)
+    assert got == want
