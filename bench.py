@@ -86,6 +86,9 @@ def parse():
                    help="grid points of the Coulomb-metric leg (0 skips it)")
     p.add_argument("--metric-vectors", type=int, default=1536, help="interpolation vectors of the metric leg")
     p.add_argument("--metric-cpu-points", type=int, default=4096, help="grid points of the metric's CPU sample")
+    p.add_argument("--fold-probe", type=int, default=0,
+                   help="run only the standalone QRCP fold (FP64 columns of 13,200 rows on four base-254 INT8 "
+                        "digit planes, 128 pivots) on this many random columns, four times, for counter passes")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="finish with three device-to-device copies of this many MiB (a known-size kernel for "
                         "calibrating HBM byte counters under rocprofv3 --pmc)")
@@ -100,6 +103,39 @@ def cpu_model() -> str:
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def copy_probe(torch, mib):
+    """A known-size elementwise kernel (reads and writes mib MiB each, three
+    times) to calibrate the FETCH_SIZE / WRITE_SIZE counters against."""
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 2.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy probe: 3 x {mib} MiB read and written by an elementwise kernel")
+
+
+def fold_probe(args, torch, capi):
+    """The QRCP fold alone at the (H2O)64 sketch height: 128 orthonormal rows
+    against args.fold_probe random FP64 columns of 13,200 rows (digitised into
+    four base-254 planes, then folded on INT8 MFMA), four calls."""
+    m, n, b = 13200, args.fold_probe, 128
+    ld = (m + 63) // 64 * 64
+    torch.manual_seed(0)
+    y = torch.randn((n, ld), dtype=torch.float64, device="cuda")
+    y[:, m:] = 0
+    q, _ = torch.linalg.qr(torch.randn((m, b), dtype=torch.float64, device="cuda"))
+    qb = q.t().contiguous()
+    proj = torch.zeros(n, dtype=torch.float64, device="cuda")
+    for it in range(4):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        capi.call("mpisdf_qrcp_fold_device", b, n, m, qb, y, 5, ld, proj, None, None)
+        torch.cuda.synchronize()
+        log(f"fold probe {it}: {(time.perf_counter() - t) * 1e3:.2f} ms (digitise + fold, {n} columns)")
+    if args.copy_probe_mib > 0:
+        copy_probe(torch, args.copy_probe_mib)
 
 
 def main():
@@ -117,6 +153,9 @@ def main():
     mp = mp_isdf_amd.mp_isdf
     if not mp.device_enabled:
         raise SystemExit("no HIP device: the benchmark runs on the GPU only")
+    if args.fold_probe > 0:
+        fold_probe(args, torch, capi)
+        return
     nwater, basis, level, k = CONFIGS[args.config]
     oversampling = (k + 9) // 10
     ell = k + oversampling
@@ -537,12 +576,7 @@ def main():
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:
-        src = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy probe: 3 x {args.copy_probe_mib} MiB device-to-device")
+        copy_probe(torch, args.copy_probe_mib)
 
 
 if __name__ == "__main__":

@@ -1,7 +1,11 @@
-// QRCP fold on INT8 MFMA with digit splitting, for sketches stored as
-// signed base-128 digit planes.
+// FP64-accurate products on INT8 MFMA with digit splitting.  The QRCP fold
+// on sketches stored as signed digit planes,
 //
 //     W = Qb^T Y,  proj2[j] += sum_t W[t,j]^2  (energy[t] += sum_j W[t,j]^2)
+//
+// and, through the same kernel with other epilogues (below), the Khatri-Rao
+// sketch of FP64 point features and the fit's S = (Phi_a Phi^T)^2 and
+// Z = L^-1 S products, all in base 254.
 //
 // Y is held as NY = 3 or 4 digit planes per column (digitize_columns, in place
 // over the FP32 sketch, same footprint): with the power of two s_j >= 2 ||y_j||,
@@ -26,9 +30,9 @@
 // In base 254 the pairs p + r <= 4 run in five groups: 14 with four Y
 // planes (at most four pairs per group: |sum| < 4 * 127^2 * m < 2^31 for
 // m < 2^15), 15 with five (at most five: m <= 26624).
-// The dropped pairs (p + r >= 6 in base 128, >= 5 in base 254) and the truncation of q leave ~1e-11 of
-// ||y_j||^2 in the folded sums (measured), covered by the QRCP's relative
-// margin.
+// The dropped pairs (p + r >= 6 in base 128, >= 5 in base 254) and the
+// truncation of q leave ~1e-11 of ||y_j||^2 in the folded sums (measured),
+// covered by the QRCP's relative margin.
 //
 // v_mfma_i32_32x32x32_i8 runs at about 1.7x the rate of the 16x16x64 form on
 // gfx950.  Q and Y digit slices are staged through double-buffered LDS, with
@@ -46,9 +50,9 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 
 // Digit schemes.  Base 128 (|d| <= 64): q in five digits, Y in three or four
 // planes, the pairs p + r <= 5 in six accumulator groups.  Base 254
-// (|d| <= 127): q in five digits, Y in four planes (an FP64 sketch), the
-// pairs p + r <= 4 (14) in five groups; the dropped pairs are 254^-7 of
-// u_t s_j.
+// (|d| <= 127): q in five digits, Y in four planes (an FP64 sketch) or five
+// (point features, S), the pairs p + r <= 4 (14 or 15) in five groups; the
+// dropped pairs are 254^-7 of u_t s_j and smaller.
 template <int BASE>
 struct DigitScheme;
 template <>

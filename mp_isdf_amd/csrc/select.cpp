@@ -360,7 +360,10 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         const int64_t dpad = (d + kDigitSlice - 1) / kDigitSlice * kDigitSlice;
         FoldI8Rows uv;
         split_khatri_rao_rows_i8(u.ptr, v.ptr, d, ell, uv, s);
-        const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
+        // Every 128-row group re-reads the batch's feature digits: batches of
+        // 16,384 points keep them (~130 MB at 1536 features) in the
+        // last-level cache across the groups.
+        const int64_t batch = std::min({n, device_batch(d, ctx.nao, 0), int64_t(16384)});
         DeviceBuffer<double> feat((size_t)batch * d), featp, fscale(batch), fnorm(batch), cscale(n);
         if (dpad != d) {
             featp.resize((size_t)batch * dpad);
