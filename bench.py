@@ -407,6 +407,16 @@ def main():
                            "launches": sk_n, "launch_ms": round(sk_s / sk_n * 1e3, 3),
                            "work": f"{terms} x 4*ell*d*n = {terms} x 4*{ell}*{mol.nao}*{n} over {sk_n} grid batches "
                                    f"({terms} product term{'s' if terms > 1 else ''} per entry)"}
+    # The QRCP's candidate Gram-Schmidt: per block, H = Q^T R and R -= Q H over
+    # every candidate, again over the new ones (FP64 TN products).
+    gs_s, gs_f = d.get("gs_seconds", 0.0), d.get("gs_flops", 0.0)
+    roofline_gs = {"kernel": "tn_gemm_kernel (QRCP candidate Gram-Schmidt, FP64)", "bound": "mfma",
+                   "achieved": round(gs_f / gs_s / 1e12, 2) if gs_s else None,
+                   "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
+                   "frac": round(gs_f / gs_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if gs_s else None,
+                   "blocks": int(d.get("blocks", 0)), "flops": gs_f,
+                   "work": f"4*k*ell*cols per pass and block (two passes: every candidate, then the new ones), "
+                           f"k = pivots before the block, ell={ell}", "traffic": None}
     # HBM bytes per launch from the committed counter passes (FETCH_SIZE and
     # WRITE_SIZE, calibrated on a known-size copy) when they cover this workload.
     traffic_file = ROOT / "profiles" / "hbm_traffic_h2o64_rand_sketch.json"
@@ -570,6 +580,7 @@ def main():
         "roofline_fit": dict(roofline_fit, seconds_in_step=round(fit_s, 3)),
         "roofline_fit_s": dict(roofline_fit_s, seconds_in_step=round(s_s, 3)),
         "roofline_sketch": dict(roofline_sketch, seconds_in_step=round(sk_s, 3)),
+        "roofline_qrcp_gs": dict(roofline_gs, seconds_in_step=round(gs_s, 3)),
         "roofline_ao": roofline_ao,
         "cpu_baseline": cpu,
         "coulomb_metric": metric_leg,

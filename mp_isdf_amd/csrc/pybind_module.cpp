@@ -252,7 +252,7 @@ public:
                                       "relative_tail", "blocks", "folds", "seconds_sketch", "seconds_qrcp",
                                       "seconds_total", "stalls", "refreshes", "fold_seconds", "fold_flops",
                                       "fold_launches", "sketch_kernel_seconds", "sketch_flops", "sketch_launches",
-                                      "fold_kind", "sketch_int8_ops"};
+                                      "fold_kind", "sketch_int8_ops", "gs_seconds", "gs_flops"};
         py::dict d;
         for (size_t i = 0; i < sizeof(names) / sizeof(names[0]); ++i) d[names[i]] = diag[i];
         // The audit's residuals (selected, dense QRCP) per objective, when it ran.

@@ -608,8 +608,9 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     FoldI8Scratch i8_scratch;
     DeviceBuffer<double> fold_part;  // the FP64 fold's per-column-block energy partials
     const bool split = opt.split_bf16_fold && ytype == kF32;
-    // Fold launches are bracketed by HIP events; durations are read at the end.
-    LaunchTimer fold_timer;
+    // Fold launches and each block's candidate Gram-Schmidt are bracketed by
+    // HIP events; durations are read at the end.
+    LaunchTimer fold_timer, gs_timer;
     // A fold streams all of Y once per group of up to 128 pivots, so a block's
     // pivots are folded in whole groups of 128 and the rest (fewer than 128)
     // are carried into the next fold: the keys then miss those projections,
@@ -702,12 +703,15 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             // had their two passes when they entered).
             const int64_t ldh = (k + 3) / 4 * 4;
             if (H.count < (size_t)(ldh * Ceff)) H.resize((size_t)(ldh * Ceff));
+            gs_timer.begin(s);
             for (int pass = 0; pass < 2; ++pass) {
                 const int64_t cols = pass == 0 ? Ceff : nnew;
                 if (cols <= 0) continue;
                 gemm_tn_f64(k, cols, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
                 gemm_tn_f64(m, cols, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
+                res.gs_flops += 4.0 * (double)k * (double)m * (double)cols;
             }
+            gs_timer.end(s);
         }
         hipLaunchKernelGGL(cand_norm_kernel, dim3((unsigned)((Ceff + 3) / 4)), dim3(256), 0, s, rc.ptr, m, ldr, Ceff,
                            rn2.ptr);
@@ -823,6 +827,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     }
     MPISDF_HIP(hipStreamSynchronize(s));
     res.fold_seconds = fold_timer.seconds();
+    res.gs_seconds = gs_timer.seconds();
     return res;
 }
 

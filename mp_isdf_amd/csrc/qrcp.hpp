@@ -27,6 +27,8 @@ struct QrcpResult {
     double tail2 = 0.0;           // residual Frobenius^2 after the last pivot (if requested)
     double fold_seconds = 0.0;    // summed HIP-event duration of the fold launches
     double fold_flops = 0.0;      // their algorithmic flops, sum 2 b m n
+    double gs_seconds = 0.0;      // summed HIP-event duration of the candidate Gram-Schmidt products
+    double gs_flops = 0.0;        // their flops: 4 k m cols per pass (H = Q^T R and R -= Q H)
     int64_t blocks = 0, folds = 0, stalls = 0, refreshes = 0;
 };
 

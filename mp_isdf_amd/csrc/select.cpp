@@ -169,6 +169,8 @@ std::vector<int64_t> select_dense(Context& ctx, const SelectArgs& a, const AxisS
         diag[kDiagFoldSeconds] = r.fold_seconds;
         diag[kDiagFoldFlops] = r.fold_flops;
         diag[kDiagFoldLaunches] = (double)r.folds;
+        diag[kDiagGsSeconds] = r.gs_seconds;
+        diag[kDiagGsFlops] = r.gs_flops;
         diag[kDiagSecondsQrcp] = seconds_since(t0);
     }
     return out;
@@ -577,6 +579,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         diag[kDiagFoldSeconds] = r.fold_seconds;
         diag[kDiagFoldFlops] = r.fold_flops;
         diag[kDiagFoldLaunches] = (double)r.folds;
+        diag[kDiagGsSeconds] = r.gs_seconds;
+        diag[kDiagGsFlops] = r.gs_flops;
         diag[kDiagSketchSeconds] = t_sketch_kernel;
         // MFMA work of the sketch: 4 ell d n per product term (three for the
         // split-BF16 form, two for the two-term FP8 form).

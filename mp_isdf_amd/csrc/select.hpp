@@ -66,6 +66,8 @@ enum Diag : int {
     kDiagSketchLaunches = 19,
     kDiagFoldKind = 20,       // 0 FP64 MFMA on FP64 Y, 1 FP64 MFMA on FP32 Y, 2 / 3 INT8 MFMA on 3 / 4 base-128 planes, 4 on 4 base-254 planes
     kDiagSketchOps = 21,      // fp64i8: INT8 operations of the digit sketch (15 pairs x 2 (2 ell64) d64 per point); else 0
+    kDiagGsSeconds = 22,      // summed duration of the QRCP's candidate Gram-Schmidt products (HIP events)
+    kDiagGsFlops = 23,        // their FP64 flops: 4 k m cols per pass
     // Audit (SelectArgs::audit): relative projection residuals of the selected
     // and of the dense QRCP pivots, quadrature columns sqrt|w| pair_g, then,
     // for the density sketches, the sketch's own columns; -1 when not run.
