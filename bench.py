@@ -47,9 +47,9 @@ PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops":
         "hbm_gbs": 8000.0}
 # The INT8 fold splits q into 5 and y into 3 (4) signed base-128 digits and
 # runs the 14 (17) digit pairs with p + r <= 5 as INT8 products; for the FP64
-# sketch, q into 5 and y into 4 base-254 digits, the 14 pairs p + r <= 4.
+# sketch, q into 4 and y into 4 base-254 digits, the 13 pairs p + r <= 4.
 # Keyed by the selection's fold_kind.
-I8_PAIRS = {2: 14, 3: 17, 4: 14}
+I8_PAIRS = {2: 14, 3: 17, 4: 13}
 # Digit pairs p + r <= 4 of five base-254 planes on each operand (the fp64i8
 # sketch and the digit form of the fit's products).
 DIGIT_PAIRS_5 = 15

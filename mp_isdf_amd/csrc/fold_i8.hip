@@ -27,9 +27,10 @@
 // accumulator (at most four pairs: |sum| < 4 * 64^2 * m < 2^31 for m < 2^16),
 // and the six accumulators are combined in FP64:
 //     W = u_t s_j sum_g 128^-(g+2) acc_g.
-// In base 254 the pairs p + r <= 4 run in five groups: 14 with four Y
-// planes (at most four pairs per group: |sum| < 4 * 127^2 * m < 2^31 for
-// m < 2^15), 15 with five (at most five: m <= 26624).
+// In base 254 the pairs p + r <= 4 run in five groups: 13 for the QRCP fold
+// (q in four digits, four Y planes: at most four pairs per group,
+// |sum| < 4 * 127^2 * m < 2^31 for m < 2^15), 15 with five digits and five
+// planes (at most five: m <= 26624).
 // The dropped pairs (p + r >= 6 in base 128, >= 5 in base 254) and the
 // truncation of q leave ~1e-11 of ||y_j||^2 in the folded sums (measured),
 // covered by the QRCP's relative margin.
@@ -50,9 +51,10 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 
 // Digit schemes.  Base 128 (|d| <= 64): q in five digits, Y in three or four
 // planes, the pairs p + r <= 5 in six accumulator groups.  Base 254
-// (|d| <= 127): q in five digits, Y in four planes (an FP64 sketch) or five
-// (point features, S), the pairs p + r <= 4 (14 or 15) in five groups; the
-// dropped pairs are 254^-7 of u_t s_j and smaller.
+// (|d| <= 127): q in five digits (four in the QRCP fold's column sums, below),
+// Y in four planes (an FP64 sketch) or five (point features, S), the pairs
+// p + r <= 4 (13, 14 or 15) in five groups; the dropped pairs are 254^-7 of
+// u_t s_j and smaller.
 template <int BASE>
 struct DigitScheme;
 template <>
@@ -63,6 +65,17 @@ template <>
 struct DigitScheme<254> {
     static constexpr int ND = 5, NG = 5;
 };
+// The QRCP fold of an FP64 sketch (four base-254 Y planes) writes each q_t in
+// four digits: its truncation, within u_t 254^-4 / 2 per entry, perturbs
+// W[t,j] by ~u_t 254^-4 ||y_j|| / (2 sqrt 3) (independent roundings), and the
+// folded sums by a zero-mean error whose deviation over a whole
+// factorisation is below 2e-11 ||y_j||^2 (sum_t W[t,j]^2 <= ||y_j||^2), far
+// inside the QRCP's margin; the fifth digit's pair (4, 0) would cost a
+// fourteenth MFMA per K-step.  A row energy sum_j W[t,j]^2 reuses one rounded
+// q_t for every column, so its error does not average out (~1e-10 relative):
+// folds that return row energies, and the sketch and fit products, keep five
+// digits.
+constexpr int kFoldQDigits254 = 4;
 // Digit pairs (p, r) with p + r < NG, p < ND and r < NY: the MFMAs of one
 // K-step.
 template <int ND, int NY, int NG>
@@ -102,11 +115,10 @@ constexpr int kBufferWord3 = 0x00020000;
 // Q digits, slice-major: digit p of q_t[k] / u_t at
 // qd[((k / 64 * ND + p) * bpad + t) * 64 + k % 64] (one slice of every plane
 // and row is 64 ND bpad contiguous bytes), qs[t] = u_t.
-template <int BASE>
+template <int BASE, int ND = DigitScheme<BASE>::ND>
 __global__ void split_q_kernel(const double* __restrict__ qb, int64_t ldqb, int64_t m, int64_t b, int64_t ldq,
                                int64_t bpad, int8_t* __restrict__ qd, double* __restrict__ qs)
 {
-    constexpr int ND = DigitScheme<BASE>::ND;
     __shared__ double red[256];
     const int64_t t = blockIdx.x;
     double mx = 0;
@@ -301,7 +313,7 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // proj2[col] += the column's sum of squares and non-finite entries counted
 // into *bad.
 constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2;
-template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold>
+template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold, int ND = DigitScheme<BASE>::ND>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
                                                                    int64_t bpad, int64_t b, int64_t m0,
@@ -315,7 +327,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                                                                    const double* __restrict__ cscale = nullptr,
                                                                    unsigned long long* __restrict__ bad = nullptr)
 {
-    constexpr int ND = DigitScheme<BASE>::ND, NG = DigitScheme<BASE>::NG;
+    constexpr int NG = DigitScheme<BASE>::NG;
     constexpr int ROWS = 32 * NR;  // Q rows staged per slice
     constexpr int NTHREADS = 128 * NR;
     using L = SliceLayout<NR>;
@@ -615,7 +627,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 // rows: Q^T rows of this launch's group (1..128).  Groups of more than 64
 // rows run on 8-wave workgroups (NR = 4, one per CU); smaller groups on
 // 4-wave workgroups (NR = 2, two per CU), which multiply no zero rows.
-template <int NY, int BASE>
+template <int NY, int BASE, int ND = DigitScheme<BASE>::ND>
 void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const int8_t* qd, const double* qs,
                     int64_t ldq, int64_t bpad, int64_t b, int64_t m0, const int8_t* yd, int64_t P, int colbytes,
                     const double* yscale, int64_t m, int64_t n, double* proj2, double* en)
@@ -623,7 +635,7 @@ void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const i
     if (rows <= 0 || rows > FBM) throw std::logic_error("INT8 fold row group out of range");
     double* const e = energy ? en : nullptr;
 #define MPISDF_FOLD(E, NR)                                                                                    \
-    hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR, NY, BASE>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
+    hipLaunchKernelGGL((fold_i8_32x32_kernel<E, NR, NY, BASE, kEpiFold, ND>), grid, dim3(128 * NR), 0, s, qd, qs, ldq, bpad, b, m0, yd, P, \
                        colbytes, yscale, m, n, proj2, e)
     if (rows > 64) {
         if (energy) MPISDF_FOLD(true, 4);
@@ -798,12 +810,15 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     const int64_t bpad = (b + FBM - 1) / FBM * FBM;
     const int64_t ldq = (m + FBK - 1) / FBK * FBK;
-    const int nd = base == 254 ? DigitScheme<254>::ND : DigitScheme<128>::ND;
+    const int nd = base == 254 ? (energy ? DigitScheme<254>::ND : kFoldQDigits254) : DigitScheme<128>::ND;
     const size_t need = (size_t)(nd * bpad * ldq);
     if (need >= (size_t(1) << 31)) throw std::invalid_argument("the INT8 fold's Q digits exceed 2 GiB");
     if (scratch.digits.count < need) scratch.digits.resize(need);
     if (scratch.scales.count < (size_t)bpad) scratch.scales.resize(bpad);
-    if (base == 254)
+    if (base == 254 && !energy)
+        hipLaunchKernelGGL((split_q_kernel<254, kFoldQDigits254>), dim3((unsigned)bpad), dim3(256), 0, stream, Qb, ldqb,
+                           m, b, ldq, bpad, scratch.digits.ptr, scratch.scales.ptr);
+    else if (base == 254)
         hipLaunchKernelGGL(split_q_kernel<254>, dim3((unsigned)bpad), dim3(256), 0, stream, Qb, ldqb, m, b, ldq, bpad,
                            scratch.digits.ptr, scratch.scales.ptr);
     else
@@ -818,7 +833,11 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     // block order.
     for (int64_t m0 = 0; m0 < b; m0 += FBM) {
         const int64_t rows = std::min<int64_t>(FBM, b - m0);
-        if (base == 254)
+        if (base == 254 && !energy)
+            launch_fold_i8<4, 254, kFoldQDigits254>(rows, false, dim3((unsigned)gx), stream, scratch.digits.ptr,
+                                                    scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n,
+                                                    proj2, scratch.part.ptr);
+        else if (base == 254)
             launch_fold_i8<4, 254>(rows, energy != nullptr, dim3((unsigned)gx), stream, scratch.digits.ptr,
                                    scratch.scales.ptr, ldq, bpad, b, m0, Yd, P, colbytes, yscale, m, n, proj2,
                                    scratch.part.ptr);
