@@ -316,7 +316,7 @@ constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2;
 template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold, int ND = DigitScheme<BASE>::ND>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
-                                                                   int64_t bpad, int64_t b, int64_t m0,
+                                                                   int64_t bpad, int64_t b, int64_t mbase,
                                                                    const int8_t* __restrict__ yd, int64_t P,
                                                                    int colbytes,
                                                                    const double* __restrict__ yscale, int64_t m,
@@ -325,7 +325,9 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                                                                    double* __restrict__ cout = nullptr,
                                                                    int64_t ldc = 0,
                                                                    const double* __restrict__ cscale = nullptr,
-                                                                   unsigned long long* __restrict__ bad = nullptr)
+                                                                   unsigned long long* __restrict__ bad = nullptr,
+                                                                   int lower = 0,
+                                                                   double* __restrict__ colpart = nullptr)
 {
     constexpr int NG = DigitScheme<BASE>::NG;
     constexpr int ROWS = 32 * NR;  // Q rows staged per slice
@@ -343,6 +345,9 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
     const int half = lane >> 5, l32 = lane & 31;
     const int rt = wave >> 1, ct = wave & 1;
     const int64_t g0 = (int64_t)blockIdx.x * FBN;
+    // This workgroup's row group: a launch covers groups of ROWS rows from
+    // mbase along y.
+    const int64_t m0 = mbase + (int64_t)blockIdx.y * ROWS;
     const int64_t col = g0 + ct * 32 + l32;  // this lane's Y column
     i16v acc[NG];
 #pragma unroll
@@ -488,7 +493,9 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         __syncthreads();                                                                             \
     } while (0)
 
-    const int64_t nsl = (m + FBK - 1) / FBK;
+    // K extent: lower-triangular rows have no entries past the group's last row.
+    const int64_t mk = lower ? std::min<int64_t>(m, m0 + std::min<int64_t>(ROWS, b - m0)) : m;
+    const int64_t nsl = (mk + FBK - 1) / FBK;
     MPISDF_LOAD_Q(0, rqA);
     MPISDF_LOAD_YD(0, ryA);
     MPISDF_LOAD_Q(FBK, rqB);
@@ -583,7 +590,12 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         if (tid < FBN) {
             const int64_t c = g0 + tid;
             const double sum = colred[0][tid] + colred[1][tid];
-            if (c < n) proj2[c] += sum;  // row groups run in order
+            if (c < n) {
+                // one partial per row group when the launch batches them (added
+                // in group order afterwards); else row groups run in order
+                if (colpart) colpart[(int64_t)blockIdx.y * n + c] = sum;
+                else proj2[c] += sum;
+            }
         }
         return;
     }
@@ -614,7 +626,10 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
         double sum = 0;
 #pragma unroll
         for (int t = 0; t < NR; ++t) sum += colred[t][tid];
-        if (c < n) proj2[c] += sum;  // row groups run in order
+        if (c < n) {
+            if (colpart) colpart[(int64_t)blockIdx.y * n + c] = sum;  // added in group order afterwards
+            else proj2[c] += sum;  // row groups run in order
+        }
     }
     if (ENERGY && tid < ROWS) {
         // energy holds this launch's per-column-block partials, [block][rows]
@@ -719,6 +734,24 @@ void split_khatri_rao_rows_i8(const double* u, const double* v, int64_t d, int64
     MPISDF_HIP(hipStreamSynchronize(stream));  // before the stacked rows are freed
 }
 
+// The 128-row groups of q that run on 8-wave workgroups in one launch (grid.y
+// walks the groups): every group but a last one of at most 64 rows, which
+// runs on its own 4-wave launch.
+static int64_t batched_groups(int64_t rows)
+{
+    const int64_t g = rows / FBM + (rows % FBM > 64 ? 1 : 0);
+    if (g > 65535) throw std::overflow_error("too many 128-row groups for one launch");
+    return g;
+}
+
+// Room for one column partial per row group and column of a batched launch.
+static double* group_partials(const FoldI8Rows& q, int64_t groups, int64_t n)
+{
+    const size_t need = (size_t)(groups * n);
+    if (q.part.count < need) q.part.resize(need);
+    return q.part.ptr;
+}
+
 void fold_rows_square_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n, double* c,
                          int64_t ldc, hipStream_t stream)
 {
@@ -728,18 +761,16 @@ void fold_rows_square_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const
     if (ldc < q.rows) throw std::invalid_argument("the product's leading dimension is shorter than its rows");
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
-    for (int64_t m0 = 0; m0 < q.rows; m0 += FBM) {
-        const int64_t rows = std::min<int64_t>(FBM, q.rows - m0);
-        if (rows > 64)
-            hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiSquare>), dim3((unsigned)gx), dim3(512), 0,
-                               stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, m0, Yd, P, 8, yscale, q.kdim,
-                               n, nullptr, nullptr, c, ldc, nullptr, nullptr);
-        else
-            hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 2, 5, 254, kEpiSquare>), dim3((unsigned)gx), dim3(256), 0,
-                               stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, m0, Yd, P, 8, yscale, q.kdim,
-                               n, nullptr, nullptr, c, ldc, nullptr, nullptr);
-        MPISDF_HIP(hipGetLastError());
-    }
+    const int64_t g = batched_groups(q.rows);
+    if (g > 0)
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiSquare>), dim3((unsigned)gx, (unsigned)g),
+                           dim3(512), 0, stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, (int64_t)0, Yd, P,
+                           8, yscale, q.kdim, n, nullptr, nullptr, c, ldc, nullptr, nullptr, 0, nullptr);
+    if (g * FBM < q.rows)
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 2, 5, 254, kEpiSquare>), dim3((unsigned)gx), dim3(256), 0,
+                           stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, g * FBM, Yd, P, 8, yscale,
+                           q.kdim, n, nullptr, nullptr, c, ldc, nullptr, nullptr, 0, nullptr);
+    MPISDF_HIP(hipGetLastError());
 }
 
 void fold_rows_khatri_rao_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
@@ -753,12 +784,14 @@ void fold_rows_khatri_rao_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, c
     if (ldy < q.rows / 2) throw std::invalid_argument("the sketch's leading dimension is shorter than its rows");
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
-    for (int64_t m0 = 0; m0 < q.rows; m0 += FBM) {
-        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiKhatriRao>), dim3((unsigned)gx), dim3(512), 0,
-                           stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, m0, Yd, P, 8, yscale, q.kdim, n,
-                           ynorm2, nullptr, y, ldy, cscale, bad);
-        MPISDF_HIP(hipGetLastError());
-    }
+    const int64_t g = batched_groups(q.rows);
+    if (g <= 0) return;
+    double* part = group_partials(q, g, n);
+    hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiKhatriRao>), dim3((unsigned)gx, (unsigned)g),
+                       dim3(512), 0, stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, (int64_t)0, Yd, P, 8,
+                       yscale, q.kdim, n, ynorm2, nullptr, y, ldy, cscale, bad, 0, part);
+    MPISDF_HIP(hipGetLastError());
+    launch_sum_partials(part, g, n, ynorm2, true, stream);  // the groups' column sums in group order
 }
 
 void fold_rows_i8(const FoldI8Rows& q, bool lower, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
@@ -770,8 +803,18 @@ void fold_rows_i8(const FoldI8Rows& q, bool lower, const int8_t* Yd, int64_t P, 
     if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
-    for (int64_t m0 = 0; m0 < q.rows; m0 += FBM) {
-        const int64_t rows = std::min<int64_t>(FBM, q.rows - m0);
+    const int64_t g = batched_groups(q.rows);
+    if (g > 0) {
+        // each group folds its own K extent (lower: up to its last row)
+        double* part = group_partials(q, g, n);
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiFold>), dim3((unsigned)gx, (unsigned)g),
+                           dim3(512), 0, stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, (int64_t)0, Yd,
+                           P, 8, yscale, q.kdim, n, proj2, nullptr, nullptr, 0, nullptr, nullptr, lower ? 1 : 0, part);
+        MPISDF_HIP(hipGetLastError());
+        launch_sum_partials(part, g, n, proj2, true, stream);  // the groups' column sums in group order
+    }
+    if (g * FBM < q.rows) {
+        const int64_t m0 = g * FBM, rows = q.rows - m0;
         // row t of a lower-triangular operand has no entries past t
         const int64_t kext = lower ? std::min<int64_t>(q.kdim, m0 + rows) : q.kdim;
         launch_fold_i8<5, 254>(rows, false, dim3((unsigned)gx), stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad,

@@ -161,6 +161,8 @@ struct FoldI8Rows {
     DeviceBuffer<int8_t> digits;
     DeviceBuffer<double> scales;
     int64_t rows = 0, kdim = 0, bpad = 0, ldq = 0;
+    // per-row-group column sums of a launch that batches the row groups
+    mutable DeviceBuffer<double> part;
 };
 void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream);
 // proj2[j] += sum_t (a_t . y_hat_j)^2 over the rows t of q, for the n
