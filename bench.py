@@ -408,15 +408,20 @@ def main():
                            "work": f"{terms} x 4*ell*d*n = {terms} x 4*{ell}*{mol.nao}*{n} over {sk_n} grid batches "
                                    f"({terms} product term{'s' if terms > 1 else ''} per entry)"}
     # The QRCP's candidate Gram-Schmidt: per block, H = Q^T R and R -= Q H over
-    # every candidate, again over the new ones (FP64 TN products).
+    # every candidate, again over the new ones (FP64 TN products; for a digit
+    # sketch the first pass's Q^T R of new candidates on INT8 digits, counted
+    # at its FP64-equivalent flops).
     gs_s, gs_f = d.get("gs_seconds", 0.0), d.get("gs_flops", 0.0)
-    roofline_gs = {"kernel": "tn_gemm_kernel (QRCP candidate Gram-Schmidt, FP64)", "bound": "mfma",
+    gs_kernel = ("tn_gemm_kernel + fold_i8_32x32_kernel (QRCP candidate Gram-Schmidt: FP64, first-pass Q^T R on "
+                 "INT8 digits)" if int(d.get("fold_kind", 0)) == 4 else
+                 "tn_gemm_kernel (QRCP candidate Gram-Schmidt, FP64)")
+    roofline_gs = {"kernel": gs_kernel, "bound": "mfma",
                    "achieved": round(gs_f / gs_s / 1e12, 2) if gs_s else None,
                    "peak": PEAK["fp64_mfma_tflops"], "unit": "TFLOP/s",
                    "frac": round(gs_f / gs_s / 1e12 / PEAK["fp64_mfma_tflops"], 4) if gs_s else None,
                    "blocks": int(d.get("blocks", 0)), "flops": gs_f,
-                   "work": f"4*k*ell*cols per pass and block (two passes: every candidate, then the new ones), "
-                           f"k = pivots before the block, ell={ell}", "traffic": None}
+                   "work": f"4*k*ell*cols FP64-equivalent flops per pass and block (two passes: every candidate, "
+                           f"then the new ones), k = pivots before the block, ell={ell}", "traffic": None}
     # HBM bytes per launch from the committed counter passes (FETCH_SIZE and
     # WRITE_SIZE, calibrated on a known-size copy) when they cover this workload.
     traffic_file = ROOT / "profiles" / "hbm_traffic_h2o64_rand_sketch.json"

@@ -307,12 +307,13 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // zero digits; there are no branches in the MFMA sequence (a branch would
 // make the compiler drain every outstanding load at its join).
 // Epilogues: kEpiFold sums W^2 per column into proj2 (and per row into
-// energy); kEpiSquare stores C[row + col * ldc] = W^2; kEpiKhatriRao takes
+// energy); kEpiSquare stores C[row + col * ldc] = W^2 and kEpiStore
+// C[row + col * ldc] = W; kEpiKhatriRao takes
 // the group's rows 64..127 as the partners of rows 0..63 and stores
 // C[m0 / 2 + r + col * ldc] = cscale[col] W[r] W[64 + r] (r < 64), with
 // proj2[col] += the column's sum of squares and non-finite entries counted
 // into *bad.
-constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2;
+constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2, kEpiStore = 3;
 template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold, int ND = DigitScheme<BASE>::ND>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
@@ -517,7 +518,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 #undef MPISDF_STASH_Q
 
     // W[t][j] = u_t s_j sum_g BASE^-(g+2) acc_g.
-    if constexpr (EPI == kEpiSquare) {
+    if constexpr (EPI == kEpiSquare || EPI == kEpiStore) {
         // lane (l32, half) owns column col and, per e / 4, four consecutive rows
 #pragma unroll
         for (int q4 = 0; q4 < 4; ++q4) {
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                 for (int g = NG - 1; g >= 0; --g) v = v * (1.0 / BASE) + (double)acc[g][e];
                 const int64_t row = m0 + rt * 32 + 8 * q4 + 4 * half + i;
                 const double wv = row < b ? v * (1.0 / ((double)BASE * BASE)) * qs[row] * sj : 0.0;
-                w4[i] = wv * wv;
+                w4[i] = EPI == kEpiSquare ? wv * wv : wv;
             }
             const int64_t row0 = m0 + rt * 32 + 8 * q4 + 4 * half;
             if (cok && row0 < b) {
@@ -836,6 +837,77 @@ void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, dou
     else
         hipLaunchKernelGGL(digitize_columns_kernel<3>, dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream, y, m, P,
                            ynorm2, scale);
+    MPISDF_HIP(hipGetLastError());
+}
+
+// Candidate digit columns side by side: column c of out (4 P bytes) is the
+// first 4 P bytes of column cand[c] of Yd (pitch colbytes P), scale oscale[c].
+__global__ void gather_digit_columns_kernel(const int8_t* __restrict__ yd, int64_t P, int colbytes,
+                                            const double* __restrict__ yscale, const int64_t* __restrict__ cand,
+                                            int8_t* __restrict__ out, double* __restrict__ oscale)
+{
+    const int64_t c = blockIdx.x, j = cand[c];
+    const uint4* src = reinterpret_cast<const uint4*>(yd + j * colbytes * P);
+    uint4* dst = reinterpret_cast<uint4*>(out + c * 4 * P);
+    for (int64_t i = threadIdx.x; i < P / 4; i += blockDim.x) dst[i] = src[i];  // 4 P bytes in 16-byte chunks
+    if (threadIdx.x == 0) oscale[c] = yscale[j];
+}
+
+void q_digits_extend(const double* Q, int64_t ldq, int64_t m, int64_t k, int64_t kcap, QDigitStore& st,
+                     hipStream_t stream)
+{
+    if (m > 32767) throw std::invalid_argument("the base-254 digits of q need at most 32767 entries");
+    if (st.cap == 0) {
+        st.cap = std::max<int64_t>(kcap, 1);
+        st.bpad = (st.cap + FBM - 1) / FBM * FBM;
+        st.ldq = (m + FBK - 1) / FBK * FBK;
+        const size_t bytes = (size_t)(kFoldQDigits254 * st.bpad * st.ldq);
+        if (bytes >= (size_t(1) << 31)) throw std::invalid_argument("the digits of Q exceed 2 GiB");
+        st.digits.resize(bytes);
+        st.scales.resize((size_t)st.bpad);
+        // rows past the digitised ones read as zero digits
+        MPISDF_HIP(hipMemsetAsync(st.digits.ptr, 0, bytes, stream));
+        MPISDF_HIP(hipMemsetAsync(st.scales.ptr, 0, st.bpad * sizeof(double), stream));
+        st.rows = 0;
+    }
+    if (k > st.cap) throw std::invalid_argument("more pivot vectors than the digit store holds");
+    if (k <= st.rows) return;
+    // rows st.rows .. k-1: row t of this launch lands at st.rows + t of the layout
+    hipLaunchKernelGGL((split_q_kernel<254, kFoldQDigits254>), dim3((unsigned)(k - st.rows)), dim3(256), 0, stream,
+                       Q + st.rows * ldq, ldq, m, k - st.rows, st.ldq, st.bpad, st.digits.ptr + st.rows * FBK,
+                       st.scales.ptr + st.rows);
+    MPISDF_HIP(hipGetLastError());
+    st.rows = k;
+}
+
+void project_candidates_i8(QDigitStore& st, int64_t m, const int8_t* Yd, int64_t P, int colbytes,
+                           const double* yscale, const int64_t* cand, int64_t nc, double* H, int64_t ldh,
+                           hipStream_t stream)
+{
+    const int64_t k = st.rows;
+    if (k <= 0 || nc <= 0) return;
+    if (P % kDigitSlice || P < m || st.ldq > P) throw std::invalid_argument("candidate digit columns need a pitch >= the digits of q");
+    if (colbytes != 4 && colbytes != 8) throw std::invalid_argument("digit columns take 4 or 8 bytes per entry");
+    if (ldh < k) throw std::invalid_argument("the projection's leading dimension is shorter than the pivots");
+    if (nc > 0x7fffffff) throw std::overflow_error("too many candidate columns");
+    const size_t need = (size_t)(nc * 4 * P);
+    if (st.gathered.count < need) st.gathered.resize(need);
+    if (st.gscales.count < (size_t)nc) st.gscales.resize((size_t)nc);
+    hipLaunchKernelGGL(gather_digit_columns_kernel, dim3((unsigned)nc), dim3(256), 0, stream, Yd, P, colbytes, yscale,
+                       cand, st.gathered.ptr, st.gscales.ptr);
+    MPISDF_HIP(hipGetLastError());
+    const int64_t gx = (nc + FBN - 1) / FBN;
+    const int64_t g = batched_groups(k);
+    if (g > 0)
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 4, 254, kEpiStore, kFoldQDigits254>),
+                           dim3((unsigned)gx, (unsigned)g), dim3(512), 0, stream, st.digits.ptr, st.scales.ptr, st.ldq,
+                           st.bpad, k, (int64_t)0, st.gathered.ptr, P, 4, st.gscales.ptr, m, nc, nullptr, nullptr, H,
+                           ldh, nullptr, nullptr, 0, nullptr);
+    if (g * FBM < k)
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 2, 4, 254, kEpiStore, kFoldQDigits254>), dim3((unsigned)gx),
+                           dim3(256), 0, stream, st.digits.ptr, st.scales.ptr, st.ldq, st.bpad, k, g * FBM,
+                           st.gathered.ptr, P, 4, st.gscales.ptr, m, nc, nullptr, nullptr, H, ldh, nullptr, nullptr,
+                           0, nullptr);
     MPISDF_HIP(hipGetLastError());
 }
 

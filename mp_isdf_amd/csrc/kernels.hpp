@@ -154,6 +154,29 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
                          int64_t P, int planes, int base, int colbytes, const double* yscale, double* proj2,
                          double* energy, FoldI8Scratch& scratch, hipStream_t stream);
 
+// The QRCP's pivot vectors q_0 .. q_{rows-1} as INT8 digits (four base-254
+// digits per entry under one power of two per vector, the fold's layout with
+// room for cap vectors), extended as pivots are added; for products of Q^T
+// with candidate columns of a base-254 digit sketch.
+struct QDigitStore {
+    DeviceBuffer<int8_t> digits;
+    DeviceBuffer<double> scales;
+    int64_t rows = 0, cap = 0, bpad = 0, ldq = 0;
+    DeviceBuffer<int8_t> gathered;   // candidate digit columns of one product
+    DeviceBuffer<double> gscales;
+};
+// Digitise the columns rows .. k-1 of Q (m x k, leading dimension ldq, FP64)
+// into st (sized for kcap vectors on first use).
+void q_digits_extend(const double* Q, int64_t ldq, int64_t m, int64_t k, int64_t kcap, QDigitStore& st,
+                     hipStream_t stream);
+// H[t + c * ldh] = q_t . y_hat_{cand[c]} for t < st.rows, c < nc: Q^T times
+// candidate columns of the four-plane base-254 digit sketch Yd (column j at
+// byte offset colbytes P j, scales yscale[j]), on INT8 MFMA (13 digit pairs;
+// within ~1e-11 ||y_hat|| of the exact products, the truncation of q).
+void project_candidates_i8(QDigitStore& st, int64_t m, const int8_t* Yd, int64_t P, int colbytes,
+                           const double* yscale, const int64_t* cand, int64_t nc, double* H, int64_t ldh,
+                           hipStream_t stream);
+
 // The rows of a row-major FP64 matrix a (rows x kdim, leading dimension lda)
 // as the fold's Q operand: five base-254 digits per entry under one power of
 // two per row (split once, folded against many digit batches).

@@ -608,6 +608,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     FoldI8Scratch i8_scratch;
     DeviceBuffer<double> fold_part;  // the FP64 fold's per-column-block energy partials
     const bool split = opt.split_bf16_fold && ytype == kF32;
+    // Base-254 digit sketches: the pivot vectors' digits, kept for the first
+    // Gram-Schmidt pass of new candidates.
+    const bool digits254 = digits && digit_base_of(ytype) == 254;
+    QDigitStore qdigits;
     // Fold launches and each block's candidate Gram-Schmidt are bracketed by
     // HIP events; durations are read at the end.
     LaunchTimer fold_timer, gs_timer;
@@ -707,7 +711,22 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             for (int pass = 0; pass < 2; ++pass) {
                 const int64_t cols = pass == 0 ? Ceff : nnew;
                 if (cols <= 0) continue;
-                gemm_tn_f64(k, cols, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
+                if (pass == 0 && digits254 && nnew > 0) {
+                    // The new candidates' columns are their digit columns, so
+                    // Q^T R for them runs on the INT8 digits (within ~1e-11
+                    // ||y||, the truncation of q); the FP64 second pass then
+                    // removes that error along Q, and R -= Q H stays FP64 (its
+                    // error would fall outside span Q).  Persisting candidates
+                    // carry FP64 residuals.
+                    q_digits_extend(Q.ptr, ldq, m, k, kmax, qdigits, s);
+                    project_candidates_i8(qdigits, m, static_cast<const int8_t*>(y), ldy, digit_column_bytes(ytype),
+                                          opt.yscale, cand.ptr, nnew, H.ptr, ldh, s);
+                    if (Ceff > nnew)
+                        gemm_tn_f64(k, Ceff - nnew, m, 1.0, Q.ptr, ldq, rc.ptr + nnew * ldr, ldr, 0.0,
+                                    H.ptr + nnew * ldh, ldh, s);
+                } else {
+                    gemm_tn_f64(k, cols, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
+                }
                 gemm_tn_f64(m, cols, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
                 res.gs_flops += 4.0 * (double)k * (double)m * (double)cols;
             }
