@@ -307,13 +307,14 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // zero digits; there are no branches in the MFMA sequence (a branch would
 // make the compiler drain every outstanding load at its join).
 // Epilogues: kEpiFold sums W^2 per column into proj2 (and per row into
-// energy); kEpiSquare stores C[row + col * ldc] = W^2 and kEpiStore
-// C[row + col * ldc] = W; kEpiKhatriRao takes
+// energy); kEpiSquare stores C[row + col * ldc] = W^2, kEpiStore
+// C[row + col * ldc] = W and kEpiSubtract C[row + col * ldc] -= W;
+// kEpiKhatriRao takes
 // the group's rows 64..127 as the partners of rows 0..63 and stores
 // C[m0 / 2 + r + col * ldc] = cscale[col] W[r] W[64 + r] (r < 64), with
 // proj2[col] += the column's sum of squares and non-finite entries counted
 // into *bad.
-constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2, kEpiStore = 3;
+constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2, kEpiStore = 3, kEpiSubtract = 4;
 template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold, int ND = DigitScheme<BASE>::ND>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
@@ -518,7 +519,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
 #undef MPISDF_STASH_Q
 
     // W[t][j] = u_t s_j sum_g BASE^-(g+2) acc_g.
-    if constexpr (EPI == kEpiSquare || EPI == kEpiStore) {
+    if constexpr (EPI == kEpiSquare || EPI == kEpiStore || EPI == kEpiSubtract) {
         // lane (l32, half) owns column col and, per e / 4, four consecutive rows
 #pragma unroll
         for (int q4 = 0; q4 < 4; ++q4) {
@@ -538,7 +539,7 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                 double* dst = cout + row0 + col * ldc;
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    if (row0 + i < b) dst[i] = w4[i];
+                    if (row0 + i < b) dst[i] = EPI == kEpiSubtract ? dst[i] - w4[i] : w4[i];
             }
         }
         return;
@@ -769,6 +770,27 @@ void fold_rows_square_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const
                            8, yscale, q.kdim, n, nullptr, nullptr, c, ldc, nullptr, nullptr, 0, nullptr);
     if (g * FBM < q.rows)
         hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 2, 5, 254, kEpiSquare>), dim3((unsigned)gx), dim3(256), 0,
+                           stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, g * FBM, Yd, P, 8, yscale,
+                           q.kdim, n, nullptr, nullptr, c, ldc, nullptr, nullptr, 0, nullptr);
+    MPISDF_HIP(hipGetLastError());
+}
+
+void fold_rows_subtract_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
+                           double* c, int64_t ldc, hipStream_t stream)
+{
+    if (n <= 0) return;
+    if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
+    if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    if (ldc < q.rows) throw std::invalid_argument("the product's leading dimension is shorter than its rows");
+    const int64_t gx = (n + FBN - 1) / FBN;
+    if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
+    const int64_t g = batched_groups(q.rows);
+    if (g > 0)
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiSubtract>), dim3((unsigned)gx, (unsigned)g),
+                           dim3(512), 0, stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, (int64_t)0, Yd, P,
+                           8, yscale, q.kdim, n, nullptr, nullptr, c, ldc, nullptr, nullptr, 0, nullptr);
+    if (g * FBM < q.rows)
+        hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 2, 5, 254, kEpiSubtract>), dim3((unsigned)gx), dim3(256), 0,
                            stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, g * FBM, Yd, P, 8, yscale,
                            q.kdim, n, nullptr, nullptr, c, ldc, nullptr, nullptr, 0, nullptr);
     MPISDF_HIP(hipGetLastError());

@@ -188,6 +188,11 @@ struct FoldI8Rows {
     mutable DeviceBuffer<double> part;
 };
 void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream);
+// c[t + j * ldc] -= a_t . y_hat_j for the rows t of q and the n five-plane
+// base-254 digit columns of Yd (pitch 8 P bytes, scales yscale[j]): the 15
+// digit pairs p + r <= 4 (within ~1e-11 of |a_t| |y_j|).  kdim <= 26624.
+void fold_rows_subtract_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const double* yscale, int64_t n,
+                           double* c, int64_t ldc, hipStream_t stream);
 // proj2[j] += sum_t (a_t . y_hat_j)^2 over the rows t of q, for the n
 // base-254 digit columns of Yd (five planes in place of FP64 entries, from
 // digitize_columns_f64_max; pitch 8 P bytes, scales yscale[j]); the 15 digit

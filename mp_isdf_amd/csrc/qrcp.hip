@@ -612,6 +612,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     // Gram-Schmidt pass of new candidates.
     const bool digits254 = digits && digit_base_of(ytype) == 254;
     QDigitStore qdigits;
+    FoldI8Rows qt_rows;         // rows of Q as digits (second pass)
+    DeviceBuffer<double> hdig;  // norms and scales of H's digit columns
     // Fold launches and each block's candidate Gram-Schmidt are bracketed by
     // HIP events; durations are read at the end.
     LaunchTimer fold_timer, gs_timer;
@@ -705,7 +707,9 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             // Classical Gram-Schmidt, H = Q^T R, R -= Q H (via Qt): one pass over
             // every candidate, a second over the new ones (the persisting ones
             // had their two passes when they entered).
-            const int64_t ldh = (k + 3) / 4 * 4;
+            // (digit sketches: H's columns are digitised in place for the
+            // second pass, which needs a pitch of whole 64-entry slices)
+            const int64_t ldh = digits254 ? (k + kDigitSlice - 1) / kDigitSlice * kDigitSlice : (k + 3) / 4 * 4;
             if (H.count < (size_t)(ldh * Ceff)) H.resize((size_t)(ldh * Ceff));
             gs_timer.begin(s);
             for (int pass = 0; pass < 2; ++pass) {
@@ -727,7 +731,18 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                 } else {
                     gemm_tn_f64(k, cols, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
                 }
-                gemm_tn_f64(m, cols, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
+                if (pass == 1 && digits254 && ldh <= kDigitRowsMaxF64) {
+                    // H is the first pass's error along Q (~1e-11 ||y||), so
+                    // R -= Q H needs only a few digits of relative accuracy:
+                    // rows of Q and columns of H as base-254 digits.
+                    split_rows_i8(Qt.ptr, ldqt, m, k, qt_rows, s);
+                    if (hdig.count < (size_t)(2 * cols)) hdig.resize((size_t)(2 * cols));
+                    digitize_columns_f64_max(H.ptr, k, cols, ldh, hdig.ptr, hdig.ptr + cols, s);
+                    fold_rows_subtract_i8(qt_rows, reinterpret_cast<const int8_t*>(H.ptr), ldh, hdig.ptr + cols, cols,
+                                          rc.ptr, ldr, s);
+                } else {
+                    gemm_tn_f64(m, cols, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
+                }
                 res.gs_flops += 4.0 * (double)k * (double)m * (double)cols;
             }
             gs_timer.end(s);
