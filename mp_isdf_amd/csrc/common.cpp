@@ -20,6 +20,27 @@ void ensure_device()
     }
 }
 
+HipStream::HipStream() { MPISDF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+
+HipStream::~HipStream()
+{
+    if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+}
+
+HipEvent::HipEvent() { MPISDF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+
+HipEvent::~HipEvent()
+{
+    if (e) (void)hipEventDestroy(e);
+}
+
+void HipEvent::record(hipStream_t s) { MPISDF_HIP(hipEventRecord(e, s)); }
+
+void HipEvent::wait(hipStream_t s) const { MPISDF_HIP(hipStreamWaitEvent(s, e, 0)); }
+
 void LaunchTimer::begin(hipStream_t s)
 {
     hipEvent_t e0, e1;

@@ -112,6 +112,26 @@ struct DeviceBuffer {
     }
 };
 
+// A stream of the default priority, destroyed with its owner (after the work
+// queued on it has finished).
+struct HipStream {
+    hipStream_t s = nullptr;
+    HipStream();
+    ~HipStream();
+    HipStream(const HipStream&) = delete;
+    HipStream& operator=(const HipStream&) = delete;
+};
+// An event without timing, for ordering work across streams.
+struct HipEvent {
+    hipEvent_t e = nullptr;
+    HipEvent();
+    ~HipEvent();
+    HipEvent(const HipEvent&) = delete;
+    HipEvent& operator=(const HipEvent&) = delete;
+    void record(hipStream_t s);
+    void wait(hipStream_t s) const;  // s waits for the last record
+};
+
 // Durations of a sequence of launches on one stream, bracketed by HIP events
 // (begin/end pairs); read after the stream has been synchronised.
 struct LaunchTimer {

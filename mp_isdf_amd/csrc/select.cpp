@@ -366,10 +366,17 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         // 16,384 points keep them (~130 MB at 1536 features) in the
         // last-level cache across the groups.
         const int64_t batch = std::min({n, device_batch(d, ctx.nao, 0), int64_t(16384)});
-        DeviceBuffer<double> feat((size_t)batch * d), featp, fscale(batch), fnorm(batch), cscale(n);
-        if (dpad != d) {
-            featp.resize((size_t)batch * dpad);
-            MPISDF_HIP(hipMemsetAsync(featp.ptr, 0, featp.count * sizeof(double), s));  // pad columns stay zero
+        // Two batch buffers: the features of batch i + 1 and their digits are
+        // prepared on a second stream while batch i's digit products run.
+        DeviceBuffer<double> feat[2], featp[2], fscale[2], fnorm[2], cscale(n);
+        for (int b = 0; b < 2; ++b) {
+            feat[b].resize((size_t)batch * d);
+            fscale[b].resize(batch);
+            fnorm[b].resize(batch);
+            if (dpad != d) {
+                featp[b].resize((size_t)batch * dpad);
+                MPISDF_HIP(hipMemsetAsync(featp[b].ptr, 0, featp[b].count * sizeof(double), s));  // pad columns stay zero
+            }
         }
         {
             std::vector<double> hc(n);
@@ -378,23 +385,35 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             cscale.upload(hc.data(), n, s);
             MPISDF_HIP(hipStreamSynchronize(s));
         }
-        LaunchTimer timer;
-        for (int64_t start = 0; start < n; start += batch) {
-            const int64_t count = std::min(batch, n - start);
+        HipStream prep;
+        HipEvent ready[2], consumed[2];
+        const int64_t nbatch = (n + batch - 1) / batch;
+        auto digits_of = [&](int b) { return dpad != d ? featp[b].ptr : feat[b].ptr; };
+        auto prepare = [&](int64_t i) {
+            const int b = (int)(i & 1);
+            const int64_t start = i * batch, count = std::min(batch, n - start);
+            if (i >= 2) consumed[b].wait(prep.s);  // batch i - 2's products have read this buffer
             evaluate_axis_device(ctx, spec, d_xyz.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
-                                 a.mo_coeff, a.nocc, feat.ptr, ws, s);
-            double* fp = feat.ptr;
-            if (dpad != d) {
-                MPISDF_HIP(hipMemcpy2DAsync(featp.ptr, dpad * sizeof(double), feat.ptr, d * sizeof(double),
-                                            d * sizeof(double), count, hipMemcpyDeviceToDevice, s));
-                fp = featp.ptr;
-            }
-            digitize_columns_f64_max(fp, d, count, dpad, fnorm.ptr, fscale.ptr, s);
+                                 a.mo_coeff, a.nocc, feat[b].ptr, ws, prep.s);
+            if (dpad != d)
+                MPISDF_HIP(hipMemcpy2DAsync(featp[b].ptr, dpad * sizeof(double), feat[b].ptr, d * sizeof(double),
+                                            d * sizeof(double), count, hipMemcpyDeviceToDevice, prep.s));
+            digitize_columns_f64_max(digits_of(b), d, count, dpad, fnorm[b].ptr, fscale[b].ptr, prep.s);
+            ready[b].record(prep.s);
+        };
+        LaunchTimer timer;
+        prepare(0);
+        for (int64_t i = 0; i < nbatch; ++i) {
+            if (i + 1 < nbatch) prepare(i + 1);
+            const int b = (int)(i & 1);
+            const int64_t start = i * batch, count = std::min(batch, n - start);
+            ready[b].wait(s);
             timer.begin(s);
-            fold_rows_khatri_rao_i8(uv, reinterpret_cast<const int8_t*>(fp), dpad, fscale.ptr, count,
+            fold_rows_khatri_rao_i8(uv, reinterpret_cast<const int8_t*>(digits_of(b)), dpad, fscale[b].ptr, count,
                                     cscale.ptr + start, y64.ptr + start * ldy64, ldy64, ynorm2.ptr + start, bad.ptr,
                                     s);
             timer.end(s);
+            consumed[b].record(s);
         }
         MPISDF_HIP(hipStreamSynchronize(s));
         t_sketch_kernel = timer.seconds();
