@@ -153,24 +153,39 @@ __global__ __launch_bounds__(256) void kr_sketch_kernel(
         __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
         const uint8_t* P = sP[buf];
         if (FMT == 0) {
+            // phi operands of K-step st + 1 are read from LDS while step st's
+            // MFMAs run (one wave per SIMD: nothing else hides that latency)
+            bf8v b[2][4];
+#pragma unroll
+            for (int gt = 0; gt < 4; ++gt)
+                b[0][gt] = *reinterpret_cast<const bf8v*>(P + (gt * 32 + l32) * SK_PITCH + 64 * half);
 #pragma unroll
             for (int st = 0; st < 4; ++st) {
-                bf8v b[4];
+                if (st < 3) {
 #pragma unroll
-                for (int gt = 0; gt < 4; ++gt)
-                    b[gt] = *reinterpret_cast<const bf8v*>(P + (gt * 32 + l32) * SK_PITCH + 64 * half + 16 * st);
+                    for (int gt = 0; gt < 4; ++gt)
+                        b[(st + 1) & 1][gt] = *reinterpret_cast<const bf8v*>(P + (gt * 32 + l32) * SK_PITCH +
+                                                                            64 * half + 16 * (st + 1));
+                }
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     const bf8v au = __builtin_bit_cast(bf8v, ua[t][st]);
                     const bf8v av = __builtin_bit_cast(bf8v, va[t][st]);
 #pragma unroll
                     for (int gt = 0; gt < 4; ++gt) {
-                        accU[t][gt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(au, b[gt], accU[t][gt], 0, 0, 0);
-                        accV[t][gt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[gt], accV[t][gt], 0, 0, 0);
+                        accU[t][gt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(au, b[st & 1][gt], accU[t][gt], 0, 0, 0);
+                        accV[t][gt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b[st & 1][gt], accV[t][gt], 0, 0, 0);
                     }
                     MPISDF_LOAD_UV(knext, t, st);
-                    __builtin_amdgcn_sched_barrier(0);
                 }
+                // next step's reads first, then MFMAs with this step's loads between them
+                if (st < 3) __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+                for (int i_ = 0; i_ < 4; ++i_) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         } else {
 #pragma unroll
