@@ -108,7 +108,8 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
                                                           double* __restrict__ colacc, double* __restrict__ rowacc,
                                                           const double* __restrict__ A2,
                                                           const double* __restrict__ cscale,
-                                                          unsigned long long* __restrict__ bad, int tri)
+                                                          unsigned long long* __restrict__ bad, int tri,
+                                                          int64_t kchunk, int64_t czstride)
 {
     __shared__ __align__(16) double sA[2][TBM * LDA_S];
     __shared__ double red[4][TBM];
@@ -167,8 +168,13 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
 
     // Triangular A (tri = 1 lower, 2 upper): this row group's nonzero columns
     // are k < m0 + 128 (lower) or k >= m0 (upper); the rest of K is skipped.
-    const int64_t k_lo = tri == 2 ? m0 : 0;
-    const int64_t k_hi = tri == 1 ? std::min<int64_t>(K, m0 + TBM) : K;
+    // Split K (kchunk > 0, a multiple of TBK): workgroup z takes K chunk z and
+    // stores its partial product at C + z czstride.
+    const int64_t kz0 = kchunk > 0 ? (int64_t)blockIdx.z * kchunk : 0;
+    const int64_t kz1 = kchunk > 0 ? kz0 + kchunk : K;
+    if (kchunk > 0) C += (int64_t)blockIdx.z * czstride;
+    const int64_t k_lo = std::max<int64_t>(tri == 2 ? m0 : 0, kz0);
+    const int64_t k_hi = std::min<int64_t>(tri == 1 ? std::min<int64_t>(K, m0 + TBM) : K, kz1);
     const int64_t nslices = k_hi > k_lo ? (k_hi - k_lo + TBK - 1) / TBK : 0;
     if (nslices > 0) {
         load_a(k_lo);
@@ -342,11 +348,26 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
     }
 }
 
+// C[i + j ldc] = (accumulate ? C : 0) + sum_z part[z zstride + i + j ldc] for
+// i < M, j < N, z ascending (the rows between M and ldc are left alone).
+__global__ void sum_split_partials_kernel(const double* __restrict__ part, int64_t splits, int64_t zstride,
+                                          int64_t M, int64_t N, double* __restrict__ C, int64_t ldc, int accumulate)
+{
+    const int64_t total = M * N;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = idx / M, off = idx - j * M + j * ldc;
+        double v = accumulate ? C[off] : 0.0;
+        for (int64_t z = 0; z < splits; ++z) v += part[z * zstride + off];
+        C[off] = v;
+    }
+}
+
 template <class TB, int EPI>
 void launch_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, const TB* B, int64_t ldb, double alpha,
                double beta, double* C, int64_t ldc, double* colacc, double* rowacc, hipStream_t stream,
                const double* A2 = nullptr, const double* cscale = nullptr, unsigned long long* bad = nullptr,
-               int tri = 0)
+               int tri = 0, int64_t splits = 1, int64_t kchunk = 0, int64_t czstride = 0)
 {
     if (M <= 0 || N <= 0) return;
     const int64_t rows = EPI == kTnKhatriRao ? TBM / 2 : TBM;
@@ -357,12 +378,13 @@ void launch_tn(int64_t M, int64_t N, int64_t K, const double* A, int64_t lda, co
     const bool vec = lda % 4 == 0 && ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(A) % 32) == 0 &&
                      (reinterpret_cast<uintptr_t>(A2) % 32) == 0 &&
                      (reinterpret_cast<uintptr_t>(B) % (4 * sizeof(TB))) == 0;
+    const dim3 grid((unsigned)gx, (unsigned)gy, (unsigned)splits);
     if (vec)
-        hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
-                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad, tri);
+        hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, true>), grid, dim3(256), 0, stream, M, N, std::max<int64_t>(K, 0),
+                           A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad, tri, kchunk, czstride);
     else
-        hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, false>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, stream, M,
-                           N, std::max<int64_t>(K, 0), A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad, tri);
+        hipLaunchKernelGGL((tn_gemm_kernel<TB, EPI, false>), grid, dim3(256), 0, stream, M, N, std::max<int64_t>(K, 0),
+                           A, lda, B, ldb, alpha, beta, C, ldc, colacc, rowacc, A2, cscale, bad, tri, kchunk, czstride);
     MPISDF_HIP(hipGetLastError());
 }
 
@@ -398,6 +420,38 @@ void gemm_tn_f64_fused(TnFused mode, int64_t M, int64_t N, int64_t K, double alp
 }
 
 int64_t tn_column_blocks(int64_t N) { return (N + TBN - 1) / TBN; }
+
+void gemm_tn_f64_split(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
+                       int64_t ldb, double beta, double* C, int64_t ldc, DeviceBuffer<double>& part,
+                       hipStream_t stream)
+{
+    if (M <= 0 || N <= 0) return;
+    if (beta != 0.0 && beta != 1.0) throw std::invalid_argument("split-K products take beta 0 or 1");
+    // Workgroups of the plain product (two fit on a CU); split K until the
+    // grid covers about two rounds of the chip, in chunks of whole slices.
+    int dev = 0, cus = 0;
+    MPISDF_HIP(hipGetDevice(&dev));
+    MPISDF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t tiles = ((N + TBN - 1) / TBN) * ((M + TBM - 1) / TBM);
+    const int64_t slots = 2 * std::max(cus, 1);
+    int64_t splits = std::min<int64_t>(16, std::max<int64_t>(1, 2 * slots / std::max<int64_t>(tiles, 1)));
+    const int64_t kchunk = ((K + splits - 1) / splits + TBK - 1) / TBK * TBK;
+    splits = kchunk > 0 ? (K + kchunk - 1) / kchunk : 1;
+    if (splits <= 1) {
+        gemm_tn_f64(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, stream);
+        return;
+    }
+    // partial z: alpha A^T B over K chunk z (M x N, leading dimension ldc);
+    // C = beta C + the partials in chunk order
+    const int64_t zstride = ldc * N;
+    if (part.count < (size_t)(splits * zstride)) part.resize((size_t)(splits * zstride));
+    launch_tn<double, kTnStore>(M, N, K, A, lda, B, ldb, alpha, 0.0, part.ptr, ldc, nullptr, nullptr, stream, nullptr,
+                                nullptr, nullptr, 0, splits, kchunk, zstride);
+    const int64_t total = M * N;
+    hipLaunchKernelGGL(sum_split_partials_kernel, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)),
+                       dim3(256), 0, stream, part.ptr, splits, zstride, M, N, C, ldc, beta == 1.0 ? 1 : 0);
+    MPISDF_HIP(hipGetLastError());
+}
 
 void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t ldq, const void* Y, bool y_f32,
                       int64_t ldy, double* proj2, double* energy, DeviceBuffer<double>& part, hipStream_t stream)

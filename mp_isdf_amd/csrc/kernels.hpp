@@ -67,6 +67,14 @@ void gemm_tn_f64_fused(TnFused mode, int64_t M, int64_t N, int64_t K, double alp
 
 int64_t tn_column_blocks(int64_t N);
 
+// gemm_tn_f64 (no triangular skip; beta 0 or 1) with K split over the grid
+// when the product has too few 128 x 128 tiles to fill the chip: partial
+// products over K chunks in ``part``, added to C in chunk order (entries
+// i < M, j < N of C only).
+void gemm_tn_f64_split(int64_t M, int64_t N, int64_t K, double alpha, const double* A, int64_t lda, const double* B,
+                       int64_t ldb, double beta, double* C, int64_t ldc, DeviceBuffer<double>& part,
+                       hipStream_t stream);
+
 // FP64 Khatri-Rao sketch on the TN kernel: ut/vt are ell x d row-major
 // (ut[s * d + a]), phi[g * ldphi + a];  y[g * ldy + s] = scale[g] (ut_s . phi_g)
 // (vt_s . phi_g) for g < count, ynorm2[g] += ||y_g||^2 (when given; summed

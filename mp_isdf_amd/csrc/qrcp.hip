@@ -613,6 +613,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     const bool digits254 = digits && digit_base_of(ytype) == 254;
     QDigitStore qdigits;
     FoldI8Rows qt_rows;         // rows of Q as digits (second pass)
+    DeviceBuffer<double> gs_part;  // split-K partials of the FP64 Gram-Schmidt products
     DeviceBuffer<double> hdig;  // norms and scales of H's digit columns
     // Fold launches and each block's candidate Gram-Schmidt are bracketed by
     // HIP events; durations are read at the end.
@@ -729,7 +730,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                         gemm_tn_f64(k, Ceff - nnew, m, 1.0, Q.ptr, ldq, rc.ptr + nnew * ldr, ldr, 0.0,
                                     H.ptr + nnew * ldh, ldh, s);
                 } else {
-                    gemm_tn_f64(k, cols, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, s);
+                    gemm_tn_f64_split(k, cols, m, 1.0, Q.ptr, ldq, rc.ptr, ldr, 0.0, H.ptr, ldh, gs_part, s);
                 }
                 if (pass == 1 && digits254 && ldh <= kDigitRowsMaxF64) {
                     // H is the first pass's error along Q (~1e-11 ||y||), so
@@ -741,7 +742,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
                     fold_rows_subtract_i8(qt_rows, reinterpret_cast<const int8_t*>(H.ptr), ldh, hdig.ptr + cols, cols,
                                           rc.ptr, ldr, s);
                 } else {
-                    gemm_tn_f64(m, cols, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, s);
+                    gemm_tn_f64_split(m, cols, k, -1.0, Qt.ptr, ldqt, H.ptr, ldh, 1.0, rc.ptr, ldr, gs_part, s);
                 }
                 res.gs_flops += 4.0 * (double)k * (double)m * (double)cols;
             }
