@@ -964,6 +964,48 @@ void fold_projections_i8(int64_t b, int64_t n, int64_t m, const double* Qb, int6
     MPISDF_HIP(hipGetLastError());
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
+    if (!energy && b > FBM) {
+        // Many row groups (a fold of every pivot so far): the groups of more
+        // than 64 rows in one launch, each writing its column partial, added
+        // in group order, then a last group of at most 64 rows; the columns in
+        // chunks that bound the partials.
+        const int64_t g = batched_groups(b);
+        const int64_t chunk = std::max<int64_t>(FBN, (int64_t(1) << 25) / std::max<int64_t>(g, 1) / FBN * FBN);
+        const size_t need_part = (size_t)(g * std::min(chunk, n));
+        if (scratch.colpart.count < need_part) scratch.colpart.resize(need_part);
+        for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+            const int64_t nc = std::min(chunk, n - c0);
+            const int8_t* yc = Yd + c0 * colbytes * P;
+            const dim3 grid((unsigned)((nc + FBN - 1) / FBN), (unsigned)g);
+#define MPISDF_FOLD_BATCH(NY, BASE, ND)                                                                          \
+    hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, NY, BASE, kEpiFold, ND>), grid, dim3(512), 0, stream,       \
+                       scratch.digits.ptr, scratch.scales.ptr, ldq, bpad, b, (int64_t)0, yc, P, colbytes,          \
+                       yscale + c0, m, nc, proj2 + c0, nullptr, nullptr, 0, nullptr, nullptr, 0, scratch.colpart.ptr)
+            if (g > 0) {
+                if (base == 254) MPISDF_FOLD_BATCH(4, 254, kFoldQDigits254);
+                else if (planes == 4) MPISDF_FOLD_BATCH(4, 128, 5);
+                else MPISDF_FOLD_BATCH(3, 128, 5);
+                MPISDF_HIP(hipGetLastError());
+                launch_sum_partials(scratch.colpart.ptr, g, nc, proj2 + c0, true, stream);
+            }
+#undef MPISDF_FOLD_BATCH
+            if (g * FBM < b) {
+                const int64_t m0 = g * FBM, rows = b - m0;
+                const dim3 g1((unsigned)((nc + FBN - 1) / FBN));
+                if (base == 254)
+                    launch_fold_i8<4, 254, kFoldQDigits254>(rows, false, g1, stream, scratch.digits.ptr,
+                                                            scratch.scales.ptr, ldq, bpad, b, m0, yc, P, colbytes,
+                                                            yscale + c0, m, nc, proj2 + c0, nullptr);
+                else if (planes == 4)
+                    launch_fold_i8<4, 128>(rows, false, g1, stream, scratch.digits.ptr, scratch.scales.ptr, ldq, bpad,
+                                           b, m0, yc, P, colbytes, yscale + c0, m, nc, proj2 + c0, nullptr);
+                else
+                    launch_fold_i8<3, 128>(rows, false, g1, stream, scratch.digits.ptr, scratch.scales.ptr, ldq, bpad,
+                                           b, m0, yc, P, colbytes, yscale + c0, m, nc, proj2 + c0, nullptr);
+            }
+        }
+        return;
+    }
     if (energy && scratch.part.count < (size_t)(gx * FBM)) scratch.part.resize((size_t)(gx * FBM));
     // One launch per group of 128 rows (Q^T rows), each with its own tile
     // count; the row energies of a group are summed over column blocks in

@@ -38,7 +38,7 @@ constexpr int TBK = 32;        // K per slice
 constexpr int LDA_S = TBK + 4;
 
 enum TnEpilogue { kTnStore = 0, kTnFold = 1, kTnFoldEnergy = 2, kTnKhatriRao = 3,
-                  kTnStoreSquare = 4, kTnStoreColSq = 5, kTnStoreRowDot = 6 };
+                  kTnStoreSquare = 4, kTnStoreColSq = 5, kTnStoreRowDot = 6, kTnFoldPart = 7 };
 
 template <class TB>
 struct BVec2;
@@ -324,7 +324,12 @@ __global__ __launch_bounds__(256, 2) void tn_gemm_kernel(int64_t M, int64_t N, i
         s += __shfl_xor(s, 16);
         s += __shfl_xor(s, 32);
         const int64_t gn = n0 + 16 * c + l16;
-        if (quarter == 0 && gn < N) colacc[gn] += s;
+        if (EPI == kTnFoldPart) {
+            // one partial per row group (launches of many groups), added in order later
+            if (quarter == 0 && gn < N) colacc[(int64_t)blockIdx.y * N + gn] = s;
+        } else if (quarter == 0 && gn < N) {
+            colacc[gn] += s;
+        }
     }
     if (EPI == kTnFoldEnergy) {
         // Row sums of W^2 over the workgroup's columns: lanes of one quarter
@@ -461,6 +466,26 @@ void fold_projections(int64_t b, int64_t n, int64_t m, const double* Qb, int64_t
     // per launch and the groups add in order; the energies are written as
     // per-column-block partials and added in block order.
     if (b <= 0 || n <= 0) return;
+    if (!energy && b > TBM) {
+        // Many row groups (a fold of every pivot so far): all groups in one
+        // launch, each writing its column partial, added in group order; the
+        // columns in chunks that bound the partials.
+        const int64_t groups = (b + TBM - 1) / TBM;
+        if (groups > 65535) throw std::overflow_error("too many pivot groups for one fold launch");
+        const int64_t chunk = std::max<int64_t>(TBN, (int64_t(1) << 25) / groups / TBN * TBN);
+        if (part.count < (size_t)(groups * std::min(chunk, n))) part.resize((size_t)(groups * std::min(chunk, n)));
+        for (int64_t c0 = 0; c0 < n; c0 += chunk) {
+            const int64_t nc = std::min(chunk, n - c0);
+            if (y_f32)
+                launch_tn<float, kTnFoldPart>(b, nc, m, Qb, ldq, static_cast<const float*>(Y) + c0 * ldy, ldy, 1.0, 0.0,
+                                              nullptr, 0, part.ptr, nullptr, stream);
+            else
+                launch_tn<double, kTnFoldPart>(b, nc, m, Qb, ldq, static_cast<const double*>(Y) + c0 * ldy, ldy, 1.0,
+                                               0.0, nullptr, 0, part.ptr, nullptr, stream);
+            launch_sum_partials(part.ptr, groups, nc, proj2 + c0, true, stream);
+        }
+        return;
+    }
     const int64_t gx = tn_column_blocks(n);
     if (energy && part.count < (size_t)(gx * TBM)) part.resize((size_t)(gx * TBM));
     for (int64_t r0 = 0; r0 < b; r0 += TBM) {

@@ -150,7 +150,8 @@ __host__ __device__ inline int64_t digit_offset(int64_t i, int r, int planes)
 struct FoldI8Scratch {
     DeviceBuffer<int8_t> digits;
     DeviceBuffer<double> scales;
-    DeviceBuffer<double> part;  // per-column-block row energies of one launch
+    DeviceBuffer<double> part;     // per-column-block row energies of one launch
+    DeviceBuffer<double> colpart;  // per-row-group column sums of a fold of many groups
 };
 // Relative bound (of ||y_j||^2) on the error a folded sum accumulates over a
 // whole factorisation with the INT8 fold; enters the keys as a margin.

@@ -76,13 +76,17 @@ __global__ void colnorm2_kernel(const T* __restrict__ y, int64_t m, int64_t n, i
 
 // Upper bound of the residual norm^2 of every unselected column: folded value
 // plus the fold's accumulated error bound plus a relative margin.
+// With tie ranks the entries are laid out in rank order, so the stable sort
+// puts equal keys in rank order.
 __global__ void keys_kernel(const double* __restrict__ ynorm2, const double* __restrict__ proj2,
                             const double* __restrict__ err2, const uint8_t* __restrict__ selected, double margin,
-                            int64_t n, double* __restrict__ keys, int64_t* __restrict__ idx)
+                            int64_t n, const int64_t* __restrict__ tie_rank, double* __restrict__ keys,
+                            int64_t* __restrict__ idx)
 {
     for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
-        keys[j] = selected[j] ? -1.0 : fmax(ynorm2[j] - proj2[j], 0.0) + err2[j] + margin * ynorm2[j];
-        idx[j] = j;
+        const int64_t pos = tie_rank ? tie_rank[j] : j;
+        keys[pos] = selected[j] ? -1.0 : fmax(ynorm2[j] - proj2[j], 0.0) + err2[j] + margin * ynorm2[j];
+        idx[pos] = j;
     }
 }
 
@@ -227,6 +231,7 @@ __global__ void cand_norm_kernel(const double* __restrict__ rc, int64_t m, int64
 //   else record the pivot (piv[k], rdiag[k]) and k += 1.
 __global__ __launch_bounds__(1024) void argmax_select_kernel(const double* __restrict__ rn2,
                                                              const int64_t* __restrict__ cand, int64_t C,
+                                                             const int64_t* __restrict__ tie_rank,
                                                              ArgMax* __restrict__ out, PivotState* __restrict__ st,
                                                              const double* __restrict__ bound_ptr,
                                                              int64_t block_start, double stop_abs, double stop_rel,
@@ -234,13 +239,15 @@ __global__ __launch_bounds__(1024) void argmax_select_kernel(const double* __res
                                                              double* __restrict__ rdiag)
 {
     if (st->stop) return;
+    // sc holds the tie rank of the best column (its index without tie ranks)
     __shared__ double sv[1024];
     __shared__ int64_t ss[1024], sc[1024];
     double best = -2.0;
     int64_t slot = -1, col = std::numeric_limits<int64_t>::max();
     for (int64_t c = threadIdx.x; c < C; c += blockDim.x) {
         const double v = rn2[c];
-        if (v > best || (v == best && cand[c] < col)) { best = v; slot = c; col = cand[c]; }
+        const int64_t rk = tie_rank ? tie_rank[cand[c]] : cand[c];
+        if (v > best || (v == best && rk < col)) { best = v; slot = c; col = rk; }
     }
     sv[threadIdx.x] = best; ss[threadIdx.x] = slot; sc[threadIdx.x] = col;
     __syncthreads();
@@ -255,7 +262,7 @@ __global__ __launch_bounds__(1024) void argmax_select_kernel(const double* __res
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const ArgMax b{sv[0], ss[0], sc[0]};
+        const ArgMax b{sv[0], ss[0], ss[0] >= 0 ? cand[ss[0]] : -1};
         *out = b;
         const double bound2 = bound_ptr ? *bound_ptr : -1.0;
         const int64_t k = st->k;
@@ -628,25 +635,70 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     // they save).  whole = true folds every pending pivot (row energies
     // wanted, or the previous block could certify nothing).
     const int64_t fold_group = 128;
+    // Dormant columns (opt.dormant): only the prefix [0, n_active) is folded;
+    // the rest keep ||y_j||^2 as their keys.  suffmax[j] = max ||y_j'||^2 over
+    // j' >= j decides how far the prefix reaches.
+    const bool dormant = opt.dormant && !opt.want_energy && !opt.want_tail;
+    int64_t n_active = n;
+    bool dormant_set = false;
+    double last_bound2 = -1.0;
+    std::vector<double> suffmax;
+    if (dormant) {
+        suffmax = ynorm2.download(s);
+        suffmax.push_back(0.0);
+        for (int64_t j = n; j-- > 0;) suffmax[j] = std::max(suffmax[j], suffmax[j + 1]);
+    }
+    // Fold pivots [q0, q0 + b) into columns [c0, c1).
+    auto fold_range = [&](int64_t q0, int64_t b, int64_t c0, int64_t c1, double* energy_out) {
+        const int64_t nc = c1 - c0;
+        if (b <= 0 || nc <= 0) return;
+        fold_timer.begin(s);
+        res.fold_flops += 2.0 * (double)b * (double)m * (double)nc;
+        if (digits)
+            fold_projections_i8(b, nc, m, Q.ptr + q0 * ldq, ldq,
+                                static_cast<const int8_t*>(y) + c0 * digit_column_bytes(ytype) * ldy, ldy, planes,
+                                digit_base_of(ytype), digit_column_bytes(ytype), opt.yscale + c0, proj2.ptr + c0,
+                                energy_out ? energy_out + q0 : nullptr, i8_scratch, s);
+        else if (split)
+            fold_projections_bf16x3(b, nc, m, Q.ptr + q0 * ldq, ldq, static_cast<const float*>(y) + c0 * ldy, ldy,
+                                    proj2.ptr + c0, energy_out ? energy_out + q0 : nullptr, fold_scratch, s,
+                                    ynorm2.ptr + c0, err2.ptr + c0);
+        else
+            fold_projections(b, nc, m, Q.ptr + q0 * ldq, ldq,
+                             ytype == kF32 ? static_cast<const void*>(static_cast<const float*>(y) + c0 * ldy)
+                                           : static_cast<const void*>(static_cast<const double*>(y) + c0 * ldy),
+                             ytype == kF32, ldy, proj2.ptr + c0, energy_out ? energy_out + q0 : nullptr, fold_part, s);
+        fold_timer.end(s);
+        ++res.folds;
+    };
     auto fold = [&](double* energy_out, bool whole) {
         int64_t b = k - folded;
         if (!whole && b >= fold_group) b = b / fold_group * fold_group;
         if (b <= 0) return;
-        fold_timer.begin(s);
-        res.fold_flops += 2.0 * (double)b * (double)m * (double)n;
-        if (digits)
-            fold_projections_i8(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const int8_t*>(y), ldy, planes,
-                                digit_base_of(ytype), digit_column_bytes(ytype), opt.yscale, proj2.ptr,
-                                energy_out ? energy_out + folded : nullptr, i8_scratch, s);
-        else if (split)
-            fold_projections_bf16x3(b, n, m, Q.ptr + folded * ldq, ldq, static_cast<const float*>(y), ldy, proj2.ptr,
-                                    energy_out ? energy_out + folded : nullptr, fold_scratch, s, ynorm2.ptr, err2.ptr);
-        else
-            fold_projections(b, n, m, Q.ptr + folded * ldq, ldq, y, ytype == kF32, ldy, proj2.ptr,
-                             energy_out ? energy_out + folded : nullptr, fold_part, s);
-        fold_timer.end(s);
+        fold_range(folded, b, 0, n_active, energy_out);
         folded += b;
-        ++res.folds;
+    };
+    // Columns [n_active, c1) join the folded prefix: every pivot folded so far
+    // is folded into them at once.
+    auto activate = [&](int64_t c1) {
+        c1 = std::min(c1, n);
+        if (c1 <= n_active) return;
+        fold_range(0, folded, n_active, c1, nullptr);
+        n_active = c1;
+    };
+    // Keep the unfolded suffix below 1% of a certification bound, widening
+    // the prefix once it passes 10% of the last one: the bound can fall ~8x in
+    // one early block ((H2O)64), and an unfolded key above it would stall the
+    // block.  A stalled block places it against its own bound.
+    auto place_dormant = [&](double b2, bool now) {
+        if (!dormant || b2 <= 0) return;
+        if (dormant_set && !now && suffmax[n_active] <= 0.1 * b2) return;
+        const double lo = 0.01 * b2;
+        const int64_t want = std::lower_bound(suffmax.begin(), suffmax.end(), lo,
+                                              [](double a, double v) { return a > v; }) - suffmax.begin();
+        if (!dormant_set && folded == 0) n_active = std::min(want, n);  // nothing folded yet: the prefix may shrink
+        else activate(want);
+        dormant_set = true;
     };
 
     // Exact FP64 recomputation of every column's folded norm (clears the
@@ -654,7 +706,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     auto refresh = [&]() {
         MPISDF_HIP(hipMemsetAsync(proj2.ptr, 0, n * sizeof(double), s));
         MPISDF_HIP(hipMemsetAsync(err2.ptr, 0, n * sizeof(double), s));
-        if (k > 0) fold_projections(k, n, m, Q.ptr, ldq, y, ytype == kF32, ldy, proj2.ptr, nullptr, fold_part, s);
+        if (k > 0)
+            fold_projections(k, n_active, m, Q.ptr, ldq, y, ytype == kF32, ldy, proj2.ptr, nullptr, fold_part, s);
         folded = k;
         ++res.refreshes;
     };
@@ -663,9 +716,10 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     const int64_t Cmax = std::max<int64_t>(C0, std::min<int64_t>(n, (int64_t(1) << 30) / std::max<int64_t>(m, 1)));
     bool stalled = false;
     while (!done && k < kmax) {
+        place_dormant(last_bound2, false);
         fold(opt.want_energy ? energy.ptr : nullptr, opt.want_energy || stalled);
         hipLaunchKernelGGL(keys_kernel, dim3(grid_for(n)), dim3(256), 0, s, ynorm2.ptr, proj2.ptr, err2.ptr,
-                           selected.ptr, opt.margin, n, keys.ptr, idx.ptr);
+                           selected.ptr, opt.margin, n, opt.tie_rank, keys.ptr, idx.ptr);
         MPISDF_HIP(hipGetLastError());
         MPISDF_HIP(hipcub::DeviceRadixSort::SortPairsDescending(sort_ws.ptr, sort_bytes, keys.ptr, keys_sorted.ptr,
                                                                 idx.ptr, idx_sorted.ptr, (int)n, 0, 64, s));
@@ -703,6 +757,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
         int64_t nnew = Ceff;
         MPISDF_HIP(hipMemcpyAsync(&nnew, d_nnew.ptr, sizeof(int64_t), hipMemcpyDeviceToHost, s));
         MPISDF_HIP(hipStreamSynchronize(s));
+        if (bound2 > 0) last_bound2 = bound2;
         Cprev = Ceff;
         if (k > 0) {
             // Classical Gram-Schmidt, H = Q^T R, R -= Q H (via Qt): one pass over
@@ -769,7 +824,8 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             const int64_t steps = std::min<int64_t>(kStepsPerSync, kmax - k);
             for (int64_t i = 0; i < steps; ++i) {
                 const int64_t kk = k + i, nt = kk - block_start;
-                hipLaunchKernelGGL(argmax_select_kernel, dim3(1), dim3(1024), 0, s, rn2.ptr, cand.ptr, Ceff, am.ptr,
+                hipLaunchKernelGGL(argmax_select_kernel, dim3(1), dim3(1024), 0, s, rn2.ptr, cand.ptr, Ceff,
+                                   opt.tie_rank, am.ptr,
                                    pstate.ptr, bound_ptr, block_start, opt.stop_abs, opt.stop_rel, opt.margin,
                                    force ? 1 : 0, piv_dev.ptr, rdiag_dev.ptr);
                 if (nt > 0)
@@ -804,9 +860,11 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             MPISDF_HIP(hipStreamSynchronize(s));
         }
         if (opt.debug)
-            fprintf(stderr, "[qrcp] block %lld: pivots %lld..%lld candidates %lld bound2 %.3e last_r %.3e folds %lld\n",
+            fprintf(stderr,
+                    "[qrcp] block %lld: pivots %lld..%lld candidates %lld bound2 %.3e last_r %.3e folds %lld "
+                    "folded columns %lld\n",
                     (long long)res.blocks, (long long)block_start, (long long)k, (long long)Ceff, bound2,
-                    res.rdiag.empty() ? 0.0 : res.rdiag.back(), (long long)res.folds);
+                    res.rdiag.empty() ? 0.0 : res.rdiag.back(), (long long)res.folds, (long long)n_active);
         if (k == kmax || done) break;
         if (k == block_start) {
             // No pivot could be certified: a non-candidate's bound ties or beats
@@ -816,6 +874,12 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
             ++res.stalls;
             stalled = true;
             if (folded < k) continue;  // first make every key exact to the pivots taken
+            // then fold the unfolded columns near the bound (each such pass
+            // folds at least one more column)
+            if (bound2 > 0 && n_active < n && suffmax[n_active] > 0.01 * bound2) {
+                place_dormant(bound2, true);
+                continue;
+            }
             if (Ceff < remaining && C < Cmax) C = std::min(2 * C, Cmax);
             else if (split && !refreshed) { refresh(); refreshed = true; }
             else if (force) break;  // even a forced pivot has no positive norm: nothing is left to factorise
@@ -862,6 +926,7 @@ QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t
     }
     MPISDF_HIP(hipStreamSynchronize(s));
     res.fold_seconds = fold_timer.seconds();
+    res.folded_columns = n_active;
     res.gs_seconds = gs_timer.seconds();
     return res;
 }

@@ -17,6 +17,16 @@ struct QrcpOptions {
     const double* ynorm2 = nullptr;  // optional device ||y_j||^2 (skips the norm pass; required for kI8Digits)
     const double* yscale = nullptr;  // kI8Digits: device per-column scales of the digit planes
     bool split_bf16_fold = false;    // FP32 Y: fold with split-BF16 MFMA (needs margin > 0)
+    // Leave columns far below the pivots unfolded: a column's key is then its
+    // ||y_j||^2, still an upper bound of its residual; a suffix of columns
+    // stays unfolded while its largest ||y_j||^2 is below a quarter of the
+    // last certification bound, and joins (one fold against every pivot so
+    // far) before it could reach it.  Pivots are unchanged; the columns
+    // should come in roughly descending ||y_j||^2.  Not with energy or tail.
+    bool dormant = false;
+    // Optional device ranks (a permutation of 0 .. n-1): ties between equal
+    // keys or norms go to the lowest rank instead of the lowest column.
+    const int64_t* tie_rank = nullptr;
 };
 
 struct QrcpResult {
@@ -29,6 +39,7 @@ struct QrcpResult {
     double fold_flops = 0.0;      // their algorithmic flops, sum 2 b m n
     double gs_seconds = 0.0;      // summed HIP-event duration of the candidate Gram-Schmidt products
     double gs_flops = 0.0;        // their flops: 4 k m cols per pass (H = Q^T R and R -= Q H)
+    int64_t folded_columns = 0;   // columns folded at the end (dormant: the active prefix)
     int64_t blocks = 0, folds = 0, stalls = 0, refreshes = 0;
 };
 

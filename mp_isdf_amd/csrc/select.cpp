@@ -68,6 +68,11 @@ __global__ void row_norm2_kernel(const double* __restrict__ f, int64_t rows, int
     q[g] = sum;
 }
 
+// Fixed-rank randomized selections on grids of this many kept points or more
+// order their columns in levels of expected sketch norm and leave the far
+// smaller ones unfolded in the QRCP (below that the extra pass does not pay).
+constexpr int64_t kDormantColumns = int64_t(1) << 18;
+
 // Device batch for feature evaluation (independent of the reference's
 // host-memory batch_size, which does not change results).
 int64_t device_batch(int64_t width, int64_t nao, int64_t extra_per_point)
@@ -316,6 +321,70 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     // form keeps four planes (three would round it to ~3e-5).
     const int planes = form == kSketchBF16x3 ? 4 : 3;
 
+    // Columns in levels of their expected sketch norm E||y_g||^2 = c_g^2
+    // ||f_g||^4 (c_g the column weight below, f_g the point's features), in
+    // factors of four down from the largest, grid order kept within a level:
+    // the QRCP then leaves the trailing levels unfolded while they stay far
+    // below its pivots (QrcpOptions::dormant), and ties between equal norms
+    // still go to the lower grid position (QrcpOptions::tie_rank).  Fixed
+    // rank only, on grids large enough for the pass to pay.
+    DeviceBuffer<int64_t> tie_rank;
+    const bool dormant = a.max_rank > 0 && n >= kDormantColumns && !a.audit;
+    if (dormant) {
+        DeviceBuffer<double> d_all, q(n);
+        d_all.upload(kept.xyz.data(), kept.xyz.size(), s);
+        const int64_t batch = std::min(n, device_batch(d, ctx.nao, 0));
+        DeviceBuffer<double> feat((size_t)batch * d);
+        AxisWorkspace ws0;
+        for (int64_t start = 0; start < n; start += batch) {
+            const int64_t count = std::min(batch, n - start);
+            evaluate_axis_device(ctx, spec, d_all.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
+                                 a.mo_coeff, a.nocc, feat.ptr, ws0, s);
+            hipLaunchKernelGGL(row_norm2_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, feat.ptr,
+                               count, d, q.ptr + start);
+            MPISDF_HIP(hipGetLastError());
+        }
+        const auto hq = q.download(s);
+        constexpr int kLevels = 48;
+        std::vector<double> est(n);
+        std::vector<int> level(n);
+        double emax = 0;
+        for (int64_t g = 0; g < n; ++g) {
+            double c2 = density ? kept.importance[g] : std::abs(kept.w[g]);
+            if (normalized) c2 /= ao_norm2[g] * ao_norm2[g];
+            est[g] = c2 * hq[g] * hq[g];
+            if (std::isfinite(est[g])) emax = std::max(emax, est[g]);
+        }
+        std::vector<int64_t> count(kLevels + 1, 0);
+        for (int64_t g = 0; g < n; ++g) {
+            const double e = est[g];
+            int lv = kLevels - 1;
+            if (e > 0 && emax > 0) lv = std::min<int>(kLevels - 1, (int)std::floor(0.5 * std::log2(emax / e)));
+            level[g] = std::max(lv, 0);
+            ++count[level[g] + 1];
+        }
+        for (int l = 0; l < kLevels; ++l) count[l + 1] += count[l];
+        std::vector<int64_t> order(n);  // new position -> position in grid order
+        for (int64_t g = 0; g < n; ++g) order[count[level[g]]++] = g;
+        Kept sorted;
+        sorted.xyz.resize(kept.xyz.size());
+        sorted.w.resize(n);
+        sorted.importance.resize(kept.importance.size());
+        sorted.original.resize(n);
+        std::vector<double> sorted_norm2(ao_norm2.size());
+        for (int64_t j = 0; j < n; ++j) {
+            const int64_t g = order[j];
+            for (int c = 0; c < 3; ++c) sorted.xyz[3 * j + c] = kept.xyz[3 * g + c];
+            sorted.w[j] = kept.w[g];
+            if (!kept.importance.empty()) sorted.importance[j] = kept.importance[g];
+            sorted.original[j] = kept.original[g];
+            if (!ao_norm2.empty()) sorted_norm2[j] = ao_norm2[g];
+        }
+        kept = std::move(sorted);
+        ao_norm2 = std::move(sorted_norm2);
+        tie_rank.upload(order.data(), n, s);
+    }
+
     std::vector<double> hu, hv;
     make_omega(d, ell, a.seed, hu, hv);
     DeviceBuffer<double> u, v;
@@ -512,6 +581,8 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     opt.want_energy = a.max_rank == 0;
     opt.want_tail = a.max_rank == 0;
     opt.debug = a.debug;
+    opt.dormant = dormant;
+    opt.tie_rank = dormant ? tie_rank.ptr : nullptr;
     const auto tq = Clock::now();
     opt.ynorm2 = ynorm2.ptr;
     // Certifying greedy pivots late in the factorisation needs folded norms
