@@ -256,6 +256,21 @@ int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n, int64_t l
                        const double* yscale, int64_t kmax, double stop_rel, int64_t* pivots, double* rdiag,
                        int64_t* count, void* stream);
 
+/* mpisdf_qrcp_device for columns the caller has reordered (no reference
+ * counterpart: the selection's own column order for large grids, exposed for
+ * tests and callers that order their columns).  tie_rank: NULL or a device
+ * permutation of 0 .. n-1; ties between equal norms go to the lowest rank
+ * (the column's position before reordering) instead of the lowest column.
+ * dormant != 0: columns whose ||y_j||^2 stays far below the certification
+ * bound are not folded (their keys stay ||y_j||^2, an upper bound), which
+ * pays when the columns come in roughly descending norm; the pivots are the
+ * same.  1024 candidate columns per block.  folded_columns (may be NULL):
+ * the columns folded by the end (n without dormant columns). */
+int mpisdf_qrcp_device_ordered(const void* y, int ytype, int64_t m, int64_t n, int64_t ldy, const double* ynorm2,
+                               const double* yscale, const int64_t* tie_rank, int32_t dormant, int64_t kmax,
+                               double stop_rel, int64_t* pivots, double* rdiag, int64_t* count,
+                               int64_t* folded_columns, void* stream);
+
 /* ---- ISDF fit on selected anchors ------------------------------------------- */
 /* Replaces the driver refit and pair-space error of the reference
  * (python/isdf.py:238-263 _physical_pair_refit, :176-191 pair error) with the

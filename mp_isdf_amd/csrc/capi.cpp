@@ -520,6 +520,37 @@ extern "C" int mpisdf_qrcp_device(const void* y, int ytype, int64_t m, int64_t n
     });
 }
 
+extern "C" int mpisdf_qrcp_device_ordered(const void* y, int ytype, int64_t m, int64_t n, int64_t ldy,
+                                          const double* ynorm2, const double* yscale, const int64_t* tie_rank,
+                                          int32_t dormant, int64_t kmax, double stop_rel, int64_t* pivots,
+                                          double* rdiag, int64_t* count, int64_t* folded_columns, void* stream)
+{
+    return guard([&] {
+        if (!y || !pivots || !count || m <= 0 || n <= 0 || kmax <= 0 || !(stop_rel >= 0))
+            throw std::invalid_argument("invalid QRCP arguments");
+        if (ytype != mpisdf::kF64 && ytype != mpisdf::kF32 && !mpisdf::is_digit_type((mpisdf::DType)ytype))
+            throw std::invalid_argument("QRCP input type must be 0 (f64), 1 (f32), 4 or 5 (3 or 4 base-128 planes) or 6 (4 base-254 planes)");
+        mpisdf::QrcpOptions opt;
+        opt.kmax = kmax;
+        opt.stop_rel = stop_rel;
+        opt.candidates = std::min<int64_t>(n, 1024);
+        opt.ynorm2 = ynorm2;
+        opt.tie_rank = tie_rank;
+        opt.dormant = dormant != 0;
+        if (mpisdf::is_digit_type((mpisdf::DType)ytype)) {
+            opt.yscale = yscale;
+            opt.margin = mpisdf::kFoldI8Margin;
+        }
+        const auto r = mpisdf::qrcp_device(y, static_cast<mpisdf::DType>(ytype), m, n, ldy, opt,
+                                           static_cast<hipStream_t>(stream));
+        const int64_t k = std::min<int64_t>(kmax, (int64_t)r.pivots.size());
+        std::copy(r.pivots.begin(), r.pivots.begin() + k, pivots);
+        if (rdiag) std::copy(r.rdiag.begin(), r.rdiag.begin() + k, rdiag);
+        *count = k;
+        if (folded_columns) *folded_columns = r.folded_columns;
+    });
+}
+
 namespace {
 
 void run_fit(mpisdf::Context& c, const double* anchor_xyz_dev, const double* anchor_w, int64_t k,

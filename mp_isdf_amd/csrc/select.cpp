@@ -27,6 +27,7 @@
 #include <memory>
 #include <random>
 #include <thread>
+#include <utility>
 
 namespace mpisdf {
 namespace {
@@ -58,14 +59,20 @@ Kept filter_points(const SelectArgs& a, const double* importance = nullptr)
     return k;
 }
 
-// q[g] = sum_a f[g * d + a]^2 for the rows of a feature batch.
-__global__ void row_norm2_kernel(const double* __restrict__ f, int64_t rows, int64_t d, double* __restrict__ q)
+// q[g] = sum_a f[g * d + a]^2 for the rows of a feature batch: one wave per
+// row (coalesced reads), its lanes' sums added by a butterfly.
+constexpr int kRowNormRows = 4;  // rows per 256-thread workgroup
+__global__ __launch_bounds__(256) void row_norm2_kernel(const double* __restrict__ f, int64_t rows, int64_t d,
+                                                        double* __restrict__ q)
 {
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t g = (int64_t)blockIdx.x * kRowNormRows + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
     if (g >= rows) return;
     double sum = 0;
-    for (int64_t a = 0; a < d; ++a) sum += f[g * d + a] * f[g * d + a];
-    q[g] = sum;
+    for (int64_t a = lane; a < d; a += 64) sum += f[g * d + a] * f[g * d + a];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane == 0) q[g] = sum;
 }
 
 // Fixed-rank randomized selections on grids of this many kept points or more
@@ -272,6 +279,29 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         throw std::invalid_argument(
             "randomized ID sketch/rank exceeds the ILP64 QRCP backend limit for this grid; "
             "reduce the sketch/rank or use a tiled QRCP backend");
+    // Omega is drawn on host threads (it depends on d, ell and the seed only)
+    // while the GPU screens and orders the points.
+    std::vector<double> hu, hv;
+    struct OmegaThread {
+        std::thread t;
+        std::exception_ptr err;
+        void join()
+        {
+            if (t.joinable()) t.join();
+            if (err) std::rethrow_exception(std::exchange(err, nullptr));
+        }
+        ~OmegaThread()
+        {
+            if (t.joinable()) t.join();
+        }
+    } omega;
+    omega.t = std::thread([&hu, &hv, &omega, d, ell, seed = a.seed] {
+        try {
+            make_omega(d, ell, seed, hu, hv);
+        } catch (...) {
+            omega.err = std::current_exception();
+        }
+    });
     std::vector<double> ao_norm2;
     if (density) {
         // Screening pass: only O(n) squared norms come back to the host.
@@ -284,7 +314,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             const int64_t count = std::min(batch, n - start);
             evaluate_axis_device(ctx, spec, d_all.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
                                  a.mo_coeff, a.nocc, feat.ptr, ws0, s);
-            hipLaunchKernelGGL(row_norm2_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, feat.ptr,
+            hipLaunchKernelGGL(row_norm2_kernel, dim3((unsigned)((count + kRowNormRows - 1) / kRowNormRows)), dim3(256), 0, s, feat.ptr,
                                count, d, q.ptr + start);
             MPISDF_HIP(hipGetLastError());
         }
@@ -340,7 +370,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             const int64_t count = std::min(batch, n - start);
             evaluate_axis_device(ctx, spec, d_all.ptr + 3 * start, count, scales.empty() ? nullptr : scales.data(),
                                  a.mo_coeff, a.nocc, feat.ptr, ws0, s);
-            hipLaunchKernelGGL(row_norm2_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, feat.ptr,
+            hipLaunchKernelGGL(row_norm2_kernel, dim3((unsigned)((count + kRowNormRows - 1) / kRowNormRows)), dim3(256), 0, s, feat.ptr,
                                count, d, q.ptr + start);
             MPISDF_HIP(hipGetLastError());
         }
@@ -356,12 +386,13 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             if (std::isfinite(est[g])) emax = std::max(emax, est[g]);
         }
         std::vector<int64_t> count(kLevels + 1, 0);
+        const int top = emax > 0 ? std::ilogb(emax) : 0;
         for (int64_t g = 0; g < n; ++g) {
             const double e = est[g];
-            int lv = kLevels - 1;
-            if (e > 0 && emax > 0) lv = std::min<int>(kLevels - 1, (int)std::floor(0.5 * std::log2(emax / e)));
-            level[g] = std::max(lv, 0);
-            ++count[level[g] + 1];
+            int lv = kLevels - 1;  // zero or non-finite estimates last
+            if (e > 0 && std::isfinite(e)) lv = std::min(kLevels - 1, std::max(0, (top - std::ilogb(e)) / 2));
+            level[g] = lv;
+            ++count[lv + 1];
         }
         for (int l = 0; l < kLevels; ++l) count[l + 1] += count[l];
         std::vector<int64_t> order(n);  // new position -> position in grid order
@@ -385,8 +416,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         tie_rank.upload(order.data(), n, s);
     }
 
-    std::vector<double> hu, hv;
-    make_omega(d, ell, a.seed, hu, hv);
+    omega.join();
     DeviceBuffer<double> u, v;
     u.upload(hu.data(), hu.size(), s);
     v.upload(hv.data(), hv.size(), s);

@@ -1,6 +1,8 @@
 """Khatri-Rao sketch kernels against the FP64 oracle sketch on the same
 phi, Omega and weights (FP64 form: 1e-12; BF16 / FP8 forms: the rounding of
 their inputs, stated per test)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -471,3 +473,46 @@ def test_fp64i8_feature_widths_off_64(mp_isdf, axes):
     want = oracle.select(mol.atm, mol.bas, mol.env, points, weights, list(axes), max_rank=40,
                          algorithm="rand_sketch", oversampling=8, seed=7)
     assert got == want
+
+
+@pytest.mark.parametrize("ytype", [0, 6])
+def test_qrcp_ordered_dormant_matches_plain(mp_isdf, ytype):
+    """The selection's order for large grids: columns in levels of norm
+    (factors of four, original order within a level) with their original
+    positions as tie ranks, and the columns far below the pivots left
+    unfolded.  The pivots map back to the plain QRCP's on the original order,
+    with the same |R_ii|; FP64 columns and their four base-254 digit planes."""
+    import torch
+    rng = np.random.default_rng(21)
+    m, n, kmax = 320, 120000, 300
+    scale = 10.0 ** (-6.0 * rng.uniform(size=n) ** 2)  # ~30% of the columns below 1e-3 of the largest
+    y = (rng.normal(size=(n, 48)) @ rng.normal(size=(48, m)) + 1e-3 * rng.normal(size=(n, m))) * scale[:, None]
+    ynorm2 = np.sum(y ** 2, axis=1)
+    level = np.floor(0.5 * np.log2(ynorm2.max() / ynorm2)).astype(np.int64)
+    order = np.argsort(level, kind="stable")
+    P = (m + 63) // 64 * 64
+
+    def run(cols, tie_rank, dormant):
+        buf = np.zeros((n, P), dtype=np.float64)
+        buf[:, :m] = cols
+        Y = torch.tensor(buf, device="cuda")
+        yn = torch.tensor(np.sum(cols ** 2, axis=1), device="cuda")
+        sc = torch.zeros(n, dtype=torch.float64, device="cuda")
+        if ytype == 6:
+            capi.call("mpisdf_digitize_columns_f64_device", Y, m, n, P, yn, sc, None)
+        piv = np.zeros(kmax, dtype=np.int64)
+        rd = np.zeros(kmax)
+        cnt = np.zeros(1, dtype=np.int64)
+        folded = np.zeros(1, dtype=np.int64)
+        rank = None if tie_rank is None else torch.tensor(tie_rank, device="cuda")
+        capi.call("mpisdf_qrcp_device_ordered", Y, ytype, m, n, P, yn, sc if ytype == 6 else None, rank,
+                  ctypes.c_int32(int(dormant)), kmax, 0.0, piv, rd, cnt, folded, None)
+        return piv[:cnt[0]], rd[:cnt[0]], int(folded[0])
+
+    piv0, rd0, f0 = run(y, None, False)
+    piv1, rd1, f1 = run(y[order], order, True)
+    print(f"\nfolded columns: plain {f0}, ordered with dormant columns {f1} of {n}")
+    assert f0 == n and f1 < n
+    assert len(piv0) == kmax and len(piv1) == kmax
+    np.testing.assert_array_equal(order[piv1], piv0)
+    np.testing.assert_allclose(rd1, rd0, rtol=1e-10)
