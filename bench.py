@@ -432,11 +432,21 @@ def main():
             rec = measured.get(value_form, {}).get(key, {})
             tag = rec.get("kernel_tag")
             if tag and tag in entry["kernel"]:
-                entry["traffic"] = round(rec["traffic_bytes_per_launch"] / 1e9, 3)
+                ratio = rec["traffic_over_algorithmic"]
+                if key == "qrcp_fold" and fold_f > 0:
+                    # this run's algorithmic bytes per launch: the digit columns
+                    # (4 ell64 bytes each) once per 128-row group folded; the
+                    # folds cover the unfolded-column prefix, and a catch-up
+                    # launch several row groups
+                    ell64 = (ell + 63) // 64 * 64
+                    alg = fold_f / fold_n / (2.0 * 128 * ell) * 4 * ell64
+                else:
+                    alg = rec["algorithmic_bytes_per_launch"]
+                entry["traffic"] = round(ratio * alg / 1e9, 3)
                 entry["traffic_unit"] = "GB per launch"
-                entry["traffic_source"] = (f"{traffic_file.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
-                                           f"algorithmic {rec['algorithmic_bytes_per_launch'] / 1e9:.3f} "
-                                           "GB per launch)")
+                entry["traffic_source"] = (f"{traffic_file.relative_to(ROOT)} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                                           f"on a probe launch: {ratio:.3f}x its algorithmic bytes; this run's "
+                                           f"algorithmic {alg / 1e9:.3f} GB per launch)")
     for entry in (roofline_fold, roofline_fit, roofline_fit_s, roofline_sketch):
         entry.setdefault("traffic", None)
     # The dominant kernel is the one with the largest summed duration in the step.
