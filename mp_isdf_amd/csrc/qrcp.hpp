@@ -18,11 +18,12 @@ struct QrcpOptions {
     const double* yscale = nullptr;  // kI8Digits: device per-column scales of the digit planes
     bool split_bf16_fold = false;    // FP32 Y: fold with split-BF16 MFMA (needs margin > 0)
     // Leave columns far below the pivots unfolded: a column's key is then its
-    // ||y_j||^2, still an upper bound of its residual; a suffix of columns
-    // stays unfolded while its largest ||y_j||^2 is below a quarter of the
-    // last certification bound, and joins (one fold against every pivot so
-    // far) before it could reach it.  Pivots are unchanged; the columns
-    // should come in roughly descending ||y_j||^2.  Not with energy or tail.
+    // ||y_j||^2, still an upper bound of its residual.  Only the prefix whose
+    // largest ||y_j||^2 can reach 1% of the last certification bound is
+    // folded; once the unfolded suffix passes 10% of it the prefix widens, and
+    // the new columns get one fold against every pivot so far.  Pivots are
+    // unchanged; the columns should come in roughly descending ||y_j||^2.
+    // Not with energy or tail.
     bool dormant = false;
     // Optional device ranks (a permutation of 0 .. n-1): ties between equal
     // keys or norms go to the lowest rank instead of the lowest column.
@@ -44,13 +45,14 @@ struct QrcpResult {
 };
 
 // Y is m x n column-major with leading dimension ldy, element type f64 or f32;
-// or, for kI8Digits / kI8Digits4, column j at byte offset 4 ldy j holding three
-// / four digits per entry (digit_offset; y_hat_j = yscale[j] sum_r d_r
-// 128^-(r+1)), folded with INT8
-// MFMA and factorised as y_hat (candidate columns are rebuilt from the same
-// digits; opt.margin must cover the fold's error).  Y is read only.  Pivot decisions use exact FP64 residual norms of the
-// candidate columns (LAPACK dgeqp3 pivot semantics: max norm, lowest index on
-// ties).
+// or, for a digit type, column j at byte offset digit_column_bytes(ytype) ldy j
+// holding digit_planes_of(ytype) signed digits per entry (digit_offset;
+// y_hat_j = yscale[j] sum_r d_r B^-(r+1), B = digit_base_of(ytype)), folded
+// with INT8 MFMA and factorised as y_hat (candidate columns are rebuilt from
+// the same digits; opt.margin must cover the fold's error).  Y is read only.
+// Pivot decisions use exact FP64 residual norms of the candidate columns
+// (LAPACK dgeqp3 pivot semantics: max norm, lowest index on ties, or lowest
+// tie rank when opt.tie_rank is given).
 QrcpResult qrcp_device(const void* y, DType ytype, int64_t m, int64_t n, int64_t ldy,
                        const QrcpOptions& opt, hipStream_t stream);
 
