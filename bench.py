@@ -50,9 +50,12 @@ PEAK = {"fp64_mfma_tflops": 78.6, "bf16_mfma_tflops": 2500.0, "fp8_mfma_tflops":
 # sketch, q into 4 and y into 4 base-254 digits, the 13 pairs p + r <= 4.
 # Keyed by the selection's fold_kind.
 I8_PAIRS = {2: 14, 3: 17, 4: 13}
-# Digit pairs p + r <= 4 of five base-254 planes on each operand (the fp64i8
-# sketch and the digit form of the fit's products).
+# Digit pairs p + r <= 4 of five base-254 planes on each operand (the digit
+# form of the fit's products).
 DIGIT_PAIRS_5 = 15
+# The fp64i8 Khatri-Rao sketch: four base-254 digits of U, V and the features,
+# digit pairs p + r <= 3.
+SKETCH_PAIRS = 10
 # Arithmetic of the fit's Z = L^-1 S product per form: 0 FP64 MFMA, 1 base-254
 # INT8 digits of the FP64 operands (every form but the all-FP64 one).
 FIT_FORM = {"fp64": 0}
@@ -392,7 +395,7 @@ def main():
                            "frac": round(sk_ops / sk_s / 1e12 / PEAK["int8_mfma_tops"], 4) if sk_s else None,
                            "fp64_equivalent_tflops": round(sk_f / sk_s / 1e12, 2) if sk_s else None,
                            "launches": sk_n * (2 * ell64 // 128), "launch_ms": None,
-                           "work": f"{DIGIT_PAIRS_5} INT8 digit-pair products of 2*(2*ell64)*d64 per grid point, "
+                           "work": f"{SKETCH_PAIRS} INT8 digit-pair products of 2*(2*ell64)*d64 per grid point, "
                                    f"ell64={ell64}, d64={dpad}, n={n} (the U and V rows stacked by 64); FP64 "
                                    f"equivalent 4*ell*d*n"}
         if sk_s:

@@ -30,7 +30,9 @@
 // In base 254 the pairs p + r <= 4 run in five groups: 13 for the QRCP fold
 // (q in four digits, four Y planes: at most four pairs per group,
 // |sum| < 4 * 127^2 * m < 2^31 for m < 2^15), 15 with five digits and five
-// planes (at most five: m <= 26624).
+// planes (at most five: m <= 26624).  The Khatri-Rao sketch takes four digits
+// on both sides and the ten pairs p + r <= 3 in four groups (the NG template
+// argument).
 // The dropped pairs (p + r >= 6 in base 128, >= 5 in base 254) and the
 // truncation of q leave ~1e-11 of ||y_j||^2 in the folded sums (measured),
 // covered by the QRCP's relative margin.
@@ -52,9 +54,10 @@ typedef int i4v __attribute__((ext_vector_type(4)));
 // Digit schemes.  Base 128 (|d| <= 64): q in five digits, Y in three or four
 // planes, the pairs p + r <= 5 in six accumulator groups.  Base 254
 // (|d| <= 127): q in five digits (four in the QRCP fold's column sums, below),
-// Y in four planes (an FP64 sketch) or five (point features, S), the pairs
-// p + r <= 4 (13, 14 or 15) in five groups; the dropped pairs are 254^-7 of
-// u_t s_j and smaller.
+// Y in four planes (an FP64 sketch or the sketch's point features) or five
+// (fit features, S), the pairs p + r <= 4 (13, 14 or 15) in five groups; the
+// dropped pairs are 254^-7 of u_t s_j and smaller.  A kernel can take fewer
+// groups (NG): the sketch runs p + r <= 3.
 template <int BASE>
 struct DigitScheme;
 template <>
@@ -315,7 +318,8 @@ typedef int i16v __attribute__((ext_vector_type(16)));
 // proj2[col] += the column's sum of squares and non-finite entries counted
 // into *bad.
 constexpr int kEpiFold = 0, kEpiSquare = 1, kEpiKhatriRao = 2, kEpiStore = 3, kEpiSubtract = 4;
-template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold, int ND = DigitScheme<BASE>::ND>
+template <bool ENERGY, int NR, int NY, int BASE, int EPI = kEpiFold, int ND = DigitScheme<BASE>::ND,
+          int NG = DigitScheme<BASE>::NG>
 __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const int8_t* __restrict__ qd,
                                                                    const double* __restrict__ qs, int64_t ldq,
                                                                    int64_t bpad, int64_t b, int64_t mbase,
@@ -331,7 +335,6 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                                                                    int lower = 0,
                                                                    double* __restrict__ colpart = nullptr)
 {
-    constexpr int NG = DigitScheme<BASE>::NG;
     constexpr int ROWS = 32 * NR;  // Q rows staged per slice
     constexpr int NTHREADS = 128 * NR;
     using L = SliceLayout<NR>;
@@ -482,7 +485,8 @@ __global__ __launch_bounds__(128 * NR, 4 / NR) void fold_i8_32x32_kernel(const i
                 __builtin_amdgcn_sched_group_barrier(kSchedMfma, 1, 0);                              \
                 __builtin_amdgcn_sched_group_barrier(kSchedVmemRead, 1, 0);                          \
             }                                                                                        \
-            __builtin_amdgcn_sched_group_barrier(kSchedMfma, kMfma - 2 * kMoves - kOps, 0);          \
+            if constexpr (kMfma > 2 * kMoves + kOps)                                                 \
+                __builtin_amdgcn_sched_group_barrier(kSchedMfma, kMfma - 2 * kMoves - kOps, 0);      \
         } else { /* 4 waves: loads, MFMAs, then the stash (measured best there) */                  \
             MPISDF_LOAD_Q(((sl) + 2) * FBK, rqL);                                                    \
             MPISDF_LOAD_YD(((sl) + 2) * FBK, ryL);                                                   \
@@ -681,33 +685,45 @@ void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* yn
 }
 
 void digitize_columns_f64_max(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
-                              hipStream_t stream)
+                              hipStream_t stream, int planes)
 {
+    if (planes != 4 && planes != 5) throw std::invalid_argument("columns are digitised into four or five planes");
     if (n <= 0) return;
     if (P % kDigitSlice || P < m) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     if (P > kDigitRowsMaxF64)
         throw std::invalid_argument("digitize_columns_f64 holds one column in registers (pitch <= 16384)");
     if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
-    hipLaunchKernelGGL((digitize_columns_f64_kernel<true, 5>), dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream,
-                       y, m, P, ynorm2, scale);
+    if (planes == 4)
+        hipLaunchKernelGGL((digitize_columns_f64_kernel<true, 4>), dim3((unsigned)n), dim3(kDigitizeThreads), 0,
+                           stream, y, m, P, ynorm2, scale);
+    else
+        hipLaunchKernelGGL((digitize_columns_f64_kernel<true, 5>), dim3((unsigned)n), dim3(kDigitizeThreads), 0,
+                           stream, y, m, P, ynorm2, scale);
     MPISDF_HIP(hipGetLastError());
 }
 
-void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream)
+void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream,
+                   int nd)
 {
     if (rows <= 0 || kdim <= 0 || lda < kdim) throw std::invalid_argument("invalid row digit operand");
+    if (nd != 4 && nd != 5) throw std::invalid_argument("row digits come in four or five base-254 digits");
     if (kdim > 32767) throw std::invalid_argument("the base-254 fold needs at most 32767 entries per row");
     out.rows = rows;
     out.kdim = kdim;
     out.bpad = (rows + FBM - 1) / FBM * FBM;
     out.ldq = (kdim + FBK - 1) / FBK * FBK;
-    const size_t need = (size_t)(DigitScheme<254>::ND * out.bpad * out.ldq);
+    out.nd = nd;
+    const size_t need = (size_t)(nd * out.bpad * out.ldq);
     if (need >= (size_t(1) << 31)) throw std::invalid_argument("the INT8 fold's row digits exceed 2 GiB");
     if (out.digits.count < need) out.digits.resize(need);
     if (out.scales.count < (size_t)out.bpad) out.scales.resize((size_t)out.bpad);
     if (out.bpad > 0x7fffffff) throw std::overflow_error("too many rows to digitise");
-    hipLaunchKernelGGL(split_q_kernel<254>, dim3((unsigned)out.bpad), dim3(256), 0, stream, a, lda, kdim, rows, out.ldq,
-                       out.bpad, out.digits.ptr, out.scales.ptr);
+    if (nd == 4)
+        hipLaunchKernelGGL((split_q_kernel<254, 4>), dim3((unsigned)out.bpad), dim3(256), 0, stream, a, lda, kdim, rows,
+                           out.ldq, out.bpad, out.digits.ptr, out.scales.ptr);
+    else
+        hipLaunchKernelGGL((split_q_kernel<254, 5>), dim3((unsigned)out.bpad), dim3(256), 0, stream, a, lda, kdim, rows,
+                           out.ldq, out.bpad, out.digits.ptr, out.scales.ptr);
     MPISDF_HIP(hipGetLastError());
 }
 
@@ -732,7 +748,7 @@ void split_khatri_rao_rows_i8(const double* u, const double* v, int64_t d, int64
     hipLaunchKernelGGL(stack_khatri_rao_kernel, dim3((unsigned)((rows * d + 255) / 256)), dim3(256), 0, stream, u, v,
                        d, ell, rows, stacked.ptr);
     MPISDF_HIP(hipGetLastError());
-    split_rows_i8(stacked.ptr, d, rows, d, out, stream);
+    split_rows_i8(stacked.ptr, d, rows, d, out, stream, kSketchDigits254);
     MPISDF_HIP(hipStreamSynchronize(stream));  // before the stacked rows are freed
 }
 
@@ -760,6 +776,7 @@ void fold_rows_square_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const
     if (n <= 0) return;
     if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    if (q.nd != 5) throw std::invalid_argument("this product takes row digits split into five digits");
     if (ldc < q.rows) throw std::invalid_argument("the product's leading dimension is shorter than its rows");
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
@@ -781,6 +798,7 @@ void fold_rows_subtract_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, con
     if (n <= 0) return;
     if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    if (q.nd != 5) throw std::invalid_argument("this product takes row digits split into five digits");
     if (ldc < q.rows) throw std::invalid_argument("the product's leading dimension is shorter than its rows");
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
@@ -804,13 +822,16 @@ void fold_rows_khatri_rao_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, c
     if (q.rows % FBM) throw std::invalid_argument("Khatri-Rao row digits come in groups of 64 + 64 rows");
     if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    if (q.nd != kSketchDigits254) throw std::invalid_argument("Khatri-Rao row digits come from split_khatri_rao_rows_i8");
     if (ldy < q.rows / 2) throw std::invalid_argument("the sketch's leading dimension is shorter than its rows");
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
     const int64_t g = batched_groups(q.rows);
     if (g <= 0) return;
     double* part = group_partials(q, g, n);
-    hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, 5, 254, kEpiKhatriRao>), dim3((unsigned)gx, (unsigned)g),
+    hipLaunchKernelGGL((fold_i8_32x32_kernel<false, 4, kSketchDigits254, 254, kEpiKhatriRao, kSketchDigits254,
+                                             kSketchDigits254>),
+                       dim3((unsigned)gx, (unsigned)g),
                        dim3(512), 0, stream, q.digits.ptr, q.scales.ptr, q.ldq, q.bpad, q.rows, (int64_t)0, Yd, P, 8,
                        yscale, q.kdim, n, ynorm2, nullptr, y, ldy, cscale, bad, 0, part);
     MPISDF_HIP(hipGetLastError());
@@ -824,6 +845,7 @@ void fold_rows_i8(const FoldI8Rows& q, bool lower, const int8_t* Yd, int64_t P, 
     if (P % kDigitSlice || P < q.kdim) throw std::invalid_argument("digit planes need a pitch that is a multiple of 64 and >= m");
     // five Y planes: up to five pairs share an accumulator, 5 * 127^2 * ldq < 2^31
     if (q.ldq > 26624) throw std::invalid_argument("the five-plane base-254 fold takes at most 26624 entries per row");
+    if (q.nd != 5) throw std::invalid_argument("this product takes row digits split into five digits");
     const int64_t gx = (n + FBN - 1) / FBN;
     if (gx > 0x7fffffff) throw std::overflow_error("fold grid too large");
     const int64_t g = batched_groups(q.rows);

@@ -121,11 +121,12 @@ void digitize_columns(float* y, int64_t m, int64_t n, int64_t P, int planes, dou
 // out: of y_hat).
 void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
                           hipStream_t stream);
-// Five base-254 planes (within scale[j] 254^-5 / 2 ~ 4.8e-13 scale[j] of
-// each entry), the scale taken from each column's largest entry (found by the
-// kernel); ynorm2 (out) receives ||y_hat_j||^2.
+// Five (or four) base-254 planes (within scale[j] 254^-5 / 2 ~ 4.8e-13
+// scale[j], or 254^-4 / 2 ~ 1.2e-10 scale[j], of each entry), the scale taken
+// from each column's largest entry (found by the kernel); ynorm2 (out)
+// receives ||y_hat_j||^2.
 void digitize_columns_f64_max(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
-                              hipStream_t stream);
+                              hipStream_t stream, int planes = 5);
 constexpr int64_t kDigitRowsMaxF64 = 16384;  // digitize_columns_f64 holds one FP64 column in registers
 constexpr int64_t kDigitRowsMax = 36864;  // digitize_columns holds one FP32 column in registers
 constexpr int64_t kDigitSlice = 64;       // entries per slice; P is a multiple of it
@@ -187,16 +188,19 @@ void project_candidates_i8(QDigitStore& st, int64_t m, const int8_t* Yd, int64_t
                            hipStream_t stream);
 
 // The rows of a row-major FP64 matrix a (rows x kdim, leading dimension lda)
-// as the fold's Q operand: five base-254 digits per entry under one power of
-// two per row (split once, folded against many digit batches).
+// as the fold's Q operand: nd (five, or four for the Khatri-Rao sketch)
+// base-254 digits per entry under one power of two per row (split once,
+// folded against many digit batches).
 struct FoldI8Rows {
     DeviceBuffer<int8_t> digits;
     DeviceBuffer<double> scales;
     int64_t rows = 0, kdim = 0, bpad = 0, ldq = 0;
+    int nd = 5;
     // per-row-group column sums of a launch that batches the row groups
     mutable DeviceBuffer<double> part;
 };
-void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream);
+void split_rows_i8(const double* a, int64_t lda, int64_t rows, int64_t kdim, FoldI8Rows& out, hipStream_t stream,
+                   int nd = 5);
 // c[t + j * ldc] -= a_t . y_hat_j for the rows t of q and the n five-plane
 // base-254 digit columns of Yd (pitch 8 P bytes, scales yscale[j]): the 15
 // digit pairs p + r <= 4 (within ~1e-11 of |a_t| |y_j|).  kdim <= 26624.
@@ -217,7 +221,14 @@ void fold_rows_square_i8(const FoldI8Rows& q, const int8_t* Yd, int64_t P, const
 // then the same rows of V (zero past the sketch), and
 //   y[s + j * ldy] = cscale[j] (u_s . y_hat_j)(v_s . y_hat_j),
 // ynorm2[j] += ||y_j||^2 (summed per group in group order), non-finite
-// entries counted into *bad.
+// entries counted into *bad.  U, V and the features (digitize_columns_f64_max
+// with kSketchDigits254 planes) come in four base-254 digits, multiplied in
+// the ten digit pairs p + r <= 3: each factor u_s . z lands within ~2e-10 of
+// its FP64 value relative to |u_s| |z| (RMS), below the ~5e-10 of the column
+// norm at which the QRCP then holds each sketch column (four planes,
+// digitize_columns_f64), so a fifth digit and the five pairs p + r = 4 would
+// buy accuracy that the next step drops.
+constexpr int kSketchDigits254 = 4;
 // The row digits fold_rows_khatri_rao_i8 takes, from u, v [d][ell] (u[a * ell + s]).
 void split_khatri_rao_rows_i8(const double* u, const double* v, int64_t d, int64_t ell, FoldI8Rows& out,
                               hipStream_t stream);

@@ -451,9 +451,9 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
     MPISDF_HIP(hipMemsetAsync(ynorm2.ptr, 0, n * sizeof(double), s));
     if (digits5) {
         // The FP64 sketch from digit products: the rows of U and V (stacked,
-        // 64 of U then the same 64 of V) and each point's features as five
-        // base-254 digits under a power of two (within ~4.8e-13 of the row's
-        // or point's largest entry); the 15 digit-pair products p + r <= 4 run
+        // 64 of U then the same 64 of V) and each point's features as four
+        // base-254 digits under a power of two (within ~1.2e-10 of the row's
+        // or point's largest entry); the ten digit-pair products p + r <= 3 run
         // exactly in INT32 on INT8 MFMA and are combined in FP64, and the
         // kernel's epilogue forms y = sqrt(h / ell) (u . z)(v . z), ||y||^2 and
         // the non-finite count as the FP64 sketch does.
@@ -497,7 +497,7 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
             if (dpad != d)
                 MPISDF_HIP(hipMemcpy2DAsync(featp[b].ptr, dpad * sizeof(double), feat[b].ptr, d * sizeof(double),
                                             d * sizeof(double), count, hipMemcpyDeviceToDevice, prep.s));
-            digitize_columns_f64_max(digits_of(b), d, count, dpad, fnorm[b].ptr, fscale[b].ptr, prep.s);
+            digitize_columns_f64_max(digits_of(b), d, count, dpad, fnorm[b].ptr, fscale[b].ptr, prep.s, kSketchDigits254);
             ready[b].record(prep.s);
         };
         LaunchTimer timer;
@@ -705,9 +705,9 @@ std::vector<int64_t> select_randomized(Context& ctx, const SelectArgs& a, const 
         // MFMA work of the sketch: 4 ell d n per product term (three for the
         // split-BF16 form, two for the two-term FP8 form).
         diag[kDiagSketchFlops] = 4.0 * (double)ell * (double)d * (double)n * (lowp ? sketch_segments(form) : 1);
-        // fp64i8: the INT8 operations, 15 digit pairs x 2 (2 ell64) dpad per point
+        // fp64i8: the INT8 operations, ten digit pairs x 2 (2 ell64) dpad per point
         if (digits5)
-            diag[kDiagSketchOps] = 15.0 * 2.0 * (2.0 * (double)ldy64) * (double)((d + 63) / 64 * 64) * (double)n;
+            diag[kDiagSketchOps] = 10.0 * 2.0 * (2.0 * (double)ldy64) * (double)((d + 63) / 64 * 64) * (double)n;
         diag[kDiagSketchLaunches] = sketch_launches;
         diag[kDiagFoldKind] = digits5 ? 4.0 : use_digits ? (planes == 4 ? 3.0 : 2.0) : (lowp ? 1.0 : 0.0);
     }
