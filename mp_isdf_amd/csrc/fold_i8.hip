@@ -221,35 +221,37 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_kernel(floa
 // FROM_MAX: the scale comes from the column's largest |y_ij| (found here)
 // rather than from its norm, and ynorm2 is output only.  NY (4 or 5) planes:
 // within s_j 254^-NY / 2 of each entry.
+// T threads per column, E entries per thread (P <= T E; the host picks the
+// smallest shape that holds the column, so short feature columns do not
+// occupy 1024-thread workgroups).
 constexpr int kDigitizeEntriesF64 = (int)(kDigitRowsMaxF64 / kDigitizeThreads);  // P <= 16384
-template <bool FROM_MAX, int NY>
-__global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(double* __restrict__ y, int64_t m,
-                                                                                int64_t P,
-                                                                                double* __restrict__ ynorm2,
-                                                                                double* __restrict__ scale)
+template <bool FROM_MAX, int NY, int T = kDigitizeThreads, int E = kDigitizeEntriesF64>
+__global__ __launch_bounds__(T) void digitize_columns_f64_kernel(double* __restrict__ y, int64_t m, int64_t P,
+                                                                 double* __restrict__ ynorm2,
+                                                                 double* __restrict__ scale)
 {
-    __shared__ double part[kDigitizeThreads / 64];
-    __shared__ double pmax[kDigitizeThreads / 64];
+    __shared__ double part[T / 64];
+    __shared__ double pmax[T / 64];
     const int64_t j = blockIdx.x;
     double* src = y + j * P;  // 8 P bytes
-    double x[kDigitizeEntriesF64];
+    double x[E];
 #pragma unroll
-    for (int e = 0; e < kDigitizeEntriesF64; ++e) {
-        const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
+    for (int e = 0; e < E; ++e) {
+        const int64_t i = threadIdx.x + (int64_t)T * e;
         x[e] = i < m ? src[i] : 0.0;
     }
     double nrm;  // >= max |y_ij|
     if (FROM_MAX) {
         double mx = 0;
 #pragma unroll
-        for (int e = 0; e < kDigitizeEntriesF64; ++e) mx = fmax(mx, fabs(x[e]));
+        for (int e = 0; e < E; ++e) mx = fmax(mx, fabs(x[e]));
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
         if ((threadIdx.x & 63) == 0) pmax[threadIdx.x / 64] = mx;
         __syncthreads();
         mx = 0;
 #pragma unroll
-        for (int w = 0; w < kDigitizeThreads / 64; ++w) mx = fmax(mx, pmax[w]);
+        for (int w = 0; w < T / 64; ++w) mx = fmax(mx, pmax[w]);
         nrm = mx;
     } else {
         nrm = sqrt(ynorm2[j]);
@@ -262,8 +264,8 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(
     int8_t* planes = reinterpret_cast<int8_t*>(src);
     double acc = 0;
 #pragma unroll
-    for (int e = 0; e < kDigitizeEntriesF64; ++e) {
-        const int64_t i = threadIdx.x + (int64_t)kDigitizeThreads * e;
+    for (int e = 0; e < E; ++e) {
+        const int64_t i = threadIdx.x + (int64_t)T * e;
         if (i < P) {
             double v = x[e] * inv;  // exact: power-of-two scaling, |v| < 1/2
             double dg[NY];
@@ -291,7 +293,7 @@ __global__ __launch_bounds__(kDigitizeThreads) void digitize_columns_f64_kernel(
     if (threadIdx.x == 0) {
         double t = 0;
 #pragma unroll
-        for (int w = 0; w < kDigitizeThreads / 64; ++w) t += part[w];
+        for (int w = 0; w < T / 64; ++w) t += part[w];
         ynorm2[j] = t;
         scale[j] = sc;
     }
@@ -671,6 +673,22 @@ void launch_fold_i8(int64_t rows, bool energy, dim3 grid, hipStream_t s, const i
 
 }  // namespace
 
+// One workgroup per column, shaped to the pitch (P <= 16384).
+template <bool FROM_MAX, int NY>
+static void launch_digitize_f64(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
+                                hipStream_t stream)
+{
+    if (P <= 256 * 6)
+        hipLaunchKernelGGL((digitize_columns_f64_kernel<FROM_MAX, NY, 256, 6>), dim3((unsigned)n), dim3(256), 0,
+                           stream, y, m, P, ynorm2, scale);
+    else if (P <= 256 * 16)
+        hipLaunchKernelGGL((digitize_columns_f64_kernel<FROM_MAX, NY, 256, 16>), dim3((unsigned)n), dim3(256), 0,
+                           stream, y, m, P, ynorm2, scale);
+    else
+        hipLaunchKernelGGL((digitize_columns_f64_kernel<FROM_MAX, NY>), dim3((unsigned)n), dim3(kDigitizeThreads), 0,
+                           stream, y, m, P, ynorm2, scale);
+}
+
 void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* ynorm2, double* scale,
                           hipStream_t stream)
 {
@@ -679,8 +697,7 @@ void digitize_columns_f64(double* y, int64_t m, int64_t n, int64_t P, double* yn
     if (P > kDigitRowsMaxF64)
         throw std::invalid_argument("digitize_columns_f64 holds one column in registers (pitch <= 16384)");
     if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
-    hipLaunchKernelGGL((digitize_columns_f64_kernel<false, 4>), dim3((unsigned)n), dim3(kDigitizeThreads), 0, stream,
-                       y, m, P, ynorm2, scale);
+    launch_digitize_f64<false, 4>(y, m, n, P, ynorm2, scale, stream);
     MPISDF_HIP(hipGetLastError());
 }
 
@@ -694,11 +711,9 @@ void digitize_columns_f64_max(double* y, int64_t m, int64_t n, int64_t P, double
         throw std::invalid_argument("digitize_columns_f64 holds one column in registers (pitch <= 16384)");
     if (n > 0x7fffffff) throw std::overflow_error("too many columns to digitise in one launch");
     if (planes == 4)
-        hipLaunchKernelGGL((digitize_columns_f64_kernel<true, 4>), dim3((unsigned)n), dim3(kDigitizeThreads), 0,
-                           stream, y, m, P, ynorm2, scale);
+        launch_digitize_f64<true, 4>(y, m, n, P, ynorm2, scale, stream);
     else
-        hipLaunchKernelGGL((digitize_columns_f64_kernel<true, 5>), dim3((unsigned)n), dim3(kDigitizeThreads), 0,
-                           stream, y, m, P, ynorm2, scale);
+        launch_digitize_f64<true, 5>(y, m, n, P, ynorm2, scale, stream);
     MPISDF_HIP(hipGetLastError());
 }
 
