@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <chrono>
 #include <cmath>
 #include <iostream>
@@ -788,12 +789,20 @@ void fill_normal(uint64_t seed, int64_t count, double* out, int workers)
             ready.store(e, std::memory_order_release);
         }
         for (auto& t : pool) t.join();
-        for (const auto& o : parts) {
-            const int64_t take = std::min<int64_t>((int64_t)o.size(), count - produced);
-            std::copy(o.begin(), o.begin() + take, out + produced);
-            produced += take;
-            if (produced == count) break;
-        }
+        // The chunks' values in chunk order, copied by the workers in parallel.
+        std::vector<int64_t> at((size_t)nchunks + 1, produced);
+        for (int64_t c = 0; c < nchunks; ++c)
+            at[(size_t)c + 1] = std::min<int64_t>(count, at[(size_t)c] + (int64_t)parts[(size_t)c].size());
+        std::atomic<int64_t> next_copy{0};
+        auto copy = [&]() {
+            for (int64_t c; (c = next_copy.fetch_add(1)) < nchunks;)
+                std::copy_n(parts[(size_t)c].begin(), at[(size_t)c + 1] - at[(size_t)c], out + at[(size_t)c]);
+        };
+        pool.clear();
+        for (int t = 1; t < std::max(1, workers); ++t) pool.emplace_back(copy);
+        copy();
+        for (auto& t : pool) t.join();
+        produced = at[(size_t)nchunks];
     }
 }
 
@@ -803,13 +812,26 @@ void make_omega(int64_t d, int64_t ell, uint64_t seed, std::vector<double>& u, s
 {
     // Two independent streams, each recorded by one thread and expanded by
     // half of the remaining host threads (at most 16 in all).
-    u.resize((size_t)(d * ell));
-    v.resize((size_t)(d * ell));
     const int threads = (int)std::max(4u, std::min(16u, std::thread::hardware_concurrency()));
     const int workers = (threads - 2) / 2;
-    std::thread tv(fill_normal, seed ^ 0x9e3779b97f4a7c15ULL, d * ell, v.data(), workers);
-    fill_normal(seed, d * ell, u.data(), workers);
+    std::exception_ptr verr;
+    std::thread tv([&] {
+        try {
+            v.resize((size_t)(d * ell));
+            fill_normal(seed ^ 0x9e3779b97f4a7c15ULL, d * ell, v.data(), workers);
+        } catch (...) {
+            verr = std::current_exception();
+        }
+    });
+    try {
+        u.resize((size_t)(d * ell));
+        fill_normal(seed, d * ell, u.data(), workers);
+    } catch (...) {
+        tv.join();
+        throw;
+    }
     tv.join();
+    if (verr) std::rethrow_exception(verr);
 }
 
 std::vector<int64_t> select_anchors(Context& ctx, const SelectArgs& a, double* diag)
